@@ -1,0 +1,198 @@
+"""ORACLE -- TEST INFRASTRUCTURE ONLY.
+
+ctypes binding of oracle/liboracle.so, the FP64 CPU restatement of PLANC's
+shared-memory NMF/NTF path (oracle/planc_oracle.cpp).  Only tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg import this module;
+the product (planc_amd) never does.
+
+How each oracle function is pinned (tests/test_oracle.py):
+
+* gemm                -- numpy matmul on random operands, all four op() pairs.
+* bppnnls             -- scipy.optimize.nnls (Lawson-Hanson, unique solution for
+                         an SPD Gram) on random problems, and the KKT conditions
+                         x >= 0, y = AtA x - Atb >= 0, x.y = 0; committed fixtures.
+* krp_leave_out_one   -- brute-force product over explicit multi-indices.
+* mttkrp (direct and dimension tree)
+                      -- numpy.einsum on tiny tensors (committed fixtures), the
+                         closed form for rank-1 tensors, and dimtree == direct,
+                         the invariant the reference records in
+                         distntf/testcases.md:3-67.
+* nmf MU / HALS / BPP -- NumPy step-by-step restatements on tiny problems
+                         (committed trajectory fixtures), monotone objective
+                         (testcases.md:3), the exact fixed point A = W H^T for
+                         MU, KKT of each BPP half-step, and the HALS column step
+                         being the exact coordinate minimiser.
+* ntf HALS / MU / BPP -- NumPy restatement with einsum MTTKRP (fixtures),
+                         monotone error, relative error checked against a direct
+                         numpy reconstruction.
+* cp_loss             -- central differences in FP64 (h = 1e-6) and the einsum
+                         reconstruction.
+* r2_update           -- brute force over the four active sets of the
+                         two-variable NNLS.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (run `make -C oracle`)")
+        L = ctypes.CDLL(path)
+        L.oracle_gemm.argtypes = [ctypes.c_char, ctypes.c_char, ctypes.c_long, ctypes.c_long,
+                                  ctypes.c_long, _dp, ctypes.c_long, _dp, ctypes.c_long, _dp,
+                                  ctypes.c_long]
+        L.oracle_bppnnls.argtypes = [_dp, ctypes.c_int, _dp, ctypes.c_long, _dp]
+        L.oracle_bppnnls.restype = ctypes.c_int
+        L.oracle_nmf.argtypes = [ctypes.c_int, _dp, ctypes.c_long, ctypes.c_long, _dp, _dp,
+                                 ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_double, _dp, _dp]
+        L.oracle_nmf.restype = ctypes.c_int
+        L.oracle_krp_leave_out_one.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_long),
+                                               ctypes.POINTER(_dp), ctypes.c_int, ctypes.c_int, _dp]
+        L.oracle_mttkrp.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_long), _dp,
+                                    ctypes.POINTER(_dp), ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_int, _dp]
+        L.oracle_ntf.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_long), _dp,
+                                 ctypes.POINTER(_dp), ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                 _dp, _dp]
+        L.oracle_ntf.restype = ctypes.c_int
+        L.oracle_cp_loss.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_long), _dp,
+                                     ctypes.POINTER(_dp), ctypes.c_int, ctypes.POINTER(_dp)]
+        L.oracle_cp_loss.restype = ctypes.c_double
+        L.oracle_r2_update.argtypes = [_dp, _dp, ctypes.c_long, _dp]
+        L.oracle_set_threads.argtypes = [ctypes.c_int]
+        L.oracle_get_threads.restype = ctypes.c_int
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _f64(a, order="F"):
+    return np.require(np.asarray(a, dtype=np.float64), requirements=[order, "A", "W", "O"])
+
+
+def set_threads(n):
+    lib().oracle_set_threads(int(n))
+
+
+def get_threads():
+    return lib().oracle_get_threads()
+
+
+def gemm(ta, tb, A, B):
+    A = _f64(A)
+    B = _f64(B)
+    M = A.shape[0] if ta == "N" else A.shape[1]
+    K = A.shape[1] if ta == "N" else A.shape[0]
+    N = B.shape[1] if tb == "N" else B.shape[0]
+    C = np.zeros((M, N), order="F")
+    lib().oracle_gemm(ta.encode(), tb.encode(), M, N, K, _p(A), A.shape[0], _p(B), B.shape[0],
+                      _p(C), M)
+    return C
+
+
+def bppnnls(AtA, AtB):
+    """Returns (X, iterations); iterations is -1 when BPP did not converge."""
+    AtA = _f64(AtA)
+    AtB = _f64(AtB)
+    if AtB.ndim == 1:
+        AtB = AtB.reshape(-1, 1, order="F")
+    n, nrhs = AtB.shape
+    X = np.zeros((n, nrhs), order="F")
+    it = lib().oracle_bppnnls(_p(AtA), n, _p(AtB), nrhs, _p(X))
+    return X, it
+
+
+def nmf(algo, A, W0, H0, iters, regW=(0.0, 0.0), regH=(0.0, 0.0), symm_reg=-1.0):
+    """algo 0 MU, 1 HALS, 2 ANLS-BPP. Returns (W, H, relerr, objective, rc)."""
+    A = _f64(A)
+    W = _f64(W0).copy(order="F")
+    H = _f64(H0).copy(order="F")
+    m, n = A.shape
+    k = W.shape[1]
+    rel = np.zeros(iters)
+    obj = np.zeros(iters)
+    rW = np.asarray(regW, dtype=np.float64)
+    rH = np.asarray(regH, dtype=np.float64)
+    rc = lib().oracle_nmf(algo, _p(A), m, n, _p(W), _p(H), k, iters, _p(rW), _p(rH),
+                          float(symm_reg), _p(rel), _p(obj))
+    return W, H, rel, obj, rc
+
+
+def _factor_ptrs(factors):
+    fs = [_f64(f) for f in factors]
+    arr = (_dp * len(fs))(*[_p(f) for f in fs])
+    return fs, arr
+
+
+def krp_leave_out_one(factors, n):
+    fs, arr = _factor_ptrs(factors)
+    dims = (ctypes.c_long * len(fs))(*[f.shape[0] for f in fs])
+    k = fs[0].shape[1]
+    rows = int(np.prod([f.shape[0] for i, f in enumerate(fs) if i != n]))
+    out = np.zeros((rows, k), order="F")
+    lib().oracle_krp_leave_out_one(len(fs), dims, arr, k, n, _p(out))
+    return out
+
+
+def mttkrp(X, factors, n, method=0, split=None):
+    """MTTKRP of mode n as PLANC lays it out: k x I_n.  X is mode-0-fastest
+    (pass a numpy array in Fortran order)."""
+    Xf = _f64(X)
+    fs, arr = _factor_ptrs(factors)
+    N = len(fs)
+    dims = (ctypes.c_long * N)(*Xf.shape)
+    k = fs[0].shape[1]
+    out = np.zeros((k, Xf.shape[n]), order="F")
+    if split is None:
+        split = N // 2
+    lib().oracle_mttkrp(N, dims, _p(Xf), arr, k, n, method, split, _p(out))
+    return out
+
+
+def ntf(algo, X, factors0, iters, dimtree=1):
+    """algo 0 MU, 1 HALS, 2 BPP. Returns (factors, lambda, relerr, rc)."""
+    Xf = _f64(X)
+    fs = [_f64(f).copy(order="F") for f in factors0]
+    arr = (_dp * len(fs))(*[_p(f) for f in fs])
+    N = len(fs)
+    dims = (ctypes.c_long * N)(*Xf.shape)
+    k = fs[0].shape[1]
+    lam = np.zeros(k)
+    rel = np.zeros(iters)
+    rc = lib().oracle_ntf(algo, N, dims, _p(Xf), arr, k, iters, dimtree, _p(lam), _p(rel))
+    return fs, lam, rel, rc
+
+
+def cp_loss(X, factors, grads=False):
+    Xf = _f64(X)
+    fs, arr = _factor_ptrs(factors)
+    N = len(fs)
+    dims = (ctypes.c_long * N)(*Xf.shape)
+    k = fs[0].shape[1]
+    if grads:
+        gs = [np.zeros(f.shape, order="F") for f in fs]
+        garr = (_dp * N)(*[_p(g) for g in gs])
+        loss = lib().oracle_cp_loss(N, dims, _p(Xf), arr, k, garr)
+        return loss, gs
+    return lib().oracle_cp_loss(N, dims, _p(Xf), arr, k, None)
+
+
+def r2_update(left, right):
+    left = _f64(left)
+    right = _f64(right)
+    out = np.zeros_like(right, order="F")
+    lib().oracle_r2_update(_p(left), _p(right), right.shape[1], _p(out))
+    return out

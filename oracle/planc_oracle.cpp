@@ -1,0 +1,1034 @@
+// ORACLE -- TEST INFRASTRUCTURE ONLY.
+//
+// FP64 CPU restatement of PLANC's shared-memory NMF/NTF hot path.  It is the
+// checker for the HIP library, never part of the product: only tests/,
+// __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.  Each
+// function cites the reference file:line whose behaviour it restates
+// (paths relative to the PLANC source tree).  Armadillo is not available, so
+// a minimal column-major matrix type stands in for arma::mat; every product
+// is an FP64 GEMM (internal OpenMP-blocked kernel).
+//
+// Parity status of each function (how it is pinned) is listed in the header
+// comment of oracle/oracle.py and in tests/test_oracle.py.
+
+#include <omp.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+namespace orc {
+
+// Constants of common/utils.h:46-49.
+static const double EPS_1EM16 = 0.00000000000000001;  // literally 1e-17
+static const double EPS = 0.000001;
+static const double EPS_1EM12 = 1e-12;
+
+// ----------------------------------------------------------------------------
+// Column-major dense matrix (stand-in for arma::mat).
+struct Mat {
+  long r = 0, c = 0;
+  std::vector<double> d;
+  Mat() {}
+  Mat(long r_, long c_) : r(r_), c(c_), d((size_t)r_ * c_, 0.0) {}
+  double &operator()(long i, long j) { return d[i + r * j]; }
+  double operator()(long i, long j) const { return d[i + r * j]; }
+  double *col(long j) { return d.data() + r * j; }
+  const double *col(long j) const { return d.data() + r * j; }
+};
+
+// ----------------------------------------------------------------------------
+// C = alpha*op(A)*op(B) + beta*C, column-major, op = 'N' or 'T'.
+// Blocked, packed, OpenMP over output tiles; splits K across threads when
+// there are fewer tiles than threads (long-K products such as KRP^T * X).
+static inline double opA(const double *A, long lda, char t, long i, long k) {
+  return t == 'N' ? A[i + k * lda] : A[k + i * lda];
+}
+
+static void gemm_block(char ta, char tb, long i0, long i1, long j0, long j1,
+                       long k0, long k1, const double *A, long lda,
+                       const double *B, long ldb, double *C, long ldc) {
+  const long KB = 256;
+  std::vector<double> Ap, Bp;
+  for (long kb = k0; kb < k1; kb += KB) {
+    long ke = std::min(k1, kb + KB), kl = ke - kb;
+    long mi = i1 - i0, nj = j1 - j0;
+    long mi8 = (mi + 7) / 8 * 8, nj4 = (nj + 3) / 4 * 4;
+    Ap.assign((size_t)kl * mi8, 0.0);
+    Bp.assign((size_t)kl * nj4, 0.0);
+    // Ap: for each row group of 8, kl x 8 (row index fastest).
+    for (long g = 0; g < mi8 / 8; ++g)
+      for (long kk = 0; kk < kl; ++kk)
+        for (long u = 0; u < 8; ++u) {
+          long i = i0 + g * 8 + u;
+          Ap[(size_t)g * kl * 8 + kk * 8 + u] = (i < i1) ? opA(A, lda, ta, i, kb + kk) : 0.0;
+        }
+    for (long g = 0; g < nj4 / 4; ++g)
+      for (long kk = 0; kk < kl; ++kk)
+        for (long u = 0; u < 4; ++u) {
+          long j = j0 + g * 4 + u;
+          // op(B)(k, j)
+          double v = 0.0;
+          if (j < j1) v = (tb == 'N') ? B[(kb + kk) + j * ldb] : B[j + (kb + kk) * ldb];
+          Bp[(size_t)g * kl * 4 + kk * 4 + u] = v;
+        }
+    for (long gi = 0; gi < mi8 / 8; ++gi)
+      for (long gj = 0; gj < nj4 / 4; ++gj) {
+        double acc[4][8];
+        for (int a = 0; a < 4; ++a)
+          for (int b = 0; b < 8; ++b) acc[a][b] = 0.0;
+        const double *ap = &Ap[(size_t)gi * kl * 8];
+        const double *bp = &Bp[(size_t)gj * kl * 4];
+        for (long kk = 0; kk < kl; ++kk) {
+          for (int a = 0; a < 4; ++a) {
+            double bv = bp[kk * 4 + a];
+            for (int b = 0; b < 8; ++b) acc[a][b] += ap[kk * 8 + b] * bv;
+          }
+        }
+        for (int a = 0; a < 4; ++a) {
+          long j = j0 + gj * 4 + a;
+          if (j >= j1) continue;
+          for (int b = 0; b < 8; ++b) {
+            long i = i0 + gi * 8 + b;
+            if (i < i1) C[i + j * ldc] += acc[a][b];
+          }
+        }
+      }
+  }
+}
+
+void gemm(char ta, char tb, long M, long N, long K, double alpha, const double *A, long lda,
+          const double *B, long ldb, double beta, double *C, long ldc) {
+  if (M <= 0 || N <= 0) return;
+  for (long j = 0; j < N; ++j)
+    for (long i = 0; i < M; ++i) C[i + j * ldc] = (beta == 0.0) ? 0.0 : beta * C[i + j * ldc];
+  if (K <= 0 || alpha == 0.0) return;
+  const long MB = 128, NB = 64;
+  long ti = (M + MB - 1) / MB, tj = (N + NB - 1) / NB, nt = ti * tj;
+  int nth = omp_get_max_threads();
+  // Product accumulated into a scratch then scaled by alpha.
+  std::vector<double> T((size_t)M * N, 0.0);
+  if (nt >= nth || K < 4096 || nth == 1) {
+#pragma omp parallel for schedule(dynamic) collapse(2)
+    for (long bi = 0; bi < ti; ++bi)
+      for (long bj = 0; bj < tj; ++bj)
+        gemm_block(ta, tb, bi * MB, std::min(M, (bi + 1) * MB), bj * NB, std::min(N, (bj + 1) * NB),
+                   0, K, A, lda, B, ldb, T.data(), M);
+  } else {
+    // Split K into nth slices; reduce partials in fixed slice order.
+    int ns = nth;
+    std::vector<std::vector<double>> part(ns);
+#pragma omp parallel for schedule(static)
+    for (int s = 0; s < ns; ++s) {
+      long k0 = K * s / ns, k1 = K * (s + 1) / ns;
+      part[s].assign((size_t)M * N, 0.0);
+      for (long bi = 0; bi < ti; ++bi)
+        for (long bj = 0; bj < tj; ++bj)
+          gemm_block(ta, tb, bi * MB, std::min(M, (bi + 1) * MB), bj * NB,
+                     std::min(N, (bj + 1) * NB), k0, k1, A, lda, B, ldb, part[s].data(), M);
+    }
+    for (int s = 0; s < ns; ++s)
+      for (size_t e = 0; e < T.size(); ++e) T[e] += part[s][e];
+  }
+  for (long j = 0; j < N; ++j)
+    for (long i = 0; i < M; ++i) C[i + j * ldc] += alpha * T[i + j * M];
+}
+
+static Mat mul(const Mat &A, char ta, const Mat &B, char tb) {
+  long M = ta == 'N' ? A.r : A.c, K = ta == 'N' ? A.c : A.r;
+  long N = tb == 'N' ? B.c : B.r;
+  Mat C(M, N);
+  gemm(ta, tb, M, N, K, 1.0, A.d.data(), A.r, B.d.data(), B.r, 0.0, C.d.data(), M);
+  return C;
+}
+
+static double fro2(const Mat &A) {
+  double s = 0;
+  for (double v : A.d) s += v * v;
+  return s;
+}
+
+// ----------------------------------------------------------------------------
+// Small dense solves: arma::solve(A, B, likely_sympd) = Cholesky, falling back
+// to LU with partial pivoting when A is not positive definite.
+static bool chol_inplace(std::vector<double> &L, int p) {
+  for (int j = 0; j < p; ++j) {
+    double s = L[j + p * j];
+    for (int t = 0; t < j; ++t) s -= L[j + p * t] * L[j + p * t];
+    if (!(s > 0.0)) return false;
+    double ljj = std::sqrt(s);
+    L[j + p * j] = ljj;
+    for (int i = j + 1; i < p; ++i) {
+      double v = L[i + p * j];
+      for (int t = 0; t < j; ++t) v -= L[i + p * t] * L[j + p * t];
+      L[i + p * j] = v / ljj;
+    }
+  }
+  return true;
+}
+
+// Solves S X = B for X (p x nrhs, column-major, in B). S is p x p.
+static void solve_sympd(const std::vector<double> &S, int p, std::vector<double> &B, long nrhs) {
+  if (p == 0) return;
+  std::vector<double> L = S;
+  if (chol_inplace(L, p)) {
+    for (long c = 0; c < nrhs; ++c) {
+      double *b = &B[(size_t)c * p];
+      for (int i = 0; i < p; ++i) {
+        double v = b[i];
+        for (int t = 0; t < i; ++t) v -= L[i + p * t] * b[t];
+        b[i] = v / L[i + p * i];
+      }
+      for (int i = p - 1; i >= 0; --i) {
+        double v = b[i];
+        for (int t = i + 1; t < p; ++t) v -= L[t + p * i] * b[t];
+        b[i] = v / L[i + p * i];
+      }
+    }
+    return;
+  }
+  // LU with partial pivoting.
+  std::vector<double> LU = S;
+  std::vector<int> piv(p);
+  for (int j = 0; j < p; ++j) {
+    int pr = j;
+    for (int i = j + 1; i < p; ++i)
+      if (std::fabs(LU[i + p * j]) > std::fabs(LU[pr + p * j])) pr = i;
+    piv[j] = pr;
+    if (pr != j)
+      for (int t = 0; t < p; ++t) std::swap(LU[j + p * t], LU[pr + p * t]);
+    double d = LU[j + p * j];
+    if (d == 0.0) continue;
+    for (int i = j + 1; i < p; ++i) {
+      LU[i + p * j] /= d;
+      for (int t = j + 1; t < p; ++t) LU[i + p * t] -= LU[i + p * j] * LU[j + p * t];
+    }
+  }
+  for (long c = 0; c < nrhs; ++c) {
+    double *b = &B[(size_t)c * p];
+    for (int j = 0; j < p; ++j)
+      if (piv[j] != j) std::swap(b[j], b[piv[j]]);
+    for (int i = 0; i < p; ++i)
+      for (int t = 0; t < i; ++t) b[i] -= LU[i + p * t] * b[t];
+    for (int i = p - 1; i >= 0; --i) {
+      for (int t = i + 1; t < p; ++t) b[i] -= LU[i + p * t] * b[t];
+      b[i] /= LU[i + p * i];
+    }
+  }
+}
+
+// ----------------------------------------------------------------------------
+// Block principal pivoting NNLS, multiple right-hand sides.
+// Restates BPPNNLS::solveNNLSMultipleRHS (nnls/bppnnls.hpp:67-191) with
+// solveNormalEqComb (:204-259) and the passive-set grouping of
+// computeCorrelationScore / SortBooleanMatrix (:272-287,
+// nnls/SortBooleanMatrix.hpp:10-45).  AtA is n x n, AtB n x nrhs, X out.
+// Returns the iteration count, or -1 when MAX_ITERATIONS = 5n is exceeded
+// (the reference calls exit(EXIT_FAILURE) there, :185-188).
+static void solve_normal_eq_comb(const double *AtA, int n, const double *AtB, long ldb,
+                                 const std::vector<long> &cols, const std::vector<uint8_t> &Pset,
+                                 double *X, long ldx) {
+  // Group columns with identical passive sets: lexicographic sort of the
+  // boolean columns, as SortBooleanMatrix does.
+  std::vector<long> idx(cols.begin(), cols.end());
+  auto less = [&](long a, long b) {
+    for (int i = 0; i < n; ++i) {
+      uint8_t x = Pset[(size_t)a * n + i], y = Pset[(size_t)b * n + i];
+      if (x < y) return true;
+      if (x > y) return false;
+    }
+    return false;
+  };
+  std::sort(idx.begin(), idx.end(), less);
+  size_t g0 = 0;
+  while (g0 < idx.size()) {
+    size_t g1 = g0 + 1;
+    while (g1 < idx.size() && !less(idx[g0], idx[g1]) && !less(idx[g1], idx[g0])) ++g1;
+    std::vector<int> P;
+    for (int i = 0; i < n; ++i)
+      if (Pset[(size_t)idx[g0] * n + i]) P.push_back(i);
+    int p = (int)P.size();
+    long nr = (long)(g1 - g0);
+    std::vector<double> S((size_t)p * p), B((size_t)p * nr);
+    for (int a = 0; a < p; ++a)
+      for (int b = 0; b < p; ++b) S[a + p * b] = AtA[P[a] + (size_t)n * P[b]];
+    for (long c = 0; c < nr; ++c)
+      for (int a = 0; a < p; ++a) B[a + (size_t)p * c] = AtB[P[a] + ldb * idx[g0 + c]];
+    solve_sympd(S, p, B, nr);
+    for (long c = 0; c < nr; ++c) {
+      double *x = X + ldx * idx[g0 + c];
+      for (int i = 0; i < n; ++i) x[i] = 0.0;
+      for (int a = 0; a < p; ++a) x[P[a]] = B[a + (size_t)p * c];
+    }
+    g0 = g1;
+  }
+}
+
+int bppnnls(const double *AtA, int n, const double *AtB, long nrhs, double *X) {
+  const long MAXIT = (long)n * 5;  // nnls/bppnnls.hpp:69
+  std::vector<double> Y((size_t)n * nrhs);
+  std::vector<uint8_t> Pset((size_t)n * nrhs, 0), NonOpt((size_t)n * nrhs), Infea((size_t)n * nrhs);
+  std::vector<int> P(nrhs, 3), Ninf(nrhs, n + 1), NotGood(nrhs, 0);
+  std::vector<uint8_t> NotOpt(nrhs, 0);
+  for (long c = 0; c < nrhs; ++c)
+    for (int i = 0; i < n; ++i) {
+      X[i + (size_t)n * c] = 0.0;
+      Y[i + (size_t)n * c] = -AtB[i + (size_t)n * c];  // AtA*0 - AtB (:73)
+    }
+  auto recompute = [&](long c) {
+    int cnt = 0;
+    for (int i = 0; i < n; ++i) {
+      size_t e = i + (size_t)n * c;
+      NonOpt[e] = (Y[e] < 0) && (Pset[e] == 0);
+      Infea[e] = (X[e] < 0) && (Pset[e] != 0);
+      cnt += NonOpt[e] + Infea[e];
+    }
+    NotGood[c] = cnt;
+    NotOpt[c] = cnt > 0;
+  };
+  long numNonOpt = 0;
+  for (long c = 0; c < nrhs; ++c) {
+    recompute(c);
+    numNonOpt += NotOpt[c];
+  }
+  long iter = 0;
+  std::vector<long> nonopt_cols;
+  while (numNonOpt > 0) {
+    ++iter;
+    if (MAXIT > 0 && iter > MAXIT) return -1;
+    for (long c = 0; c < nrhs; ++c) {
+      if (!NotOpt[c]) continue;
+      bool c1 = NotGood[c] < Ninf[c];
+      bool c2 = !c1 && NotGood[c] >= Ninf[c] && P[c] >= 1;
+      if (c1 || c2) {
+        if (c1) {
+          P[c] = 3;
+          Ninf[c] = NotGood[c];
+        } else {
+          P[c] -= 1;
+        }
+        for (int i = 0; i < n; ++i) {
+          size_t e = i + (size_t)n * c;
+          if (NonOpt[e]) Pset[e] = 1;
+          if (Infea[e]) Pset[e] = 0;
+        }
+      } else {
+        // Backup rule: flip the largest index that is non-optimal or infeasible.
+        int rowidx = -1;
+        for (int i = 0; i < n; ++i)
+          if (NonOpt[i + (size_t)n * c] + Infea[i + (size_t)n * c]) rowidx = i;
+        size_t e = rowidx + (size_t)n * c;
+        Pset[e] = Pset[e] ? 0 : 1;
+      }
+    }
+    nonopt_cols.clear();
+    for (long c = 0; c < nrhs; ++c)
+      if (NotOpt[c]) nonopt_cols.push_back(c);
+    solve_normal_eq_comb(AtA, n, AtB, n, nonopt_cols, Pset, X, n);
+    for (long c : nonopt_cols) {
+      for (int i = 0; i < n; ++i) {
+        double v = -AtB[i + (size_t)n * c];
+        for (int t = 0; t < n; ++t) v += AtA[i + (size_t)n * t] * X[t + (size_t)n * c];
+        Y[i + (size_t)n * c] = v;
+      }
+    }
+    // fixAbsNumericalError on X and Y (:170-172); all columns.
+    for (size_t e = 0; e < (size_t)n * nrhs; ++e) {
+      if (std::fabs(X[e]) < EPS_1EM12) X[e] = 0.0;
+      if (std::fabs(Y[e]) < EPS_1EM12) Y[e] = 0.0;
+    }
+    numNonOpt = 0;
+    for (long c = 0; c < nrhs; ++c) {
+      recompute(c);
+      numNonOpt += NotOpt[c];
+    }
+  }
+  return (int)iter;
+}
+
+// ----------------------------------------------------------------------------
+// Shared-memory NMF (common/nmf.hpp, nmf/mu.hpp, nmf/hals.hpp, nmf/bppnmf.hpp).
+struct NMFState {
+  const Mat *A;
+  Mat W, H;
+  long m, n, k;
+  double normA;
+  double regW[2], regH[2];
+  double symm_reg = -1.0;
+  double fit_err_sq = 0, objective = 0;
+};
+
+// applyReg, common/nmf.hpp:76-90 (the reference holds reg in an FVEC, so the
+// value is rounded to float).
+static void apply_reg(const double *reg, Mat &G) {
+  if (reg[0] > 0) {
+    float l2 = (float)reg[0];
+    for (long i = 0; i < G.r; ++i) G(i, i) += l2;
+  }
+  if (reg[1] > 0) {
+    float l1 = (float)reg[1];
+    for (double &v : G.d) v += l1;
+  }
+}
+
+// normalize_by_W, common/nmf.hpp:152-161.
+static void normalize_by_W(NMFState &s) {
+  for (long c = 0; c < s.k; ++c) {
+    double nrm = 0;
+    for (long i = 0; i < s.m; ++i) nrm += s.W(i, c) * s.W(i, c);
+    nrm = std::sqrt(nrm);
+    if (nrm > 0) {
+      for (long i = 0; i < s.m; ++i) s.W(i, c) /= nrm;
+      for (long j = 0; j < s.n; ++j) s.H(j, c) *= nrm;
+    }
+  }
+}
+
+// computeObjectiveError, common/nmf.hpp:390-424 (recomputes A^T W).
+static void objective_error(NMFState &s) {
+  Mat AtW = mul(*s.A, 'T', s.W, 'N');
+  Mat WtW = mul(s.W, 'T', s.W, 'N');
+  Mat HtH = mul(s.H, 'T', s.H, 'N');
+  double TrHtAtW = 0;
+  for (long j = 0; j < s.n; ++j)
+    for (long c = 0; c < s.k; ++c) TrHtAtW += s.H(j, c) * AtW(j, c);
+  double TrWtWHtH = 0;
+  for (long a = 0; a < s.k; ++a)
+    for (long b = 0; b < s.k; ++b) TrWtWHtH += WtW(a, b) * HtH(b, a);
+  double froW = 0, froH = 0;
+  for (long c = 0; c < s.k; ++c) {
+    froW += WtW(c, c);
+    froH += HtH(c, c);
+  }
+  double l1W = 0, l1H = 0;
+  for (long i = 0; i < s.m; ++i) {
+    double r = 0;
+    for (long c = 0; c < s.k; ++c) r += s.W(i, c);
+    l1W += r * r;
+  }
+  for (long j = 0; j < s.n; ++j) {
+    double r = 0;
+    for (long c = 0; c < s.k; ++c) r += s.H(j, c);
+    l1H += r * r;
+  }
+  s.fit_err_sq = s.normA * s.normA - 2 * TrHtAtW + TrWtWHtH;
+  s.objective = s.fit_err_sq + s.regW[0] * froW + s.regH[0] * froH + s.regW[1] * l1W +
+                s.regH[1] * l1H;
+}
+
+// printObjective's relative error, common/nmf.hpp:462,468.
+static double rel_error(const NMFState &s) {
+  double err = (s.fit_err_sq > 0) ? std::sqrt(s.fit_err_sq) : s.normA;
+  return err / s.normA;
+}
+
+// HALSNMF::computeNMF, nmf/hals.hpp:50-116 (the constructor's normalize_by_W,
+// :41,46, is applied by the caller before the loop).
+static void hals_loop(NMFState &s, int iters, double *relerr) {
+  std::vector<double> x;
+  for (int it = 0; it < iters; ++it) {
+    Mat WtA = mul(s.W, 'T', *s.A, 'N');  // k x n
+    Mat WtW = mul(s.W, 'T', s.W, 'N');
+    apply_reg(s.regH, WtW);
+    x.resize(s.n);
+    for (long c = 0; c < s.k; ++c) {
+      for (long j = 0; j < s.n; ++j) {
+        double hv = 0;
+        for (long t = 0; t < s.k; ++t) hv += s.H(j, t) * WtW(t, c);
+        double v = s.H(j, c) + (WtA(c, j) - hv);
+        x[j] = (v < EPS_1EM16) ? EPS_1EM16 : v;
+      }
+      double nrm = 0;
+      for (long j = 0; j < s.n; ++j) nrm += x[j] * x[j];
+      if (std::sqrt(nrm) != 0)
+        for (long j = 0; j < s.n; ++j) s.H(j, c) = x[j];
+    }
+    Mat AH = mul(*s.A, 'N', s.H, 'N');  // m x k
+    Mat HtH = mul(s.H, 'T', s.H, 'N');
+    apply_reg(s.regW, HtH);
+    x.resize(s.m);
+    for (long c = 0; c < s.k; ++c) {
+      for (long i = 0; i < s.m; ++i) {
+        double wv = 0;
+        for (long t = 0; t < s.k; ++t) wv += s.W(i, t) * HtH(t, c);
+        double v = (s.W(i, c) * HtH(c, c)) + (AH(i, c) - wv);
+        x[i] = (v < EPS_1EM16) ? EPS_1EM16 : v;
+      }
+      double nrm = 0;
+      for (long i = 0; i < s.m; ++i) nrm += x[i] * x[i];
+      nrm = std::sqrt(nrm);
+      if (nrm != 0)
+        for (long i = 0; i < s.m; ++i) s.W(i, c) = x[i] / nrm;
+    }
+    normalize_by_W(s);
+    objective_error(s);
+    if (relerr) relerr[it] = rel_error(s);
+  }
+}
+
+// BPPNMF::updateOtherGivenOneMultipleRHS, nmf/bppnmf.hpp:27-94, with the
+// input passed untransposed: for 'W' the right-hand side Given^T*A^T is
+// formed as (A*H)^T.
+static int bpp_update(const NMFState &s, bool forW, const Mat &given, const double *reg, Mat &other) {
+  const long CH = 2000;  // ONE_THREAD_MATRIX_SIZE, :16
+  Mat GtG = mul(given, 'T', given, 'N');
+  apply_reg(reg, GtG);
+  Mat GtIn;  // k x ncols
+  if (forW) {
+    Mat AH = mul(*s.A, 'N', given, 'N');  // m x k
+    GtIn = Mat(s.k, s.m);
+    for (long i = 0; i < s.m; ++i)
+      for (long c = 0; c < s.k; ++c) GtIn(c, i) = AH(i, c);
+  } else {
+    GtIn = mul(given, 'T', *s.A, 'N');  // k x n
+  }
+  if (s.symm_reg > 0) {  // applySymmetricReg, common/nmf.hpp:123-129
+    for (long c = 0; c < s.k; ++c) GtG(c, c) += s.symm_reg;
+    for (long j = 0; j < GtIn.c; ++j)
+      for (long c = 0; c < s.k; ++c) GtIn(c, j) += s.symm_reg * given(j, c);
+  }
+  long ncols = GtIn.c;
+  long nch = (ncols + CH - 1) / CH;
+  int fail = 0;
+  std::vector<int> rc(nch, 0);
+#pragma omp parallel for schedule(dynamic)
+  for (long ch = 0; ch < nch; ++ch) {
+    long c0 = ch * CH, c1 = std::min(ncols, c0 + CH);
+    std::vector<double> X((size_t)s.k * (c1 - c0));
+    rc[ch] = bppnnls(GtG.d.data(), (int)s.k, GtIn.col(c0), c1 - c0, X.data());
+    for (long j = c0; j < c1; ++j)
+      for (long c = 0; c < s.k; ++c) other(j, c) = X[c + (size_t)s.k * (j - c0)];
+  }
+  for (long ch = 0; ch < nch; ++ch)
+    if (rc[ch] < 0) fail = 1;
+  return fail ? -1 : 0;
+}
+
+// ----------------------------------------------------------------------------
+// Dense tensor, mode 0 fastest (common/tensor.hpp:23-28).
+struct Tens {
+  int N;
+  std::vector<long> dims;
+  const double *x;
+  long numel() const {
+    long p = 1;
+    for (long d : dims) p *= d;
+    return p;
+  }
+};
+
+// krp_leave_out_one, common/ncpfactors.hpp:171-233: rows ordered with the
+// lowest remaining mode fastest.  Out is (prod others) x k.
+static void krp_leave_out_one(const std::vector<Mat> &F, int n, long k, Mat &out) {
+  int N = (int)F.size();
+  std::vector<int> modes;
+  for (int i = 0; i < N; ++i)
+    if (i != n) modes.push_back(i);
+  long rows = 1;
+  for (int i : modes) rows *= F[i].r;
+  out = Mat(rows, k);
+#pragma omp parallel for schedule(static)
+  for (long r = 0; r < rows; ++r) {
+    long rem = r;
+    std::vector<long> dig(modes.size());
+    for (size_t t = 0; t < modes.size(); ++t) {
+      dig[t] = rem % F[modes[t]].r;
+      rem /= F[modes[t]].r;
+    }
+    for (long c = 0; c < k; ++c) {
+      double v = 1.0;
+      // The reference multiplies from the highest mode down (matorder is reversed).
+      for (int t = (int)modes.size() - 1; t >= 0; --t) v *= F[modes[t]](dig[t], c);
+      out(r, c) = v;
+    }
+  }
+}
+
+// Tensor::mttkrp, common/tensor.hpp:246-327.  Out is k x I_n.
+static void mttkrp_direct(const Tens &X, const Mat &krp, int n, long k, Mat &out) {
+  long In = X.dims[n];
+  out = Mat(k, In);
+  if (n == 0) {
+    // X_(0) (I0 x rest, column-major) * KRP  -> I0 x k, stored transposed.
+    Mat T(In, k);
+    long rest = X.numel() / In;
+    gemm('N', 'N', In, k, rest, 1.0, X.x, In, krp.d.data(), krp.r, 0.0, T.d.data(), In);
+    for (long i = 0; i < In; ++i)
+      for (long c = 0; c < k; ++c) out(c, i) = T(i, c);
+    return;
+  }
+  long ncols = 1, nmats = 1;
+  for (int i = 0; i < n; ++i) ncols *= X.dims[i];
+  for (int i = n + 1; i < X.N; ++i) nmats *= X.dims[i];
+  // For each slab p: out(c, i_n) += sum_q X[q + ncols*(i_n + In*p)] * KRP(q + ncols*p, c).
+  Mat T(In, k);
+  for (long p = 0; p < nmats; ++p) {
+    // slab as (ncols x In) column-major: element (q, i_n) at q + ncols*i_n.
+    gemm('T', 'N', In, k, ncols, 1.0, X.x + (size_t)p * ncols * In, ncols,
+         krp.d.data() + p * ncols, krp.r, 1.0, T.d.data(), In);
+  }
+  for (long i = 0; i < In; ++i)
+    for (long c = 0; c < k; ++c) out(c, i) = T(i, c);
+}
+
+// Khatri-Rao product of the factors of modes [a, b) (lowest mode fastest).
+static Mat krp_range(const std::vector<Mat> &F, int a, int b, long k) {
+  long rows = 1;
+  for (int i = a; i < b; ++i) rows *= F[i].r;
+  Mat out(rows, k);
+#pragma omp parallel for schedule(static)
+  for (long r = 0; r < rows; ++r) {
+    long rem = r;
+    std::vector<long> dig(b - a);
+    for (int t = a; t < b; ++t) {
+      dig[t - a] = rem % F[t].r;
+      rem /= F[t].r;
+    }
+    for (long c = 0; c < k; ++c) {
+      double v = 1.0;
+      for (int t = b - 1; t >= a; --t) v *= F[t](dig[t - a], c);
+      out(r, c) = v;
+    }
+  }
+  return out;
+}
+
+// Contract a projection over all modes of [a, b) except n, with the factors.
+// Pr is (prod dims[a..b)) x k column-major (proj index: lowest mode fastest),
+// or k x (prod) when kfirst.  Out k x I_n.
+static void contract_projection(const Mat &Pr, bool kfirst, const std::vector<Mat> &F, int a,
+                                int b, int n, long k, Mat &out) {
+  long In = F[n].r;
+  long total = 1;
+  for (int i = a; i < b; ++i) total *= F[i].r;
+  out = Mat(k, In);
+#pragma omp parallel for schedule(static)
+  for (long c = 0; c < k; ++c) {
+    std::vector<long> dig(b - a);
+    for (long e = 0; e < total; ++e) {
+      long rem = e;
+      for (int t = a; t < b; ++t) {
+        dig[t - a] = rem % F[t].r;
+        rem /= F[t].r;
+      }
+      double w = kfirst ? Pr(c, e) : Pr(e, c);
+      for (int t = a; t < b; ++t)
+        if (t != n) w *= F[t](dig[t - a], c);
+      out(c, dig[n - a]) += w;
+    }
+  }
+}
+
+// Dimension-tree MTTKRP (dimtree/ddt.hpp:134-243 with the shared-memory split
+// s = N/2 of ntf/auntf.hpp:123): the right contraction X_(L x R)*KRP_R is formed
+// at n = 0 and reused for n <= s; the left contraction KRP_L^T*X_(L x R) at
+// n = s+1 and reused for n > s+1.  Mathematically equal to mttkrp_direct.
+struct DimTree {
+  int s;
+  Mat P;  // L x k
+  Mat Q;  // k x R
+};
+
+static void mttkrp_dimtree(const Tens &X, const std::vector<Mat> &F, int n, long k, DimTree &dt,
+                           Mat &out) {
+  int N = X.N, s = dt.s;
+  long L = 1, R = 1;
+  for (int i = 0; i <= s; ++i) L *= X.dims[i];
+  for (int i = s + 1; i < N; ++i) R *= X.dims[i];
+  if (n == 0) {
+    Mat KR = krp_range(F, s + 1, N, k);
+    dt.P = Mat(L, k);
+    gemm('N', 'N', L, k, R, 1.0, X.x, L, KR.d.data(), R, 0.0, dt.P.d.data(), L);
+  }
+  if (n == s + 1) {
+    Mat KL = krp_range(F, 0, s + 1, k);
+    dt.Q = Mat(k, R);
+    gemm('T', 'N', k, R, L, 1.0, KL.d.data(), L, X.x, L, 0.0, dt.Q.d.data(), k);
+  }
+  if (n <= s)
+    contract_projection(dt.P, false, F, 0, s + 1, n, k, out);
+  else
+    contract_projection(dt.Q, true, F, s + 1, N, n, k, out);
+}
+
+// ----------------------------------------------------------------------------
+// AUNTF (ntf/auntf.hpp) with the NCPFactors operations it uses.
+struct NTFState {
+  Tens X;
+  std::vector<Mat> F;  // I_i x k, unit-norm columns after normalisation
+  std::vector<double> lambda;
+  long k;
+  double normX2;
+  bool dimtree;
+  DimTree dt;
+};
+
+// NCPFactors::normalize(), common/ncpfactors.hpp:329-339.
+static void ncp_normalize_all(NTFState &s) {
+  std::fill(s.lambda.begin(), s.lambda.end(), 1.0);
+  for (size_t i = 0; i < s.F.size(); ++i)
+    for (long c = 0; c < s.k; ++c) {
+      double nrm = 0;
+      for (long r = 0; r < s.F[i].r; ++r) nrm += s.F[i](r, c) * s.F[i](r, c);
+      nrm = std::sqrt(nrm);
+      if (nrm > 0)
+        for (long r = 0; r < s.F[i].r; ++r) s.F[i](r, c) /= nrm;
+      s.lambda[c] *= nrm;
+    }
+}
+
+// NCPFactors::normalize(mode), common/ncpfactors.hpp:346-351: lambda replaced.
+static void ncp_normalize_mode(NTFState &s, int mode) {
+  Mat &A = s.F[mode];
+  for (long c = 0; c < s.k; ++c) {
+    double nrm = 0;
+    for (long r = 0; r < A.r; ++r) nrm += A(r, c) * A(r, c);
+    nrm = std::sqrt(nrm);
+    s.lambda[c] = nrm;
+    if (nrm > 0)
+      for (long r = 0; r < A.r; ++r) A(r, c) /= nrm;
+  }
+}
+
+// gram_leave_out_one, common/ncpfactors.hpp:139-148.
+static Mat gram_leave_out_one(const NTFState &s, int n) {
+  Mat S(s.k, s.k);
+  std::fill(S.d.begin(), S.d.end(), 1.0);
+  for (size_t i = 0; i < s.F.size(); ++i) {
+    if ((int)i == n) continue;
+    Mat G = mul(s.F[i], 'T', s.F[i], 'N');
+    for (size_t e = 0; e < S.d.size(); ++e) S.d[e] *= G.d[e];
+  }
+  return S;
+}
+
+// AUNTF::computeObjectiveError, ntf/auntf.hpp:200-246.
+static double ntf_rel_error(const NTFState &s) {
+  Mat krp;
+  krp_leave_out_one(s.F, 0, s.k, krp);
+  long I0 = s.X.dims[0], rest = krp.r;
+  Mat U = s.F[0];
+  for (long c = 0; c < s.k; ++c)
+    for (long i = 0; i < I0; ++i) U(i, c) *= s.lambda[c];
+  std::vector<double> low((size_t)I0 * rest);
+  gemm('N', 'T', I0, rest, s.k, 1.0, U.d.data(), I0, krp.d.data(), rest, 0.0, low.data(), I0);
+  double err = 0;
+  long ne = s.X.numel();
+#pragma omp parallel for reduction(+ : err) schedule(static)
+  for (long e = 0; e < ne; ++e) {
+    double d = s.X.x[e] - low[e];
+    err += d * d;
+  }
+  return std::sqrt(err / s.normX2);
+}
+
+// NTF update rules: NTFHALS::update (ntf/ntfhals.hpp:13-29), NTFMU::update
+// (ntf/ntfmu.hpp:11-16), NTFANLSBPP::update (ntf/ntfanlsbpp.hpp:15-57).
+// M is k x I_n, S k x k; the factor is updated in place (I_n x k).
+static int ntf_update(int algo, Mat &H, const Mat &M, const Mat &S, long k) {
+  long In = H.r;
+  if (algo == 1) {  // HALS
+    std::vector<double> u(In);
+    for (long i = 0; i < k; ++i) {
+      double nrm = 0;
+      for (long r = 0; r < In; ++r) {
+        double hs = 0;
+        for (long t = 0; t < k; ++t) hs += H(r, t) * S(t, i);
+        double v = H(r, i) + (M(i, r) - hs);
+        v = (v < EPS_1EM16) ? EPS_1EM16 : v;
+        u[r] = v;
+        nrm += v * v;
+      }
+      if (nrm > 0)
+        for (long r = 0; r < In; ++r) H(r, i) = u[r];
+    }
+  } else if (algo == 0) {  // MU
+    Mat T = mul(H, 'N', S, 'N');
+    for (long r = 0; r < In; ++r)
+      for (long c = 0; c < k; ++c) H(r, c) = (H(r, c) * M(c, r)) / (T(r, c) + EPS);
+  } else if (algo == 2) {  // ANLS-BPP, chunks of 2000 right-hand sides
+    const long CH = 2000;
+    long nch = (In + CH - 1) / CH;
+    std::vector<int> rc(nch, 0);
+#pragma omp parallel for schedule(dynamic)
+    for (long ch = 0; ch < nch; ++ch) {
+      long c0 = ch * CH, c1 = std::min(In, c0 + CH);
+      std::vector<double> X((size_t)k * (c1 - c0));
+      rc[ch] = bppnnls(S.d.data(), (int)k, M.col(c0), c1 - c0, X.data());
+      for (long r = c0; r < c1; ++r)
+        for (long c = 0; c < k; ++c) H(r, c) = X[c + (size_t)k * (r - c0)];
+    }
+    for (long ch = 0; ch < nch; ++ch)
+      if (rc[ch] < 0) return -1;
+  } else {
+    return -2;
+  }
+  return 0;
+}
+
+// AUNTF::computeNTF, ntf/auntf.hpp:128-177.
+static int ntf_loop(NTFState &s, int algo, int iters, double *relerr) {
+  int N = s.X.N;
+  Mat krp, M;
+  for (int it = 0; it < iters; ++it) {
+    for (int j = 0; j < N; ++j) {
+      Mat S = gram_leave_out_one(s, j);
+      if (s.dimtree) {
+        mttkrp_dimtree(s.X, s.F, j, s.k, s.dt, M);
+      } else {
+        krp_leave_out_one(s.F, j, s.k, krp);
+        mttkrp_direct(s.X, krp, j, s.k, M);
+      }
+      int rc = ntf_update(algo, s.F[j], M, S, s.k);
+      if (rc < 0) return rc;
+      ncp_normalize_mode(s, j);
+    }
+    if (relerr) relerr[it] = ntf_rel_error(s);
+  }
+  return 0;
+}
+
+}  // namespace orc
+
+using namespace orc;
+
+// ============================================================================
+// C entry points (loaded with ctypes by the tests and bench.py).
+extern "C" {
+
+void oracle_set_threads(int n) { omp_set_num_threads(n); }
+int oracle_get_threads() { return omp_get_max_threads(); }
+
+// C = op(A) * op(B), column-major.
+void oracle_gemm(char ta, char tb, long M, long N, long K, const double *A, long lda,
+                 const double *B, long ldb, double *C, long ldc) {
+  gemm(ta, tb, M, N, K, 1.0, A, lda, B, ldb, 0.0, C, ldc);
+}
+
+int oracle_bppnnls(const double *AtA, int n, const double *AtB, long nrhs, double *X) {
+  return bppnnls(AtA, n, AtB, nrhs, X);
+}
+
+// Runs `iters` iterations of the shared-memory NMF `algo` (0 MU, 1 HALS,
+// 2 ANLS-BPP) from the given W (m x k) and H (n x k), both overwritten, as
+// NMF(A, W, H) + computeNMF() does (common/nmf.hpp:206-222).  relerr[it] is
+// printObjective's relative error after iteration it; objective[it] the
+// objective.  Returns 0, or -1 when BPP does not converge.
+int oracle_nmf(int algo, const double *A, long m, long n, double *W, double *H, int k, int iters,
+               const double *regW, const double *regH, double symm_reg, double *relerr,
+               double *objective) {
+  Mat Am(m, n);
+  std::memcpy(Am.d.data(), A, sizeof(double) * m * n);
+  NMFState s;
+  s.A = &Am;
+  s.m = m;
+  s.n = n;
+  s.k = k;
+  s.W = Mat(m, k);
+  s.H = Mat(n, k);
+  std::memcpy(s.W.d.data(), W, sizeof(double) * m * k);
+  std::memcpy(s.H.d.data(), H, sizeof(double) * n * k);
+  for (int i = 0; i < 2; ++i) {
+    s.regW[i] = regW ? regW[i] : 0.0;
+    s.regH[i] = regH ? regH[i] : 0.0;
+  }
+  s.symm_reg = symm_reg;
+  s.normA = std::sqrt(fro2(Am));
+  int rc = 0;
+  if (algo == 0) {
+    for (int it = 0; it < iters; ++it) {
+      // One MU iteration without the trailing normalisation.
+      Mat AtW = mul(*s.A, 'T', s.W, 'N');
+      Mat WtW = mul(s.W, 'T', s.W, 'N');
+      apply_reg(s.regH, WtW);
+      Mat HWtW = mul(s.H, 'N', WtW, 'N');
+      for (size_t e = 0; e < s.H.d.size(); ++e)
+        s.H.d[e] = (s.H.d[e] * AtW.d[e]) / (HWtW.d[e] + EPS_1EM16);
+      Mat AH = mul(*s.A, 'N', s.H, 'N');
+      Mat HtH = mul(s.H, 'T', s.H, 'N');
+      apply_reg(s.regW, HtH);
+      Mat WHtH = mul(s.W, 'N', HtH, 'N');
+      for (size_t e = 0; e < s.W.d.size(); ++e)
+        s.W.d[e] = (s.W.d[e] * AH.d[e]) / (WHtH.d[e] + EPS_1EM16);
+      objective_error(s);
+      if (relerr) relerr[it] = rel_error(s);
+      if (objective) objective[it] = s.objective;
+    }
+    normalize_by_W(s);  // nmf/mu.hpp:90
+  } else if (algo == 1) {
+    normalize_by_W(s);  // HALSNMF constructor, nmf/hals.hpp:46
+    for (int it = 0; it < iters; ++it) {
+      hals_loop(s, 1, relerr ? relerr + it : nullptr);
+      if (objective) objective[it] = s.objective;
+    }
+  } else if (algo == 2) {
+    for (int it = 0; it < iters; ++it) {
+      if (bpp_update(s, true, s.H, s.regW, s.W) < 0) {
+        rc = -1;
+        break;
+      }
+      if (bpp_update(s, false, s.W, s.regH, s.H) < 0) {
+        rc = -1;
+        break;
+      }
+      objective_error(s);
+      if (relerr) relerr[it] = rel_error(s);
+      if (objective) objective[it] = s.objective;
+    }
+    if (rc == 0) normalize_by_W(s);  // nmf/bppnmf.hpp:236
+  } else {
+    return -2;
+  }
+  std::memcpy(W, s.W.d.data(), sizeof(double) * m * k);
+  std::memcpy(H, s.H.d.data(), sizeof(double) * n * k);
+  return rc;
+}
+
+// KRP leaving out mode n; out is (prod others) x k column-major.
+void oracle_krp_leave_out_one(int N, const long *dims, const double *const *factors, int k, int n,
+                              double *out) {
+  std::vector<Mat> F(N);
+  for (int i = 0; i < N; ++i) {
+    F[i] = Mat(dims[i], k);
+    std::memcpy(F[i].d.data(), factors[i], sizeof(double) * dims[i] * k);
+  }
+  Mat K;
+  krp_leave_out_one(F, n, k, K);
+  std::memcpy(out, K.d.data(), sizeof(double) * K.d.size());
+}
+
+// MTTKRP of mode n, out k x I_n.  method 0: Tensor::mttkrp with a formed KRP;
+// method 1: dimension tree with split s (the full sweep 0..n is replayed so the
+// projections are formed as in a real sweep).
+void oracle_mttkrp(int N, const long *dims, const double *X, const double *const *factors, int k,
+                   int n, int method, int split, double *out) {
+  Tens T{N, std::vector<long>(dims, dims + N), X};
+  std::vector<Mat> F(N);
+  for (int i = 0; i < N; ++i) {
+    F[i] = Mat(dims[i], k);
+    std::memcpy(F[i].d.data(), factors[i], sizeof(double) * dims[i] * k);
+  }
+  Mat M;
+  if (method == 0) {
+    Mat K;
+    krp_leave_out_one(F, n, k, K);
+    mttkrp_direct(T, K, n, k, M);
+  } else {
+    DimTree dt;
+    dt.s = split;
+    for (int j = 0; j <= n; ++j) mttkrp_dimtree(T, F, j, k, dt, M);
+  }
+  std::memcpy(out, M.d.data(), sizeof(double) * M.d.size());
+}
+
+// AUNTF run: factors (I_i x k) are the raw initial factors; they are first
+// normalised with NCPFactors::normalize() (as the AUNTF constructor does,
+// ntf/auntf.hpp:88), then `iters` iterations of algo (0 MU, 1 HALS, 2 BPP)
+// run.  Factors and lambda are returned; relerr[it] is the relative error
+// after iteration it (computeObjectiveError).  dimtree: 0 direct MTTKRP,
+// 1 dimension tree with split N/2.
+int oracle_ntf(int algo, int N, const long *dims, const double *X, double *const *factors, int k,
+               int iters, int dimtree, double *lambda, double *relerr) {
+  NTFState s;
+  s.X = Tens{N, std::vector<long>(dims, dims + N), X};
+  s.k = k;
+  s.F.resize(N);
+  for (int i = 0; i < N; ++i) {
+    s.F[i] = Mat(dims[i], k);
+    std::memcpy(s.F[i].d.data(), factors[i], sizeof(double) * dims[i] * k);
+  }
+  s.lambda.assign(k, 1.0);
+  double nx = 0;
+  long ne = s.X.numel();
+#pragma omp parallel for reduction(+ : nx)
+  for (long e = 0; e < ne; ++e) nx += X[e] * X[e];
+  s.normX2 = nx;
+  s.dimtree = dimtree != 0 && N >= 3;
+  s.dt.s = N / 2;
+  ncp_normalize_all(s);
+  int rc = ntf_loop(s, algo, iters, relerr);
+  for (int i = 0; i < N; ++i) std::memcpy(factors[i], s.F[i].d.data(), sizeof(double) * dims[i] * k);
+  for (int c = 0; c < k; ++c) lambda[c] = s.lambda[c];
+  return rc;
+}
+
+// CP loss 0.5 ||X - [[A_0..A_{N-1}]]||^2 (no lambda) and, when grads is
+// non-null, its gradient with respect to every factor:
+// dL/dA_n = A_n * (Hadamard of the other Grams) - MTTKRP_n^T.  This is the
+// transposed NTFNES gradient S*Y - M of ntf/ntfnes.hpp:138-147.
+double oracle_cp_loss(int N, const long *dims, const double *X, const double *const *factors,
+                      int k, double *const *grads) {
+  Tens T{N, std::vector<long>(dims, dims + N), X};
+  std::vector<Mat> F(N);
+  for (int i = 0; i < N; ++i) {
+    F[i] = Mat(dims[i], k);
+    std::memcpy(F[i].d.data(), factors[i], sizeof(double) * dims[i] * k);
+  }
+  Mat K;
+  krp_leave_out_one(F, 0, k, K);
+  long I0 = dims[0], rest = K.r;
+  std::vector<double> low((size_t)I0 * rest);
+  gemm('N', 'T', I0, rest, k, 1.0, F[0].d.data(), I0, K.d.data(), rest, 0.0, low.data(), I0);
+  double err = 0;
+  long ne = T.numel();
+#pragma omp parallel for reduction(+ : err)
+  for (long e = 0; e < ne; ++e) {
+    double d = X[e] - low[e];
+    err += d * d;
+  }
+  if (grads) {
+    for (int n = 0; n < N; ++n) {
+      Mat S(k, k);
+      std::fill(S.d.begin(), S.d.end(), 1.0);
+      for (int i = 0; i < N; ++i) {
+        if (i == n) continue;
+        Mat G = mul(F[i], 'T', F[i], 'N');
+        for (size_t e = 0; e < S.d.size(); ++e) S.d[e] *= G.d[e];
+      }
+      Mat Kn, M;
+      krp_leave_out_one(F, n, k, Kn);
+      mttkrp_direct(T, Kn, n, k, M);
+      Mat AS = mul(F[n], 'N', S, 'N');
+      for (long r = 0; r < dims[n]; ++r)
+        for (long c = 0; c < k; ++c) grads[n][r + dims[n] * c] = AS(r, c) - M(c, r);
+    }
+  }
+  return 0.5 * err;
+}
+
+// DistR2::update, distnmf/distr2.hpp:35-61, for one side: left is the 2x2
+// Gram (column-major), right the 2 x n right-hand side; out receives the 2 x n
+// solution.
+void oracle_r2_update(const double *left, const double *right, long n, double *out) {
+  double a = left[0], c = left[1], b = left[2], d = left[3];  // column-major [[a b],[c d]]
+  // arma::solve(fast) on a 2x2 system = LU with partial pivoting (dgesv).
+  bool sw = std::fabs(c) > std::fabs(a);
+  double u11 = sw ? c : a, u12 = sw ? d : b;
+  double l21 = (sw ? a : c) / u11;
+  double u22 = (sw ? b : d) - l21 * u12;
+  double b0 = std::sqrt(a), b1 = std::sqrt(d);
+  for (long j = 0; j < n; ++j) {
+    double r0 = right[2 * j], r1 = right[2 * j + 1];
+    double z0 = sw ? r1 : r0, z1 = (sw ? r0 : r1) - l21 * z0;
+    double g1 = z1 / u22;
+    double g0 = (z0 - u12 * g1) / u11;
+    if (g0 < 0 || g1 < 0) {
+      double u = r0 / a, v = r1 / d;
+      if (b0 * u >= b1 * v)
+        v = 0;
+      else
+        u = 0;
+      g0 = u;
+      g1 = v;
+    }
+    out[2 * j] = g0;
+    out[2 * j + 1] = g1;
+  }
+}
+
+}  // extern "C"
