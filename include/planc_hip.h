@@ -1,0 +1,192 @@
+/*
+ * planc_hip.h -- C ABI of the MI355X (gfx950) implementation of PLANC's
+ * shared-memory NMF/NTF hot path.
+ *
+ * The reference (PLANC) binds its arithmetic through CBLAS/LAPACK calls and
+ * Armadillo expressions inside C++ class templates; each entry point below
+ * replaces one of those call sites or one of the class-level loops, cited as
+ * reference file:line (paths relative to the PLANC source tree).
+ *
+ * Conventions
+ *  - Matrices are column-major (Armadillo's layout): W is m x k, H is n x k,
+ *    NTF factors are I_n x k, an MTTKRP result is k x I_n, tensors are
+ *    stored with mode 0 fastest.  Leading dimensions are in elements.
+ *  - Every pointer named d_* is device memory (hipMalloc / planc_malloc).
+ *    Host buffers are named h_*.
+ *  - Calls are ordered on the context's stream and return once enqueued,
+ *    except the *_run entry points, planc_sync and the copy helpers, which
+ *    return after the stream has drained.
+ *  - Every function returns 0 (PLANC_OK) or a negative status; no C++
+ *    exception crosses this boundary.  planc_last_error() describes the last
+ *    failure on a context.
+ *  - One context per host thread.  The caller owns every buffer it passes;
+ *    the context owns its scratch.
+ */
+#ifndef PLANC_HIP_H_
+#define PLANC_HIP_H_
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes (replace the reference's exit()/assert failure paths). */
+#define PLANC_OK 0
+#define PLANC_EINVAL (-1)     /* invalid argument (shape, pointer, enum) */
+#define PLANC_EHIP (-2)       /* HIP runtime error */
+#define PLANC_ENOCONV (-3)    /* BPP exceeded 5k pivot iterations (nnls/bppnnls.hpp:185-188) */
+#define PLANC_ENOMEM (-4)     /* device allocation failed */
+#define PLANC_EUNSUPPORTED (-5)
+
+/* Arithmetic of the bulk products (A*H, W^T*A, MTTKRP).  FP32 is the plain
+ * form; BF16/FP16 read a 16-bit copy of A / X on MFMA with FP32
+ * accumulation.  Grams, normal equations and error sums are FP64 always. */
+#define PLANC_F32 0
+#define PLANC_BF16 1
+#define PLANC_F16 2
+
+/* Update algorithms; same numbering as the reference's algotype enum
+ * (common/utils.h:10-11). */
+#define PLANC_MU 0
+#define PLANC_HALS 1
+#define PLANC_ANLSBPP 2
+#define PLANC_R2 8
+
+typedef struct planc_ctx planc_ctx;
+
+/* ---- context, memory, streams ------------------------------------------- */
+const char *planc_version(void);
+int planc_create(int device, planc_ctx **out);
+int planc_destroy(planc_ctx *ctx);
+int planc_sync(planc_ctx *ctx);
+const char *planc_last_error(planc_ctx *ctx);
+/* Use an existing hipStream_t (passed as void*) instead of the context's own. */
+int planc_set_stream(planc_ctx *ctx, void *stream);
+void *planc_get_stream(planc_ctx *ctx);
+int planc_malloc(planc_ctx *ctx, void **d_ptr, size_t bytes);
+int planc_free(planc_ctx *ctx, void *d_ptr);
+int planc_memcpy_h2d(planc_ctx *ctx, void *d_dst, const void *h_src, size_t bytes);
+int planc_memcpy_d2h(planc_ctx *ctx, void *h_dst, const void *d_src, size_t bytes);
+
+/* ---- data preparation ---------------------------------------------------- */
+/* 16-bit copy of an FP32 matrix (rows x cols, column-major) for the BF16/FP16
+ * forms.  When d_colscale is non-null each column j is divided by a power of
+ * two s_j (written to d_colscale[j]) chosen so that its largest magnitude
+ * lies in [0.5, 1); the products undo the scale exactly.  No reference
+ * counterpart (the reference computes in FP64 only). */
+int planc_convert(planc_ctx *ctx, int dtype, const float *d_src, long rows, long cols, long ld_src,
+                  void *d_dst, long ld_dst, float *d_colscale);
+
+/* ---- kernel-level entry points ------------------------------------------- */
+/* C (L x k) = A (L x K) * B (K x k).  Replaces the dgemm behind `A * H`
+ * (nmf/mu.hpp:71, nmf/hals.hpp:81) and, with B a formed KRP, the mode-0
+ * MTTKRP dgemm (common/tensor.hpp:279-281).  dtype is that of A; B is FP32
+ * and is rounded to dtype on the fly; d_colscale (optional, K) are the column
+ * scales produced by planc_convert for A. */
+int planc_gemm_nn(planc_ctx *ctx, int dtype, const void *d_A, long lda, long L, long K,
+                  const float *d_colscale, const float *d_B, long ldb, int k, float *d_C, long ldc);
+
+/* C (k x L) = B^T (k x K) * A (K x L).  Replaces `W.t() * A`
+ * (nmf/hals.hpp:57, nmf/bppnmf.hpp:39) and `At * W` (nmf/mu.hpp:55, returned
+ * transposed).  d_colscale (optional, L) scales the output columns. */
+int planc_gemm_tn(planc_ctx *ctx, int dtype, const void *d_A, long lda, long K, long L,
+                  const float *d_colscale, const float *d_B, long ldb, int k, float *d_C, long ldc);
+
+/* G (k x k, FP64) = X^T X for X rows x k (FP32).  Replaces the Gram products
+ * `W.t() * W` / `H.t() * H` (nmf/mu.hpp:56,72; common/ncpfactors.hpp:144). */
+int planc_gram(planc_ctx *ctx, const float *d_X, long rows, int k, long ldx, double *d_G);
+
+/* MU update H <- H .* N ./ (H * G + eps), in place, for H n x k, the
+ * numerator N n x k with element (j, c) at d_N[j*n_rs + c*n_cs] (so a k x n
+ * product such as W^T A is passed with n_rs = k, n_cs = 1) and G k x k FP64.
+ * nmf/mu.hpp:64,79 (eps = 1e-17); ntf/ntfmu.hpp:13 (eps = 1e-6). */
+int planc_mu_update(planc_ctx *ctx, float *d_H, long n, int k, long ldh, const float *d_N,
+                    long n_rs, long n_cs, const double *d_G, double eps);
+
+/* HALS sweep over the columns of H whose rows are independent:
+ * for x = 0..k-1: H(:,x) <- max(H(:,x) + N(:,x) - H * G(:,x), 1e-17)
+ * (values below 1e-17 become 1e-17); N as in planc_mu_update.
+ * nmf/hals.hpp:67-75 and ntf/ntfhals.hpp:13-29. */
+int planc_hals_update_rows(planc_ctx *ctx, float *d_H, long n, int k, long ldh, const float *d_N,
+                           long n_rs, long n_cs, const double *d_G);
+
+/* HALS sweep over the columns of W with per-column normalisation (Gauss-Seidel
+ * across columns, one grid-wide reduction per column):
+ * Wx = W(:,x)*G(x,x) + AH(:,x) - W*G(:,x); clamp 1e-17; W(:,x) = Wx/||Wx||.
+ * nmf/hals.hpp:89-102. */
+int planc_hals_update_cols(planc_ctx *ctx, float *d_W, long m, int k, long ldw, const float *d_AH,
+                           long ldah, const double *d_G);
+
+/* normalize_by_W (common/nmf.hpp:152-161): W columns to unit 2-norm, H
+ * columns multiplied by the same norms; zero columns are left alone. */
+int planc_normalize_by_w(planc_ctx *ctx, float *d_W, long m, long ldw, float *d_H, long n,
+                         long ldh, int k);
+
+/* NCPFactors::normalize(mode) (common/ncpfactors.hpp:346-351): columns of A
+ * (rows x k) to unit norm, d_lambda (k, FP64) replaced by the norms. */
+int planc_normalize_cols(planc_ctx *ctx, float *d_A, long rows, int k, long lda, double *d_lambda);
+
+/* Hadamard product of the k x k FP64 Grams d_grams[i], i != skip
+ * (gram_leave_out_one, common/ncpfactors.hpp:139-148).  d_grams is a host
+ * array of device pointers. */
+int planc_hadamard_grams(planc_ctx *ctx, const double *const *d_grams, int nmodes, int skip, int k,
+                         double *d_out);
+
+/* MTTKRP of mode `mode` for a dense tensor (mode 0 fastest, dims[nmodes]),
+ * with the Khatri-Rao product formed on the fly, never stored.  Output is
+ * k x I_mode (ld ldo).  Replaces Tensor::mttkrp (common/tensor.hpp:246-327)
+ * plus krp_leave_out_one (common/ncpfactors.hpp:171-233).  d_factors is a
+ * host array of device pointers (I_i x k, column-major).  nmodes <= 8. */
+int planc_mttkrp(planc_ctx *ctx, int dtype, const void *d_X, int nmodes, const long *dims,
+                 const float *const *d_factors, int k, int mode, float *d_out, long ldo);
+
+/* ---- whole loops (computeNMF / computeNTF on the device) ----------------- */
+typedef struct planc_nmf_desc {
+  int algo;          /* PLANC_MU, PLANC_HALS, PLANC_ANLSBPP */
+  int dtype;         /* PLANC_F32 / PLANC_BF16 / PLANC_F16 */
+  long m, n;         /* A is m x n */
+  int k;             /* rank */
+  int iters;         /* outer iterations (num_iterations, common/nmf.hpp:481) */
+  double regW[2];    /* {L2, L1} (regW, common/nmf.hpp:487) */
+  double regH[2];
+  int compute_error; /* 1: relative error every iteration (always on in the
+                        reference's shared-memory driver, nmf/nmf.cpp:205) */
+} planc_nmf_desc;
+
+/* Runs desc->iters iterations of MUNMF / HALSNMF / BPPNMF computeNMF
+ * (nmf/mu.hpp:48-91, nmf/hals.hpp:50-116, nmf/bppnmf.hpp:187-241) from the
+ * initial W (m x k) and H (n x k), overwriting them, including the
+ * constructor/final normalize_by_W of each algorithm.  d_A is FP32 m x n
+ * (lda); when dtype is 16-bit, d_A16 (optional) is a planc_convert copy with
+ * d_colscale, else one is made in scratch.  h_relerr (host, iters, optional)
+ * receives the relative error of printObjective (common/nmf.hpp:461-468). */
+int planc_nmf_run(planc_ctx *ctx, const planc_nmf_desc *desc, const float *d_A, long lda,
+                  const void *d_A16, const float *d_colscale, float *d_W, float *d_H,
+                  double *h_relerr);
+
+typedef struct planc_ntf_desc {
+  int algo;          /* PLANC_MU, PLANC_HALS, PLANC_ANLSBPP */
+  int dtype;         /* of the MTTKRP reads of X */
+  int nmodes;        /* 3..8 */
+  long dims[8];
+  int k;
+  int iters;
+  int dimtree;       /* 1: dimension-tree MTTKRP (2 full passes over X per sweep) */
+  int compute_error; /* relative error after each iteration (-e 1) */
+} planc_ntf_desc;
+
+/* Runs AUNTF::computeNTF (ntf/auntf.hpp:128-177) with NTFHALS / NTFMU /
+ * NTFANLSBPP updates.  d_factors (host array of device pointers, I_i x k)
+ * hold the initial factors, which are first normalised as NCPFactors::
+ * normalize() does in the AUNTF constructor (ntf/auntf.hpp:88); on return
+ * they hold the unit-norm factors and d_lambda (k, FP64) the weights.
+ * d_X16 optional 16-bit copy of X for dtype != F32. */
+int planc_ntf_run(planc_ctx *ctx, const planc_ntf_desc *desc, const float *d_X, const void *d_X16,
+                  float *const *d_factors, double *d_lambda, double *h_relerr);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PLANC_HIP_H_ */

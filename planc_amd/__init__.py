@@ -1,0 +1,244 @@
+"""planc_amd -- MI355X (gfx950) implementation of PLANC's shared-memory NMF/NTF
+hot path behind a C ABI (include/planc_hip.h, libplanc_hip.so).
+
+This module is the Python face of that ABI, used by the tests and bench.py;
+the C++ face is planc_amd/host (NMF<T>, AUNTF, NCPFactors, ParseCommandLine).
+Matrices follow Armadillo's column-major layout: a torch tensor of shape
+(m, n) with strides (1, ld).  Tensors are stored with mode 0 fastest
+(strides (1, I0, I0*I1, ...)).  PyTorch only provides device memory and the
+stream; every computation is a HIP kernel of libplanc_hip.so.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import ANLSBPP, BF16, F16, F32, HALS, MU, R2, NMFDesc, NTFDesc  # noqa: F401
+
+__all__ = ["Context", "PlancError", "colmajor", "F32", "BF16", "F16", "MU", "HALS", "ANLSBPP"]
+
+
+class PlancError(RuntimeError):
+    def __init__(self, name, rc, msg):
+        super().__init__(f"{name} failed with status {rc}: {msg}")
+        self.rc = rc
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def colmajor(a, device="cuda", dtype=None):
+    """numpy array -> torch device tensor with Fortran (column-major) strides."""
+    torch = _torch()
+    arr = np.asfortranarray(a if dtype is None else np.asarray(a, dtype=dtype))
+    t = torch.from_numpy(arr)
+    return t.to(device)
+
+
+def empty_colmajor(shape, dtype=None, device="cuda"):
+    torch = _torch()
+    dtype = dtype or torch.float32
+    rev = tuple(reversed(shape))
+    t = torch.empty(rev, dtype=dtype, device=device)
+    return t.permute(*reversed(range(len(shape))))
+
+
+def _ld(t):
+    """Leading dimension of a column-major 2-D tensor."""
+    if t.dim() != 2 or t.stride(0) != 1:
+        raise ValueError("expected a column-major 2-D tensor (stride(0) == 1)")
+    return max(t.stride(1), t.shape[0])
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
+
+
+def _is_fortran_tensor(t):
+    exp = 1
+    for d in range(t.dim()):
+        if t.shape[d] > 1 and t.stride(d) != exp:
+            return False
+        exp *= t.shape[d]
+    return True
+
+
+class Context:
+    """One planc_ctx (device, stream, scratch).  Calls are ordered on torch's
+    current stream of that device."""
+
+    def __init__(self, device=0, use_torch_stream=True):
+        self.lib = _lib.load()
+        self.device = device
+        p = ctypes.c_void_p()
+        rc = self.lib.planc_create(device, ctypes.byref(p))
+        if rc != 0:
+            raise PlancError("planc_create", rc, "no HIP device")
+        self.ctx = p
+        if use_torch_stream:
+            torch = _torch()
+            s = torch.cuda.current_stream(device).cuda_stream
+            self._check(self.lib.planc_set_stream(self.ctx, ctypes.c_void_p(s)), "planc_set_stream")
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.lib.planc_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc, name):
+        if rc != 0:
+            msg = self.lib.planc_last_error(self.ctx)
+            raise PlancError(name, rc, msg.decode() if msg else "")
+
+    def sync(self):
+        self._check(self.lib.planc_sync(self.ctx), "planc_sync")
+
+    # ---- kernel-level ----------------------------------------------------
+    def convert(self, A, dtype, scaled=True):
+        """16-bit copy of a column-major FP32 matrix; returns (A16, colscale)."""
+        torch = _torch()
+        m, n = A.shape
+        A16 = empty_colmajor((m, n), dtype=torch.int16, device=A.device)
+        cs = torch.empty(n, dtype=torch.float32, device=A.device) if scaled else None
+        self._check(self.lib.planc_convert(self.ctx, dtype, _ptr(A), m, n, _ld(A), _ptr(A16), m,
+                                           _ptr(cs)), "planc_convert")
+        return A16, cs
+
+    def gemm_nn(self, A, B, dtype=F32, colscale=None):
+        """C = A (L x K) * B (K x k); A may be a 16-bit copy from convert()."""
+        torch = _torch()
+        L, K = A.shape
+        k = B.shape[1]
+        C = empty_colmajor((L, k), device=A.device)
+        self._check(self.lib.planc_gemm_nn(self.ctx, dtype, _ptr(A), _ld(A), L, K, _ptr(colscale),
+                                           _ptr(B), _ld(B), k, _ptr(C), L), "planc_gemm_nn")
+        return C
+
+    def gemm_tn(self, A, B, dtype=F32, colscale=None):
+        """C (k x L) = B^T (k x K) * A (K x L)."""
+        K, L = A.shape
+        k = B.shape[1]
+        C = empty_colmajor((k, L), device=A.device)
+        self._check(self.lib.planc_gemm_tn(self.ctx, dtype, _ptr(A), _ld(A), K, L, _ptr(colscale),
+                                           _ptr(B), _ld(B), k, _ptr(C), k), "planc_gemm_tn")
+        return C
+
+    def gram(self, X):
+        torch = _torch()
+        rows, k = X.shape
+        G = empty_colmajor((k, k), dtype=torch.float64, device=X.device)
+        self._check(self.lib.planc_gram(self.ctx, _ptr(X), rows, k, _ld(X), _ptr(G)), "planc_gram")
+        return G
+
+    def mu_update(self, H, N, G, eps, n_rs=None, n_cs=None):
+        n, k = H.shape
+        if n_rs is None:
+            n_rs, n_cs = 1, _ld(N)
+        self._check(self.lib.planc_mu_update(self.ctx, _ptr(H), n, k, _ld(H), _ptr(N), n_rs, n_cs,
+                                             _ptr(G), float(eps)), "planc_mu_update")
+
+    def hals_update_rows(self, H, N, G, n_rs=None, n_cs=None):
+        n, k = H.shape
+        if n_rs is None:
+            n_rs, n_cs = 1, _ld(N)
+        self._check(self.lib.planc_hals_update_rows(self.ctx, _ptr(H), n, k, _ld(H), _ptr(N), n_rs,
+                                                    n_cs, _ptr(G)), "planc_hals_update_rows")
+
+    def hals_update_cols(self, W, AH, G):
+        m, k = W.shape
+        self._check(self.lib.planc_hals_update_cols(self.ctx, _ptr(W), m, k, _ld(W), _ptr(AH),
+                                                    _ld(AH), _ptr(G)), "planc_hals_update_cols")
+
+    def normalize_by_w(self, W, H):
+        m, k = W.shape
+        n = H.shape[0]
+        self._check(self.lib.planc_normalize_by_w(self.ctx, _ptr(W), m, _ld(W), _ptr(H), n, _ld(H),
+                                                  k), "planc_normalize_by_w")
+
+    def normalize_cols(self, A):
+        torch = _torch()
+        rows, k = A.shape
+        lam = torch.empty(k, dtype=torch.float64, device=A.device)
+        self._check(self.lib.planc_normalize_cols(self.ctx, _ptr(A), rows, k, _ld(A), _ptr(lam)),
+                    "planc_normalize_cols")
+        return lam
+
+    def hadamard_grams(self, grams, skip):
+        torch = _torch()
+        k = grams[0].shape[0]
+        out = empty_colmajor((k, k), dtype=torch.float64, device=grams[0].device)
+        arr = (ctypes.c_void_p * len(grams))(*[g.data_ptr() for g in grams])
+        self._check(self.lib.planc_hadamard_grams(self.ctx, arr, len(grams), skip, k, _ptr(out)),
+                    "planc_hadamard_grams")
+        return out
+
+    def mttkrp(self, X, factors, mode, dtype=F32):
+        """MTTKRP of `mode` (k x I_mode); X has mode-0-fastest strides."""
+        N = len(factors)
+        if not _is_fortran_tensor(X) and dtype == F32:
+            raise ValueError("X must be stored mode-0 fastest")
+        dims = (ctypes.c_long * N)(*[f.shape[0] for f in factors])
+        k = factors[0].shape[1]
+        out = empty_colmajor((k, factors[mode].shape[0]), device=factors[0].device)
+        arr = (ctypes.c_void_p * N)(*[f.data_ptr() for f in factors])
+        self._check(self.lib.planc_mttkrp(self.ctx, dtype, _ptr(X), N, dims, arr, k, mode, _ptr(out),
+                                          k), "planc_mttkrp")
+        return out
+
+    # ---- whole loops -------------------------------------------------------
+    def nmf_run(self, A, W, H, algo, iters, dtype=F32, regW=(0.0, 0.0), regH=(0.0, 0.0),
+                compute_error=True, A16=None, colscale=None):
+        """computeNMF on the device; W (m x k) and H (n x k) are updated in
+        place.  Returns the per-iteration relative error (numpy)."""
+        m, n = A.shape
+        k = W.shape[1]
+        d = NMFDesc()
+        d.algo, d.dtype, d.m, d.n, d.k, d.iters = algo, dtype, m, n, k, iters
+        d.regW[0], d.regW[1] = regW
+        d.regH[0], d.regH[1] = regH
+        d.compute_error = 1 if compute_error else 0
+        if _ld(W) != m or _ld(H) != n:
+            raise ValueError("W and H must be tight column-major (ld = m, n)")
+        rel = np.zeros(max(iters, 1))
+        self._check(self.lib.planc_nmf_run(self.ctx, ctypes.byref(d), _ptr(A), _ld(A), _ptr(A16),
+                                           _ptr(colscale), _ptr(W), _ptr(H),
+                                           rel.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+                    "planc_nmf_run")
+        return rel[:iters]
+
+    def ntf_run(self, X, factors, algo, iters, dtype=F32, dimtree=True, compute_error=True,
+                X16=None):
+        """computeNTF on the device; factors (I_i x k, tight column-major) are
+        updated in place.  Returns (lambda, relerr)."""
+        torch = _torch()
+        N = len(factors)
+        d = NTFDesc()
+        d.algo, d.dtype, d.nmodes = algo, dtype, N
+        for i, f in enumerate(factors):
+            d.dims[i] = f.shape[0]
+            if _ld(f) != f.shape[0]:
+                raise ValueError("factors must be tight column-major")
+        d.k = factors[0].shape[1]
+        d.iters, d.dimtree, d.compute_error = iters, 1 if dimtree else 0, 1 if compute_error else 0
+        lam = torch.empty(d.k, dtype=torch.float64, device=factors[0].device)
+        arr = (ctypes.c_void_p * N)(*[f.data_ptr() for f in factors])
+        rel = np.zeros(max(iters, 1))
+        self._check(self.lib.planc_ntf_run(self.ctx, ctypes.byref(d), _ptr(X), _ptr(X16), arr,
+                                           _ptr(lam),
+                                           rel.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+                    "planc_ntf_run")
+        return lam, rel[:iters]

@@ -1,0 +1,511 @@
+// C ABI of libplanc_hip.so: contexts, memory helpers, kernel-level entry
+// points and the device-resident computeNMF / computeNTF loops.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "common.h"
+
+namespace planc {
+
+int set_error(planc_ctx *ctx, int code, const std::string &msg) {
+  if (ctx) ctx->last_error = msg;
+  return code;
+}
+
+void *ws_get(planc_ctx *ctx, int slot, size_t bytes) {
+  Workspace &w = ctx->ws[slot];
+  if (bytes == 0) bytes = 256;
+  if (w.bytes >= bytes) return w.ptr;
+  if (w.ptr) {
+    // Queued kernels may still read the old buffer.
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) return nullptr;
+    (void)hipFree(w.ptr);
+    w.ptr = nullptr;
+    w.bytes = 0;
+  }
+  size_t b = bytes + bytes / 8;
+  if (hipMalloc(&w.ptr, b) != hipSuccess) {
+    w.ptr = nullptr;
+    return nullptr;
+  }
+  w.bytes = b;
+  return w.ptr;
+}
+
+// Dimension-tree split: the s in [0, N-2] minimising the larger of the two
+// projections (left product L = prod_{i<=s} I_i, right R = prod_{i>s} I_i);
+// the reference's s = N/2 (ntf/auntf.hpp:123) when it ties.
+int balanced_split(int N, const long *dims) {
+  int best = N / 2 < N - 1 ? N / 2 : N - 2;
+  long bestv = -1;
+  for (int s = 0; s <= N - 2; ++s) {
+    long L = 1, R = 1;
+    for (int i = 0; i <= s; ++i) L *= dims[i];
+    for (int i = s + 1; i < N; ++i) R *= dims[i];
+    long v = std::max(L, R);
+    if (bestv < 0 || v < bestv || (v == bestv && s == N / 2)) {
+      bestv = v;
+      best = s;
+    }
+  }
+  return best;
+}
+
+}  // namespace planc
+
+using namespace planc;
+
+namespace {
+
+int check_ctx(planc_ctx *ctx) { return ctx ? PLANC_OK : PLANC_EINVAL; }
+
+bool valid_dtype(int dt) { return dt == PLANC_F32 || dt == PLANC_BF16 || dt == PLANC_F16; }
+
+// MTTKRP of mode n (k x I_n, ld ldo) with the KRP fused.  P is an L' x k
+// scratch for the middle modes.  Direct form: mode 0 and N-1 are one
+// product each; a middle mode n is the right contraction over modes > n
+// followed by the contraction of modes < n.
+int mttkrp_direct(planc_ctx *ctx, int dtype, const void *X, int N, const long *dims,
+                  const float *const *F, int k, int n, float *out, long ldo, float *P) {
+  long numel = 1;
+  for (int i = 0; i < N; ++i) numel *= dims[i];
+  if (N == 1) return set_error(ctx, PLANC_EINVAL, "mttkrp: need >= 2 modes");
+  if (n == 0) {
+    KrpDesc kd{};
+    kd.nf = N - 1;
+    for (int q = 1; q < N; ++q) {
+      kd.f[q - 1] = F[q];
+      kd.dim[q - 1] = dims[q];
+      kd.ld[q - 1] = dims[q];
+    }
+    return thin_gemm(ctx, dtype, 0, X, dims[0], dims[0], numel / dims[0], nullptr, 0, &kd, nullptr,
+                     nullptr, k, out, ldo, 1, 0);
+  }
+  if (n == N - 1) {
+    KrpDesc kd{};
+    kd.nf = N - 1;
+    for (int q = 0; q < N - 1; ++q) {
+      kd.f[q] = F[q];
+      kd.dim[q] = dims[q];
+      kd.ld[q] = dims[q];
+    }
+    long rest = numel / dims[N - 1];
+    return thin_gemm(ctx, dtype, 1, X, rest, dims[N - 1], rest, nullptr, 0, &kd, nullptr, nullptr, k,
+                     out, ldo, 1, 0);
+  }
+  long Lp = 1;
+  for (int i = 0; i <= n; ++i) Lp *= dims[i];
+  KrpDesc kd{};
+  kd.nf = N - 1 - n;
+  for (int q = n + 1; q < N; ++q) {
+    kd.f[q - n - 1] = F[q];
+    kd.dim[q - n - 1] = dims[q];
+    kd.ld[q - n - 1] = dims[q];
+  }
+  int rc = thin_gemm(ctx, dtype, 0, X, Lp, Lp, numel / Lp, nullptr, 0, &kd, nullptr, nullptr, k, P, Lp,
+                     0, 0);
+  if (rc) return rc;
+  long ldf[8];
+  for (int i = 0; i <= n; ++i) ldf[i] = dims[i];
+  return contract_projection(ctx, P, Lp, n + 1, dims, F, ldf, n, k, out, ldo);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *planc_version(void) { return "planc_amd 0.1 (gfx950)"; }
+
+int planc_create(int device, planc_ctx **out) {
+  if (!out) return PLANC_EINVAL;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return PLANC_EHIP;
+  if (hipSetDevice(device) != hipSuccess) return PLANC_EHIP;
+  planc_ctx *c = new planc_ctx();
+  c->device = device;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return PLANC_EHIP;
+  }
+  c->own_stream = true;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+    c->num_cu = prop.multiProcessorCount;
+  *out = c;
+  return PLANC_OK;
+}
+
+int planc_destroy(planc_ctx *ctx) {
+  if (!ctx) return PLANC_EINVAL;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto &w : ctx->ws)
+    if (w.ptr) (void)hipFree(w.ptr);
+  if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return PLANC_OK;
+}
+
+int planc_sync(planc_ctx *ctx) {
+  if (check_ctx(ctx)) return PLANC_EINVAL;
+  PLANC_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return PLANC_OK;
+}
+
+const char *planc_last_error(planc_ctx *ctx) { return ctx ? ctx->last_error.c_str() : "null context"; }
+
+int planc_set_stream(planc_ctx *ctx, void *stream) {
+  if (check_ctx(ctx)) return PLANC_EINVAL;
+  PLANC_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
+  ctx->stream = (hipStream_t)stream;
+  ctx->own_stream = false;
+  return PLANC_OK;
+}
+
+void *planc_get_stream(planc_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+int planc_malloc(planc_ctx *ctx, void **d_ptr, size_t bytes) {
+  if (check_ctx(ctx) || !d_ptr) return PLANC_EINVAL;
+  if (hipMalloc(d_ptr, bytes ? bytes : 1) != hipSuccess) return set_error(ctx, PLANC_ENOMEM, "planc_malloc");
+  return PLANC_OK;
+}
+
+int planc_free(planc_ctx *ctx, void *d_ptr) {
+  if (check_ctx(ctx)) return PLANC_EINVAL;
+  PLANC_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  PLANC_HIP_CHECK(ctx, hipFree(d_ptr));
+  return PLANC_OK;
+}
+
+int planc_memcpy_h2d(planc_ctx *ctx, void *d_dst, const void *h_src, size_t bytes) {
+  if (check_ctx(ctx)) return PLANC_EINVAL;
+  PLANC_HIP_CHECK(ctx, hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  PLANC_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return PLANC_OK;
+}
+
+int planc_memcpy_d2h(planc_ctx *ctx, void *h_dst, const void *d_src, size_t bytes) {
+  if (check_ctx(ctx)) return PLANC_EINVAL;
+  PLANC_HIP_CHECK(ctx, hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  PLANC_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return PLANC_OK;
+}
+
+int planc_convert(planc_ctx *ctx, int dtype, const float *d_src, long rows, long cols, long ld_src,
+                  void *d_dst, long ld_dst, float *d_colscale) {
+  if (check_ctx(ctx)) return PLANC_EINVAL;
+  if ((dtype != PLANC_BF16 && dtype != PLANC_F16) || rows < 0 || cols < 0 || ld_src < rows ||
+      ld_dst < rows || (rows * cols > 0 && (!d_src || !d_dst)))
+    return set_error(ctx, PLANC_EINVAL, "planc_convert: bad arguments");
+  return convert16(ctx, dtype, d_src, rows, cols, ld_src, d_dst, ld_dst, d_colscale);
+}
+
+int planc_gemm_nn(planc_ctx *ctx, int dtype, const void *d_A, long lda, long L, long K,
+                  const float *d_colscale, const float *d_B, long ldb, int k, float *d_C, long ldc) {
+  if (check_ctx(ctx)) return PLANC_EINVAL;
+  if (!valid_dtype(dtype) || lda < L || ldb < K || ldc < L || k <= 0)
+    return set_error(ctx, PLANC_EINVAL, "planc_gemm_nn: bad arguments");
+  return thin_gemm(ctx, dtype, 0, d_A, lda, L, K, d_B, ldb, nullptr, d_colscale, nullptr, k, d_C, ldc, 0,
+                   0);
+}
+
+int planc_gemm_tn(planc_ctx *ctx, int dtype, const void *d_A, long lda, long K, long L,
+                  const float *d_colscale, const float *d_B, long ldb, int k, float *d_C, long ldc) {
+  if (check_ctx(ctx)) return PLANC_EINVAL;
+  if (!valid_dtype(dtype) || lda < K || ldb < K || ldc < k || k <= 0)
+    return set_error(ctx, PLANC_EINVAL, "planc_gemm_tn: bad arguments");
+  return thin_gemm(ctx, dtype, 1, d_A, lda, L, K, d_B, ldb, nullptr, nullptr, d_colscale, k, d_C, ldc, 1,
+                   0);
+}
+
+int planc_gram(planc_ctx *ctx, const float *d_X, long rows, int k, long ldx, double *d_G) {
+  if (check_ctx(ctx)) return PLANC_EINVAL;
+  if (ldx < rows || !d_G) return set_error(ctx, PLANC_EINVAL, "planc_gram: bad arguments");
+  return gram(ctx, d_X, rows, k, ldx, d_G);
+}
+
+int planc_mu_update(planc_ctx *ctx, float *d_H, long n, int k, long ldh, const float *d_N, long n_rs,
+                    long n_cs, const double *d_G, double eps) {
+  if (check_ctx(ctx)) return PLANC_EINVAL;
+  if (ldh < n || k <= 0) return set_error(ctx, PLANC_EINVAL, "planc_mu_update: bad arguments");
+  return row_update(ctx, 0, d_H, n, k, ldh, d_N, n_rs, n_cs, d_G, (float)eps);
+}
+
+int planc_hals_update_rows(planc_ctx *ctx, float *d_H, long n, int k, long ldh, const float *d_N,
+                           long n_rs, long n_cs, const double *d_G) {
+  if (check_ctx(ctx)) return PLANC_EINVAL;
+  if (ldh < n || k <= 0) return set_error(ctx, PLANC_EINVAL, "planc_hals_update_rows: bad arguments");
+  return row_update(ctx, 1, d_H, n, k, ldh, d_N, n_rs, n_cs, d_G, 0.0f);
+}
+
+int planc_hals_update_cols(planc_ctx *ctx, float *d_W, long m, int k, long ldw, const float *d_AH,
+                           long ldah, const double *d_G) {
+  if (check_ctx(ctx)) return PLANC_EINVAL;
+  if (ldw < m || ldah < m || k <= 0) return set_error(ctx, PLANC_EINVAL, "planc_hals_update_cols: bad arguments");
+  return hals_cols(ctx, d_W, m, k, ldw, d_AH, ldah, d_G);
+}
+
+int planc_normalize_by_w(planc_ctx *ctx, float *d_W, long m, long ldw, float *d_H, long n, long ldh,
+                         int k) {
+  if (check_ctx(ctx)) return PLANC_EINVAL;
+  if (ldw < m || ldh < n || k <= 0) return set_error(ctx, PLANC_EINVAL, "planc_normalize_by_w: bad arguments");
+  double *norms = (double *)ws_get(ctx, 9, sizeof(double) * k);
+  if (!norms) return set_error(ctx, PLANC_ENOMEM, "normalize_by_w: workspace");
+  int rc = col_norms(ctx, d_W, m, k, ldw, norms);
+  if (rc) return rc;
+  return scale_cols(ctx, d_W, m, ldw, d_H, n, ldh, k, norms);
+}
+
+int planc_normalize_cols(planc_ctx *ctx, float *d_A, long rows, int k, long lda, double *d_lambda) {
+  if (check_ctx(ctx)) return PLANC_EINVAL;
+  if (lda < rows || k <= 0 || !d_lambda) return set_error(ctx, PLANC_EINVAL, "planc_normalize_cols: bad arguments");
+  int rc = col_norms(ctx, d_A, rows, k, lda, d_lambda);
+  if (rc) return rc;
+  return scale_cols(ctx, d_A, rows, lda, nullptr, 0, 0, k, d_lambda);
+}
+
+int planc_hadamard_grams(planc_ctx *ctx, const double *const *d_grams, int nmodes, int skip, int k,
+                         double *d_out) {
+  if (check_ctx(ctx)) return PLANC_EINVAL;
+  if (!d_grams || k <= 0) return set_error(ctx, PLANC_EINVAL, "planc_hadamard_grams: bad arguments");
+  return hadamard_grams(ctx, d_grams, nmodes, skip, k, d_out);
+}
+
+int planc_mttkrp(planc_ctx *ctx, int dtype, const void *d_X, int nmodes, const long *dims,
+                 const float *const *d_factors, int k, int mode, float *d_out, long ldo) {
+  if (check_ctx(ctx)) return PLANC_EINVAL;
+  if (!valid_dtype(dtype) || nmodes < 2 || nmodes > 7 || !dims || !d_factors || mode < 0 ||
+      mode >= nmodes || k <= 0 || ldo < k)
+    return set_error(ctx, PLANC_EINVAL, "planc_mttkrp: bad arguments");
+  for (int i = 0; i < nmodes; ++i)
+    if (dims[i] <= 0) return set_error(ctx, PLANC_EINVAL, "planc_mttkrp: empty mode");
+  float *P = nullptr;
+  if (mode > 0 && mode < nmodes - 1) {
+    long Lp = 1;
+    for (int i = 0; i <= mode; ++i) Lp *= dims[i];
+    P = (float *)ws_get(ctx, 4, sizeof(float) * (size_t)Lp * k);
+    if (!P) return set_error(ctx, PLANC_ENOMEM, "planc_mttkrp: workspace");
+  }
+  return mttkrp_direct(ctx, dtype, d_X, nmodes, dims, d_factors, k, mode, d_out, ldo, P);
+}
+
+// ---------------------------------------------------------------------------
+int planc_nmf_run(planc_ctx *ctx, const planc_nmf_desc *desc, const float *d_A, long lda,
+                  const void *d_A16, const float *d_colscale, float *d_W, float *d_H,
+                  double *h_relerr) {
+  if (check_ctx(ctx) || !desc) return PLANC_EINVAL;
+  const long m = desc->m, n = desc->n;
+  const int k = desc->k, iters = desc->iters, algo = desc->algo, dt = desc->dtype;
+  if (m <= 0 || n <= 0 || k <= 0 || k > 64 || iters < 0 || lda < m || !d_A || !d_W || !d_H ||
+      !valid_dtype(dt))
+    return set_error(ctx, PLANC_EINVAL, "planc_nmf_run: bad arguments (k must be 1..64)");
+  if (algo != PLANC_MU && algo != PLANC_HALS)
+    return set_error(ctx, PLANC_EUNSUPPORTED, "planc_nmf_run: algorithm not supported");
+  PLANC_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  const void *Aop = d_A;
+  const float *cs = nullptr;
+  long ldop = lda;
+  if (dt != PLANC_F32) {
+    if (d_A16) {
+      Aop = d_A16;
+      cs = d_colscale;
+    } else {
+      void *a16 = ws_get(ctx, 7, sizeof(uint16_t) * (size_t)m * n);
+      float *sc = (float *)ws_get(ctx, 8, sizeof(float) * (size_t)n);
+      if (!a16 || !sc) return set_error(ctx, PLANC_ENOMEM, "planc_nmf_run: 16-bit copy");
+      int rc = convert16(ctx, dt, d_A, m, n, lda, a16, m, sc);
+      if (rc) return rc;
+      Aop = a16;
+      cs = sc;
+      ldop = m;
+    }
+  }
+  // Loop buffers.
+  float *fbuf = (float *)ws_get(ctx, 4, sizeof(float) * ((size_t)k * n + (size_t)m * k));
+  const size_t kk = (size_t)k * k;
+  double *dbuf = (double *)ws_get(ctx, 5, sizeof(double) * (4 * kk + 8 + (size_t)iters));
+  if (!fbuf || !dbuf) return set_error(ctx, PLANC_ENOMEM, "planc_nmf_run: workspace");
+  float *WtA = fbuf;                // k x n
+  float *AH = fbuf + (size_t)k * n;  // m x k
+  double *WtW = dbuf, *HtH = dbuf + kk, *Greg = dbuf + 2 * kk, *WtW2 = dbuf + 3 * kk;
+  double *normA2 = dbuf + 4 * kk, *dotv = normA2 + 1, *nrm = normA2 + 2;  // nrm unused slot
+  double *rel = normA2 + 8;
+  (void)nrm;
+  int rc;
+#define RC(x)          \
+  do {                 \
+    rc = (x);          \
+    if (rc) return rc; \
+  } while (0)
+  if (desc->compute_error) RC(dot(ctx, d_A, lda, d_A, lda, m, n, normA2, 0));
+  if (algo == PLANC_HALS) RC(planc_normalize_by_w(ctx, d_W, m, m, d_H, n, n, k));
+  for (int it = 0; it < iters; ++it) {
+    // H half-step: W^T A, W^T W (+ regH).
+    RC(thin_gemm(ctx, dt, 1, Aop, ldop, n, m, d_W, m, nullptr, nullptr, cs, k, WtA, k, 1, 0));
+    RC(gram(ctx, d_W, m, k, m, WtW));
+    PLANC_HIP_CHECK(ctx, hipMemcpyAsync(Greg, WtW, kk * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    RC(apply_reg(ctx, Greg, k, desc->regH));
+    RC(row_update(ctx, algo == PLANC_MU ? 0 : 1, d_H, n, k, n, WtA, k, 1, Greg, 1e-17f));
+    // W half-step: A H, H^T H (+ regW).
+    RC(thin_gemm(ctx, dt, 0, Aop, ldop, m, n, d_H, n, nullptr, cs, nullptr, k, AH, m, 0, 0));
+    RC(gram(ctx, d_H, n, k, n, HtH));
+    PLANC_HIP_CHECK(ctx, hipMemcpyAsync(Greg, HtH, kk * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    RC(apply_reg(ctx, Greg, k, desc->regW));
+    if (algo == PLANC_MU)
+      RC(row_update(ctx, 0, d_W, m, k, m, AH, 1, m, Greg, 1e-17f));
+    else
+      RC(hals_cols(ctx, d_W, m, k, m, AH, m, Greg));
+    if (desc->compute_error) {
+      // <W, A H> with the H of this iteration equals tr(H^T A^T W) of
+      // computeObjectiveError (common/nmf.hpp:391-396) without the third
+      // product; both traces are invariant under normalize_by_W.
+      RC(dot(ctx, d_W, m, AH, m, m, k, dotv, 0));
+      RC(gram(ctx, d_W, m, k, m, WtW2));
+      RC(nmf_relerr(ctx, normA2, dotv, WtW2, HtH, k, rel + it, nullptr));
+    }
+    if (algo == PLANC_HALS) RC(planc_normalize_by_w(ctx, d_W, m, m, d_H, n, n, k));
+  }
+  if (algo == PLANC_MU) RC(planc_normalize_by_w(ctx, d_W, m, m, d_H, n, n, k));
+#undef RC
+  if (h_relerr && desc->compute_error && iters > 0)
+    PLANC_HIP_CHECK(ctx, hipMemcpyAsync(h_relerr, rel, sizeof(double) * iters, hipMemcpyDeviceToHost, ctx->stream));
+  PLANC_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return PLANC_OK;
+}
+
+int planc_ntf_run(planc_ctx *ctx, const planc_ntf_desc *desc, const float *d_X, const void *d_X16,
+                  float *const *d_factors, double *d_lambda, double *h_relerr) {
+  if (check_ctx(ctx) || !desc) return PLANC_EINVAL;
+  const int N = desc->nmodes, k = desc->k, iters = desc->iters, algo = desc->algo, dt = desc->dtype;
+  if (N < 2 || N > 7 || k <= 0 || k > 64 || iters < 0 || !d_X || !d_factors || !d_lambda ||
+      !valid_dtype(dt))
+    return set_error(ctx, PLANC_EINVAL, "planc_ntf_run: bad arguments (2..7 modes, k 1..64)");
+  if (algo != PLANC_MU && algo != PLANC_HALS)
+    return set_error(ctx, PLANC_EUNSUPPORTED, "planc_ntf_run: algorithm not supported");
+  PLANC_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  const long *dims = desc->dims;
+  long numel = 1, Imax = 0;
+  for (int i = 0; i < N; ++i) {
+    if (dims[i] <= 0) return set_error(ctx, PLANC_EINVAL, "planc_ntf_run: empty mode");
+    numel *= dims[i];
+    Imax = std::max(Imax, dims[i]);
+  }
+  const bool dimtree = desc->dimtree && N >= 3;
+  const int s = dimtree ? balanced_split(N, dims) : 0;
+  long L = 1, R = 1;
+  for (int i = 0; i <= s; ++i) L *= dims[i];
+  for (int i = s + 1; i < N; ++i) R *= dims[i];
+  const void *Xop = d_X;
+  if (dt != PLANC_F32) {
+    if (d_X16) {
+      Xop = d_X16;
+    } else {
+      void *x16 = ws_get(ctx, 7, sizeof(uint16_t) * (size_t)numel);
+      if (!x16) return set_error(ctx, PLANC_ENOMEM, "planc_ntf_run: 16-bit copy");
+      int rc = convert16(ctx, dt, d_X, numel, 1, numel, x16, numel, nullptr);
+      if (rc) return rc;
+      Xop = x16;
+    }
+  }
+  // Float scratch: P (L x k) and Q (R x k) for the tree (or the direct
+  // middle-mode projection), M (k x Imax).
+  long Pmax = 0;
+  if (dimtree) {
+    Pmax = L * k + R * k;
+  } else {
+    long Lp = 1;
+    for (int i = 0; i < N - 1; ++i) {
+      Lp *= dims[i];
+      if (i > 0) Pmax = std::max(Pmax, Lp * k);
+    }
+  }
+  float *fbuf = (float *)ws_get(ctx, 4, sizeof(float) * ((size_t)Pmax + (size_t)k * Imax));
+  const size_t kk = (size_t)k * k;
+  double *dbuf = (double *)ws_get(ctx, 5, sizeof(double) * ((N + 2) * kk + 8 + k + (size_t)iters));
+  if (!fbuf || !dbuf) return set_error(ctx, PLANC_ENOMEM, "planc_ntf_run: workspace");
+  float *P = fbuf, *Q = dimtree ? fbuf + (size_t)L * k : nullptr;
+  float *M = fbuf + Pmax;
+  std::vector<double *> G(N);
+  for (int i = 0; i < N; ++i) G[i] = dbuf + i * kk;
+  double *S = dbuf + N * kk;
+  double *normX2 = dbuf + (N + 1) * kk;
+  double *norms = normX2 + 8;
+  double *rel = norms + k;
+  int rc;
+#define RC(x)          \
+  do {                 \
+    rc = (x);          \
+    if (rc) return rc; \
+  } while (0)
+  if (desc->compute_error) RC(dot(ctx, d_X, numel, d_X, numel, numel, 1, normX2, 0));
+  // NCPFactors::normalize() of the initial factors (ntf/auntf.hpp:88).
+  for (int i = 0; i < N; ++i) {
+    RC(col_norms(ctx, d_factors[i], dims[i], k, dims[i], norms));
+    RC(scale_cols(ctx, d_factors[i], dims[i], dims[i], nullptr, 0, 0, k, norms));
+    RC(lambda_mul(ctx, d_lambda, norms, k, i == 0));
+    RC(gram(ctx, d_factors[i], dims[i], k, dims[i], G[i]));
+  }
+  std::vector<const double *> Gc(G.begin(), G.end());
+  long ldfL[8], ldfR[8];
+  for (int i = 0; i < N; ++i) {
+    ldfL[i] = dims[i];
+    ldfR[i] = dims[i];
+  }
+  for (int it = 0; it < iters; ++it) {
+    for (int n = 0; n < N; ++n) {
+      RC(hadamard_grams(ctx, Gc.data(), N, n, k, S));
+      if (dimtree) {
+        if (n == 0) {
+          KrpDesc kd{};
+          kd.nf = N - 1 - s;
+          for (int q = s + 1; q < N; ++q) {
+            kd.f[q - s - 1] = d_factors[q];
+            kd.dim[q - s - 1] = dims[q];
+            kd.ld[q - s - 1] = dims[q];
+          }
+          RC(thin_gemm(ctx, dt, 0, Xop, L, L, R, nullptr, 0, &kd, nullptr, nullptr, k, P, L, 0, 0));
+        }
+        if (n == s + 1) {
+          KrpDesc kd{};
+          kd.nf = s + 1;
+          for (int q = 0; q <= s; ++q) {
+            kd.f[q] = d_factors[q];
+            kd.dim[q] = dims[q];
+            kd.ld[q] = dims[q];
+          }
+          RC(thin_gemm(ctx, dt, 1, Xop, L, R, L, nullptr, 0, &kd, nullptr, nullptr, k, Q, R, 0, 0));
+        }
+        if (n <= s)
+          RC(contract_projection(ctx, P, L, s + 1, dims, d_factors, ldfL, n, k, M, k));
+        else
+          RC(contract_projection(ctx, Q, R, N - 1 - s, dims + s + 1, d_factors + s + 1, ldfR + s + 1,
+                                 n - s - 1, k, M, k));
+      } else {
+        RC(mttkrp_direct(ctx, dt, Xop, N, dims, d_factors, k, n, M, k, P));
+      }
+      float *Fn = d_factors[n];
+      if (algo == PLANC_HALS)
+        RC(row_update(ctx, 1, Fn, dims[n], k, dims[n], M, k, 1, S, 0.0f));
+      else
+        RC(row_update(ctx, 0, Fn, dims[n], k, dims[n], M, k, 1, S, 1e-6f));
+      // update_factor_mode: normalize(mode) replaces lambda (ntf/auntf.hpp:66-79).
+      RC(col_norms(ctx, Fn, dims[n], k, dims[n], d_lambda));
+      RC(scale_cols(ctx, Fn, dims[n], dims[n], nullptr, 0, 0, k, d_lambda));
+      RC(gram(ctx, Fn, dims[n], k, dims[n], G[n]));
+    }
+    if (desc->compute_error)
+      RC(ntf_relerr(ctx, normX2, d_factors[N - 1], dims[N - 1], dims[N - 1], M, k, k, d_lambda,
+                    Gc.data(), N, rel + it));
+  }
+#undef RC
+  if (h_relerr && desc->compute_error && iters > 0)
+    PLANC_HIP_CHECK(ctx, hipMemcpyAsync(h_relerr, rel, sizeof(double) * iters, hipMemcpyDeviceToHost, ctx->stream));
+  PLANC_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return PLANC_OK;
+}
+
+}  // extern "C"

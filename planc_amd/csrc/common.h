@@ -1,0 +1,163 @@
+// Shared definitions of the planc_amd HIP library (gfx950 / MI355X only).
+#ifndef PLANC_AMD_COMMON_H_
+#define PLANC_AMD_COMMON_H_
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/planc_hip.h"
+
+namespace planc {
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 halfx8 __attribute__((ext_vector_type(8)));
+
+// 16-bit storage types of the mixed-precision copies of A / X.
+struct bf16_t {
+  uint16_t v;
+};
+struct f16_t {
+  uint16_t v;
+};
+
+__host__ __device__ inline float bf16_to_float(uint16_t h) {
+  union {
+    uint32_t u;
+    float f;
+  } c;
+  c.u = (uint32_t)h << 16;
+  return c.f;
+}
+
+// Round to nearest even; NaN stays NaN.
+__host__ __device__ inline uint16_t float_to_bf16(float f) {
+  union {
+    uint32_t u;
+    float f;
+  } c;
+  c.f = f;
+  if ((c.u & 0x7f800000u) == 0x7f800000u && (c.u & 0x007fffffu)) return (uint16_t)((c.u >> 16) | 0x40);
+  uint32_t lsb = (c.u >> 16) & 1u;
+  c.u += 0x7fffu + lsb;
+  return (uint16_t)(c.u >> 16);
+}
+
+__device__ inline float f16_to_float(uint16_t h) {
+  _Float16 x;
+  __builtin_memcpy(&x, &h, 2);
+  return (float)x;
+}
+__device__ inline uint16_t float_to_f16(float f) {
+  _Float16 x = (_Float16)f;
+  uint16_t h;
+  __builtin_memcpy(&h, &x, 2);
+  return h;
+}
+
+// Value of element e of a storage array of the given dtype as float.
+template <typename T>
+__device__ inline float load_as_float(const T *p, long e);
+template <>
+__device__ inline float load_as_float<float>(const float *p, long e) {
+  return p[e];
+}
+template <>
+__device__ inline float load_as_float<bf16_t>(const bf16_t *p, long e) {
+  return bf16_to_float(p[e].v);
+}
+template <>
+__device__ inline float load_as_float<f16_t>(const f16_t *p, long e) {
+  return f16_to_float(p[e].v);
+}
+
+// Workspace cache owned by a context: grows, never shrinks until destroy.
+struct Workspace {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace planc
+
+struct planc_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  std::string last_error;
+  planc::Workspace ws[12];
+  int num_cu = 256;
+  int max_coop_blocks_hals = 0;
+};
+
+namespace planc {
+
+// Returns a device buffer of at least `bytes` for workspace slot `slot`.
+// Slots: 0 split-K partials, 1 Gram partials, 2 column-norm / HALS-sweep
+// partials, 3 dot partials, 4 loop float buffers, 5 loop double buffers,
+// 7 16-bit copy of A / X, 8 column scales of that copy, 9 entry-point norms.
+void *ws_get(planc_ctx *ctx, int slot, size_t bytes);
+
+int set_error(planc_ctx *ctx, int code, const std::string &msg);
+
+#define PLANC_HIP_CHECK(ctx, call)                                                  \
+  do {                                                                              \
+    hipError_t _e = (call);                                                         \
+    if (_e != hipSuccess)                                                           \
+      return ::planc::set_error((ctx), PLANC_EHIP,                                  \
+                                std::string(#call) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+#define PLANC_LAUNCH_CHECK(ctx) PLANC_HIP_CHECK(ctx, hipGetLastError())
+
+// Kernel-family entry points shared between translation units.
+struct KrpDesc {
+  int nf = 0;               // number of factors in the Khatri-Rao product (0: plain B)
+  const float *f[6];        // factor matrices, column-major I_q x k
+  long dim[6];              // I_q
+  long ld[6];               // leading dimension of each factor
+};
+
+// C = A * B (trans == 0; A is L x K col-major, C is L x k col-major) or
+// C = B^T * A (trans == 1; A is K x L col-major, C is k x L col-major), with
+// B the K x k operand: a column-major matrix (krp.nf == 0) or the Khatri-Rao
+// product of krp.f (lowest factor fastest).  rowscale (K, optional) scales
+// the rows of B; colscale (L, optional, trans == 1) scales output columns.
+// cmajor selects the output layout: 1 -> C[c + ldc*l] (k x L), 0 -> C[l + ldc*c].
+// splits_hint > 0 forces the split-K factor (0: automatic).
+int thin_gemm(planc_ctx *ctx, int dtype, int trans, const void *A, long lda, long L, long K,
+              const float *B, long ldb, const KrpDesc *krp, const float *rowscale,
+              const float *colscale, int k, float *C, long ldc, int cmajor, int splits_hint);
+
+// nmf_kernels.hip
+int gram(planc_ctx *ctx, const float *X, long rows, int k, long ldx, double *G);
+// kind 0: MU (eps), kind 1: HALS rows.  N(j, c) = N[j*nrs + c*ncs].
+int row_update(planc_ctx *ctx, int kind, float *H, long n, int k, long ldh, const float *N, long nrs,
+               long ncs, const double *G, float eps);
+int hals_cols(planc_ctx *ctx, float *W, long m, int k, long ldw, const float *AH, long ldah,
+              const double *G);
+int col_norms(planc_ctx *ctx, const float *X, long rows, int k, long ldx, double *norms);
+int scale_cols(planc_ctx *ctx, float *X, long rows, long ldx, float *Y, long yrows, long ldy, int k,
+               const double *norms);
+int dot(planc_ctx *ctx, const float *X, long ldx, const float *Y, long ldy, long rows, long cols,
+        double *out, int accumulate);
+int nmf_relerr(planc_ctx *ctx, const double *normA2, const double *dotv, const double *WtW,
+               const double *HtH, int k, double *relerr_out, double *fit_out);
+int apply_reg(planc_ctx *ctx, double *G, int k, const double *reg);
+int convert16(planc_ctx *ctx, int dtype, const float *A, long rows, long cols, long lda, void *out,
+              long ldo, float *colscale);
+
+// ntf_kernels.hip
+int contract_projection(planc_ctx *ctx, const float *P, long ldp, int nb, const long *d,
+                        const float *const *F, const long *ldf, int tn, int k, float *M, long ldo);
+int hadamard_grams(planc_ctx *ctx, const double *const *grams, int n, int skip, int k, double *out);
+int ntf_relerr(planc_ctx *ctx, const double *normX2, const float *F, long I, long ldf, const float *M,
+               long ldm, int k, const double *lambda, const double *const *grams, int n,
+               double *rel_out);
+int lambda_mul(planc_ctx *ctx, double *lambda, const double *norms, int k, int init);
+int transpose_to_k_major(planc_ctx *ctx, const float *in, long I, long ldi, int k, float *out, long ldo);
+
+}  // namespace planc
+
+#endif  // PLANC_AMD_COMMON_H_

@@ -1,0 +1,585 @@
+// Small-k kernels of the NMF/NTF loops: Grams (FP64 accumulation), the fused
+// MU and HALS updates, column normalisation, FP64 reductions for the error
+// terms and the 16-bit conversion of A.
+//
+// Row-parallel updates keep one row of the factor (k <= 64 values) in
+// registers and the k x k Gram in LDS, so an update costs one read and one
+// write of the factor plus one read of the numerator.
+#include <hip/hip_cooperative_groups.h>
+
+#include "common.h"
+
+namespace planc {
+
+namespace {
+
+constexpr int NTHR = 256;
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Deterministic block sum (fixed shuffle pattern, waves summed in order).
+__device__ inline double block_sum(double v, double *red) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  v = wave_sum(v);
+  __syncthreads();
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  double s = 0.0;
+  const int nw = blockDim.x >> 6;
+  for (int w = 0; w < nw; ++w) s += red[w];
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+// G = X^T X in FP64.  Each block takes a 128-row chunk into LDS; thread
+// (a, b) accumulates a 4x4 tile of G.  Partials per chunk are summed in
+// chunk order by gram_reduce_kernel.
+constexpr int GRAM_RC = 128;
+
+__global__ __launch_bounds__(NTHR) void gram_partial_kernel(const float *X, long rows, int k,
+                                                            long ldx, double *part, int kp) {
+  __shared__ float Xs[GRAM_RC][64 + 4];
+  const int tid = threadIdx.x;
+  const int ta = tid >> 4, tb = tid & 15;
+  for (long chunk = blockIdx.x; chunk * GRAM_RC < rows; chunk += gridDim.x) {
+    long r0 = chunk * GRAM_RC;
+    __syncthreads();
+    for (int e = tid; e < GRAM_RC * kp; e += NTHR) {
+      int r = e % GRAM_RC, c = e / GRAM_RC;
+      long gr = r0 + r;
+      Xs[r][c] = (gr < rows && c < k) ? X[gr + ldx * c] : 0.0f;
+    }
+    __syncthreads();
+    double acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
+    if (4 * ta < kp && 4 * tb < kp) {
+      for (int r = 0; r < GRAM_RC; ++r) {
+        double a[4], b[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          a[i] = Xs[r][4 * ta + i];
+          b[i] = Xs[r][4 * tb + i];
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) acc[i][j] += a[i] * b[j];
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          int ga = 4 * ta + i, gb = 4 * tb + j;
+          part[(size_t)chunk * kp * kp + ga + (size_t)kp * gb] = acc[i][j];
+        }
+    }
+  }
+}
+
+__global__ void gram_reduce_kernel(const double *part, long nchunks, int k, int kp, double *G) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= k * k) return;
+  int a = e % k, b = e / k;
+  double s = 0.0;
+  for (long c = 0; c < nchunks; ++c) s += part[(size_t)c * kp * kp + a + (size_t)kp * b];
+  G[a + (size_t)k * b] = s;
+}
+
+// ---------------------------------------------------------------------------
+// Loads the k x k FP64 matrix G into an LDS float tile (zero padded).
+template <int KM>
+__device__ inline void load_gram(float (&Gs)[KM][KM + 1], const double *G, int k) {
+  for (int e = threadIdx.x; e < KM * KM; e += blockDim.x) {
+    int r = e % KM, c = e / KM;
+    Gs[r][c] = (r < k && c < k) ? (float)G[r + (size_t)k * c] : 0.0f;
+  }
+  __syncthreads();
+}
+
+// MU: H <- H .* N ./ (H*G + eps); N(j, c) at N[j*nrs + c*ncs].
+template <int KM>
+__global__ __launch_bounds__(NTHR) void mu_update_kernel(float *H, long n, int k, long ldh,
+                                                         const float *N, long nrs, long ncs,
+                                                         const double *G, float eps) {
+  __shared__ float Gs[KM][KM + 1];
+  load_gram<KM>(Gs, G, k);
+  long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  float h[KM];
+#pragma unroll
+  for (int t = 0; t < KM; ++t) h[t] = (t < k) ? H[j + ldh * t] : 0.0f;
+#pragma unroll
+  for (int c = 0; c < KM; ++c) {
+    if (c < k) {
+      float s = 0.0f;
+#pragma unroll
+      for (int t = 0; t < KM; ++t) s = fmaf(h[t], Gs[t][c], s);
+      H[j + ldh * c] = (h[c] * N[j * nrs + c * ncs]) / (s + eps);
+    }
+  }
+}
+
+// HALS, independent rows (nmf/hals.hpp:67-75, ntf/ntfhals.hpp:13-29):
+// sequential over columns x, each using the already-updated columns < x.
+template <int KM>
+__global__ __launch_bounds__(NTHR) void hals_rows_kernel(float *H, long n, int k, long ldh,
+                                                         const float *N, long nrs, long ncs,
+                                                         const double *G) {
+  __shared__ float Gs[KM][KM + 1];
+  load_gram<KM>(Gs, G, k);
+  long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const float floor_v = 1e-17f;
+  float h[KM];
+#pragma unroll
+  for (int t = 0; t < KM; ++t) h[t] = (t < k) ? H[j + ldh * t] : 0.0f;
+#pragma unroll
+  for (int x = 0; x < KM; ++x) {
+    if (x < k) {
+      float s = 0.0f;
+#pragma unroll
+      for (int t = 0; t < KM; ++t) s = fmaf(h[t], Gs[t][x], s);
+      float v = h[x] + (N[j * nrs + x * ncs] - s);
+      h[x] = (v < floor_v) ? floor_v : v;
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < KM; ++t)
+    if (t < k) H[j + ldh * t] = h[t];
+}
+
+// HALS over the columns of W with per-column normalisation (nmf/hals.hpp:
+// 89-102).  Persistent cooperative kernel: each thread owns RPT rows of W in
+// registers; one grid-wide barrier per column x publishes the column's sum of
+// squares (block partials summed in block order by every block, so every
+// block derives the identical norm).
+template <int KM, int RPT>
+__global__ __launch_bounds__(NTHR) void hals_cols_coop_kernel(float *W, long m, int k, long ldw,
+                                                              const float *AH, long ldah,
+                                                              const double *G, double *part) {
+  __shared__ float Gs[KM][KM + 1];
+  __shared__ double red[NTHR / 64];
+  load_gram<KM>(Gs, G, k);
+  cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+  const long total = (long)gridDim.x * blockDim.x;
+  const long gtid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const float floor_v = 1e-17f;
+  float w[RPT][KM];
+#pragma unroll
+  for (int r = 0; r < RPT; ++r) {
+    long i = gtid + r * total;
+#pragma unroll
+    for (int t = 0; t < KM; ++t) w[r][t] = (i < m && t < k) ? W[i + ldw * t] : 0.0f;
+  }
+#pragma unroll
+  for (int x = 0; x < KM; ++x) {
+    if (x < k) {
+      float u[RPT];
+      double lsum = 0.0;
+      const float gxx = Gs[x][x];
+#pragma unroll
+      for (int r = 0; r < RPT; ++r) {
+        long i = gtid + r * total;
+        u[r] = 0.0f;
+        if (i < m) {
+          float s = 0.0f;
+#pragma unroll
+          for (int t = 0; t < KM; ++t) s = fmaf(w[r][t], Gs[t][x], s);
+          float v = (w[r][x] * gxx) + (AH[i + ldah * x] - s);
+          v = (v < floor_v) ? floor_v : v;
+          u[r] = v;
+          lsum += (double)v * (double)v;
+        }
+      }
+      double bs = block_sum(lsum, red);
+      if (threadIdx.x == 0) part[(size_t)x * gridDim.x + blockIdx.x] = bs;
+      grid.sync();
+      double tot = 0.0;
+      if (threadIdx.x < 64) {
+        for (long b = threadIdx.x; b < gridDim.x; b += 64) tot += part[(size_t)x * gridDim.x + b];
+        tot = wave_sum(tot);
+        if (threadIdx.x == 0) red[0] = tot;
+      }
+      __syncthreads();
+      tot = red[0];
+      __syncthreads();
+      double nrm = sqrt(tot);
+      if (nrm != 0.0) {
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) w[r][x] = (float)((double)u[r] / nrm);
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < RPT; ++r) {
+    long i = gtid + r * total;
+    if (i < m) {
+#pragma unroll
+      for (int t = 0; t < KM; ++t)
+        if (t < k) W[i + ldw * t] = w[r][t];
+    }
+  }
+}
+
+// Fallback for any m: one column per pair of launches.  Step 1 computes the
+// column update for every row into W(:,x) and block sums of squares.
+template <int KM>
+__global__ __launch_bounds__(NTHR) void hals_col_step_kernel(float *W, long m, int k, long ldw,
+                                                             const float *AH, long ldah,
+                                                             const double *G, int x,
+                                                             double *part) {
+  __shared__ float Gs[KM][KM + 1];
+  __shared__ double red[NTHR / 64];
+  load_gram<KM>(Gs, G, k);
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double v2 = 0.0;
+  if (i < m) {
+    float s = 0.0f;
+    for (int t = 0; t < k; ++t) s = fmaf(W[i + ldw * t], Gs[t][x], s);
+    float v = (W[i + ldw * x] * Gs[x][x]) + (AH[i + ldah * x] - s);
+    v = (v < 1e-17f) ? 1e-17f : v;
+    W[i + ldw * x] = v;
+    v2 = (double)v * v;
+  }
+  double bs = block_sum(v2, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = bs;
+}
+
+__global__ void hals_col_scale_kernel(float *W, long m, long ldw, int x, const double *part,
+                                      long nparts) {
+  __shared__ double nrm_s;
+  if (threadIdx.x < 64) {
+    double t = 0.0;
+    for (long b = threadIdx.x; b < nparts; b += 64) t += part[b];
+    t = wave_sum(t);
+    if (threadIdx.x == 0) nrm_s = sqrt(t);
+  }
+  __syncthreads();
+  double nrm = nrm_s;
+  if (nrm == 0.0) return;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (long)gridDim.x * blockDim.x)
+    W[i + ldw * x] = (float)((double)W[i + ldw * x] / nrm);
+}
+
+// ---------------------------------------------------------------------------
+// Column sums of squares of X (rows x k): partial[chunk*k + c].
+constexpr long COL_CHUNK = 8192;
+
+__global__ __launch_bounds__(NTHR) void colsumsq_partial_kernel(const float *X, long rows, int k,
+                                                                long ldx, double *part) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  long r0 = (long)blockIdx.x * COL_CHUNK;
+  long r1 = min(rows, r0 + COL_CHUNK);
+  for (int c = wave; c < k; c += NTHR / 64) {
+    double s = 0.0;
+    for (long r = r0 + lane; r < r1; r += 64) {
+      double v = X[r + ldx * c];
+      s += v * v;
+    }
+    s = wave_sum(s);
+    if (lane == 0) part[(size_t)blockIdx.x * k + c] = s;
+  }
+}
+
+// norms[c] = sqrt(sum of partials) in chunk order.
+__global__ void colnorm_finalize_kernel(const double *part, long nchunks, int k, double *norms) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= k) return;
+  double s = 0.0;
+  for (long b = 0; b < nchunks; ++b) s += part[(size_t)b * k + c];
+  norms[c] = sqrt(s);
+}
+
+// X(:,c) /= norms[c] and, if Y, Y(:,c) *= norms[c]; columns with norm 0 untouched.
+__global__ void scale_cols_kernel(float *X, long rows, long ldx, float *Y, long yrows, long ldy,
+                                  int k, const double *norms) {
+  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long tot = rows * k, toty = Y ? yrows * k : 0;
+  if (e < tot) {
+    int c = (int)(e / rows);
+    long r = e % rows;
+    double nv = norms[c];
+    if (nv > 0) X[r + ldx * c] = (float)((double)X[r + ldx * c] / nv);
+  } else if (e < tot + toty) {
+    long f = e - tot;
+    int c = (int)(f / yrows);
+    long r = f % yrows;
+    double nv = norms[c];
+    if (nv > 0) Y[r + ldy * c] = (float)((double)Y[r + ldy * c] * nv);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 reductions: sum of X .* Y over a rows x cols pair of column-major
+// matrices (Y may equal X for a sum of squares).  Two stages, fixed order.
+__global__ __launch_bounds__(NTHR) void dot_partial_kernel(const float *X, long ldx, const float *Y,
+                                                           long ldy, long rows, long cols,
+                                                           double *part) {
+  __shared__ double red[NTHR / 64];
+  long tot = rows * cols;
+  double s = 0.0;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (long)gridDim.x * blockDim.x) {
+    long r = e % rows, c = e / rows;
+    s += (double)X[r + ldx * c] * (double)Y[r + ldy * c];
+  }
+  double bs = block_sum(s, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = bs;
+}
+
+__global__ void sum_partials_kernel(const double *part, long n, double *out, int accumulate) {
+  if (threadIdx.x >= 64) return;
+  double t = 0.0;
+  for (long b = threadIdx.x; b < n; b += 64) t += part[b];
+  t = wave_sum(t);
+  if (threadIdx.x == 0) *out = accumulate ? (*out + t) : t;
+}
+
+// NMF relative error from the trace identity (common/nmf.hpp:390-424):
+// fit^2 = ||A||^2 - 2 <W, A H> + sum(WtW .* HtH); printObjective prints
+// sqrt(fit^2)/||A||, or 1 when fit^2 <= 0 (common/nmf.hpp:462).
+__global__ void nmf_relerr_kernel(const double *normA2, const double *dotv, const double *WtW,
+                                  const double *HtH, int k, double *relerr_out, double *fit_out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double tr = 0.0;
+  for (int b = 0; b < k; ++b)
+    for (int a = 0; a < k; ++a) tr += WtW[a + (size_t)k * b] * HtH[b + (size_t)k * a];
+  double fit = *normA2 - 2.0 * (*dotv) + tr;
+  double nA = sqrt(*normA2);
+  double err = (fit > 0) ? sqrt(fit) : nA;
+  *relerr_out = err / nA;
+  if (fit_out) *fit_out = fit;
+}
+
+// G += l2 * I + l1 * ones (applyReg, common/nmf.hpp:76-90; the reference
+// keeps the values in float).
+__global__ void apply_reg_kernel(double *G, int k, float l2, float l1) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= k * k) return;
+  int a = e % k, b = e / k;
+  double v = G[e];
+  if (l2 > 0 && a == b) v += l2;
+  if (l1 > 0) v += l1;
+  G[e] = v;
+}
+
+// ---------------------------------------------------------------------------
+// 16-bit copies: per-column power-of-two scale then round.
+__global__ __launch_bounds__(NTHR) void colmax_scale_kernel(const float *A, long rows, long lda,
+                                                            float *scale) {
+  __shared__ float red[NTHR / 64];
+  long c = blockIdx.x;
+  float mx = 0.0f;
+  for (long r = threadIdx.x; r < rows; r += blockDim.x) mx = fmaxf(mx, fabsf(A[r + lda * c]));
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float m = 0.0f;
+    for (int w = 0; w < NTHR / 64; ++w) m = fmaxf(m, red[w]);
+    float s = 1.0f;
+    if (m > 0.0f && isfinite(m)) {
+      int ex;
+      frexpf(m, &ex);  // m = f * 2^ex, f in [0.5, 1)
+      s = ldexpf(1.0f, ex);
+    }
+    scale[c] = s;
+  }
+}
+
+template <int DT>
+__global__ void convert_kernel(const float *A, long rows, long cols, long lda, uint16_t *out,
+                               long ldo, const float *scale) {
+  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long tot = rows * cols;
+  for (; e < tot; e += (long)gridDim.x * blockDim.x) {
+    long r = e % rows, c = e / rows;
+    float v = A[r + lda * c];
+    if (scale) v = v / scale[c];
+    out[r + ldo * c] = (DT == PLANC_BF16) ? float_to_bf16(v) : float_to_f16(v);
+  }
+}
+
+template <int KM>
+int launch_rows(planc_ctx *ctx, int kind, float *H, long n, int k, long ldh, const float *N,
+                long nrs, long ncs, const double *G, float eps) {
+  unsigned nb = (unsigned)((n + NTHR - 1) / NTHR);
+  if (nb == 0) return PLANC_OK;
+  if (kind == 0)
+    mu_update_kernel<KM><<<nb, NTHR, 0, ctx->stream>>>(H, n, k, ldh, N, nrs, ncs, G, eps);
+  else
+    hals_rows_kernel<KM><<<nb, NTHR, 0, ctx->stream>>>(H, n, k, ldh, N, nrs, ncs, G);
+  PLANC_LAUNCH_CHECK(ctx);
+  return PLANC_OK;
+}
+
+template <int KM, int RPT>
+int try_coop(planc_ctx *ctx, float *W, long m, int k, long ldw, const float *AH, long ldah,
+             const double *G, bool *done) {
+  *done = false;
+  int per_cu = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &per_cu, (const void *)hals_cols_coop_kernel<KM, RPT>, NTHR, 0);
+  if (e != hipSuccess || per_cu <= 0) return PLANC_OK;
+  long maxblocks = (long)per_cu * ctx->num_cu;
+  long need = (m + (long)NTHR * RPT - 1) / ((long)NTHR * RPT);
+  if (need > maxblocks || need <= 0) return PLANC_OK;
+  unsigned nb = (unsigned)need;
+  double *part = (double *)ws_get(ctx, 2, sizeof(double) * (size_t)KM * nb);
+  if (!part) return set_error(ctx, PLANC_ENOMEM, "hals_cols: workspace");
+  void *args[] = {&W, &m, &k, &ldw, &AH, &ldah, &G, &part};
+  PLANC_HIP_CHECK(ctx, hipLaunchCooperativeKernel((const void *)hals_cols_coop_kernel<KM, RPT>,
+                                                  dim3(nb), dim3(NTHR), args, 0, ctx->stream));
+  *done = true;
+  return PLANC_OK;
+}
+
+template <int KM>
+int hals_cols_impl(planc_ctx *ctx, float *W, long m, int k, long ldw, const float *AH, long ldah,
+                   const double *G) {
+  bool done = false;
+  int rc = try_coop<KM, 1>(ctx, W, m, k, ldw, AH, ldah, G, &done);
+  if (rc || done) return rc;
+  if (KM <= 32) {
+    rc = try_coop<KM, 2>(ctx, W, m, k, ldw, AH, ldah, G, &done);
+    if (rc || done) return rc;
+  }
+  // Fallback: two launches per column.
+  unsigned nb = (unsigned)((m + NTHR - 1) / NTHR);
+  double *part = (double *)ws_get(ctx, 2, sizeof(double) * (size_t)nb);
+  if (!part) return set_error(ctx, PLANC_ENOMEM, "hals_cols: workspace");
+  for (int x = 0; x < k; ++x) {
+    hals_col_step_kernel<KM><<<nb, NTHR, 0, ctx->stream>>>(W, m, k, ldw, AH, ldah, G, x, part);
+    PLANC_LAUNCH_CHECK(ctx);
+    unsigned sb = nb < 1024 ? nb : 1024;
+    hals_col_scale_kernel<<<sb, NTHR, 0, ctx->stream>>>(W, m, ldw, x, part, nb);
+    PLANC_LAUNCH_CHECK(ctx);
+  }
+  return PLANC_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host entry points used by the C ABI and the loop drivers.
+
+int gram(planc_ctx *ctx, const float *X, long rows, int k, long ldx, double *G) {
+  if (k <= 0 || k > 64 || rows < 0) return set_error(ctx, PLANC_EINVAL, "gram: k must be 1..64");
+  int kp = ((k + 3) / 4) * 4;
+  long nchunks = (rows + GRAM_RC - 1) / GRAM_RC;
+  if (nchunks == 0) nchunks = 1;
+  long nb = nchunks < 1024 ? nchunks : 1024;
+  // Each block writes the chunks it owns; chunks are indexed globally.
+  double *part = (double *)ws_get(ctx, 1, sizeof(double) * (size_t)nchunks * kp * kp);
+  if (!part) return set_error(ctx, PLANC_ENOMEM, "gram: workspace");
+  if (rows == 0) {
+    PLANC_HIP_CHECK(ctx, hipMemsetAsync(part, 0, sizeof(double) * kp * kp, ctx->stream));
+  } else {
+    gram_partial_kernel<<<(unsigned)nb, NTHR, 0, ctx->stream>>>(X, rows, k, ldx, part, kp);
+    PLANC_LAUNCH_CHECK(ctx);
+  }
+  gram_reduce_kernel<<<(k * k + 255) / 256, 256, 0, ctx->stream>>>(part, nchunks, k, kp, G);
+  PLANC_LAUNCH_CHECK(ctx);
+  return PLANC_OK;
+}
+
+int row_update(planc_ctx *ctx, int kind, float *H, long n, int k, long ldh, const float *N, long nrs,
+               long ncs, const double *G, float eps) {
+  if (k <= 16) return launch_rows<16>(ctx, kind, H, n, k, ldh, N, nrs, ncs, G, eps);
+  if (k <= 32) return launch_rows<32>(ctx, kind, H, n, k, ldh, N, nrs, ncs, G, eps);
+  if (k <= 48) return launch_rows<48>(ctx, kind, H, n, k, ldh, N, nrs, ncs, G, eps);
+  if (k <= 64) return launch_rows<64>(ctx, kind, H, n, k, ldh, N, nrs, ncs, G, eps);
+  return set_error(ctx, PLANC_EUNSUPPORTED, "row update: k > 64");
+}
+
+int hals_cols(planc_ctx *ctx, float *W, long m, int k, long ldw, const float *AH, long ldah,
+              const double *G) {
+  if (m <= 0) return PLANC_OK;
+  if (k <= 16) return hals_cols_impl<16>(ctx, W, m, k, ldw, AH, ldah, G);
+  if (k <= 32) return hals_cols_impl<32>(ctx, W, m, k, ldw, AH, ldah, G);
+  if (k <= 48) return hals_cols_impl<48>(ctx, W, m, k, ldw, AH, ldah, G);
+  if (k <= 64) return hals_cols_impl<64>(ctx, W, m, k, ldw, AH, ldah, G);
+  return set_error(ctx, PLANC_EUNSUPPORTED, "hals_cols: k > 64");
+}
+
+// norms (device, k doubles) of the columns of X.
+int col_norms(planc_ctx *ctx, const float *X, long rows, int k, long ldx, double *norms) {
+  long nchunks = (rows + COL_CHUNK - 1) / COL_CHUNK;
+  if (nchunks == 0) nchunks = 1;
+  double *part = (double *)ws_get(ctx, 2, sizeof(double) * (size_t)nchunks * k);
+  if (!part) return set_error(ctx, PLANC_ENOMEM, "col_norms: workspace");
+  colsumsq_partial_kernel<<<(unsigned)nchunks, NTHR, 0, ctx->stream>>>(X, rows, k, ldx, part);
+  PLANC_LAUNCH_CHECK(ctx);
+  colnorm_finalize_kernel<<<(k + 63) / 64, 64, 0, ctx->stream>>>(part, nchunks, k, norms);
+  PLANC_LAUNCH_CHECK(ctx);
+  return PLANC_OK;
+}
+
+int scale_cols(planc_ctx *ctx, float *X, long rows, long ldx, float *Y, long yrows, long ldy, int k,
+               const double *norms) {
+  long tot = rows * k + (Y ? yrows * k : 0);
+  if (tot == 0) return PLANC_OK;
+  scale_cols_kernel<<<(unsigned)((tot + 255) / 256), 256, 0, ctx->stream>>>(X, rows, ldx, Y, yrows,
+                                                                            ldy, k, norms);
+  PLANC_LAUNCH_CHECK(ctx);
+  return PLANC_OK;
+}
+
+// *out = sum(X .* Y) in FP64 (accumulate: *out += ...).
+int dot(planc_ctx *ctx, const float *X, long ldx, const float *Y, long ldy, long rows, long cols,
+        double *out, int accumulate) {
+  long tot = rows * cols;
+  long nb = (tot + NTHR * 8 - 1) / (NTHR * 8);
+  if (nb > 2048) nb = 2048;
+  if (nb < 1) nb = 1;
+  double *part = (double *)ws_get(ctx, 3, sizeof(double) * (size_t)nb);
+  if (!part) return set_error(ctx, PLANC_ENOMEM, "dot: workspace");
+  dot_partial_kernel<<<(unsigned)nb, NTHR, 0, ctx->stream>>>(X, ldx, Y, ldy, rows, cols, part);
+  PLANC_LAUNCH_CHECK(ctx);
+  sum_partials_kernel<<<1, 64, 0, ctx->stream>>>(part, nb, out, accumulate);
+  PLANC_LAUNCH_CHECK(ctx);
+  return PLANC_OK;
+}
+
+int nmf_relerr(planc_ctx *ctx, const double *normA2, const double *dotv, const double *WtW,
+               const double *HtH, int k, double *relerr_out, double *fit_out) {
+  nmf_relerr_kernel<<<1, 64, 0, ctx->stream>>>(normA2, dotv, WtW, HtH, k, relerr_out, fit_out);
+  PLANC_LAUNCH_CHECK(ctx);
+  return PLANC_OK;
+}
+
+int apply_reg(planc_ctx *ctx, double *G, int k, const double *reg) {
+  if (!reg || (!(reg[0] > 0) && !(reg[1] > 0))) return PLANC_OK;
+  apply_reg_kernel<<<(k * k + 255) / 256, 256, 0, ctx->stream>>>(G, k, (float)reg[0], (float)reg[1]);
+  PLANC_LAUNCH_CHECK(ctx);
+  return PLANC_OK;
+}
+
+int convert16(planc_ctx *ctx, int dtype, const float *A, long rows, long cols, long lda, void *out,
+              long ldo, float *colscale) {
+  if (colscale && cols > 0) {
+    colmax_scale_kernel<<<(unsigned)cols, NTHR, 0, ctx->stream>>>(A, rows, lda, colscale);
+    PLANC_LAUNCH_CHECK(ctx);
+  }
+  long tot = rows * cols;
+  if (tot == 0) return PLANC_OK;
+  long nb = (tot + 255) / 256;
+  if (nb > 65536) nb = 65536;
+  if (dtype == PLANC_BF16)
+    convert_kernel<PLANC_BF16><<<(unsigned)nb, 256, 0, ctx->stream>>>(A, rows, cols, lda, (uint16_t *)out,
+                                                                       ldo, colscale);
+  else
+    convert_kernel<PLANC_F16><<<(unsigned)nb, 256, 0, ctx->stream>>>(A, rows, cols, lda, (uint16_t *)out,
+                                                                      ldo, colscale);
+  PLANC_LAUNCH_CHECK(ctx);
+  return PLANC_OK;
+}
+
+}  // namespace planc

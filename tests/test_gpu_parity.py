@@ -1,0 +1,291 @@
+"""GPU parity tests: the HIP path (through the C ABI) against the FP64 oracle
+and NumPy on the same seeded inputs.
+
+Tolerances (stated per test): FP32 products are compared with relative error
+<= 2e-6 * sqrt(K) scaled bounds; BF16/FP16 products against the FP64 product
+of the identically rounded operands (this pins the MFMA operand layouts
+exactly, independent of the 16-bit rounding error); whole NMF/NTF runs in
+FP32 against the FP64 oracle trajectory.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+import planc_amd as P  # noqa: E402
+from oracle import oracle as O  # noqa: E402
+from planc_amd import datagen as D  # noqa: E402
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    c = P.Context(0)
+    yield c
+    c.close()
+
+
+def bf16_round(a):
+    """float32 -> bf16 (round to nearest even) -> float64."""
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    lsb = (u >> np.uint64(16)) & np.uint64(1)
+    r = ((u + np.uint64(0x7FFF) + lsb) >> np.uint64(16)) << np.uint64(16)
+    return r.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def f16_round(a):
+    return np.asarray(a, dtype=np.float32).astype(np.float16).astype(np.float64)
+
+
+def rnd(dt, a):
+    return {P.F32: lambda x: np.asarray(x, np.float32).astype(np.float64), P.BF16: bf16_round,
+            P.F16: f16_round}[dt](a)
+
+
+def rel_err(x, ref):
+    return np.linalg.norm(np.asarray(x, np.float64) - ref) / max(np.linalg.norm(ref), 1e-300)
+
+
+GEMM_CASES = [
+    # (L, K, k)
+    (128, 32, 16),
+    (300, 77, 3),
+    (1000, 513, 20),
+    (257, 4096, 33),
+    (4000, 300, 48),
+    (513, 1000, 64),
+    (200, 150, 70),
+    (64, 20000, 32),  # split-K path
+]
+
+
+@pytest.mark.parametrize("dt", [P.F32, P.BF16, P.F16])
+@pytest.mark.parametrize("L,K,k", GEMM_CASES)
+def test_gemm_nn_tn(ctx, dt, L, K, k):
+    say(f"gemm dt={dt} L={L} K={K} k={k}")
+    rng = np.random.default_rng(L * 7 + K + k)
+    A = rng.random((L, K), dtype=np.float32)
+    B = rng.random((K, k), dtype=np.float32)
+    dA = P.colmajor(A)
+    dB = P.colmajor(B)
+    if dt == P.F32:
+        Aop, cs = dA, None
+        Ar = A.astype(np.float64)
+        Br = B.astype(np.float64)
+        tol = 3e-6
+    else:
+        Aop, cs = ctx.convert(dA, dt, scaled=False)
+        Ar = rnd(dt, A)
+        Br = rnd(dt, B)
+        tol = 1e-5
+    C = ctx.gemm_nn(Aop, dB, dt, cs).cpu().numpy()
+    assert rel_err(C, Ar @ Br) < tol
+    # TN: B2 (L x k), C2 = B2^T A  (k x K)
+    B2 = rng.random((L, k), dtype=np.float32)
+    C2 = ctx.gemm_tn(Aop, P.colmajor(B2), dt, cs).cpu().numpy()
+    assert rel_err(C2, rnd(dt, B2).T @ Ar) < tol
+
+
+@pytest.mark.parametrize("dt", [P.BF16, P.F16])
+def test_gemm_column_scales(ctx, dt):
+    say(f"gemm scaled dt={dt}")
+    rng = np.random.default_rng(5)
+    L, K, k = 700, 900, 24
+    A = (rng.random((L, K)) * np.exp(rng.uniform(-8, 8, K))[None, :]).astype(np.float32)
+    B = rng.random((K, k), dtype=np.float32)
+    dA = P.colmajor(A)
+    A16, cs = ctx.convert(dA, dt, scaled=True)
+    s = cs.cpu().numpy().astype(np.float64)
+    m = np.abs(A).max(0)
+    assert np.all((m / s >= 0.5) & (m / s < 1.0))
+    Ar = rnd(dt, A / s[None, :].astype(np.float32)) * s[None, :]
+    C = ctx.gemm_nn(A16, P.colmajor(B), dt, cs).cpu().numpy()
+    Bs = rnd(dt, (B * s[:, None]).astype(np.float32))
+    assert rel_err(C, (Ar / s[None, :]) @ Bs) < 1e-5
+    B2 = rng.random((K, k), dtype=np.float32)
+    A2 = A.T.copy()  # K x L for TN with column scales over L... reuse A as K x L
+    dA2 = P.colmajor(A2)
+    A216, cs2 = ctx.convert(dA2, dt, scaled=True)
+    s2 = cs2.cpu().numpy().astype(np.float64)
+    C2 = ctx.gemm_tn(A216, P.colmajor(B2), dt, cs2).cpu().numpy()
+    A2r = rnd(dt, A2 / s2[None, :].astype(np.float32)) * s2[None, :]
+    assert rel_err(C2, rnd(dt, B2).T @ A2r) < 1e-5
+
+
+@pytest.mark.parametrize("rows,k", [(1, 1), (100, 7), (5000, 20), (65536 + 3, 64), (256, 32)])
+def test_gram_fp64(ctx, rows, k):
+    say(f"gram rows={rows} k={k}")
+    rng = np.random.default_rng(rows + k)
+    X = rng.random((rows, k), dtype=np.float32)
+    G = ctx.gram(P.colmajor(X)).cpu().numpy()
+    ref = X.astype(np.float64).T @ X.astype(np.float64)
+    assert rel_err(G, ref) < 1e-13
+
+
+@pytest.mark.parametrize("k", [2, 16, 20, 33, 64])
+def test_mu_and_hals_row_updates(ctx, k):
+    say(f"row updates k={k}")
+    rng = np.random.default_rng(k)
+    n = 777
+    H = rng.random((n, k), dtype=np.float32)
+    N = (rng.random((k, n), dtype=np.float32) * 3)  # k x n like W^T A
+    Wt = rng.random((50, k))
+    G = Wt.T @ Wt
+    dG = P.colmajor(G)
+    # MU: H .* N^T ./ (H G + eps)
+    dH = P.colmajor(H)
+    ctx.mu_update(dH, P.colmajor(N), dG, 1e-17, n_rs=k, n_cs=1)
+    Hd = H.astype(np.float64)
+    ref = Hd * N.T.astype(np.float64) / (Hd @ G + 1e-17)
+    assert rel_err(dH.cpu().numpy(), ref) < 2e-6
+    # HALS rows
+    dH = P.colmajor(H)
+    ctx.hals_update_rows(dH, P.colmajor(N), dG, n_rs=k, n_cs=1)
+    Hr = H.astype(np.float64).copy()
+    for x in range(k):
+        v = Hr[:, x] + (N[x, :].astype(np.float64) - Hr @ G[:, x])
+        Hr[:, x] = np.where(v < 1e-17, 1e-17, v)
+    assert rel_err(dH.cpu().numpy(), Hr) < 1e-5
+
+
+@pytest.mark.parametrize("m,k", [(1000, 5), (65536, 64), (3000, 20), (70000, 16), (600000, 8)])
+def test_hals_cols_sweep(ctx, m, k):
+    say(f"hals cols m={m} k={k}")
+    rng = np.random.default_rng(m + k)
+    W = rng.random((m, k), dtype=np.float32)
+    AH = (rng.random((m, k), dtype=np.float32) * 5)
+    Ht = rng.random((40, k))
+    G = Ht.T @ Ht
+    dW = P.colmajor(W)
+    ctx.hals_update_cols(dW, P.colmajor(AH), P.colmajor(G))
+    Wr = W.astype(np.float64).copy()
+    for x in range(k):
+        v = Wr[:, x] * G[x, x] + (AH[:, x].astype(np.float64) - Wr @ G[:, x])
+        v = np.where(v < 1e-17, 1e-17, v)
+        Wr[:, x] = v / np.linalg.norm(v)
+    assert rel_err(dW.cpu().numpy(), Wr) < 2e-5
+
+
+@pytest.mark.parametrize("dims,k", [((7, 8, 9), 3), ((20, 18, 16), 5), ((5, 6, 4, 3), 2),
+                                    ((33, 17, 20, 9), 16), ((64, 64, 64), 32), ((3, 4, 5, 6, 2), 4)])
+def test_mttkrp_direct_all_modes(ctx, dims, k):
+    say(f"mttkrp dims={dims} k={k}")
+    rng = np.random.default_rng(sum(dims) + k)
+    X = np.asfortranarray(rng.random(dims, dtype=np.float32))
+    F = [rng.random((d, k), dtype=np.float32) for d in dims]
+    dX = P.colmajor(X)
+    dF = [P.colmajor(f) for f in F]
+    for n in range(len(dims)):
+        ref = O.mttkrp(X.astype(np.float64), [f.astype(np.float64) for f in F], n)
+        M = ctx.mttkrp(dX, dF, n).cpu().numpy()
+        assert rel_err(M, ref) < 1e-5, f"mode {n}"
+
+
+def test_golden_mttkrp_fixture(ctx):
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "mttkrp_4way.npz"))
+    X = np.asfortranarray(g["X"].astype(np.float32))
+    F = [g[f"F{i}"].astype(np.float32) for i in range(4)]
+    dX = P.colmajor(X)
+    dF = [P.colmajor(f) for f in F]
+    for n in range(4):
+        assert rel_err(ctx.mttkrp(dX, dF, n).cpu().numpy(), g[f"M{n}"]) < 1e-5
+
+
+def _nmf_case(m, n, k, stream):
+    A = D.lowrank_matrix(m, n, k, 0.01, stream=stream).astype(np.float32)
+    W0, H0 = D.init_factors([(m, k), (n, k)], stream=stream)
+    return A, W0.astype(np.float32), H0.astype(np.float32)
+
+
+@pytest.mark.parametrize("algo", [P.MU, P.HALS])
+def test_nmf_one_iteration_factors(ctx, algo):
+    say(f"nmf one iteration algo={algo}")
+    A, W0, H0 = _nmf_case(500, 300, 10, stream=11)
+    Wr, Hr, rel_r, _, rc = O.nmf(algo, A.astype(np.float64), W0.astype(np.float64),
+                                 H0.astype(np.float64), 1)
+    dA, dW, dH = P.colmajor(A), P.colmajor(W0), P.colmajor(H0)
+    rel = ctx.nmf_run(dA, dW, dH, algo, 1)
+    # factors after one iteration: FP32 vs FP64, relative 1e-5
+    assert rel_err(dW.cpu().numpy(), Wr) < 1e-5
+    assert rel_err(dH.cpu().numpy(), Hr) < 1e-5
+    assert abs(rel[0] - rel_r[0]) < 1e-6
+
+
+@pytest.mark.parametrize("algo", [P.MU, P.HALS])
+def test_nmf_config1_trajectory(ctx, algo):
+    """Config 1 shape: A 2000 x 1000 = W0 H0 + |N(0, 0.01)|, k = 20, 20
+    iterations; the per-iteration relative error of the FP32 GPU run matches
+    the FP64 oracle to 1e-5 absolute."""
+    say(f"nmf config1 algo={algo}")
+    A, W0, H0 = _nmf_case(2000, 1000, 20, stream=1)
+    _, _, rel_r, _, rc = O.nmf(algo, A.astype(np.float64), W0.astype(np.float64),
+                               H0.astype(np.float64), 20)
+    dA, dW, dH = P.colmajor(A), P.colmajor(W0), P.colmajor(H0)
+    rel = ctx.nmf_run(dA, dW, dH, algo, 20)
+    say(f"  gpu {rel[-1]:.8f} oracle {rel_r[-1]:.8f}")
+    assert np.abs(rel - rel_r).max() < 1e-5
+    # testcases.md:3 -- decreasing objective error
+    assert np.all(np.diff(rel) <= 1e-7)
+
+
+@pytest.mark.parametrize("dt", [P.BF16, P.F16])
+@pytest.mark.parametrize("algo", [P.MU, P.HALS])
+def test_nmf_mixed_precision_final_error(ctx, algo, dt):
+    """BF16/FP16 A with FP32 accumulation ends within 1e-3 of the FP64
+    oracle's final relative error (the headline equal-error condition)."""
+    say(f"nmf mixed algo={algo} dt={dt}")
+    A, W0, H0 = _nmf_case(2000, 1000, 20, stream=1)
+    _, _, rel_r, _, _ = O.nmf(algo, A.astype(np.float64), W0.astype(np.float64),
+                              H0.astype(np.float64), 20)
+    dA, dW, dH = P.colmajor(A), P.colmajor(W0), P.colmajor(H0)
+    rel = ctx.nmf_run(dA, dW, dH, algo, 20, dtype=dt)
+    say(f"  gpu {rel[-1]:.6f} oracle {rel_r[-1]:.6f}")
+    assert abs(rel[-1] - rel_r[-1]) < 1e-3
+
+
+@pytest.mark.parametrize("algo", [P.MU, P.HALS])
+@pytest.mark.parametrize("dimtree", [True, False])
+@pytest.mark.parametrize("dims,k", [((20, 18, 16), 5), ((12, 10, 9, 8), 4)])
+def test_ntf_trajectory(ctx, algo, dimtree, dims, k):
+    say(f"ntf algo={algo} dimtree={dimtree} dims={dims}")
+    X, _ = D.cp_tensor(dims, k, 0.01, stream=2)
+    X32 = X.astype(np.float32)
+    F0 = [f.astype(np.float32) for f in D.init_factors([(d, k) for d in dims], stream=2)]
+    Fr, lam_r, rel_r, rc = O.ntf(algo, X32.astype(np.float64), [f.astype(np.float64) for f in F0],
+                                 10, dimtree=1 if dimtree else 0)
+    dX = P.colmajor(X32)
+    dF = [P.colmajor(f) for f in F0]
+    lam, rel = ctx.ntf_run(dX, dF, algo, 10, dimtree=dimtree)
+    say(f"  gpu {rel[-1]:.8f} oracle {rel_r[-1]:.8f}")
+    assert np.abs(rel - rel_r).max() < 2e-5
+    for f, fr in zip(dF, Fr):
+        assert rel_err(f.cpu().numpy(), fr) < 1e-3
+    assert rel_err(lam.cpu().numpy(), lam_r) < 1e-3
+
+
+def test_ntf_config2_shape(ctx):
+    """Config 2 shape: 256^3 CP rank-32 + 1% noise, k = 32, HALS, dimension
+    tree; 5 iterations against the FP64 oracle (relative error to 1e-5)."""
+    say("ntf config2 256^3")
+    dims, k = (256, 256, 256), 32
+    X, _ = D.cp_tensor(dims, k, 0.01, stream=2)
+    X32 = X.astype(np.float32)
+    F0 = [f.astype(np.float32) for f in D.init_factors([(d, k) for d in dims], stream=2)]
+    _, _, rel_r, _ = O.ntf(1, X32.astype(np.float64), [f.astype(np.float64) for f in F0], 5)
+    dX = P.colmajor(X32)
+    dF = [P.colmajor(f) for f in F0]
+    lam, rel = ctx.ntf_run(dX, dF, P.HALS, 5)
+    say(f"  gpu {rel} oracle {rel_r}")
+    assert np.abs(rel - rel_r).max() < 1e-5
