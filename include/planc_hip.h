@@ -78,6 +78,24 @@ int planc_memcpy_d2h(planc_ctx *ctx, void *h_dst, const void *d_src, size_t byte
 int planc_convert(planc_ctx *ctx, int dtype, const float *d_src, long rows, long cols, long ld_src,
                   void *d_dst, long ld_dst, float *d_colscale);
 
+/* Synthetic inputs on the device (Philox4x64-10; element e of stream
+ * (seed, stream) equals numpy.random.Philox(key=[seed, 0],
+ * counter=[0, stream, 0, 0]) word e).  No reference counterpart: the
+ * reference draws synthetic inputs from Armadillo's RNG (nmf/nmf.cpp:
+ * 132-167), which cannot be reproduced without Armadillo.
+ *  planc_gen_uniform:   d_out[e] = U[0,1)_e (53-bit double rounded to float).
+ *  planc_gen_lowrank:   A = W0 (m x r) * H0 (r x n) + sigma * |N(0,1)|, the
+ *                       product in FP64, the normal by Box-Muller.
+ *  planc_gen_cp_tensor: X = [[F_0..F_{N-1}]] + sigma * |N(0,1)|, mode 0 fastest. */
+int planc_gen_uniform(planc_ctx *ctx, float *d_out, long count, unsigned long long seed,
+                      unsigned long long stream);
+int planc_gen_lowrank(planc_ctx *ctx, long m, long n, int r, const float *d_W0, const float *d_H0,
+                      double sigma, unsigned long long seed, unsigned long long stream, float *d_A,
+                      long lda);
+int planc_gen_cp_tensor(planc_ctx *ctx, int nmodes, const long *dims, const float *const *d_factors,
+                        int r, double sigma, unsigned long long seed, unsigned long long stream,
+                        float *d_X);
+
 /* ---- kernel-level entry points ------------------------------------------- */
 /* C (L x k) = A (L x K) * B (K x k).  Replaces the dgemm behind `A * H`
  * (nmf/mu.hpp:71, nmf/hals.hpp:81) and, with B a formed KRP, the mode-0
@@ -140,6 +158,24 @@ int planc_hadamard_grams(planc_ctx *ctx, const double *const *d_grams, int nmode
  * host array of device pointers (I_i x k, column-major).  nmodes <= 8. */
 int planc_mttkrp(planc_ctx *ctx, int dtype, const void *d_X, int nmodes, const long *dims,
                  const float *const *d_factors, int k, int mode, float *d_out, long ldo);
+
+/* Partial MTTKRP of the dimension tree (partial_MTTKRP_with_KRP,
+ * dimtree/dimtrees.hpp:176-237): with the modes split at s (left = modes
+ * 0..s, L = prod of their dims; right = the rest, R), side 0 computes the
+ * right contraction P = X_(L x R) * KRP(right factors), an L x k projection;
+ * side 1 the left contraction KRP(left factors)^T * X_(L x R), stored as an
+ * R x k projection.  d_out is column-major with ld ldo (>= L or R). */
+int planc_mttkrp_partial(planc_ctx *ctx, int dtype, const void *d_X, int nmodes, const long *dims,
+                         const float *const *d_factors, int k, int split, int side, float *d_out,
+                         long ldo);
+
+/* Multi-TTV of the dimension tree (multi_TTV_with_KRP, dimtree/dimtrees.hpp:
+ * 418-455): contracts a projection P (E x k, column-major, ld ldp) over all
+ * modes of the block d_factors[0..nb) except `target`, giving the MTTKRP of
+ * that mode as k x dims[target] (ld ldo).  dims/d_factors describe the
+ * block's modes, lowest mode fastest in P's row index. */
+int planc_multi_ttv(planc_ctx *ctx, const float *d_P, long ldp, int nb, const long *dims,
+                    const float *const *d_factors, int target, int k, float *d_out, long ldo);
 
 /* ---- whole loops (computeNMF / computeNTF on the device) ----------------- */
 typedef struct planc_nmf_desc {

@@ -778,12 +778,13 @@ static int ntf_loop(NTFState &s, int algo, int iters, double *relerr) {
   for (int it = 0; it < iters; ++it) {
     for (int j = 0; j < N; ++j) {
       Mat S = gram_leave_out_one(s, j);
-      if (s.dimtree) {
+      // The reference forms the KRP even when the dimension tree is used
+      // (ntf/auntf.hpp:138-139); kept so the CPU timing matches its work.
+      krp_leave_out_one(s.F, j, s.k, krp);
+      if (s.dimtree)
         mttkrp_dimtree(s.X, s.F, j, s.k, s.dt, M);
-      } else {
-        krp_leave_out_one(s.F, j, s.k, krp);
+      else
         mttkrp_direct(s.X, krp, j, s.k, M);
-      }
       int rc = ntf_update(algo, s.F[j], M, S, s.k);
       if (rc < 0) return rc;
       ncp_normalize_mode(s, j);

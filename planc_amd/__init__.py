@@ -107,6 +107,34 @@ class Context:
     def sync(self):
         self._check(self.lib.planc_sync(self.ctx), "planc_sync")
 
+    # ---- synthetic inputs on the device ----------------------------------
+    def gen_uniform(self, shape, seed, stream):
+        """Column-major FP32 U[0,1) tensor (same stream as datagen.uniform)."""
+        out = empty_colmajor(shape, device=f"cuda:{self.device}")
+        n = int(np.prod(shape))
+        self._check(self.lib.planc_gen_uniform(self.ctx, _ptr(out), n, seed, stream),
+                    "planc_gen_uniform")
+        return out
+
+    def gen_lowrank(self, W0, H0, sigma, seed, stream):
+        m, r = W0.shape
+        n = H0.shape[1]
+        A = empty_colmajor((m, n), device=W0.device)
+        self._check(self.lib.planc_gen_lowrank(self.ctx, m, n, r, _ptr(W0), _ptr(H0), float(sigma),
+                                               seed, stream, _ptr(A), m), "planc_gen_lowrank")
+        return A
+
+    def gen_cp_tensor(self, factors, sigma, seed, stream):
+        N = len(factors)
+        dims = [f.shape[0] for f in factors]
+        X = empty_colmajor(tuple(dims), device=factors[0].device)
+        cd = (ctypes.c_long * N)(*dims)
+        arr = (ctypes.c_void_p * N)(*[f.data_ptr() for f in factors])
+        self._check(self.lib.planc_gen_cp_tensor(self.ctx, N, cd, arr, factors[0].shape[1],
+                                                 float(sigma), seed, stream, _ptr(X)),
+                    "planc_gen_cp_tensor")
+        return X
+
     # ---- kernel-level ----------------------------------------------------
     def convert(self, A, dtype, scaled=True):
         """16-bit copy of a column-major FP32 matrix; returns (A16, colscale)."""
@@ -197,6 +225,34 @@ class Context:
         arr = (ctypes.c_void_p * N)(*[f.data_ptr() for f in factors])
         self._check(self.lib.planc_mttkrp(self.ctx, dtype, _ptr(X), N, dims, arr, k, mode, _ptr(out),
                                           k), "planc_mttkrp")
+        return out
+
+    def mttkrp_partial(self, X, factors, split, side, dtype=F32, out=None):
+        """Dimension-tree partial MTTKRP: side 0 -> L x k right contraction,
+        side 1 -> R x k left contraction (split s: left = modes 0..s)."""
+        N = len(factors)
+        dims = [f.shape[0] for f in factors]
+        k = factors[0].shape[1]
+        L = int(np.prod(dims[:split + 1]))
+        R = int(np.prod(dims[split + 1:]))
+        rows = L if side == 0 else R
+        if out is None:
+            out = empty_colmajor((rows, k), device=factors[0].device)
+        cd = (ctypes.c_long * N)(*dims)
+        arr = (ctypes.c_void_p * N)(*[f.data_ptr() for f in factors])
+        self._check(self.lib.planc_mttkrp_partial(self.ctx, dtype, _ptr(X), N, cd, arr, k, split,
+                                                  side, _ptr(out), rows), "planc_mttkrp_partial")
+        return out
+
+    def multi_ttv(self, Pm, factors, target):
+        """Contract projection Pm (E x k) over the block modes except target."""
+        nb = len(factors)
+        k = factors[0].shape[1]
+        out = empty_colmajor((k, factors[target].shape[0]), device=Pm.device)
+        cd = (ctypes.c_long * nb)(*[f.shape[0] for f in factors])
+        arr = (ctypes.c_void_p * nb)(*[f.data_ptr() for f in factors])
+        self._check(self.lib.planc_multi_ttv(self.ctx, _ptr(Pm), _ld(Pm), nb, cd, arr, target, k,
+                                             _ptr(out), k), "planc_multi_ttv")
         return out
 
     # ---- whole loops -------------------------------------------------------

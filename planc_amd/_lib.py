@@ -51,6 +51,13 @@ SIGNATURES = {
     "planc_memcpy_d2h": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t]),
     "planc_convert": (c_int, [c_void_p, c_int, c_void_p, c_long, c_long, c_long, c_void_p, c_long,
                               c_void_p]),
+    "planc_gen_uniform": (c_int, [c_void_p, c_void_p, c_long, ctypes.c_ulonglong,
+                                  ctypes.c_ulonglong]),
+    "planc_gen_lowrank": (c_int, [c_void_p, c_long, c_long, c_int, c_void_p, c_void_p, c_double,
+                                  ctypes.c_ulonglong, ctypes.c_ulonglong, c_void_p, c_long]),
+    "planc_gen_cp_tensor": (c_int, [c_void_p, c_int, ctypes.POINTER(c_long),
+                                    ctypes.POINTER(c_void_p), c_int, c_double, ctypes.c_ulonglong,
+                                    ctypes.c_ulonglong, c_void_p]),
     "planc_gemm_nn": (c_int, [c_void_p, c_int, c_void_p, c_long, c_long, c_long, c_void_p, c_void_p,
                               c_long, c_int, c_void_p, c_long]),
     "planc_gemm_tn": (c_int, [c_void_p, c_int, c_void_p, c_long, c_long, c_long, c_void_p, c_void_p,
@@ -69,6 +76,11 @@ SIGNATURES = {
                                      c_void_p]),
     "planc_mttkrp": (c_int, [c_void_p, c_int, c_void_p, c_int, ctypes.POINTER(c_long),
                              ctypes.POINTER(c_void_p), c_int, c_int, c_void_p, c_long]),
+    "planc_mttkrp_partial": (c_int, [c_void_p, c_int, c_void_p, c_int, ctypes.POINTER(c_long),
+                                     ctypes.POINTER(c_void_p), c_int, c_int, c_int, c_void_p,
+                                     c_long]),
+    "planc_multi_ttv": (c_int, [c_void_p, c_void_p, c_long, c_int, ctypes.POINTER(c_long),
+                                ctypes.POINTER(c_void_p), c_int, c_int, c_void_p, c_long]),
     "planc_nmf_run": (c_int, [c_void_p, ctypes.POINTER(NMFDesc), c_void_p, c_long, c_void_p,
                               c_void_p, c_void_p, c_void_p, ctypes.POINTER(c_double)]),
     "planc_ntf_run": (c_int, [c_void_p, ctypes.POINTER(NTFDesc), c_void_p, c_void_p,

@@ -293,6 +293,58 @@ int planc_mttkrp(planc_ctx *ctx, int dtype, const void *d_X, int nmodes, const l
   return mttkrp_direct(ctx, dtype, d_X, nmodes, dims, d_factors, k, mode, d_out, ldo, P);
 }
 
+int planc_mttkrp_partial(planc_ctx *ctx, int dtype, const void *d_X, int nmodes, const long *dims,
+                         const float *const *d_factors, int k, int split, int side, float *d_out,
+                         long ldo) {
+  if (check_ctx(ctx)) return PLANC_EINVAL;
+  if (!valid_dtype(dtype) || nmodes < 2 || nmodes > 7 || !dims || !d_factors || split < 0 ||
+      split > nmodes - 2 || (side != 0 && side != 1) || k <= 0 || !d_X || !d_out)
+    return set_error(ctx, PLANC_EINVAL, "planc_mttkrp_partial: bad arguments");
+  long L = 1, R = 1;
+  for (int i = 0; i < nmodes; ++i) {
+    if (dims[i] <= 0) return set_error(ctx, PLANC_EINVAL, "planc_mttkrp_partial: empty mode");
+    if (i <= split)
+      L *= dims[i];
+    else
+      R *= dims[i];
+  }
+  KrpDesc kd{};
+  if (side == 0) {
+    if (ldo < L) return set_error(ctx, PLANC_EINVAL, "planc_mttkrp_partial: ldo < L");
+    kd.nf = nmodes - 1 - split;
+    for (int q = split + 1; q < nmodes; ++q) {
+      kd.f[q - split - 1] = d_factors[q];
+      kd.dim[q - split - 1] = dims[q];
+      kd.ld[q - split - 1] = dims[q];
+    }
+    return thin_gemm(ctx, dtype, 0, d_X, L, L, R, nullptr, 0, &kd, nullptr, nullptr, k, d_out, ldo, 0, 0);
+  }
+  if (ldo < R) return set_error(ctx, PLANC_EINVAL, "planc_mttkrp_partial: ldo < R");
+  kd.nf = split + 1;
+  for (int q = 0; q <= split; ++q) {
+    kd.f[q] = d_factors[q];
+    kd.dim[q] = dims[q];
+    kd.ld[q] = dims[q];
+  }
+  return thin_gemm(ctx, dtype, 1, d_X, L, R, L, nullptr, 0, &kd, nullptr, nullptr, k, d_out, ldo, 0, 0);
+}
+
+int planc_multi_ttv(planc_ctx *ctx, const float *d_P, long ldp, int nb, const long *dims,
+                    const float *const *d_factors, int target, int k, float *d_out, long ldo) {
+  if (check_ctx(ctx)) return PLANC_EINVAL;
+  if (!d_P || nb < 1 || nb > 7 || !dims || !d_factors || target < 0 || target >= nb || k <= 0 ||
+      ldo < k)
+    return set_error(ctx, PLANC_EINVAL, "planc_multi_ttv: bad arguments");
+  long E = 1, ldf[8];
+  for (int i = 0; i < nb; ++i) {
+    if (dims[i] <= 0) return set_error(ctx, PLANC_EINVAL, "planc_multi_ttv: empty mode");
+    E *= dims[i];
+    ldf[i] = dims[i];
+  }
+  if (ldp < E) return set_error(ctx, PLANC_EINVAL, "planc_multi_ttv: ldp < rows of P");
+  return contract_projection(ctx, d_P, ldp, nb, dims, d_factors, ldf, target, k, d_out, ldo);
+}
+
 // ---------------------------------------------------------------------------
 int planc_nmf_run(planc_ctx *ctx, const planc_nmf_desc *desc, const float *d_A, long lda,
                   const void *d_A16, const float *d_colscale, float *d_W, float *d_H,
