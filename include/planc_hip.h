@@ -177,6 +177,18 @@ int planc_mttkrp_partial(planc_ctx *ctx, int dtype, const void *d_X, int nmodes,
 int planc_multi_ttv(planc_ctx *ctx, const float *d_P, long ldp, int nb, const long *dims,
                     const float *const *d_factors, int target, int k, float *d_out, long ldo);
 
+/* Batched block-principal-pivoting NNLS, FP64 arithmetic: for every column j
+ * solves min ||A x - a_j|| s.t. x >= 0 from the normal equations
+ * G = A^T A (k x k FP64, k <= 64) and b_j = A^T a_j, starting from x = 0 with
+ * the reference's exchange rules, 1e-12 clamps and 5k iteration cap.
+ * Replaces BPPNNLS::solveNNLS (nnls/bppnnls.hpp:23-191) as called per
+ * 2000-column chunk by nmf/bppnmf.hpp:57-89 and ntf/ntfanlsbpp.hpp:22-55.
+ * b_j(i) = d_B[j*b_cs + i*b_rs]; x_j(i) is written to d_X[j*x_cs + i*x_rs].
+ * Synchronises; returns PLANC_ENOCONV when a column exceeded 5k pivots
+ * (the reference calls exit(EXIT_FAILURE), nnls/bppnnls.hpp:185-188). */
+int planc_bpp(planc_ctx *ctx, const double *d_G, int k, const float *d_B, long b_cs, long b_rs,
+              long nrhs, float *d_X, long x_cs, long x_rs);
+
 /* ---- whole loops (computeNMF / computeNTF on the device) ----------------- */
 typedef struct planc_nmf_desc {
   int algo;          /* PLANC_MU, PLANC_HALS, PLANC_ANLSBPP */

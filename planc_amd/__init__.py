@@ -255,6 +255,15 @@ class Context:
                                              _ptr(out), k), "planc_multi_ttv")
         return out
 
+    def bpp(self, G, B):
+        """Batched FP64 BPP NNLS: X (k x nrhs) with X(:,j) = argmin over x >= 0
+        given G = A^T A (k x k FP64) and B = A^T [a_j] (k x nrhs FP32)."""
+        k, nrhs = B.shape
+        X = empty_colmajor((k, nrhs), device=B.device)
+        self._check(self.lib.planc_bpp(self.ctx, _ptr(G), k, _ptr(B), _ld(B), 1, nrhs, _ptr(X), k,
+                                       1), "planc_bpp")
+        return X
+
     # ---- whole loops -------------------------------------------------------
     def nmf_run(self, A, W, H, algo, iters, dtype=F32, regW=(0.0, 0.0), regH=(0.0, 0.0),
                 compute_error=True, A16=None, colscale=None):

@@ -81,6 +81,8 @@ SIGNATURES = {
                                      c_long]),
     "planc_multi_ttv": (c_int, [c_void_p, c_void_p, c_long, c_int, ctypes.POINTER(c_long),
                                 ctypes.POINTER(c_void_p), c_int, c_int, c_void_p, c_long]),
+    "planc_bpp": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_long, c_long, c_long, c_void_p,
+                          c_long, c_long]),
     "planc_nmf_run": (c_int, [c_void_p, ctypes.POINTER(NMFDesc), c_void_p, c_long, c_void_p,
                               c_void_p, c_void_p, c_void_p, ctypes.POINTER(c_double)]),
     "planc_ntf_run": (c_int, [c_void_p, ctypes.POINTER(NTFDesc), c_void_p, c_void_p,

@@ -355,7 +355,7 @@ int planc_nmf_run(planc_ctx *ctx, const planc_nmf_desc *desc, const float *d_A, 
   if (m <= 0 || n <= 0 || k <= 0 || k > 64 || iters < 0 || lda < m || !d_A || !d_W || !d_H ||
       !valid_dtype(dt))
     return set_error(ctx, PLANC_EINVAL, "planc_nmf_run: bad arguments (k must be 1..64)");
-  if (algo != PLANC_MU && algo != PLANC_HALS)
+  if (algo != PLANC_MU && algo != PLANC_HALS && algo != PLANC_ANLSBPP)
     return set_error(ctx, PLANC_EUNSUPPORTED, "planc_nmf_run: algorithm not supported");
   PLANC_HIP_CHECK(ctx, hipSetDevice(ctx->device));
   const void *Aop = d_A;
@@ -393,7 +393,40 @@ int planc_nmf_run(planc_ctx *ctx, const planc_nmf_desc *desc, const float *d_A, 
     rc = (x);          \
     if (rc) return rc; \
   } while (0)
-  if (desc->compute_error) RC(dot(ctx, d_A, lda, d_A, lda, m, n, normA2, 0));
+  if (desc->compute_error) RC(dot(ctx, d_A, 1, lda, d_A, 1, lda, m, n, normA2, 0));
+  if (algo == PLANC_ANLSBPP) {
+    // BPPNMF::computeNMF (nmf/bppnmf.hpp:187-241): W given H first, then H
+    // given W, each a batched NNLS on the FP64 normal equations.
+    int *dflag = (int *)ws_get(ctx, 11, sizeof(int));
+    if (!dflag) return set_error(ctx, PLANC_ENOMEM, "planc_nmf_run: workspace");
+    PLANC_HIP_CHECK(ctx, hipMemsetAsync(dflag, 0, sizeof(int), ctx->stream));
+    for (int it = 0; it < iters; ++it) {
+      RC(gram(ctx, d_H, n, k, n, HtH));
+      PLANC_HIP_CHECK(ctx, hipMemcpyAsync(Greg, HtH, kk * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+      RC(apply_reg(ctx, Greg, k, desc->regW));
+      RC(thin_gemm(ctx, dt, 0, Aop, ldop, m, n, d_H, n, nullptr, cs, nullptr, k, AH, m, 0, 0));
+      RC(bpp_solve(ctx, Greg, k, AH, 1, m, m, d_W, 1, m, dflag));
+      RC(gram(ctx, d_W, m, k, m, WtW));
+      PLANC_HIP_CHECK(ctx, hipMemcpyAsync(Greg, WtW, kk * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+      RC(apply_reg(ctx, Greg, k, desc->regH));
+      RC(thin_gemm(ctx, dt, 1, Aop, ldop, n, m, d_W, m, nullptr, nullptr, cs, k, WtA, k, 1, 0));
+      RC(bpp_solve(ctx, Greg, k, WtA, k, 1, n, d_H, 1, n, dflag));
+      if (desc->compute_error) {
+        // tr(H^T A^T W) = <H, (W^T A)^T> with the W^T A of the H half-step.
+        RC(dot(ctx, d_H, 1, n, WtA, k, 1, n, k, dotv, 0));
+        RC(gram(ctx, d_H, n, k, n, HtH));
+        RC(nmf_relerr(ctx, normA2, dotv, WtW, HtH, k, rel + it, nullptr));
+      }
+    }
+    RC(planc_normalize_by_w(ctx, d_W, m, m, d_H, n, n, k));
+    int hflag = 0;
+    PLANC_HIP_CHECK(ctx, hipMemcpyAsync(&hflag, dflag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    if (h_relerr && desc->compute_error && iters > 0)
+      PLANC_HIP_CHECK(ctx, hipMemcpyAsync(h_relerr, rel, sizeof(double) * iters, hipMemcpyDeviceToHost, ctx->stream));
+    PLANC_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    if (hflag) return set_error(ctx, PLANC_ENOCONV, "BPP failed: more than 5k pivot iterations");
+    return PLANC_OK;
+  }
   if (algo == PLANC_HALS) RC(planc_normalize_by_w(ctx, d_W, m, m, d_H, n, n, k));
   for (int it = 0; it < iters; ++it) {
     // H half-step: W^T A, W^T W (+ regH).
@@ -415,7 +448,7 @@ int planc_nmf_run(planc_ctx *ctx, const planc_nmf_desc *desc, const float *d_A, 
       // <W, A H> with the H of this iteration equals tr(H^T A^T W) of
       // computeObjectiveError (common/nmf.hpp:391-396) without the third
       // product; both traces are invariant under normalize_by_W.
-      RC(dot(ctx, d_W, m, AH, m, m, k, dotv, 0));
+      RC(dot(ctx, d_W, 1, m, AH, 1, m, m, k, dotv, 0));
       RC(gram(ctx, d_W, m, k, m, WtW2));
       RC(nmf_relerr(ctx, normA2, dotv, WtW2, HtH, k, rel + it, nullptr));
     }
@@ -436,7 +469,7 @@ int planc_ntf_run(planc_ctx *ctx, const planc_ntf_desc *desc, const float *d_X, 
   if (N < 2 || N > 7 || k <= 0 || k > 64 || iters < 0 || !d_X || !d_factors || !d_lambda ||
       !valid_dtype(dt))
     return set_error(ctx, PLANC_EINVAL, "planc_ntf_run: bad arguments (2..7 modes, k 1..64)");
-  if (algo != PLANC_MU && algo != PLANC_HALS)
+  if (algo != PLANC_MU && algo != PLANC_HALS && algo != PLANC_ANLSBPP)
     return set_error(ctx, PLANC_EUNSUPPORTED, "planc_ntf_run: algorithm not supported");
   PLANC_HIP_CHECK(ctx, hipSetDevice(ctx->device));
   const long *dims = desc->dims;
@@ -493,7 +526,10 @@ int planc_ntf_run(planc_ctx *ctx, const planc_ntf_desc *desc, const float *d_X, 
     rc = (x);          \
     if (rc) return rc; \
   } while (0)
-  if (desc->compute_error) RC(dot(ctx, d_X, numel, d_X, numel, numel, 1, normX2, 0));
+  int *dflag = (int *)ws_get(ctx, 11, sizeof(int));
+  if (!dflag) return set_error(ctx, PLANC_ENOMEM, "planc_ntf_run: workspace");
+  PLANC_HIP_CHECK(ctx, hipMemsetAsync(dflag, 0, sizeof(int), ctx->stream));
+  if (desc->compute_error) RC(dot(ctx, d_X, 1, numel, d_X, 1, numel, numel, 1, normX2, 0));
   // NCPFactors::normalize() of the initial factors (ntf/auntf.hpp:88).
   for (int i = 0; i < N; ++i) {
     RC(col_norms(ctx, d_factors[i], dims[i], k, dims[i], norms));
@@ -542,8 +578,10 @@ int planc_ntf_run(planc_ctx *ctx, const planc_ntf_desc *desc, const float *d_X, 
       float *Fn = d_factors[n];
       if (algo == PLANC_HALS)
         RC(row_update(ctx, 1, Fn, dims[n], k, dims[n], M, k, 1, S, 0.0f));
-      else
+      else if (algo == PLANC_MU)
         RC(row_update(ctx, 0, Fn, dims[n], k, dims[n], M, k, 1, S, 1e-6f));
+      else
+        RC(bpp_solve(ctx, S, k, M, k, 1, dims[n], Fn, 1, dims[n], dflag));
       // update_factor_mode: normalize(mode) replaces lambda (ntf/auntf.hpp:66-79).
       RC(col_norms(ctx, Fn, dims[n], k, dims[n], d_lambda));
       RC(scale_cols(ctx, Fn, dims[n], dims[n], nullptr, 0, 0, k, d_lambda));
@@ -554,9 +592,29 @@ int planc_ntf_run(planc_ctx *ctx, const planc_ntf_desc *desc, const float *d_X, 
                     Gc.data(), N, rel + it));
   }
 #undef RC
+  int hflag = 0;
+  PLANC_HIP_CHECK(ctx, hipMemcpyAsync(&hflag, dflag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   if (h_relerr && desc->compute_error && iters > 0)
     PLANC_HIP_CHECK(ctx, hipMemcpyAsync(h_relerr, rel, sizeof(double) * iters, hipMemcpyDeviceToHost, ctx->stream));
   PLANC_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  if (hflag) return set_error(ctx, PLANC_ENOCONV, "BPP failed: more than 5k pivot iterations");
+  return PLANC_OK;
+}
+
+int planc_bpp(planc_ctx *ctx, const double *d_G, int k, const float *d_B, long b_cs, long b_rs,
+              long nrhs, float *d_X, long x_cs, long x_rs) {
+  if (check_ctx(ctx)) return PLANC_EINVAL;
+  if (!d_G || !d_B || !d_X || k <= 0 || nrhs < 0)
+    return set_error(ctx, PLANC_EINVAL, "planc_bpp: bad arguments");
+  int *dflag = (int *)ws_get(ctx, 11, sizeof(int));
+  if (!dflag) return set_error(ctx, PLANC_ENOMEM, "planc_bpp: workspace");
+  PLANC_HIP_CHECK(ctx, hipMemsetAsync(dflag, 0, sizeof(int), ctx->stream));
+  int rc = bpp_solve(ctx, d_G, k, d_B, b_cs, b_rs, nrhs, d_X, x_cs, x_rs, dflag);
+  if (rc) return rc;
+  int hflag = 0;
+  PLANC_HIP_CHECK(ctx, hipMemcpyAsync(&hflag, dflag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  PLANC_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  if (hflag) return set_error(ctx, PLANC_ENOCONV, "BPP failed: more than 5k pivot iterations");
   return PLANC_OK;
 }
 

@@ -96,7 +96,8 @@ namespace planc {
 // Returns a device buffer of at least `bytes` for workspace slot `slot`.
 // Slots: 0 split-K partials, 1 Gram partials, 2 column-norm / HALS-sweep
 // partials, 3 dot partials, 4 loop float buffers, 5 loop double buffers,
-// 7 16-bit copy of A / X, 8 column scales of that copy, 9 entry-point norms.
+// 7 16-bit copy of A / X, 8 column scales of that copy, 9 entry-point norms,
+// 10 BPP per-column iteration counts, 11 BPP failure flag.
 void *ws_get(planc_ctx *ctx, int slot, size_t bytes);
 
 int set_error(planc_ctx *ctx, int code, const std::string &msg);
@@ -140,13 +141,18 @@ int hals_cols(planc_ctx *ctx, float *W, long m, int k, long ldw, const float *AH
 int col_norms(planc_ctx *ctx, const float *X, long rows, int k, long ldx, double *norms);
 int scale_cols(planc_ctx *ctx, float *X, long rows, long ldx, float *Y, long yrows, long ldy, int k,
                const double *norms);
-int dot(planc_ctx *ctx, const float *X, long ldx, const float *Y, long ldy, long rows, long cols,
-        double *out, int accumulate);
+int dot(planc_ctx *ctx, const float *X, long xrs, long xcs, const float *Y, long yrs, long ycs,
+        long rows, long cols, double *out, int accumulate);
 int nmf_relerr(planc_ctx *ctx, const double *normA2, const double *dotv, const double *WtW,
                const double *HtH, int k, double *relerr_out, double *fit_out);
 int apply_reg(planc_ctx *ctx, double *G, int k, const double *reg);
 int convert16(planc_ctx *ctx, int dtype, const float *A, long rows, long cols, long lda, void *out,
               long ldo, float *colscale);
+
+// bpp.hip: column j of the right-hand side is B[j*b_cs + i*b_rs]; the
+// solution goes to X[j*x_cs + i*x_rs]; *d_failed |= (some column > 5k pivots).
+int bpp_solve(planc_ctx *ctx, const double *G, int k, const float *B, long b_cs, long b_rs, long nrhs,
+              float *X, long x_cs, long x_rs, int *d_failed);
 
 // ntf_kernels.hip
 int contract_projection(planc_ctx *ctx, const float *P, long ldp, int nb, const long *d,

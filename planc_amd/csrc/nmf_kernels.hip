@@ -319,15 +319,15 @@ __global__ void scale_cols_kernel(float *X, long rows, long ldx, float *Y, long 
 // ---------------------------------------------------------------------------
 // FP64 reductions: sum of X .* Y over a rows x cols pair of column-major
 // matrices (Y may equal X for a sum of squares).  Two stages, fixed order.
-__global__ __launch_bounds__(NTHR) void dot_partial_kernel(const float *X, long ldx, const float *Y,
-                                                           long ldy, long rows, long cols,
-                                                           double *part) {
+__global__ __launch_bounds__(NTHR) void dot_partial_kernel(const float *X, long xrs, long xcs,
+                                                           const float *Y, long yrs, long ycs,
+                                                           long rows, long cols, double *part) {
   __shared__ double red[NTHR / 64];
   long tot = rows * cols;
   double s = 0.0;
   for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (long)gridDim.x * blockDim.x) {
     long r = e % rows, c = e / rows;
-    s += (double)X[r + ldx * c] * (double)Y[r + ldy * c];
+    s += (double)X[r * xrs + xcs * c] * (double)Y[r * yrs + ycs * c];
   }
   double bs = block_sum(s, red);
   if (threadIdx.x == 0) part[blockIdx.x] = bs;
@@ -532,16 +532,18 @@ int scale_cols(planc_ctx *ctx, float *X, long rows, long ldx, float *Y, long yro
   return PLANC_OK;
 }
 
-// *out = sum(X .* Y) in FP64 (accumulate: *out += ...).
-int dot(planc_ctx *ctx, const float *X, long ldx, const float *Y, long ldy, long rows, long cols,
-        double *out, int accumulate) {
+// *out = sum_{r,c} X(r,c) Y(r,c) in FP64 (accumulate: *out += ...), with
+// X(r,c) = X[r*xrs + c*xcs] and likewise Y.
+int dot(planc_ctx *ctx, const float *X, long xrs, long xcs, const float *Y, long yrs, long ycs,
+        long rows, long cols, double *out, int accumulate) {
   long tot = rows * cols;
   long nb = (tot + NTHR * 8 - 1) / (NTHR * 8);
   if (nb > 2048) nb = 2048;
   if (nb < 1) nb = 1;
   double *part = (double *)ws_get(ctx, 3, sizeof(double) * (size_t)nb);
   if (!part) return set_error(ctx, PLANC_ENOMEM, "dot: workspace");
-  dot_partial_kernel<<<(unsigned)nb, NTHR, 0, ctx->stream>>>(X, ldx, Y, ldy, rows, cols, part);
+  dot_partial_kernel<<<(unsigned)nb, NTHR, 0, ctx->stream>>>(X, xrs, xcs, Y, yrs, ycs, rows, cols,
+                                                             part);
   PLANC_LAUNCH_CHECK(ctx);
   sum_partials_kernel<<<1, 64, 0, ctx->stream>>>(part, nb, out, accumulate);
   PLANC_LAUNCH_CHECK(ctx);

@@ -289,3 +289,78 @@ def test_ntf_config2_shape(ctx):
     lam, rel = ctx.ntf_run(dX, dF, P.HALS, 5)
     say(f"  gpu {rel} oracle {rel_r}")
     assert np.abs(rel - rel_r).max() < 1e-5
+
+
+@pytest.mark.parametrize("k,nrhs", [(1, 7), (8, 12), (20, 3000), (33, 500), (64, 700)])
+def test_bpp_batched_matches_oracle(ctx, k, nrhs):
+    """Batched FP64 BPP vs the oracle (itself pinned to scipy's NNLS): the
+    same active sets and solutions to FP32 output rounding."""
+    say(f"bpp k={k} nrhs={nrhs}")
+    rng = np.random.default_rng(k * 100 + nrhs)
+    Wt = rng.random((3 * k + 5, k))
+    Bm = rng.standard_normal((3 * k + 5, nrhs)) + 0.3
+    G = Wt.T @ Wt
+    B = (Wt.T @ Bm).astype(np.float32)
+    Xr, it = O.bppnnls(G, B.astype(np.float64))
+    assert it > 0
+    X = ctx.bpp(P.colmajor(G), P.colmajor(B)).cpu().numpy()
+    assert np.array_equal(X > 0, Xr > 0)
+    assert rel_err(X, Xr) < 1e-6
+
+
+def test_bpp_golden_nnls_fixture(ctx):
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "nnls.npz"))
+    X = ctx.bpp(P.colmajor(g["AtA"]), P.colmajor(g["AtB"].astype(np.float32))).cpu().numpy()
+    assert rel_err(X, g["X"]) < 1e-6
+
+
+def test_nmf_bpp_trajectory(ctx):
+    say("nmf bpp")
+    A, W0, H0 = _nmf_case(600, 400, 10, stream=12)
+    Wr, Hr, rel_r, _, rc = O.nmf(2, A.astype(np.float64), W0.astype(np.float64),
+                                 H0.astype(np.float64), 5)
+    assert rc == 0
+    dA, dW, dH = P.colmajor(A), P.colmajor(W0), P.colmajor(H0)
+    rel = ctx.nmf_run(dA, dW, dH, P.ANLSBPP, 5)
+    say(f"  gpu {rel} oracle {rel_r}")
+    assert np.abs(rel - rel_r).max() < 1e-5
+    assert rel_err(dW.cpu().numpy(), Wr) < 1e-4
+    assert rel_err(dH.cpu().numpy(), Hr) < 1e-4
+
+
+@pytest.mark.parametrize("dimtree", [True, False])
+def test_ntf_bpp_trajectory(ctx, dimtree):
+    say(f"ntf bpp dimtree={dimtree}")
+    dims, k = (20, 18, 16), 5
+    X, _ = D.cp_tensor(dims, k, 0.01, stream=3)
+    X32 = X.astype(np.float32)
+    F0 = [f.astype(np.float32) for f in D.init_factors([(d, k) for d in dims], stream=3)]
+    Fr, lam_r, rel_r, rc = O.ntf(2, X32.astype(np.float64), [f.astype(np.float64) for f in F0], 8,
+                                 dimtree=1 if dimtree else 0)
+    assert rc == 0
+    dF = [P.colmajor(f) for f in F0]
+    lam, rel = ctx.ntf_run(P.colmajor(X32), dF, P.ANLSBPP, 8, dimtree=dimtree)
+    say(f"  gpu {rel[-1]:.8f} oracle {rel_r[-1]:.8f}")
+    assert np.abs(rel - rel_r).max() < 2e-5
+
+
+def test_partial_mttkrp_and_multi_ttv(ctx):
+    """Dimension-tree pieces through the C ABI: the right/left partial
+    contractions and the multi-TTV reproduce every mode's MTTKRP."""
+    dims, k = (9, 10, 11, 12), 6
+    rng = np.random.default_rng(77)
+    X = np.asfortranarray(rng.random(dims, dtype=np.float32))
+    F = [rng.random((d, k), dtype=np.float32) for d in dims]
+    dX, dF = P.colmajor(X), [P.colmajor(f) for f in F]
+    Fd = [f.astype(np.float64) for f in F]
+    for s in range(3):
+        Pr = ctx.mttkrp_partial(dX, dF, s, 0)
+        Qr = ctx.mttkrp_partial(dX, dF, s, 1)
+        for n in range(4):
+            ref = O.mttkrp(X.astype(np.float64), Fd, n)
+            if n <= s:
+                M = ctx.multi_ttv(Pr, dF[:s + 1], n)
+            else:
+                M = ctx.multi_ttv(Qr, dF[s + 1:], n - s - 1)
+            assert rel_err(M.cpu().numpy(), ref) < 1e-5, (s, n)
