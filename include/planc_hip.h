@@ -189,6 +189,24 @@ int planc_multi_ttv(planc_ctx *ctx, const float *d_P, long ldp, int nb, const lo
 int planc_bpp(planc_ctx *ctx, const double *d_G, int k, const float *d_B, long b_cs, long b_rs,
               long nrhs, float *d_X, long x_cs, long x_rs);
 
+/* CP reconstruction loss 0.5 ||X - [[A_0..A_{N-1}]]||^2 (no lambda), FP64
+ * result in *h_loss: one fused kernel reconstructs tiles of X from
+ * Khatri-Rao tiles formed in LDS and reduces the squared residual, reading X
+ * once.  The reference evaluates the same residual by forming the full
+ * reconstruction (AUNTF::computeObjectiveError, ntf/auntf.hpp:200-246).
+ * Synchronises. */
+int planc_cp_loss_fwd(planc_ctx *ctx, const float *d_X, int nmodes, const long *dims,
+                      const float *const *d_factors, int k, double *h_loss);
+
+/* Gradient of that loss with respect to every factor:
+ * d_grads[n] (I_n x k) = A_n * (Hadamard of the other Grams) - MTTKRP_n^T,
+ * the transposed NTFNES gradient S*Y - M (ntf/ntfnes.hpp:138-147) without
+ * its lambda shift; the MTTKRPs come from the two partial contractions of
+ * the dimension tree with the KRP recomputed, never stored.  3..7 modes.
+ * Synchronises. */
+int planc_cp_loss_bwd(planc_ctx *ctx, const float *d_X, int nmodes, const long *dims,
+                      const float *const *d_factors, int k, float *const *d_grads);
+
 /* ---- whole loops (computeNMF / computeNTF on the device) ----------------- */
 typedef struct planc_nmf_desc {
   int algo;          /* PLANC_MU, PLANC_HALS, PLANC_ANLSBPP */

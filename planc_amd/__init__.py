@@ -264,6 +264,27 @@ class Context:
                                        1), "planc_bpp")
         return X
 
+    def cp_loss(self, X, factors):
+        """0.5 ||X - [[A_0..A_{N-1}]]||^2 (FP64 float), fused on the device."""
+        N = len(factors)
+        cd = (ctypes.c_long * N)(*[f.shape[0] for f in factors])
+        arr = (ctypes.c_void_p * N)(*[f.data_ptr() for f in factors])
+        out = ctypes.c_double()
+        self._check(self.lib.planc_cp_loss_fwd(self.ctx, _ptr(X), N, cd, arr, factors[0].shape[1],
+                                               ctypes.byref(out)), "planc_cp_loss_fwd")
+        return out.value
+
+    def cp_loss_grad(self, X, factors):
+        """[A_n S_n - MTTKRP_n^T] for every mode (I_n x k, column-major)."""
+        N = len(factors)
+        grads = [empty_colmajor(tuple(f.shape), device=f.device) for f in factors]
+        cd = (ctypes.c_long * N)(*[f.shape[0] for f in factors])
+        arr = (ctypes.c_void_p * N)(*[f.data_ptr() for f in factors])
+        garr = (ctypes.c_void_p * N)(*[g.data_ptr() for g in grads])
+        self._check(self.lib.planc_cp_loss_bwd(self.ctx, _ptr(X), N, cd, arr, factors[0].shape[1],
+                                               garr), "planc_cp_loss_bwd")
+        return grads
+
     # ---- whole loops -------------------------------------------------------
     def nmf_run(self, A, W, H, algo, iters, dtype=F32, regW=(0.0, 0.0), regH=(0.0, 0.0),
                 compute_error=True, A16=None, colscale=None):

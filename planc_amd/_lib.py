@@ -81,6 +81,10 @@ SIGNATURES = {
                                      c_long]),
     "planc_multi_ttv": (c_int, [c_void_p, c_void_p, c_long, c_int, ctypes.POINTER(c_long),
                                 ctypes.POINTER(c_void_p), c_int, c_int, c_void_p, c_long]),
+    "planc_cp_loss_fwd": (c_int, [c_void_p, c_void_p, c_int, ctypes.POINTER(c_long),
+                                  ctypes.POINTER(c_void_p), c_int, ctypes.POINTER(c_double)]),
+    "planc_cp_loss_bwd": (c_int, [c_void_p, c_void_p, c_int, ctypes.POINTER(c_long),
+                                  ctypes.POINTER(c_void_p), c_int, ctypes.POINTER(c_void_p)]),
     "planc_bpp": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_long, c_long, c_long, c_void_p,
                           c_long, c_long]),
     "planc_nmf_run": (c_int, [c_void_p, ctypes.POINTER(NMFDesc), c_void_p, c_long, c_void_p,

@@ -601,6 +601,85 @@ int planc_ntf_run(planc_ctx *ctx, const planc_ntf_desc *desc, const float *d_X, 
   return PLANC_OK;
 }
 
+int planc_cp_loss_fwd(planc_ctx *ctx, const float *d_X, int nmodes, const long *dims,
+                      const float *const *d_factors, int k, double *h_loss) {
+  if (check_ctx(ctx)) return PLANC_EINVAL;
+  if (!d_X || !dims || !d_factors || !h_loss || nmodes < 2 || nmodes > 7 || k <= 0 || k > 64)
+    return set_error(ctx, PLANC_EINVAL, "planc_cp_loss_fwd: bad arguments");
+  for (int i = 0; i < nmodes; ++i)
+    if (dims[i] <= 0) return set_error(ctx, PLANC_EINVAL, "planc_cp_loss_fwd: empty mode");
+  double *dl = (double *)ws_get(ctx, 12, sizeof(double));
+  if (!dl) return set_error(ctx, PLANC_ENOMEM, "planc_cp_loss_fwd: workspace");
+  int rc = cp_loss_forward(ctx, d_X, nmodes, dims, d_factors, k, dl);
+  if (rc) return rc;
+  PLANC_HIP_CHECK(ctx, hipMemcpyAsync(h_loss, dl, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  PLANC_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return PLANC_OK;
+}
+
+int planc_cp_loss_bwd(planc_ctx *ctx, const float *d_X, int nmodes, const long *dims,
+                      const float *const *d_factors, int k, float *const *d_grads) {
+  if (check_ctx(ctx)) return PLANC_EINVAL;
+  const int N = nmodes;
+  if (!d_X || !dims || !d_factors || !d_grads || N < 3 || N > 7 || k <= 0 || k > 64)
+    return set_error(ctx, PLANC_EINVAL, "planc_cp_loss_bwd: bad arguments (3..7 modes, k 1..64)");
+  long Imax = 0;
+  for (int i = 0; i < N; ++i) {
+    if (dims[i] <= 0) return set_error(ctx, PLANC_EINVAL, "planc_cp_loss_bwd: empty mode");
+    Imax = std::max(Imax, dims[i]);
+  }
+  const int s = cp_split(N, dims);
+  long L = 1, R = 1;
+  for (int i = 0; i <= s; ++i) L *= dims[i];
+  for (int i = s + 1; i < N; ++i) R *= dims[i];
+  float *fb = (float *)ws_get(ctx, 13, sizeof(float) * ((size_t)(L + R) * k + (size_t)k * Imax));
+  const size_t kk = (size_t)k * k;
+  double *db = (double *)ws_get(ctx, 14, sizeof(double) * (N + 1) * kk);
+  if (!fb || !db) return set_error(ctx, PLANC_ENOMEM, "planc_cp_loss_bwd: workspace");
+  float *P = fb, *Q = fb + (size_t)L * k, *M = fb + (size_t)(L + R) * k;
+  std::vector<const double *> G(N);
+  double *S = db + N * kk;
+  int rc;
+  for (int i = 0; i < N; ++i) {
+    G[i] = db + i * kk;
+    rc = gram(ctx, d_factors[i], dims[i], k, dims[i], db + i * kk);
+    if (rc) return rc;
+  }
+  KrpDesc kr{}, kl{};
+  kr.nf = N - 1 - s;
+  for (int q = s + 1; q < N; ++q) {
+    kr.f[q - s - 1] = d_factors[q];
+    kr.dim[q - s - 1] = dims[q];
+    kr.ld[q - s - 1] = dims[q];
+  }
+  kl.nf = s + 1;
+  for (int q = 0; q <= s; ++q) {
+    kl.f[q] = d_factors[q];
+    kl.dim[q] = dims[q];
+    kl.ld[q] = dims[q];
+  }
+  rc = thin_gemm(ctx, PLANC_F32, 0, d_X, L, L, R, nullptr, 0, &kr, nullptr, nullptr, k, P, L, 0, 0);
+  if (rc) return rc;
+  rc = thin_gemm(ctx, PLANC_F32, 1, d_X, L, R, L, nullptr, 0, &kl, nullptr, nullptr, k, Q, R, 0, 0);
+  if (rc) return rc;
+  long ldf[8];
+  for (int i = 0; i < N; ++i) ldf[i] = dims[i];
+  for (int n = 0; n < N; ++n) {
+    if (n <= s)
+      rc = contract_projection(ctx, P, L, s + 1, dims, d_factors, ldf, n, k, M, k);
+    else
+      rc = contract_projection(ctx, Q, R, N - 1 - s, dims + s + 1, d_factors + s + 1, ldf + s + 1,
+                               n - s - 1, k, M, k);
+    if (rc) return rc;
+    rc = hadamard_grams(ctx, G.data(), N, n, k, S);
+    if (rc) return rc;
+    rc = cp_grad_from_mttkrp(ctx, d_factors[n], dims[n], k, S, M, d_grads[n]);
+    if (rc) return rc;
+  }
+  PLANC_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return PLANC_OK;
+}
+
 int planc_bpp(planc_ctx *ctx, const double *d_G, int k, const float *d_B, long b_cs, long b_rs,
               long nrhs, float *d_X, long x_cs, long x_rs) {
   if (check_ctx(ctx)) return PLANC_EINVAL;

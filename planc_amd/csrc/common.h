@@ -86,7 +86,7 @@ struct planc_ctx {
   hipStream_t stream = nullptr;
   bool own_stream = false;
   std::string last_error;
-  planc::Workspace ws[12];
+  planc::Workspace ws[16];
   int num_cu = 256;
   int max_coop_blocks_hals = 0;
 };
@@ -97,7 +97,8 @@ namespace planc {
 // Slots: 0 split-K partials, 1 Gram partials, 2 column-norm / HALS-sweep
 // partials, 3 dot partials, 4 loop float buffers, 5 loop double buffers,
 // 7 16-bit copy of A / X, 8 column scales of that copy, 9 entry-point norms,
-// 10 BPP per-column iteration counts, 11 BPP failure flag.
+// 10 BPP per-column iteration counts, 11 BPP failure flag, 12 CP-loss scalar,
+// 13 CP-gradient float buffers, 14 CP-gradient Grams.
 void *ws_get(planc_ctx *ctx, int slot, size_t bytes);
 
 int set_error(planc_ctx *ctx, int code, const std::string &msg);
@@ -153,6 +154,13 @@ int convert16(planc_ctx *ctx, int dtype, const float *A, long rows, long cols, l
 // solution goes to X[j*x_cs + i*x_rs]; *d_failed |= (some column > 5k pivots).
 int bpp_solve(planc_ctx *ctx, const double *G, int k, const float *B, long b_cs, long b_rs, long nrhs,
               float *X, long x_cs, long x_rs, int *d_failed);
+
+// cploss.hip
+int cp_loss_forward(planc_ctx *ctx, const float *X, int N, const long *dims, const float *const *F,
+                    int k, double *d_loss);
+int cp_grad_from_mttkrp(planc_ctx *ctx, const float *A, long I, int k, const double *S, const float *M,
+                        float *grad);
+int cp_split(int N, const long *dims);
 
 // ntf_kernels.hip
 int contract_projection(planc_ctx *ctx, const float *P, long ldp, int nb, const long *d,

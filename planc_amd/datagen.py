@@ -62,3 +62,12 @@ def cp_tensor(dims, r, noise_rel, stream):
     X = np.einsum(",".join(s + "z" for s in subs) + "->" + subs, *F, optimize=True)
     X = X + noise_rel * X.mean() * np.abs(normal(NOISE_SEED, stream, dims))
     return np.asfortranarray(X), F
+
+
+def gamma2(seed, stream, shape, scale=0.5):
+    """Gamma(shape 2, scale) = -scale * (ln U1 + ln U2), from words 2e, 2e+1."""
+    n = int(np.prod(shape))
+    w = words(seed, stream, 2 * n)
+    u = 1.0 - (w >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    g = -scale * (np.log(u[0::2]) + np.log(u[1::2]))
+    return g.reshape(shape, order="F")

@@ -364,3 +364,89 @@ def test_partial_mttkrp_and_multi_ttv(ctx):
             else:
                 M = ctx.multi_ttv(Qr, dF[s + 1:], n - s - 1)
             assert rel_err(M.cpu().numpy(), ref) < 1e-5, (s, n)
+
+
+def _cp_case(dims, k, stream, gamma=False):
+    if gamma:
+        F = [D.gamma2(D.DATA_SEED, 16 * stream + i, (d, k)) for i, d in enumerate(dims)]
+    else:
+        F = [D.uniform(D.DATA_SEED, 16 * stream + i, (d, k)) for i, d in enumerate(dims)]
+    subs = "abcdefgh"[:len(dims)]
+    X = np.einsum(",".join(s + "z" for s in subs) + "->" + subs, *F, optimize=True)
+    X = X + 0.01 * X.mean() * np.abs(D.normal(D.NOISE_SEED, stream, dims))
+    G = [D.gamma2(D.INIT_SEED, 16 * stream + i, (d, k)) if gamma else
+         D.uniform(D.INIT_SEED, 16 * stream + i, (d, k)) for i, d in enumerate(dims)]
+    return (np.asfortranarray(X.astype(np.float32)), [g.astype(np.float32) for g in G])
+
+
+@pytest.mark.parametrize("dims,k", [((7, 8, 9), 3), ((12, 10, 9, 8), 5), ((30, 20, 25), 48),
+                                    ((5, 6, 7, 4, 3), 4)])
+def test_cp_loss_forward_backward(ctx, dims, k):
+    """Fused forward vs the FP64 oracle loss; backward vs the oracle's
+    NTFNES-form gradient (relative 1e-5 / 1e-4)."""
+    say(f"cp loss dims={dims} k={k}")
+    X, F = _cp_case(dims, k, stream=4)
+    Fd = [f.astype(np.float64) for f in F]
+    loss_r, grads_r = O.cp_loss(X.astype(np.float64), Fd, grads=True)
+    dX, dF = P.colmajor(X), [P.colmajor(f) for f in F]
+    loss = ctx.cp_loss(dX, dF)
+    say(f"  loss gpu {loss:.10e} oracle {loss_r:.10e}")
+    assert abs(loss - loss_r) / loss_r < 1e-5
+    grads = ctx.cp_loss_grad(dX, dF)
+    for g, gr in zip(grads, grads_r):
+        assert rel_err(g.cpu().numpy(), gr) < 1e-4
+
+
+def test_cp_loss_gradient_central_differences(ctx):
+    """GPU gradient against FP64 central differences (h = 1e-6) of the loss."""
+    dims, k = (6, 5, 4), 3
+    X, F = _cp_case(dims, k, stream=6)
+    Fd = [f.astype(np.float64) for f in F]
+    grads = ctx.cp_loss_grad(P.colmajor(X), [P.colmajor(f) for f in F])
+    h = 1e-6
+    for n in range(3):
+        g = grads[n].cpu().numpy()
+        for (i, c) in [(0, 0), (dims[n] - 1, k - 1), (2, 1)]:
+            Fp = [f.copy() for f in Fd]
+            Fm = [f.copy() for f in Fd]
+            Fp[n][i, c] += h
+            Fm[n][i, c] -= h
+            fd = (O.cp_loss(X.astype(np.float64), Fp) - O.cp_loss(X.astype(np.float64), Fm)) / (2 * h)
+            assert abs(g[i, c] - fd) <= 1e-4 * max(1.0, abs(fd)), (n, i, c, g[i, c], fd)
+
+
+def test_cp_loss_autograd_function(ctx):
+    from planc_amd.autograd import cp_loss
+    dims, k = (9, 8, 7, 6), 4
+    X, F = _cp_case(dims, k, stream=7)
+    dX = P.colmajor(X)
+    dF = [P.colmajor(f).requires_grad_(True) for f in F]
+    loss = cp_loss(dX, dF)
+    (2.0 * loss).backward()
+    _, grads_r = O.cp_loss(X.astype(np.float64), [f.astype(np.float64) for f in F], grads=True)
+    for f, gr in zip(dF, grads_r):
+        assert rel_err(f.grad.cpu().numpy(), 2.0 * gr) < 1e-4
+
+
+def test_cp_loss_config4_96_4way(ctx):
+    """Config 4: 96^4 FP32, rank 48, Gamma(2, 0.5) factors, 1% noise: fused
+    forward and the gradient against the FP64 oracle at full size."""
+    say("cp loss 96^4 k=48")
+    dims, k = (96, 96, 96, 96), 48
+    Fg = [ctx.gen_uniform((d, k), D.DATA_SEED, 16 * 8 + i) for i, d in enumerate(dims)]
+    Fg = [P.colmajor(D.gamma2(D.DATA_SEED, 16 * 8 + i, (d, k)).astype(np.float32))
+          for i, d in enumerate(dims)]
+    mean = float(np.prod([f.cpu().numpy().astype(np.float64).mean(0) for f in Fg], axis=0).sum())
+    dX = ctx.gen_cp_tensor(Fg, 0.01 * mean, D.NOISE_SEED, 8)
+    F = [P.colmajor(D.gamma2(D.INIT_SEED, 16 * 8 + i, (d, k)).astype(np.float32))
+         for i, d in enumerate(dims)]
+    loss = ctx.cp_loss(dX, F)
+    grads = ctx.cp_loss_grad(dX, F)
+    Xh = dX.cpu().numpy().astype(np.float64)
+    Fh = [f.cpu().numpy().astype(np.float64) for f in F]
+    O.set_threads(16)
+    loss_r, grads_r = O.cp_loss(Xh, Fh, grads=True)
+    say(f"  loss gpu {loss:.8e} oracle {loss_r:.8e}")
+    assert abs(loss - loss_r) / loss_r < 1e-5
+    for g, gr in zip(grads, grads_r):
+        assert rel_err(g.cpu().numpy(), gr) < 1e-4
