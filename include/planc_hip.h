@@ -207,6 +207,52 @@ int planc_cp_loss_fwd(planc_ctx *ctx, const float *d_X, int nmodes, const long *
 int planc_cp_loss_bwd(planc_ctx *ctx, const float *d_X, int nmodes, const long *dims,
                       const float *const *d_factors, int k, float *const *d_grads);
 
+/* Column subset of a CSR matrix (m x n): the node matrix A0(:, cols) of
+ * hiernmf (Node::allocate, the sparse selection A0 * S, hiernmf/node.hpp:
+ * 46-64).  h_cols: sorted, unique column ids (host).  Rows are kept, columns
+ * renumbered 0..ncols-1, entry order and values unchanged (bit-exact).
+ * d_out_indptr has m + 1 entries; the outputs must hold *h_out_nnz entries
+ * (call once with null outputs to get the count).  Synchronises. */
+int planc_csr_col_subset(planc_ctx *ctx, long m, long n, const long *d_indptr, const int *d_indices,
+                         const float *d_data, const long *h_cols, long ncols, long *d_out_indptr,
+                         int *d_out_indices, float *d_out_data, long *h_out_nnz);
+
+/* Hierarchical rank-2 NMF over a complete tree of `depth` levels
+ * (HierNMFDriver::buildTree, hiernmf/hiernmf.cpp:65-184, with a level-by-level
+ * schedule instead of the score-ordered frontier): every node of a level runs
+ * `iters` iterations of the rank-2 NMF of Node::split (hiernmf/node.hpp:
+ * 104-188; DistAUNMF order H then W, DistR2 update) in one batched, hipGraph-
+ * captured launch sequence; its columns split by H(:,0) >= H(:,1), the larger
+ * side becoming child 2i+1; every child gets sigma by an `iters`-step power
+ * iteration (Node::compute_sigma).  Node seeds: initseed * index (rank 0);
+ * W from stream 0, H from stream 1, the power-iteration start from normals
+ * of stream 2 (Philox).  A is a host CSR (m x n, FP32 values, sorted column
+ * indices).  Outputs (host): h_nodes (capacity 2^depth - 1) in level order,
+ * h_cols (capacity n * depth) each node's sorted columns at its col_offset,
+ * optional h_H (2 doubles per entry of h_cols: the node's final H rows) and
+ * h_W (2*m doubles per node, indexed by node index; zero rows where the
+ * node's matrix is empty). */
+typedef struct planc_hier_desc {
+  long m, n;
+  int depth;     /* levels, root = level 0 */
+  int iters;     /* NMF and power iterations per node (-t) */
+  int initseed;  /* --seed */
+} planc_hier_desc;
+
+typedef struct planc_hier_node {
+  long index;      /* 0 root, children 2i+1 / 2i+2 */
+  int level;
+  long ncols;
+  long col_offset; /* into h_cols / h_H */
+  double sigma;    /* 0 for the root (RootNode) */
+  double score;    /* Node::compute_score, hiernmf/node.hpp:71-81 */
+  int lvalid, rvalid;
+} planc_hier_node;
+
+int planc_hiernmf_run(planc_ctx *ctx, const planc_hier_desc *desc, const long *h_indptr,
+                      const int *h_indices, const float *h_data, planc_hier_node *h_nodes,
+                      int *h_num_nodes, long *h_cols, double *h_H, double *h_W);
+
 /* ---- whole loops (computeNMF / computeNTF on the device) ----------------- */
 typedef struct planc_nmf_desc {
   int algo;          /* PLANC_MU, PLANC_HALS, PLANC_ANLSBPP */

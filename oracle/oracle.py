@@ -29,6 +29,11 @@ How each oracle function is pinned (tests/test_oracle.py):
                          reconstruction.
 * r2_update           -- brute force over the four active sets of the
                          two-variable NNLS.
+* r2_nmf_csr          -- dense NumPy loop of the same DistAUNMF order on a tiny
+                         node; every H/W step a KKT point of its 2-variable
+                         NNLS (r2_update pinned above).
+* powiter_csr         -- numpy.linalg.svd: sigma_max(A_node)^2 on a matrix
+                         with a clear spectral gap.
 """
 import ctypes
 import os
@@ -69,6 +74,13 @@ def lib():
                                      ctypes.POINTER(_dp), ctypes.c_int, ctypes.POINTER(_dp)]
         L.oracle_cp_loss.restype = ctypes.c_double
         L.oracle_r2_update.argtypes = [_dp, _dp, ctypes.c_long, _dp]
+        _lp = ctypes.POINTER(ctypes.c_long)
+        _ip = ctypes.POINTER(ctypes.c_int)
+        L.oracle_r2_nmf_csr.argtypes = [ctypes.c_long, _lp, _ip, _dp, _lp, ctypes.c_long,
+                                        ctypes.c_long, _dp, _dp, ctypes.c_int]
+        L.oracle_powiter_csr.argtypes = [ctypes.c_long, _lp, _ip, _dp, _lp, ctypes.c_long,
+                                         ctypes.c_long, _dp, ctypes.c_int]
+        L.oracle_powiter_csr.restype = ctypes.c_double
         L.oracle_set_threads.argtypes = [ctypes.c_int]
         L.oracle_get_threads.restype = ctypes.c_int
         _LIB = L
@@ -196,3 +208,32 @@ def r2_update(left, right):
     out = np.zeros_like(right, order="F")
     lib().oracle_r2_update(_p(left), _p(right), right.shape[1], _p(out))
     return out
+
+
+def _csr_args(indptr, indices, data):
+    ip = np.ascontiguousarray(indptr, dtype=np.int64)
+    ix = np.ascontiguousarray(indices, dtype=np.int32)
+    dv = np.ascontiguousarray(data, dtype=np.float64)
+    return ip, ix, dv
+
+
+def r2_nmf_csr(m, n, indptr, indices, data, cols, W0, H0, iters):
+    """Rank-2 NMF of the node A(:, cols) from W0 (m x 2), H0 (ncols x 2)."""
+    ip, ix, dv = _csr_args(indptr, indices, data)
+    c = np.ascontiguousarray(cols, dtype=np.int64)
+    W = _f64(W0).copy(order="F")
+    H = _f64(H0).copy(order="F")
+    lp = ctypes.POINTER(ctypes.c_long)
+    lib().oracle_r2_nmf_csr(m, ip.ctypes.data_as(lp), ix.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                            _p(dv), c.ctypes.data_as(lp), len(c), n, _p(W), _p(H), iters)
+    return W, H
+
+
+def powiter_csr(m, n, indptr, indices, data, cols, q0, iters):
+    ip, ix, dv = _csr_args(indptr, indices, data)
+    c = np.ascontiguousarray(cols, dtype=np.int64)
+    q = np.ascontiguousarray(q0, dtype=np.float64)
+    lp = ctypes.POINTER(ctypes.c_long)
+    return lib().oracle_powiter_csr(m, ip.ctypes.data_as(lp),
+                                    ix.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _p(dv),
+                                    c.ctypes.data_as(lp), len(c), n, _p(q), iters)

@@ -1032,4 +1032,97 @@ void oracle_r2_update(const double *left, const double *right, long n, double *o
   }
 }
 
+// Rank-2 NMF of one hierarchical node (Node::split, hiernmf/node.hpp:104-188):
+// DistAUNMF::computeNMF at p = 1 (distnmf/aunmf.hpp:495-657), H given W then
+// W given H with DistR2's update, on A_node = A0(:, cols) (all m rows kept, as
+// Node::allocate does).  A0 is CSR (m x n); W (m x 2) and H (ncols x 2) are
+// column-major and overwritten.
+void oracle_r2_nmf_csr(long m, const long *indptr, const int *indices, const double *data,
+                       const long *cols, long ncols, long n, double *W, double *H, int iters) {
+  std::vector<long> colmap(n, -1);
+  for (long j = 0; j < ncols; ++j) colmap[cols[j]] = j;
+  std::vector<double> WtA(2 * ncols), AH(2 * m), G(4), out;
+  for (int it = 0; it < iters; ++it) {
+    double g00 = 0, g01 = 0, g11 = 0;
+    for (long i = 0; i < m; ++i) {
+      double a = W[i], b = W[i + m];
+      g00 += a * a;
+      g01 += a * b;
+      g11 += b * b;
+    }
+    G = {g00, g01, g01, g11};
+    std::fill(WtA.begin(), WtA.end(), 0.0);
+    for (long i = 0; i < m; ++i)
+      for (long e = indptr[i]; e < indptr[i + 1]; ++e) {
+        long j = colmap[indices[e]];
+        if (j < 0) continue;
+        WtA[2 * j] += W[i] * data[e];
+        WtA[2 * j + 1] += W[i + m] * data[e];
+      }
+    out.assign(2 * ncols, 0.0);
+    oracle_r2_update(G.data(), WtA.data(), ncols, out.data());
+    for (long j = 0; j < ncols; ++j) {
+      H[j] = out[2 * j];
+      H[j + ncols] = out[2 * j + 1];
+    }
+    g00 = g01 = g11 = 0;
+    for (long j = 0; j < ncols; ++j) {
+      double a = H[j], b = H[j + ncols];
+      g00 += a * a;
+      g01 += a * b;
+      g11 += b * b;
+    }
+    G = {g00, g01, g01, g11};
+    std::fill(AH.begin(), AH.end(), 0.0);
+    for (long i = 0; i < m; ++i)
+      for (long e = indptr[i]; e < indptr[i + 1]; ++e) {
+        long j = colmap[indices[e]];
+        if (j < 0) continue;
+        AH[2 * i] += data[e] * H[j];
+        AH[2 * i + 1] += data[e] * H[j + ncols];
+      }
+    out.assign(2 * m, 0.0);
+    oracle_r2_update(G.data(), AH.data(), m, out.data());
+    for (long i = 0; i < m; ++i) {
+      W[i] = out[2 * i];
+      W[i + m] = out[2 * i + 1];
+    }
+  }
+}
+
+// powIter (hiernmf/matutils.hpp:37-120) at p = 1 on A0(:, cols) from q0:
+// q = q0/||q0||; repeat: z = A q, q = A^T z, sigma = ||q||, q /= sigma.
+double oracle_powiter_csr(long m, const long *indptr, const int *indices, const double *data,
+                          const long *cols, long ncols, long n, const double *q0, int iters) {
+  std::vector<long> colmap(n, -1);
+  for (long j = 0; j < ncols; ++j) colmap[cols[j]] = j;
+  std::vector<double> q(q0, q0 + ncols), z(m);
+  double nq = 0;
+  for (double v : q) nq += v * v;
+  nq = std::sqrt(nq);
+  for (double &v : q) v /= nq;
+  double sigma = 1;
+  for (int it = 0; it < iters; ++it) {
+    for (long i = 0; i < m; ++i) {
+      double s = 0;
+      for (long e = indptr[i]; e < indptr[i + 1]; ++e) {
+        long j = colmap[indices[e]];
+        if (j >= 0) s += data[e] * q[j];
+      }
+      z[i] = s;
+    }
+    std::fill(q.begin(), q.end(), 0.0);
+    for (long i = 0; i < m; ++i)
+      for (long e = indptr[i]; e < indptr[i + 1]; ++e) {
+        long j = colmap[indices[e]];
+        if (j >= 0) q[j] += data[e] * z[i];
+      }
+    double s2 = 0;
+    for (double v : q) s2 += v * v;
+    sigma = std::sqrt(s2);
+    for (double &v : q) v /= sigma;
+  }
+  return sigma;
+}
+
 }  // extern "C"

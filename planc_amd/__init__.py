@@ -13,7 +13,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import ANLSBPP, BF16, F16, F32, HALS, MU, R2, NMFDesc, NTFDesc  # noqa: F401
+from ._lib import ANLSBPP, BF16, F16, F32, HALS, MU, R2, HierDesc, HierNode, NMFDesc, NTFDesc  # noqa: F401
 
 __all__ = ["Context", "PlancError", "colmajor", "F32", "BF16", "F16", "MU", "HALS", "ANLSBPP"]
 
@@ -284,6 +284,57 @@ class Context:
         self._check(self.lib.planc_cp_loss_bwd(self.ctx, _ptr(X), N, cd, arr, factors[0].shape[1],
                                                garr), "planc_cp_loss_bwd")
         return grads
+
+    def csr_col_subset(self, indptr, indices, data, n, cols):
+        """Device CSR column subset A(:, cols) (cols sorted, unique).  indptr,
+        indices, data are device tensors (int64, int32, float32)."""
+        torch = _torch()
+        m = indptr.numel() - 1
+        c = np.ascontiguousarray(cols, dtype=np.int64)
+        out_ptr = torch.empty(m + 1, dtype=torch.int64, device=indptr.device)
+        cnt = ctypes.c_long()
+        lp = ctypes.POINTER(ctypes.c_long)
+        self._check(self.lib.planc_csr_col_subset(self.ctx, m, n, _ptr(indptr), _ptr(indices),
+                                                  _ptr(data), c.ctypes.data_as(lp), len(c),
+                                                  _ptr(out_ptr), None, None, ctypes.byref(cnt)),
+                    "planc_csr_col_subset")
+        oi = torch.empty(max(cnt.value, 1), dtype=torch.int32, device=indptr.device)
+        ov = torch.empty(max(cnt.value, 1), dtype=torch.float32, device=indptr.device)
+        self._check(self.lib.planc_csr_col_subset(self.ctx, m, n, _ptr(indptr), _ptr(indices),
+                                                  _ptr(data), c.ctypes.data_as(lp), len(c),
+                                                  _ptr(out_ptr), _ptr(oi), _ptr(ov),
+                                                  ctypes.byref(cnt)), "planc_csr_col_subset")
+        return out_ptr, oi[:cnt.value], ov[:cnt.value]
+
+    def hiernmf_run(self, m, n, indptr, indices, data, depth, iters, initseed, want_H=False,
+                    want_W=False):
+        """Hierarchical rank-2 NMF over a complete tree (host CSR input).
+        Returns (nodes: list of dicts, cols (int64 array), H or None, W or None)."""
+        d = HierDesc(m, n, depth, iters, initseed)
+        cap = (1 << depth) - 1
+        nodes = (HierNode * cap)()
+        nn = ctypes.c_int()
+        ip = np.ascontiguousarray(indptr, dtype=np.int64)
+        ix = np.ascontiguousarray(indices, dtype=np.int32)
+        dv = np.ascontiguousarray(data, dtype=np.float32)
+        cols = np.zeros(max(n * depth, 1), dtype=np.int64)
+        H = np.zeros(2 * max(n * depth, 1)) if want_H else None
+        W = np.zeros(2 * m * cap) if want_W else None
+        dp = ctypes.POINTER(ctypes.c_double)
+        self._check(self.lib.planc_hiernmf_run(
+            self.ctx, ctypes.byref(d), ip.ctypes.data_as(ctypes.POINTER(ctypes.c_long)),
+            ix.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+            dv.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), nodes, ctypes.byref(nn),
+            cols.ctypes.data_as(ctypes.POINTER(ctypes.c_long)),
+            H.ctypes.data_as(dp) if H is not None else None,
+            W.ctypes.data_as(dp) if W is not None else None), "planc_hiernmf_run")
+        out = []
+        for i in range(nn.value):
+            x = nodes[i]
+            out.append({"index": x.index, "level": x.level, "ncols": x.ncols,
+                        "col_offset": x.col_offset, "sigma": x.sigma, "score": x.score,
+                        "lvalid": x.lvalid, "rvalid": x.rvalid})
+        return out, cols, H, W
 
     # ---- whole loops -------------------------------------------------------
     def nmf_run(self, A, W, H, algo, iters, dtype=F32, regW=(0.0, 0.0), regH=(0.0, 0.0),

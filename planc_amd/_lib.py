@@ -31,6 +31,16 @@ class NMFDesc(ctypes.Structure):
                 ("compute_error", c_int)]
 
 
+class HierDesc(ctypes.Structure):
+    _fields_ = [("m", c_long), ("n", c_long), ("depth", c_int), ("iters", c_int),
+                ("initseed", c_int)]
+
+
+class HierNode(ctypes.Structure):
+    _fields_ = [("index", c_long), ("level", c_int), ("ncols", c_long), ("col_offset", c_long),
+                ("sigma", c_double), ("score", c_double), ("lvalid", c_int), ("rvalid", c_int)]
+
+
 class NTFDesc(ctypes.Structure):
     _fields_ = [("algo", c_int), ("dtype", c_int), ("nmodes", c_int), ("dims", c_long * 8),
                 ("k", c_int), ("iters", c_int), ("dimtree", c_int), ("compute_error", c_int)]
@@ -87,6 +97,14 @@ SIGNATURES = {
                                   ctypes.POINTER(c_void_p), c_int, ctypes.POINTER(c_void_p)]),
     "planc_bpp": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_long, c_long, c_long, c_void_p,
                           c_long, c_long]),
+    "planc_csr_col_subset": (c_int, [c_void_p, c_long, c_long, c_void_p, c_void_p, c_void_p,
+                                     ctypes.POINTER(c_long), c_long, c_void_p, c_void_p, c_void_p,
+                                     ctypes.POINTER(c_long)]),
+    "planc_hiernmf_run": (c_int, [c_void_p, ctypes.POINTER(HierDesc), ctypes.POINTER(c_long),
+                                  ctypes.POINTER(c_int), ctypes.POINTER(c_float),
+                                  ctypes.POINTER(HierNode), ctypes.POINTER(c_int),
+                                  ctypes.POINTER(c_long), ctypes.POINTER(c_double),
+                                  ctypes.POINTER(c_double)]),
     "planc_nmf_run": (c_int, [c_void_p, ctypes.POINTER(NMFDesc), c_void_p, c_long, c_void_p,
                               c_void_p, c_void_p, c_void_p, ctypes.POINTER(c_double)]),
     "planc_ntf_run": (c_int, [c_void_p, ctypes.POINTER(NTFDesc), c_void_p, c_void_p,

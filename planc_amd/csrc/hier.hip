@@ -1,0 +1,1078 @@
+// Hierarchical rank-2 NMF (hiernmf/hiernmf.cpp:65-184, hiernmf/node.hpp:46-188)
+// with all nodes of a tree level processed together.
+//
+// A level is the block-diagonal matrix diag(A(:, cols_1), ..., A(:, cols_t))
+// of its nodes' column subsets.  It is stored twice on the device:
+//  * block CSR over compacted rows: row r = (node t, original row i) for the
+//    rows where A(:, cols_t) has entries, columns = level column ids;
+//  * level CSC: column lc = original column perm[lc], entries' rows mapped to
+//    compacted row ids.
+// Both keep the original entry order (rows ascending within a column,
+// columns ascending within a row), so every sum is taken in a fixed order.
+//
+// Rank-2 NMF of every node (DistAUNMF::computeNMF at p = 1 with DistR2's
+// update, distnmf/aunmf.hpp:495-657, distnmf/distr2.hpp:35-61): H given W,
+// then W given H, 2 x 2 normal equations in FP64.  W keeps only the rows of
+// A(:, cols_t) with entries: an empty row of the node's matrix has A H = 0,
+// so DistR2 sets that row of W to 0 at the first W update and it stays 0;
+// it contributes to W^T W only at iteration 0, whose Gram is taken over all
+// m rows of the initial W (drawn from the node's Philox stream).
+// The iteration loop of a level is captured once into a hipGraph and
+// launched as one graph.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "common.h"
+#include "philox.h"
+
+namespace planc {
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Exclusive scan of int counts (n up to 2^31), three phases.
+constexpr int SCAN_B = 1024;
+
+__global__ void scan_block_sums(const int *in, long n, long *bsum) {
+  __shared__ long red[SCAN_B / 64];
+  long base = (long)blockIdx.x * SCAN_B;
+  long s = 0;
+  for (int t = threadIdx.x; t < SCAN_B; t += blockDim.x) {
+    long i = base + t;
+    if (i < n) s += in[i];
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    long t = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w];
+    bsum[blockIdx.x] = t;
+  }
+}
+
+__global__ void scan_sums_serial(long *bsum, long nb, long *total) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  long run = 0;
+  for (long b = 0; b < nb; ++b) {
+    long v = bsum[b];
+    bsum[b] = run;
+    run += v;
+  }
+  *total = run;
+}
+
+// One thread per element group: each block rescans its 1024 elements.
+__global__ void scan_apply(const int *in, long n, const long *bsum, long *out) {
+  __shared__ long vals[SCAN_B];
+  long base = (long)blockIdx.x * SCAN_B;
+  for (int t = threadIdx.x; t < SCAN_B; t += blockDim.x) {
+    long i = base + t;
+    vals[t] = (i < n) ? in[i] : 0;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    long run = bsum[blockIdx.x];
+    for (int t = 0; t < SCAN_B; ++t) {
+      long v = vals[t];
+      vals[t] = run;
+      run += v;
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < SCAN_B; t += blockDim.x) {
+    long i = base + t;
+    if (i < n) out[i] = vals[t];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Level construction.
+__global__ void level_col_len(const long *perm, long nlev, const long *cptr0, int *len) {
+  long lc = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (lc >= nlev) return;
+  long c = perm[lc];
+  len[lc] = (int)(cptr0[c + 1] - cptr0[c]);
+}
+
+// count[t*m + i] = entries of row i in node t's columns (one thread per row).
+__global__ void level_row_count(long m, const long *rptr0, const int *cidx0, const int *colmap,
+                                const int *node_of_col, int *count) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  for (long e = rptr0[i]; e < rptr0[i + 1]; ++e) {
+    int lc = colmap[cidx0[e]];
+    if (lc < 0) continue;
+    long t = node_of_col[lc];
+    count[t * m + i] += 1;
+  }
+}
+
+__global__ void flag_nonzero(const int *count, long n, int *flag) {
+  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) flag[e] = count[e] > 0;
+}
+
+// For each nonempty (t, i): compacted row id from the flag scan.
+__global__ void level_rows(const int *count, long nm, long m, const long *rowid, int *row_node,
+                           int *row_orig, int *row_len) {
+  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nm || count[e] == 0) return;
+  long r = rowid[e];
+  row_node[r] = (int)(e / m);
+  row_orig[r] = (int)(e % m);
+  row_len[r] = count[e];
+}
+
+// Block CSR fill, one thread per original row (entry order preserved).
+__global__ void level_csr_fill(long m, const long *rptr0, const int *cidx0, const float *val0,
+                               const int *colmap, const int *node_of_col, const long *rowid,
+                               const long *row_ptr, int *cursor, int *csr_col, float *csr_val) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  for (long e = rptr0[i]; e < rptr0[i + 1]; ++e) {
+    int lc = colmap[cidx0[e]];
+    if (lc < 0) continue;
+    long t = node_of_col[lc];
+    long r = rowid[t * m + i];
+    long pos = row_ptr[r] + cursor[r];
+    cursor[r] += 1;
+    csr_col[pos] = lc;
+    csr_val[pos] = val0[e];
+  }
+}
+
+// Level CSC fill, one thread per level column.
+__global__ void level_csc_fill(const long *perm, long nlev, long m, const long *cptr0,
+                               const int *ridx0, const float *cval0, const int *node_of_col,
+                               const long *rowid, const long *col_ptr, int *csc_row,
+                               float *csc_val) {
+  long lc = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (lc >= nlev) return;
+  long c = perm[lc];
+  long t = node_of_col[lc];
+  long o = col_ptr[lc];
+  for (long e = cptr0[c]; e < cptr0[c + 1]; ++e) {
+    csc_row[o] = (int)rowid[t * m + ridx0[e]];
+    csc_val[o] = cval0[e];
+    ++o;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Rank-2 normal equations (DistR2::update, distnmf/distr2.hpp:35-61): LU
+// with partial pivoting of the 2x2 Gram (arma::solve fast), then columns with
+// a negative entry replaced by the better single-variable solution.
+__device__ inline void r2_solve(const double *G, double r0, double r1, double &x0, double &x1) {
+  double a = G[0], c = G[1], b = G[2], d = G[3];
+  bool sw = fabs(c) > fabs(a);
+  double u11 = sw ? c : a, u12 = sw ? d : b;
+  double l21 = (sw ? a : c) / u11;
+  double u22 = (sw ? b : d) - l21 * u12;
+  double z0 = sw ? r1 : r0, z1 = (sw ? r0 : r1) - l21 * z0;
+  double g1 = z1 / u22;
+  double g0 = (z0 - u12 * g1) / u11;
+  if (g0 < 0 || g1 < 0) {
+    double u = r0 / a, v = r1 / d;
+    if (sqrt(a) * u >= sqrt(d) * v)
+      v = 0;
+    else
+      u = 0;
+    g0 = u;
+    g1 = v;
+  }
+  x0 = g0;
+  x1 = g1;
+}
+
+struct Chunk {
+  int node;
+  long b, e;
+};
+
+// Partial Grams of 2-column factors over chunks of contiguous rows.
+__global__ void gram2_chunks(const Chunk *ch, const double *X, double *part) {
+  __shared__ double red[3][4];
+  Chunk c = ch[blockIdx.x];
+  double s00 = 0, s01 = 0, s11 = 0;
+  for (long r = c.b + threadIdx.x; r < c.e; r += blockDim.x) {
+    double x0 = X[2 * r], x1 = X[2 * r + 1];
+    s00 += x0 * x0;
+    s01 += x0 * x1;
+    s11 += x1 * x1;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    s00 += __shfl_xor(s00, off, 64);
+    s01 += __shfl_xor(s01, off, 64);
+    s11 += __shfl_xor(s11, off, 64);
+  }
+  int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[0][w] = s00;
+    red[1][w] = s01;
+    red[2][w] = s11;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t0 = 0, t1 = 0, t2 = 0;
+    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) {
+      t0 += red[0][q];
+      t1 += red[1][q];
+      t2 += red[2][q];
+    }
+    part[3 * blockIdx.x] = t0;
+    part[3 * blockIdx.x + 1] = t1;
+    part[3 * blockIdx.x + 2] = t2;
+  }
+}
+
+// Node Gram (column-major 2x2) from its chunks [cb[t], cb[t+1]).
+__global__ void gram2_finalize(const double *part, const int *cb, int nn, double *G) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nn) return;
+  double a = 0, b = 0, c = 0;
+  for (int q = cb[t]; q < cb[t + 1]; ++q) {
+    a += part[3 * q];
+    b += part[3 * q + 1];
+    c += part[3 * q + 2];
+  }
+  G[4 * t] = a;
+  G[4 * t + 1] = b;
+  G[4 * t + 2] = b;
+  G[4 * t + 3] = c;
+}
+
+// Gram over all m rows of the initial W of every node (iteration 0).
+__global__ void init_gram_chunks(const Chunk *ch, const unsigned long long *seeds, long m,
+                                 double *part) {
+  __shared__ double red[3][4];
+  Chunk c = ch[blockIdx.x];
+  uint64_t sd = seeds[c.node];
+  double s00 = 0, s01 = 0, s11 = 0;
+  for (long i = c.b + threadIdx.x; i < c.e; i += blockDim.x) {
+    double x0 = planc_rng::u01(planc_rng::word(sd, 0, (uint64_t)i));
+    double x1 = planc_rng::u01(planc_rng::word(sd, 0, (uint64_t)(i + m)));
+    s00 += x0 * x0;
+    s01 += x0 * x1;
+    s11 += x1 * x1;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    s00 += __shfl_xor(s00, off, 64);
+    s01 += __shfl_xor(s01, off, 64);
+    s11 += __shfl_xor(s11, off, 64);
+  }
+  int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[0][w] = s00;
+    red[1][w] = s01;
+    red[2][w] = s11;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t0 = 0, t1 = 0, t2 = 0;
+    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) {
+      t0 += red[0][q];
+      t1 += red[1][q];
+      t2 += red[2][q];
+    }
+    part[3 * blockIdx.x] = t0;
+    part[3 * blockIdx.x + 1] = t1;
+    part[3 * blockIdx.x + 2] = t2;
+  }
+}
+
+// Initial factors from each node's Philox stream: W(i, c) word i + m c of
+// stream 0, H(j, c) word j + n_t c of stream 1 (j local column).
+__global__ void init_factors(long nrows, const int *row_node, const int *row_orig, long nlev,
+                             const int *node_of_col, const long *node_col0,
+                             const unsigned long long *seeds, long m, double *W, double *H) {
+  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < nrows) {
+    uint64_t sd = seeds[row_node[e]];
+    long i = row_orig[e];
+    W[2 * e] = planc_rng::u01(planc_rng::word(sd, 0, (uint64_t)i));
+    W[2 * e + 1] = planc_rng::u01(planc_rng::word(sd, 0, (uint64_t)(i + m)));
+  } else if (e < nrows + nlev) {
+    long lc = e - nrows;
+    int t = node_of_col[lc];
+    long j = lc - node_col0[t], nt = node_col0[t + 1] - node_col0[t];
+    uint64_t sd = seeds[t];
+    H[2 * lc] = planc_rng::u01(planc_rng::word(sd, 1, (uint64_t)j));
+    H[2 * lc + 1] = planc_rng::u01(planc_rng::word(sd, 1, (uint64_t)(j + nt)));
+  }
+}
+
+// H update: WtA(:, lc) over the column's entries, then the R2 rule with the
+// node's W^T W.
+__global__ void h_update(long nlev, const long *col_ptr, const int *csc_row, const float *csc_val,
+                         const int *node_of_col, const double *W, const double *Gw, double *H) {
+  long lc = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (lc >= nlev) return;
+  double s0 = 0, s1 = 0;
+  for (long e = col_ptr[lc]; e < col_ptr[lc + 1]; ++e) {
+    double v = csc_val[e];
+    int r = csc_row[e];
+    s0 += W[2 * r] * v;
+    s1 += W[2 * r + 1] * v;
+  }
+  double x0, x1;
+  r2_solve(Gw + 4 * node_of_col[lc], s0, s1, x0, x1);
+  H[2 * lc] = x0;
+  H[2 * lc + 1] = x1;
+}
+
+// W update: (A H)(r, :) over the compacted row's entries, then the R2 rule
+// with the node's H^T H.  G lanes per row (a sub-wave group, G in
+// {1, 4, 16}, picked per level by timing); rows longer than `longlen` are
+// left to w_update_long (one block per row) -- the power-law row
+// popularity of term-document matrices puts a few very long rows next to
+// millions of short ones.
+template <int G>
+__device__ inline double group_sum(double v) {
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, G);
+  return v;
+}
+
+template <int G>
+__global__ void w_update_short(long nrows, const long *row_ptr, const int *csr_col,
+                               const float *csr_val, const int *row_node, const double *H,
+                               const double *Gh, double *W, long longlen) {
+  long g = ((long)blockIdx.x * blockDim.x + threadIdx.x) / G;
+  int lane = threadIdx.x % G;
+  if (g >= nrows) return;
+  long b = row_ptr[g], e1 = row_ptr[g + 1];
+  if (e1 - b > longlen) return;
+  double s0 = 0, s1 = 0;
+  for (long e = b + lane; e < e1; e += G) {
+    double v = csr_val[e];
+    int lc = csr_col[e];
+    s0 += v * H[2 * lc];
+    s1 += v * H[2 * lc + 1];
+  }
+  s0 = group_sum<G>(s0);
+  s1 = group_sum<G>(s1);
+  if (lane == 0) {
+    double x0, x1;
+    r2_solve(Gh + 4 * row_node[g], s0, s1, x0, x1);
+    W[2 * g] = x0;
+    W[2 * g + 1] = x1;
+  }
+}
+
+__device__ inline double block_sum256(double v, double *red) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return red[0] + red[1] + red[2] + red[3];
+}
+
+__global__ void w_update_long(const int *list, const long *row_ptr, const int *csr_col,
+                              const float *csr_val, const int *row_node, const double *H,
+                              const double *Gh, double *W) {
+  __shared__ double red[4];
+  long r = list[blockIdx.x];
+  double s0 = 0, s1 = 0;
+  for (long e = row_ptr[r] + threadIdx.x; e < row_ptr[r + 1]; e += blockDim.x) {
+    double v = csr_val[e];
+    int lc = csr_col[e];
+    s0 += v * H[2 * lc];
+    s1 += v * H[2 * lc + 1];
+  }
+  s0 = block_sum256(s0, red);
+  s1 = block_sum256(s1, red);
+  if (threadIdx.x == 0) {
+    double x0, x1;
+    r2_solve(Gh + 4 * row_node[r], s0, s1, x0, x1);
+    W[2 * r] = x0;
+    W[2 * r + 1] = x1;
+  }
+}
+
+__global__ void flag_long(const long *row_ptr, long nrows, long longlen, int *flag) {
+  long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < nrows) flag[r] = (row_ptr[r + 1] - row_ptr[r]) > longlen;
+}
+
+__global__ void compact_list(const int *flag, const long *pos, long n, int *list) {
+  long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n && flag[r]) list[pos[r]] = (int)r;
+}
+
+// Power iteration on A_t^T A_t (hiernmf/matutils.hpp:37-120): z = A q / s,
+// q = A^T z, s = ||q|| per node.
+__global__ void pow_init(long nlev, const int *node_of_col, const long *node_col0,
+                         const unsigned long long *seeds, double *q) {
+  long lc = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (lc >= nlev) return;
+  int t = node_of_col[lc];
+  q[lc] = planc_rng::normal(seeds[t], 2, (uint64_t)(lc - node_col0[t]));
+}
+
+template <int G>
+__global__ void pow_rows_short(long nrows, const long *row_ptr, const int *csr_col,
+                               const float *csr_val, const int *row_node, const double *q,
+                               const double *sig, double *z, long longlen) {
+  long g = ((long)blockIdx.x * blockDim.x + threadIdx.x) / G;
+  int lane = threadIdx.x % G;
+  if (g >= nrows) return;
+  long b = row_ptr[g], e1 = row_ptr[g + 1];
+  if (e1 - b > longlen) return;
+  double s = 0;
+  for (long e = b + lane; e < e1; e += G) s += (double)csr_val[e] * q[csr_col[e]];
+  s = group_sum<G>(s);
+  if (lane == 0) z[g] = s / sig[row_node[g]];
+}
+
+__global__ void pow_rows_long(const int *list, const long *row_ptr, const int *csr_col,
+                              const float *csr_val, const int *row_node, const double *q,
+                              const double *sig, double *z) {
+  __shared__ double red[4];
+  long r = list[blockIdx.x];
+  double s = 0;
+  for (long e = row_ptr[r] + threadIdx.x; e < row_ptr[r + 1]; e += blockDim.x)
+    s += (double)csr_val[e] * q[csr_col[e]];
+  s = block_sum256(s, red);
+  if (threadIdx.x == 0) z[r] = s / sig[row_node[r]];
+}
+
+__global__ void pow_cols(long nlev, const long *col_ptr, const int *csc_row, const float *csc_val,
+                         const double *z, double *q) {
+  long lc = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (lc >= nlev) return;
+  double s = 0;
+  for (long e = col_ptr[lc]; e < col_ptr[lc + 1]; ++e) s += (double)csc_val[e] * z[csc_row[e]];
+  q[lc] = s;
+}
+
+__global__ void sumsq_chunks(const Chunk *ch, const double *x, double *part) {
+  __shared__ double red[4];
+  Chunk c = ch[blockIdx.x];
+  double s = 0;
+  for (long r = c.b + threadIdx.x; r < c.e; r += blockDim.x) s += x[r] * x[r];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0;
+    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) t += red[q];
+    part[blockIdx.x] = t;
+  }
+}
+
+__global__ void norm_finalize(const double *part, const int *cb, int nn, double *sig) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nn) return;
+  double s = 0;
+  for (int q = cb[t]; q < cb[t + 1]; ++q) s += part[q];
+  sig[t] = sqrt(s);
+}
+
+__global__ void fill_ones(double *x, int n) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) x[t] = 1.0;
+}
+
+__global__ void split_mask(long nlev, const double *H, unsigned char *mask) {
+  long lc = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (lc < nlev) mask[lc] = H[2 * lc] >= H[2 * lc + 1];
+}
+
+__global__ void scatter_w(long nrows, const int *row_orig, const double *Wc, long m, int slot_node,
+                          const int *row_node, double *Wfull) {
+  long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows || row_node[r] != slot_node) return;
+  long i = row_orig[r];
+  Wfull[i] = Wc[2 * r];
+  Wfull[i + m] = Wc[2 * r + 1];
+}
+
+__global__ void subset_count(long m, const long *indptr, const int *indices, const int *colmap, int *len) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  int c = 0;
+  for (long e = indptr[i]; e < indptr[i + 1]; ++e) c += colmap[indices[e]] >= 0;
+  len[i] = c;
+}
+
+__global__ void subset_fill(long m, const long *indptr, const int *indices, const float *data,
+                            const int *colmap, const long *optr, int *oidx, float *oval) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  long o = optr[i];
+  for (long e = indptr[i]; e < indptr[i + 1]; ++e) {
+    int lc = colmap[indices[e]];
+    if (lc < 0) continue;
+    oidx[o] = lc;
+    oval[o] = data[e];
+    ++o;
+  }
+}
+
+inline unsigned nb(long n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+// Device buffer owned by the level engine.
+template <class T>
+struct DBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  int get(planc_ctx *ctx, size_t want) {
+    if (want == 0) want = 1;
+    if (n >= want) return PLANC_OK;
+    if (p) {
+      (void)hipStreamSynchronize(ctx->stream);
+      (void)hipFree(p);
+    }
+    p = nullptr;
+    n = 0;
+    if (hipMalloc(&p, sizeof(T) * want) != hipSuccess) return set_error(ctx, PLANC_ENOMEM, "hiernmf: device memory");
+    n = want;
+    return PLANC_OK;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+struct Engine {
+  planc_ctx *ctx;
+  long m = 0, n = 0, nnz = 0;
+  DBuf<long> rptr0, cptr0;
+  DBuf<int> cidx0, ridx0;
+  DBuf<float> val0, cval0;
+  // level
+  DBuf<long> perm, col_ptr, row_ptr, rowid, bsum, node_col0, scratch_l;
+  DBuf<int> colmap, node_of_col, count, flag, row_node, row_orig, row_len, col_len, cursor, csr_col,
+      csc_row, cb_rows, cb_cols, cb_init, lflag, llist;
+  DBuf<long> lpos;
+  DBuf<double> Wscratch;
+  DBuf<float> csr_val, csc_val;
+  DBuf<double> W, H, Gw, Gh, part, q, z, sig;
+  DBuf<unsigned long long> seeds;
+  DBuf<Chunk> ch_rows, ch_cols, ch_init;
+  DBuf<unsigned char> mask;
+
+  ~Engine() {
+    rptr0.release(); cptr0.release(); cidx0.release(); ridx0.release(); val0.release();
+    cval0.release(); perm.release(); col_ptr.release(); row_ptr.release(); rowid.release();
+    bsum.release(); node_col0.release(); scratch_l.release(); colmap.release();
+    node_of_col.release(); count.release(); flag.release(); row_node.release();
+    row_orig.release(); row_len.release(); col_len.release(); cursor.release();
+    csr_col.release(); csc_row.release(); cb_rows.release(); cb_cols.release();
+    cb_init.release(); csr_val.release(); csc_val.release(); W.release(); H.release();
+    Gw.release(); Gh.release(); part.release(); q.release(); z.release(); sig.release();
+    seeds.release(); ch_rows.release(); ch_cols.release(); ch_init.release(); mask.release();
+    lflag.release(); llist.release(); lpos.release(); Wscratch.release();
+  }
+
+  // Launches of the row kernels with G lanes per row.
+  void launch_w_short(int G, long nrows, double *Wout, long longlen) {
+    long thr = nrows * G;
+    unsigned nbk = (unsigned)((thr + 255) / 256);
+    if (nbk == 0) return;
+    if (G == 1)
+      w_update_short<1><<<nbk, 256, 0, ctx->stream>>>(nrows, row_ptr.p, csr_col.p, csr_val.p, row_node.p, H.p, Gh.p, Wout, longlen);
+    else if (G == 4)
+      w_update_short<4><<<nbk, 256, 0, ctx->stream>>>(nrows, row_ptr.p, csr_col.p, csr_val.p, row_node.p, H.p, Gh.p, Wout, longlen);
+    else
+      w_update_short<16><<<nbk, 256, 0, ctx->stream>>>(nrows, row_ptr.p, csr_col.p, csr_val.p, row_node.p, H.p, Gh.p, Wout, longlen);
+  }
+  void launch_pow_short(int G, long nrows, long longlen) {
+    long thr = nrows * G;
+    unsigned nbk = (unsigned)((thr + 255) / 256);
+    if (nbk == 0) return;
+    if (G == 1)
+      pow_rows_short<1><<<nbk, 256, 0, ctx->stream>>>(nrows, row_ptr.p, csr_col.p, csr_val.p, row_node.p, q.p, sig.p, z.p, longlen);
+    else if (G == 4)
+      pow_rows_short<4><<<nbk, 256, 0, ctx->stream>>>(nrows, row_ptr.p, csr_col.p, csr_val.p, row_node.p, q.p, sig.p, z.p, longlen);
+    else
+      pow_rows_short<16><<<nbk, 256, 0, ctx->stream>>>(nrows, row_ptr.p, csr_col.p, csr_val.p, row_node.p, q.p, sig.p, z.p, longlen);
+  }
+  // Lanes per row for this level: the fastest of {1, 4, 16} over a few
+  // timed launches of the W update into a scratch buffer.
+  int tune_group(long nrows, long longlen) {
+    if (nrows == 0) return 1;
+    if (Wscratch.get(ctx, 2 * nrows)) return 1;
+    const int cand[3] = {1, 4, 16};
+    float best = 1e30f;
+    int bestg = 1;
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return 1;
+    if (hipEventCreate(&e1) != hipSuccess) {
+      (void)hipEventDestroy(e0);
+      return 1;
+    }
+    for (int c : cand) {
+      launch_w_short(c, nrows, Wscratch.p, longlen);
+      (void)hipEventRecord(e0, ctx->stream);
+      for (int rep = 0; rep < 3; ++rep) launch_w_short(c, nrows, Wscratch.p, longlen);
+      (void)hipEventRecord(e1, ctx->stream);
+      (void)hipEventSynchronize(e1);
+      float ms = 0;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      if (ms < best) {
+        best = ms;
+        bestg = c;
+      }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return bestg;
+  }
+
+  // out[i] = sum_{j < i} in[j]; returns the total (synchronises).
+  int scan(const int *in, long cnt, long *out, long *total) {
+    long nbk = (cnt + SCAN_B - 1) / SCAN_B;
+    if (nbk == 0) {
+      *total = 0;
+      return PLANC_OK;
+    }
+    int rc = bsum.get(ctx, nbk + 1);
+    if (rc) return rc;
+    scan_block_sums<<<(unsigned)nbk, 256, 0, ctx->stream>>>(in, cnt, bsum.p);
+    scan_sums_serial<<<1, 64, 0, ctx->stream>>>(bsum.p, nbk, bsum.p + nbk);
+    scan_apply<<<(unsigned)nbk, 256, 0, ctx->stream>>>(in, cnt, bsum.p, out);
+    PLANC_LAUNCH_CHECK(ctx);
+    PLANC_HIP_CHECK(ctx, hipMemcpyAsync(total, bsum.p + nbk, sizeof(long), hipMemcpyDeviceToHost, ctx->stream));
+    PLANC_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return PLANC_OK;
+  }
+};
+
+// Chunks of at most `chunk` consecutive items per node (node t owns
+// [off[t], off[t+1])); cb[t] is the first chunk of node t.
+void make_chunks(const std::vector<long> &off, long chunk, std::vector<Chunk> &ch, std::vector<int> &cb) {
+  ch.clear();
+  cb.assign(off.size(), 0);
+  for (size_t t = 0; t + 1 < off.size(); ++t) {
+    cb[t] = (int)ch.size();
+    for (long b = off[t]; b < off[t + 1]; b += chunk) ch.push_back({(int)t, b, std::min(off[t + 1], b + chunk)});
+  }
+  cb[off.size() - 1] = (int)ch.size();
+}
+
+}  // namespace
+
+}  // namespace planc
+
+using namespace planc;
+
+extern "C" {
+
+int planc_hiernmf_run(planc_ctx *ctx, const planc_hier_desc *desc, const long *h_indptr,
+                      const int *h_indices, const float *h_data, planc_hier_node *h_nodes,
+                      int *h_num_nodes, long *h_cols, double *h_H, double *h_W) {
+  if (!ctx || !desc || !h_indptr || !h_indices || !h_data || !h_nodes || !h_num_nodes || !h_cols)
+    return PLANC_EINVAL;
+  const long m = desc->m, n = desc->n;
+  const int depth = desc->depth, iters = desc->iters;
+  if (m <= 0 || n <= 0 || depth < 1 || depth > 24 || iters < 0)
+    return set_error(ctx, PLANC_EINVAL, "planc_hiernmf_run: bad arguments");
+  const long nnz = h_indptr[m];
+  if (nnz < 0 || nnz >= (1L << 31)) return set_error(ctx, PLANC_EINVAL, "planc_hiernmf_run: nnz");
+  for (long i = 0; i < m; ++i)
+    if (h_indptr[i + 1] < h_indptr[i]) return set_error(ctx, PLANC_EINVAL, "planc_hiernmf_run: indptr");
+  for (long e = 0; e < nnz; ++e)
+    if (h_indices[e] < 0 || h_indices[e] >= n) return set_error(ctx, PLANC_EINVAL, "planc_hiernmf_run: column index");
+  PLANC_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  Engine E;
+  E.ctx = ctx;
+  E.m = m;
+  E.n = n;
+  E.nnz = nnz;
+  int rc;
+#define RC(x)          \
+  do {                 \
+    rc = (x);          \
+    if (rc) return rc; \
+  } while (0)
+  // CSC of A on the host (counting sort, rows ascending within a column).
+  std::vector<long> cptr(n + 1, 0);
+  for (long e = 0; e < nnz; ++e) cptr[h_indices[e] + 1]++;
+  for (long j = 0; j < n; ++j) cptr[j + 1] += cptr[j];
+  std::vector<int> ridx(nnz);
+  std::vector<float> cval(nnz);
+  {
+    std::vector<long> cur(cptr.begin(), cptr.end() - 1);
+    for (long i = 0; i < m; ++i)
+      for (long e = h_indptr[i]; e < h_indptr[i + 1]; ++e) {
+        long p = cur[h_indices[e]]++;
+        ridx[p] = (int)i;
+        cval[p] = h_data[e];
+      }
+  }
+  RC(E.rptr0.get(ctx, m + 1));
+  RC(E.cptr0.get(ctx, n + 1));
+  RC(E.cidx0.get(ctx, nnz));
+  RC(E.ridx0.get(ctx, nnz));
+  RC(E.val0.get(ctx, nnz));
+  RC(E.cval0.get(ctx, nnz));
+  auto up = [&](void *d, const void *h, size_t b) {
+    return hipMemcpyAsync(d, h, b, hipMemcpyHostToDevice, ctx->stream);
+  };
+  PLANC_HIP_CHECK(ctx, up(E.rptr0.p, h_indptr, sizeof(long) * (m + 1)));
+  PLANC_HIP_CHECK(ctx, up(E.cptr0.p, cptr.data(), sizeof(long) * (n + 1)));
+  if (nnz) {
+    PLANC_HIP_CHECK(ctx, up(E.cidx0.p, h_indices, sizeof(int) * nnz));
+    PLANC_HIP_CHECK(ctx, up(E.ridx0.p, ridx.data(), sizeof(int) * nnz));
+    PLANC_HIP_CHECK(ctx, up(E.val0.p, h_data, sizeof(float) * nnz));
+    PLANC_HIP_CHECK(ctx, up(E.cval0.p, cval.data(), sizeof(float) * nnz));
+  }
+
+  // Tree state.
+  struct HNode {
+    long index;
+    int level;
+    std::vector<long> cols;
+    double sigma = 0, score = 0;
+    int lvalid = 0, rvalid = 0;
+    long lchild = -1, rchild = -1;
+    long out_slot = -1;
+  };
+  std::vector<HNode> nodes;
+  nodes.push_back({0, 0, {}, 0, 0, 0, 0, -1, -1, -1});
+  nodes[0].cols.resize(n);
+  for (long j = 0; j < n; ++j) nodes[0].cols[j] = j;
+  std::vector<long> level_nodes = {0};
+  long cols_used = 0;
+  const long cap_nodes = (1L << depth) - 1;
+  const unsigned long long seed0 = (unsigned long long)(long long)desc->initseed;
+
+  for (int lev = 0; lev < depth && !level_nodes.empty(); ++lev) {
+    const int nn = (int)level_nodes.size();
+    // Level columns: concatenation of the nodes' (sorted) column lists.
+    std::vector<long> perm, noff(nn + 1, 0);
+    for (int t = 0; t < nn; ++t) {
+      const auto &c = nodes[level_nodes[t]].cols;
+      perm.insert(perm.end(), c.begin(), c.end());
+      noff[t + 1] = noff[t] + (long)c.size();
+    }
+    const long nlev = (long)perm.size();
+    std::vector<int> colmap(n, -1), nodeof(nlev);
+    for (int t = 0; t < nn; ++t)
+      for (long lc = noff[t]; lc < noff[t + 1]; ++lc) {
+        colmap[perm[lc]] = (int)lc;
+        nodeof[lc] = t;
+      }
+    std::vector<unsigned long long> seeds(nn);
+    for (int t = 0; t < nn; ++t)  // set_seed(initseed * index + rank), hiernmf/node.hpp:123
+      seeds[t] = seed0 * (unsigned long long)nodes[level_nodes[t]].index;
+    RC(E.perm.get(ctx, nlev));
+    RC(E.colmap.get(ctx, n));
+    RC(E.node_of_col.get(ctx, nlev));
+    RC(E.node_col0.get(ctx, nn + 1));
+    RC(E.seeds.get(ctx, nn));
+    PLANC_HIP_CHECK(ctx, up(E.perm.p, perm.data(), sizeof(long) * nlev));
+    PLANC_HIP_CHECK(ctx, up(E.colmap.p, colmap.data(), sizeof(int) * n));
+    PLANC_HIP_CHECK(ctx, up(E.node_of_col.p, nodeof.data(), sizeof(int) * nlev));
+    PLANC_HIP_CHECK(ctx, up(E.node_col0.p, noff.data(), sizeof(long) * (nn + 1)));
+    PLANC_HIP_CHECK(ctx, up(E.seeds.p, seeds.data(), sizeof(unsigned long long) * nn));
+    // Level CSC pointers.
+    RC(E.col_len.get(ctx, nlev));
+    RC(E.col_ptr.get(ctx, nlev + 1));
+    level_col_len<<<nb(nlev), 256, 0, ctx->stream>>>(E.perm.p, nlev, E.cptr0.p, E.col_len.p);
+    long lnnz = 0;
+    RC(E.scan(E.col_len.p, nlev, E.col_ptr.p, &lnnz));
+    PLANC_HIP_CHECK(ctx, up(E.col_ptr.p + nlev, &lnnz, sizeof(long)));
+    // Compacted rows.
+    const long nm = (long)nn * m;
+    RC(E.count.get(ctx, nm));
+    RC(E.flag.get(ctx, nm));
+    RC(E.rowid.get(ctx, nm));
+    PLANC_HIP_CHECK(ctx, hipMemsetAsync(E.count.p, 0, sizeof(int) * nm, ctx->stream));
+    level_row_count<<<nb(m), 256, 0, ctx->stream>>>(m, E.rptr0.p, E.cidx0.p, E.colmap.p,
+                                                   E.node_of_col.p, E.count.p);
+    flag_nonzero<<<nb(nm), 256, 0, ctx->stream>>>(E.count.p, nm, E.flag.p);
+    long nrows = 0;
+    RC(E.scan(E.flag.p, nm, E.rowid.p, &nrows));
+    RC(E.row_node.get(ctx, nrows));
+    RC(E.row_orig.get(ctx, nrows));
+    RC(E.row_len.get(ctx, nrows));
+    RC(E.row_ptr.get(ctx, nrows + 1));
+    level_rows<<<nb(nm), 256, 0, ctx->stream>>>(E.count.p, nm, m, E.rowid.p, E.row_node.p,
+                                               E.row_orig.p, E.row_len.p);
+    long lnnz2 = 0;
+    RC(E.scan(E.row_len.p, nrows, E.row_ptr.p, &lnnz2));
+    PLANC_HIP_CHECK(ctx, up(E.row_ptr.p + nrows, &lnnz2, sizeof(long)));
+    if (lnnz2 != lnnz) return set_error(ctx, PLANC_EINVAL, "hiernmf: level CSR/CSC mismatch");
+    RC(E.cursor.get(ctx, nrows));
+    RC(E.csr_col.get(ctx, lnnz));
+    RC(E.csr_val.get(ctx, lnnz));
+    RC(E.csc_row.get(ctx, lnnz));
+    RC(E.csc_val.get(ctx, lnnz));
+    PLANC_HIP_CHECK(ctx, hipMemsetAsync(E.cursor.p, 0, sizeof(int) * (nrows ? nrows : 1), ctx->stream));
+    level_csr_fill<<<nb(m), 256, 0, ctx->stream>>>(m, E.rptr0.p, E.cidx0.p, E.val0.p, E.colmap.p,
+                                                  E.node_of_col.p, E.rowid.p, E.row_ptr.p,
+                                                  E.cursor.p, E.csr_col.p, E.csr_val.p);
+    level_csc_fill<<<nb(nlev), 256, 0, ctx->stream>>>(E.perm.p, nlev, m, E.cptr0.p, E.ridx0.p,
+                                                     E.cval0.p, E.node_of_col.p, E.rowid.p,
+                                                     E.col_ptr.p, E.csc_row.p, E.csc_val.p);
+    PLANC_LAUNCH_CHECK(ctx);
+    // Rows longer than LONGLEN get a block each (list built on the device).
+    const long LONGLEN = 2048;
+    RC(E.lflag.get(ctx, nrows ? nrows : 1));
+    RC(E.lpos.get(ctx, nrows ? nrows : 1));
+    long nlong = 0;
+    if (nrows) {
+      flag_long<<<nb(nrows), 256, 0, ctx->stream>>>(E.row_ptr.p, nrows, LONGLEN, E.lflag.p);
+      RC(E.scan(E.lflag.p, nrows, E.lpos.p, &nlong));
+    }
+    RC(E.llist.get(ctx, nlong ? nlong : 1));
+    if (nlong) compact_list<<<nb(nrows), 256, 0, ctx->stream>>>(E.lflag.p, E.lpos.p, nrows, E.llist.p);
+    PLANC_LAUNCH_CHECK(ctx);
+    // Node row ranges (rows of node t are contiguous: t-major flag scan).
+    std::vector<int> rnode(nrows);
+    if (nrows)
+      PLANC_HIP_CHECK(ctx, hipMemcpyAsync(rnode.data(), E.row_node.p, sizeof(int) * nrows,
+                                          hipMemcpyDeviceToHost, ctx->stream));
+    PLANC_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    std::vector<long> roff(nn + 1, 0);
+    for (long r = 0; r < nrows; ++r) roff[rnode[r] + 1]++;
+    for (int t = 0; t < nn; ++t) roff[t + 1] += roff[t];
+    std::vector<Chunk> chr, chc, chi;
+    std::vector<int> cbr, cbc, cbi;
+    make_chunks(roff, 4096, chr, cbr);
+    make_chunks(noff, 4096, chc, cbc);
+    std::vector<long> ioff(nn + 1);
+    for (int t = 0; t <= nn; ++t) ioff[t] = (long)t * m;
+    make_chunks(ioff, 16384, chi, cbi);
+    // init chunks address rows 0..m-1 of each node: rebase to local rows.
+    for (auto &c : chi) {
+      long base = (long)c.node * m;
+      c.b -= base;
+      c.e -= base;
+    }
+    RC(E.ch_rows.get(ctx, chr.size()));
+    RC(E.ch_cols.get(ctx, chc.size()));
+    RC(E.ch_init.get(ctx, chi.size()));
+    RC(E.cb_rows.get(ctx, nn + 1));
+    RC(E.cb_cols.get(ctx, nn + 1));
+    RC(E.cb_init.get(ctx, nn + 1));
+    if (!chr.empty()) PLANC_HIP_CHECK(ctx, up(E.ch_rows.p, chr.data(), sizeof(Chunk) * chr.size()));
+    if (!chc.empty()) PLANC_HIP_CHECK(ctx, up(E.ch_cols.p, chc.data(), sizeof(Chunk) * chc.size()));
+    PLANC_HIP_CHECK(ctx, up(E.ch_init.p, chi.data(), sizeof(Chunk) * chi.size()));
+    PLANC_HIP_CHECK(ctx, up(E.cb_rows.p, cbr.data(), sizeof(int) * (nn + 1)));
+    PLANC_HIP_CHECK(ctx, up(E.cb_cols.p, cbc.data(), sizeof(int) * (nn + 1)));
+    PLANC_HIP_CHECK(ctx, up(E.cb_init.p, cbi.data(), sizeof(int) * (nn + 1)));
+    size_t np = std::max(std::max(chr.size(), chc.size()), chi.size());
+    RC(E.part.get(ctx, 3 * np + 3));
+    RC(E.W.get(ctx, 2 * (nrows ? nrows : 1)));
+    RC(E.H.get(ctx, 2 * nlev));
+    RC(E.Gw.get(ctx, 4 * nn));
+    RC(E.Gh.get(ctx, 4 * nn));
+    RC(E.sig.get(ctx, nn));
+    RC(E.q.get(ctx, nlev));
+    RC(E.z.get(ctx, nrows ? nrows : 1));
+    RC(E.mask.get(ctx, nlev));
+
+    // Sigma of this level's nodes (children created by the previous level):
+    // 100-step power iteration of Node::compute_sigma (hiernmf/node.hpp:66-69).
+    std::vector<double> sig(nn, 0.0);
+    if (lev > 0) {
+      pow_init<<<nb(nlev), 256, 0, ctx->stream>>>(nlev, E.node_of_col.p, E.node_col0.p, E.seeds.p, E.q.p);
+      sumsq_chunks<<<(unsigned)chc.size(), 256, 0, ctx->stream>>>(E.ch_cols.p, E.q.p, E.part.p);
+      norm_finalize<<<nb(nn), 256, 0, ctx->stream>>>(E.part.p, E.cb_cols.p, nn, E.sig.p);
+      const int Gp = E.tune_group(nrows, LONGLEN);
+      for (int it = 0; it < iters; ++it) {
+        E.launch_pow_short(Gp, nrows, LONGLEN);
+        if (nlong)
+          pow_rows_long<<<(unsigned)nlong, 256, 0, ctx->stream>>>(E.llist.p, E.row_ptr.p, E.csr_col.p,
+                                                                  E.csr_val.p, E.row_node.p, E.q.p,
+                                                                  E.sig.p, E.z.p);
+        pow_cols<<<nb(nlev), 256, 0, ctx->stream>>>(nlev, E.col_ptr.p, E.csc_row.p, E.csc_val.p, E.z.p, E.q.p);
+        sumsq_chunks<<<(unsigned)chc.size(), 256, 0, ctx->stream>>>(E.ch_cols.p, E.q.p, E.part.p);
+        norm_finalize<<<nb(nn), 256, 0, ctx->stream>>>(E.part.p, E.cb_cols.p, nn, E.sig.p);
+      }
+      PLANC_LAUNCH_CHECK(ctx);
+      if (iters > 0)
+        PLANC_HIP_CHECK(ctx, hipMemcpyAsync(sig.data(), E.sig.p, sizeof(double) * nn,
+                                            hipMemcpyDeviceToHost, ctx->stream));
+      PLANC_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+      for (int t = 0; t < nn; ++t) nodes[level_nodes[t]].sigma = iters > 0 ? sig[t] : 0.0;
+    }
+    // Record the level's nodes.
+    for (int t = 0; t < nn; ++t) {
+      HNode &nd = nodes[level_nodes[t]];
+      nd.out_slot = cols_used;
+      std::memcpy(h_cols + cols_used, nd.cols.data(), sizeof(long) * nd.cols.size());
+      cols_used += (long)nd.cols.size();
+    }
+    if (lev == depth - 1) break;  // leaves are not split
+
+    // Rank-2 NMF of every node of the level: the iteration loop is captured
+    // into one hipGraph and launched once.
+    init_factors<<<nb(nrows + nlev), 256, 0, ctx->stream>>>(nrows, E.row_node.p, E.row_orig.p, nlev,
+                                                            E.node_of_col.p, E.node_col0.p, E.seeds.p,
+                                                            m, E.W.p, E.H.p);
+    init_gram_chunks<<<(unsigned)chi.size(), 256, 0, ctx->stream>>>(E.ch_init.p, E.seeds.p, m, E.part.p);
+    gram2_finalize<<<nb(nn), 256, 0, ctx->stream>>>(E.part.p, E.cb_init.p, nn, E.Gw.p);
+    PLANC_LAUNCH_CHECK(ctx);
+    const int Gw = E.tune_group(nrows, LONGLEN);
+    if (iters > 0) {
+      hipGraph_t graph;
+      hipGraphExec_t exec;
+      PLANC_HIP_CHECK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+      for (int it = 0; it < iters; ++it) {
+        if (it > 0) {
+          if (!chr.empty())
+            gram2_chunks<<<(unsigned)chr.size(), 256, 0, ctx->stream>>>(E.ch_rows.p, E.W.p, E.part.p);
+          gram2_finalize<<<nb(nn), 256, 0, ctx->stream>>>(E.part.p, E.cb_rows.p, nn, E.Gw.p);
+        }
+        h_update<<<nb(nlev), 256, 0, ctx->stream>>>(nlev, E.col_ptr.p, E.csc_row.p, E.csc_val.p,
+                                                   E.node_of_col.p, E.W.p, E.Gw.p, E.H.p);
+        gram2_chunks<<<(unsigned)chc.size(), 256, 0, ctx->stream>>>(E.ch_cols.p, E.H.p, E.part.p);
+        gram2_finalize<<<nb(nn), 256, 0, ctx->stream>>>(E.part.p, E.cb_cols.p, nn, E.Gh.p);
+        E.launch_w_short(Gw, nrows, E.W.p, LONGLEN);
+        if (nlong)
+          w_update_long<<<(unsigned)nlong, 256, 0, ctx->stream>>>(E.llist.p, E.row_ptr.p, E.csr_col.p,
+                                                                  E.csr_val.p, E.row_node.p, E.H.p,
+                                                                  E.Gh.p, E.W.p);
+      }
+      hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
+      if (ce != hipSuccess) return set_error(ctx, PLANC_EHIP, std::string("hiernmf capture: ") + hipGetErrorString(ce));
+      ce = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      if (ce != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        return set_error(ctx, PLANC_EHIP, std::string("hiernmf instantiate: ") + hipGetErrorString(ce));
+      }
+      ce = hipGraphLaunch(exec, ctx->stream);
+      (void)hipStreamSynchronize(ctx->stream);
+      (void)hipGraphExecDestroy(exec);
+      (void)hipGraphDestroy(graph);
+      if (ce != hipSuccess) return set_error(ctx, PLANC_EHIP, std::string("hiernmf launch: ") + hipGetErrorString(ce));
+    }
+    // Split rule H(:,0) >= H(:,1), larger side to the left child
+    // (hiernmf/node.hpp:147-185).
+    split_mask<<<nb(nlev), 256, 0, ctx->stream>>>(nlev, E.H.p, E.mask.p);
+    std::vector<unsigned char> mk(nlev);
+    PLANC_HIP_CHECK(ctx, hipMemcpyAsync(mk.data(), E.mask.p, nlev, hipMemcpyDeviceToHost, ctx->stream));
+    std::vector<double> Hh;
+    if (h_H) {
+      Hh.resize(2 * nlev);
+      PLANC_HIP_CHECK(ctx, hipMemcpyAsync(Hh.data(), E.H.p, sizeof(double) * 2 * nlev, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    std::vector<double> Wc;
+    std::vector<int> rorig;
+    if (h_W) {
+      Wc.resize(2 * nrows);
+      rorig.resize(nrows);
+      if (nrows) {
+        PLANC_HIP_CHECK(ctx, hipMemcpyAsync(Wc.data(), E.W.p, sizeof(double) * 2 * nrows, hipMemcpyDeviceToHost, ctx->stream));
+        PLANC_HIP_CHECK(ctx, hipMemcpyAsync(rorig.data(), E.row_orig.p, sizeof(int) * nrows, hipMemcpyDeviceToHost, ctx->stream));
+      }
+    }
+    PLANC_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    std::vector<long> next;
+    for (int t = 0; t < nn; ++t) {
+      long id = level_nodes[t];
+      long slot = nodes[id].out_slot;  // H / W rows of this node
+      if (h_H)
+        for (long lc = noff[t]; lc < noff[t + 1]; ++lc) {
+          h_H[2 * (slot + (lc - noff[t]))] = Hh[2 * lc];
+          h_H[2 * (slot + (lc - noff[t])) + 1] = Hh[2 * lc + 1];
+        }
+      if (h_W) {
+        long widx = nodes[id].index;
+        double *Wf = h_W + (size_t)widx * 2 * m;
+        std::fill(Wf, Wf + 2 * m, 0.0);
+        for (long r = roff[t]; r < roff[t + 1]; ++r) {
+          Wf[rorig[r]] = Wc[2 * r];
+          Wf[rorig[r] + m] = Wc[2 * r + 1];
+        }
+      }
+      std::vector<long> L, R;
+      for (long lc = noff[t]; lc < noff[t + 1]; ++lc) (mk[lc] ? L : R).push_back(perm[lc]);
+      if (R.size() > L.size()) std::swap(L, R);
+      HNode &nd = nodes[id];
+      nd.lvalid = !L.empty();
+      nd.rvalid = !R.empty();
+      long idx = nd.index;
+      int lv = nd.level;
+      if (nd.lvalid) {
+        nodes.push_back({2 * idx + 1, lv + 1, L, 0, 0, 0, 0, -1, -1, -1});
+        nodes[id].lchild = (long)nodes.size() - 1;
+        next.push_back((long)nodes.size() - 1);
+      }
+      if (nodes[id].rvalid) {
+        nodes.push_back({2 * idx + 2, lv + 1, R, 0, 0, 0, 0, -1, -1, -1});
+        nodes[id].rchild = (long)nodes.size() - 1;
+        next.push_back((long)nodes.size() - 1);
+      }
+    }
+    level_nodes.swap(next);
+  }
+#undef RC
+  // Scores (Node::compute_score, hiernmf/node.hpp:71-81) and the node table.
+  long cnt = 0;
+  for (auto &nd : nodes) {
+    if (nd.out_slot < 0) continue;
+    if (cnt >= cap_nodes) break;
+    double ls = nd.lchild >= 0 ? nodes[nd.lchild].sigma : 0.0;
+    double rs = nd.rchild >= 0 ? nodes[nd.rchild].sigma : 0.0;
+    if (nd.lvalid && nd.rvalid)
+      nd.score = (ls + rs) - nd.sigma;
+    else if (nd.lvalid)
+      nd.score = ls - nd.sigma;
+    else if (nd.rvalid)
+      nd.score = rs - nd.sigma;
+    else
+      nd.score = 0;
+    planc_hier_node &o = h_nodes[cnt++];
+    o.index = nd.index;
+    o.level = nd.level;
+    o.ncols = (long)nd.cols.size();
+    o.col_offset = nd.out_slot;
+    o.sigma = nd.sigma;
+    o.score = nd.score;
+    o.lvalid = nd.lvalid;
+    o.rvalid = nd.rvalid;
+  }
+  *h_num_nodes = (int)cnt;
+  return PLANC_OK;
+}
+
+int planc_csr_col_subset(planc_ctx *ctx, long m, long n, const long *d_indptr, const int *d_indices,
+                         const float *d_data, const long *h_cols, long ncols, long *d_out_indptr,
+                         int *d_out_indices, float *d_out_data, long *h_out_nnz) {
+  if (!ctx) return PLANC_EINVAL;
+  if (m < 0 || n < 0 || ncols < 0 || !d_indptr || !d_out_indptr || !h_out_nnz || (ncols && !h_cols))
+    return set_error(ctx, PLANC_EINVAL, "planc_csr_col_subset: bad arguments");
+  std::vector<int> colmap(n > 0 ? n : 1, -1);
+  for (long j = 0; j < ncols; ++j) {
+    if (h_cols[j] < 0 || h_cols[j] >= n || (j > 0 && h_cols[j] <= h_cols[j - 1]))
+      return set_error(ctx, PLANC_EINVAL, "planc_csr_col_subset: columns must be sorted, unique, in range");
+    colmap[h_cols[j]] = (int)j;
+  }
+  Engine E;
+  E.ctx = ctx;
+  int rc;
+  if ((rc = E.colmap.get(ctx, colmap.size()))) return rc;
+  if ((rc = E.row_len.get(ctx, m + 1))) return rc;
+  PLANC_HIP_CHECK(ctx, hipMemcpyAsync(E.colmap.p, colmap.data(), sizeof(int) * colmap.size(),
+                                      hipMemcpyHostToDevice, ctx->stream));
+  subset_count<<<nb(m), 256, 0, ctx->stream>>>(m, d_indptr, d_indices, E.colmap.p, E.row_len.p);
+  PLANC_LAUNCH_CHECK(ctx);
+  long total = 0;
+  if ((rc = E.scan(E.row_len.p, m, d_out_indptr, &total))) return rc;
+  PLANC_HIP_CHECK(ctx, hipMemcpyAsync(d_out_indptr + m, &total, sizeof(long), hipMemcpyHostToDevice, ctx->stream));
+  if (total > 0) {
+    if (!d_out_indices || !d_out_data) return set_error(ctx, PLANC_EINVAL, "planc_csr_col_subset: null outputs");
+    subset_fill<<<nb(m), 256, 0, ctx->stream>>>(m, d_indptr, d_indices, d_data, E.colmap.p, d_out_indptr,
+                                               d_out_indices, d_out_data);
+    PLANC_LAUNCH_CHECK(ctx);
+  }
+  PLANC_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  *h_out_nnz = total;
+  return PLANC_OK;
+}
+
+}  // extern "C"

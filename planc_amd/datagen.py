@@ -71,3 +71,25 @@ def gamma2(seed, stream, shape, scale=0.5):
     u = 1.0 - (w >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
     g = -scale * (np.log(u[0::2]) + np.log(u[1::2]))
     return g.reshape(shape, order="F")
+
+
+def sparse_powerlaw(m, n, density, alpha, stream):
+    """CSR (indptr int64, indices int32 sorted per row, data float32) of an
+    m x n matrix: every column draws round(density*m) rows with probability
+    proportional to (i+1)^-alpha (inverse-CDF sampling of Philox uniforms),
+    duplicates dropped; values exp(N(0,1)) (log-normal)."""
+    per = max(1, int(round(density * m)))
+    w = (np.arange(1, m + 1, dtype=np.float64)) ** (-alpha)
+    cdf = np.cumsum(w)
+    cdf /= cdf[-1]
+    u = uniform(DATA_SEED, 16 * stream + 3, (n * per,))
+    rows = np.minimum(np.searchsorted(cdf, u, side="right"), m - 1).astype(np.int64)
+    cols = np.repeat(np.arange(n, dtype=np.int64), per)
+    key = np.unique(rows * n + cols)
+    r = key // n
+    c = key % n
+    vals = np.exp(normal(NOISE_SEED, stream + 16, (key.size,))).astype(np.float32)
+    indptr = np.zeros(m + 1, dtype=np.int64)
+    np.add.at(indptr, r + 1, 1)
+    indptr = np.cumsum(indptr)
+    return indptr, c.astype(np.int32), vals

@@ -450,3 +450,94 @@ def test_cp_loss_config4_96_4way(ctx):
     assert abs(loss - loss_r) / loss_r < 1e-5
     for g, gr in zip(grads, grads_r):
         assert rel_err(g.cpu().numpy(), gr) < 1e-4
+
+
+def test_csr_col_subset_bit_exact_fixture(ctx):
+    """Column subset of a CSR matrix against scipy.sparse (committed fixture):
+    indptr, indices and values bit-exact."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "csr_subset.npz"))
+    ip = torch.from_numpy(g["indptr"]).cuda()
+    ix = torch.from_numpy(g["indices"]).cuda()
+    dv = torch.from_numpy(g["data"]).cuda()
+    optr, oidx, oval = ctx.csr_col_subset(ip, ix, dv, int(g["shape"][1]), g["cols"])
+    assert np.array_equal(optr.cpu().numpy(), g["sub_indptr"])
+    assert np.array_equal(oidx.cpu().numpy(), g["sub_indices"])
+    assert np.array_equal(oval.cpu().numpy().view(np.uint32), g["sub_data"].view(np.uint32))
+
+
+def _hier_reference(m, n, ip, ix, dv, depth, iters, seed):
+    """Level-by-level tree from the oracle's per-node rank-2 NMF and power
+    iteration, with the same Philox initial values as the device."""
+    nodes = []
+    level = [(0, np.arange(n))]
+    for lev in range(depth):
+        nxt = []
+        for idx, cols in level:
+            sd = (seed * idx) % (1 << 64)
+            rec = {"index": idx, "level": lev, "cols": cols, "sigma": 0.0}
+            if lev > 0:
+                q0 = D.normal(sd, 2, (len(cols),))
+                rec["sigma"] = O.powiter_csr(m, n, ip, ix, dv, cols, q0, iters)
+            if lev < depth - 1:
+                W0 = D.uniform(sd, 0, (m, 2))
+                H0 = D.uniform(sd, 1, (len(cols), 2))
+                W, H = O.r2_nmf_csr(m, n, ip, ix, dv, cols, W0, H0, iters)
+                rec["W"], rec["H"] = W, H
+                left = H[:, 0] >= H[:, 1]
+                L, R = cols[left], cols[~left]
+                if len(R) > len(L):
+                    L, R = R, L
+                if len(L):
+                    nxt.append((2 * idx + 1, L))
+                if len(R):
+                    nxt.append((2 * idx + 2, R))
+            nodes.append(rec)
+        level = nxt
+    return nodes
+
+
+def test_hiernmf_tree_matches_oracle(ctx):
+    """Batched level-synchronous rank-2 tree vs the oracle's node-by-node
+    computation (full m rows there, compacted rows on the device): same
+    column splits, H, W and sigma."""
+    say("hiernmf small tree")
+    m, n, depth, iters, seed = 300, 400, 4, 15, 193957
+    ip, ix, dv = D.sparse_powerlaw(m, n, 0.03, 1.1, stream=9)
+    ref = _hier_reference(m, n, ip, ix, dv.astype(np.float64), depth, iters, seed)
+    out, cols, H, W = ctx.hiernmf_run(m, n, ip, ix, dv, depth, iters, seed, want_H=True, want_W=True)
+    assert len(out) == len(ref)
+    for o, r in zip(out, ref):
+        assert o["index"] == r["index"] and o["level"] == r["level"]
+        c = cols[o["col_offset"]:o["col_offset"] + o["ncols"]]
+        assert np.array_equal(c, r["cols"])
+        if r["level"] > 0:
+            assert abs(o["sigma"] - r["sigma"]) <= 1e-9 * max(1.0, r["sigma"])
+        if "H" in r:
+            h = H[2 * o["col_offset"]:2 * (o["col_offset"] + o["ncols"])].reshape(-1, 2)
+            assert rel_err(h, r["H"]) < 1e-9
+            w = W[2 * m * o["index"]:2 * m * (o["index"] + 1)].reshape(2, m).T
+            assert rel_err(w, r["W"]) < 1e-9
+
+
+def test_hiernmf_level_invariants_medium(ctx):
+    """20000 x 60000 power-law CSR, depth 7: every level partitions the
+    columns of its parents, children are indexed 2i+1 / 2i+2 with the larger
+    side left, and scores follow Node::compute_score."""
+    say("hiernmf medium tree")
+    m, n, depth = 20000, 60000, 7
+    ip, ix, dv = D.sparse_powerlaw(m, n, 2e-4, 1.1, stream=10)
+    out, cols, _, _ = ctx.hiernmf_run(m, n, ip, ix, dv, depth, 30, 7)
+    by_index = {o["index"]: o for o in out}
+    for o in out:
+        c = cols[o["col_offset"]:o["col_offset"] + o["ncols"]]
+        assert np.all(np.diff(c) > 0)
+        kids = [by_index.get(2 * o["index"] + 1), by_index.get(2 * o["index"] + 2)]
+        kids = [k for k in kids if k is not None]
+        if kids and o["level"] < depth - 1:
+            u = np.concatenate([cols[k["col_offset"]:k["col_offset"] + k["ncols"]] for k in kids])
+            assert np.array_equal(np.sort(u), c)
+            if len(kids) == 2:
+                assert kids[0]["ncols"] >= kids[1]["ncols"]
+                assert abs(o["score"] - (kids[0]["sigma"] + kids[1]["sigma"] - o["sigma"])) < 1e-6 * (1 + abs(o["score"]))
+    assert max(o["level"] for o in out) == depth - 1

@@ -233,3 +233,43 @@ def test_cp_loss_gradient_central_differences():
             Fm[n][i, c] -= h
             fd = (O.cp_loss(X, Fp) - O.cp_loss(X, Fm)) / (2 * h)
             np.testing.assert_allclose(G[n][i, c], fd, rtol=1e-6, atol=1e-8)
+
+
+def _small_csr(m, n, density, seed):
+    import scipy.sparse as sp
+    A = sp.random(m, n, density=density, format="csr", random_state=np.random.RandomState(seed))
+    A.data = np.exp(np.random.default_rng(seed).standard_normal(A.nnz))
+    A.sort_indices()
+    return A
+
+
+def test_r2_nmf_csr_matches_dense_numpy_loop():
+    """DistAUNMF order (H given W, then W given H) with DistR2's rule, run
+    densely in NumPy with the brute-force-pinned r2_update; every half-step
+    is a KKT point of its two-variable NNLS."""
+    A = _small_csr(40, 30, 0.2, 1)
+    cols = np.array([1, 2, 5, 7, 8, 11, 12, 20, 21, 29])
+    rng = np.random.default_rng(2)
+    W0, H0 = rng.random((40, 2)), rng.random((len(cols), 2))
+    W, H = O.r2_nmf_csr(40, 30, A.indptr, A.indices, A.data, cols, W0, H0, 5)
+    Ad = A.toarray()[:, cols]
+    Wn, Hn = W0.copy(), H0.copy()
+    for _ in range(5):
+        Hn = O.r2_update(Wn.T @ Wn, Wn.T @ Ad).T
+        Wn = O.r2_update(Hn.T @ Hn, (Ad @ Hn).T).T
+    np.testing.assert_allclose(H, Hn, rtol=1e-10, atol=1e-12)
+    np.testing.assert_allclose(W, Wn, rtol=1e-10, atol=1e-12)
+    # the last W step: each row solves min ||a_i - H w|| over w >= 0 (KKT)
+    G = H.T @ H
+    Y = W @ G - Ad @ H
+    assert W.min() >= 0 and Y.min() > -1e-9 and np.abs((W * Y).sum()) < 1e-8
+
+
+def test_powiter_csr_matches_svd():
+    A = _small_csr(60, 50, 0.15, 3)
+    cols = np.arange(0, 50, 2)
+    Ad = A.toarray()[:, cols]
+    q0 = np.random.default_rng(4).standard_normal(len(cols))
+    s = O.powiter_csr(60, 50, A.indptr, A.indices, A.data, cols, q0, 300)
+    sv = np.linalg.svd(Ad, compute_uv=False)
+    np.testing.assert_allclose(s, sv[0] ** 2, rtol=1e-8)
