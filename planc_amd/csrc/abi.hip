@@ -112,6 +112,41 @@ int mttkrp_direct(planc_ctx *ctx, int dtype, const void *X, int N, const long *d
   return contract_projection(ctx, P, Lp, n + 1, dims, F, ldf, n, k, out, ldo);
 }
 
+__global__ void counter_set_kernel(int *c, int v) { *c = v; }
+__global__ void counter_append_kernel(const double *src, double *dst, int *c) {
+  dst[*c] = *src;
+  *c += 1;
+}
+
+int counter_set(planc_ctx *ctx, int *c, int v) {
+  counter_set_kernel<<<1, 1, 0, ctx->stream>>>(c, v);
+  PLANC_LAUNCH_CHECK(ctx);
+  return PLANC_OK;
+}
+
+int counter_append(planc_ctx *ctx, const double *src, double *dst, int *c) {
+  counter_append_kernel<<<1, 1, 0, ctx->stream>>>(src, dst, c);
+  PLANC_LAUNCH_CHECK(ctx);
+  return PLANC_OK;
+}
+
+bool graph_lookup(planc_ctx *ctx, const std::vector<uintptr_t> &key, hipGraphExec_t *exec) {
+  for (auto &g : ctx->graphs)
+    if (g.key == key) {
+      *exec = g.exec;
+      return true;
+    }
+  return false;
+}
+
+void graph_store(planc_ctx *ctx, const std::vector<uintptr_t> &key, hipGraphExec_t exec) {
+  if (ctx->graphs.size() >= 16) {
+    (void)hipGraphExecDestroy(ctx->graphs.front().exec);
+    ctx->graphs.erase(ctx->graphs.begin());
+  }
+  ctx->graphs.push_back({key, exec});
+}
+
 }  // namespace
 
 extern "C" {
@@ -142,6 +177,7 @@ int planc_destroy(planc_ctx *ctx) {
   if (!ctx) return PLANC_EINVAL;
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
+  for (auto &g : ctx->graphs) (void)hipGraphExecDestroy(g.exec);
   for (auto &w : ctx->ws)
     if (w.ptr) (void)hipFree(w.ptr);
   if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
@@ -510,7 +546,7 @@ int planc_ntf_run(planc_ctx *ctx, const planc_ntf_desc *desc, const float *d_X, 
   }
   float *fbuf = (float *)ws_get(ctx, 4, sizeof(float) * ((size_t)Pmax + (size_t)k * Imax));
   const size_t kk = (size_t)k * k;
-  double *dbuf = (double *)ws_get(ctx, 5, sizeof(double) * ((N + 2) * kk + 8 + k + (size_t)iters));
+  double *dbuf = (double *)ws_get(ctx, 5, sizeof(double) * ((N + 2) * kk + 8 + k + (size_t)iters + 1));
   if (!fbuf || !dbuf) return set_error(ctx, PLANC_ENOMEM, "planc_ntf_run: workspace");
   float *P = fbuf, *Q = dimtree ? fbuf + (size_t)L * k : nullptr;
   float *M = fbuf + Pmax;
@@ -543,7 +579,11 @@ int planc_ntf_run(planc_ctx *ctx, const planc_ntf_desc *desc, const float *d_X, 
     ldfL[i] = dims[i];
     ldfR[i] = dims[i];
   }
-  for (int it = 0; it < iters; ++it) {
+  // One outer iteration (all mode updates + the error), written so the same
+  // launch sequence can be captured into a hipGraph: no allocation, no host
+  // synchronisation, the error written to *rel_out.
+  auto one_iteration = [&](double *rel_out) -> int {
+    int rc;
     for (int n = 0; n < N; ++n) {
       RC(hadamard_grams(ctx, Gc.data(), N, n, k, S));
       if (dimtree) {
@@ -589,7 +629,42 @@ int planc_ntf_run(planc_ctx *ctx, const planc_ntf_desc *desc, const float *d_X, 
     }
     if (desc->compute_error)
       RC(ntf_relerr(ctx, normX2, d_factors[N - 1], dims[N - 1], dims[N - 1], M, k, k, d_lambda,
-                    Gc.data(), N, rel + it));
+                    Gc.data(), N, rel_out));
+    return PLANC_OK;
+  };
+  if (iters > 0) RC(one_iteration(rel));
+  if (iters > 1) {
+    // Iterations 1.. replay a captured graph of one iteration (launch-bound
+    // at small shapes: ~30 kernels per iteration); the error goes through a
+    // device counter into rel[1..].
+    double *relslot = rel + iters;  // one spare slot after rel[]
+    int *counter = (int *)ws_get(ctx, 15, sizeof(int));
+    if (!counter) return set_error(ctx, PLANC_ENOMEM, "planc_ntf_run: workspace");
+    RC(counter_set(ctx, counter, 1));
+    hipGraphExec_t exec = nullptr;
+    std::vector<uintptr_t> key = {1, (uintptr_t)N, (uintptr_t)k, (uintptr_t)algo, (uintptr_t)dt,
+                                  (uintptr_t)iters,
+                                  (uintptr_t)dimtree, (uintptr_t)desc->compute_error,
+                                  (uintptr_t)Xop, (uintptr_t)d_lambda, (uintptr_t)rel};
+    for (int i = 0; i < N; ++i) {
+      key.push_back((uintptr_t)d_factors[i]);
+      key.push_back((uintptr_t)dims[i]);
+    }
+    for (auto &w : ctx->ws) key.push_back((uintptr_t)w.ptr);
+    if (!graph_lookup(ctx, key, &exec)) {
+      hipGraph_t graph;
+      PLANC_HIP_CHECK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+      int crc = one_iteration(relslot);
+      if (crc == PLANC_OK && desc->compute_error) crc = counter_append(ctx, relslot, rel, counter);
+      hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
+      if (crc) return crc;
+      if (ce != hipSuccess) return set_error(ctx, PLANC_EHIP, std::string("ntf capture: ") + hipGetErrorString(ce));
+      ce = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (ce != hipSuccess) return set_error(ctx, PLANC_EHIP, std::string("ntf instantiate: ") + hipGetErrorString(ce));
+      graph_store(ctx, key, exec);
+    }
+    for (int it = 1; it < iters; ++it) PLANC_HIP_CHECK(ctx, hipGraphLaunch(exec, ctx->stream));
   }
 #undef RC
   int hflag = 0;

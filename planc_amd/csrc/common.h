@@ -6,6 +6,7 @@
 #include <stdint.h>
 
 #include <string>
+#include <vector>
 
 #include "../../include/planc_hip.h"
 
@@ -81,7 +82,13 @@ struct Workspace {
 
 }  // namespace planc
 
+struct planc_graph_entry {
+  std::vector<uintptr_t> key;
+  hipGraphExec_t exec;
+};
+
 struct planc_ctx {
+  std::vector<planc_graph_entry> graphs;  // instantiated loop graphs, keyed by shapes and pointers
   int device = 0;
   hipStream_t stream = nullptr;
   bool own_stream = false;
@@ -98,7 +105,7 @@ namespace planc {
 // partials, 3 dot partials, 4 loop float buffers, 5 loop double buffers,
 // 7 16-bit copy of A / X, 8 column scales of that copy, 9 entry-point norms,
 // 10 BPP per-column iteration counts, 11 BPP failure flag, 12 CP-loss scalar,
-// 13 CP-gradient float buffers, 14 CP-gradient Grams.
+// 13 CP-gradient float buffers, 14 CP-gradient Grams, 15 loop-graph counter.
 void *ws_get(planc_ctx *ctx, int slot, size_t bytes);
 
 int set_error(planc_ctx *ctx, int code, const std::string &msg);

@@ -178,42 +178,49 @@ __global__ __launch_bounds__(NTHR) void hals_cols_coop_kernel(float *W, long m, 
 #pragma unroll
     for (int t = 0; t < KM; ++t) w[r][t] = (i < m && t < k) ? W[i + ldw * t] : 0.0f;
   }
+  // The column loop is not unrolled (it holds a grid barrier); registers are
+  // only indexed by the unrolled t loops, w[r][x] is read and written through
+  // selects.
+  for (int x = 0; x < k; ++x) {
+    float u[RPT];
+    double lsum = 0.0;
+    const float gxx = Gs[x][x];
 #pragma unroll
-  for (int x = 0; x < KM; ++x) {
-    if (x < k) {
-      float u[RPT];
-      double lsum = 0.0;
-      const float gxx = Gs[x][x];
+    for (int r = 0; r < RPT; ++r) {
+      long i = gtid + r * total;
+      u[r] = 0.0f;
+      float s = 0.0f, wx = 0.0f;
+#pragma unroll
+      for (int t = 0; t < KM; ++t) {
+        s = fmaf(w[r][t], Gs[t][x], s);
+        wx = (t == x) ? w[r][t] : wx;
+      }
+      if (i < m) {
+        float v = (wx * gxx) + (AH[i + ldah * x] - s);
+        v = (v < floor_v) ? floor_v : v;
+        u[r] = v;
+        lsum += (double)v * (double)v;
+      }
+    }
+    double bs = block_sum(lsum, red);
+    if (threadIdx.x == 0) part[(size_t)x * gridDim.x + blockIdx.x] = bs;
+    grid.sync();
+    double tot = 0.0;
+    if (threadIdx.x < 64) {
+      for (long b = threadIdx.x; b < gridDim.x; b += 64) tot += part[(size_t)x * gridDim.x + b];
+      tot = wave_sum(tot);
+      if (threadIdx.x == 0) red[0] = tot;
+    }
+    __syncthreads();
+    tot = red[0];
+    __syncthreads();
+    double nrm = sqrt(tot);
+    if (nrm != 0.0) {
 #pragma unroll
       for (int r = 0; r < RPT; ++r) {
-        long i = gtid + r * total;
-        u[r] = 0.0f;
-        if (i < m) {
-          float s = 0.0f;
+        float nv = (float)((double)u[r] / nrm);
 #pragma unroll
-          for (int t = 0; t < KM; ++t) s = fmaf(w[r][t], Gs[t][x], s);
-          float v = (w[r][x] * gxx) + (AH[i + ldah * x] - s);
-          v = (v < floor_v) ? floor_v : v;
-          u[r] = v;
-          lsum += (double)v * (double)v;
-        }
-      }
-      double bs = block_sum(lsum, red);
-      if (threadIdx.x == 0) part[(size_t)x * gridDim.x + blockIdx.x] = bs;
-      grid.sync();
-      double tot = 0.0;
-      if (threadIdx.x < 64) {
-        for (long b = threadIdx.x; b < gridDim.x; b += 64) tot += part[(size_t)x * gridDim.x + b];
-        tot = wave_sum(tot);
-        if (threadIdx.x == 0) red[0] = tot;
-      }
-      __syncthreads();
-      tot = red[0];
-      __syncthreads();
-      double nrm = sqrt(tot);
-      if (nrm != 0.0) {
-#pragma unroll
-        for (int r = 0; r < RPT; ++r) w[r][x] = (float)((double)u[r] / nrm);
+        for (int t = 0; t < KM; ++t) w[r][t] = (t == x) ? nv : w[r][t];
       }
     }
   }
