@@ -178,6 +178,7 @@ int planc_destroy(planc_ctx *ctx) {
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   for (auto &g : ctx->graphs) (void)hipGraphExecDestroy(g.exec);
+  if (ctx->capture_stream) (void)hipStreamDestroy(ctx->capture_stream);
   for (auto &w : ctx->ws)
     if (w.ptr) (void)hipFree(w.ptr);
   if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
@@ -652,16 +653,11 @@ int planc_ntf_run(planc_ctx *ctx, const planc_ntf_desc *desc, const float *d_X, 
     }
     for (auto &w : ctx->ws) key.push_back((uintptr_t)w.ptr);
     if (!graph_lookup(ctx, key, &exec)) {
-      hipGraph_t graph;
-      PLANC_HIP_CHECK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-      int crc = one_iteration(relslot);
-      if (crc == PLANC_OK && desc->compute_error) crc = counter_append(ctx, relslot, rel, counter);
-      hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
-      if (crc) return crc;
-      if (ce != hipSuccess) return set_error(ctx, PLANC_EHIP, std::string("ntf capture: ") + hipGetErrorString(ce));
-      ce = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(graph);
-      if (ce != hipSuccess) return set_error(ctx, PLANC_EHIP, std::string("ntf instantiate: ") + hipGetErrorString(ce));
+      RC(capture_graph(ctx, [&]() -> int {
+        int crc = one_iteration(relslot);
+        if (crc == PLANC_OK && desc->compute_error) crc = counter_append(ctx, relslot, rel, counter);
+        return crc;
+      }, &exec));
       graph_store(ctx, key, exec);
     }
     for (int it = 1; it < iters; ++it) PLANC_HIP_CHECK(ctx, hipGraphLaunch(exec, ctx->stream));

@@ -89,6 +89,7 @@ struct planc_graph_entry {
 
 struct planc_ctx {
   std::vector<planc_graph_entry> graphs;  // instantiated loop graphs, keyed by shapes and pointers
+  hipStream_t capture_stream = nullptr;   // private stream graphs are captured on
   int device = 0;
   hipStream_t stream = nullptr;
   bool own_stream = false;
@@ -119,6 +120,38 @@ int set_error(planc_ctx *ctx, int code, const std::string &msg);
   } while (0)
 
 #define PLANC_LAUNCH_CHECK(ctx) PLANC_HIP_CHECK(ctx, hipGetLastError())
+
+// Captures everything `record` launches on ctx->stream into a graph: the
+// work is recorded on the context's private capture stream (the caller's
+// stream may be the legacy NULL stream, which cannot be captured) and the
+// instantiated graph is returned for launching on ctx->stream.
+template <class F>
+int capture_graph(planc_ctx *ctx, F record, hipGraphExec_t *exec) {
+  if (!ctx->capture_stream) {
+    if (hipStreamCreateWithFlags(&ctx->capture_stream, hipStreamNonBlocking) != hipSuccess)
+      return set_error(ctx, PLANC_EHIP, "capture stream");
+  }
+  hipStream_t saved = ctx->stream;
+  ctx->stream = ctx->capture_stream;
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) {
+    ctx->stream = saved;
+    return set_error(ctx, PLANC_EHIP, std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
+  }
+  int rc = record();
+  e = hipStreamEndCapture(ctx->stream, &graph);
+  ctx->stream = saved;
+  if (rc) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess) return set_error(ctx, PLANC_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  e = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) return set_error(ctx, PLANC_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  return PLANC_OK;
+}
 
 // Kernel-family entry points shared between translation units.
 struct KrpDesc {

@@ -918,36 +918,30 @@ int planc_hiernmf_run(planc_ctx *ctx, const planc_hier_desc *desc, const long *h
     PLANC_LAUNCH_CHECK(ctx);
     const int Gw = E.tune_group(nrows, LONGLEN);
     if (iters > 0) {
-      hipGraph_t graph;
-      hipGraphExec_t exec;
-      PLANC_HIP_CHECK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-      for (int it = 0; it < iters; ++it) {
-        if (it > 0) {
-          if (!chr.empty())
-            gram2_chunks<<<(unsigned)chr.size(), 256, 0, ctx->stream>>>(E.ch_rows.p, E.W.p, E.part.p);
-          gram2_finalize<<<nb(nn), 256, 0, ctx->stream>>>(E.part.p, E.cb_rows.p, nn, E.Gw.p);
+      hipGraphExec_t exec = nullptr;
+      RC(capture_graph(ctx, [&]() -> int {
+        for (int it = 0; it < iters; ++it) {
+          if (it > 0) {
+            if (!chr.empty())
+              gram2_chunks<<<(unsigned)chr.size(), 256, 0, ctx->stream>>>(E.ch_rows.p, E.W.p, E.part.p);
+            gram2_finalize<<<nb(nn), 256, 0, ctx->stream>>>(E.part.p, E.cb_rows.p, nn, E.Gw.p);
+          }
+          h_update<<<nb(nlev), 256, 0, ctx->stream>>>(nlev, E.col_ptr.p, E.csc_row.p, E.csc_val.p,
+                                                     E.node_of_col.p, E.W.p, E.Gw.p, E.H.p);
+          gram2_chunks<<<(unsigned)chc.size(), 256, 0, ctx->stream>>>(E.ch_cols.p, E.H.p, E.part.p);
+          gram2_finalize<<<nb(nn), 256, 0, ctx->stream>>>(E.part.p, E.cb_cols.p, nn, E.Gh.p);
+          E.launch_w_short(Gw, nrows, E.W.p, LONGLEN);
+          if (nlong)
+            w_update_long<<<(unsigned)nlong, 256, 0, ctx->stream>>>(E.llist.p, E.row_ptr.p, E.csr_col.p,
+                                                                    E.csr_val.p, E.row_node.p, E.H.p,
+                                                                    E.Gh.p, E.W.p);
         }
-        h_update<<<nb(nlev), 256, 0, ctx->stream>>>(nlev, E.col_ptr.p, E.csc_row.p, E.csc_val.p,
-                                                   E.node_of_col.p, E.W.p, E.Gw.p, E.H.p);
-        gram2_chunks<<<(unsigned)chc.size(), 256, 0, ctx->stream>>>(E.ch_cols.p, E.H.p, E.part.p);
-        gram2_finalize<<<nb(nn), 256, 0, ctx->stream>>>(E.part.p, E.cb_cols.p, nn, E.Gh.p);
-        E.launch_w_short(Gw, nrows, E.W.p, LONGLEN);
-        if (nlong)
-          w_update_long<<<(unsigned)nlong, 256, 0, ctx->stream>>>(E.llist.p, E.row_ptr.p, E.csr_col.p,
-                                                                  E.csr_val.p, E.row_node.p, E.H.p,
-                                                                  E.Gh.p, E.W.p);
-      }
-      hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
-      if (ce != hipSuccess) return set_error(ctx, PLANC_EHIP, std::string("hiernmf capture: ") + hipGetErrorString(ce));
-      ce = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-      if (ce != hipSuccess) {
-        (void)hipGraphDestroy(graph);
-        return set_error(ctx, PLANC_EHIP, std::string("hiernmf instantiate: ") + hipGetErrorString(ce));
-      }
-      ce = hipGraphLaunch(exec, ctx->stream);
+        PLANC_LAUNCH_CHECK(ctx);
+        return PLANC_OK;
+      }, &exec));
+      hipError_t ce = hipGraphLaunch(exec, ctx->stream);
       (void)hipStreamSynchronize(ctx->stream);
       (void)hipGraphExecDestroy(exec);
-      (void)hipGraphDestroy(graph);
       if (ce != hipSuccess) return set_error(ctx, PLANC_EHIP, std::string("hiernmf launch: ") + hipGetErrorString(ce));
     }
     // Split rule H(:,0) >= H(:,1), larger side to the left child
