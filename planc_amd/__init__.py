@@ -46,10 +46,15 @@ def empty_colmajor(shape, dtype=None, device="cuda"):
 
 
 def _ld(t):
-    """Leading dimension of a column-major 2-D tensor."""
-    if t.dim() != 2 or t.stride(0) != 1:
+    """Leading dimension of a column-major 2-D tensor (element (i, j) at
+    i + ld*j); the stride of a dimension of extent 1 is irrelevant."""
+    if t.dim() != 2:
+        raise ValueError("expected a 2-D tensor")
+    m, n = t.shape
+    if m > 1 and t.stride(0) != 1:
         raise ValueError("expected a column-major 2-D tensor (stride(0) == 1)")
-    return max(t.stride(1), t.shape[0])
+    ld = t.stride(1) if n > 1 else m
+    return max(ld, m, 1)
 
 
 def _ptr(t):
