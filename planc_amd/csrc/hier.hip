@@ -1058,8 +1058,7 @@ int planc_csr_col_subset(planc_ctx *ctx, long m, long n, const long *d_indptr, c
   long total = 0;
   if ((rc = E.scan(E.row_len.p, m, d_out_indptr, &total))) return rc;
   PLANC_HIP_CHECK(ctx, hipMemcpyAsync(d_out_indptr + m, &total, sizeof(long), hipMemcpyHostToDevice, ctx->stream));
-  if (total > 0) {
-    if (!d_out_indices || !d_out_data) return set_error(ctx, PLANC_EINVAL, "planc_csr_col_subset: null outputs");
+  if (total > 0 && d_out_indices && d_out_data) {
     subset_fill<<<nb(m), 256, 0, ctx->stream>>>(m, d_indptr, d_indices, d_data, E.colmap.p, d_out_indptr,
                                                d_out_indices, d_out_data);
     PLANC_LAUNCH_CHECK(ctx);
