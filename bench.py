@@ -127,6 +127,24 @@ def nmf_leg(P, D, ctx, torch, algo, m, n, k, iters, stream=3):
     return out
 
 
+def hier_leg(P, D, ctx, m=100000, n=500000, density=1e-4, depth=11, iters=100):
+    """Batched hierarchical NMF on a power-law term-document CSR (host CSR in,
+    one level-batched launch sequence per tree level)."""
+    t0 = time.perf_counter()
+    indptr, indices, data = D.sparse_powerlaw(m, n, density, 1.0, 7)
+    gen_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    nodes, cols, _, _ = ctx.hiernmf_run(m, n, indptr, indices, data, depth, iters, 193957)
+    el = time.perf_counter() - t0
+    sizes = [x["ncols"] for x in nodes]
+    return {"seconds": round(el, 4), "nodes": len(nodes), "nodes_per_s": round(len(nodes) / el, 1),
+            "nnz": int(indptr[-1]), "iters_per_node": iters,
+            "node_cols_min": int(min(sizes)), "node_cols_max": int(max(sizes)),
+            "generate_s": round(gen_s, 2),
+            "config": f"{m}x{n} CSR, density {density}, log-normal values, power-law rows, "
+                      f"depth {depth}, {iters} iterations per node (host CSR in, tree out)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -269,6 +287,9 @@ def main():
         log("bench: NMF 65536x16384 k=64 fp32 vs bf16")
         legs["nmf_mu_65536x16384_k64"] = nmf_leg(P, D, ctx, torch, P.MU, 65536, 16384, 64, 20)
         legs["nmf_hals_65536x16384_k64"] = nmf_leg(P, D, ctx, torch, P.HALS, 65536, 16384, 64, 20)
+
+        log("bench: hierarchical rank-2 NMF, 100000x500000 CSR, depth 11")
+        legs["hiernmf_100000x500000_depth11"] = hier_leg(P, D, ctx)
 
     line = {
         "metric": METRIC,

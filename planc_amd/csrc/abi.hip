@@ -413,6 +413,21 @@ int planc_nmf_run(planc_ctx *ctx, const planc_nmf_desc *desc, const float *d_A, 
       ldop = m;
     }
   }
+  // 16-bit: a transposed copy makes A*H a TN product too (its contraction
+  // index n contiguous), so both products stream with 16-byte MFMA operands.
+  const void *AopT = nullptr;
+  if (dt != PLANC_F32) {
+    void *at = ws_get(ctx, 16, sizeof(uint16_t) * (size_t)m * n);
+    if (!at) return set_error(ctx, PLANC_ENOMEM, "planc_nmf_run: transposed 16-bit copy");
+    int rc = transpose16(ctx, Aop, m, n, ldop, at, n);
+    if (rc) return rc;
+    AopT = at;
+  }
+  // A*H (m x k, ld m).
+  auto a_times_h = [&](float *out) -> int {
+    if (AopT) return thin_gemm(ctx, dt, 1, AopT, n, m, n, d_H, n, nullptr, cs, nullptr, k, out, m, 0, 0);
+    return thin_gemm(ctx, dt, 0, Aop, ldop, m, n, d_H, n, nullptr, cs, nullptr, k, out, m, 0, 0);
+  };
   // Loop buffers.
   float *fbuf = (float *)ws_get(ctx, 4, sizeof(float) * ((size_t)k * n + (size_t)m * k));
   const size_t kk = (size_t)k * k;
@@ -441,7 +456,7 @@ int planc_nmf_run(planc_ctx *ctx, const planc_nmf_desc *desc, const float *d_A, 
       RC(gram(ctx, d_H, n, k, n, HtH));
       PLANC_HIP_CHECK(ctx, hipMemcpyAsync(Greg, HtH, kk * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
       RC(apply_reg(ctx, Greg, k, desc->regW));
-      RC(thin_gemm(ctx, dt, 0, Aop, ldop, m, n, d_H, n, nullptr, cs, nullptr, k, AH, m, 0, 0));
+      RC(a_times_h(AH));
       RC(bpp_solve(ctx, Greg, k, AH, 1, m, m, d_W, 1, m, dflag));
       RC(gram(ctx, d_W, m, k, m, WtW));
       PLANC_HIP_CHECK(ctx, hipMemcpyAsync(Greg, WtW, kk * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
@@ -473,7 +488,7 @@ int planc_nmf_run(planc_ctx *ctx, const planc_nmf_desc *desc, const float *d_A, 
     RC(apply_reg(ctx, Greg, k, desc->regH));
     RC(row_update(ctx, algo == PLANC_MU ? 0 : 1, d_H, n, k, n, WtA, k, 1, Greg, 1e-17f));
     // W half-step: A H, H^T H (+ regW).
-    RC(thin_gemm(ctx, dt, 0, Aop, ldop, m, n, d_H, n, nullptr, cs, nullptr, k, AH, m, 0, 0));
+    RC(a_times_h(AH));
     RC(gram(ctx, d_H, n, k, n, HtH));
     PLANC_HIP_CHECK(ctx, hipMemcpyAsync(Greg, HtH, kk * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
     RC(apply_reg(ctx, Greg, k, desc->regW));
@@ -533,6 +548,16 @@ int planc_ntf_run(planc_ctx *ctx, const planc_ntf_desc *desc, const float *d_X, 
       Xop = x16;
     }
   }
+  // 16-bit dimension tree: the transposed copy (R x L) turns the right
+  // contraction X_(LxR) * KRP_R into a TN product with contiguous k.
+  const void *XopT = nullptr;
+  if (dt != PLANC_F32 && dimtree) {
+    void *xt = ws_get(ctx, 16, sizeof(uint16_t) * (size_t)numel);
+    if (!xt) return set_error(ctx, PLANC_ENOMEM, "planc_ntf_run: transposed 16-bit copy");
+    int rc = transpose16(ctx, Xop, L, R, L, xt, R);
+    if (rc) return rc;
+    XopT = xt;
+  }
   // Float scratch: P (L x k) and Q (R x k) for the tree (or the direct
   // middle-mode projection), M (k x Imax).
   long Pmax = 0;
@@ -585,8 +610,13 @@ int planc_ntf_run(planc_ctx *ctx, const planc_ntf_desc *desc, const float *d_X, 
   // synchronisation, the error written to *rel_out.
   auto one_iteration = [&](double *rel_out) -> int {
     int rc;
+    bool err_done = false;
     for (int n = 0; n < N; ++n) {
-      RC(hadamard_grams(ctx, Gc.data(), N, n, k, S));
+      const bool fused = mode_update_fits(dims[n], k);
+      if (!fused || algo == PLANC_ANLSBPP) RC(hadamard_grams(ctx, Gc.data(), N, n, k, S));
+      // This mode's MTTKRP: element (x, i) at Mp[i * mrs + x * mcs].
+      const float *Mp = M;
+      long mrs = k, mcs = 1;
       if (dimtree) {
         if (n == 0) {
           KrpDesc kd{};
@@ -596,7 +626,10 @@ int planc_ntf_run(planc_ctx *ctx, const planc_ntf_desc *desc, const float *d_X, 
             kd.dim[q - s - 1] = dims[q];
             kd.ld[q - s - 1] = dims[q];
           }
-          RC(thin_gemm(ctx, dt, 0, Xop, L, L, R, nullptr, 0, &kd, nullptr, nullptr, k, P, L, 0, 0));
+          if (XopT)
+            RC(thin_gemm(ctx, dt, 1, XopT, R, L, R, nullptr, 0, &kd, nullptr, nullptr, k, P, L, 0, 0));
+          else
+            RC(thin_gemm(ctx, dt, 0, Xop, L, L, R, nullptr, 0, &kd, nullptr, nullptr, k, P, L, 0, 0));
         }
         if (n == s + 1) {
           KrpDesc kd{};
@@ -608,15 +641,33 @@ int planc_ntf_run(planc_ctx *ctx, const planc_ntf_desc *desc, const float *d_X, 
           }
           RC(thin_gemm(ctx, dt, 1, Xop, L, R, L, nullptr, 0, &kd, nullptr, nullptr, k, Q, R, 0, 0));
         }
-        if (n <= s)
+        if (n <= s) {
           RC(contract_projection(ctx, P, L, s + 1, dims, d_factors, ldfL, n, k, M, k));
-        else
+        } else if (fused && N - 1 - s == 1) {
+          // A one-mode right block: the projection is the MTTKRP itself (R x k).
+          Mp = Q;
+          mrs = 1;
+          mcs = R;
+        } else {
           RC(contract_projection(ctx, Q, R, N - 1 - s, dims + s + 1, d_factors + s + 1, ldfR + s + 1,
                                  n - s - 1, k, M, k));
+        }
       } else {
         RC(mttkrp_direct(ctx, dt, Xop, N, dims, d_factors, k, n, M, k, P));
       }
       float *Fn = d_factors[n];
+      if (fused) {
+        int kind = algo == PLANC_HALS ? 1 : 0;
+        if (algo == PLANC_ANLSBPP) {
+          RC(bpp_solve(ctx, S, k, Mp, mrs, mcs, dims[n], Fn, 1, dims[n], dflag));
+          kind = 2;
+        }
+        const bool last_err = desc->compute_error && n == N - 1;
+        RC(mode_update(ctx, kind, 1e-6f, Fn, dims[n], dims[n], k, Mp, mrs, mcs, Gc.data(), N, n, G[n],
+                       d_lambda, normX2, last_err ? rel_out : nullptr));
+        err_done = err_done || last_err;
+        continue;
+      }
       if (algo == PLANC_HALS)
         RC(row_update(ctx, 1, Fn, dims[n], k, dims[n], M, k, 1, S, 0.0f));
       else if (algo == PLANC_MU)
@@ -628,7 +679,7 @@ int planc_ntf_run(planc_ctx *ctx, const planc_ntf_desc *desc, const float *d_X, 
       RC(scale_cols(ctx, Fn, dims[n], dims[n], nullptr, 0, 0, k, d_lambda));
       RC(gram(ctx, Fn, dims[n], k, dims[n], G[n]));
     }
-    if (desc->compute_error)
+    if (desc->compute_error && !err_done)
       RC(ntf_relerr(ctx, normX2, d_factors[N - 1], dims[N - 1], dims[N - 1], M, k, k, d_lambda,
                     Gc.data(), N, rel_out));
     return PLANC_OK;

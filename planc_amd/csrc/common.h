@@ -6,6 +6,7 @@
 #include <stdint.h>
 
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/planc_hip.h"
@@ -94,9 +95,11 @@ struct planc_ctx {
   hipStream_t stream = nullptr;
   bool own_stream = false;
   std::string last_error;
-  planc::Workspace ws[16];
+  planc::Workspace ws[24];
   int num_cu = 256;
   int max_coop_blocks_hals = 0;
+  // Tuned streaming-product geometries: shape key -> {cw, wk, nsplit}.
+  std::vector<std::pair<std::vector<long>, std::vector<long>>> gemm_tuned;
 };
 
 namespace planc {
@@ -104,9 +107,10 @@ namespace planc {
 // Returns a device buffer of at least `bytes` for workspace slot `slot`.
 // Slots: 0 split-K partials, 1 Gram partials, 2 column-norm / HALS-sweep
 // partials, 3 dot partials, 4 loop float buffers, 5 loop double buffers,
-// 7 16-bit copy of A / X, 8 column scales of that copy, 9 entry-point norms,
+// 6 packed 16-bit thin operand, 7 16-bit copy of A / X, 8 column scales of that copy, 9 entry-point norms,
 // 10 BPP per-column iteration counts, 11 BPP failure flag, 12 CP-loss scalar,
-// 13 CP-gradient float buffers, 14 CP-gradient Grams, 15 loop-graph counter.
+// 13 CP-gradient float buffers, 14 CP-gradient Grams, 15 loop-graph counter,
+// 16 transposed 16-bit copy of A / X.
 void *ws_get(planc_ctx *ctx, int slot, size_t bytes);
 
 int set_error(planc_ctx *ctx, int code, const std::string &msg);
@@ -189,6 +193,8 @@ int nmf_relerr(planc_ctx *ctx, const double *normA2, const double *dotv, const d
 int apply_reg(planc_ctx *ctx, double *G, int k, const double *reg);
 int convert16(planc_ctx *ctx, int dtype, const float *A, long rows, long cols, long lda, void *out,
               long ldo, float *colscale);
+// out (cols x rows) = in^T, 16-bit elements.
+int transpose16(planc_ctx *ctx, const void *in, long rows, long cols, long ldi, void *out, long ldo);
 
 // bpp.hip: column j of the right-hand side is B[j*b_cs + i*b_rs]; the
 // solution goes to X[j*x_cs + i*x_rs]; *d_failed |= (some column > 5k pivots).
@@ -211,6 +217,12 @@ int ntf_relerr(planc_ctx *ctx, const double *normX2, const float *F, long I, lon
                double *rel_out);
 int lambda_mul(planc_ctx *ctx, double *lambda, const double *norms, int k, int init);
 int transpose_to_k_major(planc_ctx *ctx, const float *in, long I, long ldi, int k, float *out, long ldo);
+// Fused AUNTF mode update (Hadamard, row update, normalize(mode), Gram, error)
+// for a factor with k * (I + 1) <= 24576; kind 0 MU, 1 HALS, 2 none.
+bool mode_update_fits(long I, int k);
+int mode_update(planc_ctx *ctx, int kind, float eps, float *F, long I, long ldf, int k, const float *M,
+                long mrs, long mcs, const double *const *grams, int ng, int skip, double *gout,
+                double *lambda, const double *normX2, double *rel_out);
 
 }  // namespace planc
 

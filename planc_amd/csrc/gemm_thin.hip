@@ -6,264 +6,123 @@
 // A is the big column-major operand (FP32, BF16 or FP16) streamed from HBM
 // exactly once per product; B is the thin K x k operand, either a factor
 // matrix or a Khatri-Rao product of up to six factors generated on the fly
-// (never stored).  k <= 64 per launch (wider k loops over 64-column slices).
+// (never stored in FP32).  k <= 64 per launch (wider k loops over 64-column
+// slices).
 //
-// Block = 4 wavefronts, tile = 128 rows of the MFMA A-operand x all k columns.
-// Each K-chunk is staged through LDS (double buffered, one barrier per
-// chunk): the A tile with 16-byte coalesced loads, the B tile computed by the
-// block.  FP32 uses v_mfma_f32_16x16x4f32, 16-bit inputs
-// v_mfma_f32_16x16x32_{bf16,f16}, all accumulating in FP32.  When the long
-// dimension alone gives too few blocks, K is split across blocks and the
-// partials are summed by a second kernel in fixed split order (deterministic,
-// no atomics).
+// Streaming kernels (FP32 NN, FP32 TN, 16-bit TN).  The big operand goes
+// straight from HBM into MFMA operand registers, 16 bytes per lane, with no
+// LDS staging: each lane's 16-byte word holds 4 (FP32) or 8 (16-bit)
+// consecutive elements of one column, and the MFMA row/k slots are
+// permuted so that those elements are exactly what the instruction needs.
+//   FP32 TN, v_mfma_f32_16x16x4f32: lane (lc = lane%16, lg = lane/16) holds
+//     A(kb + 4lg + j, l0 + lc), j = 0..3; the j-th MFMA contracts
+//     k = kb + 4g + j in k-slot g, so one load feeds 4 MFMAs per 16 columns
+//     of B^T (B(kb + 4lg + j, c) comes from one 16-byte load of B's column).
+//   FP32 NN: lane holds A(l0 + 4lc + j, kb + 4lg + u) for load u = 0..3; the
+//     (u, j) MFMA has row slot i <-> row l0 + 4i + j, k-slot g <-> kb + 4g + u.
+//     Output row l0 + 16lg + 4v + j then sits in lane (lc, lg) at D_j[v].
+//   16-bit TN, v_mfma_f32_16x16x32_{bf16,f16}: lane holds A(kb + 8lg .. +7,
+//     l0 + lc) -- exactly the MFMA B-operand layout; the thin operand is
+//     pre-packed once per product into the A-operand layout (16-bit, 16 B
+//     per lane per 16 columns per 32 k) by pack_thin16_kernel.
+// A wave owns 16*CW columns (TN) or 64 rows (NN).  The four waves of a block
+// either take different column/row groups over the same k range (the thin
+// operand then hits in L1 for three of them) or split the k range of one
+// group in an interleaved order and are summed in LDS in fixed order.  When
+// that still leaves too few waves, blockIdx.y splits K across blocks: each
+// block writes its partial tile and a second kernel sums the partials in
+// split order (deterministic, no atomics).
+//
+// 16-bit NN (a standalone product on a 16-bit copy without its transpose)
+// uses an LDS-transposing kernel with a separate split-K reduction.
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
 #include "common.h"
 
 namespace planc {
 
 namespace {
 
-constexpr int BL = 128;
 constexpr int NTHR = 256;
 
-struct KrpDev {
-  int nf;
-  const float *f[6];
-  int dim[6];
-  long ld[6];
-};
-
-struct GemmParams {
-  const void *A;
-  long lda, L, K;
+// ---------------------------------------------------------------------------
+// Thin operand: B(g, c) for g < K, c < k.
+struct ThinDev {
   const float *B;
   long ldb;
-  KrpDev krp;
   const float *rowscale;
-  const float *colscale;
+  int nf;
+  const float *f[6];
+  unsigned dim[6];
+  long ld[6];
   int k;
-  float *C;
-  long ldc;
-  int cmajor;  // 1: C is k x L (C[c + ldc*l]); 0: C is L x k (C[l + ldc*c])
-  float *partial;
-  int splits;
-  long kchunk;
+  int vec;  // 16-byte loads valid at 4-aligned g (alignment and shape checked on the host)
 };
 
-__device__ inline void krp_digits(const KrpDev &kd, long g, int *d) {
+__device__ inline float thin_at(const ThinDev &t, unsigned g, int c) {
+  float v;
+  if (t.nf == 0) {
+    v = t.B[g + t.ldb * c];
+  } else {
+    v = 1.0f;
+    unsigned rem = g;
 #pragma unroll
-  for (int q = 0; q < 6; ++q) {
-    if (q < kd.nf) {
-      d[q] = (int)(g % kd.dim[q]);
-      g /= kd.dim[q];
+    for (int q = 0; q < 6; ++q) {
+      if (q < t.nf) {
+        unsigned nr = rem / t.dim[q];
+        unsigned d = rem - nr * t.dim[q];
+        rem = nr;
+        v *= t.f[q][d + t.ld[q] * c];
+      }
     }
   }
-}
-
-__device__ inline float krp_value(const KrpDev &kd, const int *d, int c) {
-  float v = 1.0f;
-#pragma unroll
-  for (int q = 5; q >= 0; --q)
-    if (q < kd.nf) v *= kd.f[q][d[q] + kd.ld[q] * c];
+  if (t.rowscale) v *= t.rowscale[g];
   return v;
 }
 
-__device__ inline void krp_next(const KrpDev &kd, int *d) {
+// B(kb + u, c), u = 0..3 (zero at g >= kend or c >= k).  kb % 4 == 0.
+__device__ inline floatx4 thin4(const ThinDev &t, long kb, long kend, int c) {
+  floatx4 v = {0.f, 0.f, 0.f, 0.f};
+  if (c >= t.k) return v;
+  if (t.vec && kb + 3 < kend) {
+    if (t.nf == 0) {
+      v = *(const floatx4 *)(t.B + kb + t.ldb * c);
+    } else {
+      unsigned rest = (unsigned)kb / t.dim[0];
+      unsigned d0 = (unsigned)kb - rest * t.dim[0];
+      float s = 1.0f;
 #pragma unroll
-  for (int q = 0; q < 6; ++q) {
-    if (q < kd.nf) {
-      if (++d[q] < kd.dim[q]) return;
-      d[q] = 0;
-    }
-  }
-}
-
-// Fills v[0..n) with B(gk0 + u, c) for u < n (0 outside K / k).
-template <int n>
-__device__ inline void b_values(const GemmParams &p, long gk0, long kend, int c, float *v) {
-  if (c >= p.k) {
-#pragma unroll
-    for (int u = 0; u < n; ++u) v[u] = 0.0f;
-    return;
-  }
-  if (p.krp.nf == 0) {
-#pragma unroll
-    for (int u = 0; u < n; ++u) {
-      long gk = gk0 + u;
-      float x = 0.0f;
-      if (gk < kend) {
-        x = p.B[gk + p.ldb * c];
-        if (p.rowscale) x *= p.rowscale[gk];
+      for (int q = 1; q < 6; ++q) {
+        if (q < t.nf) {
+          unsigned nr = rest / t.dim[q];
+          unsigned d = rest - nr * t.dim[q];
+          rest = nr;
+          s *= t.f[q][d + t.ld[q] * c];
+        }
       }
-      v[u] = x;
+      v = *(const floatx4 *)(t.f[0] + d0 + t.ld[0] * c);
+      v[0] *= s;
+      v[1] *= s;
+      v[2] *= s;
+      v[3] *= s;
+    }
+    if (t.rowscale) {
+      floatx4 r = *(const floatx4 *)(t.rowscale + kb);
+      v[0] *= r[0];
+      v[1] *= r[1];
+      v[2] *= r[2];
+      v[3] *= r[3];
     }
   } else {
-    int d[6];
-    krp_digits(p.krp, gk0 < kend ? gk0 : 0, d);
 #pragma unroll
-    for (int u = 0; u < n; ++u) {
-      long gk = gk0 + u;
-      float x = 0.0f;
-      if (gk < kend) {
-        x = krp_value(p.krp, d, c);
-        if (p.rowscale) x *= p.rowscale[gk];
-        krp_next(p.krp, d);
-      }
-      v[u] = x;
-    }
+    for (int u = 0; u < 4; ++u)
+      if (kb + u < kend) v[u] = thin_at(t, (unsigned)(kb + u), c);
   }
+  return v;
 }
 
-template <int NT, int TRANS>
-__device__ inline void epilogue(const GemmParams &p, floatx4 (&acc)[2][NT], long l0, int split) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  constexpr int KP = 16 * NT;
-#pragma unroll
-  for (int r = 0; r < 2; ++r)
-#pragma unroll
-    for (int t = 0; t < NT; ++t)
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        long gl = l0 + wave * 32 + 16 * r + 4 * (lane >> 4) + i;
-        int c = 16 * t + (lane & 15);
-        if (gl >= p.L) continue;
-        float v = acc[r][t][i];
-        if (p.splits > 1) {
-          p.partial[((long)split * p.L + gl) * KP + c] = v;
-        } else if (c < p.k) {
-          if (TRANS && p.colscale) v *= p.colscale[gl];
-          if (p.cmajor)
-            p.C[c + p.ldc * gl] = v;
-          else
-            p.C[gl + p.ldc * c] = v;
-        }
-      }
-}
-
-// ---------------------------------------------------------------------------
-// FP32 operands: v_mfma_f32_16x16x4f32.  LDS: As[kk][row], Bs[kk][col].
-template <int NT, int TRANS, bool VEC>
-__global__ __launch_bounds__(NTHR) void gemm_f32_kernel(GemmParams p) {
-  constexpr int BK = 32, SA = BL + 16, KP = 16 * NT, SB = ((KP + 15) / 32) * 32 + 16;
-  constexpr int BGROUPS = (BK / 4) * KP;  // groups of 4 consecutive kk of one column
-  constexpr int BITER = (BGROUPS + NTHR - 1) / NTHR;
-  __shared__ float As[2][BK][SA];
-  __shared__ float Bs[2][BK][SB];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const long l0 = (long)blockIdx.x * BL;
-  const int split = blockIdx.y;
-  const long kbeg = (long)split * p.kchunk;
-  const long kend = min(p.K, kbeg + p.kchunk);
-  const int nst = (int)((kend - kbeg + BK - 1) / BK);
-  const float *A = (const float *)p.A;
-
-  floatx4 ra[4];
-  float rb[BITER][4];
-
-  auto load_A = [&](long k0) {
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      int q = tid + NTHR * v;
-      if (!TRANS) {
-        int kk = q >> 5, l = (q & 31) * 4;
-        long gl = l0 + l, gk = k0 + kk;
-        if (VEC && gl + 3 < p.L && gk < kend) {
-          ra[v] = *(const floatx4 *)(A + gl + p.lda * gk);
-        } else {
-#pragma unroll
-          for (int u = 0; u < 4; ++u) ra[v][u] = (gl + u < p.L && gk < kend) ? A[gl + u + p.lda * gk] : 0.0f;
-        }
-      } else {
-        int j = q >> 3, kk = (q & 7) * 4;
-        long gj = l0 + j, gk = k0 + kk;
-        if (VEC && gj < p.L && gk + 3 < kend) {
-          ra[v] = *(const floatx4 *)(A + gk + p.lda * gj);
-        } else {
-#pragma unroll
-          for (int u = 0; u < 4; ++u) ra[v][u] = (gj < p.L && gk + u < kend) ? A[gk + u + p.lda * gj] : 0.0f;
-        }
-      }
-    }
-  };
-  auto store_A = [&](int buf) {
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      int q = tid + NTHR * v;
-      if (!TRANS) {
-        int kk = q >> 5, l = (q & 31) * 4;
-        *(floatx4 *)&As[buf][kk][l] = ra[v];
-      } else {
-        int j = q >> 3, kk = (q & 7) * 4;
-#pragma unroll
-        for (int u = 0; u < 4; ++u) As[buf][kk + u][j] = ra[v][u];
-      }
-    }
-  };
-  auto load_B = [&](long k0) {
-#pragma unroll
-    for (int v = 0; v < BITER; ++v) {
-      int g = tid + NTHR * v;
-      if (g < BGROUPS) {
-        int c = g / (BK / 4), kk = (g % (BK / 4)) * 4;
-        b_values<4>(p, k0 + kk, kend, c, rb[v]);
-      }
-    }
-  };
-  auto store_B = [&](int buf) {
-#pragma unroll
-    for (int v = 0; v < BITER; ++v) {
-      int g = tid + NTHR * v;
-      if (g < BGROUPS) {
-        int c = g / (BK / 4), kk = (g % (BK / 4)) * 4;
-#pragma unroll
-        for (int u = 0; u < 4; ++u) Bs[buf][kk + u][c] = rb[v][u];
-      }
-    }
-  };
-
-  floatx4 acc[2][NT];
-#pragma unroll
-  for (int r = 0; r < 2; ++r)
-#pragma unroll
-    for (int t = 0; t < NT; ++t) acc[r][t] = floatx4{0.f, 0.f, 0.f, 0.f};
-
-  if (nst > 0) {
-    load_A(kbeg);
-    load_B(kbeg);
-    store_A(0);
-    store_B(0);
-    __syncthreads();
-  }
-  for (int s = 0; s < nst; ++s) {
-    const int buf = s & 1;
-    const bool more = s + 1 < nst;
-    if (more) {
-      load_A(kbeg + (long)(s + 1) * BK);
-      load_B(kbeg + (long)(s + 1) * BK);
-    }
-#pragma unroll
-    for (int ks = 0; ks < BK / 4; ++ks) {
-      int kk = ks * 4 + (lane >> 4);
-      float a0 = As[buf][kk][wave * 32 + (lane & 15)];
-      float a1 = As[buf][kk][wave * 32 + 16 + (lane & 15)];
-#pragma unroll
-      for (int t = 0; t < NT; ++t) {
-        float b = Bs[buf][kk][16 * t + (lane & 15)];
-        acc[0][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b, acc[0][t], 0, 0, 0);
-        acc[1][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b, acc[1][t], 0, 0, 0);
-      }
-    }
-    if (more) {
-      store_A(buf ^ 1);
-      store_B(buf ^ 1);
-    }
-    __syncthreads();
-  }
-  epilogue<NT, TRANS>(p, acc, l0, split);
-}
-
-// ---------------------------------------------------------------------------
-// 16-bit operands: v_mfma_f32_16x16x32_{bf16,f16}.  LDS: As[row][kk],
-// Bs[col][kk], 8 consecutive kk per lane read as one 16-byte word.
 template <typename TA>
 __device__ inline uint16_t to16(float x);
 template <>
@@ -288,8 +147,533 @@ __device__ inline floatx4 mfma16<f16_t>(const uint4 &a, const uint4 &b, floatx4 
                                                 __builtin_bit_cast(halfx8, b), c, 0, 0, 0);
 }
 
-template <typename TA, int NT, int TRANS, bool VEC>
-__global__ __launch_bounds__(NTHR) void gemm_h_kernel(GemmParams p) {
+__device__ inline floatx4 mfma4(float a, float b, floatx4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// ---------------------------------------------------------------------------
+struct StreamParams {
+  const void *A;
+  long lda, L, K;
+  ThinDev th;
+  const uint4 *thin16;  // packed 16-bit thin operand (16-bit TN)
+  const float *colscale;
+  float *C;
+  long ldc;
+  int cmajor;
+  int wgrp;   // column (TN) / row (NN) groups per block; wgrp * wk == 4
+  int wk;     // k phases per group
+  long kchunk;  // k range of one blockIdx.y (multiple of the k step)
+  int nsplit;
+  float *part;
+  int avec;   // 16-byte loads of A valid (lda and base aligned)
+  int cvec;   // 16-byte stores of C valid
+};
+
+// Sums the k-phase accumulators of a block in LDS (phase 0 .. wk-1 order).
+// With a K split the phase-0 waves then store the block's partial tile
+// (summed over splits by splitk_reduce_kernel) and false is returned;
+// otherwise returns true in the waves that hold a final tile.
+template <int NA>
+__device__ inline bool finish_tile(const StreamParams &p, floatx4 (&acc)[NA], floatx4 (*red)[NA][64]) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int grp = wave % p.wgrp, kph = wave / p.wgrp;
+  if (p.wk > 1) {
+    if (kph > 0) {
+#pragma unroll
+      for (int a = 0; a < NA; ++a) red[wave - p.wgrp][a][lane] = acc[a];
+    }
+    __syncthreads();
+    if (kph == 0) {
+      for (int q = 1; q < p.wk; ++q) {
+#pragma unroll
+        for (int a = 0; a < NA; ++a) {
+          floatx4 r = red[grp + (q - 1) * p.wgrp][a][lane];
+          acc[a] += r;
+        }
+      }
+    }
+  }
+  if (kph != 0) return false;
+  if (p.nsplit <= 1) return true;
+  // Partial tile of (split, group): NA x 64 floatx4, lane fastest.
+  floatx4 *dst = (floatx4 *)p.part +
+                 (((long)blockIdx.y * gridDim.x + blockIdx.x) * p.wgrp + grp) * (long)NA * 64;
+#pragma unroll
+  for (int a = 0; a < NA; ++a) dst[a * 64 + lane] = acc[a];
+  return false;
+}
+
+// Output element (c, l) of a TN product.
+__device__ inline void store_tn(const StreamParams &p, long l, int c, float v) {
+  if (p.cmajor)
+    p.C[c + p.ldc * l] = v;
+  else
+    p.C[l + p.ldc * c] = v;
+}
+
+// TN accumulator a = cw*NT + t of a group starting at column l0:
+// v[u] = C(16t + 4lg + u, l0 + 16cw + lc).
+template <int NT>
+__device__ inline void store_acc_tn(const StreamParams &p, long l0, int a, floatx4 v) {
+  const int lane = threadIdx.x & 63, lc = lane & 15, lg = lane >> 4;
+  const int cw = a / NT, t = a % NT;
+  const long l = l0 + 16 * cw + lc;
+  if (l >= p.L) return;
+  const float sc = p.colscale ? p.colscale[l] : 1.0f;
+  v[0] *= sc;
+  v[1] *= sc;
+  v[2] *= sc;
+  v[3] *= sc;
+  const int c = 16 * t + 4 * lg;
+  if (p.cmajor && p.cvec && c + 3 < p.th.k) {
+    *(floatx4 *)(p.C + c + p.ldc * l) = v;
+  } else {
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (c + u < p.th.k) store_tn(p, l, c + u, v[u]);
+  }
+}
+
+// NN accumulator a = t*4 + j of a group starting at row l0:
+// v[u] = C(l0 + 16lg + 4u + j, 16t + lc).
+__device__ inline void store_acc_nn(const StreamParams &p, long l0, int a, floatx4 v) {
+  const int lane = threadIdx.x & 63, lc = lane & 15, lg = lane >> 4;
+  const int t = a / 4, j = a % 4;
+  const int c = 16 * t + lc;
+  if (c >= p.th.k) return;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const long l = l0 + 16 * lg + 4 * u + j;
+    if (l >= p.L) continue;
+    if (p.cmajor)
+      p.C[c + p.ldc * l] = v[u];
+    else
+      p.C[l + p.ldc * c] = v[u];
+  }
+}
+
+template <int NT, int CW>
+__device__ inline void epilogue_tn(const StreamParams &p, floatx4 (&acc)[CW * NT], long l0) {
+#pragma unroll
+  for (int a = 0; a < CW * NT; ++a) store_acc_tn<NT>(p, l0, a, acc[a]);
+}
+
+// Sum over the K splits (split order) of one accumulator slab per block:
+// block (group g, accumulator a); wave w adds splits w, w+4, ...; the four
+// wave sums are combined in wave order.  kind 0: TN with NT, CW; 1: NN.
+template <int KIND, int NT, int CW>
+__global__ __launch_bounds__(256) void splitk_reduce_kernel(StreamParams p, int NA, long ngroups) {
+  __shared__ floatx4 red[4][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const long g = blockIdx.x;
+  const int a = blockIdx.y;
+  const long split_stride = ngroups * (long)NA * 64;
+  const floatx4 *src = (const floatx4 *)p.part + g * (long)NA * 64 + (long)a * 64 + lane;
+  floatx4 s = {0.f, 0.f, 0.f, 0.f};
+  int sp = wave;
+  for (; sp + 12 < p.nsplit; sp += 16) {
+    floatx4 v0 = src[(long)sp * split_stride], v1 = src[(long)(sp + 4) * split_stride];
+    floatx4 v2 = src[(long)(sp + 8) * split_stride], v3 = src[(long)(sp + 12) * split_stride];
+    s += v0;
+    s += v1;
+    s += v2;
+    s += v3;
+  }
+  for (; sp < p.nsplit; sp += 4) s += src[(long)sp * split_stride];
+  red[wave][lane] = s;
+  __syncthreads();
+  if (wave != 0) return;
+  floatx4 t = red[0][lane];
+  t += red[1][lane];
+  t += red[2][lane];
+  t += red[3][lane];
+  if (KIND == 0)
+    store_acc_tn<NT>(p, g * (16 * CW), a, t);
+  else
+    store_acc_nn(p, g * 64, a, t);
+}
+
+// ---------------------------------------------------------------------------
+// Software pipeline over this wave's full k-steps i = 0 .. nsw-1: a ring of
+// PD register buffers, load(i, buf) issues the loads of step i, mma(buf)
+// consumes them.  The steady-state body has no branches (prefetch indexes
+// are clamped to the last step, a re-load that hits in cache), so each mma
+// waits only for the oldest buffer's loads.
+template <int PD, class Buf, class Load, class Mma>
+__device__ inline void stream_steps(long nsw, Buf (&buf)[PD], Load load, Mma mma) {
+  if (nsw <= 0) return;
+#pragma unroll
+  for (int u = 0; u < PD; ++u) load(u < nsw ? (long)u : nsw - 1, buf[u]);
+  long i0 = 0;
+  for (; i0 + PD <= nsw; i0 += PD) {
+#pragma unroll
+    for (int u = 0; u < PD; ++u) {
+      mma(buf[u]);
+      __builtin_amdgcn_sched_barrier(0);
+      const long nx = i0 + u + PD;
+      load(nx < nsw ? nx : nsw - 1, buf[u]);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < PD - 1; ++u)
+    if (i0 + u < nsw) mma(buf[u]);
+}
+
+// Thin-operand access modes of the FP32 kernels.
+constexpr int THIN_PLAIN = 0;  // B column (or the single KRP factor), 16-byte loads
+constexpr int THIN_KRP2 = 2;   // two-factor KRP: 16-byte load of f0 times one f1 value
+constexpr int THIN_SAFE = 9;   // anything else: masked scalar loads, generic KRP, rowscale
+
+// Per-lane column base pointers of the thin operand for c = 16t + lc
+// (clamped to k - 1: the extra columns only feed outputs that are not stored).
+template <int NT>
+__device__ inline void thin_cols(const ThinDev &th, const float *(&tb)[NT], const float *(&tf1)[NT]) {
+  const int lc = threadIdx.x & 15;
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    int c = 16 * t + lc;
+    c = c < th.k ? c : th.k - 1;
+    if (th.nf == 0) {
+      tb[t] = th.B + th.ldb * c;
+      tf1[t] = nullptr;
+    } else {
+      tb[t] = th.f[0] + th.ld[0] * c;
+      tf1[t] = th.nf > 1 ? th.f[1] + th.ld[1] * c : nullptr;
+    }
+  }
+}
+
+// B(kb + 0..3, 16t + lc) for a full step (kb + 3 < K), as raw loads: a[t]
+// and, for THIN_KRP2, the second factor's value sc[t] that multiplies them
+// (applied by thin_apply when the step is consumed, so the loads stay in
+// flight).
+template <int NT, int TM>
+__device__ inline void thin_step(const ThinDev &th, long kb, const float *const (&tb)[NT],
+                                 const float *const (&tf1)[NT], floatx4 (&a)[NT], float (&sc)[NT]) {
+  if (TM == THIN_PLAIN) {
+#pragma unroll
+    for (int t = 0; t < NT; ++t) a[t] = *(const floatx4 *)(tb[t] + kb);
+  } else if (TM == THIN_KRP2) {
+    const unsigned r = (unsigned)kb / th.dim[0];
+    const unsigned d0 = (unsigned)kb - r * th.dim[0];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      a[t] = *(const floatx4 *)(tb[t] + d0);
+      sc[t] = tf1[t][r];
+    }
+  } else {
+    const int lc = threadIdx.x & 15;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) a[t] = thin4(th, kb, kb + 4, 16 * t + lc);
+  }
+}
+
+template <int NT, int TM>
+__device__ inline void thin_apply(floatx4 (&a)[NT], const float (&sc)[NT]) {
+  if (TM == THIN_KRP2) {
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      a[t][0] *= sc[t];
+      a[t][1] *= sc[t];
+      a[t][2] *= sc[t];
+      a[t][3] *= sc[t];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// FP32 TN.  Step = 16 k.
+template <int NT, int CW>
+struct BufTN {
+  floatx4 x[CW];
+  floatx4 a[NT];
+  float sc[NT];
+};
+
+template <int NT, int CW, int TM>
+__global__ __launch_bounds__(NTHR) void tn_f32_kernel(StreamParams p) {
+  constexpr int STEP = 16, PD = 3, NA = CW * NT;
+  __shared__ floatx4 red[3][NA][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lc = lane & 15, lg = lane >> 4;
+  const int grp = wave % p.wgrp, kph = wave / p.wgrp;
+  const long l0 = ((long)blockIdx.x * p.wgrp + grp) * (16 * CW);
+  const long kbeg = (long)blockIdx.y * p.kchunk;
+  const long kend = min(p.K, kbeg + p.kchunk);
+  const long nfull = kend > kbeg ? (kend - kbeg) / STEP : 0;
+  const float *A = (const float *)p.A;
+  const float *col[CW];
+#pragma unroll
+  for (int cw = 0; cw < CW; ++cw) {
+    const long l = l0 + 16 * cw + lc;
+    col[cw] = A + p.lda * (l < p.L ? l : p.L - 1);
+  }
+  const float *tb[NT], *tf1[NT];
+  thin_cols<NT>(p.th, tb, tf1);
+  floatx4 acc[NA];
+#pragma unroll
+  for (int a = 0; a < NA; ++a) acc[a] = floatx4{0.f, 0.f, 0.f, 0.f};
+  auto mma = [&](BufTN<NT, CW> &b) {
+    thin_apply<NT, TM>(b.a, b.sc);
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int cw = 0; cw < CW; ++cw) acc[cw * NT + t] = mfma4(b.a[t][j], b.x[cw][j], acc[cw * NT + t]);
+  };
+  auto load = [&](long i, BufTN<NT, CW> &b) {
+    const long kb = kbeg + (kph + i * p.wk) * STEP + 4 * lg;
+#pragma unroll
+    for (int cw = 0; cw < CW; ++cw) {
+      if (TM != THIN_SAFE) {
+        b.x[cw] = *(const floatx4 *)(col[cw] + kb);
+      } else {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) b.x[cw][u] = col[cw][kb + u];
+      }
+    }
+    thin_step<NT, TM>(p.th, kb, tb, tf1, b.a, b.sc);
+  };
+  BufTN<NT, CW> buf[PD];
+  const long nsw = nfull > kph ? (nfull - kph + p.wk - 1) / p.wk : 0;
+  stream_steps<PD>(nsw, buf, load, mma);
+  if (kend - kbeg > nfull * STEP && nfull % p.wk == kph) {
+    // The partial last step, masked.
+    BufTN<NT, CW> b;
+    const long kb = kbeg + nfull * STEP + 4 * lg;
+#pragma unroll
+    for (int cw = 0; cw < CW; ++cw)
+#pragma unroll
+      for (int u = 0; u < 4; ++u) b.x[cw][u] = kb + u < kend ? col[cw][kb + u] : 0.0f;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      b.a[t] = thin4(p.th, kb, kend, 16 * t + lc);
+      b.sc[t] = 1.0f;
+    }
+    mma(b);
+  }
+  if (finish_tile<NA>(p, acc, red)) epilogue_tn<NT, CW>(p, acc, l0);
+}
+
+// ---------------------------------------------------------------------------
+// Packs the thin operand into the 16-bit MFMA A-operand layout:
+// out[(step*NT + t)*64 + lane] = B(32 step + 8 lg + 0..7, 16 t + lc).
+template <typename TA>
+__global__ void pack_thin16_kernel(ThinDev th, long K, int NT, long nsteps, uint4 *out) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nsteps * NT * 64) return;
+  const int lane = (int)(e & 63);
+  const long st = e >> 6;
+  const int t = (int)(st % NT);
+  const long step = st / NT;
+  const int c = 16 * t + (lane & 15);
+  const long kb = 32 * step + 8 * (lane >> 4);
+  floatx4 lo = thin4(th, kb, K, c), hi = thin4(th, kb + 4, K, c);
+  uint16_t h[8];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    h[u] = to16<TA>(lo[u]);
+    h[4 + u] = to16<TA>(hi[u]);
+  }
+  out[e] = __builtin_bit_cast(uint4, h);
+}
+
+// 16-bit TN.  Step = 32 k.  SAFE: scalar loads of A (unaligned lda/base).
+template <int NT, int CW>
+struct BufH {
+  uint4 x[CW];
+  uint4 a[NT];
+};
+
+template <typename TA, int NT, int CW, int SAFE>
+__global__ __launch_bounds__(NTHR) void tn_h_kernel(StreamParams p) {
+  constexpr int STEP = 32, PD = 3, NA = CW * NT;
+  __shared__ floatx4 red[3][NA][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lc = lane & 15, lg = lane >> 4;
+  const int grp = wave % p.wgrp, kph = wave / p.wgrp;
+  const long l0 = ((long)blockIdx.x * p.wgrp + grp) * (16 * CW);
+  const long kbeg = (long)blockIdx.y * p.kchunk;
+  const long kend = min(p.K, kbeg + p.kchunk);
+  const long nfull = kend > kbeg ? (kend - kbeg) / STEP : 0;
+  const long step0 = kbeg / STEP;
+  const uint16_t *A = (const uint16_t *)p.A;
+  const uint16_t *col[CW];
+#pragma unroll
+  for (int cw = 0; cw < CW; ++cw) {
+    const long l = l0 + 16 * cw + lc;
+    col[cw] = A + p.lda * (l < p.L ? l : p.L - 1);
+  }
+  floatx4 acc[NA];
+#pragma unroll
+  for (int a = 0; a < NA; ++a) acc[a] = floatx4{0.f, 0.f, 0.f, 0.f};
+  auto mma = [&](BufH<NT, CW> &b) {
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+      for (int cw = 0; cw < CW; ++cw) acc[cw * NT + t] = mfma16<TA>(b.a[t], b.x[cw], acc[cw * NT + t]);
+  };
+  const uint4 *pk = p.thin16 + lane;
+  auto load = [&](long i, BufH<NT, CW> &b) {
+    const long s = kph + i * p.wk;
+    const long kb = kbeg + s * STEP + 8 * lg;
+#pragma unroll
+    for (int cw = 0; cw < CW; ++cw) {
+      if (!SAFE) {
+        b.x[cw] = *(const uint4 *)(col[cw] + kb);
+      } else {
+        uint16_t h[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) h[u] = col[cw][kb + u];
+        b.x[cw] = __builtin_bit_cast(uint4, h);
+      }
+    }
+    const uint4 *src = pk + (step0 + s) * NT * 64;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) b.a[t] = src[t * 64];
+  };
+  BufH<NT, CW> buf[PD];
+  const long nsw = nfull > kph ? (nfull - kph + p.wk - 1) / p.wk : 0;
+  stream_steps<PD>(nsw, buf, load, mma);
+  if (kend - kbeg > nfull * STEP && nfull % p.wk == kph) {
+    BufH<NT, CW> b;
+    const long kb = kbeg + nfull * STEP + 8 * lg;
+#pragma unroll
+    for (int cw = 0; cw < CW; ++cw) {
+      uint16_t h[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) h[u] = kb + u < kend ? col[cw][kb + u] : (uint16_t)0;
+      b.x[cw] = __builtin_bit_cast(uint4, h);
+    }
+    const uint4 *src = pk + (step0 + nfull) * NT * 64;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) b.a[t] = src[t * 64];
+    mma(b);
+  }
+  if (finish_tile<NA>(p, acc, red)) epilogue_tn<NT, CW>(p, acc, l0);
+}
+
+// ---------------------------------------------------------------------------
+// FP32 NN.  Step = 16 k; a wave owns 64 rows.  acc[t*4 + j][v] holds
+// C(l0 + 16lg + 4v + j, 16t + lc).  Fast modes need L % 4 == 0 (a lane's
+// four rows are in or out together; rows past L re-read rows L-4..L-1).
+template <int NT>
+struct BufNN {
+  floatx4 x[4];
+  floatx4 a[NT];
+  float sc[NT];
+};
+
+template <int NT, int TM>
+__global__ __launch_bounds__(NTHR) void nn_f32_kernel(StreamParams p) {
+  constexpr int STEP = 16, PD = 2, NA = 4 * NT;
+  __shared__ floatx4 red[3][NA][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lc = lane & 15, lg = lane >> 4;
+  const int grp = wave % p.wgrp, kph = wave / p.wgrp;
+  const long l0 = ((long)blockIdx.x * p.wgrp + grp) * 64;
+  const long kbeg = (long)blockIdx.y * p.kchunk;
+  const long kend = min(p.K, kbeg + p.kchunk);
+  const long nfull = kend > kbeg ? (kend - kbeg) / STEP : 0;
+  const float *A = (const float *)p.A;
+  const long lr = l0 + 4 * lc;  // first of this lane's 4 rows
+  const float *base = A + (TM != THIN_SAFE ? (lr < p.L - 4 ? lr : p.L - 4) : 0);
+  long rows[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) rows[j] = lr + j < p.L ? lr + j : p.L - 1;
+  const float *tb[NT], *tf1[NT];
+  thin_cols<NT>(p.th, tb, tf1);
+  floatx4 acc[NA];
+#pragma unroll
+  for (int a = 0; a < NA; ++a) acc[a] = floatx4{0.f, 0.f, 0.f, 0.f};
+  auto mma = [&](BufNN<NT> &b) {
+    thin_apply<NT, TM>(b.a, b.sc);
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[t * 4 + j] = mfma4(b.x[u][j], b.a[t][u], acc[t * 4 + j]);
+  };
+  auto load = [&](long i, BufNN<NT> &b) {
+    const long kb = kbeg + (kph + i * p.wk) * STEP + 4 * lg;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (TM != THIN_SAFE) {
+        b.x[u] = *(const floatx4 *)(base + p.lda * (kb + u));
+      } else {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) b.x[u][j] = A[rows[j] + p.lda * (kb + u)];
+      }
+    }
+    thin_step<NT, TM>(p.th, kb, tb, tf1, b.a, b.sc);
+  };
+  BufNN<NT> buf[PD];
+  const long nsw = nfull > kph ? (nfull - kph + p.wk - 1) / p.wk : 0;
+  stream_steps<PD>(nsw, buf, load, mma);
+  if (kend - kbeg > nfull * STEP && nfull % p.wk == kph) {
+    BufNN<NT> b;
+    const long kb = kbeg + nfull * STEP + 4 * lg;
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b.x[u][j] = kb + u < kend ? A[rows[j] + p.lda * (kb + u)] : 0.0f;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      b.a[t] = thin4(p.th, kb, kend, 16 * t + lc);
+      b.sc[t] = 1.0f;
+    }
+    mma(b);
+  }
+  if (!finish_tile<NA>(p, acc, red)) return;
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int c = 16 * t + lc;
+    if (c >= p.th.k) continue;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const long l = l0 + 16 * lg + 4 * v;
+      floatx4 o = {acc[t * 4 + 0][v], acc[t * 4 + 1][v], acc[t * 4 + 2][v], acc[t * 4 + 3][v]};
+      if (!p.cmajor && p.cvec && l + 3 < p.L) {
+        *(floatx4 *)(p.C + l + p.ldc * c) = o;
+      } else {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          if (l + j >= p.L) continue;
+          if (p.cmajor)
+            p.C[c + p.ldc * (l + j)] = o[j];
+          else
+            p.C[l + j + p.ldc * c] = o[j];
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 16-bit NN: LDS-transposing kernel (A(L x K) is L-contiguous, the 16-bit
+// MFMA wants 8 consecutive k per lane).  Block = 128 rows x all k columns;
+// K-chunks of 64 staged through LDS (double buffered), 8x4 register blocks
+// transposed on the way in.
+constexpr int BL = 128;
+
+struct GemmParams {
+  const void *A;
+  long lda, L, K;
+  ThinDev th;
+  float *C;
+  long ldc;
+  int cmajor;
+  float *partial;
+  int splits;
+  long kchunk;
+};
+
+template <typename TA, int NT, bool VEC>
+__global__ __launch_bounds__(NTHR) void nn_h_kernel(GemmParams p) {
   constexpr int BK = 64, SA = BK + 8, KP = 16 * NT;
   constexpr int BGROUPS = (BK / 8) * KP;  // groups of 8 consecutive kk of one column
   constexpr int BITER = (BGROUPS + NTHR - 1) / NTHR;
@@ -307,57 +691,31 @@ __global__ __launch_bounds__(NTHR) void gemm_h_kernel(GemmParams p) {
   uint4 rb[BITER];
 
   auto load_A = [&](long k0) {
-    if (!TRANS) {
-      // 8 rows x 4 kk per thread: 4 loads of 8 consecutive rows of one column.
-      int rg = tid & 15, kg = tid >> 4;
-      long gl = l0 + 8 * rg;
+    // 8 rows x 4 kk per thread: 4 loads of 8 consecutive rows of one column.
+    int rg = tid & 15, kg = tid >> 4;
+    long gl = l0 + 8 * rg;
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        long gk = k0 + 4 * kg + u;
-        if (VEC && gl + 7 < p.L && gk < kend) {
-          ra[u] = *(const uint4 *)(A + gl + p.lda * gk);
-        } else {
-          uint16_t t[8];
+    for (int u = 0; u < 4; ++u) {
+      long gk = k0 + 4 * kg + u;
+      if (VEC && gl + 7 < p.L && gk < kend) {
+        ra[u] = *(const uint4 *)(A + gl + p.lda * gk);
+      } else {
+        uint16_t t[8];
 #pragma unroll
-          for (int e = 0; e < 8; ++e) t[e] = (gl + e < p.L && gk < kend) ? A[gl + e + p.lda * gk] : 0;
-          ra[u] = __builtin_bit_cast(uint4, t);
-        }
-      }
-    } else {
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        int q = tid + NTHR * v;
-        int j = q >> 3, kk = (q & 7) * 8;
-        long gj = l0 + j, gk = k0 + kk;
-        if (VEC && gj < p.L && gk + 7 < kend) {
-          ra[v] = *(const uint4 *)(A + gk + p.lda * gj);
-        } else {
-          uint16_t t[8];
-#pragma unroll
-          for (int e = 0; e < 8; ++e) t[e] = (gj < p.L && gk + e < kend) ? A[gk + e + p.lda * gj] : 0;
-          ra[v] = __builtin_bit_cast(uint4, t);
-        }
+        for (int e = 0; e < 8; ++e) t[e] = (gl + e < p.L && gk < kend) ? A[gl + e + p.lda * gk] : 0;
+        ra[u] = __builtin_bit_cast(uint4, t);
       }
     }
   };
   auto store_A = [&](int buf) {
-    if (!TRANS) {
-      int rg = tid & 15, kg = tid >> 4;
-      uint16_t t[4][8];
+    int rg = tid & 15, kg = tid >> 4;
+    uint16_t t[4][8];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) __builtin_memcpy(t[u], &ra[u], 16);
+    for (int u = 0; u < 4; ++u) __builtin_memcpy(t[u], &ra[u], 16);
 #pragma unroll
-      for (int r = 0; r < 8; ++r) {
-        uint16_t w[4] = {t[0][r], t[1][r], t[2][r], t[3][r]};
-        *(uint2 *)&As[buf][8 * rg + r][4 * kg] = __builtin_bit_cast(uint2, w);
-      }
-    } else {
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        int q = tid + NTHR * v;
-        int j = q >> 3, kk = (q & 7) * 8;
-        *(uint4 *)&As[buf][j][kk] = ra[v];
-      }
+    for (int r = 0; r < 8; ++r) {
+      uint16_t w[4] = {t[0][r], t[1][r], t[2][r], t[3][r]};
+      *(uint2 *)&As[buf][8 * rg + r][4 * kg] = __builtin_bit_cast(uint2, w);
     }
   };
   auto load_B = [&](long k0) {
@@ -366,11 +724,13 @@ __global__ __launch_bounds__(NTHR) void gemm_h_kernel(GemmParams p) {
       int g = tid + NTHR * v;
       if (g < BGROUPS) {
         int c = g / (BK / 8), kk = (g % (BK / 8)) * 8;
-        float x[8];
-        b_values<8>(p, k0 + kk, kend, c, x);
+        floatx4 lo = thin4(p.th, k0 + kk, kend, c), hi = thin4(p.th, k0 + kk + 4, kend, c);
         uint16_t h[8];
 #pragma unroll
-        for (int e = 0; e < 8; ++e) h[e] = to16<TA>(x[e]);
+        for (int e = 0; e < 4; ++e) {
+          h[e] = to16<TA>(lo[e]);
+          h[4 + e] = to16<TA>(hi[e]);
+        }
         rb[v] = __builtin_bit_cast(uint4, h);
       }
     }
@@ -424,61 +784,316 @@ __global__ __launch_bounds__(NTHR) void gemm_h_kernel(GemmParams p) {
     }
     __syncthreads();
   }
-  epilogue<NT, TRANS>(p, acc, l0, split);
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        long gl = l0 + wave * 32 + 16 * r + 4 * (lane >> 4) + i;
+        int c = 16 * t + (lane & 15);
+        if (gl >= p.L) continue;
+        float v = acc[r][t][i];
+        if (p.splits > 1) {
+          p.partial[((long)split * p.L + gl) * KP + c] = v;
+        } else if (c < p.th.k) {
+          if (p.cmajor)
+            p.C[c + p.ldc * gl] = v;
+          else
+            p.C[gl + p.ldc * c] = v;
+        }
+      }
 }
 
-// Sum of split-K partials (fixed order) into the final layout.
+// Sum of the 16-bit NN split-K partials (fixed order) into the final layout.
 __global__ void reduce_splits_kernel(const float *partial, int splits, long L, int KP, int k,
-                                     float *C, long ldc, int cmajor, const float *colscale) {
+                                     float *C, long ldc, int cmajor) {
   long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= L * k) return;
   long gl = idx / k;
   int c = (int)(idx % k);
   float s = 0.0f;
   for (int sp = 0; sp < splits; ++sp) s += partial[((long)sp * L + gl) * KP + c];
-  if (colscale) s *= colscale[gl];
   if (cmajor)
     C[c + ldc * gl] = s;
   else
     C[gl + ldc * c] = s;
 }
 
-template <int NT, int TRANS>
-void launch_nt(int dtype, bool vec, dim3 grid, hipStream_t st, const GemmParams &p) {
-  if (dtype == PLANC_F32) {
-    if (vec)
-      gemm_f32_kernel<NT, TRANS, true><<<grid, NTHR, 0, st>>>(p);
-    else
-      gemm_f32_kernel<NT, TRANS, false><<<grid, NTHR, 0, st>>>(p);
-  } else if (dtype == PLANC_BF16) {
-    if (vec)
-      gemm_h_kernel<bf16_t, NT, TRANS, true><<<grid, NTHR, 0, st>>>(p);
-    else
-      gemm_h_kernel<bf16_t, NT, TRANS, false><<<grid, NTHR, 0, st>>>(p);
-  } else {
-    if (vec)
-      gemm_h_kernel<f16_t, NT, TRANS, true><<<grid, NTHR, 0, st>>>(p);
-    else
-      gemm_h_kernel<f16_t, NT, TRANS, false><<<grid, NTHR, 0, st>>>(p);
-  }
+// ---------------------------------------------------------------------------
+// Launch geometry of a streaming product: columns per wave (16*cw, TN), k
+// phases per group inside a block (wk; wgrp = 4 / wk groups per block) and
+// K splits across blocks.
+struct Geometry {
+  int cw, wgrp, wk, nsplit;
+  long kchunk, blocks_x, groups;
+  long tile_bytes;  // partial tile of one group
+};
+
+Geometry make_geometry(long L, long K, int trans, int dtype, int NT, int cw, int wk, long ns) {
+  Geometry g{};
+  const int step = dtype == PLANC_F32 ? 16 : 32;
+  g.cw = trans ? cw : 1;
+  g.groups = trans ? (L + 16 * cw - 1) / (16 * cw) : (L + 63) / 64;
+  g.tile_bytes = trans ? (long)cw * NT * 64 * 16 : 4L * NT * 64 * 16;
+  g.wk = wk;
+  g.wgrp = 4 / wk;
+  g.blocks_x = (g.groups + g.wgrp - 1) / g.wgrp;
+  const long ksteps = (K + step - 1) / step;
+  if (ns > ksteps) ns = ksteps > 0 ? ksteps : 1;
+  long kchunk = (ksteps + ns - 1) / ns * step;
+  g.kchunk = kchunk > 0 ? kchunk : step;
+  g.nsplit = (int)((K + g.kchunk - 1) / g.kchunk);
+  if (g.nsplit < 1) g.nsplit = 1;
+  return g;
 }
 
-template <int TRANS>
-void launch(int NT, int dtype, bool vec, dim3 grid, hipStream_t st, const GemmParams &p) {
-  switch (NT) {
-    case 1:
-      launch_nt<1, TRANS>(dtype, vec, grid, st, p);
-      break;
-    case 2:
-      launch_nt<2, TRANS>(dtype, vec, grid, st, p);
-      break;
-    case 3:
-      launch_nt<3, TRANS>(dtype, vec, grid, st, p);
-      break;
-    default:
-      launch_nt<4, TRANS>(dtype, vec, grid, st, p);
-      break;
+// Candidate geometries: every (cw, wk) with K splits 1, 2, 4, ... while the
+// partial traffic stays under 10% of A, every wave keeps at least 8 k steps
+// and the grid stays under 16k waves.  The first entry is the static choice
+// used when tuning is off or impossible (graph capture).
+std::vector<Geometry> candidate_geometries(const planc_ctx *ctx, long L, long K, int trans, int dtype,
+                                           int NT) {
+  std::vector<Geometry> out;
+  const int step = dtype == PLANC_F32 ? 16 : 32;
+  const long ksteps = (K + step - 1) / step;
+  const int esz = dtype == PLANC_F32 ? 4 : 2;
+  const double abytes = (double)L * (double)K * esz;
+  std::vector<int> cws;
+  if (!trans)
+    cws = {1};
+  else if (dtype == PLANC_F32)
+    cws = {2, 4};
+  else
+    cws = {4};
+  // Static choice: one wave per SIMD or more, fewest partials.
+  {
+    const int cw = cws[0];
+    int wk = 1;
+    Geometry g = make_geometry(L, K, trans, dtype, NT, cw, 1, 1);
+    while (wk < 4 && g.blocks_x * 4 < 4L * ctx->num_cu && ksteps >= 64L * wk) {
+      wk *= 2;
+      g = make_geometry(L, K, trans, dtype, NT, cw, wk, 1);
+    }
+    long ns = 1;
+    while (g.blocks_x * 4 * ns < 4L * ctx->num_cu) {
+      Geometry h = make_geometry(L, K, trans, dtype, NT, cw, wk, ns * 2);
+      if (2.0 * h.nsplit * h.groups * h.tile_bytes > 0.06 * abytes || ksteps / ((long)h.nsplit * wk) < 8) break;
+      ns *= 2;
+      g = h;
+    }
+    out.push_back(g);
   }
+  for (int cw : cws)
+    for (int wk = 1; wk <= 4; wk *= 2) {
+      if (wk > 1 && ksteps < 8L * wk) break;
+      for (long ns = 1; ns <= 4096; ns *= 2) {
+        Geometry g = make_geometry(L, K, trans, dtype, NT, cw, wk, ns);
+        if (ns > 1 && (g.nsplit < ns || ksteps / ((long)g.nsplit * wk) < 8)) break;
+        if (ns > 1 && 2.0 * g.nsplit * g.groups * g.tile_bytes > 0.10 * abytes) break;
+        const long waves = g.blocks_x * 4 * g.nsplit;
+        if (ns > 1 && waves > 16L * 1024) break;
+        if (waves < 2L * ctx->num_cu && ns < 4096) continue;  // too few waves: try more splits
+        out.push_back(g);
+      }
+    }
+  return out;
+}
+
+int thin_mode(const ThinDev &th, bool avec, const float *rowscale) {
+  if (!avec || rowscale || !th.vec) return THIN_SAFE;
+  if (th.nf <= 1) return THIN_PLAIN;
+  if (th.nf == 2) return THIN_KRP2;
+  return THIN_SAFE;
+}
+
+template <int NT, int CW>
+void launch_tn_f32(int tm, dim3 grid, hipStream_t st, const StreamParams &p) {
+  if (tm == THIN_PLAIN)
+    tn_f32_kernel<NT, CW, THIN_PLAIN><<<grid, NTHR, 0, st>>>(p);
+  else if (tm == THIN_KRP2)
+    tn_f32_kernel<NT, CW, THIN_KRP2><<<grid, NTHR, 0, st>>>(p);
+  else
+    tn_f32_kernel<NT, CW, THIN_SAFE><<<grid, NTHR, 0, st>>>(p);
+}
+template <int NT>
+void launch_nn_f32(int tm, dim3 grid, hipStream_t st, const StreamParams &p) {
+  if (tm == THIN_PLAIN)
+    nn_f32_kernel<NT, THIN_PLAIN><<<grid, NTHR, 0, st>>>(p);
+  else if (tm == THIN_KRP2)
+    nn_f32_kernel<NT, THIN_KRP2><<<grid, NTHR, 0, st>>>(p);
+  else
+    nn_f32_kernel<NT, THIN_SAFE><<<grid, NTHR, 0, st>>>(p);
+}
+template <typename TA, int NT>
+void launch_tn_h(bool safe, dim3 grid, hipStream_t st, const StreamParams &p) {
+  if (safe)
+    tn_h_kernel<TA, NT, 4, 1><<<grid, NTHR, 0, st>>>(p);
+  else
+    tn_h_kernel<TA, NT, 4, 0><<<grid, NTHR, 0, st>>>(p);
+}
+template <int NT, int CW>
+void launch_reduce_tn(dim3 g, hipStream_t st, const StreamParams &p, int NA, long ngroups) {
+  splitk_reduce_kernel<0, NT, CW><<<g, 256, 0, st>>>(p, NA, ngroups);
+}
+
+// Launches one streaming product (and its split-K reduction) with geometry g.
+int launch_stream(planc_ctx *ctx, int dtype, int trans, int NT, int tm, StreamParams p, const Geometry &g) {
+  p.wgrp = g.wgrp;
+  p.wk = g.wk;
+  p.kchunk = g.kchunk;
+  p.nsplit = g.nsplit;
+  // Partial tiles are indexed by padded group (blocks_x * wgrp >= groups).
+  const long gpad = g.blocks_x * g.wgrp;
+  if (g.nsplit > 1) {
+    p.part = (float *)ws_get(ctx, 0, (size_t)g.nsplit * gpad * g.tile_bytes);
+    if (!p.part) return set_error(ctx, PLANC_ENOMEM, "thin_gemm: workspace");
+  }
+  dim3 grid((unsigned)g.blocks_x, (unsigned)g.nsplit);
+  hipStream_t st = ctx->stream;
+  if (trans && dtype == PLANC_F32) {
+#define TNF(NTV)                                        \
+  do {                                                  \
+    if (g.cw == 4)                                      \
+      launch_tn_f32<NTV, 4>(tm, grid, st, p);           \
+    else                                                \
+      launch_tn_f32<NTV, 2>(tm, grid, st, p);           \
+  } while (0)
+    if (NT == 1) TNF(1);
+    else if (NT == 2) TNF(2);
+    else if (NT == 3) TNF(3);
+    else TNF(4);
+#undef TNF
+  } else if (trans) {
+    const bool safe = !p.avec;
+    if (dtype == PLANC_BF16) {
+      if (NT == 1) launch_tn_h<bf16_t, 1>(safe, grid, st, p);
+      else if (NT == 2) launch_tn_h<bf16_t, 2>(safe, grid, st, p);
+      else if (NT == 3) launch_tn_h<bf16_t, 3>(safe, grid, st, p);
+      else launch_tn_h<bf16_t, 4>(safe, grid, st, p);
+    } else {
+      if (NT == 1) launch_tn_h<f16_t, 1>(safe, grid, st, p);
+      else if (NT == 2) launch_tn_h<f16_t, 2>(safe, grid, st, p);
+      else if (NT == 3) launch_tn_h<f16_t, 3>(safe, grid, st, p);
+      else launch_tn_h<f16_t, 4>(safe, grid, st, p);
+    }
+  } else {
+    if (NT == 1) launch_nn_f32<1>(tm, grid, st, p);
+    else if (NT == 2) launch_nn_f32<2>(tm, grid, st, p);
+    else if (NT == 3) launch_nn_f32<3>(tm, grid, st, p);
+    else launch_nn_f32<4>(tm, grid, st, p);
+  }
+  PLANC_LAUNCH_CHECK(ctx);
+  if (g.nsplit > 1) {
+    const int NA = trans ? g.cw * NT : 4 * NT;
+    dim3 rgrid((unsigned)gpad, (unsigned)NA);
+    if (!trans) {
+      splitk_reduce_kernel<1, 1, 1><<<rgrid, 256, 0, st>>>(p, NA, gpad);
+    } else {
+#define RED(NTV)                                                 \
+  do {                                                           \
+    if (g.cw == 4)                                               \
+      launch_reduce_tn<NTV, 4>(rgrid, st, p, NA, gpad);          \
+    else                                                         \
+      launch_reduce_tn<NTV, 2>(rgrid, st, p, NA, gpad);          \
+  } while (0)
+      if (NT == 1) RED(1);
+      else if (NT == 2) RED(2);
+      else if (NT == 3) RED(3);
+      else RED(4);
+#undef RED
+    }
+    PLANC_LAUNCH_CHECK(ctx);
+  }
+  return PLANC_OK;
+}
+
+bool stream_capturing(hipStream_t st) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(st, &cs) != hipSuccess) return true;
+  return cs != hipStreamCaptureStatusNone;
+}
+
+// Picks the geometry of a product shape: cached per context; otherwise
+// every candidate is timed on the live operands (one warm-up launch and two
+// timed launches each; the products are idempotent) unless tuning is off
+// (PLANC_AUTOTUNE=0) or the stream is being captured into a graph, where the
+// static choice is used.
+int pick_geometry(planc_ctx *ctx, int dtype, int trans, int NT, int tm, const StreamParams &p,
+                  Geometry *out) {
+  const std::vector<long> key = {dtype, trans, NT, tm, p.L, p.K, p.lda, p.th.k, p.avec, p.cmajor};
+  for (const auto &e : ctx->gemm_tuned)
+    if (e.first == key) {
+      const auto &v = e.second;
+      *out = make_geometry(p.L, p.K, trans, dtype, NT, (int)v[0], (int)v[1], v[2]);
+      return PLANC_OK;
+    }
+  std::vector<Geometry> cands = candidate_geometries(ctx, p.L, p.K, trans, dtype, NT);
+  const char *env = getenv("PLANC_AUTOTUNE");
+  const bool tune = !(env && env[0] == '0') && !stream_capturing(ctx->stream) && cands.size() > 1;
+  Geometry best = cands[0];
+  if (tune) {
+    hipEvent_t e0, e1;
+    PLANC_HIP_CHECK(ctx, hipEventCreate(&e0));
+    PLANC_HIP_CHECK(ctx, hipEventCreate(&e1));
+    float best_ms = 0.0f;
+    bool have = false;
+    for (const Geometry &g : cands) {
+      int rc = launch_stream(ctx, dtype, trans, NT, tm, p, g);  // warm-up (and workspace growth)
+      if (rc) return rc;
+      PLANC_HIP_CHECK(ctx, hipEventRecord(e0, ctx->stream));
+      for (int r = 0; r < 2; ++r) {
+        rc = launch_stream(ctx, dtype, trans, NT, tm, p, g);
+        if (rc) return rc;
+      }
+      PLANC_HIP_CHECK(ctx, hipEventRecord(e1, ctx->stream));
+      PLANC_HIP_CHECK(ctx, hipEventSynchronize(e1));
+      float ms = 0.0f;
+      PLANC_HIP_CHECK(ctx, hipEventElapsedTime(&ms, e0, e1));
+      if (!have || ms < best_ms) {
+        best_ms = ms;
+        best = g;
+        have = true;
+      }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
+  if (!stream_capturing(ctx->stream))
+    ctx->gemm_tuned.push_back({key, {best.cw, best.wk, (long)best.nsplit}});
+  *out = best;
+  return PLANC_OK;
+}
+
+int fill_thin(planc_ctx *ctx, ThinDev &th, const float *B, long ldb, const KrpDesc *krp,
+              const float *rowscale, long K, int c0, int kc) {
+  th = ThinDev{};
+  th.k = kc;
+  th.rowscale = rowscale;
+  bool vec = rowscale == nullptr || ((uintptr_t)rowscale % 16 == 0);
+  if (krp && krp->nf > 0) {
+    if (krp->nf > 6) return set_error(ctx, PLANC_EINVAL, "thin_gemm: more than 6 KRP factors");
+    long prod = 1;
+    for (int q = 0; q < krp->nf; ++q) prod *= krp->dim[q];
+    if (prod != K) return set_error(ctx, PLANC_EINVAL, "thin_gemm: KRP rows != K");
+    if (K >= (1L << 31)) return set_error(ctx, PLANC_EINVAL, "thin_gemm: KRP with 2^31 rows or more");
+    th.nf = krp->nf;
+    for (int q = 0; q < krp->nf; ++q) {
+      th.f[q] = krp->f[q] + krp->ld[q] * c0;
+      th.dim[q] = (unsigned)krp->dim[q];
+      th.ld[q] = krp->ld[q];
+    }
+    vec = vec && krp->dim[0] % 4 == 0 && krp->ld[0] % 4 == 0 && ((uintptr_t)krp->f[0] % 16 == 0);
+  } else {
+    if (!B) return set_error(ctx, PLANC_EINVAL, "thin_gemm: null B");
+    if (K >= (1L << 32)) return set_error(ctx, PLANC_EINVAL, "thin_gemm: K too large");
+    th.nf = 0;
+    th.B = B + ldb * c0;
+    th.ldb = ldb;
+    vec = vec && ldb % 4 == 0 && ((uintptr_t)B % 16 == 0);
+  }
+  th.vec = vec ? 1 : 0;
+  return PLANC_OK;
 }
 
 }  // namespace
@@ -492,76 +1107,119 @@ int thin_gemm(planc_ctx *ctx, int dtype, int trans, const void *A, long lda, lon
   if (L == 0) return PLANC_OK;
   const int esz = dtype == PLANC_F32 ? 4 : 2;
   const int vecw = 16 / esz;
-  const bool vec = (lda % vecw == 0) && (((uintptr_t)A) % 16 == 0);
-  const int BK = dtype == PLANC_F32 ? 32 : 64;
+  const bool avec = (lda % vecw == 0) && (((uintptr_t)A) % 16 == 0);
   for (int c0 = 0; c0 < k; c0 += 64) {
-    int kc = (k - c0 < 64) ? (k - c0) : 64;
-    int NT = (kc + 15) / 16;
-    GemmParams p{};
-    p.A = A;
-    p.lda = lda;
-    p.L = L;
-    p.K = K;
-    p.rowscale = rowscale;
-    p.colscale = colscale;
-    p.k = kc;
-    p.krp.nf = 0;
-    if (krp && krp->nf > 0) {
-      if (krp->nf > 6) return set_error(ctx, PLANC_EINVAL, "thin_gemm: more than 6 KRP factors");
-      long prod = 1;
-      for (int q = 0; q < krp->nf; ++q) prod *= krp->dim[q];
-      if (prod != K) return set_error(ctx, PLANC_EINVAL, "thin_gemm: KRP rows != K");
-      p.krp.nf = krp->nf;
-      for (int q = 0; q < krp->nf; ++q) {
-        p.krp.f[q] = krp->f[q] + krp->ld[q] * c0;
-        p.krp.dim[q] = (int)krp->dim[q];
-        p.krp.ld[q] = krp->ld[q];
-      }
-    } else {
-      if (!B) return set_error(ctx, PLANC_EINVAL, "thin_gemm: null B");
-      p.B = B + ldb * c0;
-      p.ldb = ldb;
+    const int kc = (k - c0 < 64) ? (k - c0) : 64;
+    const int NT = (kc + 15) / 16;
+    ThinDev th;
+    int rc = fill_thin(ctx, th, B, ldb, krp, rowscale, K, c0, kc);
+    if (rc) return rc;
+    float *Cs = cmajor ? C + c0 : C + ldc * c0;
+    if (K == 0) {
+      // Empty contraction: C = 0.
+      PLANC_HIP_CHECK(ctx, hipMemset2DAsync(Cs, sizeof(float) * ldc, 0,
+                                            sizeof(float) * (cmajor ? kc : L), cmajor ? L : kc,
+                                            ctx->stream));
+      continue;
     }
-    p.C = cmajor ? C + c0 : C + ldc * c0;
-    p.ldc = ldc;
-    p.cmajor = cmajor;
-    long lblocks = (L + BL - 1) / BL;
-    int splits = 1;
-    if (K > 0) {
-      long target = 4L * ctx->num_cu;
-      long want = (target + lblocks - 1) / lblocks;
-      long maxs = (K + 4L * BK - 1) / (4L * BK);  // each split >= 4 chunks
+    if (dtype != PLANC_F32 && !trans) {
+      // 16-bit NN through the LDS-transposing kernel.
+      GemmParams p{};
+      p.A = A;
+      p.lda = lda;
+      p.L = L;
+      p.K = K;
+      p.th = th;
+      p.C = Cs;
+      p.ldc = ldc;
+      p.cmajor = cmajor;
+      const int BK = 64;
+      long lblocks = (L + BL - 1) / BL;
+      long want = (4L * ctx->num_cu + lblocks - 1) / lblocks;
+      long maxs = (K + 4L * BK - 1) / (4L * BK);
       if (want > maxs) want = maxs;
       if (want < 1) want = 1;
       if (splits_hint > 0) want = splits_hint;
       if (want > 4096) want = 4096;
-      splits = (int)want;
-    }
-    long kchunk = (K + splits - 1) / splits;
-    kchunk = ((kchunk + BK - 1) / BK) * BK;
-    if (kchunk == 0) kchunk = BK;
-    splits = (int)((K + kchunk - 1) / kchunk);
-    if (splits < 1) splits = 1;
-    p.splits = splits;
-    p.kchunk = kchunk;
-    int KP = 16 * NT;
-    if (splits > 1) {
-      p.partial = (float *)ws_get(ctx, 0, sizeof(float) * (size_t)splits * L * KP);
-      if (!p.partial) return set_error(ctx, PLANC_ENOMEM, "thin_gemm: workspace");
-    }
-    dim3 grid((unsigned)lblocks, (unsigned)splits);
-    if (trans)
-      launch<1>(NT, dtype, vec, grid, ctx->stream, p);
-    else
-      launch<0>(NT, dtype, vec, grid, ctx->stream, p);
-    PLANC_LAUNCH_CHECK(ctx);
-    if (splits > 1) {
-      long tot = L * kc;
-      unsigned nb = (unsigned)((tot + 255) / 256);
-      reduce_splits_kernel<<<nb, 256, 0, ctx->stream>>>(p.partial, splits, L, KP, kc, p.C, ldc,
-                                                        cmajor, trans ? colscale : nullptr);
+      long kchunk = (K + want - 1) / want;
+      kchunk = ((kchunk + BK - 1) / BK) * BK;
+      int splits = (int)((K + kchunk - 1) / kchunk);
+      p.splits = splits;
+      p.kchunk = kchunk;
+      const int KP = 16 * NT;
+      if (splits > 1) {
+        p.partial = (float *)ws_get(ctx, 0, sizeof(float) * (size_t)splits * L * KP);
+        if (!p.partial) return set_error(ctx, PLANC_ENOMEM, "thin_gemm: workspace");
+      }
+      dim3 grid((unsigned)lblocks, (unsigned)splits);
+      const bool vec = avec;
+#define NNH(TA, NTV)                                                                   \
+  do {                                                                                 \
+    if (vec)                                                                           \
+      nn_h_kernel<TA, NTV, true><<<grid, NTHR, 0, ctx->stream>>>(p);                   \
+    else                                                                               \
+      nn_h_kernel<TA, NTV, false><<<grid, NTHR, 0, ctx->stream>>>(p);                  \
+  } while (0)
+      if (dtype == PLANC_BF16) {
+        if (NT == 1) NNH(bf16_t, 1);
+        else if (NT == 2) NNH(bf16_t, 2);
+        else if (NT == 3) NNH(bf16_t, 3);
+        else NNH(bf16_t, 4);
+      } else {
+        if (NT == 1) NNH(f16_t, 1);
+        else if (NT == 2) NNH(f16_t, 2);
+        else if (NT == 3) NNH(f16_t, 3);
+        else NNH(f16_t, 4);
+      }
+#undef NNH
       PLANC_LAUNCH_CHECK(ctx);
+      if (splits > 1) {
+        long tot = L * kc;
+        unsigned nb = (unsigned)((tot + 255) / 256);
+        reduce_splits_kernel<<<nb, 256, 0, ctx->stream>>>(p.partial, splits, L, KP, kc, Cs, ldc, cmajor);
+        PLANC_LAUNCH_CHECK(ctx);
+      }
+      continue;
     }
+
+    // Streaming kernels.
+    StreamParams p{};
+    p.A = A;
+    p.lda = lda;
+    p.L = L;
+    p.K = K;
+    p.th = th;
+    p.colscale = trans ? colscale : nullptr;
+    p.C = Cs;
+    p.ldc = ldc;
+    p.cmajor = cmajor;
+    p.avec = avec ? 1 : 0;
+    p.cvec = (ldc % 4 == 0 && ((uintptr_t)Cs % 16 == 0)) ? 1 : 0;
+    int tm = thin_mode(th, avec, rowscale);
+    if (!trans && L % 4 != 0) tm = THIN_SAFE;
+    if (trans && dtype != PLANC_F32) {
+      // 16-bit TN: the thin operand packed once into the MFMA A-operand layout.
+      const long ksteps = (K + 31) / 32;
+      uint4 *pk = (uint4 *)ws_get(ctx, 6, (size_t)ksteps * NT * 64 * 16);
+      if (!pk) return set_error(ctx, PLANC_ENOMEM, "thin_gemm: packed operand");
+      const long tot = ksteps * NT * 64;
+      const unsigned nb = (unsigned)((tot + 255) / 256);
+      if (dtype == PLANC_BF16)
+        pack_thin16_kernel<bf16_t><<<nb, 256, 0, ctx->stream>>>(th, K, NT, ksteps, pk);
+      else
+        pack_thin16_kernel<f16_t><<<nb, 256, 0, ctx->stream>>>(th, K, NT, ksteps, pk);
+      PLANC_LAUNCH_CHECK(ctx);
+      p.thin16 = pk;
+    }
+    Geometry g;
+    if (splits_hint > 0) {
+      g = make_geometry(L, K, trans, dtype, NT, trans && dtype == PLANC_F32 ? 2 : 4, 1, splits_hint);
+    } else {
+      rc = pick_geometry(ctx, dtype, trans, NT, tm, p, &g);
+      if (rc) return rc;
+    }
+    rc = launch_stream(ctx, dtype, trans, NT, tm, p, g);
+    if (rc) return rc;
   }
   return PLANC_OK;
 }

@@ -414,6 +414,24 @@ __global__ void convert_kernel(const float *A, long rows, long cols, long lda, u
   }
 }
 
+// out (cols x rows, ld ldo) = in^T for a 16-bit in (rows x cols, ld ldi):
+// 64 x 64 tiles through LDS, both sides coalesced.
+__global__ __launch_bounds__(256) void transpose16_kernel(const uint16_t *in, long rows, long cols,
+                                                          long ldi, uint16_t *out, long ldo) {
+  __shared__ uint16_t tile[64][66];
+  const long r0 = (long)blockIdx.x * 64, c0 = (long)blockIdx.y * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  for (int j = ty; j < 64; j += 4) {
+    const long r = r0 + tx, c = c0 + j;
+    tile[j][tx] = (r < rows && c < cols) ? in[r + ldi * c] : (uint16_t)0;
+  }
+  __syncthreads();
+  for (int j = ty; j < 64; j += 4) {
+    const long c = c0 + tx, r = r0 + j;
+    if (r < rows && c < cols) out[c + ldo * r] = tile[tx][j];
+  }
+}
+
 template <int KM>
 int launch_rows(planc_ctx *ctx, int kind, float *H, long n, int k, long ldh, const float *N,
                 long nrs, long ncs, const double *G, float eps) {
@@ -442,8 +460,15 @@ int try_coop(planc_ctx *ctx, float *W, long m, int k, long ldw, const float *AH,
   double *part = (double *)ws_get(ctx, 2, sizeof(double) * (size_t)KM * nb);
   if (!part) return set_error(ctx, PLANC_ENOMEM, "hals_cols: workspace");
   void *args[] = {&W, &m, &k, &ldw, &AH, &ldah, &G, &part};
-  PLANC_HIP_CHECK(ctx, hipLaunchCooperativeKernel((const void *)hals_cols_coop_kernel<KM, RPT>,
-                                                  dim3(nb), dim3(NTHR), args, 0, ctx->stream));
+  hipError_t le = hipLaunchCooperativeKernel((const void *)hals_cols_coop_kernel<KM, RPT>, dim3(nb),
+                                             dim3(NTHR), args, 0, ctx->stream);
+  if (le != hipSuccess) {
+    // The runtime can refuse a cooperative launch (e.g. while another
+    // process holds queues on the device); nothing was launched, the
+    // caller takes the two-launches-per-column path.
+    (void)hipGetLastError();
+    return PLANC_OK;
+  }
   *done = true;
   return PLANC_OK;
 }
@@ -587,6 +612,16 @@ int convert16(planc_ctx *ctx, int dtype, const float *A, long rows, long cols, l
   else
     convert_kernel<PLANC_F16><<<(unsigned)nb, 256, 0, ctx->stream>>>(A, rows, cols, lda, (uint16_t *)out,
                                                                       ldo, colscale);
+  PLANC_LAUNCH_CHECK(ctx);
+  return PLANC_OK;
+}
+
+int transpose16(planc_ctx *ctx, const void *in, long rows, long cols, long ldi, void *out, long ldo) {
+  if (rows <= 0 || cols <= 0) return PLANC_OK;
+  const long gx = (rows + 63) / 64, gy = (cols + 63) / 64;
+  if (gy > 65535) return set_error(ctx, PLANC_EINVAL, "transpose16: too many columns");
+  transpose16_kernel<<<dim3((unsigned)gx, (unsigned)gy), 256, 0, ctx->stream>>>(
+      (const uint16_t *)in, rows, cols, ldi, (uint16_t *)out, ldo);
   PLANC_LAUNCH_CHECK(ctx);
   return PLANC_OK;
 }

@@ -30,65 +30,105 @@ struct ContractParams {
 };
 
 // Product of F_t(digit_t(idx), c) over block positions [t0, t1) with idx the
-// mixed-radix index of those modes (lowest fastest).
-__device__ inline float weight(const ContractParams &p, long idx, int t0, int t1, int c) {
+// mixed-radix index of those modes (lowest fastest).  idx < 2^32.
+__device__ inline float weight(const ContractParams &p, unsigned idx, int t0, int t1, int c) {
   float w = 1.0f;
   for (int t = t0; t < t1; ++t) {
-    long dg = idx % p.d[t];
-    idx /= p.d[t];
+    unsigned nx = idx / (unsigned)p.d[t];
+    unsigned dg = idx - nx * (unsigned)p.d[t];
+    idx = nx;
     w *= p.F[t][dg + p.ldf[t] * c];
   }
   return w;
 }
 
-// Small inner extent: lanes walk consecutive output indices i, the four
-// wavefronts split the outer index b, summed in wave order.
-__global__ __launch_bounds__(NTHR) void contract_small_inner_kernel(ContractParams p) {
-  __shared__ float red[4][64];
+constexpr int WMAX = 4096;  // weights held in LDS per block
+
+// inner == 1 (target mode first in the block): lanes walk consecutive output
+// indices i (coalesced), the 16 wavefronts split the outer index b
+// (b = wave, wave + 16, ...), summed in wave order.
+__global__ __launch_bounds__(1024) void contract_outer_kernel(ContractParams p) {
+  __shared__ float wo[WMAX];
+  __shared__ float red[16][64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int c = blockIdx.y;
   const long i = (long)blockIdx.x * 64 + lane;
+  const bool cached = p.outer <= WMAX;
+  if (cached)
+    for (long b = threadIdx.x; b < p.outer; b += blockDim.x) wo[b] = weight(p, (unsigned)b, p.tn + 1, p.nb, c);
+  __syncthreads();
   float acc = 0.0f;
   if (i < p.In) {
-    for (long b = wave; b < p.outer; b += 4) {
-      float wo = weight(p, b, p.tn + 1, p.nb, c);
-      float s = 0.0f;
-      for (long a = 0; a < p.inner; ++a) {
-        float wi = weight(p, a, 0, p.tn, c);
-        s = fmaf(p.P[(a + p.inner * (i + p.In * b)) + p.ldp * c], wi, s);
+    const float *src = p.P + p.ldp * c + i;
+    const long st = p.In * 16;
+    long b = wave;
+    if (cached) {
+      for (; b + 16 * 7 < p.outer; b += 16 * 8) {
+        float v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = src[p.In * b + st * u];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc = fmaf(v[u], wo[b + 16 * u], acc);
       }
-      acc = fmaf(s, wo, acc);
+      for (; b < p.outer; b += 16) acc = fmaf(src[p.In * b], wo[b], acc);
+    } else {
+      for (; b < p.outer; b += 16) acc = fmaf(src[p.In * b], weight(p, (unsigned)b, p.tn + 1, p.nb, c), acc);
     }
   }
   red[wave][lane] = acc;
   __syncthreads();
   if (wave == 0 && i < p.In) {
-    float t = red[0][lane] + red[1][lane] + red[2][lane] + red[3][lane];
+    float t = 0.0f;
+#pragma unroll
+    for (int w = 0; w < 16; ++w) t += red[w][lane];
     p.M[c + p.ldo * i] = t;
   }
 }
 
-// Large inner extent: one block per (i, c), threads over (a, b) with a
-// fastest (contiguous reads), deterministic block reduction.
-__global__ __launch_bounds__(NTHR) void contract_large_inner_kernel(ContractParams p) {
-  __shared__ double red[NTHR / 64];
-  const long i = blockIdx.x;
-  const int c = blockIdx.y;
-  const long tot = p.inner * p.outer;
-  float acc = 0.0f;
-  for (long f = threadIdx.x; f < tot; f += NTHR) {
-    long a = f % p.inner, b = f / p.inner;
-    float w = weight(p, a, 0, p.tn, c) * weight(p, b, p.tn + 1, p.nb, c);
-    acc = fmaf(p.P[(a + p.inner * (i + p.In * b)) + p.ldp * c], w, acc);
-  }
-  double v = wave_sum_d((double)acc);
+// inner > 1: one block per (tile of TI output indices, c); threads walk the
+// contiguous inner index a, loop over b; each output is a deterministic
+// block reduction.
+constexpr int TI = 8;
+__global__ __launch_bounds__(256) void contract_inner_kernel(ContractParams p) {
+  __shared__ float wi[WMAX];
+  __shared__ float wo[WMAX];
+  __shared__ double red[TI][4];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (lane == 0) red[wave] = v;
+  const int c = blockIdx.y;
+  const long i0 = (long)blockIdx.x * TI;
+  const bool ci = p.inner <= WMAX, co = p.outer <= WMAX;
+  if (ci)
+    for (long a = threadIdx.x; a < p.inner; a += blockDim.x) wi[a] = weight(p, (unsigned)a, 0, p.tn, c);
+  if (co)
+    for (long b = threadIdx.x; b < p.outer; b += blockDim.x) wo[b] = weight(p, (unsigned)b, p.tn + 1, p.nb, c);
   __syncthreads();
-  if (threadIdx.x == 0) {
-    double s = 0.0;
-    for (int w = 0; w < NTHR / 64; ++w) s += red[w];
-    p.M[c + p.ldo * i] = (float)s;
+  float acc[TI];
+#pragma unroll
+  for (int t = 0; t < TI; ++t) acc[t] = 0.0f;
+  const float *Pc = p.P + p.ldp * c;
+  for (long b = 0; b < p.outer; ++b) {
+    const float wb = co ? wo[b] : weight(p, (unsigned)b, p.tn + 1, p.nb, c);
+    for (long a = threadIdx.x; a < p.inner; a += blockDim.x) {
+      const float w = (ci ? wi[a] : weight(p, (unsigned)a, 0, p.tn, c)) * wb;
+#pragma unroll
+      for (int t = 0; t < TI; ++t) {
+        const long i = i0 + t;
+        if (i < p.In) acc[t] = fmaf(Pc[a + p.inner * (i + p.In * b)], w, acc[t]);
+      }
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < TI; ++t) {
+    double v = wave_sum_d((double)acc[t]);
+    if (lane == 0) red[t][wave] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < TI) {
+    const long i = i0 + threadIdx.x;
+    if (i < p.In) {
+      double s = red[threadIdx.x][0] + red[threadIdx.x][1] + red[threadIdx.x][2] + red[threadIdx.x][3];
+      p.M[c + p.ldo * i] = (float)s;
+    }
   }
 }
 
@@ -169,6 +209,167 @@ __global__ void transpose_kernel(const float *in, long I, long ldi, int k, float
   out[c + ldo * i] = in[i + ldi * c];
 }
 
+// ---------------------------------------------------------------------------
+// One mode update of AUNTF for a factor that fits one block's LDS
+// (k * (I + 1) <= MODE_LDS_FLOATS): S = Hadamard of the other Grams, the
+// HALS (ntf/ntfhals.hpp:13-29) or MU row update from this mode's MTTKRP,
+// normalize(mode) (ntf/auntf.hpp:66-79: lambda = column 2-norms, columns
+// divided by them), the FP64 Gram of the new factor, and optionally the CP
+// relative error of ntf/auntf.hpp:224-245 from this mode's MTTKRP.  One
+// launch replaces six.  kind: 0 MU, 1 HALS, 2 no row update (the factor was
+// solved already, e.g. by BPP).
+struct ModeParams {
+  float *F;
+  long I, ldf;
+  int k;
+  const float *M;  // MTTKRP: element (x, i) at M[i * mrs + x * mcs]
+  long mrs, mcs;
+  const double *g[8];
+  int ng, skip;
+  double *gout, *lambda;
+  int kind;
+  float eps;
+  const double *normX2;
+  double *rel_out;
+};
+
+constexpr int MODE_LDS_FLOATS = 24576;
+constexpr int MODE_THREADS = 512;
+
+__device__ inline double block_sum_mode(double v, double *red) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  v = wave_sum_d(v);
+  __syncthreads();
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int w = 0; w < MODE_THREADS / 64; ++w) s += red[w];
+  return s;
+}
+
+template <int KM>
+__global__ __launch_bounds__(MODE_THREADS) void mode_update_kernel(ModeParams p) {
+  __shared__ float Sf[KM][KM + 1];
+  __shared__ float Fs[MODE_LDS_FLOATS];  // column c at Fs[c * (I + 1) + i]
+  __shared__ double nrm[KM];
+  __shared__ double red[MODE_THREADS / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int k = p.k;
+  const long I = p.I, ldl = I + 1;
+  for (int e = tid; e < KM * KM; e += MODE_THREADS) {
+    const int a = e % KM, b = e / KM;
+    float v = 0.0f;
+    if (a < k && b < k) {
+      double d = 1.0;
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        if (q < p.ng && q != p.skip) d *= p.g[q][a + (long)k * b];
+      v = (float)d;
+    }
+    Sf[a][b] = v;
+  }
+  __syncthreads();
+  for (long i = tid; i < I; i += MODE_THREADS) {
+    float h[KM];
+#pragma unroll
+    for (int t = 0; t < KM; ++t) h[t] = (t < k) ? p.F[i + p.ldf * t] : 0.0f;
+    // The column loops are not unrolled; h[] is only indexed by unrolled t
+    // loops (selects), so it stays in registers.
+    if (p.kind == 1) {
+      const float floor_v = 1e-17f;
+#pragma unroll 1
+      for (int x = 0; x < k; ++x) {
+        float s = 0.0f, hx = 0.0f;
+#pragma unroll
+        for (int t = 0; t < KM; ++t) {
+          s = fmaf(h[t], Sf[t][x], s);
+          hx = (t == x) ? h[t] : hx;
+        }
+        float v = hx + (p.M[i * p.mrs + x * p.mcs] - s);
+        v = (v < floor_v) ? floor_v : v;
+#pragma unroll
+        for (int t = 0; t < KM; ++t) h[t] = (t == x) ? v : h[t];
+      }
+#pragma unroll
+      for (int t = 0; t < KM; ++t)
+        if (t < k) Fs[t * ldl + i] = h[t];
+    } else if (p.kind == 0) {
+#pragma unroll 1
+      for (int c = 0; c < k; ++c) {
+        float s = 0.0f, hc = 0.0f;
+#pragma unroll
+        for (int t = 0; t < KM; ++t) {
+          s = fmaf(h[t], Sf[t][c], s);
+          hc = (t == c) ? h[t] : hc;
+        }
+        Fs[c * ldl + i] = (hc * p.M[i * p.mrs + c * p.mcs]) / (s + p.eps);
+      }
+    } else {
+#pragma unroll
+      for (int t = 0; t < KM; ++t)
+        if (t < k) Fs[t * ldl + i] = h[t];
+    }
+  }
+  __syncthreads();
+  for (int c = wave; c < k; c += MODE_THREADS / 64) {
+    double s = 0.0;
+    for (long i = lane; i < I; i += 64) {
+      const double v = Fs[c * ldl + i];
+      s += v * v;
+    }
+    s = wave_sum_d(s);
+    if (lane == 0) {
+      const double nv = sqrt(s);
+      nrm[c] = nv;
+      p.lambda[c] = nv;
+    }
+  }
+  __syncthreads();
+  for (long e = tid; e < I * k; e += MODE_THREADS) {
+    const int c = (int)(e / I);
+    const long i = e - (long)c * I;
+    float v = Fs[c * ldl + i];
+    const double nv = nrm[c];
+    if (nv > 0) v = (float)((double)v / nv);
+    Fs[c * ldl + i] = v;
+    p.F[i + p.ldf * c] = v;
+  }
+  __syncthreads();
+  for (int e = tid; e < k * k; e += MODE_THREADS) {
+    const int a = e % k, b = e / k;
+    if (a > b) continue;
+    double s = 0.0;
+    const float *fa = Fs + a * ldl, *fb = Fs + b * ldl;
+    for (long i = 0; i < I; ++i) s += (double)fa[i] * (double)fb[i];
+    p.gout[a + (long)k * b] = s;
+    p.gout[b + (long)k * a] = s;
+  }
+  if (!p.rel_out) return;
+  __syncthreads();
+  double t1 = 0.0;
+  for (long e = tid; e < I * k; e += MODE_THREADS) {
+    const int c = (int)(e / I);
+    const long i = e - (long)c * I;
+    t1 += nrm[c] * (double)Fs[c * ldl + i] * (double)p.M[i * p.mrs + c * p.mcs];
+  }
+  double xh = 0.0;
+  for (int e = tid; e < k * k; e += MODE_THREADS) {
+    const int a = e % k, b = e / k;
+    double v = nrm[a] * nrm[b];
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      if (q < p.ng) v *= (q == p.skip) ? p.gout[e] : p.g[q][e];
+    xh += v;
+  }
+  t1 = block_sum_mode(t1, red);
+  xh = block_sum_mode(xh, red);
+  if (tid == 0) {
+    double e2 = *p.normX2 - 2.0 * t1 + xh;
+    if (e2 < 0) e2 = 0;
+    *p.rel_out = sqrt(e2 / *p.normX2);
+  }
+}
+
 }  // namespace
 
 // M (k x I_n) from a projection P (E x k) over a block of modes with dims d,
@@ -195,12 +396,14 @@ int contract_projection(planc_ctx *ctx, const float *P, long ldp, int nb, const 
   }
   p.In = d[tn];
   if (p.In == 0) return PLANC_OK;
-  if (p.inner < 32) {
+  long E = p.inner * p.In * p.outer;
+  if (E >= (1L << 32)) return set_error(ctx, PLANC_EINVAL, "contract: block of 2^32 rows or more");
+  if (p.inner == 1) {
     dim3 grid((unsigned)((p.In + 63) / 64), (unsigned)k);
-    contract_small_inner_kernel<<<grid, NTHR, 0, ctx->stream>>>(p);
+    contract_outer_kernel<<<grid, 1024, 0, ctx->stream>>>(p);
   } else {
-    dim3 grid((unsigned)p.In, (unsigned)k);
-    contract_large_inner_kernel<<<grid, NTHR, 0, ctx->stream>>>(p);
+    dim3 grid((unsigned)((p.In + TI - 1) / TI), (unsigned)k);
+    contract_inner_kernel<<<grid, 256, 0, ctx->stream>>>(p);
   }
   PLANC_LAUNCH_CHECK(ctx);
   return PLANC_OK;
@@ -244,6 +447,46 @@ int transpose_to_k_major(planc_ctx *ctx, const float *in, long I, long ldi, int 
   long tot = I * k;
   if (tot == 0) return PLANC_OK;
   transpose_kernel<<<(unsigned)((tot + 255) / 256), 256, 0, ctx->stream>>>(in, I, ldi, k, out, ldo);
+  PLANC_LAUNCH_CHECK(ctx);
+  return PLANC_OK;
+}
+
+}  // namespace planc
+
+namespace planc {
+
+bool mode_update_fits(long I, int k) { return k >= 1 && k <= 64 && (long)k * (I + 1) <= MODE_LDS_FLOATS; }
+
+int mode_update(planc_ctx *ctx, int kind, float eps, float *F, long I, long ldf, int k, const float *M,
+                long mrs, long mcs, const double *const *grams, int ng, int skip, double *gout,
+                double *lambda, const double *normX2, double *rel_out) {
+  if (!mode_update_fits(I, k) || ng < 1 || ng > 8 || skip < 0 || skip >= ng)
+    return set_error(ctx, PLANC_EINVAL, "mode_update: factor does not fit one block");
+  ModeParams p{};
+  p.F = F;
+  p.I = I;
+  p.ldf = ldf;
+  p.k = k;
+  p.M = M;
+  p.mrs = mrs;
+  p.mcs = mcs;
+  for (int q = 0; q < ng; ++q) p.g[q] = grams[q];
+  p.ng = ng;
+  p.skip = skip;
+  p.gout = gout;
+  p.lambda = lambda;
+  p.kind = kind;
+  p.eps = eps;
+  p.normX2 = normX2;
+  p.rel_out = rel_out;
+  if (k <= 16)
+    mode_update_kernel<16><<<1, MODE_THREADS, 0, ctx->stream>>>(p);
+  else if (k <= 32)
+    mode_update_kernel<32><<<1, MODE_THREADS, 0, ctx->stream>>>(p);
+  else if (k <= 48)
+    mode_update_kernel<48><<<1, MODE_THREADS, 0, ctx->stream>>>(p);
+  else
+    mode_update_kernel<64><<<1, MODE_THREADS, 0, ctx->stream>>>(p);
   PLANC_LAUNCH_CHECK(ctx);
   return PLANC_OK;
 }
