@@ -283,7 +283,11 @@ int planc_hals_update_cols(planc_ctx *ctx, float *d_W, long m, int k, long ldw, 
                            long ldah, const double *d_G) {
   if (check_ctx(ctx)) return PLANC_EINVAL;
   if (ldw < m || ldah < m || k <= 0) return set_error(ctx, PLANC_EINVAL, "planc_hals_update_cols: bad arguments");
-  return hals_cols(ctx, d_W, m, k, ldw, d_AH, ldah, d_G);
+  int rc = hals_sync_reset(ctx);
+  if (rc) return rc;
+  rc = hals_cols(ctx, d_W, m, k, ldw, d_AH, ldah, d_G);
+  if (rc) return rc;
+  return hals_sync_check(ctx);
 }
 
 int planc_normalize_by_w(planc_ctx *ctx, float *d_W, long m, long ldw, float *d_H, long n, long ldh,
@@ -479,7 +483,12 @@ int planc_nmf_run(planc_ctx *ctx, const planc_nmf_desc *desc, const float *d_A, 
     if (hflag) return set_error(ctx, PLANC_ENOCONV, "BPP failed: more than 5k pivot iterations");
     return PLANC_OK;
   }
-  if (algo == PLANC_HALS) RC(planc_normalize_by_w(ctx, d_W, m, m, d_H, n, n, k));
+  double *wnorms = (double *)ws_get(ctx, 9, sizeof(double) * k);
+  if (!wnorms) return set_error(ctx, PLANC_ENOMEM, "planc_nmf_run: workspace");
+  if (algo == PLANC_HALS) {
+    RC(hals_sync_reset(ctx));
+    RC(planc_normalize_by_w(ctx, d_W, m, m, d_H, n, n, k));
+  }
   for (int it = 0; it < iters; ++it) {
     // H half-step: W^T A, W^T W (+ regH).
     RC(thin_gemm(ctx, dt, 1, Aop, ldop, n, m, d_W, m, nullptr, nullptr, cs, k, WtA, k, 1, 0));
@@ -496,21 +505,26 @@ int planc_nmf_run(planc_ctx *ctx, const planc_nmf_desc *desc, const float *d_A, 
       RC(row_update(ctx, 0, d_W, m, k, m, AH, 1, m, Greg, 1e-17f));
     else
       RC(hals_cols(ctx, d_W, m, k, m, AH, m, Greg));
+    if (desc->compute_error || algo == PLANC_HALS) RC(gram(ctx, d_W, m, k, m, WtW2));
     if (desc->compute_error) {
       // <W, A H> with the H of this iteration equals tr(H^T A^T W) of
       // computeObjectiveError (common/nmf.hpp:391-396) without the third
       // product; both traces are invariant under normalize_by_W.
       RC(dot(ctx, d_W, 1, m, AH, 1, m, m, k, dotv, 0));
-      RC(gram(ctx, d_W, m, k, m, WtW2));
       RC(nmf_relerr(ctx, normA2, dotv, WtW2, HtH, k, rel + it, nullptr));
     }
-    if (algo == PLANC_HALS) RC(planc_normalize_by_w(ctx, d_W, m, m, d_H, n, n, k));
+    if (algo == PLANC_HALS) {
+      // normalize_by_W: the column norms are the square roots of diag(W^T W).
+      RC(gram_norms(ctx, WtW2, k, wnorms));
+      RC(scale_cols(ctx, d_W, m, m, d_H, n, n, k, wnorms));
+    }
   }
   if (algo == PLANC_MU) RC(planc_normalize_by_w(ctx, d_W, m, m, d_H, n, n, k));
 #undef RC
   if (h_relerr && desc->compute_error && iters > 0)
     PLANC_HIP_CHECK(ctx, hipMemcpyAsync(h_relerr, rel, sizeof(double) * iters, hipMemcpyDeviceToHost, ctx->stream));
   PLANC_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  if (algo == PLANC_HALS) return hals_sync_check(ctx);
   return PLANC_OK;
 }
 

@@ -110,7 +110,7 @@ namespace planc {
 // 6 packed 16-bit thin operand, 7 16-bit copy of A / X, 8 column scales of that copy, 9 entry-point norms,
 // 10 BPP per-column iteration counts, 11 BPP failure flag, 12 CP-loss scalar,
 // 13 CP-gradient float buffers, 14 CP-gradient Grams, 15 loop-graph counter,
-// 16 transposed 16-bit copy of A / X.
+// 16 transposed 16-bit copy of A / X, 17 HALS grid-barrier counter and flag.
 void *ws_get(planc_ctx *ctx, int slot, size_t bytes);
 
 int set_error(planc_ctx *ctx, int code, const std::string &msg);
@@ -191,6 +191,12 @@ int dot(planc_ctx *ctx, const float *X, long xrs, long xcs, const float *Y, long
 int nmf_relerr(planc_ctx *ctx, const double *normA2, const double *dotv, const double *WtW,
                const double *HtH, int k, double *relerr_out, double *fit_out);
 int apply_reg(planc_ctx *ctx, double *G, int k, const double *reg);
+// Time-out flag of the HALS column sweep's grid barrier: reset before a
+// run, checked (synchronising) after it.
+int hals_sync_reset(planc_ctx *ctx);
+int hals_sync_check(planc_ctx *ctx);
+// norms[c] = sqrt(G(c, c)).
+int gram_norms(planc_ctx *ctx, const double *G, int k, double *norms);
 int convert16(planc_ctx *ctx, int dtype, const float *A, long rows, long cols, long lda, void *out,
               long ldo, float *colscale);
 // out (cols x rows) = in^T, 16-bit elements.

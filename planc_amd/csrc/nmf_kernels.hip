@@ -5,8 +5,6 @@
 // Row-parallel updates keep one row of the factor (k <= 64 values) in
 // registers and the k x k Gram in LDS, so an update costs one read and one
 // write of the factor plus one read of the numerator.
-#include <hip/hip_cooperative_groups.h>
-
 #include "common.h"
 
 namespace planc {
@@ -35,9 +33,11 @@ __device__ inline double block_sum(double v, double *red) {
 }
 
 // ---------------------------------------------------------------------------
-// G = X^T X in FP64.  Each block takes a 128-row chunk into LDS; thread
-// (a, b) accumulates a 4x4 tile of G.  Partials per chunk are summed in
-// chunk order by gram_reduce_kernel.
+// G = X^T X in FP64.  Each block walks 128-row chunks (grid-stride) through
+// LDS; thread (a, b) accumulates a 4x4 tile of G across all its chunks and
+// the block writes one partial.  gram_reduce_kernel sums the block partials
+// in block order (four waves over interleaved blocks, combined in wave
+// order).
 constexpr int GRAM_RC = 128;
 
 __global__ __launch_bounds__(NTHR) void gram_partial_kernel(const float *X, long rows, int k,
@@ -45,6 +45,12 @@ __global__ __launch_bounds__(NTHR) void gram_partial_kernel(const float *X, long
   __shared__ float Xs[GRAM_RC][64 + 4];
   const int tid = threadIdx.x;
   const int ta = tid >> 4, tb = tid & 15;
+  double acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
+  const bool active = 4 * ta < kp && 4 * tb < kp;
   for (long chunk = blockIdx.x; chunk * GRAM_RC < rows; chunk += gridDim.x) {
     long r0 = chunk * GRAM_RC;
     __syncthreads();
@@ -54,12 +60,7 @@ __global__ __launch_bounds__(NTHR) void gram_partial_kernel(const float *X, long
       Xs[r][c] = (gr < rows && c < k) ? X[gr + ldx * c] : 0.0f;
     }
     __syncthreads();
-    double acc[4][4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
-    if (4 * ta < kp && 4 * tb < kp) {
+    if (active) {
       for (int r = 0; r < GRAM_RC; ++r) {
         double a[4], b[4];
 #pragma unroll
@@ -72,24 +73,45 @@ __global__ __launch_bounds__(NTHR) void gram_partial_kernel(const float *X, long
 #pragma unroll
           for (int j = 0; j < 4; ++j) acc[i][j] += a[i] * b[j];
       }
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          int ga = 4 * ta + i, gb = 4 * tb + j;
-          part[(size_t)chunk * kp * kp + ga + (size_t)kp * gb] = acc[i][j];
-        }
     }
+  }
+  if (active) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        int ga = 4 * ta + i, gb = 4 * tb + j;
+        part[(size_t)blockIdx.x * kp * kp + ga + (size_t)kp * gb] = acc[i][j];
+      }
   }
 }
 
-__global__ void gram_reduce_kernel(const double *part, long nchunks, int k, int kp, double *G) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= k * k) return;
-  int a = e % k, b = e / k;
+// Block of 256 threads per 64 consecutive entries e of the kp x kp partial.
+__global__ __launch_bounds__(NTHR) void gram_reduce_kernel(const double *part, long nparts, int k, int kp,
+                                                           double *G) {
+  __shared__ double red[4][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const long e = (long)blockIdx.x * 64 + lane;
+  const long kk = (long)kp * kp;
   double s = 0.0;
-  for (long c = 0; c < nchunks; ++c) s += part[(size_t)c * kp * kp + a + (size_t)kp * b];
-  G[a + (size_t)k * b] = s;
+  if (e < kk) {
+    long b = wave;
+    for (; b + 12 < nparts; b += 16) {
+      double v0 = part[b * kk + e], v1 = part[(b + 4) * kk + e];
+      double v2 = part[(b + 8) * kk + e], v3 = part[(b + 12) * kk + e];
+      s += v0;
+      s += v1;
+      s += v2;
+      s += v3;
+    }
+    for (; b < nparts; b += 4) s += part[b * kk + e];
+  }
+  red[wave][lane] = s;
+  __syncthreads();
+  if (wave == 0 && e < kk) {
+    const int a = (int)(e % kp), b = (int)(e / kp);
+    if (a < k && b < k) G[a + (long)k * b] = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -112,16 +134,19 @@ __global__ __launch_bounds__(NTHR) void mu_update_kernel(float *H, long n, int k
   load_gram<KM>(Gs, G, k);
   long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
-  float h[KM];
+  float h[KM], nv[KM];
 #pragma unroll
-  for (int t = 0; t < KM; ++t) h[t] = (t < k) ? H[j + ldh * t] : 0.0f;
+  for (int t = 0; t < KM; ++t) {
+    h[t] = (t < k) ? H[j + ldh * t] : 0.0f;
+    nv[t] = (t < k) ? N[j * nrs + t * ncs] : 0.0f;
+  }
 #pragma unroll
   for (int c = 0; c < KM; ++c) {
     if (c < k) {
       float s = 0.0f;
 #pragma unroll
       for (int t = 0; t < KM; ++t) s = fmaf(h[t], Gs[t][c], s);
-      H[j + ldh * c] = (h[c] * N[j * nrs + c * ncs]) / (s + eps);
+      H[j + ldh * c] = (h[c] * nv[c]) / (s + eps);
     }
   }
 }
@@ -137,16 +162,19 @@ __global__ __launch_bounds__(NTHR) void hals_rows_kernel(float *H, long n, int k
   long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   const float floor_v = 1e-17f;
-  float h[KM];
+  float h[KM], nv[KM];
 #pragma unroll
-  for (int t = 0; t < KM; ++t) h[t] = (t < k) ? H[j + ldh * t] : 0.0f;
+  for (int t = 0; t < KM; ++t) {
+    h[t] = (t < k) ? H[j + ldh * t] : 0.0f;
+    nv[t] = (t < k) ? N[j * nrs + t * ncs] : 0.0f;
+  }
 #pragma unroll
   for (int x = 0; x < KM; ++x) {
     if (x < k) {
       float s = 0.0f;
 #pragma unroll
       for (int t = 0; t < KM; ++t) s = fmaf(h[t], Gs[t][x], s);
-      float v = h[x] + (N[j * nrs + x * ncs] - s);
+      float v = h[x] + (nv[x] - s);
       h[x] = (v < floor_v) ? floor_v : v;
     }
   }
@@ -155,19 +183,51 @@ __global__ __launch_bounds__(NTHR) void hals_rows_kernel(float *H, long n, int k
     if (t < k) H[j + ldh * t] = h[t];
 }
 
+// Grid-wide barrier of a cooperative launch (all blocks co-resident).  Thread
+// 0 of each block, after its earlier stores have completed, adds one arrival
+// to *counter and waits until gridDim.x arrivals of generation gen are
+// visible.  Agent-scope atomics only, no cache writeback: data exchanged
+// across it must be written with agent-scope atomic stores and read with
+// agent-scope atomic loads.  The wait is bounded: on time-out *err is set,
+// later barriers return at once and the caller reports the failure.
+__device__ inline bool grid_barrier(int *counter, int *err, int gen) {
+  __shared__ int s_ok;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __hip_atomic_fetch_add(counter, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int target = (gen + 1) * (int)gridDim.x;
+    int ok = 1;
+    for (int spin = 0;; ++spin) {
+      if (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) break;
+      if (spin >= (1 << 20) || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+        __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        ok = 0;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    s_ok = ok;
+  }
+  __syncthreads();
+  return s_ok != 0;
+}
+
+__global__ void zero_int_kernel(int *p) { *p = 0; }
+
 // HALS over the columns of W with per-column normalisation (nmf/hals.hpp:
 // 89-102).  Persistent cooperative kernel: each thread owns RPT rows of W in
-// registers; one grid-wide barrier per column x publishes the column's sum of
+// registers; one grid barrier per column x publishes the column's sum of
 // squares (block partials summed in block order by every block, so every
-// block derives the identical norm).
+// block derives the identical norm).  sync[0] is the barrier counter (zero
+// at launch), sync[1] the time-out flag.
 template <int KM, int RPT>
 __global__ __launch_bounds__(NTHR) void hals_cols_coop_kernel(float *W, long m, int k, long ldw,
                                                               const float *AH, long ldah,
-                                                              const double *G, double *part) {
+                                                              const double *G, double *part, int *sync) {
   __shared__ float Gs[KM][KM + 1];
   __shared__ double red[NTHR / 64];
   load_gram<KM>(Gs, G, k);
-  cooperative_groups::grid_group grid = cooperative_groups::this_grid();
   const long total = (long)gridDim.x * blockDim.x;
   const long gtid = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const float floor_v = 1e-17f;
@@ -203,11 +263,14 @@ __global__ __launch_bounds__(NTHR) void hals_cols_coop_kernel(float *W, long m, 
       }
     }
     double bs = block_sum(lsum, red);
-    if (threadIdx.x == 0) part[(size_t)x * gridDim.x + blockIdx.x] = bs;
-    grid.sync();
+    if (threadIdx.x == 0)
+      __hip_atomic_store(&part[(size_t)x * gridDim.x + blockIdx.x], bs, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    if (!grid_barrier(sync, sync + 1, x)) return;
     double tot = 0.0;
     if (threadIdx.x < 64) {
-      for (long b = threadIdx.x; b < gridDim.x; b += 64) tot += part[(size_t)x * gridDim.x + b];
+      for (long b = threadIdx.x; b < gridDim.x; b += 64)
+        tot += __hip_atomic_load(&part[(size_t)x * gridDim.x + b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       tot = wave_sum(tot);
       if (threadIdx.x == 0) red[0] = tot;
     }
@@ -277,7 +340,7 @@ __global__ void hals_col_scale_kernel(float *W, long m, long ldw, int x, const d
 
 // ---------------------------------------------------------------------------
 // Column sums of squares of X (rows x k): partial[chunk*k + c].
-constexpr long COL_CHUNK = 8192;
+constexpr long COL_CHUNK = 1024;
 
 __global__ __launch_bounds__(NTHR) void colsumsq_partial_kernel(const float *X, long rows, int k,
                                                                 long ldx, double *part) {
@@ -286,7 +349,16 @@ __global__ __launch_bounds__(NTHR) void colsumsq_partial_kernel(const float *X, 
   long r1 = min(rows, r0 + COL_CHUNK);
   for (int c = wave; c < k; c += NTHR / 64) {
     double s = 0.0;
-    for (long r = r0 + lane; r < r1; r += 64) {
+    long r = r0 + lane;
+    for (; r + 192 < r1; r += 256) {
+      const double v0 = X[r + ldx * c], v1 = X[r + 64 + ldx * c];
+      const double v2 = X[r + 128 + ldx * c], v3 = X[r + 192 + ldx * c];
+      s += v0 * v0;
+      s += v1 * v1;
+      s += v2 * v2;
+      s += v3 * v3;
+    }
+    for (; r < r1; r += 64) {
       double v = X[r + ldx * c];
       s += v * v;
     }
@@ -351,17 +423,28 @@ __global__ void sum_partials_kernel(const double *part, long n, double *out, int
 // NMF relative error from the trace identity (common/nmf.hpp:390-424):
 // fit^2 = ||A||^2 - 2 <W, A H> + sum(WtW .* HtH); printObjective prints
 // sqrt(fit^2)/||A||, or 1 when fit^2 <= 0 (common/nmf.hpp:462).
-__global__ void nmf_relerr_kernel(const double *normA2, const double *dotv, const double *WtW,
-                                  const double *HtH, int k, double *relerr_out, double *fit_out) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__global__ __launch_bounds__(NTHR) void nmf_relerr_kernel(const double *normA2, const double *dotv,
+                                                          const double *WtW, const double *HtH, int k,
+                                                          double *relerr_out, double *fit_out) {
+  __shared__ double red[NTHR / 64];
   double tr = 0.0;
-  for (int b = 0; b < k; ++b)
-    for (int a = 0; a < k; ++a) tr += WtW[a + (size_t)k * b] * HtH[b + (size_t)k * a];
+  for (int e = threadIdx.x; e < k * k; e += NTHR) {
+    const int a = e % k, b = e / k;
+    tr += WtW[a + (size_t)k * b] * HtH[b + (size_t)k * a];
+  }
+  tr = block_sum(tr, red);
+  if (threadIdx.x != 0) return;
   double fit = *normA2 - 2.0 * (*dotv) + tr;
   double nA = sqrt(*normA2);
   double err = (fit > 0) ? sqrt(fit) : nA;
   *relerr_out = err / nA;
   if (fit_out) *fit_out = fit;
+}
+
+// norms[c] = sqrt(G(c, c)): the column 2-norms of X from its FP64 Gram.
+__global__ void gram_norms_kernel(const double *G, int k, double *norms) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < k) norms[c] = sqrt(G[c + (size_t)k * c]);
 }
 
 // G += l2 * I + l1 * ones (applyReg, common/nmf.hpp:76-90; the reference
@@ -458,8 +541,11 @@ int try_coop(planc_ctx *ctx, float *W, long m, int k, long ldw, const float *AH,
   if (need > maxblocks || need <= 0) return PLANC_OK;
   unsigned nb = (unsigned)need;
   double *part = (double *)ws_get(ctx, 2, sizeof(double) * (size_t)KM * nb);
-  if (!part) return set_error(ctx, PLANC_ENOMEM, "hals_cols: workspace");
-  void *args[] = {&W, &m, &k, &ldw, &AH, &ldah, &G, &part};
+  int *sync = (int *)ws_get(ctx, 17, 2 * sizeof(int));
+  if (!part || !sync) return set_error(ctx, PLANC_ENOMEM, "hals_cols: workspace");
+  zero_int_kernel<<<1, 1, 0, ctx->stream>>>(sync);
+  PLANC_LAUNCH_CHECK(ctx);
+  void *args[] = {&W, &m, &k, &ldw, &AH, &ldah, &G, &part, &sync};
   hipError_t le = hipLaunchCooperativeKernel((const void *)hals_cols_coop_kernel<KM, RPT>, dim3(nb),
                                              dim3(NTHR), args, 0, ctx->stream);
   if (le != hipSuccess) {
@@ -507,17 +593,14 @@ int gram(planc_ctx *ctx, const float *X, long rows, int k, long ldx, double *G) 
   int kp = ((k + 3) / 4) * 4;
   long nchunks = (rows + GRAM_RC - 1) / GRAM_RC;
   if (nchunks == 0) nchunks = 1;
-  long nb = nchunks < 1024 ? nchunks : 1024;
-  // Each block writes the chunks it owns; chunks are indexed globally.
-  double *part = (double *)ws_get(ctx, 1, sizeof(double) * (size_t)nchunks * kp * kp);
+  long nb = nchunks < ctx->num_cu ? nchunks : ctx->num_cu;
+  // One partial per block (each block accumulates its chunks).
+  double *part = (double *)ws_get(ctx, 1, sizeof(double) * (size_t)nb * kp * kp);
   if (!part) return set_error(ctx, PLANC_ENOMEM, "gram: workspace");
-  if (rows == 0) {
-    PLANC_HIP_CHECK(ctx, hipMemsetAsync(part, 0, sizeof(double) * kp * kp, ctx->stream));
-  } else {
-    gram_partial_kernel<<<(unsigned)nb, NTHR, 0, ctx->stream>>>(X, rows, k, ldx, part, kp);
-    PLANC_LAUNCH_CHECK(ctx);
-  }
-  gram_reduce_kernel<<<(k * k + 255) / 256, 256, 0, ctx->stream>>>(part, nchunks, k, kp, G);
+  // rows == 0: the kernel still writes zero partials.
+  gram_partial_kernel<<<(unsigned)nb, NTHR, 0, ctx->stream>>>(X, rows, k, ldx, part, kp);
+  PLANC_LAUNCH_CHECK(ctx);
+  gram_reduce_kernel<<<(unsigned)((kp * kp + 63) / 64), NTHR, 0, ctx->stream>>>(part, nb, k, kp, G);
   PLANC_LAUNCH_CHECK(ctx);
   return PLANC_OK;
 }
@@ -584,7 +667,32 @@ int dot(planc_ctx *ctx, const float *X, long xrs, long xcs, const float *Y, long
 
 int nmf_relerr(planc_ctx *ctx, const double *normA2, const double *dotv, const double *WtW,
                const double *HtH, int k, double *relerr_out, double *fit_out) {
-  nmf_relerr_kernel<<<1, 64, 0, ctx->stream>>>(normA2, dotv, WtW, HtH, k, relerr_out, fit_out);
+  nmf_relerr_kernel<<<1, NTHR, 0, ctx->stream>>>(normA2, dotv, WtW, HtH, k, relerr_out, fit_out);
+  PLANC_LAUNCH_CHECK(ctx);
+  return PLANC_OK;
+}
+
+// Clears / reads the grid-barrier time-out flag of the HALS column sweep.
+int hals_sync_reset(planc_ctx *ctx) {
+  int *sync = (int *)ws_get(ctx, 17, 2 * sizeof(int));
+  if (!sync) return set_error(ctx, PLANC_ENOMEM, "hals_cols: workspace");
+  zero_int_kernel<<<1, 1, 0, ctx->stream>>>(sync + 1);
+  PLANC_LAUNCH_CHECK(ctx);
+  return PLANC_OK;
+}
+
+int hals_sync_check(planc_ctx *ctx) {
+  int *sync = (int *)ws_get(ctx, 17, 2 * sizeof(int));
+  if (!sync) return set_error(ctx, PLANC_ENOMEM, "hals_cols: workspace");
+  int flag = 0;
+  PLANC_HIP_CHECK(ctx, hipMemcpyAsync(&flag, sync + 1, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  PLANC_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  if (flag) return set_error(ctx, PLANC_EHIP, "HALS column sweep: grid barrier timed out");
+  return PLANC_OK;
+}
+
+int gram_norms(planc_ctx *ctx, const double *G, int k, double *norms) {
+  gram_norms_kernel<<<(k + 63) / 64, 64, 0, ctx->stream>>>(G, k, norms);
   PLANC_LAUNCH_CHECK(ctx);
   return PLANC_OK;
 }

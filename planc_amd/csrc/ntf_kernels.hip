@@ -270,22 +270,28 @@ __global__ __launch_bounds__(MODE_THREADS) void mode_update_kernel(ModeParams p)
   }
   __syncthreads();
   for (long i = tid; i < I; i += MODE_THREADS) {
-    float h[KM];
+    // The row of F and this row's MTTKRP values, loaded up front (independent
+    // loads in flight together).
+    float h[KM], mv[KM];
 #pragma unroll
-    for (int t = 0; t < KM; ++t) h[t] = (t < k) ? p.F[i + p.ldf * t] : 0.0f;
+    for (int t = 0; t < KM; ++t) {
+      h[t] = (t < k) ? p.F[i + p.ldf * t] : 0.0f;
+      mv[t] = (t < k) ? p.M[i * p.mrs + t * p.mcs] : 0.0f;
+    }
     // The column loops are not unrolled; h[] is only indexed by unrolled t
     // loops (selects), so it stays in registers.
     if (p.kind == 1) {
       const float floor_v = 1e-17f;
 #pragma unroll 1
       for (int x = 0; x < k; ++x) {
-        float s = 0.0f, hx = 0.0f;
+        float s = 0.0f, hx = 0.0f, mx = 0.0f;
 #pragma unroll
         for (int t = 0; t < KM; ++t) {
           s = fmaf(h[t], Sf[t][x], s);
           hx = (t == x) ? h[t] : hx;
+          mx = (t == x) ? mv[t] : mx;
         }
-        float v = hx + (p.M[i * p.mrs + x * p.mcs] - s);
+        float v = hx + (mx - s);
         v = (v < floor_v) ? floor_v : v;
 #pragma unroll
         for (int t = 0; t < KM; ++t) h[t] = (t == x) ? v : h[t];
@@ -296,13 +302,14 @@ __global__ __launch_bounds__(MODE_THREADS) void mode_update_kernel(ModeParams p)
     } else if (p.kind == 0) {
 #pragma unroll 1
       for (int c = 0; c < k; ++c) {
-        float s = 0.0f, hc = 0.0f;
+        float s = 0.0f, hc = 0.0f, mc = 0.0f;
 #pragma unroll
         for (int t = 0; t < KM; ++t) {
           s = fmaf(h[t], Sf[t][c], s);
           hc = (t == c) ? h[t] : hc;
+          mc = (t == c) ? mv[t] : mc;
         }
-        Fs[c * ldl + i] = (hc * p.M[i * p.mrs + c * p.mcs]) / (s + p.eps);
+        Fs[c * ldl + i] = (hc * mc) / (s + p.eps);
       }
     } else {
 #pragma unroll
