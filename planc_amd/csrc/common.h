@@ -224,7 +224,7 @@ int ntf_relerr(planc_ctx *ctx, const double *normX2, const float *F, long I, lon
 int lambda_mul(planc_ctx *ctx, double *lambda, const double *norms, int k, int init);
 int transpose_to_k_major(planc_ctx *ctx, const float *in, long I, long ldi, int k, float *out, long ldo);
 // Fused AUNTF mode update (Hadamard, row update, normalize(mode), Gram, error)
-// for a factor with k * (I + 1) <= 24576; kind 0 MU, 1 HALS, 2 none.
+// for a factor that fits one block (k * (I + 8) <= 24576 floats); kind 0 MU, 1 HALS, 2 none.
 bool mode_update_fits(long I, int k);
 int mode_update(planc_ctx *ctx, int kind, float eps, float *F, long I, long ldf, int k, const float *M,
                 long mrs, long mcs, const double *const *grams, int ng, int skip, double *gout,

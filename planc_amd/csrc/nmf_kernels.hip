@@ -115,14 +115,32 @@ __global__ __launch_bounds__(NTHR) void gram_reduce_kernel(const double *part, l
 }
 
 // ---------------------------------------------------------------------------
-// Loads the k x k FP64 matrix G into an LDS float tile (zero padded).
+// Loads the k x k FP64 matrix G into LDS as floats, transposed and zero
+// padded: Gt[x][t] = G(t, x), so the column G(:, x) that a row update
+// contracts with is one contiguous row of 16-byte words (uniform address:
+// one LDS broadcast per word for the whole wave).
 template <int KM>
-__device__ inline void load_gram(float (&Gs)[KM][KM + 1], const double *G, int k) {
+__device__ inline void load_gram(float (&Gt)[KM][KM], const double *G, int k) {
   for (int e = threadIdx.x; e < KM * KM; e += blockDim.x) {
-    int r = e % KM, c = e / KM;
-    Gs[r][c] = (r < k && c < k) ? (float)G[r + (size_t)k * c] : 0.0f;
+    int t = e % KM, x = e / KM;
+    Gt[x][t] = (t < k && x < k) ? (float)G[t + (size_t)k * x] : 0.0f;
   }
   __syncthreads();
+}
+
+// s + sum_t h[t] * G(t, x), in t order, from the transposed LDS tile.
+template <int KM>
+__device__ inline float dot_gcol(const float (&h)[KM], const float (&Gt)[KM][KM], int x, float s) {
+  const floatx4 *g = (const floatx4 *)&Gt[x][0];
+#pragma unroll
+  for (int q = 0; q < KM / 4; ++q) {
+    const floatx4 v = g[q];
+    s = fmaf(h[4 * q + 0], v[0], s);
+    s = fmaf(h[4 * q + 1], v[1], s);
+    s = fmaf(h[4 * q + 2], v[2], s);
+    s = fmaf(h[4 * q + 3], v[3], s);
+  }
+  return s;
 }
 
 // MU: H <- H .* N ./ (H*G + eps); N(j, c) at N[j*nrs + c*ncs].
@@ -130,7 +148,7 @@ template <int KM>
 __global__ __launch_bounds__(NTHR) void mu_update_kernel(float *H, long n, int k, long ldh,
                                                          const float *N, long nrs, long ncs,
                                                          const double *G, float eps) {
-  __shared__ float Gs[KM][KM + 1];
+  __shared__ __attribute__((aligned(16))) float Gs[KM][KM];
   load_gram<KM>(Gs, G, k);
   long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
@@ -143,9 +161,7 @@ __global__ __launch_bounds__(NTHR) void mu_update_kernel(float *H, long n, int k
 #pragma unroll
   for (int c = 0; c < KM; ++c) {
     if (c < k) {
-      float s = 0.0f;
-#pragma unroll
-      for (int t = 0; t < KM; ++t) s = fmaf(h[t], Gs[t][c], s);
+      const float s = dot_gcol<KM>(h, Gs, c, 0.0f);
       H[j + ldh * c] = (h[c] * nv[c]) / (s + eps);
     }
   }
@@ -157,7 +173,7 @@ template <int KM>
 __global__ __launch_bounds__(NTHR) void hals_rows_kernel(float *H, long n, int k, long ldh,
                                                          const float *N, long nrs, long ncs,
                                                          const double *G) {
-  __shared__ float Gs[KM][KM + 1];
+  __shared__ __attribute__((aligned(16))) float Gs[KM][KM];
   load_gram<KM>(Gs, G, k);
   long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
@@ -171,9 +187,7 @@ __global__ __launch_bounds__(NTHR) void hals_rows_kernel(float *H, long n, int k
 #pragma unroll
   for (int x = 0; x < KM; ++x) {
     if (x < k) {
-      float s = 0.0f;
-#pragma unroll
-      for (int t = 0; t < KM; ++t) s = fmaf(h[t], Gs[t][x], s);
+      const float s = dot_gcol<KM>(h, Gs, x, 0.0f);
       float v = h[x] + (nv[x] - s);
       h[x] = (v < floor_v) ? floor_v : v;
     }
@@ -225,7 +239,7 @@ template <int KM, int RPT>
 __global__ __launch_bounds__(NTHR) void hals_cols_coop_kernel(float *W, long m, int k, long ldw,
                                                               const float *AH, long ldah,
                                                               const double *G, double *part, int *sync) {
-  __shared__ float Gs[KM][KM + 1];
+  __shared__ __attribute__((aligned(16))) float Gs[KM][KM];
   __shared__ double red[NTHR / 64];
   load_gram<KM>(Gs, G, k);
   const long total = (long)gridDim.x * blockDim.x;
@@ -249,12 +263,10 @@ __global__ __launch_bounds__(NTHR) void hals_cols_coop_kernel(float *W, long m, 
     for (int r = 0; r < RPT; ++r) {
       long i = gtid + r * total;
       u[r] = 0.0f;
-      float s = 0.0f, wx = 0.0f;
+      float wx = 0.0f;
 #pragma unroll
-      for (int t = 0; t < KM; ++t) {
-        s = fmaf(w[r][t], Gs[t][x], s);
-        wx = (t == x) ? w[r][t] : wx;
-      }
+      for (int t = 0; t < KM; ++t) wx = (t == x) ? w[r][t] : wx;
+      const float s = dot_gcol<KM>(w[r], Gs, x, 0.0f);
       if (i < m) {
         float v = (wx * gxx) + (AH[i + ldah * x] - s);
         v = (v < floor_v) ? floor_v : v;
@@ -305,14 +317,14 @@ __global__ __launch_bounds__(NTHR) void hals_col_step_kernel(float *W, long m, i
                                                              const float *AH, long ldah,
                                                              const double *G, int x,
                                                              double *part) {
-  __shared__ float Gs[KM][KM + 1];
+  __shared__ __attribute__((aligned(16))) float Gs[KM][KM];
   __shared__ double red[NTHR / 64];
   load_gram<KM>(Gs, G, k);
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   double v2 = 0.0;
   if (i < m) {
     float s = 0.0f;
-    for (int t = 0; t < k; ++t) s = fmaf(W[i + ldw * t], Gs[t][x], s);
+    for (int t = 0; t < k; ++t) s = fmaf(W[i + ldw * t], Gs[x][t], s);
     float v = (W[i + ldw * x] * Gs[x][x]) + (AH[i + ldah * x] - s);
     v = (v < 1e-17f) ? 1e-17f : v;
     W[i + ldw * x] = v;

@@ -234,6 +234,10 @@ struct ModeParams {
 };
 
 constexpr int MODE_LDS_FLOATS = 24576;
+
+// Column stride of the factor tile in LDS: a multiple of 4 (16-byte column
+// reads in the Gram) plus 4 (columns start in different banks).
+__host__ __device__ inline long mode_ld(long I) { return (I + 3) / 4 * 4 + 4; }
 constexpr int MODE_THREADS = 512;
 
 __device__ inline double block_sum_mode(double v, double *red) {
@@ -249,13 +253,13 @@ __device__ inline double block_sum_mode(double v, double *red) {
 
 template <int KM>
 __global__ __launch_bounds__(MODE_THREADS) void mode_update_kernel(ModeParams p) {
-  __shared__ float Sf[KM][KM + 1];
-  __shared__ float Fs[MODE_LDS_FLOATS];  // column c at Fs[c * (I + 1) + i]
+  __shared__ __attribute__((aligned(16))) float Sf[KM][KM];  // Sf[x][t] = S(t, x)
+  __shared__ __attribute__((aligned(16))) float Fs[MODE_LDS_FLOATS];  // column c at Fs[c * mode_ld(I) + i]
   __shared__ double nrm[KM];
   __shared__ double red[MODE_THREADS / 64];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int k = p.k;
-  const long I = p.I, ldl = I + 1;
+  const long I = p.I, ldl = mode_ld(I);
   for (int e = tid; e < KM * KM; e += MODE_THREADS) {
     const int a = e % KM, b = e / KM;
     float v = 0.0f;
@@ -266,7 +270,7 @@ __global__ __launch_bounds__(MODE_THREADS) void mode_update_kernel(ModeParams p)
         if (q < p.ng && q != p.skip) d *= p.g[q][a + (long)k * b];
       v = (float)d;
     }
-    Sf[a][b] = v;
+    Sf[b][a] = v;
   }
   __syncthreads();
   for (long i = tid; i < I; i += MODE_THREADS) {
@@ -285,9 +289,15 @@ __global__ __launch_bounds__(MODE_THREADS) void mode_update_kernel(ModeParams p)
 #pragma unroll 1
       for (int x = 0; x < k; ++x) {
         float s = 0.0f, hx = 0.0f, mx = 0.0f;
+        const floatx4 *sx = (const floatx4 *)&Sf[x][0];
+#pragma unroll
+        for (int q = 0; q < KM / 4; ++q) {
+          const floatx4 g = sx[q];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) s = fmaf(h[4 * q + u], g[u], s);
+        }
 #pragma unroll
         for (int t = 0; t < KM; ++t) {
-          s = fmaf(h[t], Sf[t][x], s);
           hx = (t == x) ? h[t] : hx;
           mx = (t == x) ? mv[t] : mx;
         }
@@ -303,9 +313,15 @@ __global__ __launch_bounds__(MODE_THREADS) void mode_update_kernel(ModeParams p)
 #pragma unroll 1
       for (int c = 0; c < k; ++c) {
         float s = 0.0f, hc = 0.0f, mc = 0.0f;
+        const floatx4 *sc = (const floatx4 *)&Sf[c][0];
+#pragma unroll
+        for (int q = 0; q < KM / 4; ++q) {
+          const floatx4 g = sc[q];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) s = fmaf(h[4 * q + u], g[u], s);
+        }
 #pragma unroll
         for (int t = 0; t < KM; ++t) {
-          s = fmaf(h[t], Sf[t][c], s);
           hc = (t == c) ? h[t] : hc;
           mc = (t == c) ? mv[t] : mc;
         }
@@ -347,7 +363,15 @@ __global__ __launch_bounds__(MODE_THREADS) void mode_update_kernel(ModeParams p)
     if (a > b) continue;
     double s = 0.0;
     const float *fa = Fs + a * ldl, *fb = Fs + b * ldl;
-    for (long i = 0; i < I; ++i) s += (double)fa[i] * (double)fb[i];
+    long i = 0;
+    for (; i + 4 <= I; i += 4) {
+      const floatx4 va = *(const floatx4 *)(fa + i), vb = *(const floatx4 *)(fb + i);
+      s += (double)va[0] * (double)vb[0];
+      s += (double)va[1] * (double)vb[1];
+      s += (double)va[2] * (double)vb[2];
+      s += (double)va[3] * (double)vb[3];
+    }
+    for (; i < I; ++i) s += (double)fa[i] * (double)fb[i];
     p.gout[a + (long)k * b] = s;
     p.gout[b + (long)k * a] = s;
   }
@@ -462,7 +486,7 @@ int transpose_to_k_major(planc_ctx *ctx, const float *in, long I, long ldi, int 
 
 namespace planc {
 
-bool mode_update_fits(long I, int k) { return k >= 1 && k <= 64 && (long)k * (I + 1) <= MODE_LDS_FLOATS; }
+bool mode_update_fits(long I, int k) { return k >= 1 && k <= 64 && (long)k * mode_ld(I) <= MODE_LDS_FLOATS; }
 
 int mode_update(planc_ctx *ctx, int kind, float eps, float *F, long I, long ldf, int k, const float *M,
                 long mrs, long mcs, const double *const *grams, int ng, int skip, double *gout,
