@@ -110,7 +110,8 @@ namespace planc {
 // 6 packed 16-bit thin operand, 7 16-bit copy of A / X, 8 column scales of that copy, 9 entry-point norms,
 // 10 BPP per-column iteration counts, 11 BPP failure flag, 12 CP-loss scalar,
 // 13 CP-gradient float buffers, 14 CP-gradient Grams, 15 loop-graph counter,
-// 16 transposed 16-bit copy of A / X, 17 HALS grid-barrier counter and flag.
+// 16 transposed 16-bit copy of A / X, 17 HALS grid-barrier counter and flag,
+// 18 fused NTF mode-update partials.
 void *ws_get(planc_ctx *ctx, int slot, size_t bytes);
 
 int set_error(planc_ctx *ctx, int code, const std::string &msg);
@@ -224,7 +225,7 @@ int ntf_relerr(planc_ctx *ctx, const double *normX2, const float *F, long I, lon
 int lambda_mul(planc_ctx *ctx, double *lambda, const double *norms, int k, int init);
 int transpose_to_k_major(planc_ctx *ctx, const float *in, long I, long ldi, int k, float *out, long ldo);
 // Fused AUNTF mode update (Hadamard, row update, normalize(mode), Gram, error)
-// for a factor that fits one block (k * (I + 8) <= 24576 floats); kind 0 MU, 1 HALS, 2 none.
+// in two launches; kind 0 MU, 1 HALS, 2 none (factor already solved).
 bool mode_update_fits(long I, int k);
 int mode_update(planc_ctx *ctx, int kind, float eps, float *F, long I, long ldf, int k, const float *M,
                 long mrs, long mcs, const double *const *grams, int ng, int skip, double *gout,

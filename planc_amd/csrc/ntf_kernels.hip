@@ -210,14 +210,20 @@ __global__ void transpose_kernel(const float *in, long I, long ldi, int k, float
 }
 
 // ---------------------------------------------------------------------------
-// One mode update of AUNTF for a factor that fits one block's LDS
-// (k * (I + 1) <= MODE_LDS_FLOATS): S = Hadamard of the other Grams, the
-// HALS (ntf/ntfhals.hpp:13-29) or MU row update from this mode's MTTKRP,
-// normalize(mode) (ntf/auntf.hpp:66-79: lambda = column 2-norms, columns
-// divided by them), the FP64 Gram of the new factor, and optionally the CP
-// relative error of ntf/auntf.hpp:224-245 from this mode's MTTKRP.  One
-// launch replaces six.  kind: 0 MU, 1 HALS, 2 no row update (the factor was
-// solved already, e.g. by BPP).
+// One mode update of AUNTF in two launches: S = Hadamard of the other Grams,
+// the HALS (ntf/ntfhals.hpp:13-29) or MU row update from this mode's
+// MTTKRP, normalize(mode) (ntf/auntf.hpp:66-79: lambda = column 2-norms,
+// columns divided by them), the FP64 Gram of the new factor and optionally
+// the CP relative error of ntf/auntf.hpp:224-245 from this mode's MTTKRP.
+//   mode_rows_kernel: 64 rows per block, a quad of lanes per row (each lane
+//     holds k/4 of the row, the column dot products are quad sums), writes
+//     the unnormalised rows u and accumulates per block the FP64 Gram of u
+//     and, per column, sum_i u(i, c) M(c, i).
+//   mode_finalize_kernel (one block): sums the block partials in block order;
+//     norms = sqrt(diag(Gram u)), lambda = norms, Gram of the normalised
+//     factor = D^-1 (Gram u) D^-1, the columns of F divided by the norms,
+//     the error.
+// kind: 0 MU, 1 HALS, 2 no row update (the factor was solved already, BPP).
 struct ModeParams {
   float *F;
   long I, ldf;
@@ -231,174 +237,193 @@ struct ModeParams {
   float eps;
   const double *normX2;
   double *rel_out;
+  double *part;  // per block: k*k Gram of u, then k column dots
+  int scale_in_finalize;
 };
 
-constexpr int MODE_LDS_FLOATS = 24576;
-
-// Column stride of the factor tile in LDS: a multiple of 4 (16-byte column
-// reads in the Gram) plus 4 (columns start in different banks).
-__host__ __device__ inline long mode_ld(long I) { return (I + 3) / 4 * 4 + 4; }
-constexpr int MODE_THREADS = 512;
-
-__device__ inline double block_sum_mode(double v, double *red) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  v = wave_sum_d(v);
-  __syncthreads();
-  if (lane == 0) red[wave] = v;
-  __syncthreads();
-  double s = 0.0;
-  for (int w = 0; w < MODE_THREADS / 64; ++w) s += red[w];
-  return s;
-}
+constexpr int MR = 64;  // rows per block-iteration of mode_rows_kernel
 
 template <int KM>
-__global__ __launch_bounds__(MODE_THREADS) void mode_update_kernel(ModeParams p) {
-  __shared__ __attribute__((aligned(16))) float Sf[KM][KM];  // Sf[x][t] = S(t, x)
-  __shared__ __attribute__((aligned(16))) float Fs[MODE_LDS_FLOATS];  // column c at Fs[c * mode_ld(I) + i]
-  __shared__ double nrm[KM];
-  __shared__ double red[MODE_THREADS / 64];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+__global__ __launch_bounds__(256) void mode_rows_kernel(ModeParams p) {
+  constexpr int Q = KM / 4;   // row values per lane
+  constexpr int NE = (KM * KM + 255) / 256;  // Gram entries per thread
+  __shared__ __attribute__((aligned(16))) float St[KM][KM];  // St[x][t] = S(t, x)
+  __shared__ float Us[KM][MR + 1];                           // u(row, c) at Us[c][row]
+  __shared__ float Ms[KM][MR + 1];                           // M(c, row)
+  const int tid = threadIdx.x, q = tid & 3, rr = tid >> 2;
   const int k = p.k;
-  const long I = p.I, ldl = mode_ld(I);
-  for (int e = tid; e < KM * KM; e += MODE_THREADS) {
+  for (int e = tid; e < KM * KM; e += 256) {
     const int a = e % KM, b = e / KM;
-    float v = 0.0f;
-    if (a < k && b < k) {
-      double d = 1.0;
+    double d = 1.0;
 #pragma unroll
-      for (int q = 0; q < 8; ++q)
-        if (q < p.ng && q != p.skip) d *= p.g[q][a + (long)k * b];
-      v = (float)d;
+    for (int g = 0; g < 8; ++g) {
+      const bool use = g < p.ng && g != p.skip && a < k && b < k;
+      const double *src = p.g[g < p.ng ? g : 0];
+      const double v = src[(a < k && b < k) ? a + (long)k * b : 0];
+      d *= use ? v : 1.0;
     }
-    Sf[b][a] = v;
+    St[b][a] = (a < k && b < k) ? (float)d : 0.0f;
   }
-  __syncthreads();
-  for (long i = tid; i < I; i += MODE_THREADS) {
-    // The row of F and this row's MTTKRP values, loaded up front (independent
-    // loads in flight together).
-    float h[KM], mv[KM];
+  double acc[NE];
 #pragma unroll
-    for (int t = 0; t < KM; ++t) {
-      h[t] = (t < k) ? p.F[i + p.ldf * t] : 0.0f;
-      mv[t] = (t < k) ? p.M[i * p.mrs + t * p.mcs] : 0.0f;
+  for (int j = 0; j < NE; ++j) acc[j] = 0.0;
+  double dacc = 0.0;
+  __syncthreads();
+  for (long r0 = (long)blockIdx.x * MR; r0 < p.I; r0 += (long)gridDim.x * MR) {
+    const long i = r0 + rr;
+    const bool valid = i < p.I;
+    const long ic = valid ? i : p.I - 1;
+    float h[Q], mv[Q];
+#pragma unroll
+    for (int u = 0; u < Q; ++u) {
+      const int t = q * Q + u;
+      const int tc = t < k ? t : k - 1;
+      h[u] = t < k ? p.F[ic + p.ldf * tc] : 0.0f;
+      mv[u] = t < k ? p.M[ic * p.mrs + tc * p.mcs] : 0.0f;
     }
-    // The column loops are not unrolled; h[] is only indexed by unrolled t
-    // loops (selects), so it stays in registers.
     if (p.kind == 1) {
-      const float floor_v = 1e-17f;
-#pragma unroll 1
-      for (int x = 0; x < k; ++x) {
-        float s = 0.0f, hx = 0.0f, mx = 0.0f;
-        const floatx4 *sx = (const floatx4 *)&Sf[x][0];
 #pragma unroll
-        for (int q = 0; q < KM / 4; ++q) {
-          const floatx4 g = sx[q];
+      for (int x = 0; x < KM; ++x) {
+        if (x < k) {
+          const float *sx = &St[x][q * Q];
+          float ps = 0.0f;
 #pragma unroll
-          for (int u = 0; u < 4; ++u) s = fmaf(h[4 * q + u], g[u], s);
+          for (int u = 0; u < Q; ++u) ps = fmaf(h[u], sx[u], ps);
+          ps += __shfl_xor(ps, 1, 64);
+          ps += __shfl_xor(ps, 2, 64);
+          float v = h[x % Q] + (mv[x % Q] - ps);
+          v = v < 1e-17f ? 1e-17f : v;
+          if (q == x / Q) h[x % Q] = v;
         }
-#pragma unroll
-        for (int t = 0; t < KM; ++t) {
-          hx = (t == x) ? h[t] : hx;
-          mx = (t == x) ? mv[t] : mx;
-        }
-        float v = hx + (mx - s);
-        v = (v < floor_v) ? floor_v : v;
-#pragma unroll
-        for (int t = 0; t < KM; ++t) h[t] = (t == x) ? v : h[t];
       }
-#pragma unroll
-      for (int t = 0; t < KM; ++t)
-        if (t < k) Fs[t * ldl + i] = h[t];
     } else if (p.kind == 0) {
-#pragma unroll 1
-      for (int c = 0; c < k; ++c) {
-        float s = 0.0f, hc = 0.0f, mc = 0.0f;
-        const floatx4 *sc = (const floatx4 *)&Sf[c][0];
+      float hn[Q];
 #pragma unroll
-        for (int q = 0; q < KM / 4; ++q) {
-          const floatx4 g = sc[q];
+      for (int u = 0; u < Q; ++u) hn[u] = h[u];
 #pragma unroll
-          for (int u = 0; u < 4; ++u) s = fmaf(h[4 * q + u], g[u], s);
+      for (int c = 0; c < KM; ++c) {
+        if (c < k) {
+          const float *sc = &St[c][q * Q];
+          float ps = 0.0f;
+#pragma unroll
+          for (int u = 0; u < Q; ++u) ps = fmaf(h[u], sc[u], ps);
+          ps += __shfl_xor(ps, 1, 64);
+          ps += __shfl_xor(ps, 2, 64);
+          const float v = (h[c % Q] * mv[c % Q]) / (ps + p.eps);
+          if (q == c / Q) hn[c % Q] = v;
         }
-#pragma unroll
-        for (int t = 0; t < KM; ++t) {
-          hc = (t == c) ? h[t] : hc;
-          mc = (t == c) ? mv[t] : mc;
-        }
-        Fs[c * ldl + i] = (hc * mc) / (s + p.eps);
       }
-    } else {
 #pragma unroll
-      for (int t = 0; t < KM; ++t)
-        if (t < k) Fs[t * ldl + i] = h[t];
+      for (int u = 0; u < Q; ++u) h[u] = hn[u];
+    }
+#pragma unroll
+    for (int u = 0; u < Q; ++u) {
+      const int t = q * Q + u;
+      Us[t][rr] = valid ? h[u] : 0.0f;
+      Ms[t][rr] = valid ? mv[u] : 0.0f;
+      if (valid && t < k) p.F[i + p.ldf * t] = h[u];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < NE; ++j) {
+      const int e = tid + 256 * j;
+      const int a = e % KM, b = e / KM;
+      if (e < KM * KM && a <= b && b < k) {
+        double s = 0.0;
+#pragma unroll 8
+        for (int r = 0; r < MR; ++r) s += (double)Us[a][r] * (double)Us[b][r];
+        acc[j] += s;
+      }
+    }
+    if (tid < k) {
+      double s = 0.0;
+#pragma unroll 8
+      for (int r = 0; r < MR; ++r) s += (double)Us[tid][r] * (double)Ms[tid][r];
+      dacc += s;
+    }
+    __syncthreads();
+  }
+  double *dst = p.part + (long)blockIdx.x * (k * k + k);
+#pragma unroll
+  for (int j = 0; j < NE; ++j) {
+    const int e = tid + 256 * j;
+    const int a = e % KM, b = e / KM;
+    if (e < KM * KM && a <= b && b < k) {
+      dst[a + k * b] = acc[j];
+      dst[b + k * a] = acc[j];
     }
   }
-  __syncthreads();
-  for (int c = wave; c < k; c += MODE_THREADS / 64) {
+  if (tid < k) dst[k * k + tid] = dacc;
+}
+
+__global__ __launch_bounds__(256) void mode_finalize_kernel(ModeParams p, int nblocks) {
+  __shared__ double Gu[64 * 64 + 64];
+  __shared__ double nrm[64];
+  __shared__ double red[4];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int k = p.k, kk = k * k;
+  for (int e = tid; e < kk + k; e += 256) {
     double s = 0.0;
-    for (long i = lane; i < I; i += 64) {
-      const double v = Fs[c * ldl + i];
-      s += v * v;
-    }
-    s = wave_sum_d(s);
-    if (lane == 0) {
-      const double nv = sqrt(s);
-      nrm[c] = nv;
-      p.lambda[c] = nv;
-    }
+    for (int b = 0; b < nblocks; ++b) s += p.part[(long)b * (kk + k) + e];
+    Gu[e] = s;
   }
   __syncthreads();
-  for (long e = tid; e < I * k; e += MODE_THREADS) {
-    const int c = (int)(e / I);
-    const long i = e - (long)c * I;
-    float v = Fs[c * ldl + i];
-    const double nv = nrm[c];
-    if (nv > 0) v = (float)((double)v / nv);
-    Fs[c * ldl + i] = v;
-    p.F[i + p.ldf * c] = v;
+  if (tid < k) {
+    const double nv = sqrt(Gu[tid + k * tid]);
+    nrm[tid] = nv;
+    p.lambda[tid] = nv;
   }
   __syncthreads();
-  for (int e = tid; e < k * k; e += MODE_THREADS) {
+  for (int e = tid; e < kk; e += 256) {
     const int a = e % k, b = e / k;
-    if (a > b) continue;
-    double s = 0.0;
-    const float *fa = Fs + a * ldl, *fb = Fs + b * ldl;
-    long i = 0;
-    for (; i + 4 <= I; i += 4) {
-      const floatx4 va = *(const floatx4 *)(fa + i), vb = *(const floatx4 *)(fb + i);
-      s += (double)va[0] * (double)vb[0];
-      s += (double)va[1] * (double)vb[1];
-      s += (double)va[2] * (double)vb[2];
-      s += (double)va[3] * (double)vb[3];
+    const double sa = nrm[a] > 0 ? nrm[a] : 1.0, sb = nrm[b] > 0 ? nrm[b] : 1.0;
+    p.gout[e] = Gu[e] / (sa * sb);
+  }
+  if (p.scale_in_finalize) {
+    for (int c = 0; c < k; ++c) {
+      const double nv = nrm[c];
+      if (!(nv > 0)) continue;
+      float *col = p.F + p.ldf * c;
+      for (long i = tid; i < p.I; i += 256) col[i] = (float)((double)col[i] / nv);
     }
-    for (; i < I; ++i) s += (double)fa[i] * (double)fb[i];
-    p.gout[a + (long)k * b] = s;
-    p.gout[b + (long)k * a] = s;
   }
   if (!p.rel_out) return;
-  __syncthreads();
-  double t1 = 0.0;
-  for (long e = tid; e < I * k; e += MODE_THREADS) {
-    const int c = (int)(e / I);
-    const long i = e - (long)c * I;
-    t1 += nrm[c] * (double)Fs[c * ldl + i] * (double)p.M[i * p.mrs + c * p.mcs];
-  }
-  double xh = 0.0;
-  for (int e = tid; e < k * k; e += MODE_THREADS) {
+  double t1 = 0.0, xh = 0.0;
+  for (int c = tid; c < k; c += 256) t1 += Gu[kk + c];
+  for (int e = tid; e < kk; e += 256) {
     const int a = e % k, b = e / k;
     double v = nrm[a] * nrm[b];
+    const double sa = nrm[a] > 0 ? nrm[a] : 1.0, sb = nrm[b] > 0 ? nrm[b] : 1.0;
 #pragma unroll
-    for (int q = 0; q < 8; ++q)
-      if (q < p.ng) v *= (q == p.skip) ? p.gout[e] : p.g[q][e];
+    for (int g = 0; g < 8; ++g)
+      if (g < p.ng) v *= (g == p.skip) ? Gu[e] / (sa * sb) : p.g[g][e];
     xh += v;
   }
-  t1 = block_sum_mode(t1, red);
-  xh = block_sum_mode(xh, red);
+  t1 = wave_sum_d(t1);
+  xh = wave_sum_d(xh);
+  __syncthreads();
+  if (lane == 0) red[wave] = t1;
+  __syncthreads();
+  if (tid == 0) t1 = red[0] + red[1] + red[2] + red[3];
+  __syncthreads();
+  if (lane == 0) red[wave] = xh;
+  __syncthreads();
   if (tid == 0) {
+    xh = red[0] + red[1] + red[2] + red[3];
     double e2 = *p.normX2 - 2.0 * t1 + xh;
     if (e2 < 0) e2 = 0;
     *p.rel_out = sqrt(e2 / *p.normX2);
   }
+}
+
+// Divides the columns of F by the norms (F large enough for many blocks).
+__global__ void mode_scale_kernel(float *F, long I, long ldf, int k, const double *norms) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= I * k) return;
+  const int c = (int)(e / I);
+  const long i = e - (long)c * I;
+  const double nv = norms[c];
+  if (nv > 0) F[i + ldf * c] = (float)((double)F[i + ldf * c] / nv);
 }
 
 }  // namespace
@@ -486,13 +511,13 @@ int transpose_to_k_major(planc_ctx *ctx, const float *in, long I, long ldi, int 
 
 namespace planc {
 
-bool mode_update_fits(long I, int k) { return k >= 1 && k <= 64 && (long)k * mode_ld(I) <= MODE_LDS_FLOATS; }
+bool mode_update_fits(long I, int k) { return k >= 1 && k <= 64 && I >= 1 && I < (1L << 31); }
 
 int mode_update(planc_ctx *ctx, int kind, float eps, float *F, long I, long ldf, int k, const float *M,
                 long mrs, long mcs, const double *const *grams, int ng, int skip, double *gout,
                 double *lambda, const double *normX2, double *rel_out) {
   if (!mode_update_fits(I, k) || ng < 1 || ng > 8 || skip < 0 || skip >= ng)
-    return set_error(ctx, PLANC_EINVAL, "mode_update: factor does not fit one block");
+    return set_error(ctx, PLANC_EINVAL, "mode_update: bad arguments");
   ModeParams p{};
   p.F = F;
   p.I = I;
@@ -510,15 +535,27 @@ int mode_update(planc_ctx *ctx, int kind, float eps, float *F, long I, long ldf,
   p.eps = eps;
   p.normX2 = normX2;
   p.rel_out = rel_out;
+  long nb = (I + MR - 1) / MR;
+  if (nb > ctx->num_cu) nb = ctx->num_cu;
+  p.part = (double *)ws_get(ctx, 18, sizeof(double) * (size_t)nb * (k * k + k));
+  if (!p.part) return set_error(ctx, PLANC_ENOMEM, "mode_update: workspace");
+  p.scale_in_finalize = I * k <= 32768 ? 1 : 0;
   if (k <= 16)
-    mode_update_kernel<16><<<1, MODE_THREADS, 0, ctx->stream>>>(p);
+    mode_rows_kernel<16><<<(unsigned)nb, 256, 0, ctx->stream>>>(p);
   else if (k <= 32)
-    mode_update_kernel<32><<<1, MODE_THREADS, 0, ctx->stream>>>(p);
+    mode_rows_kernel<32><<<(unsigned)nb, 256, 0, ctx->stream>>>(p);
   else if (k <= 48)
-    mode_update_kernel<48><<<1, MODE_THREADS, 0, ctx->stream>>>(p);
+    mode_rows_kernel<48><<<(unsigned)nb, 256, 0, ctx->stream>>>(p);
   else
-    mode_update_kernel<64><<<1, MODE_THREADS, 0, ctx->stream>>>(p);
+    mode_rows_kernel<64><<<(unsigned)nb, 256, 0, ctx->stream>>>(p);
   PLANC_LAUNCH_CHECK(ctx);
+  mode_finalize_kernel<<<1, 256, 0, ctx->stream>>>(p, (int)nb);
+  PLANC_LAUNCH_CHECK(ctx);
+  if (!p.scale_in_finalize) {
+    const long tot = I * k;
+    mode_scale_kernel<<<(unsigned)((tot + 255) / 256), 256, 0, ctx->stream>>>(F, I, ldf, k, lambda);
+    PLANC_LAUNCH_CHECK(ctx);
+  }
   return PLANC_OK;
 }
 
