@@ -489,23 +489,37 @@ int planc_nmf_run(planc_ctx *ctx, const planc_nmf_desc *desc, const float *d_A, 
     RC(hals_sync_reset(ctx));
     RC(planc_normalize_by_w(ctx, d_W, m, m, d_H, n, n, k));
   }
+  const bool regW_on = desc->regW[0] > 0 || desc->regW[1] > 0;
+  const bool regH_on = desc->regH[0] > 0 || desc->regH[1] > 0;
+  // W^T W of the current W: computed once per iteration (after the W
+  // half-step) and carried into the next H half-step -- rescaled by the
+  // normalisation for HALS -- instead of a second pass over W.
+  bool wtw_valid = false;
   for (int it = 0; it < iters; ++it) {
     // H half-step: W^T A, W^T W (+ regH).
     RC(thin_gemm(ctx, dt, 1, Aop, ldop, n, m, d_W, m, nullptr, nullptr, cs, k, WtA, k, 1, 0));
-    RC(gram(ctx, d_W, m, k, m, WtW));
-    PLANC_HIP_CHECK(ctx, hipMemcpyAsync(Greg, WtW, kk * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
-    RC(apply_reg(ctx, Greg, k, desc->regH));
-    RC(row_update(ctx, algo == PLANC_MU ? 0 : 1, d_H, n, k, n, WtA, k, 1, Greg, 1e-17f));
+    if (!wtw_valid) RC(gram(ctx, d_W, m, k, m, WtW));
+    const double *Gh = WtW;
+    if (regH_on) {
+      PLANC_HIP_CHECK(ctx, hipMemcpyAsync(Greg, WtW, kk * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+      RC(apply_reg(ctx, Greg, k, desc->regH));
+      Gh = Greg;
+    }
+    RC(row_update(ctx, algo == PLANC_MU ? 0 : 1, d_H, n, k, n, WtA, k, 1, Gh, 1e-17f));
     // W half-step: A H, H^T H (+ regW).
     RC(a_times_h(AH));
     RC(gram(ctx, d_H, n, k, n, HtH));
-    PLANC_HIP_CHECK(ctx, hipMemcpyAsync(Greg, HtH, kk * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
-    RC(apply_reg(ctx, Greg, k, desc->regW));
+    const double *Gw = HtH;
+    if (regW_on) {
+      PLANC_HIP_CHECK(ctx, hipMemcpyAsync(Greg, HtH, kk * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+      RC(apply_reg(ctx, Greg, k, desc->regW));
+      Gw = Greg;
+    }
     if (algo == PLANC_MU)
-      RC(row_update(ctx, 0, d_W, m, k, m, AH, 1, m, Greg, 1e-17f));
+      RC(row_update(ctx, 0, d_W, m, k, m, AH, 1, m, Gw, 1e-17f));
     else
-      RC(hals_cols(ctx, d_W, m, k, m, AH, m, Greg));
-    if (desc->compute_error || algo == PLANC_HALS) RC(gram(ctx, d_W, m, k, m, WtW2));
+      RC(hals_cols(ctx, d_W, m, k, m, AH, m, Gw));
+    RC(gram(ctx, d_W, m, k, m, WtW2));
     if (desc->compute_error) {
       // <W, A H> with the H of this iteration equals tr(H^T A^T W) of
       // computeObjectiveError (common/nmf.hpp:391-396) without the third
@@ -514,10 +528,15 @@ int planc_nmf_run(planc_ctx *ctx, const planc_nmf_desc *desc, const float *d_A, 
       RC(nmf_relerr(ctx, normA2, dotv, WtW2, HtH, k, rel + it, nullptr));
     }
     if (algo == PLANC_HALS) {
-      // normalize_by_W: the column norms are the square roots of diag(W^T W).
+      // normalize_by_W: the column norms are the square roots of diag(W^T W);
+      // the next W^T W is D^-1 (W^T W) D^-1.
       RC(gram_norms(ctx, WtW2, k, wnorms));
       RC(scale_cols(ctx, d_W, m, m, d_H, n, n, k, wnorms));
+      RC(gram_rescale(ctx, WtW2, k, wnorms, WtW));
+    } else {
+      PLANC_HIP_CHECK(ctx, hipMemcpyAsync(WtW, WtW2, kk * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
     }
+    wtw_valid = true;
   }
   if (algo == PLANC_MU) RC(planc_normalize_by_w(ctx, d_W, m, m, d_H, n, n, k));
 #undef RC

@@ -196,6 +196,8 @@ int apply_reg(planc_ctx *ctx, double *G, int k, const double *reg);
 // run, checked (synchronising) after it.
 int hals_sync_reset(planc_ctx *ctx);
 int hals_sync_check(planc_ctx *ctx);
+// out = D^-1 G D^-1 with D = diag(norms) (entries 0 taken as 1).
+int gram_rescale(planc_ctx *ctx, const double *G, int k, const double *norms, double *out);
 // norms[c] = sqrt(G(c, c)).
 int gram_norms(planc_ctx *ctx, const double *G, int k, double *norms);
 int convert16(planc_ctx *ctx, int dtype, const float *A, long rows, long cols, long lda, void *out,

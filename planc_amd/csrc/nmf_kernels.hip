@@ -453,6 +453,16 @@ __global__ __launch_bounds__(NTHR) void nmf_relerr_kernel(const double *normA2, 
   if (fit_out) *fit_out = fit;
 }
 
+// out(a, b) = G(a, b) / (s_a s_b), s = norms (1 where a norm is 0): the
+// Gram of X after its columns are divided by their norms.
+__global__ void gram_rescale_kernel(const double *G, int k, const double *norms, double *out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= k * k) return;
+  const int a = e % k, b = e / k;
+  const double sa = norms[a] > 0 ? norms[a] : 1.0, sb = norms[b] > 0 ? norms[b] : 1.0;
+  out[e] = G[e] / (sa * sb);
+}
+
 // norms[c] = sqrt(G(c, c)): the column 2-norms of X from its FP64 Gram.
 __global__ void gram_norms_kernel(const double *G, int k, double *norms) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -700,6 +710,12 @@ int hals_sync_check(planc_ctx *ctx) {
   PLANC_HIP_CHECK(ctx, hipMemcpyAsync(&flag, sync + 1, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   PLANC_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   if (flag) return set_error(ctx, PLANC_EHIP, "HALS column sweep: grid barrier timed out");
+  return PLANC_OK;
+}
+
+int gram_rescale(planc_ctx *ctx, const double *G, int k, const double *norms, double *out) {
+  gram_rescale_kernel<<<(k * k + 255) / 256, 256, 0, ctx->stream>>>(G, k, norms, out);
+  PLANC_LAUNCH_CHECK(ctx);
   return PLANC_OK;
 }
 
