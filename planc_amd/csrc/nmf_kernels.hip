@@ -6,6 +6,7 @@
 // registers and the k x k Gram in LDS, so an update costs one read and one
 // write of the factor plus one read of the numerator.
 #include "common.h"
+#include "rowupd.h"
 
 namespace planc {
 
@@ -143,58 +144,34 @@ __device__ inline float dot_gcol(const float (&h)[KM], const float (&Gt)[KM][KM]
   return s;
 }
 
-// MU: H <- H .* N ./ (H*G + eps); N(j, c) at N[j*nrs + c*ncs].
+// MU (kind 0: H <- H .* N ./ (H*G + eps)) and HALS rows (kind 1,
+// nmf/hals.hpp:67-75, ntf/ntfhals.hpp:13-29: columns in order, each using
+// the already-updated columns before it), N(j, c) at N[j*nrs + c*ncs].  A
+// quad of lanes per row (quad_row_update), 64 rows per block.
 template <int KM>
-__global__ __launch_bounds__(NTHR) void mu_update_kernel(float *H, long n, int k, long ldh,
+__global__ __launch_bounds__(NTHR) void rows_quad_kernel(float *H, long n, int k, long ldh,
                                                          const float *N, long nrs, long ncs,
-                                                         const double *G, float eps) {
+                                                         const double *G, int kind, float eps) {
+  constexpr int Q = KM / 4;
   __shared__ __attribute__((aligned(16))) float Gs[KM][KM];
   load_gram<KM>(Gs, G, k);
-  long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int q = threadIdx.x & 3;
+  const long j = (long)blockIdx.x * (NTHR / 4) + (threadIdx.x >> 2);
   if (j >= n) return;
-  float h[KM], nv[KM];
+  float h[Q], nv[Q];
 #pragma unroll
-  for (int t = 0; t < KM; ++t) {
-    h[t] = (t < k) ? H[j + ldh * t] : 0.0f;
-    nv[t] = (t < k) ? N[j * nrs + t * ncs] : 0.0f;
+  for (int u = 0; u < Q; ++u) {
+    const int t = q * Q + u;
+    const int tc = t < k ? t : k - 1;
+    h[u] = t < k ? H[j + ldh * tc] : 0.0f;
+    nv[u] = t < k ? N[j * nrs + tc * ncs] : 0.0f;
   }
+  quad_row_update<KM>(h, nv, Gs, q, k, kind, eps);
 #pragma unroll
-  for (int c = 0; c < KM; ++c) {
-    if (c < k) {
-      const float s = dot_gcol<KM>(h, Gs, c, 0.0f);
-      H[j + ldh * c] = (h[c] * nv[c]) / (s + eps);
-    }
+  for (int u = 0; u < Q; ++u) {
+    const int t = q * Q + u;
+    if (t < k) H[j + ldh * t] = h[u];
   }
-}
-
-// HALS, independent rows (nmf/hals.hpp:67-75, ntf/ntfhals.hpp:13-29):
-// sequential over columns x, each using the already-updated columns < x.
-template <int KM>
-__global__ __launch_bounds__(NTHR) void hals_rows_kernel(float *H, long n, int k, long ldh,
-                                                         const float *N, long nrs, long ncs,
-                                                         const double *G) {
-  __shared__ __attribute__((aligned(16))) float Gs[KM][KM];
-  load_gram<KM>(Gs, G, k);
-  long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
-  const float floor_v = 1e-17f;
-  float h[KM], nv[KM];
-#pragma unroll
-  for (int t = 0; t < KM; ++t) {
-    h[t] = (t < k) ? H[j + ldh * t] : 0.0f;
-    nv[t] = (t < k) ? N[j * nrs + t * ncs] : 0.0f;
-  }
-#pragma unroll
-  for (int x = 0; x < KM; ++x) {
-    if (x < k) {
-      const float s = dot_gcol<KM>(h, Gs, x, 0.0f);
-      float v = h[x] + (nv[x] - s);
-      h[x] = (v < floor_v) ? floor_v : v;
-    }
-  }
-#pragma unroll
-  for (int t = 0; t < KM; ++t)
-    if (t < k) H[j + ldh * t] = h[t];
 }
 
 // Grid-wide barrier of a cooperative launch (all blocks co-resident).  Thread
@@ -540,12 +517,10 @@ __global__ __launch_bounds__(256) void transpose16_kernel(const uint16_t *in, lo
 template <int KM>
 int launch_rows(planc_ctx *ctx, int kind, float *H, long n, int k, long ldh, const float *N,
                 long nrs, long ncs, const double *G, float eps) {
-  unsigned nb = (unsigned)((n + NTHR - 1) / NTHR);
+  const long rows_per_block = NTHR / 4;
+  unsigned nb = (unsigned)((n + rows_per_block - 1) / rows_per_block);
   if (nb == 0) return PLANC_OK;
-  if (kind == 0)
-    mu_update_kernel<KM><<<nb, NTHR, 0, ctx->stream>>>(H, n, k, ldh, N, nrs, ncs, G, eps);
-  else
-    hals_rows_kernel<KM><<<nb, NTHR, 0, ctx->stream>>>(H, n, k, ldh, N, nrs, ncs, G);
+  rows_quad_kernel<KM><<<nb, NTHR, 0, ctx->stream>>>(H, n, k, ldh, N, nrs, ncs, G, kind == 0 ? 0 : 1, eps);
   PLANC_LAUNCH_CHECK(ctx);
   return PLANC_OK;
 }

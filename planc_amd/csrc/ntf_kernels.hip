@@ -2,6 +2,7 @@
 // (multi-TTV of the dimension tree), Hadamard of Grams, and the FP64 terms
 // of the CP relative error.
 #include "common.h"
+#include "rowupd.h"
 
 namespace planc {
 
@@ -83,6 +84,38 @@ __global__ __launch_bounds__(1024) void contract_outer_kernel(ContractParams p) 
     for (int w = 0; w < 16; ++w) t += red[w][lane];
     p.M[c + p.ldo * i] = t;
   }
+}
+
+// inner > 1, inner % 4 == 0, 16-byte aligned columns: one wavefront per
+// output (i, c); lanes read 16 consecutive bytes of the contiguous inner
+// index, weights from LDS; FP64 wave reduction in fixed order.
+__global__ __launch_bounds__(256) void contract_wave_kernel(ContractParams p) {
+  __shared__ __attribute__((aligned(16))) float wi[WMAX];
+  __shared__ float wo[WMAX];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int c = blockIdx.y;
+  const long i = (long)blockIdx.x * 4 + wave;
+  for (long a = threadIdx.x; a < p.inner; a += blockDim.x) wi[a] = weight(p, (unsigned)a, 0, p.tn, c);
+  for (long b = threadIdx.x; b < p.outer; b += blockDim.x) wo[b] = weight(p, (unsigned)b, p.tn + 1, p.nb, c);
+  __syncthreads();
+  if (i >= p.In) return;
+  float acc = 0.0f;
+  const float *Pc = p.P + p.ldp * c;
+  for (long b = 0; b < p.outer; ++b) {
+    const float *src = Pc + p.inner * (i + p.In * b);
+    float sb = 0.0f;
+    for (long a = 4 * lane; a < p.inner; a += 256) {
+      const floatx4 v = *(const floatx4 *)(src + a);
+      const floatx4 w = *(const floatx4 *)&wi[a];
+      sb = fmaf(v[0], w[0], sb);
+      sb = fmaf(v[1], w[1], sb);
+      sb = fmaf(v[2], w[2], sb);
+      sb = fmaf(v[3], w[3], sb);
+    }
+    acc = fmaf(sb, wo[b], acc);
+  }
+  const double r = wave_sum_d((double)acc);
+  if (lane == 0) p.M[c + p.ldo * i] = (float)r;
 }
 
 // inner > 1: one block per (tile of TI output indices, c); threads walk the
@@ -281,41 +314,7 @@ __global__ __launch_bounds__(256) void mode_rows_kernel(ModeParams p) {
       h[u] = t < k ? p.F[ic + p.ldf * tc] : 0.0f;
       mv[u] = t < k ? p.M[ic * p.mrs + tc * p.mcs] : 0.0f;
     }
-    if (p.kind == 1) {
-#pragma unroll
-      for (int x = 0; x < KM; ++x) {
-        if (x < k) {
-          const float *sx = &St[x][q * Q];
-          float ps = 0.0f;
-#pragma unroll
-          for (int u = 0; u < Q; ++u) ps = fmaf(h[u], sx[u], ps);
-          ps += __shfl_xor(ps, 1, 64);
-          ps += __shfl_xor(ps, 2, 64);
-          float v = h[x % Q] + (mv[x % Q] - ps);
-          v = v < 1e-17f ? 1e-17f : v;
-          if (q == x / Q) h[x % Q] = v;
-        }
-      }
-    } else if (p.kind == 0) {
-      float hn[Q];
-#pragma unroll
-      for (int u = 0; u < Q; ++u) hn[u] = h[u];
-#pragma unroll
-      for (int c = 0; c < KM; ++c) {
-        if (c < k) {
-          const float *sc = &St[c][q * Q];
-          float ps = 0.0f;
-#pragma unroll
-          for (int u = 0; u < Q; ++u) ps = fmaf(h[u], sc[u], ps);
-          ps += __shfl_xor(ps, 1, 64);
-          ps += __shfl_xor(ps, 2, 64);
-          const float v = (h[c % Q] * mv[c % Q]) / (ps + p.eps);
-          if (q == c / Q) hn[c % Q] = v;
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < Q; ++u) h[u] = hn[u];
-    }
+    quad_row_update<KM>(h, mv, St, q, k, p.kind, p.eps);
 #pragma unroll
     for (int u = 0; u < Q; ++u) {
       const int t = q * Q + u;
@@ -380,11 +379,22 @@ __global__ __launch_bounds__(256) void mode_finalize_kernel(ModeParams p, int nb
     p.gout[e] = Gu[e] / (sa * sb);
   }
   if (p.scale_in_finalize) {
-    for (int c = 0; c < k; ++c) {
-      const double nv = nrm[c];
-      if (!(nv > 0)) continue;
-      float *col = p.F + p.ldf * c;
-      for (long i = tid; i < p.I; i += 256) col[i] = (float)((double)col[i] / nv);
+    // Eight independent loads in flight per thread, then the stores.
+    const unsigned I = (unsigned)p.I, tot = I * (unsigned)k;
+    for (unsigned base = tid; base < tot; base += 256 * 8) {
+      float v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const unsigned e = base + 256u * u;
+        const unsigned c = e / I, i = e - c * I;
+        v[u] = e < tot ? p.F[i + p.ldf * c] : 0.0f;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const unsigned e = base + 256u * u;
+        const unsigned c = e / I, i = e - c * I;
+        if (e < tot && nrm[c] > 0) p.F[i + p.ldf * c] = (float)((double)v[u] / nrm[c]);
+      }
     }
   }
   if (!p.rel_out) return;
@@ -457,6 +467,10 @@ int contract_projection(planc_ctx *ctx, const float *P, long ldp, int nb, const 
   if (p.inner == 1) {
     dim3 grid((unsigned)((p.In + 63) / 64), (unsigned)k);
     contract_outer_kernel<<<grid, 1024, 0, ctx->stream>>>(p);
+  } else if (p.inner % 4 == 0 && p.inner <= WMAX && p.outer <= WMAX && ldp % 4 == 0 &&
+             ((uintptr_t)P % 16) == 0) {
+    dim3 grid((unsigned)((p.In + 3) / 4), (unsigned)k);
+    contract_wave_kernel<<<grid, 256, 0, ctx->stream>>>(p);
   } else {
     dim3 grid((unsigned)((p.In + TI - 1) / TI), (unsigned)k);
     contract_inner_kernel<<<grid, 256, 0, ctx->stream>>>(p);

@@ -1,0 +1,60 @@
+// Row updates of MU and HALS with a quad of lanes per row (shared by the NMF
+// and NTF kernels).  Lane q of the quad holds the Q = KM/4 consecutive row
+// values t = q*Q .. q*Q+Q-1 of h and of its numerator row; St is the k x k
+// FP32 Gram in LDS, transposed (St[x][t] = G(t, x)).  A column dot product
+// h . G(:, x) is the quad sum of the four per-lane partial chains.
+#ifndef PLANC_AMD_ROWUPD_H_
+#define PLANC_AMD_ROWUPD_H_
+
+#include "common.h"
+
+namespace planc {
+
+// kind 1 HALS (ntf/ntfhals.hpp:13-29, nmf/hals.hpp:67-75): columns in order,
+// each using the already-updated columns before it, floored at 1e-17.
+// kind 0 MU (nmf/mu.hpp:58-75): h .* n ./ (h G + eps) from the old row.
+// kind 2: no update.  q: the lane's position in its quad.
+template <int KM>
+__device__ inline void quad_row_update(float (&h)[KM / 4], const float (&nv)[KM / 4],
+                                       const float (&St)[KM][KM], int q, int k, int kind, float eps) {
+  constexpr int Q = KM / 4;
+  if (kind == 1) {
+#pragma unroll
+    for (int x = 0; x < KM; ++x) {
+      if (x < k) {
+        const float *sx = &St[x][q * Q];
+        float ps = 0.0f;
+#pragma unroll
+        for (int u = 0; u < Q; ++u) ps = fmaf(h[u], sx[u], ps);
+        ps += __shfl_xor(ps, 1, 64);
+        ps += __shfl_xor(ps, 2, 64);
+        float v = h[x % Q] + (nv[x % Q] - ps);
+        v = v < 1e-17f ? 1e-17f : v;
+        if (q == x / Q) h[x % Q] = v;
+      }
+    }
+  } else if (kind == 0) {
+    float hn[Q];
+#pragma unroll
+    for (int u = 0; u < Q; ++u) hn[u] = h[u];
+#pragma unroll
+    for (int c = 0; c < KM; ++c) {
+      if (c < k) {
+        const float *sc = &St[c][q * Q];
+        float ps = 0.0f;
+#pragma unroll
+        for (int u = 0; u < Q; ++u) ps = fmaf(h[u], sc[u], ps);
+        ps += __shfl_xor(ps, 1, 64);
+        ps += __shfl_xor(ps, 2, 64);
+        const float v = (h[c % Q] * nv[c % Q]) / (ps + eps);
+        if (q == c / Q) hn[c % Q] = v;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < Q; ++u) h[u] = hn[u];
+  }
+}
+
+}  // namespace planc
+
+#endif  // PLANC_AMD_ROWUPD_H_
