@@ -641,7 +641,10 @@ int planc_ntf_run(planc_ctx *ctx, const planc_ntf_desc *desc, const float *d_X, 
   // One outer iteration (all mode updates + the error), written so the same
   // launch sequence can be captured into a hipGraph: no allocation, no host
   // synchronisation, the error written to *rel_out.
-  auto one_iteration = [&](double *rel_out) -> int {
+  // rel_counter null: the error goes to *rel_out; otherwise to
+  // rel_out[*rel_counter] (device counter, incremented).
+  double *relslot = rel + iters;  // one spare slot after rel[]
+  auto one_iteration = [&](double *rel_out, int *rel_counter) -> int {
     int rc;
     bool err_done = false;
     for (int n = 0; n < N; ++n) {
@@ -697,7 +700,7 @@ int planc_ntf_run(planc_ctx *ctx, const planc_ntf_desc *desc, const float *d_X, 
         }
         const bool last_err = desc->compute_error && n == N - 1;
         RC(mode_update(ctx, kind, 1e-6f, Fn, dims[n], dims[n], k, Mp, mrs, mcs, Gc.data(), N, n, G[n],
-                       d_lambda, normX2, last_err ? rel_out : nullptr));
+                       d_lambda, normX2, last_err ? rel_out : nullptr, rel_counter));
         err_done = err_done || last_err;
         continue;
       }
@@ -712,17 +715,17 @@ int planc_ntf_run(planc_ctx *ctx, const planc_ntf_desc *desc, const float *d_X, 
       RC(scale_cols(ctx, Fn, dims[n], dims[n], nullptr, 0, 0, k, d_lambda));
       RC(gram(ctx, Fn, dims[n], k, dims[n], G[n]));
     }
-    if (desc->compute_error && !err_done)
+    if (desc->compute_error && !err_done) {
       RC(ntf_relerr(ctx, normX2, d_factors[N - 1], dims[N - 1], dims[N - 1], M, k, k, d_lambda,
-                    Gc.data(), N, rel_out));
+                    Gc.data(), N, rel_counter ? relslot : rel_out));
+      if (rel_counter) RC(counter_append(ctx, relslot, rel_out, rel_counter));
+    }
     return PLANC_OK;
   };
-  if (iters > 0) RC(one_iteration(rel));
+  if (iters > 0) RC(one_iteration(rel, nullptr));
   if (iters > 1) {
     // Iterations 1.. replay a captured graph of one iteration (launch-bound
-    // at small shapes: ~30 kernels per iteration); the error goes through a
-    // device counter into rel[1..].
-    double *relslot = rel + iters;  // one spare slot after rel[]
+    // at small shapes); the error goes through a device counter into rel[1..].
     int *counter = (int *)ws_get(ctx, 15, sizeof(int));
     if (!counter) return set_error(ctx, PLANC_ENOMEM, "planc_ntf_run: workspace");
     RC(counter_set(ctx, counter, 1));
@@ -738,9 +741,7 @@ int planc_ntf_run(planc_ctx *ctx, const planc_ntf_desc *desc, const float *d_X, 
     for (auto &w : ctx->ws) key.push_back((uintptr_t)w.ptr);
     if (!graph_lookup(ctx, key, &exec)) {
       RC(capture_graph(ctx, [&]() -> int {
-        int crc = one_iteration(relslot);
-        if (crc == PLANC_OK && desc->compute_error) crc = counter_append(ctx, relslot, rel, counter);
-        return crc;
+        return one_iteration(rel, counter);
       }, &exec));
       graph_store(ctx, key, exec);
     }

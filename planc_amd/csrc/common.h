@@ -229,9 +229,11 @@ int transpose_to_k_major(planc_ctx *ctx, const float *in, long I, long ldi, int 
 // Fused AUNTF mode update (Hadamard, row update, normalize(mode), Gram, error)
 // in two launches; kind 0 MU, 1 HALS, 2 none (factor already solved).
 bool mode_update_fits(long I, int k);
+// rel_out: where the relative error goes (null: none); with rel_counter it
+// is rel_out[*rel_counter] and the device counter is incremented.
 int mode_update(planc_ctx *ctx, int kind, float eps, float *F, long I, long ldf, int k, const float *M,
                 long mrs, long mcs, const double *const *grams, int ng, int skip, double *gout,
-                double *lambda, const double *normX2, double *rel_out);
+                double *lambda, const double *normX2, double *rel_out, int *rel_counter);
 
 }  // namespace planc
 

@@ -271,7 +271,7 @@ struct ModeParams {
   const double *normX2;
   double *rel_out;
   double *part;  // per block: k*k Gram of u, then k column dots
-  int scale_in_finalize;
+  int *rel_counter;  // non-null: the error goes to rel_out[*rel_counter], which is incremented
 };
 
 constexpr int MR = 64;  // rows per block-iteration of mode_rows_kernel
@@ -285,6 +285,22 @@ __global__ __launch_bounds__(256) void mode_rows_kernel(ModeParams p) {
   __shared__ float Ms[KM][MR + 1];                           // M(c, row)
   const int tid = threadIdx.x, q = tid & 3, rr = tid >> 2;
   const int k = p.k;
+  const long stride = (long)gridDim.x * MR;
+  // Row values and MTTKRP values of rows r0 + rr (clamped to a valid row).
+  auto load_rows = [&](long r0, float(&h)[Q], float(&mv)[Q]) {
+    const long i = r0 + rr;
+    const long ic = i < p.I ? i : p.I - 1;
+#pragma unroll
+    for (int u = 0; u < Q; ++u) {
+      const int t = q * Q + u;
+      const int tc = t < k ? t : k - 1;
+      h[u] = t < k ? p.F[ic + p.ldf * tc] : 0.0f;
+      mv[u] = t < k ? p.M[ic * p.mrs + tc * p.mcs] : 0.0f;
+    }
+  };
+  float h[Q], mv[Q];
+  long r0 = (long)blockIdx.x * MR;
+  load_rows(r0, h, mv);  // in flight while S is built
   for (int e = tid; e < KM * KM; e += 256) {
     const int a = e % KM, b = e / KM;
     double d = 1.0;
@@ -302,18 +318,9 @@ __global__ __launch_bounds__(256) void mode_rows_kernel(ModeParams p) {
   for (int j = 0; j < NE; ++j) acc[j] = 0.0;
   double dacc = 0.0;
   __syncthreads();
-  for (long r0 = (long)blockIdx.x * MR; r0 < p.I; r0 += (long)gridDim.x * MR) {
+  for (; r0 < p.I; r0 += stride) {
     const long i = r0 + rr;
     const bool valid = i < p.I;
-    const long ic = valid ? i : p.I - 1;
-    float h[Q], mv[Q];
-#pragma unroll
-    for (int u = 0; u < Q; ++u) {
-      const int t = q * Q + u;
-      const int tc = t < k ? t : k - 1;
-      h[u] = t < k ? p.F[ic + p.ldf * tc] : 0.0f;
-      mv[u] = t < k ? p.M[ic * p.mrs + tc * p.mcs] : 0.0f;
-    }
     quad_row_update<KM>(h, mv, St, q, k, p.kind, p.eps);
 #pragma unroll
     for (int u = 0; u < Q; ++u) {
@@ -322,6 +329,7 @@ __global__ __launch_bounds__(256) void mode_rows_kernel(ModeParams p) {
       Ms[t][rr] = valid ? mv[u] : 0.0f;
       if (valid && t < k) p.F[i + p.ldf * t] = h[u];
     }
+    if (r0 + stride < p.I) load_rows(r0 + stride, h, mv);  // next rows in flight
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < NE; ++j) {
@@ -329,7 +337,7 @@ __global__ __launch_bounds__(256) void mode_rows_kernel(ModeParams p) {
       const int a = e % KM, b = e / KM;
       if (e < KM * KM && a <= b && b < k) {
         double s = 0.0;
-#pragma unroll 8
+#pragma unroll 4
         for (int r = 0; r < MR; ++r) s += (double)Us[a][r] * (double)Us[b][r];
         acc[j] += s;
       }
@@ -355,58 +363,57 @@ __global__ __launch_bounds__(256) void mode_rows_kernel(ModeParams p) {
   if (tid < k) dst[k * k + tid] = dacc;
 }
 
+// Every block sums the block partials in block order (identical results in
+// every block), derives the norms and divides its slice of F by them; block
+// 0 also writes lambda, the Gram of the normalised factor and the error.
+// Rolled loops keep this run-once kernel's code small (instruction fetch,
+// not arithmetic, bounds a kernel this short).
 __global__ __launch_bounds__(256) void mode_finalize_kernel(ModeParams p, int nblocks) {
   __shared__ double Gu[64 * 64 + 64];
   __shared__ double nrm[64];
   __shared__ double red[4];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int k = p.k, kk = k * k;
+  const long tot = p.I * k;
+  // This block's factor values, loaded before the partial sums are waited on.
+  const long e0 = (long)blockIdx.x * 256 + tid;
+  const long estride = (long)gridDim.x * 256;
+  float f0 = 0.0f;
+  if (e0 < tot) {
+    const long c = e0 / p.I, i = e0 - c * p.I;
+    f0 = p.F[i + p.ldf * c];
+  }
   for (int e = tid; e < kk + k; e += 256) {
     double s = 0.0;
+#pragma unroll 4
     for (int b = 0; b < nblocks; ++b) s += p.part[(long)b * (kk + k) + e];
     Gu[e] = s;
   }
   __syncthreads();
-  if (tid < k) {
-    const double nv = sqrt(Gu[tid + k * tid]);
-    nrm[tid] = nv;
-    p.lambda[tid] = nv;
-  }
+  if (tid < k) nrm[tid] = sqrt(Gu[tid + k * tid]);
   __syncthreads();
+  for (long e = e0; e < tot; e += estride) {
+    const long c = e / p.I, i = e - c * p.I;
+    const float v = e == e0 ? f0 : p.F[i + p.ldf * c];
+    if (nrm[c] > 0) p.F[i + p.ldf * c] = (float)((double)v / nrm[c]);
+  }
+  if (blockIdx.x != 0) return;
+  if (tid < k) p.lambda[tid] = nrm[tid];
   for (int e = tid; e < kk; e += 256) {
     const int a = e % k, b = e / k;
     const double sa = nrm[a] > 0 ? nrm[a] : 1.0, sb = nrm[b] > 0 ? nrm[b] : 1.0;
     p.gout[e] = Gu[e] / (sa * sb);
-  }
-  if (p.scale_in_finalize) {
-    // Eight independent loads in flight per thread, then the stores.
-    const unsigned I = (unsigned)p.I, tot = I * (unsigned)k;
-    for (unsigned base = tid; base < tot; base += 256 * 8) {
-      float v[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const unsigned e = base + 256u * u;
-        const unsigned c = e / I, i = e - c * I;
-        v[u] = e < tot ? p.F[i + p.ldf * c] : 0.0f;
-      }
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const unsigned e = base + 256u * u;
-        const unsigned c = e / I, i = e - c * I;
-        if (e < tot && nrm[c] > 0) p.F[i + p.ldf * c] = (float)((double)v[u] / nrm[c]);
-      }
-    }
   }
   if (!p.rel_out) return;
   double t1 = 0.0, xh = 0.0;
   for (int c = tid; c < k; c += 256) t1 += Gu[kk + c];
   for (int e = tid; e < kk; e += 256) {
     const int a = e % k, b = e / k;
-    double v = nrm[a] * nrm[b];
     const double sa = nrm[a] > 0 ? nrm[a] : 1.0, sb = nrm[b] > 0 ? nrm[b] : 1.0;
+    double v = nrm[a] * nrm[b] * (Gu[e] / (sa * sb));
 #pragma unroll
     for (int g = 0; g < 8; ++g)
-      if (g < p.ng) v *= (g == p.skip) ? Gu[e] / (sa * sb) : p.g[g][e];
+      if (g < p.ng && g != p.skip) v *= p.g[g][e];
     xh += v;
   }
   t1 = wave_sum_d(t1);
@@ -422,19 +429,17 @@ __global__ __launch_bounds__(256) void mode_finalize_kernel(ModeParams p, int nb
     xh = red[0] + red[1] + red[2] + red[3];
     double e2 = *p.normX2 - 2.0 * t1 + xh;
     if (e2 < 0) e2 = 0;
-    *p.rel_out = sqrt(e2 / *p.normX2);
+    const double rel = sqrt(e2 / *p.normX2);
+    if (p.rel_counter) {
+      const int idx = *p.rel_counter;
+      p.rel_out[idx] = rel;
+      *p.rel_counter = idx + 1;
+    } else {
+      *p.rel_out = rel;
+    }
   }
 }
 
-// Divides the columns of F by the norms (F large enough for many blocks).
-__global__ void mode_scale_kernel(float *F, long I, long ldf, int k, const double *norms) {
-  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= I * k) return;
-  const int c = (int)(e / I);
-  const long i = e - (long)c * I;
-  const double nv = norms[c];
-  if (nv > 0) F[i + ldf * c] = (float)((double)F[i + ldf * c] / nv);
-}
 
 }  // namespace
 
@@ -529,7 +534,7 @@ bool mode_update_fits(long I, int k) { return k >= 1 && k <= 64 && I >= 1 && I <
 
 int mode_update(planc_ctx *ctx, int kind, float eps, float *F, long I, long ldf, int k, const float *M,
                 long mrs, long mcs, const double *const *grams, int ng, int skip, double *gout,
-                double *lambda, const double *normX2, double *rel_out) {
+                double *lambda, const double *normX2, double *rel_out, int *rel_counter) {
   if (!mode_update_fits(I, k) || ng < 1 || ng > 8 || skip < 0 || skip >= ng)
     return set_error(ctx, PLANC_EINVAL, "mode_update: bad arguments");
   ModeParams p{};
@@ -553,7 +558,7 @@ int mode_update(planc_ctx *ctx, int kind, float eps, float *F, long I, long ldf,
   if (nb > ctx->num_cu) nb = ctx->num_cu;
   p.part = (double *)ws_get(ctx, 18, sizeof(double) * (size_t)nb * (k * k + k));
   if (!p.part) return set_error(ctx, PLANC_ENOMEM, "mode_update: workspace");
-  p.scale_in_finalize = I * k <= 32768 ? 1 : 0;
+  p.rel_counter = rel_counter;
   if (k <= 16)
     mode_rows_kernel<16><<<(unsigned)nb, 256, 0, ctx->stream>>>(p);
   else if (k <= 32)
@@ -563,13 +568,10 @@ int mode_update(planc_ctx *ctx, int kind, float eps, float *F, long I, long ldf,
   else
     mode_rows_kernel<64><<<(unsigned)nb, 256, 0, ctx->stream>>>(p);
   PLANC_LAUNCH_CHECK(ctx);
-  mode_finalize_kernel<<<1, 256, 0, ctx->stream>>>(p, (int)nb);
+  long fb = (I * k + 255) / 256;
+  if (fb > 1024) fb = 1024;
+  mode_finalize_kernel<<<(unsigned)fb, 256, 0, ctx->stream>>>(p, (int)nb);
   PLANC_LAUNCH_CHECK(ctx);
-  if (!p.scale_in_finalize) {
-    const long tot = I * k;
-    mode_scale_kernel<<<(unsigned)((tot + 255) / 256), 256, 0, ctx->stream>>>(F, I, ldf, k, lambda);
-    PLANC_LAUNCH_CHECK(ctx);
-  }
   return PLANC_OK;
 }
 
