@@ -276,7 +276,7 @@ struct ModeParams {
 
 constexpr int MR = 64;  // rows per block-iteration of mode_rows_kernel
 
-template <int KM>
+template <int KM, int KIND>
 __global__ __launch_bounds__(256) void mode_rows_kernel(ModeParams p) {
   constexpr int Q = KM / 4;   // row values per lane
   constexpr int NE = (KM * KM + 255) / 256;  // Gram entries per thread
@@ -321,7 +321,7 @@ __global__ __launch_bounds__(256) void mode_rows_kernel(ModeParams p) {
   for (; r0 < p.I; r0 += stride) {
     const long i = r0 + rr;
     const bool valid = i < p.I;
-    quad_row_update<KM>(h, mv, St, q, k, p.kind, p.eps);
+    quad_row_update<KM>(h, mv, St, q, k, KIND, p.eps);
 #pragma unroll
     for (int u = 0; u < Q; ++u) {
       const int t = q * Q + u;
@@ -559,14 +559,24 @@ int mode_update(planc_ctx *ctx, int kind, float eps, float *F, long I, long ldf,
   p.part = (double *)ws_get(ctx, 18, sizeof(double) * (size_t)nb * (k * k + k));
   if (!p.part) return set_error(ctx, PLANC_ENOMEM, "mode_update: workspace");
   p.rel_counter = rel_counter;
+#define ROWS(KMV)                                                        \
+  do {                                                                   \
+    if (kind == 0)                                                       \
+      mode_rows_kernel<KMV, 0><<<(unsigned)nb, 256, 0, ctx->stream>>>(p); \
+    else if (kind == 1)                                                  \
+      mode_rows_kernel<KMV, 1><<<(unsigned)nb, 256, 0, ctx->stream>>>(p); \
+    else                                                                 \
+      mode_rows_kernel<KMV, 2><<<(unsigned)nb, 256, 0, ctx->stream>>>(p); \
+  } while (0)
   if (k <= 16)
-    mode_rows_kernel<16><<<(unsigned)nb, 256, 0, ctx->stream>>>(p);
+    ROWS(16);
   else if (k <= 32)
-    mode_rows_kernel<32><<<(unsigned)nb, 256, 0, ctx->stream>>>(p);
+    ROWS(32);
   else if (k <= 48)
-    mode_rows_kernel<48><<<(unsigned)nb, 256, 0, ctx->stream>>>(p);
+    ROWS(48);
   else
-    mode_rows_kernel<64><<<(unsigned)nb, 256, 0, ctx->stream>>>(p);
+    ROWS(64);
+#undef ROWS
   PLANC_LAUNCH_CHECK(ctx);
   long fb = (I * k + 255) / 256;
   if (fb > 1024) fb = 1024;

@@ -18,37 +18,50 @@ template <int KM>
 __device__ inline void quad_row_update(float (&h)[KM / 4], const float (&nv)[KM / 4],
                                        const float (&St)[KM][KM], int q, int k, int kind, float eps) {
   constexpr int Q = KM / 4;
+  // The column loop is rolled (small code: these kernels run once per mode
+  // and are bound by instruction fetch); the lane's values h[x % Q] are read
+  // and written through selects so h stays in registers.
   if (kind == 1) {
+#pragma unroll 1
+    for (int x = 0; x < k; ++x) {
+      const float *sx = &St[x][q * Q];
+      float ps = 0.0f, hx = 0.0f, mx = 0.0f;
+      const int xo = x % Q;
 #pragma unroll
-    for (int x = 0; x < KM; ++x) {
-      if (x < k) {
-        const float *sx = &St[x][q * Q];
-        float ps = 0.0f;
-#pragma unroll
-        for (int u = 0; u < Q; ++u) ps = fmaf(h[u], sx[u], ps);
-        ps += __shfl_xor(ps, 1, 64);
-        ps += __shfl_xor(ps, 2, 64);
-        float v = h[x % Q] + (nv[x % Q] - ps);
-        v = v < 1e-17f ? 1e-17f : v;
-        if (q == x / Q) h[x % Q] = v;
+      for (int u = 0; u < Q; ++u) {
+        ps = fmaf(h[u], sx[u], ps);
+        hx = (u == xo) ? h[u] : hx;
+        mx = (u == xo) ? nv[u] : mx;
       }
+      ps += __shfl_xor(ps, 1, 64);
+      ps += __shfl_xor(ps, 2, 64);
+      float v = hx + (mx - ps);
+      v = v < 1e-17f ? 1e-17f : v;
+      const bool own = q == x / Q;
+#pragma unroll
+      for (int u = 0; u < Q; ++u) h[u] = (own && u == xo) ? v : h[u];
     }
   } else if (kind == 0) {
     float hn[Q];
 #pragma unroll
     for (int u = 0; u < Q; ++u) hn[u] = h[u];
+#pragma unroll 1
+    for (int c = 0; c < k; ++c) {
+      const float *sc = &St[c][q * Q];
+      float ps = 0.0f, hc = 0.0f, mc = 0.0f;
+      const int co = c % Q;
 #pragma unroll
-    for (int c = 0; c < KM; ++c) {
-      if (c < k) {
-        const float *sc = &St[c][q * Q];
-        float ps = 0.0f;
-#pragma unroll
-        for (int u = 0; u < Q; ++u) ps = fmaf(h[u], sc[u], ps);
-        ps += __shfl_xor(ps, 1, 64);
-        ps += __shfl_xor(ps, 2, 64);
-        const float v = (h[c % Q] * nv[c % Q]) / (ps + eps);
-        if (q == c / Q) hn[c % Q] = v;
+      for (int u = 0; u < Q; ++u) {
+        ps = fmaf(h[u], sc[u], ps);
+        hc = (u == co) ? h[u] : hc;
+        mc = (u == co) ? nv[u] : mc;
       }
+      ps += __shfl_xor(ps, 1, 64);
+      ps += __shfl_xor(ps, 2, 64);
+      const float v = (hc * mc) / (ps + eps);
+      const bool own = q == c / Q;
+#pragma unroll
+      for (int u = 0; u < Q; ++u) hn[u] = (own && u == co) ? v : hn[u];
     }
 #pragma unroll
     for (int u = 0; u < Q; ++u) h[u] = hn[u];
