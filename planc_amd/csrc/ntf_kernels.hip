@@ -278,12 +278,15 @@ constexpr int MR = 64;  // rows per block-iteration of mode_rows_kernel
 
 template <int KM, int KIND>
 __global__ __launch_bounds__(256) void mode_rows_kernel(ModeParams p) {
-  constexpr int Q = KM / 4;   // row values per lane
-  constexpr int NE = (KM * KM + 255) / 256;  // Gram entries per thread
+  constexpr int Q = KM / 4;          // row values per lane
+  constexpr int NT = KM / 16;        // 16 x 16 tiles per Gram side
+  constexpr int NUP = NT * (NT + 1) / 2;  // upper-triangle tiles
+  constexpr int TPW = (NUP + 3) / 4;      // tiles per wave
   __shared__ __attribute__((aligned(16))) float St[KM][KM];  // St[x][t] = S(t, x)
   __shared__ float Us[KM][MR + 1];                           // u(row, c) at Us[c][row]
   __shared__ float Ms[KM][MR + 1];                           // M(c, row)
   const int tid = threadIdx.x, q = tid & 3, rr = tid >> 2;
+  const int lane = tid & 63, wave = tid >> 6;
   const int k = p.k;
   const long stride = (long)gridDim.x * MR;
   // Row values and MTTKRP values of rows r0 + rr (clamped to a valid row).
@@ -297,6 +300,16 @@ __global__ __launch_bounds__(256) void mode_rows_kernel(ModeParams p) {
       h[u] = t < k ? p.F[ic + p.ldf * tc] : 0.0f;
       mv[u] = t < k ? p.M[ic * p.mrs + tc * p.mcs] : 0.0f;
     }
+  };
+  // Upper-triangle tile index -> (ta, tb), ta <= tb.
+  auto tile_of = [&](int t, int &ta, int &tb) {
+    ta = 0;
+    int rem = t;
+    while (rem >= NT - ta) {
+      rem -= NT - ta;
+      ++ta;
+    }
+    tb = ta + rem;
   };
   float h[Q], mv[Q];
   long r0 = (long)blockIdx.x * MR;
@@ -313,9 +326,9 @@ __global__ __launch_bounds__(256) void mode_rows_kernel(ModeParams p) {
     }
     St[b][a] = (a < k && b < k) ? (float)d : 0.0f;
   }
-  double acc[NE];
+  doublex4 gacc[TPW];
 #pragma unroll
-  for (int j = 0; j < NE; ++j) acc[j] = 0.0;
+  for (int j = 0; j < TPW; ++j) gacc[j] = doublex4{0.0, 0.0, 0.0, 0.0};
   double dacc = 0.0;
   __syncthreads();
   for (; r0 < p.I; r0 += stride) {
@@ -331,36 +344,59 @@ __global__ __launch_bounds__(256) void mode_rows_kernel(ModeParams p) {
     }
     if (r0 + stride < p.I) load_rows(r0 + stride, h, mv);  // next rows in flight
     __syncthreads();
+    // Gram of this chunk's rows on FP64 MFMA: D(a, b) += sum_r U(r, a) U(r, b)
+    // per 16 x 16 tile; lane (lane%16, lane/16) feeds A = U(r0 + lane/16, a0 +
+    // lane%16) and B = U(r0 + lane/16, b0 + lane%16).
 #pragma unroll
-    for (int j = 0; j < NE; ++j) {
-      const int e = tid + 256 * j;
-      const int a = e % KM, b = e / KM;
-      if (e < KM * KM && a <= b && b < k) {
-        double s = 0.0;
+    for (int j = 0; j < TPW; ++j) {
+      const int t = wave + 4 * j;
+      if (t < NUP) {
+        int ta, tb;
+        tile_of(t, ta, tb);
+        const float *ua = &Us[16 * ta + (lane & 15)][0];
+        const float *ub = &Us[16 * tb + (lane & 15)][0];
 #pragma unroll 4
-        for (int r = 0; r < MR; ++r) s += (double)Us[a][r] * (double)Us[b][r];
-        acc[j] += s;
+        for (int r = lane >> 4; r < MR; r += 4)
+          gacc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)ua[r], (double)ub[r], gacc[j], 0, 0, 0);
       }
     }
-    if (tid < k) {
-      double s = 0.0;
-#pragma unroll 8
-      for (int r = 0; r < MR; ++r) s += (double)Us[tid][r] * (double)Ms[tid][r];
-      dacc += s;
+    // Column dots sum_r U(r, c) M(c, r): a quad per column, quad sum.
+    if (tid < 4 * KM) {
+      const int c = tid >> 2, part = tid & 3;
+      double sd = 0.0;
+      for (int r = part; r < MR; r += 4) sd += (double)Us[c][r] * (double)Ms[c][r];
+      dacc += sd;
     }
     __syncthreads();
   }
   double *dst = p.part + (long)blockIdx.x * (k * k + k);
+  // FP64 D layout: lane holds D(lane / 16 + 4 v, lane % 16), v = 0..3.
 #pragma unroll
-  for (int j = 0; j < NE; ++j) {
-    const int e = tid + 256 * j;
-    const int a = e % KM, b = e / KM;
-    if (e < KM * KM && a <= b && b < k) {
-      dst[a + k * b] = acc[j];
-      dst[b + k * a] = acc[j];
+  for (int j = 0; j < TPW; ++j) {
+    const int t = wave + 4 * j;
+    if (t < NUP) {
+      int ta, tb;
+      tile_of(t, ta, tb);
+      const int b = 16 * tb + (lane & 15);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int a = 16 * ta + (lane >> 4) + 4 * v;
+        if (a < k && b < k && (ta != tb || a <= b)) {
+          dst[a + k * b] = gacc[j][v];
+          dst[b + k * a] = gacc[j][v];
+        }
+      }
     }
   }
-  if (tid < k) dst[k * k + tid] = dacc;
+  if (tid < 4 * KM) {
+    // fixed-order quad combine: ((d0 + d1) + (d2 + d3))
+    double d1 = __shfl_xor(dacc, 1, 64);
+    double s01 = (tid & 1) ? d1 + dacc : dacc + d1;
+    double s23 = __shfl_xor(s01, 2, 64);
+    double tot = (tid & 2) ? s23 + s01 : s01 + s23;
+    const int c = tid >> 2;
+    if ((tid & 3) == 0 && c < k) dst[k * k + c] = tot;
+  }
 }
 
 // Every block sums the block partials in block order (identical results in

@@ -10,6 +10,15 @@
 
 namespace planc {
 
+// Sum over the four lanes of a quad with DPP quad permutes (no LDS traffic);
+// every lane of the quad gets ((v0 + v1) + (v2 + v3)).
+__device__ inline float quad_sum(float v) {
+  const int x1 = __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0xB1, 0xF, 0xF, false);  // lanes 1,0,3,2
+  v += __builtin_bit_cast(float, x1);
+  const int x2 = __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x4E, 0xF, 0xF, false);  // lanes 2,3,0,1
+  return v + __builtin_bit_cast(float, x2);
+}
+
 // kind 1 HALS (ntf/ntfhals.hpp:13-29, nmf/hals.hpp:67-75): columns in order,
 // each using the already-updated columns before it, floored at 1e-17.
 // kind 0 MU (nmf/mu.hpp:58-75): h .* n ./ (h G + eps) from the old row.
@@ -33,8 +42,7 @@ __device__ inline void quad_row_update(float (&h)[KM / 4], const float (&nv)[KM 
         hx = (u == xo) ? h[u] : hx;
         mx = (u == xo) ? nv[u] : mx;
       }
-      ps += __shfl_xor(ps, 1, 64);
-      ps += __shfl_xor(ps, 2, 64);
+      ps = quad_sum(ps);
       float v = hx + (mx - ps);
       v = v < 1e-17f ? 1e-17f : v;
       const bool own = q == x / Q;
@@ -56,8 +64,7 @@ __device__ inline void quad_row_update(float (&h)[KM / 4], const float (&nv)[KM 
         hc = (u == co) ? h[u] : hc;
         mc = (u == co) ? nv[u] : mc;
       }
-      ps += __shfl_xor(ps, 1, 64);
-      ps += __shfl_xor(ps, 2, 64);
+      ps = quad_sum(ps);
       const float v = (hc * mc) / (ps + eps);
       const bool own = q == c / Q;
 #pragma unroll
