@@ -34,56 +34,85 @@ __device__ inline double block_sum(double v, double *red) {
 }
 
 // ---------------------------------------------------------------------------
-// G = X^T X in FP64.  Each block walks 128-row chunks (grid-stride) through
-// LDS; thread (a, b) accumulates a 4x4 tile of G across all its chunks and
-// the block writes one partial.  gram_reduce_kernel sums the block partials
-// in block order (four waves over interleaved blocks, combined in wave
-// order).
-constexpr int GRAM_RC = 128;
+// G = X^T X on FP64 MFMA (v_mfma_f64_16x16x4f64): 64-row chunks of X go
+// through LDS (the next chunk's loads are in flight during the current
+// chunk's MFMAs); wave w owns the upper-triangle 16 x 16 tiles w, w+4, ...
+// of the KM x KM Gram and accumulates them over its block's chunks.  One
+// full KM x KM partial per block (mirrored), summed by gram_reduce_kernel.
+constexpr int GCR = 64;
 
-__global__ __launch_bounds__(NTHR) void gram_partial_kernel(const float *X, long rows, int k,
-                                                            long ldx, double *part, int kp) {
-  __shared__ float Xs[GRAM_RC][64 + 4];
-  const int tid = threadIdx.x;
-  const int ta = tid >> 4, tb = tid & 15;
-  double acc[4][4];
+template <int KM>
+__global__ __launch_bounds__(NTHR) void gram_mfma_kernel(const float *X, long rows, int k, long ldx,
+                                                         double *part) {
+  constexpr int NT = KM / 16, NUP = NT * (NT + 1) / 2, TPW = (NUP + 3) / 4;
+  constexpr int PER = GCR * KM / NTHR;  // chunk values per thread
+  __shared__ float Xs[KM][GCR + 1];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  auto tile_of = [&](int t, int &ta, int &tb) {
+    ta = 0;
+    int rem = t;
+    while (rem >= NT - ta) {
+      rem -= NT - ta;
+      ++ta;
+    }
+    tb = ta + rem;
+  };
+  auto load = [&](long r0, float(&v)[PER]) {
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < PER; ++j) {
+      const int e = tid + NTHR * j, r = e % GCR, c = e / GCR;
+      const long gr = r0 + r;
+      v[j] = (gr < rows && c < k) ? X[gr + ldx * c] : 0.0f;
+    }
+  };
+  doublex4 acc[TPW];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
-  const bool active = 4 * ta < kp && 4 * tb < kp;
-  for (long chunk = blockIdx.x; chunk * GRAM_RC < rows; chunk += gridDim.x) {
-    long r0 = chunk * GRAM_RC;
+  for (int j = 0; j < TPW; ++j) acc[j] = doublex4{0.0, 0.0, 0.0, 0.0};
+  const long stride = (long)gridDim.x * GCR;
+  float v[PER];
+  long r0 = (long)blockIdx.x * GCR;
+  if (r0 < rows) load(r0, v);
+  for (; r0 < rows; r0 += stride) {
     __syncthreads();
-    for (int e = tid; e < GRAM_RC * kp; e += NTHR) {
-      int r = e % GRAM_RC, c = e / GRAM_RC;
-      long gr = r0 + r;
-      Xs[r][c] = (gr < rows && c < k) ? X[gr + ldx * c] : 0.0f;
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      const int e = tid + NTHR * j;
+      Xs[e / GCR][e % GCR] = v[j];
     }
     __syncthreads();
-    if (active) {
-      for (int r = 0; r < GRAM_RC; ++r) {
-        double a[4], b[4];
+    if (r0 + stride < rows) load(r0 + stride, v);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          a[i] = Xs[r][4 * ta + i];
-          b[i] = Xs[r][4 * tb + i];
-        }
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-          for (int j = 0; j < 4; ++j) acc[i][j] += a[i] * b[j];
+    for (int j = 0; j < TPW; ++j) {
+      const int t = wave + 4 * j;
+      if (t < NUP) {
+        int ta, tb;
+        tile_of(t, ta, tb);
+        const float *ua = &Xs[16 * ta + (lane & 15)][0];
+        const float *ub = &Xs[16 * tb + (lane & 15)][0];
+#pragma unroll 4
+        for (int r = lane >> 4; r < GCR; r += 4)
+          acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)ua[r], (double)ub[r], acc[j], 0, 0, 0);
       }
     }
   }
-  if (active) {
+  // FP64 D layout: lane holds D(lane / 16 + 4 v, lane % 16).
+  double *dst = part + (long)blockIdx.x * KM * KM;
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+  for (int j = 0; j < TPW; ++j) {
+    const int t = wave + 4 * j;
+    if (t < NUP) {
+      int ta, tb;
+      tile_of(t, ta, tb);
+      const int b = 16 * tb + (lane & 15);
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        int ga = 4 * ta + i, gb = 4 * tb + j;
-        part[(size_t)blockIdx.x * kp * kp + ga + (size_t)kp * gb] = acc[i][j];
+      for (int w = 0; w < 4; ++w) {
+        const int a = 16 * ta + (lane >> 4) + 4 * w;
+        if (ta != tb || a <= b) {
+          dst[a + KM * b] = acc[j][w];
+          dst[b + KM * a] = acc[j][w];
+        }
       }
+    }
   }
 }
 
@@ -587,15 +616,22 @@ int hals_cols_impl(planc_ctx *ctx, float *W, long m, int k, long ldw, const floa
 
 int gram(planc_ctx *ctx, const float *X, long rows, int k, long ldx, double *G) {
   if (k <= 0 || k > 64 || rows < 0) return set_error(ctx, PLANC_EINVAL, "gram: k must be 1..64");
-  int kp = ((k + 3) / 4) * 4;
-  long nchunks = (rows + GRAM_RC - 1) / GRAM_RC;
+  const int kp = ((k + 15) / 16) * 16;
+  long nchunks = (rows + GCR - 1) / GCR;
   if (nchunks == 0) nchunks = 1;
   long nb = nchunks < ctx->num_cu ? nchunks : ctx->num_cu;
-  // One partial per block (each block accumulates its chunks).
+  // One partial per block (each block accumulates its chunks; rows == 0
+  // still writes zero partials).
   double *part = (double *)ws_get(ctx, 1, sizeof(double) * (size_t)nb * kp * kp);
   if (!part) return set_error(ctx, PLANC_ENOMEM, "gram: workspace");
-  // rows == 0: the kernel still writes zero partials.
-  gram_partial_kernel<<<(unsigned)nb, NTHR, 0, ctx->stream>>>(X, rows, k, ldx, part, kp);
+  if (kp == 16)
+    gram_mfma_kernel<16><<<(unsigned)nb, NTHR, 0, ctx->stream>>>(X, rows, k, ldx, part);
+  else if (kp == 32)
+    gram_mfma_kernel<32><<<(unsigned)nb, NTHR, 0, ctx->stream>>>(X, rows, k, ldx, part);
+  else if (kp == 48)
+    gram_mfma_kernel<48><<<(unsigned)nb, NTHR, 0, ctx->stream>>>(X, rows, k, ldx, part);
+  else
+    gram_mfma_kernel<64><<<(unsigned)nb, NTHR, 0, ctx->stream>>>(X, rows, k, ldx, part);
   PLANC_LAUNCH_CHECK(ctx);
   gram_reduce_kernel<<<(unsigned)((kp * kp + 63) / 64), NTHR, 0, ctx->stream>>>(part, nb, k, kp, G);
   PLANC_LAUNCH_CHECK(ctx);
