@@ -195,7 +195,7 @@ __global__ __launch_bounds__(NTHR) void rows_quad_kernel(float *H, long n, int k
     h[u] = t < k ? H[j + ldh * tc] : 0.0f;
     nv[u] = t < k ? N[j * nrs + tc * ncs] : 0.0f;
   }
-  quad_row_update<KM>(h, nv, Gs, q, k, kind, eps);
+  quad_row_update<KM, false>(h, nv, Gs, q, k, kind, eps);
 #pragma unroll
   for (int u = 0; u < Q; ++u) {
     const int t = q * Q + u;
@@ -543,6 +543,61 @@ __global__ __launch_bounds__(256) void transpose16_kernel(const uint16_t *in, lo
   }
 }
 
+// The same with 16-byte global loads and stores (rows, cols, ldi, ldo all
+// multiples of 8 and 16-byte aligned bases): a thread moves 8 consecutive
+// elements of a source column in and 8 of a destination column out.
+__global__ __launch_bounds__(256) void transpose16_vec_kernel(const uint16_t *in, long rows, long cols,
+                                                              long ldi, uint16_t *out, long ldo) {
+  __shared__ uint16_t tile[64][66];  // tile[c][r]
+  const long r0 = (long)blockIdx.x * 64, c0 = (long)blockIdx.y * 64;
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int q = threadIdx.x + 256 * j, c = q >> 3, rc = (q & 7) * 8;
+    const long gr = r0 + rc, gc = c0 + c;
+    uint4 v = {0u, 0u, 0u, 0u};
+    if (gr < rows && gc < cols) v = *(const uint4 *)(in + gr + ldi * gc);
+    uint16_t h[8];
+    __builtin_memcpy(h, &v, 16);
+#pragma unroll
+    for (int u = 0; u < 8; ++u) tile[c][rc + u] = h[u];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int q = threadIdx.x + 256 * j, r = q >> 3, cc = (q & 7) * 8;
+    const long gr = r0 + r, gc = c0 + cc;
+    if (gr < rows && gc < cols) {
+      uint16_t h[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) h[u] = tile[cc + u][r];
+      *(uint4 *)(out + gc + ldo * gr) = __builtin_bit_cast(uint4, h);
+    }
+  }
+}
+
+// FP64 sum of X .* Y over n contiguous elements (16-byte loads), one partial
+// per block.
+__global__ __launch_bounds__(NTHR) void dot_flat_kernel(const float *X, const float *Y, long n,
+                                                        double *part) {
+  __shared__ double red[NTHR / 64];
+  const long n4 = n / 4;
+  double s = 0.0;
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n4; e += stride) {
+    const floatx4 x = ((const floatx4 *)X)[e], y = ((const floatx4 *)Y)[e];
+    s += (double)x[0] * (double)y[0];
+    s += (double)x[1] * (double)y[1];
+    s += (double)x[2] * (double)y[2];
+    s += (double)x[3] * (double)y[3];
+  }
+  if (blockIdx.x == 0 && threadIdx.x < n - 4 * n4) {
+    const long e = 4 * n4 + threadIdx.x;
+    s += (double)X[e] * (double)Y[e];
+  }
+  double bs = block_sum(s, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = bs;
+}
+
 template <int KM>
 int launch_rows(planc_ctx *ctx, int kind, float *H, long n, int k, long ldh, const float *N,
                 long nrs, long ncs, const double *G, float eps) {
@@ -690,8 +745,13 @@ int dot(planc_ctx *ctx, const float *X, long xrs, long xcs, const float *Y, long
   if (nb < 1) nb = 1;
   double *part = (double *)ws_get(ctx, 3, sizeof(double) * (size_t)nb);
   if (!part) return set_error(ctx, PLANC_ENOMEM, "dot: workspace");
-  dot_partial_kernel<<<(unsigned)nb, NTHR, 0, ctx->stream>>>(X, xrs, xcs, Y, yrs, ycs, rows, cols,
-                                                             part);
+  const bool flat = xrs == 1 && yrs == 1 && (cols == 1 || (xcs == rows && ycs == rows)) &&
+                    (uintptr_t)X % 16 == 0 && (uintptr_t)Y % 16 == 0;
+  if (flat)
+    dot_flat_kernel<<<(unsigned)nb, NTHR, 0, ctx->stream>>>(X, Y, tot, part);
+  else
+    dot_partial_kernel<<<(unsigned)nb, NTHR, 0, ctx->stream>>>(X, xrs, xcs, Y, yrs, ycs, rows, cols,
+                                                               part);
   PLANC_LAUNCH_CHECK(ctx);
   sum_partials_kernel<<<1, 64, 0, ctx->stream>>>(part, nb, out, accumulate);
   PLANC_LAUNCH_CHECK(ctx);
@@ -767,8 +827,14 @@ int transpose16(planc_ctx *ctx, const void *in, long rows, long cols, long ldi, 
   if (rows <= 0 || cols <= 0) return PLANC_OK;
   const long gx = (rows + 63) / 64, gy = (cols + 63) / 64;
   if (gy > 65535) return set_error(ctx, PLANC_EINVAL, "transpose16: too many columns");
-  transpose16_kernel<<<dim3((unsigned)gx, (unsigned)gy), 256, 0, ctx->stream>>>(
-      (const uint16_t *)in, rows, cols, ldi, (uint16_t *)out, ldo);
+  const bool vec = rows % 8 == 0 && cols % 8 == 0 && ldi % 8 == 0 && ldo % 8 == 0 &&
+                   (uintptr_t)in % 16 == 0 && (uintptr_t)out % 16 == 0;
+  if (vec)
+    transpose16_vec_kernel<<<dim3((unsigned)gx, (unsigned)gy), 256, 0, ctx->stream>>>(
+        (const uint16_t *)in, rows, cols, ldi, (uint16_t *)out, ldo);
+  else
+    transpose16_kernel<<<dim3((unsigned)gx, (unsigned)gy), 256, 0, ctx->stream>>>(
+        (const uint16_t *)in, rows, cols, ldi, (uint16_t *)out, ldo);
   PLANC_LAUNCH_CHECK(ctx);
   return PLANC_OK;
 }

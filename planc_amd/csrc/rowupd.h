@@ -23,52 +23,85 @@ __device__ inline float quad_sum(float v) {
 // each using the already-updated columns before it, floored at 1e-17.
 // kind 0 MU (nmf/mu.hpp:58-75): h .* n ./ (h G + eps) from the old row.
 // kind 2: no update.  q: the lane's position in its quad.
-template <int KM>
+// ROLLED: the column loop stays a loop and the lane's values h[x % Q] are
+// read and written through selects (small code, for kernels that run once
+// per call and are bound by instruction fetch); otherwise the loop is fully
+// unrolled with static register indexes (fewer instructions per column, for
+// kernels over many rows).
+template <int KM, bool ROLLED>
 __device__ inline void quad_row_update(float (&h)[KM / 4], const float (&nv)[KM / 4],
                                        const float (&St)[KM][KM], int q, int k, int kind, float eps) {
   constexpr int Q = KM / 4;
-  // The column loop is rolled (small code: these kernels run once per mode
-  // and are bound by instruction fetch); the lane's values h[x % Q] are read
-  // and written through selects so h stays in registers.
   if (kind == 1) {
+    if constexpr (ROLLED) {
 #pragma unroll 1
-    for (int x = 0; x < k; ++x) {
-      const float *sx = &St[x][q * Q];
-      float ps = 0.0f, hx = 0.0f, mx = 0.0f;
-      const int xo = x % Q;
+      for (int x = 0; x < k; ++x) {
+        const float *sx = &St[x][q * Q];
+        float ps = 0.0f, hx = 0.0f, mx = 0.0f;
+        const int xo = x % Q;
 #pragma unroll
-      for (int u = 0; u < Q; ++u) {
-        ps = fmaf(h[u], sx[u], ps);
-        hx = (u == xo) ? h[u] : hx;
-        mx = (u == xo) ? nv[u] : mx;
+        for (int u = 0; u < Q; ++u) {
+          ps = fmaf(h[u], sx[u], ps);
+          hx = (u == xo) ? h[u] : hx;
+          mx = (u == xo) ? nv[u] : mx;
+        }
+        ps = quad_sum(ps);
+        float v = hx + (mx - ps);
+        v = v < 1e-17f ? 1e-17f : v;
+        const bool own = q == x / Q;
+#pragma unroll
+        for (int u = 0; u < Q; ++u) h[u] = (own && u == xo) ? v : h[u];
       }
-      ps = quad_sum(ps);
-      float v = hx + (mx - ps);
-      v = v < 1e-17f ? 1e-17f : v;
-      const bool own = q == x / Q;
+    } else {
 #pragma unroll
-      for (int u = 0; u < Q; ++u) h[u] = (own && u == xo) ? v : h[u];
+      for (int x = 0; x < KM; ++x) {
+        if (x < k) {
+          const float *sx = &St[x][q * Q];
+          float ps = 0.0f;
+#pragma unroll
+          for (int u = 0; u < Q; ++u) ps = fmaf(h[u], sx[u], ps);
+          ps = quad_sum(ps);
+          float v = h[x % Q] + (nv[x % Q] - ps);
+          v = v < 1e-17f ? 1e-17f : v;
+          if (q == x / Q) h[x % Q] = v;
+        }
+      }
     }
   } else if (kind == 0) {
     float hn[Q];
 #pragma unroll
     for (int u = 0; u < Q; ++u) hn[u] = h[u];
+    if constexpr (ROLLED) {
 #pragma unroll 1
-    for (int c = 0; c < k; ++c) {
-      const float *sc = &St[c][q * Q];
-      float ps = 0.0f, hc = 0.0f, mc = 0.0f;
-      const int co = c % Q;
+      for (int c = 0; c < k; ++c) {
+        const float *sc = &St[c][q * Q];
+        float ps = 0.0f, hc = 0.0f, mc = 0.0f;
+        const int co = c % Q;
 #pragma unroll
-      for (int u = 0; u < Q; ++u) {
-        ps = fmaf(h[u], sc[u], ps);
-        hc = (u == co) ? h[u] : hc;
-        mc = (u == co) ? nv[u] : mc;
+        for (int u = 0; u < Q; ++u) {
+          ps = fmaf(h[u], sc[u], ps);
+          hc = (u == co) ? h[u] : hc;
+          mc = (u == co) ? nv[u] : mc;
+        }
+        ps = quad_sum(ps);
+        const float v = (hc * mc) / (ps + eps);
+        const bool own = q == c / Q;
+#pragma unroll
+        for (int u = 0; u < Q; ++u) hn[u] = (own && u == co) ? v : hn[u];
       }
-      ps = quad_sum(ps);
-      const float v = (hc * mc) / (ps + eps);
-      const bool own = q == c / Q;
+    } else {
 #pragma unroll
-      for (int u = 0; u < Q; ++u) hn[u] = (own && u == co) ? v : hn[u];
+      for (int c = 0; c < KM; ++c) {
+        if (c < k) {
+          const float *sc = &St[c][q * Q];
+          float ps = 0.0f;
+#pragma unroll
+          for (int u = 0; u < Q; ++u) ps = fmaf(h[u], sc[u], ps);
+          ps = quad_sum(ps);
+          const float v = (h[c % Q] * nv[c % Q]) / (ps + eps);
+          if (q == c / Q) hn[c % Q] = v;
+        }
+      }
     }
 #pragma unroll
     for (int u = 0; u < Q; ++u) h[u] = hn[u];
