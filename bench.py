@@ -214,13 +214,13 @@ def main():
             traffic = None
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "gemm_f32_kernel<NT=2,NN> (right partial MTTKRP, X_(65536x256)*A_2)",
+                "kernel": "nn_f32_kernel<NT=2> (right partial MTTKRP, X_(65536x256)*A_2; FP32 MFMA, operands streamed to registers)",
                 "algorithmic_bytes": bytes_right, "avg_ms": round(t_right, 5),
                 "note": "the 67 MB tensor fits the 256 MB Infinity Cache; see legs.mttkrp_128^4 "
                         "for a tensor that does not"}
     left_info = {"avg_ms": round(t_left, 5),
                  "achieved_GBs": round(bytes_left / (t_left * 1e-3) / 1e9, 1),
-                 "kernels": "gemm_f32_kernel<NT=2,TN> split-K + reduce_splits_kernel"}
+                 "kernels": "tn_f32_kernel (KRP of A_0, A_1 fused) split-K + splitk_reduce_kernel"}
 
     # ---- parity and CPU baseline over a bounded sample -------------------
     cpu = None
