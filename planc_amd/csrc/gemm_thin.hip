@@ -459,18 +459,26 @@ __global__ __launch_bounds__(NTHR) void tn_f32_kernel(StreamParams p) {
 }
 
 // ---------------------------------------------------------------------------
-// Packs the thin operand into the 16-bit MFMA A-operand layout:
-// out[(step*NT + t)*64 + lane] = B(32 step + 8 lg + 0..7, 16 t + lc).
+// Packs the thin operand into the 16-bit MFMA A-operand layout, in 64-k
+// super-steps S split into two halves h: for lane (lc, lg),
+// out[((2S + h)*NT + t)*64 + lc + 16 lg] = B(64 S + 16 lg + 8 h + 0..7, 16 t + lc).
+// pack_index() finds the word of any 8-aligned k block.
+__device__ inline long pack_index(long kk, int t, int lc, int NT) {
+  const long S = kk >> 6;
+  const int rem = (int)(kk & 63);
+  return ((2 * S + ((rem >> 3) & 1)) * NT + t) * 64 + lc + 16 * (rem >> 4);
+}
+
 template <typename TA>
-__global__ void pack_thin16_kernel(ThinDev th, long K, int NT, long nsteps, uint4 *out) {
+__global__ void pack_thin16_kernel(ThinDev th, long K, int NT, long nsuper, uint4 *out) {
   const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= nsteps * NT * 64) return;
+  if (e >= nsuper * 2 * NT * 64) return;
   const int lane = (int)(e & 63);
   const long st = e >> 6;
   const int t = (int)(st % NT);
-  const long step = st / NT;
+  const long sh = st / NT;  // 2 S + h
   const int c = 16 * t + (lane & 15);
-  const long kb = 32 * step + 8 * (lane >> 4);
+  const long kb = 64 * (sh >> 1) + 16 * (lane >> 4) + 8 * (sh & 1);
   floatx4 lo = thin4(th, kb, K, c), hi = thin4(th, kb + 4, K, c);
   uint16_t h[8];
 #pragma unroll
@@ -481,16 +489,20 @@ __global__ void pack_thin16_kernel(ThinDev th, long K, int NT, long nsteps, uint
   out[e] = __builtin_bit_cast(uint4, h);
 }
 
-// 16-bit TN.  Step = 32 k.  SAFE: scalar loads of A (unaligned lda/base).
-template <int NT, int CW>
+// 16-bit TN.  DUAL = 0: step = 32 k, a lane reads 8 consecutive k of a
+// column (16 bytes); DUAL = 1: step = 64 k, a lane reads 16 consecutive k
+// (two 16-byte words, a full 128-byte line per column across the four lane
+// groups) feeding two MFMAs.  SAFE: scalar loads of A (unaligned lda/base).
+template <int NT, int CW, int NS>
 struct BufH {
-  uint4 x[CW];
-  uint4 a[NT];
+  uint4 x[CW][NS];
+  uint4 a[NT][NS];
 };
 
-template <typename TA, int NT, int CW, int SAFE>
+template <typename TA, int NT, int CW, int SAFE, int DUAL>
 __global__ __launch_bounds__(NTHR) void tn_h_kernel(StreamParams p) {
-  constexpr int STEP = 32, PD = 3, NA = CW * NT;
+  constexpr int NS = DUAL ? 2 : 1, STEP = 32 * NS, PD = DUAL ? 2 : 3, NA = CW * NT;
+  constexpr int LW = 8 * NS;  // k per lane per step
   __shared__ floatx4 red[3][NA][64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int lc = lane & 15, lg = lane >> 4;
@@ -499,7 +511,6 @@ __global__ __launch_bounds__(NTHR) void tn_h_kernel(StreamParams p) {
   const long kbeg = (long)blockIdx.y * p.kchunk;
   const long kend = min(p.K, kbeg + p.kchunk);
   const long nfull = kend > kbeg ? (kend - kbeg) / STEP : 0;
-  const long step0 = kbeg / STEP;
   const uint16_t *A = (const uint16_t *)p.A;
   const uint16_t *col[CW];
 #pragma unroll
@@ -510,47 +521,58 @@ __global__ __launch_bounds__(NTHR) void tn_h_kernel(StreamParams p) {
   floatx4 acc[NA];
 #pragma unroll
   for (int a = 0; a < NA; ++a) acc[a] = floatx4{0.f, 0.f, 0.f, 0.f};
-  auto mma = [&](BufH<NT, CW> &b) {
+  auto mma = [&](BufH<NT, CW, NS> &b) {
 #pragma unroll
-    for (int t = 0; t < NT; ++t)
+    for (int h = 0; h < NS; ++h)
 #pragma unroll
-      for (int cw = 0; cw < CW; ++cw) acc[cw * NT + t] = mfma16<TA>(b.a[t], b.x[cw], acc[cw * NT + t]);
+      for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int cw = 0; cw < CW; ++cw)
+          acc[cw * NT + t] = mfma16<TA>(b.a[t][h], b.x[cw][h], acc[cw * NT + t]);
   };
-  const uint4 *pk = p.thin16 + lane;
-  auto load = [&](long i, BufH<NT, CW> &b) {
+  auto load_thin = [&](long kb, BufH<NT, CW, NS> &b) {
+#pragma unroll
+    for (int h = 0; h < NS; ++h)
+#pragma unroll
+      for (int t = 0; t < NT; ++t) b.a[t][h] = p.thin16[pack_index(kb + 8 * h, t, lc, NT)];
+  };
+  auto load = [&](long i, BufH<NT, CW, NS> &b) {
     const long s = kph + i * p.wk;
-    const long kb = kbeg + s * STEP + 8 * lg;
+    const long kb = kbeg + s * STEP + LW * lg;
 #pragma unroll
-    for (int cw = 0; cw < CW; ++cw) {
-      if (!SAFE) {
-        b.x[cw] = *(const uint4 *)(col[cw] + kb);
-      } else {
-        uint16_t h[8];
+    for (int cw = 0; cw < CW; ++cw)
 #pragma unroll
-        for (int u = 0; u < 8; ++u) h[u] = col[cw][kb + u];
-        b.x[cw] = __builtin_bit_cast(uint4, h);
+      for (int h = 0; h < NS; ++h) {
+        if (!SAFE) {
+          b.x[cw][h] = *(const uint4 *)(col[cw] + kb + 8 * h);
+        } else {
+          uint16_t v[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) v[u] = col[cw][kb + 8 * h + u];
+          b.x[cw][h] = __builtin_bit_cast(uint4, v);
+        }
       }
-    }
-    const uint4 *src = pk + (step0 + s) * NT * 64;
-#pragma unroll
-    for (int t = 0; t < NT; ++t) b.a[t] = src[t * 64];
+    load_thin(kb, b);
   };
-  BufH<NT, CW> buf[PD];
+  BufH<NT, CW, NS> buf[PD];
   const long nsw = nfull > kph ? (nfull - kph + p.wk - 1) / p.wk : 0;
   stream_steps<PD>(nsw, buf, load, mma);
   if (kend - kbeg > nfull * STEP && nfull % p.wk == kph) {
-    BufH<NT, CW> b;
-    const long kb = kbeg + nfull * STEP + 8 * lg;
+    BufH<NT, CW, NS> b;
+    const long kb = kbeg + nfull * STEP + LW * lg;
 #pragma unroll
-    for (int cw = 0; cw < CW; ++cw) {
-      uint16_t h[8];
+    for (int cw = 0; cw < CW; ++cw)
 #pragma unroll
-      for (int u = 0; u < 8; ++u) h[u] = kb + u < kend ? col[cw][kb + u] : (uint16_t)0;
-      b.x[cw] = __builtin_bit_cast(uint4, h);
-    }
-    const uint4 *src = pk + (step0 + nfull) * NT * 64;
+      for (int h = 0; h < NS; ++h) {
+        uint16_t v[8];
 #pragma unroll
-    for (int t = 0; t < NT; ++t) b.a[t] = src[t * 64];
+        for (int u = 0; u < 8; ++u) {
+          const long kk = kb + 8 * h + u;
+          v[u] = kk < kend ? col[cw][kk] : (uint16_t)0;
+        }
+        b.x[cw][h] = __builtin_bit_cast(uint4, v);
+      }
+    load_thin(kb, b);
     mma(b);
   }
   if (finish_tile<NA>(p, acc, red)) epilogue_tn<NT, CW>(p, acc, l0);
@@ -826,13 +848,16 @@ __global__ void reduce_splits_kernel(const float *partial, int splits, long L, i
 // K splits across blocks.
 struct Geometry {
   int cw, wgrp, wk, nsplit;
+  int dual;  // 16-bit TN: 64-k steps with two 16-byte loads per lane and column
   long kchunk, blocks_x, groups;
   long tile_bytes;  // partial tile of one group
 };
 
-Geometry make_geometry(long L, long K, int trans, int dtype, int NT, int cw, int wk, long ns) {
+Geometry make_geometry(long L, long K, int trans, int dtype, int NT, int cw, int wk, long ns,
+                       int dual = 0) {
   Geometry g{};
-  const int step = dtype == PLANC_F32 ? 16 : 32;
+  g.dual = (trans && dtype != PLANC_F32) ? dual : 0;
+  const int step = dtype == PLANC_F32 ? 16 : (g.dual ? 64 : 32);
   g.cw = trans ? cw : 1;
   g.groups = trans ? (L + 16 * cw - 1) / (16 * cw) : (L + 63) / 64;
   g.tile_bytes = trans ? (long)cw * NT * 64 * 16 : 4L * NT * 64 * 16;
@@ -884,12 +909,14 @@ std::vector<Geometry> candidate_geometries(const planc_ctx *ctx, long L, long K,
     }
     out.push_back(g);
   }
+  const int ndual = (trans && dtype != PLANC_F32) ? 2 : 1;
+  for (int dual = 0; dual < ndual; ++dual)
   for (int cw : cws)
     for (int wk = 1; wk <= 4; wk *= 2) {
-      if (wk > 1 && ksteps < 8L * wk) break;
+      if (wk > 1 && ksteps < 8L * wk * (dual ? 2 : 1)) break;
       for (long ns = 1; ns <= 4096; ns *= 2) {
-        Geometry g = make_geometry(L, K, trans, dtype, NT, cw, wk, ns);
-        if (ns > 1 && (g.nsplit < ns || ksteps / ((long)g.nsplit * wk) < 8)) break;
+        Geometry g = make_geometry(L, K, trans, dtype, NT, cw, wk, ns, dual);
+        if (ns > 1 && (g.nsplit < ns || ksteps / ((long)g.nsplit * wk * (dual ? 2 : 1)) < 8)) break;
         if (ns > 1 && 2.0 * g.nsplit * g.groups * g.tile_bytes > 0.10 * abytes) break;
         const long waves = g.blocks_x * 4 * g.nsplit;
         if (ns > 1 && waves > 16L * 1024) break;
@@ -926,11 +953,13 @@ void launch_nn_f32(int tm, dim3 grid, hipStream_t st, const StreamParams &p) {
     nn_f32_kernel<NT, THIN_SAFE><<<grid, NTHR, 0, st>>>(p);
 }
 template <typename TA, int NT>
-void launch_tn_h(bool safe, dim3 grid, hipStream_t st, const StreamParams &p) {
+void launch_tn_h(bool safe, int dual, dim3 grid, hipStream_t st, const StreamParams &p) {
   if (safe)
-    tn_h_kernel<TA, NT, 4, 1><<<grid, NTHR, 0, st>>>(p);
+    tn_h_kernel<TA, NT, 4, 1, 0><<<grid, NTHR, 0, st>>>(p);
+  else if (dual)
+    tn_h_kernel<TA, NT, 4, 0, 1><<<grid, NTHR, 0, st>>>(p);
   else
-    tn_h_kernel<TA, NT, 4, 0><<<grid, NTHR, 0, st>>>(p);
+    tn_h_kernel<TA, NT, 4, 0, 0><<<grid, NTHR, 0, st>>>(p);
 }
 template <int NT, int CW>
 void launch_reduce_tn(dim3 g, hipStream_t st, const StreamParams &p, int NA, long ngroups) {
@@ -967,15 +996,15 @@ int launch_stream(planc_ctx *ctx, int dtype, int trans, int NT, int tm, StreamPa
   } else if (trans) {
     const bool safe = !p.avec;
     if (dtype == PLANC_BF16) {
-      if (NT == 1) launch_tn_h<bf16_t, 1>(safe, grid, st, p);
-      else if (NT == 2) launch_tn_h<bf16_t, 2>(safe, grid, st, p);
-      else if (NT == 3) launch_tn_h<bf16_t, 3>(safe, grid, st, p);
-      else launch_tn_h<bf16_t, 4>(safe, grid, st, p);
+      if (NT == 1) launch_tn_h<bf16_t, 1>(safe, g.dual, grid, st, p);
+      else if (NT == 2) launch_tn_h<bf16_t, 2>(safe, g.dual, grid, st, p);
+      else if (NT == 3) launch_tn_h<bf16_t, 3>(safe, g.dual, grid, st, p);
+      else launch_tn_h<bf16_t, 4>(safe, g.dual, grid, st, p);
     } else {
-      if (NT == 1) launch_tn_h<f16_t, 1>(safe, grid, st, p);
-      else if (NT == 2) launch_tn_h<f16_t, 2>(safe, grid, st, p);
-      else if (NT == 3) launch_tn_h<f16_t, 3>(safe, grid, st, p);
-      else launch_tn_h<f16_t, 4>(safe, grid, st, p);
+      if (NT == 1) launch_tn_h<f16_t, 1>(safe, g.dual, grid, st, p);
+      else if (NT == 2) launch_tn_h<f16_t, 2>(safe, g.dual, grid, st, p);
+      else if (NT == 3) launch_tn_h<f16_t, 3>(safe, g.dual, grid, st, p);
+      else launch_tn_h<f16_t, 4>(safe, g.dual, grid, st, p);
     }
   } else {
     if (NT == 1) launch_nn_f32<1>(tm, grid, st, p);
@@ -1025,7 +1054,7 @@ int pick_geometry(planc_ctx *ctx, int dtype, int trans, int NT, int tm, const St
   for (const auto &e : ctx->gemm_tuned)
     if (e.first == key) {
       const auto &v = e.second;
-      *out = make_geometry(p.L, p.K, trans, dtype, NT, (int)v[0], (int)v[1], v[2]);
+      *out = make_geometry(p.L, p.K, trans, dtype, NT, (int)v[0], (int)v[1], v[2], (int)v[3]);
       return PLANC_OK;
     }
   std::vector<Geometry> cands = candidate_geometries(ctx, p.L, p.K, trans, dtype, NT);
@@ -1060,7 +1089,7 @@ int pick_geometry(planc_ctx *ctx, int dtype, int trans, int NT, int tm, const St
     (void)hipEventDestroy(e1);
   }
   if (!stream_capturing(ctx->stream))
-    ctx->gemm_tuned.push_back({key, {best.cw, best.wk, (long)best.nsplit}});
+    ctx->gemm_tuned.push_back({key, {best.cw, best.wk, (long)best.nsplit, best.dual}});
   *out = best;
   return PLANC_OK;
 }
@@ -1199,15 +1228,15 @@ int thin_gemm(planc_ctx *ctx, int dtype, int trans, const void *A, long lda, lon
     if (!trans && L % 4 != 0) tm = THIN_SAFE;
     if (trans && dtype != PLANC_F32) {
       // 16-bit TN: the thin operand packed once into the MFMA A-operand layout.
-      const long ksteps = (K + 31) / 32;
-      uint4 *pk = (uint4 *)ws_get(ctx, 6, (size_t)ksteps * NT * 64 * 16);
+      const long nsuper = (K + 63) / 64;
+      uint4 *pk = (uint4 *)ws_get(ctx, 6, (size_t)nsuper * 2 * NT * 64 * 16);
       if (!pk) return set_error(ctx, PLANC_ENOMEM, "thin_gemm: packed operand");
-      const long tot = ksteps * NT * 64;
+      const long tot = nsuper * 2 * NT * 64;
       const unsigned nb = (unsigned)((tot + 255) / 256);
       if (dtype == PLANC_BF16)
-        pack_thin16_kernel<bf16_t><<<nb, 256, 0, ctx->stream>>>(th, K, NT, ksteps, pk);
+        pack_thin16_kernel<bf16_t><<<nb, 256, 0, ctx->stream>>>(th, K, NT, nsuper, pk);
       else
-        pack_thin16_kernel<f16_t><<<nb, 256, 0, ctx->stream>>>(th, K, NT, ksteps, pk);
+        pack_thin16_kernel<f16_t><<<nb, 256, 0, ctx->stream>>>(th, K, NT, nsuper, pk);
       PLANC_LAUNCH_CHECK(ctx);
       p.thin16 = pk;
     }
