@@ -26,66 +26,19 @@
 #include "common.h"
 #include "philox.h"
 
+// After common.h: hipCUB picks its backend from the platform macro that
+// hip_runtime.h defines.
+#include <hipcub/device/device_radix_sort.hpp>
+#include <hipcub/device/device_scan.hpp>
+
 namespace planc {
 
 namespace {
 
 // ---------------------------------------------------------------------------
-// Exclusive scan of int counts (n up to 2^31), three phases.
-constexpr int SCAN_B = 1024;
-
-__global__ void scan_block_sums(const int *in, long n, long *bsum) {
-  __shared__ long red[SCAN_B / 64];
-  long base = (long)blockIdx.x * SCAN_B;
-  long s = 0;
-  for (int t = threadIdx.x; t < SCAN_B; t += blockDim.x) {
-    long i = base + t;
-    if (i < n) s += in[i];
-  }
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    long t = 0;
-    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w];
-    bsum[blockIdx.x] = t;
-  }
-}
-
-__global__ void scan_sums_serial(long *bsum, long nb, long *total) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  long run = 0;
-  for (long b = 0; b < nb; ++b) {
-    long v = bsum[b];
-    bsum[b] = run;
-    run += v;
-  }
-  *total = run;
-}
-
-// One thread per element group: each block rescans its 1024 elements.
-__global__ void scan_apply(const int *in, long n, const long *bsum, long *out) {
-  __shared__ long vals[SCAN_B];
-  long base = (long)blockIdx.x * SCAN_B;
-  for (int t = threadIdx.x; t < SCAN_B; t += blockDim.x) {
-    long i = base + t;
-    vals[t] = (i < n) ? in[i] : 0;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    long run = bsum[blockIdx.x];
-    for (int t = 0; t < SCAN_B; ++t) {
-      long v = vals[t];
-      vals[t] = run;
-      run += v;
-    }
-  }
-  __syncthreads();
-  for (int t = threadIdx.x; t < SCAN_B; t += blockDim.x) {
-    long i = base + t;
-    if (i < n) out[i] = vals[t];
-  }
+// Total of an exclusive scan: out[n-1] + in[n-1].
+__global__ void scan_total(const int *in, const long *out, long n, long *total) {
+  if (threadIdx.x == 0) *total = out[n - 1] + in[n - 1];
 }
 
 // ---------------------------------------------------------------------------
@@ -97,65 +50,77 @@ __global__ void level_col_len(const long *perm, long nlev, const long *cptr0, in
   len[lc] = (int)(cptr0[c + 1] - cptr0[c]);
 }
 
-// count[t*m + i] = entries of row i in node t's columns (one thread per row).
-__global__ void level_row_count(long m, const long *rptr0, const int *cidx0, const int *colmap,
-                                const int *node_of_col, int *count) {
-  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m) return;
-  for (long e = rptr0[i]; e < rptr0[i + 1]; ++e) {
-    int lc = colmap[cidx0[e]];
-    if (lc < 0) continue;
-    long t = node_of_col[lc];
-    count[t * m + i] += 1;
+// Row of every CSR entry (binary search of indptr, one thread per entry).
+__global__ void entry_rows(long m, long nnz, const long *rptr0, int *row0) {
+  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nnz) return;
+  long lo = 0, hi = m - 1;  // largest i with rptr0[i] <= e
+  while (lo < hi) {
+    long mid = (lo + hi + 1) >> 1;
+    if (rptr0[mid] <= e) lo = mid;
+    else hi = mid - 1;
+  }
+  row0[e] = (int)lo;
+}
+
+// Sort key of every entry: its node at this level, nn for entries whose
+// column belongs to no node (they sort last and are dropped).
+__global__ void level_keys(long nnz, const int *cidx0, const int *colmap, const int *node_of_col,
+                           unsigned nn, unsigned *key, int *ent) {
+  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nnz) return;
+  int lc = colmap[cidx0[e]];
+  key[e] = lc < 0 ? nn : (unsigned)node_of_col[lc];
+  ent[e] = (int)e;
+}
+
+// After the stable sort by node, entries are ordered by (node, row, original
+// position): position p starts a block-CSR row when its (node, row) differs
+// from position p-1.  Also writes the level CSR columns and values.
+__global__ void level_run_flags(long lnnz, const unsigned *key, const int *ent, const int *row0,
+                                const int *cidx0, const float *val0, const int *colmap, int *flag,
+                                int *csr_col, float *csr_val) {
+  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= lnnz) return;
+  int e = ent[p];
+  int f = 1;
+  if (p > 0) {
+    int e1 = ent[p - 1];
+    f = key[p] != key[p - 1] || row0[e] != row0[e1];
+  }
+  flag[p] = f;
+  csr_col[p] = colmap[cidx0[e]];
+  csr_val[p] = val0[e];
+}
+
+// Compacted row of every level entry, and each row's start, node and
+// original row.
+__global__ void level_run_rows(long lnnz, const unsigned *key, const int *ent, const int *row0,
+                               const int *flag, const long *rstart, int *rid_of_entry, long *row_ptr,
+                               int *row_node, int *row_orig) {
+  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= lnnz) return;
+  int e = ent[p];
+  long r = rstart[p] + flag[p] - 1;
+  rid_of_entry[e] = (int)r;
+  if (flag[p]) {
+    row_ptr[r] = p;
+    row_node[r] = (int)key[p];
+    row_orig[r] = row0[e];
   }
 }
 
-__global__ void flag_nonzero(const int *count, long n, int *flag) {
-  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e < n) flag[e] = count[e] > 0;
-}
-
-// For each nonempty (t, i): compacted row id from the flag scan.
-__global__ void level_rows(const int *count, long nm, long m, const long *rowid, int *row_node,
-                           int *row_orig, int *row_len) {
-  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= nm || count[e] == 0) return;
-  long r = rowid[e];
-  row_node[r] = (int)(e / m);
-  row_orig[r] = (int)(e % m);
-  row_len[r] = count[e];
-}
-
-// Block CSR fill, one thread per original row (entry order preserved).
-__global__ void level_csr_fill(long m, const long *rptr0, const int *cidx0, const float *val0,
-                               const int *colmap, const int *node_of_col, const long *rowid,
-                               const long *row_ptr, int *cursor, int *csr_col, float *csr_val) {
-  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m) return;
-  for (long e = rptr0[i]; e < rptr0[i + 1]; ++e) {
-    int lc = colmap[cidx0[e]];
-    if (lc < 0) continue;
-    long t = node_of_col[lc];
-    long r = rowid[t * m + i];
-    long pos = row_ptr[r] + cursor[r];
-    cursor[r] += 1;
-    csr_col[pos] = lc;
-    csr_val[pos] = val0[e];
-  }
-}
-
-// Level CSC fill, one thread per level column.
-__global__ void level_csc_fill(const long *perm, long nlev, long m, const long *cptr0,
-                               const int *ridx0, const float *cval0, const int *node_of_col,
-                               const long *rowid, const long *col_ptr, int *csc_row,
-                               float *csc_val) {
+// Level CSC fill, one thread per level column; csc2csr maps a CSC entry of A
+// to its CSR entry.
+__global__ void level_csc_fill(const long *perm, long nlev, const long *cptr0, const int *csc2csr,
+                               const float *cval0, const int *rid_of_entry, const long *col_ptr,
+                               int *csc_row, float *csc_val) {
   long lc = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (lc >= nlev) return;
   long c = perm[lc];
-  long t = node_of_col[lc];
   long o = col_ptr[lc];
   for (long e = cptr0[c]; e < cptr0[c + 1]; ++e) {
-    csc_row[o] = (int)rowid[t * m + ridx0[e]];
+    csc_row[o] = rid_of_entry[csc2csr[e]];
     csc_val[o] = cval0[e];
     ++o;
   }
@@ -308,17 +273,56 @@ __global__ void init_factors(long nrows, const int *row_node, const int *row_ori
 
 // H update: WtA(:, lc) over the column's entries, then the R2 rule with the
 // node's W^T W.
+// Sums over the entries b, b+S, b+2S, ... < e1 of val * X[idx], taken in
+// that order; four entries' index/value loads are issued before their
+// gathers so each step waits on one memory round trip, not four.
+__device__ inline void dot2_range(long b, long e1, long S, const int *__restrict__ idx,
+                                  const float *__restrict__ val, const double2 *__restrict__ X, double &s0,
+                                  double &s1) {
+  long e = b;
+  for (; e + 3 * S < e1; e += 4 * S) {
+    int i0 = idx[e], i1 = idx[e + S], i2 = idx[e + 2 * S], i3 = idx[e + 3 * S];
+    double v0 = val[e], v1 = val[e + S], v2 = val[e + 2 * S], v3 = val[e + 3 * S];
+    double2 x0 = X[i0], x1 = X[i1], x2 = X[i2], x3 = X[i3];
+    s0 += v0 * x0.x;
+    s1 += v0 * x0.y;
+    s0 += v1 * x1.x;
+    s1 += v1 * x1.y;
+    s0 += v2 * x2.x;
+    s1 += v2 * x2.y;
+    s0 += v3 * x3.x;
+    s1 += v3 * x3.y;
+  }
+  for (; e < e1; e += S) {
+    double v = val[e];
+    double2 x = X[idx[e]];
+    s0 += v * x.x;
+    s1 += v * x.y;
+  }
+}
+__device__ inline double dot1_range(long b, long e1, long S, const int *__restrict__ idx,
+                                    const float *__restrict__ val, const double *__restrict__ X) {
+  double s = 0;
+  long e = b;
+  for (; e + 3 * S < e1; e += 4 * S) {
+    int i0 = idx[e], i1 = idx[e + S], i2 = idx[e + 2 * S], i3 = idx[e + 3 * S];
+    double v0 = val[e], v1 = val[e + S], v2 = val[e + 2 * S], v3 = val[e + 3 * S];
+    double x0 = X[i0], x1 = X[i1], x2 = X[i2], x3 = X[i3];
+    s += v0 * x0;
+    s += v1 * x1;
+    s += v2 * x2;
+    s += v3 * x3;
+  }
+  for (; e < e1; e += S) s += (double)val[e] * X[idx[e]];
+  return s;
+}
+
 __global__ void h_update(long nlev, const long *col_ptr, const int *csc_row, const float *csc_val,
                          const int *node_of_col, const double *W, const double *Gw, double *H) {
   long lc = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (lc >= nlev) return;
   double s0 = 0, s1 = 0;
-  for (long e = col_ptr[lc]; e < col_ptr[lc + 1]; ++e) {
-    double v = csc_val[e];
-    int r = csc_row[e];
-    s0 += W[2 * r] * v;
-    s1 += W[2 * r + 1] * v;
-  }
+  dot2_range(col_ptr[lc], col_ptr[lc + 1], 1, csc_row, csc_val, (const double2 *)W, s0, s1);
   double x0, x1;
   r2_solve(Gw + 4 * node_of_col[lc], s0, s1, x0, x1);
   H[2 * lc] = x0;
@@ -326,9 +330,9 @@ __global__ void h_update(long nlev, const long *col_ptr, const int *csc_row, con
 }
 
 // W update: (A H)(r, :) over the compacted row's entries, then the R2 rule
-// with the node's H^T H.  G lanes per row (a sub-wave group, G in
-// {1, 4, 16}, picked per level by timing); rows longer than `longlen` are
-// left to w_update_long (one block per row) -- the power-law row
+// with the node's H^T H.  G lanes per row (a sub-wave group or a whole
+// wave, G in {4, 16, 64}, picked per level by timing); rows longer than `longlen` are
+// left to the segmented long-row kernels -- the power-law row
 // popularity of term-document matrices puts a few very long rows next to
 // millions of short ones.
 template <int G>
@@ -348,12 +352,7 @@ __global__ void w_update_short(long nrows, const long *row_ptr, const int *csr_c
   long b = row_ptr[g], e1 = row_ptr[g + 1];
   if (e1 - b > longlen) return;
   double s0 = 0, s1 = 0;
-  for (long e = b + lane; e < e1; e += G) {
-    double v = csr_val[e];
-    int lc = csr_col[e];
-    s0 += v * H[2 * lc];
-    s1 += v * H[2 * lc + 1];
-  }
+  dot2_range(b + lane, e1, G, csr_col, csr_val, (const double2 *)H, s0, s1);
   s0 = group_sum<G>(s0);
   s1 = group_sum<G>(s1);
   if (lane == 0) {
@@ -373,26 +372,40 @@ __device__ inline double block_sum256(double v, double *red) {
   return red[0] + red[1] + red[2] + red[3];
 }
 
-__global__ void w_update_long(const int *list, const long *row_ptr, const int *csr_col,
-                              const float *csr_val, const int *row_node, const double *H,
-                              const double *Gh, double *W) {
+// Long rows are cut into segments of at most SEG_LEN entries (a block
+// each, partial sums to seg_part); the per-row finish adds a row's segment
+// partials in segment order, so the sums are taken in a fixed order.
+constexpr long SEG_LEN = 4096;
+
+__global__ void w_update_seg(const int *seg_row, const long *seg_b, const int *list, const long *row_ptr,
+                             const int *csr_col, const float *csr_val, const double *H, double *seg_part) {
   __shared__ double red[4];
-  long r = list[blockIdx.x];
+  long r = list[seg_row[blockIdx.x]];
+  long b = seg_b[blockIdx.x];
+  long e1 = min(b + SEG_LEN, row_ptr[r + 1]);
   double s0 = 0, s1 = 0;
-  for (long e = row_ptr[r] + threadIdx.x; e < row_ptr[r + 1]; e += blockDim.x) {
-    double v = csr_val[e];
-    int lc = csr_col[e];
-    s0 += v * H[2 * lc];
-    s1 += v * H[2 * lc + 1];
-  }
+  dot2_range(b + threadIdx.x, e1, blockDim.x, csr_col, csr_val, (const double2 *)H, s0, s1);
   s0 = block_sum256(s0, red);
   s1 = block_sum256(s1, red);
   if (threadIdx.x == 0) {
-    double x0, x1;
-    r2_solve(Gh + 4 * row_node[r], s0, s1, x0, x1);
-    W[2 * r] = x0;
-    W[2 * r + 1] = x1;
+    seg_part[2 * blockIdx.x] = s0;
+    seg_part[2 * blockIdx.x + 1] = s1;
   }
+}
+__global__ void w_update_long_fin(long nlong, const int *list, const long *seg_off, const double *seg_part,
+                                  const int *row_node, const double *Gh, double *W) {
+  long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nlong) return;
+  long r = list[j];
+  double s0 = 0, s1 = 0;
+  for (long q = seg_off[j]; q < seg_off[j + 1]; ++q) {
+    s0 += seg_part[2 * q];
+    s1 += seg_part[2 * q + 1];
+  }
+  double x0, x1;
+  r2_solve(Gh + 4 * row_node[r], s0, s1, x0, x1);
+  W[2 * r] = x0;
+  W[2 * r + 1] = x1;
 }
 
 __global__ void flag_long(const long *row_ptr, long nrows, long longlen, int *flag) {
@@ -403,6 +416,25 @@ __global__ void flag_long(const long *row_ptr, long nrows, long longlen, int *fl
 __global__ void compact_list(const int *flag, const long *pos, long n, int *list) {
   long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (r < n && flag[r]) list[pos[r]] = (int)r;
+}
+
+__global__ void long_seg_count(const int *list, const long *row_ptr, long nlong, int *cnt) {
+  long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nlong) return;
+  long r = list[j];
+  cnt[j] = (int)((row_ptr[r + 1] - row_ptr[r] + SEG_LEN - 1) / SEG_LEN);
+}
+
+__global__ void long_seg_fill(const int *list, const long *row_ptr, long nlong, const long *seg_off,
+                              int *seg_row, long *seg_b) {
+  long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nlong) return;
+  long r = list[j];
+  long q = seg_off[j];
+  for (long b = row_ptr[r]; b < row_ptr[r + 1]; b += SEG_LEN, ++q) {
+    seg_row[q] = (int)j;
+    seg_b[q] = b;
+  }
 }
 
 // Power iteration on A_t^T A_t (hiernmf/matutils.hpp:37-120): z = A q / s,
@@ -424,31 +456,36 @@ __global__ void pow_rows_short(long nrows, const long *row_ptr, const int *csr_c
   if (g >= nrows) return;
   long b = row_ptr[g], e1 = row_ptr[g + 1];
   if (e1 - b > longlen) return;
-  double s = 0;
-  for (long e = b + lane; e < e1; e += G) s += (double)csr_val[e] * q[csr_col[e]];
+  double s = dot1_range(b + lane, e1, G, csr_col, csr_val, q);
   s = group_sum<G>(s);
   if (lane == 0) z[g] = s / sig[row_node[g]];
 }
 
-__global__ void pow_rows_long(const int *list, const long *row_ptr, const int *csr_col,
-                              const float *csr_val, const int *row_node, const double *q,
-                              const double *sig, double *z) {
+__global__ void pow_rows_seg(const int *seg_row, const long *seg_b, const int *list, const long *row_ptr,
+                             const int *csr_col, const float *csr_val, const double *q, double *seg_part) {
   __shared__ double red[4];
-  long r = list[blockIdx.x];
-  double s = 0;
-  for (long e = row_ptr[r] + threadIdx.x; e < row_ptr[r + 1]; e += blockDim.x)
-    s += (double)csr_val[e] * q[csr_col[e]];
+  long r = list[seg_row[blockIdx.x]];
+  long b = seg_b[blockIdx.x];
+  long e1 = min(b + SEG_LEN, row_ptr[r + 1]);
+  double s = dot1_range(b + threadIdx.x, e1, blockDim.x, csr_col, csr_val, q);
   s = block_sum256(s, red);
-  if (threadIdx.x == 0) z[r] = s / sig[row_node[r]];
+  if (threadIdx.x == 0) seg_part[blockIdx.x] = s;
+}
+__global__ void pow_rows_long_fin(long nlong, const int *list, const long *seg_off, const double *seg_part,
+                                  const int *row_node, const double *sig, double *z) {
+  long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nlong) return;
+  long r = list[j];
+  double s = 0;
+  for (long q = seg_off[j]; q < seg_off[j + 1]; ++q) s += seg_part[q];
+  z[r] = s / sig[row_node[r]];
 }
 
 __global__ void pow_cols(long nlev, const long *col_ptr, const int *csc_row, const float *csc_val,
                          const double *z, double *q) {
   long lc = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (lc >= nlev) return;
-  double s = 0;
-  for (long e = col_ptr[lc]; e < col_ptr[lc + 1]; ++e) s += (double)csc_val[e] * z[csc_row[e]];
-  q[lc] = s;
+  q[lc] = dot1_range(col_ptr[lc], col_ptr[lc + 1], 1, csc_row, csc_val, z);
 }
 
 __global__ void sumsq_chunks(const Chunk *ch, const double *x, double *part) {
@@ -550,9 +587,13 @@ struct Engine {
   DBuf<int> cidx0, ridx0;
   DBuf<float> val0, cval0;
   // level
-  DBuf<long> perm, col_ptr, row_ptr, rowid, bsum, node_col0, scratch_l;
-  DBuf<int> colmap, node_of_col, count, flag, row_node, row_orig, row_len, col_len, cursor, csr_col,
-      csc_row, cb_rows, cb_cols, cb_init, lflag, llist;
+  DBuf<long> perm, col_ptr, row_ptr, rstart, bsum, node_col0;
+  DBuf<int> colmap, node_of_col, flag, row_node, row_orig, row_len, col_len, csr_col, csc_row,
+      cb_rows, cb_cols, cb_init, lflag, llist, row0, csc2csr, ent, ent_s, rid_of_entry, lcnt, seg_row;
+  DBuf<long> seg_off, seg_b;
+  DBuf<double> seg_part;
+  DBuf<unsigned> key, key_s;
+  DBuf<unsigned char> temp;
   DBuf<long> lpos;
   DBuf<double> Wscratch;
   DBuf<float> csr_val, csc_val;
@@ -563,15 +604,17 @@ struct Engine {
 
   ~Engine() {
     rptr0.release(); cptr0.release(); cidx0.release(); ridx0.release(); val0.release();
-    cval0.release(); perm.release(); col_ptr.release(); row_ptr.release(); rowid.release();
-    bsum.release(); node_col0.release(); scratch_l.release(); colmap.release();
-    node_of_col.release(); count.release(); flag.release(); row_node.release();
-    row_orig.release(); row_len.release(); col_len.release(); cursor.release();
-    csr_col.release(); csc_row.release(); cb_rows.release(); cb_cols.release();
-    cb_init.release(); csr_val.release(); csc_val.release(); W.release(); H.release();
-    Gw.release(); Gh.release(); part.release(); q.release(); z.release(); sig.release();
-    seeds.release(); ch_rows.release(); ch_cols.release(); ch_init.release(); mask.release();
-    lflag.release(); llist.release(); lpos.release(); Wscratch.release();
+    cval0.release(); perm.release(); col_ptr.release(); row_ptr.release(); rstart.release();
+    bsum.release(); node_col0.release(); colmap.release(); node_of_col.release();
+    flag.release(); row_node.release(); row_orig.release(); row_len.release();
+    col_len.release(); csr_col.release(); csc_row.release(); cb_rows.release();
+    cb_cols.release(); cb_init.release(); lflag.release(); llist.release(); row0.release();
+    csc2csr.release(); ent.release(); ent_s.release(); rid_of_entry.release(); key.release();
+    key_s.release(); temp.release(); csr_val.release(); csc_val.release(); W.release();
+    H.release(); Gw.release(); Gh.release(); part.release(); q.release(); z.release();
+    sig.release(); seeds.release(); ch_rows.release(); ch_cols.release(); ch_init.release();
+    mask.release(); lpos.release(); Wscratch.release(); lcnt.release(); seg_row.release();
+    seg_off.release(); seg_b.release(); seg_part.release();
   }
 
   // Launches of the row kernels with G lanes per row.
@@ -579,8 +622,8 @@ struct Engine {
     long thr = nrows * G;
     unsigned nbk = (unsigned)((thr + 255) / 256);
     if (nbk == 0) return;
-    if (G == 1)
-      w_update_short<1><<<nbk, 256, 0, ctx->stream>>>(nrows, row_ptr.p, csr_col.p, csr_val.p, row_node.p, H.p, Gh.p, Wout, longlen);
+    if (G == 64)
+      w_update_short<64><<<nbk, 256, 0, ctx->stream>>>(nrows, row_ptr.p, csr_col.p, csr_val.p, row_node.p, H.p, Gh.p, Wout, longlen);
     else if (G == 4)
       w_update_short<4><<<nbk, 256, 0, ctx->stream>>>(nrows, row_ptr.p, csr_col.p, csr_val.p, row_node.p, H.p, Gh.p, Wout, longlen);
     else
@@ -590,26 +633,26 @@ struct Engine {
     long thr = nrows * G;
     unsigned nbk = (unsigned)((thr + 255) / 256);
     if (nbk == 0) return;
-    if (G == 1)
-      pow_rows_short<1><<<nbk, 256, 0, ctx->stream>>>(nrows, row_ptr.p, csr_col.p, csr_val.p, row_node.p, q.p, sig.p, z.p, longlen);
+    if (G == 64)
+      pow_rows_short<64><<<nbk, 256, 0, ctx->stream>>>(nrows, row_ptr.p, csr_col.p, csr_val.p, row_node.p, q.p, sig.p, z.p, longlen);
     else if (G == 4)
       pow_rows_short<4><<<nbk, 256, 0, ctx->stream>>>(nrows, row_ptr.p, csr_col.p, csr_val.p, row_node.p, q.p, sig.p, z.p, longlen);
     else
       pow_rows_short<16><<<nbk, 256, 0, ctx->stream>>>(nrows, row_ptr.p, csr_col.p, csr_val.p, row_node.p, q.p, sig.p, z.p, longlen);
   }
-  // Lanes per row for this level: the fastest of {1, 4, 16} over a few
+  // Lanes per row for this level: the fastest of {4, 16, 64} over a few
   // timed launches of the W update into a scratch buffer.
   int tune_group(long nrows, long longlen) {
-    if (nrows == 0) return 1;
-    if (Wscratch.get(ctx, 2 * nrows)) return 1;
-    const int cand[3] = {1, 4, 16};
+    if (nrows == 0) return 16;
+    if (Wscratch.get(ctx, 2 * nrows)) return 16;
+    const int cand[3] = {4, 16, 64};
     float best = 1e30f;
-    int bestg = 1;
+    int bestg = 16;
     hipEvent_t e0, e1;
-    if (hipEventCreate(&e0) != hipSuccess) return 1;
+    if (hipEventCreate(&e0) != hipSuccess) return 16;
     if (hipEventCreate(&e1) != hipSuccess) {
       (void)hipEventDestroy(e0);
-      return 1;
+      return 16;
     }
     for (int c : cand) {
       launch_w_short(c, nrows, Wscratch.p, longlen);
@@ -629,21 +672,35 @@ struct Engine {
     return bestg;
   }
 
-  // out[i] = sum_{j < i} in[j]; returns the total (synchronises).
+  // out[i] = sum_{j < i} in[j] (64-bit sums); returns the total (synchronises).
   int scan(const int *in, long cnt, long *out, long *total) {
-    long nbk = (cnt + SCAN_B - 1) / SCAN_B;
-    if (nbk == 0) {
+    if (cnt <= 0) {
       *total = 0;
       return PLANC_OK;
     }
-    int rc = bsum.get(ctx, nbk + 1);
+    size_t bytes = 0;
+    PLANC_HIP_CHECK(ctx, hipcub::DeviceScan::ExclusiveScan(nullptr, bytes, in, out, hipcub::Sum(), 0L, cnt,
+                                                           ctx->stream));
+    int rc = temp.get(ctx, bytes);
     if (rc) return rc;
-    scan_block_sums<<<(unsigned)nbk, 256, 0, ctx->stream>>>(in, cnt, bsum.p);
-    scan_sums_serial<<<1, 64, 0, ctx->stream>>>(bsum.p, nbk, bsum.p + nbk);
-    scan_apply<<<(unsigned)nbk, 256, 0, ctx->stream>>>(in, cnt, bsum.p, out);
+    if ((rc = bsum.get(ctx, 1))) return rc;
+    PLANC_HIP_CHECK(ctx, hipcub::DeviceScan::ExclusiveScan(temp.p, bytes, in, out, hipcub::Sum(), 0L, cnt,
+                                                           ctx->stream));
+    scan_total<<<1, 64, 0, ctx->stream>>>(in, out, cnt, bsum.p);
     PLANC_LAUNCH_CHECK(ctx);
-    PLANC_HIP_CHECK(ctx, hipMemcpyAsync(total, bsum.p + nbk, sizeof(long), hipMemcpyDeviceToHost, ctx->stream));
+    PLANC_HIP_CHECK(ctx, hipMemcpyAsync(total, bsum.p, sizeof(long), hipMemcpyDeviceToHost, ctx->stream));
     PLANC_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return PLANC_OK;
+  }
+  // Stable sort of (key, ent) pairs by the low `bits` bits of key.
+  int sort_pairs(long cnt, int bits) {
+    size_t bytes = 0;
+    PLANC_HIP_CHECK(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, key.p, key_s.p, ent.p, ent_s.p,
+                                                            cnt, 0, bits, ctx->stream));
+    int rc = temp.get(ctx, bytes);
+    if (rc) return rc;
+    PLANC_HIP_CHECK(ctx, hipcub::DeviceRadixSort::SortPairs(temp.p, bytes, key.p, key_s.p, ent.p, ent_s.p,
+                                                            cnt, 0, bits, ctx->stream));
     return PLANC_OK;
   }
 };
@@ -699,7 +756,7 @@ int planc_hiernmf_run(planc_ctx *ctx, const planc_hier_desc *desc, const long *h
   std::vector<long> cptr(n + 1, 0);
   for (long e = 0; e < nnz; ++e) cptr[h_indices[e] + 1]++;
   for (long j = 0; j < n; ++j) cptr[j + 1] += cptr[j];
-  std::vector<int> ridx(nnz);
+  std::vector<int> ridx(nnz), c2r(nnz);
   std::vector<float> cval(nnz);
   {
     std::vector<long> cur(cptr.begin(), cptr.end() - 1);
@@ -708,6 +765,7 @@ int planc_hiernmf_run(planc_ctx *ctx, const planc_hier_desc *desc, const long *h
         long p = cur[h_indices[e]]++;
         ridx[p] = (int)i;
         cval[p] = h_data[e];
+        c2r[p] = (int)e;
       }
   }
   RC(E.rptr0.get(ctx, m + 1));
@@ -716,6 +774,15 @@ int planc_hiernmf_run(planc_ctx *ctx, const planc_hier_desc *desc, const long *h
   RC(E.ridx0.get(ctx, nnz));
   RC(E.val0.get(ctx, nnz));
   RC(E.cval0.get(ctx, nnz));
+  RC(E.csc2csr.get(ctx, nnz));
+  RC(E.row0.get(ctx, nnz));
+  RC(E.key.get(ctx, nnz));
+  RC(E.key_s.get(ctx, nnz));
+  RC(E.ent.get(ctx, nnz));
+  RC(E.ent_s.get(ctx, nnz));
+  RC(E.rid_of_entry.get(ctx, nnz));
+  RC(E.flag.get(ctx, nnz));
+  RC(E.rstart.get(ctx, nnz));
   auto up = [&](void *d, const void *h, size_t b) {
     return hipMemcpyAsync(d, h, b, hipMemcpyHostToDevice, ctx->stream);
   };
@@ -726,6 +793,9 @@ int planc_hiernmf_run(planc_ctx *ctx, const planc_hier_desc *desc, const long *h
     PLANC_HIP_CHECK(ctx, up(E.ridx0.p, ridx.data(), sizeof(int) * nnz));
     PLANC_HIP_CHECK(ctx, up(E.val0.p, h_data, sizeof(float) * nnz));
     PLANC_HIP_CHECK(ctx, up(E.cval0.p, cval.data(), sizeof(float) * nnz));
+    PLANC_HIP_CHECK(ctx, up(E.csc2csr.p, c2r.data(), sizeof(int) * nnz));
+    entry_rows<<<nb(nnz), 256, 0, ctx->stream>>>(m, nnz, E.rptr0.p, E.row0.p);
+    PLANC_LAUNCH_CHECK(ctx);
   }
 
   // Tree state.
@@ -783,41 +853,41 @@ int planc_hiernmf_run(planc_ctx *ctx, const planc_hier_desc *desc, const long *h
     long lnnz = 0;
     RC(E.scan(E.col_len.p, nlev, E.col_ptr.p, &lnnz));
     PLANC_HIP_CHECK(ctx, up(E.col_ptr.p + nlev, &lnnz, sizeof(long)));
-    // Compacted rows.
-    const long nm = (long)nn * m;
-    RC(E.count.get(ctx, nm));
-    RC(E.flag.get(ctx, nm));
-    RC(E.rowid.get(ctx, nm));
-    PLANC_HIP_CHECK(ctx, hipMemsetAsync(E.count.p, 0, sizeof(int) * nm, ctx->stream));
-    level_row_count<<<nb(m), 256, 0, ctx->stream>>>(m, E.rptr0.p, E.cidx0.p, E.colmap.p,
-                                                   E.node_of_col.p, E.count.p);
-    flag_nonzero<<<nb(nm), 256, 0, ctx->stream>>>(E.count.p, nm, E.flag.p);
+    // Block-CSR rows: a stable sort of the entries by node keeps (row,
+    // original position) order inside each node; runs of equal (node, row)
+    // are the compacted rows, numbered node-major.
     long nrows = 0;
-    RC(E.scan(E.flag.p, nm, E.rowid.p, &nrows));
-    RC(E.row_node.get(ctx, nrows));
-    RC(E.row_orig.get(ctx, nrows));
-    RC(E.row_len.get(ctx, nrows));
-    RC(E.row_ptr.get(ctx, nrows + 1));
-    level_rows<<<nb(nm), 256, 0, ctx->stream>>>(E.count.p, nm, m, E.rowid.p, E.row_node.p,
-                                               E.row_orig.p, E.row_len.p);
-    long lnnz2 = 0;
-    RC(E.scan(E.row_len.p, nrows, E.row_ptr.p, &lnnz2));
-    PLANC_HIP_CHECK(ctx, up(E.row_ptr.p + nrows, &lnnz2, sizeof(long)));
-    if (lnnz2 != lnnz) return set_error(ctx, PLANC_EINVAL, "hiernmf: level CSR/CSC mismatch");
-    RC(E.cursor.get(ctx, nrows));
     RC(E.csr_col.get(ctx, lnnz));
     RC(E.csr_val.get(ctx, lnnz));
     RC(E.csc_row.get(ctx, lnnz));
     RC(E.csc_val.get(ctx, lnnz));
-    PLANC_HIP_CHECK(ctx, hipMemsetAsync(E.cursor.p, 0, sizeof(int) * (nrows ? nrows : 1), ctx->stream));
-    level_csr_fill<<<nb(m), 256, 0, ctx->stream>>>(m, E.rptr0.p, E.cidx0.p, E.val0.p, E.colmap.p,
-                                                  E.node_of_col.p, E.rowid.p, E.row_ptr.p,
-                                                  E.cursor.p, E.csr_col.p, E.csr_val.p);
-    level_csc_fill<<<nb(nlev), 256, 0, ctx->stream>>>(E.perm.p, nlev, m, E.cptr0.p, E.ridx0.p,
-                                                     E.cval0.p, E.node_of_col.p, E.rowid.p,
-                                                     E.col_ptr.p, E.csc_row.p, E.csc_val.p);
+    if (lnnz > 0) {
+      int bits = 1;
+      while (bits < 31 && (1L << bits) <= (long)nn) ++bits;
+      level_keys<<<nb(nnz), 256, 0, ctx->stream>>>(nnz, E.cidx0.p, E.colmap.p, E.node_of_col.p,
+                                                   (unsigned)nn, E.key.p, E.ent.p);
+      PLANC_LAUNCH_CHECK(ctx);
+      RC(E.sort_pairs(nnz, bits));
+      level_run_flags<<<nb(lnnz), 256, 0, ctx->stream>>>(lnnz, E.key_s.p, E.ent_s.p, E.row0.p, E.cidx0.p,
+                                                          E.val0.p, E.colmap.p, E.flag.p, E.csr_col.p,
+                                                          E.csr_val.p);
+      PLANC_LAUNCH_CHECK(ctx);
+      RC(E.scan(E.flag.p, lnnz, E.rstart.p, &nrows));
+    }
+    RC(E.row_node.get(ctx, nrows));
+    RC(E.row_orig.get(ctx, nrows));
+    RC(E.row_ptr.get(ctx, nrows + 1));
+    if (lnnz > 0)
+      level_run_rows<<<nb(lnnz), 256, 0, ctx->stream>>>(lnnz, E.key_s.p, E.ent_s.p, E.row0.p, E.flag.p,
+                                                         E.rstart.p, E.rid_of_entry.p, E.row_ptr.p,
+                                                         E.row_node.p, E.row_orig.p);
+    PLANC_HIP_CHECK(ctx, up(E.row_ptr.p + nrows, &lnnz, sizeof(long)));
+    level_csc_fill<<<nb(nlev), 256, 0, ctx->stream>>>(E.perm.p, nlev, E.cptr0.p, E.csc2csr.p, E.cval0.p,
+                                                     E.rid_of_entry.p, E.col_ptr.p, E.csc_row.p,
+                                                     E.csc_val.p);
     PLANC_LAUNCH_CHECK(ctx);
-    // Rows longer than LONGLEN get a block each (list built on the device).
+    // Rows longer than LONGLEN are cut into SEG_LEN segments, a block each
+    // (lists built on the device).
     const long LONGLEN = 2048;
     RC(E.lflag.get(ctx, nrows ? nrows : 1));
     RC(E.lpos.get(ctx, nrows ? nrows : 1));
@@ -827,7 +897,22 @@ int planc_hiernmf_run(planc_ctx *ctx, const planc_hier_desc *desc, const long *h
       RC(E.scan(E.lflag.p, nrows, E.lpos.p, &nlong));
     }
     RC(E.llist.get(ctx, nlong ? nlong : 1));
-    if (nlong) compact_list<<<nb(nrows), 256, 0, ctx->stream>>>(E.lflag.p, E.lpos.p, nrows, E.llist.p);
+    RC(E.lcnt.get(ctx, nlong + 1));
+    RC(E.seg_off.get(ctx, nlong + 1));
+    long nseg = 0;
+    if (nlong) {
+      compact_list<<<nb(nrows), 256, 0, ctx->stream>>>(E.lflag.p, E.lpos.p, nrows, E.llist.p);
+      long_seg_count<<<nb(nlong), 256, 0, ctx->stream>>>(E.llist.p, E.row_ptr.p, nlong, E.lcnt.p);
+      PLANC_LAUNCH_CHECK(ctx);
+      RC(E.scan(E.lcnt.p, nlong, E.seg_off.p, &nseg));
+      PLANC_HIP_CHECK(ctx, up(E.seg_off.p + nlong, &nseg, sizeof(long)));
+    }
+    RC(E.seg_row.get(ctx, nseg));
+    RC(E.seg_b.get(ctx, nseg));
+    RC(E.seg_part.get(ctx, 2 * nseg));
+    if (nlong)
+      long_seg_fill<<<nb(nlong), 256, 0, ctx->stream>>>(E.llist.p, E.row_ptr.p, nlong, E.seg_off.p, E.seg_row.p,
+                                                        E.seg_b.p);
     PLANC_LAUNCH_CHECK(ctx);
     // Node row ranges (rows of node t are contiguous: t-major flag scan).
     std::vector<int> rnode(nrows);
@@ -884,10 +969,12 @@ int planc_hiernmf_run(planc_ctx *ctx, const planc_hier_desc *desc, const long *h
       const int Gp = E.tune_group(nrows, LONGLEN);
       for (int it = 0; it < iters; ++it) {
         E.launch_pow_short(Gp, nrows, LONGLEN);
-        if (nlong)
-          pow_rows_long<<<(unsigned)nlong, 256, 0, ctx->stream>>>(E.llist.p, E.row_ptr.p, E.csr_col.p,
-                                                                  E.csr_val.p, E.row_node.p, E.q.p,
-                                                                  E.sig.p, E.z.p);
+        if (nlong) {
+          pow_rows_seg<<<(unsigned)nseg, 256, 0, ctx->stream>>>(E.seg_row.p, E.seg_b.p, E.llist.p, E.row_ptr.p,
+                                                                E.csr_col.p, E.csr_val.p, E.q.p, E.seg_part.p);
+          pow_rows_long_fin<<<nb(nlong, 64), 64, 0, ctx->stream>>>(nlong, E.llist.p, E.seg_off.p, E.seg_part.p,
+                                                                   E.row_node.p, E.sig.p, E.z.p);
+        }
         pow_cols<<<nb(nlev), 256, 0, ctx->stream>>>(nlev, E.col_ptr.p, E.csc_row.p, E.csc_val.p, E.z.p, E.q.p);
         sumsq_chunks<<<(unsigned)chc.size(), 256, 0, ctx->stream>>>(E.ch_cols.p, E.q.p, E.part.p);
         norm_finalize<<<nb(nn), 256, 0, ctx->stream>>>(E.part.p, E.cb_cols.p, nn, E.sig.p);
@@ -931,10 +1018,12 @@ int planc_hiernmf_run(planc_ctx *ctx, const planc_hier_desc *desc, const long *h
           gram2_chunks<<<(unsigned)chc.size(), 256, 0, ctx->stream>>>(E.ch_cols.p, E.H.p, E.part.p);
           gram2_finalize<<<nb(nn), 256, 0, ctx->stream>>>(E.part.p, E.cb_cols.p, nn, E.Gh.p);
           E.launch_w_short(Gw, nrows, E.W.p, LONGLEN);
-          if (nlong)
-            w_update_long<<<(unsigned)nlong, 256, 0, ctx->stream>>>(E.llist.p, E.row_ptr.p, E.csr_col.p,
-                                                                    E.csr_val.p, E.row_node.p, E.H.p,
-                                                                    E.Gh.p, E.W.p);
+          if (nlong) {
+            w_update_seg<<<(unsigned)nseg, 256, 0, ctx->stream>>>(E.seg_row.p, E.seg_b.p, E.llist.p, E.row_ptr.p,
+                                                                  E.csr_col.p, E.csr_val.p, E.H.p, E.seg_part.p);
+            w_update_long_fin<<<nb(nlong, 64), 64, 0, ctx->stream>>>(nlong, E.llist.p, E.seg_off.p, E.seg_part.p,
+                                                                     E.row_node.p, E.Gh.p, E.W.p);
+          }
         }
         PLANC_LAUNCH_CHECK(ctx);
         return PLANC_OK;

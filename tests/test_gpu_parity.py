@@ -553,6 +553,36 @@ def test_hiernmf_tree_matches_oracle(ctx):
             assert rel_err(w, r["W"]) < 1e-9
 
 
+def test_hiernmf_long_rows_match_oracle(ctx):
+    """Rows longer than 2048 entries take the segmented long-row kernels
+    (4096-entry segments, partials added in segment order): a row of 12000
+    entries (three segments), one of 5000 (two) and one of 2100 (one) next
+    to short rows, checked against the oracle's node-by-node tree."""
+    say("hiernmf long rows")
+    m, n, depth, iters, seed = 400, 20000, 3, 12, 4242
+    rng = np.random.default_rng(21)
+    lens = rng.integers(0, 40, size=m)
+    lens[0], lens[5], lens[9] = 12000, 5000, 2100
+    ix = np.concatenate([np.sort(rng.choice(n, size=int(k), replace=False)) for k in lens]).astype(np.int32)
+    ip = np.zeros(m + 1, dtype=np.int64)
+    ip[1:] = np.cumsum(lens)
+    dv = rng.lognormal(0.0, 1.0, size=int(ip[-1])).astype(np.float32)
+    ref = _hier_reference(m, n, ip, ix, dv.astype(np.float64), depth, iters, seed)
+    out, cols, H, W = ctx.hiernmf_run(m, n, ip, ix, dv, depth, iters, seed, want_H=True, want_W=True)
+    assert len(out) == len(ref)
+    for o, r in zip(out, ref):
+        assert o["index"] == r["index"] and o["level"] == r["level"]
+        c = cols[o["col_offset"]:o["col_offset"] + o["ncols"]]
+        assert np.array_equal(c, r["cols"])
+        if r["level"] > 0:
+            assert abs(o["sigma"] - r["sigma"]) <= 1e-9 * max(1.0, r["sigma"])
+        if "H" in r:
+            h = H[2 * o["col_offset"]:2 * (o["col_offset"] + o["ncols"])].reshape(-1, 2)
+            assert rel_err(h, r["H"]) < 1e-9
+            w = W[2 * m * o["index"]:2 * m * (o["index"] + 1)].reshape(2, m).T
+            assert rel_err(w, r["W"]) < 1e-9
+
+
 def test_hiernmf_level_invariants_medium(ctx):
     """20000 x 60000 power-law CSR, depth 7: every level partitions the
     columns of its parents, children are indexed 2i+1 / 2i+2 with the larger
