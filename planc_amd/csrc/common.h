@@ -99,7 +99,7 @@ struct planc_ctx {
   planc::Workspace ws[24];
   int num_cu = 256;
   int max_coop_blocks_hals = 0;
-  // Tuned streaming-product geometries: shape key -> {cw, wk, nsplit}.
+  // Tuned streaming-product geometries: shape key -> {cw, wk, nsplit, dual, shared}.
   std::vector<std::pair<std::vector<long>, std::vector<long>>> gemm_tuned;
 };
 

@@ -259,6 +259,35 @@ __device__ inline void epilogue_tn(const StreamParams &p, floatx4 (&acc)[CW * NT
   for (int a = 0; a < CW * NT; ++a) store_acc_tn<NT>(p, l0, a, acc[a]);
 }
 
+// NN tile of a wave at row l0: rows l0 + 16lg + 4v + j of column 16t + lc,
+// four consecutive rows per 16-byte store where the layout allows.
+template <int NT>
+__device__ inline void epilogue_nn(const StreamParams &p, floatx4 (&acc)[4 * NT], long l0) {
+  const int lane = threadIdx.x & 63, lc = lane & 15, lg = lane >> 4;
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int c = 16 * t + lc;
+    if (c >= p.th.k) continue;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const long l = l0 + 16 * lg + 4 * v;
+      floatx4 o = {acc[t * 4 + 0][v], acc[t * 4 + 1][v], acc[t * 4 + 2][v], acc[t * 4 + 3][v]};
+      if (!p.cmajor && p.cvec && l + 3 < p.L) {
+        *(floatx4 *)(p.C + l + p.ldc * c) = o;
+      } else {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          if (l + j >= p.L) continue;
+          if (p.cmajor)
+            p.C[c + p.ldc * (l + j)] = o[j];
+          else
+            p.C[l + j + p.ldc * c] = o[j];
+        }
+      }
+    }
+  }
+}
+
 // Sum over the K splits (split order) of one accumulator slab per block:
 // block (group g, accumulator a); wave w adds splits w, w+4, ...; the four
 // wave sums are combined in wave order.  kind 0: TN with NT, CW; 1: NN.
@@ -650,29 +679,359 @@ __global__ __launch_bounds__(NTHR) void nn_f32_kernel(StreamParams p) {
     }
     mma(b);
   }
-  if (!finish_tile<NA>(p, acc, red)) return;
+  if (finish_tile<NA>(p, acc, red)) epilogue_nn<NT>(p, acc, l0);
+}
+
+// ---------------------------------------------------------------------------
+// Block-shared thin operand.  With four waves on four row/column groups over
+// the same k range, every wave needs the same thin-operand words each step;
+// loading them per wave costs L1/L2 traffic close to that of A itself.  The
+// shared kernels instead stage each k step of the thin operand once per
+// block in an LDS ring of TS stages, in the MFMA lane layout (W 16-byte words
+// per step, written by the block's first W threads), and the waves read
+// their operand words from LDS right before the MFMAs.  Only A stays in the
+// per-wave register ring.
+//
+// Iteration i (all 256 threads, one barrier per step):
+//   mma(A buf i % PD, stage i % TS); load A step i + PD into that buffer;
+//   store thin step i + TS - 1 (raw words loaded PD iterations earlier into
+//   slot (i + TS - 1) % PD) into stage (i + TS - 1) % TS, which iteration
+//   i - 1 finished reading before the last barrier; load the raw words of
+//   step i + TS - 1 + PD into the freed slot; barrier.
+// Prefetch indexes past the last step are clamped to it (re-loads that hit in
+// cache), so the body has no branches on the step count.  The trip count is
+// uniform over the block (wk == 1), as the barriers need.
+template <int PD, int TS, class BufA, class Raw, class LoadA, class RawLoad, class RawStore, class Mma>
+__device__ inline void stream_steps_shared(long nsw, BufA (&buf)[PD], Raw (&raw)[PD], LoadA loadA,
+                                           RawLoad rload, RawStore rstore, Mma mma) {
+  if (nsw <= 0) return;
+  auto clamp = [&](long s) { return s < nsw ? s : nsw - 1; };
 #pragma unroll
-  for (int t = 0; t < NT; ++t) {
-    const int c = 16 * t + lc;
-    if (c >= p.th.k) continue;
+  for (int j = 0; j < TS - 1; ++j) {
+    rload(clamp(j), raw[0]);
+    rstore(raw[0], j);
+  }
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const long l = l0 + 16 * lg + 4 * v;
-      floatx4 o = {acc[t * 4 + 0][v], acc[t * 4 + 1][v], acc[t * 4 + 2][v], acc[t * 4 + 3][v]};
-      if (!p.cmajor && p.cvec && l + 3 < p.L) {
-        *(floatx4 *)(p.C + l + p.ldc * c) = o;
-      } else {
+  for (int q = 0; q < PD; ++q) rload(clamp(TS - 1 + q), raw[(TS - 1 + q) % PD]);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          if (l + j >= p.L) continue;
-          if (p.cmajor)
-            p.C[c + p.ldc * (l + j)] = o[j];
-          else
-            p.C[l + j + p.ldc * c] = o[j];
-        }
-      }
+  for (int u = 0; u < PD; ++u) loadA(clamp(u), buf[u]);
+  __syncthreads();
+  auto body = [&](long i, BufA &b, Raw &r) __attribute__((always_inline)) {
+    mma(b, (int)(i % TS));
+    __builtin_amdgcn_sched_barrier(0);
+    loadA(clamp(i + PD), b);
+    __builtin_amdgcn_sched_barrier(0);
+    rstore(r, (int)((i + TS - 1) % TS));
+    rload(clamp(i + TS - 1 + PD), r);
+    __builtin_amdgcn_sched_barrier(0);
+    __syncthreads();
+  };
+  long i = 0;
+  for (; i + PD <= nsw; i += PD) {
+#pragma unroll
+    for (int u = 0; u < PD; ++u) body(i + u, buf[u], raw[(u + TS - 1) % PD]);
+  }
+#pragma unroll
+  for (int u = 0; u < PD - 1; ++u)
+    if (i + u < nsw) body(i + u, buf[u], raw[(u + TS - 1) % PD]);
+}
+
+// FP32 thin-operand words of one producer thread: B(kb + 0..3, c) for its
+// column c = 16t + lc (clamped to k - 1) and kb = kbeg + 16 s + 4 lg.
+struct RawF32 {
+  floatx4 a;
+  float sc;
+};
+
+template <int TM>
+struct ThinProducerF32 {
+  const float *tb, *tf1;
+  unsigned dim0;
+  int t, ln;
+  bool on;
+  __device__ ThinProducerF32(const ThinDev &th, int NT) {
+    const int tid = threadIdx.x;
+    on = tid < NT * 64;
+    t = on ? tid >> 6 : 0;
+    ln = tid & 63;
+    int c = 16 * t + (ln & 15);
+    c = c < th.k ? c : th.k - 1;
+    tb = th.nf == 0 ? th.B + th.ldb * c : th.f[0] + th.ld[0] * c;
+    tf1 = th.nf > 1 ? th.f[1] + th.ld[1] * c : nullptr;
+    dim0 = th.nf > 0 ? th.dim[0] : 1u;
+  }
+  __device__ void load(long kb, RawF32 &r) const {
+    if (!on) return;
+    kb += 4 * (ln >> 4);
+    if (TM == THIN_KRP2) {
+      const unsigned q = (unsigned)kb / dim0;
+      const unsigned d0 = (unsigned)kb - q * dim0;
+      r.a = *(const floatx4 *)(tb + d0);
+      r.sc = tf1[q];
+    } else {
+      r.a = *(const floatx4 *)(tb + kb);
+      r.sc = 1.0f;
     }
   }
+  template <int NT>
+  __device__ void store(const RawF32 &r, floatx4 (*ts)[NT][64], int stage) const {
+    if (!on) return;
+    floatx4 v = r.a;
+    if (TM == THIN_KRP2) {
+      v[0] *= r.sc;
+      v[1] *= r.sc;
+      v[2] *= r.sc;
+      v[3] *= r.sc;
+    }
+    ts[stage][t][ln] = v;
+  }
+};
+
+constexpr int SH_TS = 3;  // LDS stages of the shared thin operand
+
+// FP32 NN with the block-shared thin operand (wk == 1: wave w owns rows
+// l0 = (4 blockIdx.x + w) * 64).  Layout and epilogue as nn_f32_kernel.
+template <int NT, int TM>
+__global__ __launch_bounds__(NTHR) void nn_f32_sh_kernel(StreamParams p) {
+  constexpr int STEP = 16, PD = 2, NA = 4 * NT;
+  __shared__ floatx4 ts[SH_TS][NT][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lc = lane & 15, lg = lane >> 4;
+  const long l0 = ((long)blockIdx.x * 4 + wave) * 64;
+  const long kbeg = (long)blockIdx.y * p.kchunk;
+  const long kend = min(p.K, kbeg + p.kchunk);
+  const long nfull = kend > kbeg ? (kend - kbeg) / STEP : 0;
+  const float *A = (const float *)p.A;
+  const long lr = l0 + 4 * lc;
+  const float *base = A + (lr < p.L - 4 ? lr : p.L - 4);
+  floatx4 acc[NA];
+#pragma unroll
+  for (int a = 0; a < NA; ++a) acc[a] = floatx4{0.f, 0.f, 0.f, 0.f};
+  struct BufA {
+    floatx4 x[4];
+  };
+  const ThinProducerF32<TM> prod(p.th, NT);
+  auto loadA = [&](long s, BufA &b) {
+    const long kb = kbeg + s * STEP + 4 * lg;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) b.x[u] = *(const floatx4 *)(base + p.lda * (kb + u));
+  };
+  auto rload = [&](long s, RawF32 &r) { prod.load(kbeg + s * STEP, r); };
+  auto rstore = [&](const RawF32 &r, int stage) { prod.template store<NT>(r, ts, stage); };
+  auto mma = [&](BufA &b, int stage) {
+    floatx4 a[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) a[t] = ts[stage][t][lane];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[t * 4 + j] = mfma4(b.x[u][j], a[t][u], acc[t * 4 + j]);
+  };
+  BufA buf[PD];
+  RawF32 raw[PD];
+  stream_steps_shared<PD, SH_TS>(nfull, buf, raw, loadA, rload, rstore, mma);
+  if (kend - kbeg > nfull * STEP) {
+    // The partial last step, masked.
+    long rows[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) rows[j] = lr + j < p.L ? lr + j : p.L - 1;
+    BufA b;
+    const long kb = kbeg + nfull * STEP + 4 * lg;
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b.x[u][j] = kb + u < kend ? A[rows[j] + p.lda * (kb + u)] : 0.0f;
+    floatx4 a[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) a[t] = thin4(p.th, kb, kend, 16 * t + lc);
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[t * 4 + j] = mfma4(b.x[u][j], a[t][u], acc[t * 4 + j]);
+  }
+  if (p.nsplit > 1) {
+    floatx4 *dst = (floatx4 *)p.part + (((long)blockIdx.y * gridDim.x + blockIdx.x) * 4 + wave) * (long)NA * 64;
+#pragma unroll
+    for (int a = 0; a < NA; ++a) dst[a * 64 + lane] = acc[a];
+    return;
+  }
+  epilogue_nn<NT>(p, acc, l0);
+}
+
+// FP32 TN with the block-shared thin operand (wave w owns columns
+// l0 = (4 blockIdx.x + w) * 16 CW).  Layout and epilogue as tn_f32_kernel.
+template <int NT, int CW, int TM>
+__global__ __launch_bounds__(NTHR) void tn_f32_sh_kernel(StreamParams p) {
+  constexpr int STEP = 16, PD = 3, NA = CW * NT;
+  __shared__ floatx4 ts[SH_TS][NT][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lc = lane & 15, lg = lane >> 4;
+  const long l0 = ((long)blockIdx.x * 4 + wave) * (16 * CW);
+  const long kbeg = (long)blockIdx.y * p.kchunk;
+  const long kend = min(p.K, kbeg + p.kchunk);
+  const long nfull = kend > kbeg ? (kend - kbeg) / STEP : 0;
+  const float *A = (const float *)p.A;
+  const float *col[CW];
+#pragma unroll
+  for (int cw = 0; cw < CW; ++cw) {
+    const long l = l0 + 16 * cw + lc;
+    col[cw] = A + p.lda * (l < p.L ? l : p.L - 1);
+  }
+  floatx4 acc[NA];
+#pragma unroll
+  for (int a = 0; a < NA; ++a) acc[a] = floatx4{0.f, 0.f, 0.f, 0.f};
+  struct BufA {
+    floatx4 x[CW];
+  };
+  const ThinProducerF32<TM> prod(p.th, NT);
+  auto loadA = [&](long s, BufA &b) {
+    const long kb = kbeg + s * STEP + 4 * lg;
+#pragma unroll
+    for (int cw = 0; cw < CW; ++cw) b.x[cw] = *(const floatx4 *)(col[cw] + kb);
+  };
+  auto rload = [&](long s, RawF32 &r) { prod.load(kbeg + s * STEP, r); };
+  auto rstore = [&](const RawF32 &r, int stage) { prod.template store<NT>(r, ts, stage); };
+  auto mma = [&](BufA &b, int stage) {
+    floatx4 a[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) a[t] = ts[stage][t][lane];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int cw = 0; cw < CW; ++cw) acc[cw * NT + t] = mfma4(a[t][j], b.x[cw][j], acc[cw * NT + t]);
+  };
+  BufA buf[PD];
+  RawF32 raw[PD];
+  stream_steps_shared<PD, SH_TS>(nfull, buf, raw, loadA, rload, rstore, mma);
+  if (kend - kbeg > nfull * STEP) {
+    BufA b;
+    const long kb = kbeg + nfull * STEP + 4 * lg;
+#pragma unroll
+    for (int cw = 0; cw < CW; ++cw)
+#pragma unroll
+      for (int u = 0; u < 4; ++u) b.x[cw][u] = kb + u < kend ? col[cw][kb + u] : 0.0f;
+    floatx4 a[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) a[t] = thin4(p.th, kb, kend, 16 * t + lc);
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int cw = 0; cw < CW; ++cw) acc[cw * NT + t] = mfma4(a[t][j], b.x[cw][j], acc[cw * NT + t]);
+  }
+  if (p.nsplit > 1) {
+    floatx4 *dst = (floatx4 *)p.part + (((long)blockIdx.y * gridDim.x + blockIdx.x) * 4 + wave) * (long)NA * 64;
+#pragma unroll
+    for (int a = 0; a < NA; ++a) dst[a * 64 + lane] = acc[a];
+    return;
+  }
+  epilogue_tn<NT, CW>(p, acc, l0);
+}
+
+// 16-bit TN (64-k steps, two 16-byte words per lane and column as the DUAL
+// tn_h_kernel) with the block-shared thin operand: the packed words of one
+// step (2 halves x NT tiles x 64 lanes, contiguous in the packed buffer) are
+// copied into the LDS stage by the block's threads (two words per thread
+// beyond 256).
+template <typename TA, int NT, int CW>
+__global__ __launch_bounds__(NTHR) void tn_h_sh_kernel(StreamParams p) {
+  constexpr int NS = 2, STEP = 64, PD = 2, NA = CW * NT;
+  constexpr int LW = 8 * NS;
+  constexpr int WORDS = NS * NT * 64;  // <= 2 * NTHR
+  __shared__ uint4 ts[SH_TS][WORDS];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lc = lane & 15, lg = lane >> 4;
+  const long l0 = ((long)blockIdx.x * 4 + wave) * (16 * CW);
+  const long kbeg = (long)blockIdx.y * p.kchunk;
+  const long kend = min(p.K, kbeg + p.kchunk);
+  const long nfull = kend > kbeg ? (kend - kbeg) / STEP : 0;
+  const uint16_t *A = (const uint16_t *)p.A;
+  const uint16_t *col[CW];
+#pragma unroll
+  for (int cw = 0; cw < CW; ++cw) {
+    const long l = l0 + 16 * cw + lc;
+    col[cw] = A + p.lda * (l < p.L ? l : p.L - 1);
+  }
+  floatx4 acc[NA];
+#pragma unroll
+  for (int a = 0; a < NA; ++a) acc[a] = floatx4{0.f, 0.f, 0.f, 0.f};
+  struct BufA {
+    uint4 x[CW][NS];
+  };
+  // Words threadIdx.x and threadIdx.x + NTHR of a step, as one 32-byte vector.
+  typedef unsigned RawH __attribute__((ext_vector_type(8)));
+  // Step s starts at word (kbeg / STEP + s) * WORDS of the packed buffer.
+  const uint4 *pk = p.thin16 + (kbeg / STEP) * (long)WORDS;
+  auto loadA = [&](long s, BufA &b) {
+    const long kb = kbeg + s * STEP + LW * lg;
+#pragma unroll
+    for (int cw = 0; cw < CW; ++cw)
+#pragma unroll
+      for (int h = 0; h < NS; ++h) b.x[cw][h] = *(const uint4 *)(col[cw] + kb + 8 * h);
+  };
+  const int w0 = threadIdx.x, w1 = threadIdx.x + NTHR;
+  auto rload = [&](long s, RawH &r) {
+    uint4 a = {0u, 0u, 0u, 0u}, b = {0u, 0u, 0u, 0u};
+    if (w0 < WORDS) a = pk[s * WORDS + w0];
+    if (WORDS > NTHR && w1 < WORDS) b = pk[s * WORDS + w1];
+    r = RawH{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+  };
+  auto rstore = [&](const RawH &r, int stage) {
+    if (w0 < WORDS) ts[stage][w0] = uint4{r[0], r[1], r[2], r[3]};
+    if (WORDS > NTHR && w1 < WORDS) ts[stage][w1] = uint4{r[4], r[5], r[6], r[7]};
+  };
+  auto mma = [&](BufA &b, int stage) {
+    const uint4 *st = ts[stage] + lane;
+#pragma unroll
+    for (int h = 0; h < NS; ++h)
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const uint4 a = st[(h * NT + t) * 64];
+#pragma unroll
+        for (int cw = 0; cw < CW; ++cw) acc[cw * NT + t] = mfma16<TA>(a, b.x[cw][h], acc[cw * NT + t]);
+      }
+  };
+  BufA buf[PD];
+  RawH raw[PD];
+  stream_steps_shared<PD, SH_TS>(nfull, buf, raw, loadA, rload, rstore, mma);
+  if (kend - kbeg > nfull * STEP) {
+    BufA b;
+    const long kb = kbeg + nfull * STEP + LW * lg;
+#pragma unroll
+    for (int cw = 0; cw < CW; ++cw)
+#pragma unroll
+      for (int h = 0; h < NS; ++h) {
+        uint16_t v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const long kk = kb + 8 * h + u;
+          v[u] = kk < kend ? col[cw][kk] : (uint16_t)0;
+        }
+        b.x[cw][h] = __builtin_bit_cast(uint4, v);
+      }
+#pragma unroll
+    for (int h = 0; h < NS; ++h)
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const uint4 a = p.thin16[pack_index(kb + 8 * h, t, lc, NT)];
+#pragma unroll
+        for (int cw = 0; cw < CW; ++cw) acc[cw * NT + t] = mfma16<TA>(a, b.x[cw][h], acc[cw * NT + t]);
+      }
+  }
+  if (p.nsplit > 1) {
+    floatx4 *dst = (floatx4 *)p.part + (((long)blockIdx.y * gridDim.x + blockIdx.x) * 4 + wave) * (long)NA * 64;
+#pragma unroll
+    for (int a = 0; a < NA; ++a) dst[a * 64 + lane] = acc[a];
+    return;
+  }
+  epilogue_tn<NT, CW>(p, acc, l0);
 }
 
 // ---------------------------------------------------------------------------
@@ -848,14 +1207,20 @@ __global__ void reduce_splits_kernel(const float *partial, int splits, long L, i
 // K splits across blocks.
 struct Geometry {
   int cw, wgrp, wk, nsplit;
-  int dual;  // 16-bit TN: 64-k steps with two 16-byte loads per lane and column
+  int dual;    // 16-bit TN: 64-k steps with two 16-byte loads per lane and column
+  int shared;  // thin operand staged once per block in LDS (wk == 1; 16-bit: dual)
   long kchunk, blocks_x, groups;
   long tile_bytes;  // partial tile of one group
 };
 
 Geometry make_geometry(long L, long K, int trans, int dtype, int NT, int cw, int wk, long ns,
-                       int dual = 0) {
+                       int dual = 0, int shared = 0) {
   Geometry g{};
+  g.shared = shared;
+  if (shared) {
+    wk = 1;
+    if (trans && dtype != PLANC_F32) dual = 1;
+  }
   g.dual = (trans && dtype != PLANC_F32) ? dual : 0;
   const int step = dtype == PLANC_F32 ? 16 : (g.dual ? 64 : 32);
   g.cw = trans ? cw : 1;
@@ -878,7 +1243,7 @@ Geometry make_geometry(long L, long K, int trans, int dtype, int NT, int cw, int
 // and the grid stays under 16k waves.  The first entry is the static choice
 // used when tuning is off or impossible (graph capture).
 std::vector<Geometry> candidate_geometries(const planc_ctx *ctx, long L, long K, int trans, int dtype,
-                                           int NT) {
+                                           int NT, bool shared_ok) {
   std::vector<Geometry> out;
   const int step = dtype == PLANC_F32 ? 16 : 32;
   const long ksteps = (K + step - 1) / step;
@@ -924,6 +1289,22 @@ std::vector<Geometry> candidate_geometries(const planc_ctx *ctx, long L, long K,
         out.push_back(g);
       }
     }
+  // Block-shared thin operand: one k phase, four groups per block.
+  const std::vector<int> scws = trans ? std::vector<int>{2, 4} : std::vector<int>{1};
+  if (shared_ok)
+    for (int cw : scws) {
+      const int sstep = dtype == PLANC_F32 ? 16 : 64;
+      const long sk = (K + sstep - 1) / sstep;
+      for (long ns = 1; ns <= 4096; ns *= 2) {
+        Geometry g = make_geometry(L, K, trans, dtype, NT, cw, 1, ns, 1, 1);
+        if (ns > 1 && (g.nsplit < ns || sk / g.nsplit < 8)) break;
+        if (ns > 1 && 2.0 * g.nsplit * g.groups * g.tile_bytes > 0.10 * abytes) break;
+        const long waves = g.blocks_x * 4 * g.nsplit;
+        if (ns > 1 && waves > 16L * 1024) break;
+        if (waves < 2L * ctx->num_cu && ns < 4096) continue;
+        out.push_back(g);
+      }
+    }
   return out;
 }
 
@@ -962,6 +1343,27 @@ void launch_tn_h(bool safe, int dual, dim3 grid, hipStream_t st, const StreamPar
     tn_h_kernel<TA, NT, 4, 0, 0><<<grid, NTHR, 0, st>>>(p);
 }
 template <int NT, int CW>
+void launch_tn_f32_sh(int tm, dim3 grid, hipStream_t st, const StreamParams &p) {
+  if (tm == THIN_PLAIN)
+    tn_f32_sh_kernel<NT, CW, THIN_PLAIN><<<grid, NTHR, 0, st>>>(p);
+  else
+    tn_f32_sh_kernel<NT, CW, THIN_KRP2><<<grid, NTHR, 0, st>>>(p);
+}
+template <int NT>
+void launch_nn_f32_sh(int tm, dim3 grid, hipStream_t st, const StreamParams &p) {
+  if (tm == THIN_PLAIN)
+    nn_f32_sh_kernel<NT, THIN_PLAIN><<<grid, NTHR, 0, st>>>(p);
+  else
+    nn_f32_sh_kernel<NT, THIN_KRP2><<<grid, NTHR, 0, st>>>(p);
+}
+template <typename TA, int CW>
+void launch_tn_h_sh(int NT, dim3 grid, hipStream_t st, const StreamParams &p) {
+  if (NT == 1) tn_h_sh_kernel<TA, 1, CW><<<grid, NTHR, 0, st>>>(p);
+  else if (NT == 2) tn_h_sh_kernel<TA, 2, CW><<<grid, NTHR, 0, st>>>(p);
+  else if (NT == 3) tn_h_sh_kernel<TA, 3, CW><<<grid, NTHR, 0, st>>>(p);
+  else tn_h_sh_kernel<TA, 4, CW><<<grid, NTHR, 0, st>>>(p);
+}
+template <int NT, int CW>
 void launch_reduce_tn(dim3 g, hipStream_t st, const StreamParams &p, int NA, long ngroups) {
   splitk_reduce_kernel<0, NT, CW><<<g, 256, 0, st>>>(p, NA, ngroups);
 }
@@ -980,7 +1382,33 @@ int launch_stream(planc_ctx *ctx, int dtype, int trans, int NT, int tm, StreamPa
   }
   dim3 grid((unsigned)g.blocks_x, (unsigned)g.nsplit);
   hipStream_t st = ctx->stream;
-  if (trans && dtype == PLANC_F32) {
+  if (g.shared && trans && dtype == PLANC_F32) {
+#define TNS(NTV)                                        \
+  do {                                                  \
+    if (g.cw == 4)                                      \
+      launch_tn_f32_sh<NTV, 4>(tm, grid, st, p);        \
+    else                                                \
+      launch_tn_f32_sh<NTV, 2>(tm, grid, st, p);        \
+  } while (0)
+    if (NT == 1) TNS(1);
+    else if (NT == 2) TNS(2);
+    else if (NT == 3) TNS(3);
+    else TNS(4);
+#undef TNS
+  } else if (g.shared && trans) {
+    if (dtype == PLANC_BF16) {
+      if (g.cw == 4) launch_tn_h_sh<bf16_t, 4>(NT, grid, st, p);
+      else launch_tn_h_sh<bf16_t, 2>(NT, grid, st, p);
+    } else {
+      if (g.cw == 4) launch_tn_h_sh<f16_t, 4>(NT, grid, st, p);
+      else launch_tn_h_sh<f16_t, 2>(NT, grid, st, p);
+    }
+  } else if (g.shared) {
+    if (NT == 1) launch_nn_f32_sh<1>(tm, grid, st, p);
+    else if (NT == 2) launch_nn_f32_sh<2>(tm, grid, st, p);
+    else if (NT == 3) launch_nn_f32_sh<3>(tm, grid, st, p);
+    else launch_nn_f32_sh<4>(tm, grid, st, p);
+  } else if (trans && dtype == PLANC_F32) {
 #define TNF(NTV)                                        \
   do {                                                  \
     if (g.cw == 4)                                      \
@@ -1047,19 +1475,32 @@ bool stream_capturing(hipStream_t st) {
 // every candidate is timed on the live operands (one warm-up launch and two
 // timed launches each; the products are idempotent) unless tuning is off
 // (PLANC_AUTOTUNE=0) or the stream is being captured into a graph, where the
-// static choice is used.
+// static choice is used.  PLANC_TUNE_LOG=1 prints each candidate's time.
 int pick_geometry(planc_ctx *ctx, int dtype, int trans, int NT, int tm, const StreamParams &p,
                   Geometry *out) {
-  const std::vector<long> key = {dtype, trans, NT, tm, p.L, p.K, p.lda, p.th.k, p.avec, p.cmajor};
+  // PLANC_SHARED_THIN: 0 never stages the thin operand in LDS, 1 always does
+  // where the kernel exists (both used to test each path); unset: tuned.
+  const char *senv = getenv("PLANC_SHARED_THIN");
+  const int smode = senv && senv[0] == '0' ? 0 : (senv && senv[0] == '1' ? 1 : 2);
+  const std::vector<long> key = {dtype, trans, NT, tm, p.L, p.K, p.lda, p.th.k, p.avec, p.cmajor, smode};
   for (const auto &e : ctx->gemm_tuned)
     if (e.first == key) {
       const auto &v = e.second;
-      *out = make_geometry(p.L, p.K, trans, dtype, NT, (int)v[0], (int)v[1], v[2], (int)v[3]);
+      *out = make_geometry(p.L, p.K, trans, dtype, NT, (int)v[0], (int)v[1], v[2], (int)v[3], (int)v[4]);
       return PLANC_OK;
     }
-  std::vector<Geometry> cands = candidate_geometries(ctx, p.L, p.K, trans, dtype, NT);
+  const bool shared_ok = (dtype == PLANC_F32 ? tm != THIN_SAFE : (trans && p.avec)) && smode != 0;
+  std::vector<Geometry> cands = candidate_geometries(ctx, p.L, p.K, trans, dtype, NT, shared_ok);
+  if (shared_ok && smode == 1) {
+    std::vector<Geometry> sh;
+    for (const Geometry &g : cands)
+      if (g.shared) sh.push_back(g);
+    if (!sh.empty()) cands.swap(sh);
+  }
   const char *env = getenv("PLANC_AUTOTUNE");
   const bool tune = !(env && env[0] == '0') && !stream_capturing(ctx->stream) && cands.size() > 1;
+  const char *lenv = getenv("PLANC_TUNE_LOG");  // 1: print every candidate's time to stderr
+  const bool log = lenv && lenv[0] == '1';
   Geometry best = cands[0];
   if (tune) {
     hipEvent_t e0, e1;
@@ -1079,6 +1520,10 @@ int pick_geometry(planc_ctx *ctx, int dtype, int trans, int NT, int tm, const St
       PLANC_HIP_CHECK(ctx, hipEventSynchronize(e1));
       float ms = 0.0f;
       PLANC_HIP_CHECK(ctx, hipEventElapsedTime(&ms, e0, e1));
+      if (log)
+        fprintf(stderr,
+                "planc tune dtype %d trans %d L %ld K %ld k %d: cw %d wk %d nsplit %ld dual %d shared %d -> %.2f us\n",
+                dtype, trans, p.L, p.K, p.th.k, g.cw, g.wk, (long)g.nsplit, g.dual, g.shared, ms * 500.0f);
       if (!have || ms < best_ms) {
         best_ms = ms;
         best = g;
@@ -1089,7 +1534,7 @@ int pick_geometry(planc_ctx *ctx, int dtype, int trans, int NT, int tm, const St
     (void)hipEventDestroy(e1);
   }
   if (!stream_capturing(ctx->stream))
-    ctx->gemm_tuned.push_back({key, {best.cw, best.wk, (long)best.nsplit, best.dual}});
+    ctx->gemm_tuned.push_back({key, {best.cw, best.wk, (long)best.nsplit, best.dual, best.shared}});
   *out = best;
   return PLANC_OK;
 }

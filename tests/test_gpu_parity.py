@@ -73,10 +73,14 @@ GEMM_CASES = [
 ]
 
 
+@pytest.mark.parametrize("shared", ["0", "1"])
 @pytest.mark.parametrize("dt", [P.F32, P.BF16, P.F16])
 @pytest.mark.parametrize("L,K,k", GEMM_CASES)
-def test_gemm_nn_tn(ctx, dt, L, K, k):
-    say(f"gemm dt={dt} L={L} K={K} k={k}")
+def test_gemm_nn_tn(ctx, monkeypatch, dt, L, K, k, shared):
+    """Both streaming families: per-wave thin operand (PLANC_SHARED_THIN=0)
+    and the block-shared LDS ring (=1, where the kernel applies)."""
+    monkeypatch.setenv("PLANC_SHARED_THIN", shared)
+    say(f"gemm dt={dt} L={L} K={K} k={k} shared={shared}")
     rng = np.random.default_rng(L * 7 + K + k)
     A = rng.random((L, K), dtype=np.float32)
     B = rng.random((K, k), dtype=np.float32)
@@ -205,10 +209,13 @@ def test_hals_cols_sweep(ctx, m, k):
     assert rel_err(dW.cpu().numpy(), Wr) < 2e-5
 
 
+@pytest.mark.parametrize("shared", ["0", "1"])
 @pytest.mark.parametrize("dims,k", [((7, 8, 9), 3), ((20, 18, 16), 5), ((5, 6, 4, 3), 2),
-                                    ((33, 17, 20, 9), 16), ((64, 64, 64), 32), ((3, 4, 5, 6, 2), 4)])
-def test_mttkrp_direct_all_modes(ctx, dims, k):
-    say(f"mttkrp dims={dims} k={k}")
+                                    ((33, 17, 20, 9), 16), ((64, 64, 64), 32), ((3, 4, 5, 6, 2), 4),
+                                    ((16, 32, 24, 40), 48)])
+def test_mttkrp_direct_all_modes(ctx, monkeypatch, dims, k, shared):
+    monkeypatch.setenv("PLANC_SHARED_THIN", shared)
+    say(f"mttkrp dims={dims} k={k} shared={shared}")
     rng = np.random.default_rng(sum(dims) + k)
     X = np.asfortranarray(rng.random(dims, dtype=np.float32))
     F = [rng.random((d, k), dtype=np.float32) for d in dims]
@@ -397,6 +404,43 @@ def test_partial_mttkrp_and_multi_ttv(ctx):
             else:
                 M = ctx.multi_ttv(Qr, dF[s + 1:], n - s - 1)
             assert rel_err(M.cpu().numpy(), ref) < 1e-5, (s, n)
+
+
+@pytest.mark.parametrize("shared", ["0", "1"])
+@pytest.mark.parametrize("dt", [P.F32, P.BF16, P.F16])
+def test_partial_mttkrp_streaming_paths(ctx, monkeypatch, dt, shared):
+    """Right (X_(L x R) * KRP_R) and left (KRP_L^T * X) partial contractions
+    with two-factor Khatri-Rao operands generated on the fly, through the
+    per-wave and the block-shared thin-operand kernels, in FP32 and 16-bit
+    (16-bit inputs rounded the same way in the FP64 reference)."""
+    monkeypatch.setenv("PLANC_SHARED_THIN", shared)
+    dims, k = (16, 20, 24, 36), 40
+    say(f"partial mttkrp dt={dt} shared={shared}")
+    rng = np.random.default_rng(91)
+    X = np.asfortranarray(rng.random(dims, dtype=np.float32))
+    F = [rng.random((d, k), dtype=np.float32) for d in dims]
+    dF = [P.colmajor(f) for f in F]
+    dX = P.colmajor(X.reshape(-1, order="F")[:, None])
+    if dt == P.F32:
+        Xop, Xr = dX, X.astype(np.float64)
+        tol = 3e-6
+    else:
+        Xop, _ = ctx.convert(dX, dt, scaled=False)
+        Xr = rnd(dt, X.reshape(-1, order="F")).reshape(dims, order="F")
+        tol = 1e-5
+    L, R = dims[0] * dims[1], dims[2] * dims[3]
+    X2 = Xr.reshape(L, R, order="F")
+
+    def krp(a, b):
+        # FP32 Khatri-Rao rows (row = i_a + I_a * i_b), rounded like the kernels' thin operand
+        v = np.einsum("az,bz->baz", a.astype(np.float64), b.astype(np.float64)).reshape(-1, k)
+        return rnd(dt, v.astype(np.float32))
+
+    krp_r, krp_l = krp(F[2], F[3]), krp(F[0], F[1])
+    Pm = ctx.mttkrp_partial(Xop, dF, 1, 0, dtype=dt).cpu().numpy()
+    assert rel_err(Pm, X2 @ krp_r) < tol
+    Qm = ctx.mttkrp_partial(Xop, dF, 1, 1, dtype=dt).cpu().numpy()
+    assert rel_err(Qm, X2.T @ krp_l) < tol
 
 
 def _cp_case(dims, k, stream, gamma=False):
