@@ -37,6 +37,7 @@
 // uses an LDS-transposing kernel with a separate split-K reduction.
 #include <cstdio>
 #include <cstdlib>
+#include <string>
 #include <vector>
 
 #include "common.h"
@@ -259,11 +260,40 @@ __device__ inline void epilogue_tn(const StreamParams &p, floatx4 (&acc)[CW * NT
   for (int a = 0; a < CW * NT; ++a) store_acc_tn<NT>(p, l0, a, acc[a]);
 }
 
-// NN tile of a wave at row l0: rows l0 + 16lg + 4v + j of column 16t + lc,
-// four consecutive rows per 16-byte store where the layout allows.
+// NN tile of a wave at row l0: lane (lc, lg) holds rows l0 + 16lg + 4v + j
+// of column 16t + lc in acc[4t + j][v].  For a full 64-row tile of an
+// L x k output with 16-byte stores, the 16 columns of each t go through the
+// wave's LDS slab ep (16 columns x EP_LD floats, padded against bank
+// conflicts) so that every store instruction writes four whole 256-byte
+// column segments; otherwise four consecutive rows per 16-byte store (or
+// single elements).  LDS operations of one wave complete in order, so the
+// slab needs no barrier, only the compiler fences around it.
+constexpr int EP_LD = 68;
 template <int NT>
-__device__ inline void epilogue_nn(const StreamParams &p, floatx4 (&acc)[4 * NT], long l0) {
+__device__ inline void epilogue_nn(const StreamParams &p, floatx4 (&acc)[4 * NT], long l0, float *ep) {
   const int lane = threadIdx.x & 63, lc = lane & 15, lg = lane >> 4;
+  if (ep && !p.cmajor && p.cvec && l0 + 63 < p.L) {
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        floatx4 o = {acc[t * 4 + 0][v], acc[t * 4 + 1][v], acc[t * 4 + 2][v], acc[t * 4 + 3][v]};
+        *(floatx4 *)(ep + lc * EP_LD + 16 * lg + 4 * v) = o;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int cl = 4 * q + lg;  // column of this lane's 16 bytes
+        const floatx4 o = *(const floatx4 *)(ep + cl * EP_LD + 4 * lc);
+        const int c = 16 * t + cl;
+        if (c < p.th.k) *(floatx4 *)(p.C + l0 + 4 * lc + p.ldc * c) = o;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+    }
+    return;
+  }
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
     const int c = 16 * t + lc;
@@ -679,7 +709,10 @@ __global__ __launch_bounds__(NTHR) void nn_f32_kernel(StreamParams p) {
     }
     mma(b);
   }
-  if (finish_tile<NA>(p, acc, red)) epilogue_nn<NT>(p, acc, l0);
+  if (finish_tile<NA>(p, acc, red)) {
+    __shared__ __attribute__((aligned(16))) float ep[4][16 * EP_LD];
+    epilogue_nn<NT>(p, acc, l0, ep[wave]);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -859,7 +892,8 @@ __global__ __launch_bounds__(NTHR) void nn_f32_sh_kernel(StreamParams p) {
     for (int a = 0; a < NA; ++a) dst[a * 64 + lane] = acc[a];
     return;
   }
-  epilogue_nn<NT>(p, acc, l0);
+  __shared__ __attribute__((aligned(16))) float ep[4][16 * EP_LD];
+  epilogue_nn<NT>(p, acc, l0, ep[wave]);
 }
 
 // FP32 TN with the block-shared thin operand (wave w owns columns
@@ -1478,10 +1512,12 @@ bool stream_capturing(hipStream_t st) {
 // static choice is used.  PLANC_TUNE_LOG=1 prints each candidate's time.
 int pick_geometry(planc_ctx *ctx, int dtype, int trans, int NT, int tm, const StreamParams &p,
                   Geometry *out) {
-  // PLANC_SHARED_THIN: 0 never stages the thin operand in LDS, 1 always does
-  // where the kernel exists (both used to test each path); unset: tuned.
-  const char *senv = getenv("PLANC_SHARED_THIN");
-  const int smode = senv && senv[0] == '0' ? 0 : (senv && senv[0] == '1' ? 1 : 2);
+  // PLANC_STREAM_VARIANT restricts the tuner to one kernel family (used to
+  // test each path): "plain" (per-wave thin operand) or "shared" (block-shared
+  // LDS thin operand, where it applies); unset: both.
+  const char *venv = getenv("PLANC_STREAM_VARIANT");
+  const std::string vs = venv ? venv : "";
+  const int smode = vs == "plain" ? 0 : (vs == "shared" ? 1 : 2);
   const std::vector<long> key = {dtype, trans, NT, tm, p.L, p.K, p.lda, p.th.k, p.avec, p.cmajor, smode};
   for (const auto &e : ctx->gemm_tuned)
     if (e.first == key) {

@@ -73,14 +73,14 @@ GEMM_CASES = [
 ]
 
 
-@pytest.mark.parametrize("shared", ["0", "1"])
+@pytest.mark.parametrize("variant", ["plain", "shared"])
 @pytest.mark.parametrize("dt", [P.F32, P.BF16, P.F16])
 @pytest.mark.parametrize("L,K,k", GEMM_CASES)
-def test_gemm_nn_tn(ctx, monkeypatch, dt, L, K, k, shared):
-    """Both streaming families: per-wave thin operand (PLANC_SHARED_THIN=0)
-    and the block-shared LDS ring (=1, where the kernel applies)."""
-    monkeypatch.setenv("PLANC_SHARED_THIN", shared)
-    say(f"gemm dt={dt} L={L} K={K} k={k} shared={shared}")
+def test_gemm_nn_tn(ctx, monkeypatch, dt, L, K, k, variant):
+    """Both streaming families: per-wave thin operand and the block-shared
+    LDS ring (where it applies; otherwise the tuner's other candidates run)."""
+    monkeypatch.setenv("PLANC_STREAM_VARIANT", variant)
+    say(f"gemm dt={dt} L={L} K={K} k={k} variant={variant}")
     rng = np.random.default_rng(L * 7 + K + k)
     A = rng.random((L, K), dtype=np.float32)
     B = rng.random((K, k), dtype=np.float32)
@@ -209,13 +209,13 @@ def test_hals_cols_sweep(ctx, m, k):
     assert rel_err(dW.cpu().numpy(), Wr) < 2e-5
 
 
-@pytest.mark.parametrize("shared", ["0", "1"])
+@pytest.mark.parametrize("variant", ["plain", "shared"])
 @pytest.mark.parametrize("dims,k", [((7, 8, 9), 3), ((20, 18, 16), 5), ((5, 6, 4, 3), 2),
                                     ((33, 17, 20, 9), 16), ((64, 64, 64), 32), ((3, 4, 5, 6, 2), 4),
                                     ((16, 32, 24, 40), 48)])
-def test_mttkrp_direct_all_modes(ctx, monkeypatch, dims, k, shared):
-    monkeypatch.setenv("PLANC_SHARED_THIN", shared)
-    say(f"mttkrp dims={dims} k={k} shared={shared}")
+def test_mttkrp_direct_all_modes(ctx, monkeypatch, dims, k, variant):
+    monkeypatch.setenv("PLANC_STREAM_VARIANT", variant)
+    say(f"mttkrp dims={dims} k={k} variant={variant}")
     rng = np.random.default_rng(sum(dims) + k)
     X = np.asfortranarray(rng.random(dims, dtype=np.float32))
     F = [rng.random((d, k), dtype=np.float32) for d in dims]
@@ -406,16 +406,16 @@ def test_partial_mttkrp_and_multi_ttv(ctx):
             assert rel_err(M.cpu().numpy(), ref) < 1e-5, (s, n)
 
 
-@pytest.mark.parametrize("shared", ["0", "1"])
+@pytest.mark.parametrize("variant", ["plain", "shared"])
 @pytest.mark.parametrize("dt", [P.F32, P.BF16, P.F16])
-def test_partial_mttkrp_streaming_paths(ctx, monkeypatch, dt, shared):
+def test_partial_mttkrp_streaming_paths(ctx, monkeypatch, dt, variant):
     """Right (X_(L x R) * KRP_R) and left (KRP_L^T * X) partial contractions
     with two-factor Khatri-Rao operands generated on the fly, through the
     per-wave and the block-shared thin-operand kernels, in FP32 and 16-bit
     (16-bit inputs rounded the same way in the FP64 reference)."""
-    monkeypatch.setenv("PLANC_SHARED_THIN", shared)
+    monkeypatch.setenv("PLANC_STREAM_VARIANT", variant)
     dims, k = (16, 20, 24, 36), 40
-    say(f"partial mttkrp dt={dt} shared={shared}")
+    say(f"partial mttkrp dt={dt} variant={variant}")
     rng = np.random.default_rng(91)
     X = np.asfortranarray(rng.random(dims, dtype=np.float32))
     F = [rng.random((d, k), dtype=np.float32) for d in dims]
