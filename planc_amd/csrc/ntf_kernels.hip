@@ -248,10 +248,11 @@ __global__ void transpose_kernel(const float *in, long I, long ldi, int k, float
 // MTTKRP, normalize(mode) (ntf/auntf.hpp:66-79: lambda = column 2-norms,
 // columns divided by them), the FP64 Gram of the new factor and optionally
 // the CP relative error of ntf/auntf.hpp:224-245 from this mode's MTTKRP.
-//   mode_rows_kernel: 64 rows per block, a quad of lanes per row (each lane
-//     holds k/4 of the row, the column dot products are quad sums), writes
-//     the unnormalised rows u and accumulates per block the FP64 Gram of u
-//     and, per column, sum_i u(i, c) M(c, i).
+//   mode_rows_kernel: MRT = 64 rows per block (16-row blocks over more CUs
+//     measured slower at 256 x 32), a quad of lanes per row (each lane holds
+//     k/4 of the row, the column dot products are quad sums), writes the
+//     unnormalised rows u and accumulates per block the FP64 Gram of u and,
+//     per column, sum_i u(i, c) M(c, i).
 //   mode_finalize_kernel (one block): sums the block partials in block order;
 //     norms = sqrt(diag(Gram u)), lambda = norms, Gram of the normalised
 //     factor = D^-1 (Gram u) D^-1, the columns of F divided by the norms,
@@ -274,14 +275,16 @@ struct ModeParams {
   int *rel_counter;  // non-null: the error goes to rel_out[*rel_counter], which is incremented
 };
 
-constexpr int MR = 64;  // rows per block-iteration of mode_rows_kernel
-
-template <int KM, int KIND>
-__global__ __launch_bounds__(256) void mode_rows_kernel(ModeParams p) {
+template <int KM, int KIND, int MRT>
+__global__ __launch_bounds__(4 * MRT) void mode_rows_kernel(ModeParams p) {
+  constexpr int MR = MRT;            // rows per block iteration
+  constexpr int NTH = 4 * MR;        // threads: a quad per row
+  constexpr int NW = NTH / 64;       // waves
   constexpr int Q = KM / 4;          // row values per lane
   constexpr int NT = KM / 16;        // 16 x 16 tiles per Gram side
   constexpr int NUP = NT * (NT + 1) / 2;  // upper-triangle tiles
-  constexpr int TPW = (NUP + 3) / 4;      // tiles per wave
+  constexpr int TPW = (NUP + NW - 1) / NW;  // tiles per wave
+  constexpr int NDP = (4 * KM + NTH - 1) / NTH;  // (column, quarter) dot pairs per thread
   __shared__ __attribute__((aligned(16))) float St[KM][KM];  // St[x][t] = S(t, x)
   __shared__ float Us[KM][MR + 1];                           // u(row, c) at Us[c][row]
   __shared__ float Ms[KM][MR + 1];                           // M(c, row)
@@ -314,22 +317,42 @@ __global__ __launch_bounds__(256) void mode_rows_kernel(ModeParams p) {
   float h[Q], mv[Q];
   long r0 = (long)blockIdx.x * MR;
   load_rows(r0, h, mv);  // in flight while S is built
-  for (int e = tid; e < KM * KM; e += 256) {
-    const int a = e % KM, b = e / KM;
-    double d = 1.0;
+  // S = Hadamard of the other Grams: every load of the block's entries is
+  // issued before the first product (one memory round trip, not one per
+  // entry).
+  {
+    constexpr int NE = (KM * KM + NTH - 1) / NTH;
+    double gv[NE][4];
 #pragma unroll
-    for (int g = 0; g < 8; ++g) {
-      const bool use = g < p.ng && g != p.skip && a < k && b < k;
-      const double *src = p.g[g < p.ng ? g : 0];
-      const double v = src[(a < k && b < k) ? a + (long)k * b : 0];
-      d *= use ? v : 1.0;
+    for (int it = 0; it < NE; ++it) {
+      const int e = tid + it * NTH;
+      const int a = e % KM, b = e / KM;
+      const bool in = e < KM * KM && a < k && b < k;
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        const int gg = g < p.skip ? g : g + 1;  // the g-th Gram other than skip
+        gv[it][g] = (in && gg < p.ng) ? p.g[gg][a + (long)k * b] : 1.0;
+      }
     }
-    St[b][a] = (a < k && b < k) ? (float)d : 0.0f;
+#pragma unroll
+    for (int it = 0; it < NE; ++it) {
+      const int e = tid + it * NTH;
+      const int a = e % KM, b = e / KM;
+      double d = gv[it][0];  // in Gram order, as the sequential product
+      d *= gv[it][1];
+      d *= gv[it][2];
+      d *= gv[it][3];
+      for (int gg = p.skip <= 4 ? 5 : 4; gg < p.ng; ++gg)  // more than five modes: the rest
+        if (gg != p.skip) d *= (a < k && b < k) ? p.g[gg][a + (long)k * b] : 1.0;
+      if (e < KM * KM) St[b][a] = (a < k && b < k) ? (float)d : 0.0f;
+    }
   }
   doublex4 gacc[TPW];
 #pragma unroll
   for (int j = 0; j < TPW; ++j) gacc[j] = doublex4{0.0, 0.0, 0.0, 0.0};
-  double dacc = 0.0;
+  double dacc[NDP];
+#pragma unroll
+  for (int j = 0; j < NDP; ++j) dacc[j] = 0.0;
   __syncthreads();
   for (; r0 < p.I; r0 += stride) {
     const long i = r0 + rr;
@@ -349,7 +372,7 @@ __global__ __launch_bounds__(256) void mode_rows_kernel(ModeParams p) {
     // lane%16) and B = U(r0 + lane/16, b0 + lane%16).
 #pragma unroll
     for (int j = 0; j < TPW; ++j) {
-      const int t = wave + 4 * j;
+      const int t = wave + NW * j;
       if (t < NUP) {
         int ta, tb;
         tile_of(t, ta, tb);
@@ -360,12 +383,17 @@ __global__ __launch_bounds__(256) void mode_rows_kernel(ModeParams p) {
           gacc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)ua[r], (double)ub[r], gacc[j], 0, 0, 0);
       }
     }
-    // Column dots sum_r U(r, c) M(c, r): a quad per column, quad sum.
-    if (tid < 4 * KM) {
-      const int c = tid >> 2, part = tid & 3;
-      double sd = 0.0;
-      for (int r = part; r < MR; r += 4) sd += (double)Us[c][r] * (double)Ms[c][r];
-      dacc += sd;
+    // Column dots sum_r U(r, c) M(c, r): pair (c, quarter) = tid + NTH j,
+    // the four quarters of a column in consecutive lanes (quad sum below).
+#pragma unroll
+    for (int j = 0; j < NDP; ++j) {
+      const int pi = tid + NTH * j;
+      if (pi < 4 * KM) {
+        const int c = pi >> 2, part = pi & 3;
+        double sd = 0.0;
+        for (int r = part; r < MR; r += 4) sd += (double)Us[c][r] * (double)Ms[c][r];
+        dacc[j] += sd;
+      }
     }
     __syncthreads();
   }
@@ -373,7 +401,7 @@ __global__ __launch_bounds__(256) void mode_rows_kernel(ModeParams p) {
   // FP64 D layout: lane holds D(lane / 16 + 4 v, lane % 16), v = 0..3.
 #pragma unroll
   for (int j = 0; j < TPW; ++j) {
-    const int t = wave + 4 * j;
+    const int t = wave + NW * j;
     if (t < NUP) {
       int ta, tb;
       tile_of(t, ta, tb);
@@ -388,14 +416,17 @@ __global__ __launch_bounds__(256) void mode_rows_kernel(ModeParams p) {
       }
     }
   }
-  if (tid < 4 * KM) {
-    // fixed-order quad combine: ((d0 + d1) + (d2 + d3))
-    double d1 = __shfl_xor(dacc, 1, 64);
-    double s01 = (tid & 1) ? d1 + dacc : dacc + d1;
+#pragma unroll
+  for (int j = 0; j < NDP; ++j) {
+    // fixed-order quad combine: ((d0 + d1) + (d2 + d3)); every lane of a
+    // quad takes part (the shuffles need all of them).
+    const int pi = tid + NTH * j;
+    double d1 = __shfl_xor(dacc[j], 1, 64);
+    double s01 = (tid & 1) ? d1 + dacc[j] : dacc[j] + d1;
     double s23 = __shfl_xor(s01, 2, 64);
     double tot = (tid & 2) ? s23 + s01 : s01 + s23;
-    const int c = tid >> 2;
-    if ((tid & 3) == 0 && c < k) dst[k * k + c] = tot;
+    const int c = pi >> 2;
+    if (pi < 4 * KM && (pi & 3) == 0 && c < k) dst[k * k + c] = tot;
   }
 }
 
@@ -590,20 +621,22 @@ int mode_update(planc_ctx *ctx, int kind, float eps, float *F, long I, long ldf,
   p.eps = eps;
   p.normX2 = normX2;
   p.rel_out = rel_out;
-  long nb = (I + MR - 1) / MR;
+  const int mr = 64;
+  long nb = (I + mr - 1) / mr;
   if (nb > ctx->num_cu) nb = ctx->num_cu;
   p.part = (double *)ws_get(ctx, 18, sizeof(double) * (size_t)nb * (k * k + k));
   if (!p.part) return set_error(ctx, PLANC_ENOMEM, "mode_update: workspace");
   p.rel_counter = rel_counter;
-#define ROWS(KMV)                                                        \
-  do {                                                                   \
-    if (kind == 0)                                                       \
-      mode_rows_kernel<KMV, 0><<<(unsigned)nb, 256, 0, ctx->stream>>>(p); \
-    else if (kind == 1)                                                  \
-      mode_rows_kernel<KMV, 1><<<(unsigned)nb, 256, 0, ctx->stream>>>(p); \
-    else                                                                 \
-      mode_rows_kernel<KMV, 2><<<(unsigned)nb, 256, 0, ctx->stream>>>(p); \
+#define ROWS_MR(KMV, MRV)                                                          \
+  do {                                                                             \
+    if (kind == 0)                                                                 \
+      mode_rows_kernel<KMV, 0, MRV><<<(unsigned)nb, 4 * MRV, 0, ctx->stream>>>(p); \
+    else if (kind == 1)                                                            \
+      mode_rows_kernel<KMV, 1, MRV><<<(unsigned)nb, 4 * MRV, 0, ctx->stream>>>(p); \
+    else                                                                           \
+      mode_rows_kernel<KMV, 2, MRV><<<(unsigned)nb, 4 * MRV, 0, ctx->stream>>>(p); \
   } while (0)
+#define ROWS(KMV) ROWS_MR(KMV, 64)
   if (k <= 16)
     ROWS(16);
   else if (k <= 32)
@@ -613,6 +646,7 @@ int mode_update(planc_ctx *ctx, int kind, float eps, float *F, long I, long ldf,
   else
     ROWS(64);
 #undef ROWS
+#undef ROWS_MR
   PLANC_LAUNCH_CHECK(ctx);
   long fb = (I * k + 255) / 256;
   if (fb > 1024) fb = 1024;

@@ -292,8 +292,11 @@ def test_nmf_mixed_precision_final_error(ctx, algo, dt):
 
 @pytest.mark.parametrize("algo", [P.MU, P.HALS])
 @pytest.mark.parametrize("dimtree", [True, False])
-@pytest.mark.parametrize("dims,k", [((20, 18, 16), 5), ((12, 10, 9, 8), 4)])
+@pytest.mark.parametrize("dims,k", [((20, 18, 16), 5), ((12, 10, 9, 8), 4), ((1024, 12, 10), 20),
+                                    ((5000, 9, 8), 6)])
 def test_ntf_trajectory(ctx, algo, dimtree, dims, k):
+    """Modes of one to many 64-row blocks (block partials summed in block
+    order), against the FP64 oracle trajectory."""
     say(f"ntf algo={algo} dimtree={dimtree} dims={dims}")
     X, _ = D.cp_tensor(dims, k, 0.01, stream=2)
     X32 = X.astype(np.float32)
