@@ -139,10 +139,15 @@ bool graph_lookup(planc_ctx *ctx, const std::vector<uintptr_t> &key, hipGraphExe
   return false;
 }
 
-void graph_store(planc_ctx *ctx, const std::vector<uintptr_t> &key, hipGraphExec_t exec) {
+// Caches exec under key; at 16 entries the oldest one other than keep is
+// destroyed (no cached graph is in flight: callers store before launching).
+void graph_store(planc_ctx *ctx, const std::vector<uintptr_t> &key, hipGraphExec_t exec,
+                 hipGraphExec_t keep = nullptr) {
   if (ctx->graphs.size() >= 16) {
-    (void)hipGraphExecDestroy(ctx->graphs.front().exec);
-    ctx->graphs.erase(ctx->graphs.begin());
+    auto victim = ctx->graphs.begin();
+    if (victim->exec == keep) ++victim;
+    (void)hipGraphExecDestroy(victim->exec);
+    ctx->graphs.erase(victim);
   }
   ctx->graphs.push_back({key, exec});
 }
@@ -724,28 +729,42 @@ int planc_ntf_run(planc_ctx *ctx, const planc_ntf_desc *desc, const float *d_X, 
   };
   if (iters > 0) RC(one_iteration(rel, nullptr));
   if (iters > 1) {
-    // Iterations 1.. replay a captured graph of one iteration (launch-bound
-    // at small shapes); the error goes through a device counter into rel[1..].
+    // Iterations 1.. replay a captured graph of GU iterations (at small
+    // shapes an iteration is a few tens of microseconds and every graph
+    // launch costs several); the remainder runs eagerly.  The graph is
+    // captured (not run) on the first call with iters > 1 whatever its
+    // length, so later calls of any length find it cached.  The error goes
+    // through a device counter into rel[1..].
     int *counter = (int *)ws_get(ctx, 15, sizeof(int));
     if (!counter) return set_error(ctx, PLANC_ENOMEM, "planc_ntf_run: workspace");
     RC(counter_set(ctx, counter, 1));
-    hipGraphExec_t exec = nullptr;
-    std::vector<uintptr_t> key = {1, (uintptr_t)N, (uintptr_t)k, (uintptr_t)algo, (uintptr_t)dt,
-                                  (uintptr_t)iters,
-                                  (uintptr_t)dimtree, (uintptr_t)desc->compute_error,
-                                  (uintptr_t)Xop, (uintptr_t)d_lambda, (uintptr_t)rel};
-    for (int i = 0; i < N; ++i) {
-      key.push_back((uintptr_t)d_factors[i]);
-      key.push_back((uintptr_t)dims[i]);
-    }
-    for (auto &w : ctx->ws) key.push_back((uintptr_t)w.ptr);
-    if (!graph_lookup(ctx, key, &exec)) {
-      RC(capture_graph(ctx, [&]() -> int {
-        return one_iteration(rel, counter);
-      }, &exec));
-      graph_store(ctx, key, exec);
-    }
-    for (int it = 1; it < iters; ++it) PLANC_HIP_CHECK(ctx, hipGraphLaunch(exec, ctx->stream));
+    const int GU = 8;
+    const int rest = iters - 1, full = rest / GU, rem = rest % GU;
+    auto graph_of = [&](int n_it, hipGraphExec_t *exec, hipGraphExec_t keep) -> int {
+      std::vector<uintptr_t> key = {1, (uintptr_t)N, (uintptr_t)k, (uintptr_t)algo, (uintptr_t)dt,
+                                    (uintptr_t)n_it, (uintptr_t)dimtree, (uintptr_t)desc->compute_error,
+                                    (uintptr_t)Xop, (uintptr_t)d_lambda, (uintptr_t)rel};
+      for (int i = 0; i < N; ++i) {
+        key.push_back((uintptr_t)d_factors[i]);
+        key.push_back((uintptr_t)dims[i]);
+      }
+      for (auto &w : ctx->ws) key.push_back((uintptr_t)w.ptr);
+      if (graph_lookup(ctx, key, exec)) return PLANC_OK;
+      int rc2 = capture_graph(ctx, [&]() -> int {
+        for (int u = 0; u < n_it; ++u) {
+          const int r = one_iteration(rel, counter);
+          if (r) return r;
+        }
+        return PLANC_OK;
+      }, exec);
+      if (rc2) return rc2;
+      graph_store(ctx, key, *exec, keep);
+      return PLANC_OK;
+    };
+    hipGraphExec_t exec_full = nullptr;
+    RC(graph_of(GU, &exec_full, nullptr));
+    for (int it = 0; it < full; ++it) PLANC_HIP_CHECK(ctx, hipGraphLaunch(exec_full, ctx->stream));
+    for (int it = 0; it < rem; ++it) RC(one_iteration(rel, counter));
   }
 #undef RC
   int hflag = 0;
