@@ -517,6 +517,115 @@ __global__ __launch_bounds__(NTHR) void tn_f32_kernel(StreamParams p) {
   if (finish_tile<NA>(p, acc, red)) epilogue_tn<NT, CW>(p, acc, l0);
 }
 
+// FP32 TN with 32-k steps: lane (lc, lg) reads A(kb + 8lg .. + 7, l0 + 16cw +
+// lc) as two 16-byte words, so the four lane groups cover a whole 128-byte
+// line of each column per step (the 16-k kernel reads half lines, two steps
+// apart).  Half h feeds the MFMAs with k-slot g <-> kb + 8g + 4h + j and the
+// thin words B(kb + 8lg + 4h + 0..3, c).  Fast thin modes only.
+template <int NT, int CW>
+struct BufTN2 {
+  floatx4 x[CW][2];
+  floatx4 a[NT][2];
+  float sc[NT][2];
+};
+
+template <int NT, int CW, int TM>
+__global__ __launch_bounds__(NTHR) void tn_f32d_kernel(StreamParams p) {
+  constexpr int STEP = 32, PD = 2, NA = CW * NT;
+  __shared__ floatx4 red[3][NA][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lc = lane & 15, lg = lane >> 4;
+  const int grp = wave % p.wgrp, kph = wave / p.wgrp;
+  const long l0 = ((long)blockIdx.x * p.wgrp + grp) * (16 * CW);
+  const long kbeg = (long)blockIdx.y * p.kchunk;
+  const long kend = min(p.K, kbeg + p.kchunk);
+  const long nfull = kend > kbeg ? (kend - kbeg) / STEP : 0;
+  const float *A = (const float *)p.A;
+  const float *col[CW];
+#pragma unroll
+  for (int cw = 0; cw < CW; ++cw) {
+    const long l = l0 + 16 * cw + lc;
+    col[cw] = A + p.lda * (l < p.L ? l : p.L - 1);
+  }
+  const float *tb[NT], *tf1[NT];
+  thin_cols<NT>(p.th, tb, tf1);
+  floatx4 acc[NA];
+#pragma unroll
+  for (int a = 0; a < NA; ++a) acc[a] = floatx4{0.f, 0.f, 0.f, 0.f};
+  auto mma = [&](BufTN2<NT, CW> &b) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      floatx4 ah[NT];
+      float sh[NT];
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        ah[t] = b.a[t][h];
+        sh[t] = b.sc[t][h];
+      }
+      thin_apply<NT, TM>(ah, sh);
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+#pragma unroll
+          for (int cw = 0; cw < CW; ++cw) acc[cw * NT + t] = mfma4(ah[t][j], b.x[cw][h][j], acc[cw * NT + t]);
+    }
+  };
+  auto load = [&](long i, BufTN2<NT, CW> &b) {
+    const long kb = kbeg + (kph + i * p.wk) * STEP + 8 * lg;
+#pragma unroll
+    for (int cw = 0; cw < CW; ++cw) {
+      b.x[cw][0] = *(const floatx4 *)(col[cw] + kb);
+      b.x[cw][1] = *(const floatx4 *)(col[cw] + kb + 4);
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      floatx4 ah[NT];
+      float sh[NT];
+      thin_step<NT, TM>(p.th, kb + 4 * h, tb, tf1, ah, sh);
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        b.a[t][h] = ah[t];
+        b.sc[t][h] = sh[t];
+      }
+    }
+  };
+  BufTN2<NT, CW> buf[PD];
+  const long nsw = nfull > kph ? (nfull - kph + p.wk - 1) / p.wk : 0;
+  stream_steps<PD>(nsw, buf, load, mma);
+  if (kend - kbeg > nfull * STEP && nfull % p.wk == kph) {
+    // The partial last step, masked.
+    BufTN2<NT, CW> b;
+    const long kb = kbeg + nfull * STEP + 8 * lg;
+#pragma unroll
+    for (int cw = 0; cw < CW; ++cw)
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const long kk = kb + 4 * h + u;
+          b.x[cw][h][u] = kk < kend ? col[cw][kk] : 0.0f;
+        }
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        b.a[t][h] = thin4(p.th, kb + 4 * h, kend, 16 * t + lc);
+        b.sc[t][h] = 1.0f;
+      }
+    // thin4 already applied the KRP scale: multiply by one.
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+#pragma unroll
+          for (int cw = 0; cw < CW; ++cw) acc[cw * NT + t] = mfma4(b.a[t][h][j], b.x[cw][h][j], acc[cw * NT + t]);
+  }
+  if (finish_tile<NA>(p, acc, red)) epilogue_tn<NT, CW>(p, acc, l0);
+}
+
 // ---------------------------------------------------------------------------
 // Packs the thin operand into the 16-bit MFMA A-operand layout, in 64-k
 // super-steps S split into two halves h: for lane (lc, lg),
@@ -1253,10 +1362,10 @@ Geometry make_geometry(long L, long K, int trans, int dtype, int NT, int cw, int
   g.shared = shared;
   if (shared) {
     wk = 1;
-    if (trans && dtype != PLANC_F32) dual = 1;
+    dual = (trans && dtype != PLANC_F32) ? 1 : 0;
   }
-  g.dual = (trans && dtype != PLANC_F32) ? dual : 0;
-  const int step = dtype == PLANC_F32 ? 16 : (g.dual ? 64 : 32);
+  g.dual = trans ? dual : 0;
+  const int step = dtype == PLANC_F32 ? (g.dual ? 32 : 16) : (g.dual ? 64 : 32);
   g.cw = trans ? cw : 1;
   g.groups = trans ? (L + 16 * cw - 1) / (16 * cw) : (L + 63) / 64;
   g.tile_bytes = trans ? (long)cw * NT * 64 * 16 : 4L * NT * 64 * 16;
@@ -1277,7 +1386,7 @@ Geometry make_geometry(long L, long K, int trans, int dtype, int NT, int cw, int
 // and the grid stays under 16k waves.  The first entry is the static choice
 // used when tuning is off or impossible (graph capture).
 std::vector<Geometry> candidate_geometries(const planc_ctx *ctx, long L, long K, int trans, int dtype,
-                                           int NT, bool shared_ok) {
+                                           int NT, bool shared_ok, bool f32dual_ok) {
   std::vector<Geometry> out;
   const int step = dtype == PLANC_F32 ? 16 : 32;
   const long ksteps = (K + step - 1) / step;
@@ -1308,7 +1417,8 @@ std::vector<Geometry> candidate_geometries(const planc_ctx *ctx, long L, long K,
     }
     out.push_back(g);
   }
-  const int ndual = (trans && dtype != PLANC_F32) ? 2 : 1;
+  // dual: 64-k (16-bit) or 32-k (FP32, fast thin modes) steps, whole 128-byte lines per column.
+  const int ndual = (trans && (dtype != PLANC_F32 || f32dual_ok)) ? 2 : 1;
   for (int dual = 0; dual < ndual; ++dual)
   for (int cw : cws)
     for (int wk = 1; wk <= 4; wk *= 2) {
@@ -1350,8 +1460,12 @@ int thin_mode(const ThinDev &th, bool avec, const float *rowscale) {
 }
 
 template <int NT, int CW>
-void launch_tn_f32(int tm, dim3 grid, hipStream_t st, const StreamParams &p) {
-  if (tm == THIN_PLAIN)
+void launch_tn_f32(int tm, int dual, dim3 grid, hipStream_t st, const StreamParams &p) {
+  if (dual && tm == THIN_PLAIN)
+    tn_f32d_kernel<NT, CW, THIN_PLAIN><<<grid, NTHR, 0, st>>>(p);
+  else if (dual)
+    tn_f32d_kernel<NT, CW, THIN_KRP2><<<grid, NTHR, 0, st>>>(p);
+  else if (tm == THIN_PLAIN)
     tn_f32_kernel<NT, CW, THIN_PLAIN><<<grid, NTHR, 0, st>>>(p);
   else if (tm == THIN_KRP2)
     tn_f32_kernel<NT, CW, THIN_KRP2><<<grid, NTHR, 0, st>>>(p);
@@ -1446,9 +1560,9 @@ int launch_stream(planc_ctx *ctx, int dtype, int trans, int NT, int tm, StreamPa
 #define TNF(NTV)                                        \
   do {                                                  \
     if (g.cw == 4)                                      \
-      launch_tn_f32<NTV, 4>(tm, grid, st, p);           \
+      launch_tn_f32<NTV, 4>(tm, g.dual, grid, st, p);   \
     else                                                \
-      launch_tn_f32<NTV, 2>(tm, grid, st, p);           \
+      launch_tn_f32<NTV, 2>(tm, g.dual, grid, st, p);   \
   } while (0)
     if (NT == 1) TNF(1);
     else if (NT == 2) TNF(2);
@@ -1513,11 +1627,12 @@ bool stream_capturing(hipStream_t st) {
 int pick_geometry(planc_ctx *ctx, int dtype, int trans, int NT, int tm, const StreamParams &p,
                   Geometry *out) {
   // PLANC_STREAM_VARIANT restricts the tuner to one kernel family (used to
-  // test each path): "plain" (per-wave thin operand) or "shared" (block-shared
-  // LDS thin operand, where it applies); unset: both.
+  // test each path): "plain" (per-wave thin operand, 16/32-k steps), "dual"
+  // (per-wave, 32-k FP32 / 64-k 16-bit TN steps) or "shared" (block-shared
+  // LDS thin operand), each where it applies; unset: all.
   const char *venv = getenv("PLANC_STREAM_VARIANT");
   const std::string vs = venv ? venv : "";
-  const int smode = vs == "plain" ? 0 : (vs == "shared" ? 1 : 2);
+  const int smode = vs == "plain" ? 0 : (vs == "shared" ? 1 : (vs == "dual" ? 3 : 2));
   const std::vector<long> key = {dtype, trans, NT, tm, p.L, p.K, p.lda, p.th.k, p.avec, p.cmajor, smode};
   for (const auto &e : ctx->gemm_tuned)
     if (e.first == key) {
@@ -1525,13 +1640,14 @@ int pick_geometry(planc_ctx *ctx, int dtype, int trans, int NT, int tm, const St
       *out = make_geometry(p.L, p.K, trans, dtype, NT, (int)v[0], (int)v[1], v[2], (int)v[3], (int)v[4]);
       return PLANC_OK;
     }
-  const bool shared_ok = (dtype == PLANC_F32 ? tm != THIN_SAFE : (trans && p.avec)) && smode != 0;
-  std::vector<Geometry> cands = candidate_geometries(ctx, p.L, p.K, trans, dtype, NT, shared_ok);
-  if (shared_ok && smode == 1) {
-    std::vector<Geometry> sh;
+  const bool shared_ok = (dtype == PLANC_F32 ? tm != THIN_SAFE : (trans && p.avec)) && (smode == 1 || smode == 2);
+  std::vector<Geometry> cands =
+      candidate_geometries(ctx, p.L, p.K, trans, dtype, NT, shared_ok, dtype == PLANC_F32 && tm != THIN_SAFE);
+  if (smode != 2) {
+    std::vector<Geometry> sel;
     for (const Geometry &g : cands)
-      if (g.shared) sh.push_back(g);
-    if (!sh.empty()) cands.swap(sh);
+      if (smode == 1 ? g.shared != 0 : (g.shared == 0 && (smode == 3) == (g.dual != 0))) sel.push_back(g);
+    if (!sel.empty()) cands.swap(sel);
   }
   const char *env = getenv("PLANC_AUTOTUNE");
   const bool tune = !(env && env[0] == '0') && !stream_capturing(ctx->stream) && cands.size() > 1;

@@ -357,7 +357,7 @@ __global__ __launch_bounds__(4 * MRT) void mode_rows_kernel(ModeParams p) {
   for (; r0 < p.I; r0 += stride) {
     const long i = r0 + rr;
     const bool valid = i < p.I;
-    quad_row_update<KM, true>(h, mv, St, q, k, KIND, p.eps);
+    quad_row_update<KM, false>(h, mv, St, q, k, KIND, p.eps);
 #pragma unroll
     for (int u = 0; u < Q; ++u) {
       const int t = q * Q + u;

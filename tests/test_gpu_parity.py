@@ -73,12 +73,13 @@ GEMM_CASES = [
 ]
 
 
-@pytest.mark.parametrize("variant", ["plain", "shared"])
+@pytest.mark.parametrize("variant", ["plain", "dual", "shared"])
 @pytest.mark.parametrize("dt", [P.F32, P.BF16, P.F16])
 @pytest.mark.parametrize("L,K,k", GEMM_CASES)
 def test_gemm_nn_tn(ctx, monkeypatch, dt, L, K, k, variant):
-    """Both streaming families: per-wave thin operand and the block-shared
-    LDS ring (where it applies; otherwise the tuner's other candidates run)."""
+    """Every streaming family: per-wave thin operand with 16/32-k steps, with
+    32-k (FP32) / 64-k (16-bit) TN steps, and the block-shared LDS ring
+    (each where it applies; otherwise the tuner's other candidates run)."""
     monkeypatch.setenv("PLANC_STREAM_VARIANT", variant)
     say(f"gemm dt={dt} L={L} K={K} k={k} variant={variant}")
     rng = np.random.default_rng(L * 7 + K + k)
@@ -209,7 +210,7 @@ def test_hals_cols_sweep(ctx, m, k):
     assert rel_err(dW.cpu().numpy(), Wr) < 2e-5
 
 
-@pytest.mark.parametrize("variant", ["plain", "shared"])
+@pytest.mark.parametrize("variant", ["plain", "dual", "shared"])
 @pytest.mark.parametrize("dims,k", [((7, 8, 9), 3), ((20, 18, 16), 5), ((5, 6, 4, 3), 2),
                                     ((33, 17, 20, 9), 16), ((64, 64, 64), 32), ((3, 4, 5, 6, 2), 4),
                                     ((16, 32, 24, 40), 48)])
@@ -409,7 +410,7 @@ def test_partial_mttkrp_and_multi_ttv(ctx):
             assert rel_err(M.cpu().numpy(), ref) < 1e-5, (s, n)
 
 
-@pytest.mark.parametrize("variant", ["plain", "shared"])
+@pytest.mark.parametrize("variant", ["plain", "dual", "shared"])
 @pytest.mark.parametrize("dt", [P.F32, P.BF16, P.F16])
 def test_partial_mttkrp_streaming_paths(ctx, monkeypatch, dt, variant):
     """Right (X_(L x R) * KRP_R) and left (KRP_L^T * X) partial contractions
