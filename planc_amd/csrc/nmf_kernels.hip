@@ -241,12 +241,14 @@ __global__ void zero_int_kernel(int *p) { *p = 0; }
 // squares (block partials summed in block order by every block, so every
 // block derives the identical norm).  sync[0] is the barrier counter (zero
 // at launch), sync[1] the time-out flag.
-template <int KM, int RPT>
-__global__ __launch_bounds__(NTHR) void hals_cols_coop_kernel(float *W, long m, int k, long ldw,
-                                                              const float *AH, long ldah,
-                                                              const double *G, double *part, int *sync) {
+// BS: threads per block; 1024-thread blocks make four times fewer barrier
+// participants (the per-column counter atomics serialise at one address).
+template <int KM, int RPT, int BS>
+__global__ __launch_bounds__(BS) void hals_cols_coop_kernel(float *W, long m, int k, long ldw,
+                                                            const float *AH, long ldah,
+                                                            const double *G, double *part, int *sync) {
   __shared__ __attribute__((aligned(16))) float Gs[KM][KM];
-  __shared__ double red[NTHR / 64];
+  __shared__ double red[BS / 64];
   load_gram<KM>(Gs, G, k);
   const long total = (long)gridDim.x * blockDim.x;
   const long gtid = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -257,6 +259,15 @@ __global__ __launch_bounds__(NTHR) void hals_cols_coop_kernel(float *W, long m, 
     long i = gtid + r * total;
 #pragma unroll
     for (int t = 0; t < KM; ++t) w[r][t] = (i < m && t < k) ? W[i + ldw * t] : 0.0f;
+  }
+  // AH(i, x) of the next column is loaded one column ahead, so its latency
+  // hides behind the current column's barrier instead of stalling the
+  // column update.
+  float ahn[RPT];
+#pragma unroll
+  for (int r = 0; r < RPT; ++r) {
+    long i = gtid + r * total;
+    ahn[r] = i < m ? AH[i] : 0.0f;
   }
   // The column loop is not unrolled (it holds a grid barrier); registers are
   // only indexed by the unrolled t loops, w[r][x] is read and written through
@@ -273,8 +284,10 @@ __global__ __launch_bounds__(NTHR) void hals_cols_coop_kernel(float *W, long m, 
 #pragma unroll
       for (int t = 0; t < KM; ++t) wx = (t == x) ? w[r][t] : wx;
       const float s = dot_gcol<KM>(w[r], Gs, x, 0.0f);
+      const float ax = ahn[r];
+      if (i < m && x + 1 < k) ahn[r] = AH[i + ldah * (x + 1)];
       if (i < m) {
-        float v = (wx * gxx) + (AH[i + ldah * x] - s);
+        float v = (wx * gxx) + (ax - s);
         v = (v < floor_v) ? floor_v : v;
         u[r] = v;
         lsum += (double)v * (double)v;
@@ -609,16 +622,16 @@ int launch_rows(planc_ctx *ctx, int kind, float *H, long n, int k, long ldh, con
   return PLANC_OK;
 }
 
-template <int KM, int RPT>
+template <int KM, int RPT, int BS>
 int try_coop(planc_ctx *ctx, float *W, long m, int k, long ldw, const float *AH, long ldah,
              const double *G, bool *done) {
   *done = false;
   int per_cu = 0;
   hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      &per_cu, (const void *)hals_cols_coop_kernel<KM, RPT>, NTHR, 0);
+      &per_cu, (const void *)hals_cols_coop_kernel<KM, RPT, BS>, BS, 0);
   if (e != hipSuccess || per_cu <= 0) return PLANC_OK;
   long maxblocks = (long)per_cu * ctx->num_cu;
-  long need = (m + (long)NTHR * RPT - 1) / ((long)NTHR * RPT);
+  long need = (m + (long)BS * RPT - 1) / ((long)BS * RPT);
   if (need > maxblocks || need <= 0) return PLANC_OK;
   unsigned nb = (unsigned)need;
   double *part = (double *)ws_get(ctx, 2, sizeof(double) * (size_t)KM * nb);
@@ -627,8 +640,8 @@ int try_coop(planc_ctx *ctx, float *W, long m, int k, long ldw, const float *AH,
   zero_int_kernel<<<1, 1, 0, ctx->stream>>>(sync);
   PLANC_LAUNCH_CHECK(ctx);
   void *args[] = {&W, &m, &k, &ldw, &AH, &ldah, &G, &part, &sync};
-  hipError_t le = hipLaunchCooperativeKernel((const void *)hals_cols_coop_kernel<KM, RPT>, dim3(nb),
-                                             dim3(NTHR), args, 0, ctx->stream);
+  hipError_t le = hipLaunchCooperativeKernel((const void *)hals_cols_coop_kernel<KM, RPT, BS>, dim3(nb),
+                                             dim3(BS), args, 0, ctx->stream);
   if (le != hipSuccess) {
     // The runtime can refuse a cooperative launch (e.g. while another
     // process holds queues on the device); nothing was launched, the
@@ -644,10 +657,15 @@ template <int KM>
 int hals_cols_impl(planc_ctx *ctx, float *W, long m, int k, long ldw, const float *AH, long ldah,
                    const double *G) {
   bool done = false;
-  int rc = try_coop<KM, 1>(ctx, W, m, k, ldw, AH, ldah, G, &done);
+  int rc = PLANC_OK;
+  if constexpr (KM <= 48) {  // at k = 64 a 1024-thread block leaves too few registers per row
+    rc = try_coop<KM, 1, 1024>(ctx, W, m, k, ldw, AH, ldah, G, &done);
+    if (rc || done) return rc;
+  }
+  rc = try_coop<KM, 1, NTHR>(ctx, W, m, k, ldw, AH, ldah, G, &done);
   if (rc || done) return rc;
   if (KM <= 32) {
-    rc = try_coop<KM, 2>(ctx, W, m, k, ldw, AH, ldah, G, &done);
+    rc = try_coop<KM, 2, NTHR>(ctx, W, m, k, ldw, AH, ldah, G, &done);
     if (rc || done) return rc;
   }
   // Fallback: two launches per column.
