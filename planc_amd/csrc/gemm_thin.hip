@@ -25,16 +25,22 @@
 //     l0 + lc) -- exactly the MFMA B-operand layout; the thin operand is
 //     pre-packed once per product into the A-operand layout (16-bit, 16 B
 //     per lane per 16 columns per 32 k) by pack_thin16_kernel.
-// A wave owns 16*CW columns (TN) or 64 rows (NN).  The four waves of a block
-// either take different column/row groups over the same k range (the thin
-// operand then hits in L1 for three of them) or split the k range of one
-// group in an interleaved order and are summed in LDS in fixed order.  When
-// that still leaves too few waves, blockIdx.y splits K across blocks: each
-// block writes its partial tile and a second kernel sums the partials in
-// split order (deterministic, no atomics).
+//   16-bit NN: lane reads the words A(l0 + 8lc .. +7, kb + 8lg + u), u =
+//     0..7, and transposes that 8 x 8 block in registers into eight MFMA
+//     A-operands (row slot lc <-> row l0 + 8lc + j); the packed thin operand
+//     is the B-operand.
+// A wave owns 16*CW columns (TN), 64 rows (FP32 NN) or 128 rows (16-bit NN).
+// The four waves of a block either take different column/row groups over the
+// same k range or split the k range of one group in an interleaved order and
+// are summed in LDS in fixed order.  When that still leaves too few waves,
+// blockIdx.y splits K across blocks: each block writes its partial tile and a
+// second kernel sums the partials in split order (deterministic, no atomics).
+// Variants chosen per shape by the tuner: 32-k FP32 / 64-k 16-bit TN steps
+// (whole 128-byte lines per column per step) and the block-shared thin
+// operand (an LDS ring filled once per block).
 //
-// 16-bit NN (a standalone product on a 16-bit copy without its transpose)
-// uses an LDS-transposing kernel with a separate split-K reduction.
+// 16-bit NN with rows not in 16-byte groups (L % 8 != 0 or unaligned) uses an
+// LDS-transposing kernel with a separate split-K reduction.
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -254,6 +260,24 @@ __device__ inline void store_acc_nn(const StreamParams &p, long l0, int a, float
   }
 }
 
+// Element (l, c) of a 16-bit NN accumulator a = 8t + j of the group at row
+// l0: v[u] = C(l0 + 32lg + 8u + j, 16t + lc).
+__device__ inline void store_acc_nn16(const StreamParams &p, long l0, int a, floatx4 v) {
+  const int lane = threadIdx.x & 63, lc = lane & 15, lg = lane >> 4;
+  const int t = a / 8, j = a % 8;
+  const int c = 16 * t + lc;
+  if (c >= p.th.k) return;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const long l = l0 + 32 * lg + 8 * u + j;
+    if (l >= p.L) continue;
+    if (p.cmajor)
+      p.C[c + p.ldc * l] = v[u];
+    else
+      p.C[l + p.ldc * c] = v[u];
+  }
+}
+
 template <int NT, int CW>
 __device__ inline void epilogue_tn(const StreamParams &p, floatx4 (&acc)[CW * NT], long l0) {
 #pragma unroll
@@ -349,8 +373,10 @@ __global__ __launch_bounds__(256) void splitk_reduce_kernel(StreamParams p, int 
   t += red[3][lane];
   if (KIND == 0)
     store_acc_tn<NT>(p, g * (16 * CW), a, t);
-  else
+  else if (KIND == 1)
     store_acc_nn(p, g * 64, a, t);
+  else
+    store_acc_nn16(p, g * 128, a, t);
 }
 
 // ---------------------------------------------------------------------------
@@ -1178,6 +1204,119 @@ __global__ __launch_bounds__(NTHR) void tn_h_sh_kernel(StreamParams p) {
 }
 
 // ---------------------------------------------------------------------------
+// Streaming 16-bit NN (A(L x K) L-contiguous, L % 8 == 0, 16-byte aligned
+// columns).  A wave owns 128 rows: lane (lc, lg) reads the 16-byte words
+// A(l0 + 8lc .. + 7, kb + 8lg + u), u = 0..7 (per load instruction four
+// 256-byte column segments), and transposes its 8 x 8 block of 16-bit values
+// in registers, so word j holds A(l0 + 8lc + j, kb + 8lg .. + 7): the MFMA
+// A-operand with row slot lc <-> row l0 + 8lc + j.  The thin operand is the
+// packed 16-bit B-operand (pack_thin16_kernel, as for 16-bit TN).  acc[8t +
+// j][v] = C(l0 + 32lg + 8v + j, 16t + lc): every lane ends with 32
+// consecutive rows of its column.  One k phase per group (wk == 1).
+template <int NT>
+struct BufNN16 {
+  uint4 w[8];
+  uint4 b[NT];
+};
+
+__device__ inline uint4 transpose8_word(const uint4 (&w)[8], int j) {
+  // halves j of w[0..7] -> one word (element u from w[u])
+  const int sh = 16 * (j & 1);
+  auto pick = [&](int u) -> unsigned {
+    const unsigned d = (j >> 1) == 0 ? w[u].x : (j >> 1) == 1 ? w[u].y : (j >> 1) == 2 ? w[u].z : w[u].w;
+    return (d >> sh) & 0xFFFFu;
+  };
+  uint4 o;
+  o.x = pick(0) | (pick(1) << 16);
+  o.y = pick(2) | (pick(3) << 16);
+  o.z = pick(4) | (pick(5) << 16);
+  o.w = pick(6) | (pick(7) << 16);
+  return o;
+}
+
+template <typename TA, int NT>
+__global__ __launch_bounds__(NTHR) void nn16_kernel(StreamParams p) {
+  constexpr int STEP = 32, PD = 2, NA = 8 * NT;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lc = lane & 15, lg = lane >> 4;
+  const long l0 = ((long)blockIdx.x * 4 + wave) * 128;
+  const long kbeg = (long)blockIdx.y * p.kchunk;
+  const long kend = min(p.K, kbeg + p.kchunk);
+  const long nfull = kend > kbeg ? (kend - kbeg) / STEP : 0;
+  const uint16_t *A = (const uint16_t *)p.A;
+  const long r0 = l0 + 8 * lc;  // first of this lane's 8 rows (L % 8 == 0)
+  const uint16_t *base = A + (r0 < p.L ? r0 : p.L - 8);
+  floatx4 acc[NA];
+#pragma unroll
+  for (int a = 0; a < NA; ++a) acc[a] = floatx4{0.f, 0.f, 0.f, 0.f};
+  auto mma = [&](BufNN16<NT> &b) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const uint4 o = transpose8_word(b.w, j);
+#pragma unroll
+      for (int t = 0; t < NT; ++t) acc[8 * t + j] = mfma16<TA>(o, b.b[t], acc[8 * t + j]);
+    }
+  };
+  auto load = [&](long i, BufNN16<NT> &b) {
+    const long kb = kbeg + i * STEP + 8 * lg;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) b.w[u] = *(const uint4 *)(base + p.lda * (kb + u));
+#pragma unroll
+    for (int t = 0; t < NT; ++t) b.b[t] = p.thin16[pack_index(kb, t, lc, NT)];
+  };
+  BufNN16<NT> buf[PD];
+  stream_steps<PD>(nfull, buf, load, mma);
+  if (kend - kbeg > nfull * STEP) {
+    // The partial last step, masked (element loads; the packed thin operand
+    // is zero past K).
+    BufNN16<NT> b;
+    const long kb = kbeg + nfull * STEP + 8 * lg;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      uint16_t v[8];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) v[e] = kb + u < kend ? base[e + p.lda * (kb + u)] : (uint16_t)0;
+      b.w[u] = __builtin_bit_cast(uint4, v);
+    }
+#pragma unroll
+    for (int t = 0; t < NT; ++t) b.b[t] = p.thin16[pack_index(kb, t, lc, NT)];
+    mma(b);
+  }
+  if (p.nsplit > 1) {
+    floatx4 *dst = (floatx4 *)p.part + (((long)blockIdx.y * gridDim.x + blockIdx.x) * 4 + wave) * (long)NA * 64;
+#pragma unroll
+    for (int a = 0; a < NA; ++a) dst[a * 64 + lane] = acc[a];
+    return;
+  }
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int c = 16 * t + lc;
+    if (c >= p.th.k) continue;
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+#pragma unroll
+      for (int hf = 0; hf < 2; ++hf) {
+        const long l = l0 + 32 * lg + 8 * v + 4 * hf;
+        const floatx4 o = {acc[8 * t + 4 * hf + 0][v], acc[8 * t + 4 * hf + 1][v], acc[8 * t + 4 * hf + 2][v],
+                           acc[8 * t + 4 * hf + 3][v]};
+        if (!p.cmajor && p.cvec && l + 3 < p.L) {
+          *(floatx4 *)(p.C + l + p.ldc * c) = o;
+        } else {
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            if (l + e >= p.L) continue;
+            if (p.cmajor)
+              p.C[c + p.ldc * (l + e)] = o[e];
+            else
+              p.C[l + e + p.ldc * c] = o[e];
+          }
+        }
+      }
+  }
+}
+
+
+// ---------------------------------------------------------------------------
 // 16-bit NN: LDS-transposing kernel (A(L x K) is L-contiguous, the 16-bit
 // MFMA wants 8 consecutive k per lane).  Block = 128 rows x all k columns;
 // K-chunks of 64 staged through LDS (double buffered), 8x4 register blocks
@@ -1364,11 +1503,13 @@ Geometry make_geometry(long L, long K, int trans, int dtype, int NT, int cw, int
     wk = 1;
     dual = (trans && dtype != PLANC_F32) ? 1 : 0;
   }
+  const bool nn16 = !trans && dtype != PLANC_F32;  // streaming 16-bit NN: 128 rows per wave, wk 1
+  if (nn16) wk = 1;
   g.dual = trans ? dual : 0;
   const int step = dtype == PLANC_F32 ? (g.dual ? 32 : 16) : (g.dual ? 64 : 32);
   g.cw = trans ? cw : 1;
-  g.groups = trans ? (L + 16 * cw - 1) / (16 * cw) : (L + 63) / 64;
-  g.tile_bytes = trans ? (long)cw * NT * 64 * 16 : 4L * NT * 64 * 16;
+  g.groups = trans ? (L + 16 * cw - 1) / (16 * cw) : (nn16 ? (L + 127) / 128 : (L + 63) / 64);
+  g.tile_bytes = trans ? (long)cw * NT * 64 * 16 : (nn16 ? 8L : 4L) * NT * 64 * 16;
   g.wk = wk;
   g.wgrp = 4 / wk;
   g.blocks_x = (g.groups + g.wgrp - 1) / g.wgrp;
@@ -1392,6 +1533,19 @@ std::vector<Geometry> candidate_geometries(const planc_ctx *ctx, long L, long K,
   const long ksteps = (K + step - 1) / step;
   const int esz = dtype == PLANC_F32 ? 4 : 2;
   const double abytes = (double)L * (double)K * esz;
+  if (!trans && dtype != PLANC_F32) {
+    // Streaming 16-bit NN: one k phase, K splits only.
+    for (long ns = 1; ns <= 4096; ns *= 2) {
+      Geometry g = make_geometry(L, K, trans, dtype, NT, 1, 1, ns);
+      if (ns > 1 && (g.nsplit < ns || ksteps / g.nsplit < 8)) break;
+      if (ns > 1 && 2.0 * g.nsplit * g.groups * g.tile_bytes > 0.10 * abytes) break;
+      const long waves = g.blocks_x * 4 * g.nsplit;
+      if (ns > 1 && waves > 16L * 1024) break;
+      if (!out.empty() && waves < 2L * ctx->num_cu && ns < 4096) continue;
+      out.push_back(g);
+    }
+    return out;
+  }
   std::vector<int> cws;
   if (!trans)
     cws = {1};
@@ -1530,6 +1684,27 @@ int launch_stream(planc_ctx *ctx, int dtype, int trans, int NT, int tm, StreamPa
   }
   dim3 grid((unsigned)g.blocks_x, (unsigned)g.nsplit);
   hipStream_t st = ctx->stream;
+  if (!trans && dtype != PLANC_F32) {
+#define NN16(TA)                                                    \
+  do {                                                              \
+    if (NT == 1) nn16_kernel<TA, 1><<<grid, NTHR, 0, st>>>(p);      \
+    else if (NT == 2) nn16_kernel<TA, 2><<<grid, NTHR, 0, st>>>(p); \
+    else if (NT == 3) nn16_kernel<TA, 3><<<grid, NTHR, 0, st>>>(p); \
+    else nn16_kernel<TA, 4><<<grid, NTHR, 0, st>>>(p);              \
+  } while (0)
+    if (dtype == PLANC_BF16)
+      NN16(bf16_t);
+    else
+      NN16(f16_t);
+#undef NN16
+    PLANC_LAUNCH_CHECK(ctx);
+    if (g.nsplit > 1) {
+      const int NA = 8 * NT;
+      splitk_reduce_kernel<2, 1, 1><<<dim3((unsigned)gpad, (unsigned)NA), 256, 0, st>>>(p, NA, gpad);
+      PLANC_LAUNCH_CHECK(ctx);
+    }
+    return PLANC_OK;
+  }
   if (g.shared && trans && dtype == PLANC_F32) {
 #define TNS(NTV)                                        \
   do {                                                  \
@@ -1748,8 +1923,9 @@ int thin_gemm(planc_ctx *ctx, int dtype, int trans, const void *A, long lda, lon
                                             ctx->stream));
       continue;
     }
-    if (dtype != PLANC_F32 && !trans) {
-      // 16-bit NN through the LDS-transposing kernel.
+    if (dtype != PLANC_F32 && !trans && !(avec && L % 8 == 0)) {
+      // 16-bit NN with rows that do not come in 16-byte groups: the
+      // LDS-transposing kernel.
       GemmParams p{};
       p.A = A;
       p.lda = lda;
@@ -1823,8 +1999,9 @@ int thin_gemm(planc_ctx *ctx, int dtype, int trans, const void *A, long lda, lon
     p.cvec = (ldc % 4 == 0 && ((uintptr_t)Cs % 16 == 0)) ? 1 : 0;
     int tm = thin_mode(th, avec, rowscale);
     if (!trans && L % 4 != 0) tm = THIN_SAFE;
-    if (trans && dtype != PLANC_F32) {
-      // 16-bit TN: the thin operand packed once into the MFMA A-operand layout.
+    if (dtype != PLANC_F32) {
+      // 16-bit TN / NN: the thin operand packed once into the 16-bit MFMA
+      // operand layout (8 consecutive k of one column per lane).
       const long nsuper = (K + 63) / 64;
       uint4 *pk = (uint4 *)ws_get(ctx, 6, (size_t)nsuper * 2 * NT * 64 * 16);
       if (!pk) return set_error(ctx, PLANC_ENOMEM, "thin_gemm: packed operand");
