@@ -111,6 +111,13 @@ int planc_gemm_nn(planc_ctx *ctx, int dtype, const void *d_A, long lda, long L, 
 int planc_gemm_tn(planc_ctx *ctx, int dtype, const void *d_A, long lda, long K, long L,
                   const float *d_colscale, const float *d_B, long ldb, int k, float *d_C, long ldc);
 
+/* Device read bandwidth (no counterpart in the reference; a measurement
+ * utility for roofline fractions): reads the caller's buffer d_buf of
+ * `bytes` (16-byte aligned, >= 16 MiB) `reps` times with plain 16-byte
+ * grid-stride loads on the context stream and returns the mean rate in GB/s
+ * in *gbps.  The buffer's contents are not changed. */
+int planc_read_bandwidth(planc_ctx *ctx, const void *d_buf, size_t bytes, int reps, double *gbps);
+
 /* G (k x k, FP64) = X^T X for X rows x k (FP32).  Replaces the Gram products
  * `W.t() * W` / `H.t() * H` (nmf/mu.hpp:56,72; common/ncpfactors.hpp:144). */
 int planc_gram(planc_ctx *ctx, const float *d_X, long rows, int k, long ldx, double *d_G);

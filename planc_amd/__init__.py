@@ -170,6 +170,16 @@ class Context:
                                            _ptr(B), _ld(B), k, _ptr(C), k), "planc_gemm_tn")
         return C
 
+    def read_bandwidth(self, buf, reps=20):
+        """Device read bandwidth (GB/s) over the bytes of a device tensor
+        (16-byte aligned, at least 16 MiB): the achievable ceiling that
+        roofline fractions are quoted against besides the 8 TB/s nameplate."""
+        out = ctypes.c_double(0.0)
+        nbytes = buf.numel() * buf.element_size()
+        self._check(self.lib.planc_read_bandwidth(self.ctx, _ptr(buf), nbytes, int(reps), ctypes.byref(out)),
+                    "planc_read_bandwidth")
+        return out.value
+
     def gram(self, X):
         torch = _torch()
         rows, k = X.shape

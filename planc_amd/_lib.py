@@ -49,6 +49,7 @@ class NTFDesc(ctypes.Structure):
 # name -> (restype, argtypes); the exported C ABI.
 SIGNATURES = {
     "planc_version": (c_char_p, []),
+    "planc_read_bandwidth": (c_int, [c_void_p, c_void_p, c_size_t, c_int, ctypes.POINTER(c_double)]),
     "planc_create": (c_int, [c_int, ctypes.POINTER(c_void_p)]),
     "planc_destroy": (c_int, [c_void_p]),
     "planc_sync": (c_int, [c_void_p]),

@@ -264,6 +264,13 @@ int planc_gemm_tn(planc_ctx *ctx, int dtype, const void *d_A, long lda, long K, 
                    0);
 }
 
+int planc_read_bandwidth(planc_ctx *ctx, const void *d_buf, size_t bytes, int reps, double *gbps) {
+  if (check_ctx(ctx)) return PLANC_EINVAL;
+  if (!d_buf || !gbps || bytes < (16u << 20) || reps < 1 || ((uintptr_t)d_buf % 16) != 0)
+    return set_error(ctx, PLANC_EINVAL, "planc_read_bandwidth: bad arguments");
+  return read_bandwidth(ctx, d_buf, bytes, reps, gbps);
+}
+
 int planc_gram(planc_ctx *ctx, const float *d_X, long rows, int k, long ldx, double *d_G) {
   if (check_ctx(ctx)) return PLANC_EINVAL;
   if (ldx < rows || !d_G) return set_error(ctx, PLANC_EINVAL, "planc_gram: bad arguments");

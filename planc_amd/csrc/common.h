@@ -180,6 +180,7 @@ int thin_gemm(planc_ctx *ctx, int dtype, int trans, const void *A, long lda, lon
 
 // nmf_kernels.hip
 int gram(planc_ctx *ctx, const float *X, long rows, int k, long ldx, double *G);
+int read_bandwidth(planc_ctx *ctx, const void *buf, size_t bytes, int reps, double *gbps);
 // kind 0: MU (eps), kind 1: HALS rows.  N(j, c) = N[j*nrs + c*ncs].
 int row_update(planc_ctx *ctx, int kind, float *H, long n, int k, long ldh, const float *N, long nrs,
                long ncs, const double *G, float eps);

@@ -34,6 +34,17 @@ __device__ inline double block_sum(double v, double *red) {
 }
 
 // ---------------------------------------------------------------------------
+// Read-bandwidth probe: 16-byte grid-stride loads, one per iteration (the
+// pattern that measured fastest on MI355X), summed so the loads stay live.
+__global__ __launch_bounds__(NTHR) void read_probe_kernel(const floatx4 *p, long n4, float *out) {
+  const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long stride = (long)gridDim.x * blockDim.x;
+  floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+  for (long i = gid; i < n4; i += stride) acc += p[i];
+  out[gid] = acc[0] + acc[1] + acc[2] + acc[3];
+}
+
+// ---------------------------------------------------------------------------
 // G = X^T X on FP64 MFMA (v_mfma_f64_16x16x4f64): 64-row chunks of X go
 // through LDS (the next chunk's loads are in flight during the current
 // chunk's MFMAs); wave w owns the upper-triangle 16 x 16 tiles w, w+4, ...
@@ -686,6 +697,27 @@ int hals_cols_impl(planc_ctx *ctx, float *W, long m, int k, long ldw, const floa
 
 // ---------------------------------------------------------------------------
 // Host entry points used by the C ABI and the loop drivers.
+
+int read_bandwidth(planc_ctx *ctx, const void *buf, size_t bytes, int reps, double *gbps) {
+  constexpr unsigned NB = 2048;
+  float *out = (float *)ws_get(ctx, 3, sizeof(float) * NB * NTHR);
+  if (!out) return set_error(ctx, PLANC_ENOMEM, "read_bandwidth: workspace");
+  const long n4 = (long)(bytes / 16);
+  hipEvent_t e0, e1;
+  PLANC_HIP_CHECK(ctx, hipEventCreate(&e0));
+  PLANC_HIP_CHECK(ctx, hipEventCreate(&e1));
+  read_probe_kernel<<<NB, NTHR, 0, ctx->stream>>>((const floatx4 *)buf, n4, out);  // warm-up
+  PLANC_HIP_CHECK(ctx, hipEventRecord(e0, ctx->stream));
+  for (int r = 0; r < reps; ++r) read_probe_kernel<<<NB, NTHR, 0, ctx->stream>>>((const floatx4 *)buf, n4, out);
+  PLANC_HIP_CHECK(ctx, hipEventRecord(e1, ctx->stream));
+  PLANC_HIP_CHECK(ctx, hipEventSynchronize(e1));
+  float ms = 0.0f;
+  PLANC_HIP_CHECK(ctx, hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *gbps = (double)n4 * 16.0 * reps / ((double)ms * 1e-3) / 1e9;
+  return PLANC_OK;
+}
 
 int gram(planc_ctx *ctx, const float *X, long rows, int k, long ldx, double *G) {
   if (k <= 0 || k > 64 || rows < 0) return set_error(ctx, PLANC_EINVAL, "gram: k must be 1..64");

@@ -652,3 +652,18 @@ def test_hiernmf_level_invariants_medium(ctx):
                 assert kids[0]["ncols"] >= kids[1]["ncols"]
                 assert abs(o["score"] - (kids[0]["sigma"] + kids[1]["sigma"] - o["sigma"])) < 1e-6 * (1 + abs(o["score"]))
     assert max(o["level"] for o in out) == depth - 1
+
+
+def test_read_bandwidth_probe(ctx):
+    """The read-bandwidth utility returns a plausible device rate and leaves
+    the buffer unchanged; undersized or misaligned buffers are rejected."""
+    say("read bandwidth probe")
+    buf = torch.arange(1 << 23, dtype=torch.float32, device="cuda")  # 32 MiB
+    ref = buf.clone()
+    gbs = ctx.read_bandwidth(buf, 5)
+    assert 100.0 < gbs < 20000.0
+    assert torch.equal(buf, ref)
+    with pytest.raises(RuntimeError):
+        ctx.read_bandwidth(buf[: 1 << 20], 5)  # 4 MiB: below the 16 MiB minimum
+    with pytest.raises(RuntimeError):
+        ctx.read_bandwidth(buf[1:], 5)  # not 16-byte aligned
