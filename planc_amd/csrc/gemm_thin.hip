@@ -1234,7 +1234,10 @@ __device__ inline uint4 transpose8_word(const uint4 (&w)[8], int j) {
   return o;
 }
 
-template <typename TA, int NT>
+// SH: the thin words of each step staged once per block in an LDS ring
+// (stream_steps_shared; the block's four waves share the k range).
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+template <typename TA, int NT, int SH>
 __global__ __launch_bounds__(NTHR) void nn16_kernel(StreamParams p) {
   constexpr int STEP = 32, PD = 2, NA = 8 * NT;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1264,8 +1267,44 @@ __global__ __launch_bounds__(NTHR) void nn16_kernel(StreamParams p) {
 #pragma unroll
     for (int t = 0; t < NT; ++t) b.b[t] = p.thin16[pack_index(kb, t, lc, NT)];
   };
-  BufNN16<NT> buf[PD];
-  stream_steps<PD>(nfull, buf, load, mma);
+  if constexpr (SH) {
+    __shared__ u32x4 ts[SH_TS][NT * 64];
+    struct BufA {
+      uint4 w[8];
+    };
+    const int tid = threadIdx.x;
+    const bool prod = tid < NT * 64;
+    const int pt = prod ? tid >> 6 : 0, pl = tid & 63;
+    auto loadA = [&](long i, BufA &b) {
+      const long kb = kbeg + i * STEP + 8 * lg;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) b.w[u] = *(const uint4 *)(base + p.lda * (kb + u));
+    };
+    auto rload = [&](long i, u32x4 &r) {
+      if (prod)
+        r = __builtin_bit_cast(u32x4, p.thin16[pack_index(kbeg + i * STEP + 8 * (pl >> 4), pt, pl & 15, NT)]);
+    };
+    auto rstore = [&](const u32x4 &r, int stage) {
+      if (prod) ts[stage][tid] = r;
+    };
+    auto mma_sh = [&](BufA &b, int stage) {
+      uint4 bt[NT];
+#pragma unroll
+      for (int t = 0; t < NT; ++t) bt[t] = __builtin_bit_cast(uint4, ts[stage][t * 64 + lane]);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const uint4 o = transpose8_word(b.w, j);
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[8 * t + j] = mfma16<TA>(o, bt[t], acc[8 * t + j]);
+      }
+    };
+    BufA buf[PD];
+    u32x4 raw[PD];
+    stream_steps_shared<PD, SH_TS>(nfull, buf, raw, loadA, rload, rstore, mma_sh);
+  } else {
+    BufNN16<NT> buf[PD];
+    stream_steps<PD>(nfull, buf, load, mma);
+  }
   if (kend - kbeg > nfull * STEP) {
     // The partial last step, masked (element loads; the packed thin operand
     // is zero past K).
@@ -1534,16 +1573,18 @@ std::vector<Geometry> candidate_geometries(const planc_ctx *ctx, long L, long K,
   const int esz = dtype == PLANC_F32 ? 4 : 2;
   const double abytes = (double)L * (double)K * esz;
   if (!trans && dtype != PLANC_F32) {
-    // Streaming 16-bit NN: one k phase, K splits only.
-    for (long ns = 1; ns <= 4096; ns *= 2) {
-      Geometry g = make_geometry(L, K, trans, dtype, NT, 1, 1, ns);
-      if (ns > 1 && (g.nsplit < ns || ksteps / g.nsplit < 8)) break;
-      if (ns > 1 && 2.0 * g.nsplit * g.groups * g.tile_bytes > 0.10 * abytes) break;
-      const long waves = g.blocks_x * 4 * g.nsplit;
-      if (ns > 1 && waves > 16L * 1024) break;
-      if (!out.empty() && waves < 2L * ctx->num_cu && ns < 4096) continue;
-      out.push_back(g);
-    }
+    // Streaming 16-bit NN: one k phase, K splits only; per-wave or
+    // block-shared thin operand.
+    for (int sh = 0; sh < (shared_ok ? 2 : 1); ++sh)
+      for (long ns = 1; ns <= 4096; ns *= 2) {
+        Geometry g = make_geometry(L, K, trans, dtype, NT, 1, 1, ns, 0, sh);
+        if (ns > 1 && (g.nsplit < ns || ksteps / g.nsplit < 8)) break;
+        if (ns > 1 && 2.0 * g.nsplit * g.groups * g.tile_bytes > 0.10 * abytes) break;
+        const long waves = g.blocks_x * 4 * g.nsplit;
+        if (ns > 1 && waves > 16L * 1024) break;
+        if (!out.empty() && waves < 2L * ctx->num_cu && ns < 4096) continue;
+        out.push_back(g);
+      }
     return out;
   }
   std::vector<int> cws;
@@ -1685,17 +1726,20 @@ int launch_stream(planc_ctx *ctx, int dtype, int trans, int NT, int tm, StreamPa
   dim3 grid((unsigned)g.blocks_x, (unsigned)g.nsplit);
   hipStream_t st = ctx->stream;
   if (!trans && dtype != PLANC_F32) {
-#define NN16(TA)                                                    \
-  do {                                                              \
-    if (NT == 1) nn16_kernel<TA, 1><<<grid, NTHR, 0, st>>>(p);      \
-    else if (NT == 2) nn16_kernel<TA, 2><<<grid, NTHR, 0, st>>>(p); \
-    else if (NT == 3) nn16_kernel<TA, 3><<<grid, NTHR, 0, st>>>(p); \
-    else nn16_kernel<TA, 4><<<grid, NTHR, 0, st>>>(p);              \
+#define NN16(TA, SHV)                                                    \
+  do {                                                                   \
+    if (NT == 1) nn16_kernel<TA, 1, SHV><<<grid, NTHR, 0, st>>>(p);      \
+    else if (NT == 2) nn16_kernel<TA, 2, SHV><<<grid, NTHR, 0, st>>>(p); \
+    else if (NT == 3) nn16_kernel<TA, 3, SHV><<<grid, NTHR, 0, st>>>(p); \
+    else nn16_kernel<TA, 4, SHV><<<grid, NTHR, 0, st>>>(p);              \
   } while (0)
-    if (dtype == PLANC_BF16)
-      NN16(bf16_t);
-    else
-      NN16(f16_t);
+    if (dtype == PLANC_BF16) {
+      if (g.shared) NN16(bf16_t, 1);
+      else NN16(bf16_t, 0);
+    } else {
+      if (g.shared) NN16(f16_t, 1);
+      else NN16(f16_t, 0);
+    }
 #undef NN16
     PLANC_LAUNCH_CHECK(ctx);
     if (g.nsplit > 1) {
@@ -1815,7 +1859,7 @@ int pick_geometry(planc_ctx *ctx, int dtype, int trans, int NT, int tm, const St
       *out = make_geometry(p.L, p.K, trans, dtype, NT, (int)v[0], (int)v[1], v[2], (int)v[3], (int)v[4]);
       return PLANC_OK;
     }
-  const bool shared_ok = (dtype == PLANC_F32 ? tm != THIN_SAFE : (trans && p.avec)) && (smode == 1 || smode == 2);
+  const bool shared_ok = (dtype == PLANC_F32 ? tm != THIN_SAFE : p.avec != 0) && (smode == 1 || smode == 2);
   std::vector<Geometry> cands =
       candidate_geometries(ctx, p.L, p.K, trans, dtype, NT, shared_ok, dtype == PLANC_F32 && tm != THIN_SAFE);
   if (smode != 2) {
