@@ -20,9 +20,10 @@ Also reported on the same line:
                    that sample.
   legs          -- the mixed-precision comparisons of the headline metric:
                    NTF-HALS 256^3 in BF16/FP16, NMF-MU and NMF-HALS on
-                   65536 x 16384 (k = 64) in FP32 vs BF16, and the MTTKRP of a
+                   65536 x 16384 (k = 64) in FP32 vs BF16, the MTTKRP of a
                    128^4 tensor (k = 48, 1.07 GB, beyond the 256 MB Infinity
-                   Cache) for the HBM roofline fraction.
+                   Cache) for the HBM roofline fraction, NTF-HALS on that
+                   tensor in FP32 vs FP16, and the hierarchical NMF.
 
 Usage: python bench.py [--gpus 1] [--steps 30] [--warmup 3] [--no-legs]
        [--no-cpu] [--profile-kernel]
@@ -290,7 +291,22 @@ def main():
                         "frac_of_8TBs": round(b4 / (t4 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                         "frac_of_achievable_read": round(b4 / (t4 * 1e-3) / 1e9 / read_ceiling, 4)}
         legs["mttkrp_128^4_k48_right_partial"] = mt
-        del X4, P4, F4
+        del P4, F4
+        # NTF-HALS on the 128^4 tensor (beyond the Infinity Cache), rank 48:
+        # FP32 against FP16 MFMA in the MTTKRP, 10 timed iterations each after
+        # 3 warm-up ones, final relative errors side by side.
+        log("bench: NTF-HALS 128^4 k=48 fp32 vs fp16")
+        nt = {}
+        r32, el32, rel32 = ntf_rate(P, ctx, torch, X4, init4, 3, 10, P.F32)
+        nt["fp32"] = {"iters_per_s": r32, "final_relerr": float(rel32[-1])}
+        X416, _ = ctx.convert(flat(X4), P.F16, scaled=False)
+        r16, el16, rel16 = ntf_rate(P, ctx, torch, X4, init4, 3, 10, P.F16, X16=X416)
+        nt["fp16"] = {"iters_per_s": r16, "final_relerr": float(rel16[-1]),
+                      "relerr_gap_vs_fp32": abs(float(rel16[-1]) - float(rel32[-1]))}
+        nt["speedup_fp16_over_fp32"] = r16 / r32
+        nt["config"] = "128^4 CP rank-48 model + 1% noise, k=48, dimension tree, 10 timed iterations"
+        legs["ntf_hals_128^4_k48"] = nt
+        del X4, X416
         torch.cuda.empty_cache()
 
         log("bench: NMF 65536x16384 k=64 fp32 vs bf16")
