@@ -314,6 +314,30 @@ def test_ntf_trajectory(ctx, algo, dimtree, dims, k):
     assert rel_err(lam.cpu().numpy(), lam_r) < 1e-3
 
 
+@pytest.mark.parametrize("dt", [P.BF16, P.F16])
+@pytest.mark.parametrize("dimtree", [True, False])
+@pytest.mark.parametrize("dims,k", [((40, 36, 32), 8), ((16, 12, 10, 8), 6)])
+def test_ntf_mixed_precision_final_error(ctx, dt, dimtree, dims, k):
+    """NTF-HALS with BF16/FP16 MFMA in the MTTKRP (16-bit copy of X and its
+    transpose, FP32 factors, FP64 Grams): the final relative error after 15
+    iterations is within 1e-3 of the FP64 oracle's, the precision criterion
+    the mixed-precision path is held to."""
+    say(f"ntf mixed dt={dt} dimtree={dimtree} dims={dims}")
+    X, _ = D.cp_tensor(dims, k, 0.01, stream=4)
+    X32 = X.astype(np.float32)
+    F0 = [f.astype(np.float32) for f in D.init_factors([(d, k) for d in dims], stream=4)]
+    _, _, rel_r, _ = O.ntf(1, X32.astype(np.float64), [f.astype(np.float64) for f in F0], 15,
+                           dimtree=1 if dimtree else 0)
+    dX = P.colmajor(X32)
+    flat = dX.permute(*reversed(range(dX.dim()))).reshape(-1)[:, None]  # memory order
+    X16, _ = ctx.convert(flat, dt, scaled=False)
+    dF = [P.colmajor(f) for f in F0]
+    lam, rel = ctx.ntf_run(dX, dF, P.HALS, 15, dtype=dt, dimtree=dimtree, X16=X16)
+    say(f"  gpu {rel[-1]:.6f} oracle {rel_r[-1]:.6f}")
+    assert np.all(np.isfinite(rel))
+    assert abs(rel[-1] - rel_r[-1]) < 1e-3
+
+
 def test_ntf_config2_shape(ctx):
     """Config 2 shape: 256^3 CP rank-32 + 1% noise, k = 32, HALS, dimension
     tree; 5 iterations against the FP64 oracle (relative error to 1e-5)."""
