@@ -593,10 +593,12 @@ int planc_ntf_run(planc_ctx *ctx, const planc_ntf_desc *desc, const float *d_X, 
       Xop = x16;
     }
   }
-  // 16-bit dimension tree: the transposed copy (R x L) turns the right
-  // contraction X_(LxR) * KRP_R into a TN product with contiguous k.
+  // 16-bit dimension tree: the right contraction X_(LxR) * KRP_R runs on the
+  // streaming 16-bit NN kernel when X's rows come in 16-byte groups (L % 8
+  // == 0, aligned copy); otherwise a transposed copy (R x L) turns it into a
+  // TN product with contiguous k.
   const void *XopT = nullptr;
-  if (dt != PLANC_F32 && dimtree) {
+  if (dt != PLANC_F32 && dimtree && !(L % 8 == 0 && (uintptr_t)Xop % 16 == 0)) {
     void *xt = ws_get(ctx, 16, sizeof(uint16_t) * (size_t)numel);
     if (!xt) return set_error(ctx, PLANC_ENOMEM, "planc_ntf_run: transposed 16-bit copy");
     int rc = transpose16(ctx, Xop, L, R, L, xt, R);
