@@ -26,8 +26,25 @@ __device__ inline float quad_sum(float v) {
 // ROLLED: the column loop stays a loop and the lane's values h[x % Q] are
 // read and written through selects (small code, for kernels that run once
 // per call and are bound by instruction fetch); otherwise the loop is fully
-// unrolled with static register indexes (fewer instructions per column, for
-// kernels over many rows).
+// unrolled with static register indexes and no branch per column: the Gram
+// row of the next column is read from LDS while this one is computed, the
+// columns go in groups of four, and a group's columns at or past k (zero
+// Gram rows and numerators, as every caller pads them) are computed and then
+// zeroed instead of branched around (fewer instructions per column, for
+// kernels over many rows and for the sequential HALS sweep).
+template <int KM>
+__device__ inline void load_st_row(const float (&St)[KM][KM], int x, int q, float (&dst)[KM / 4]) {
+  const float *src = &St[x][q * (KM / 4)];
+#pragma unroll
+  for (int u = 0; u < KM / 4; ++u) dst[u] = src[u];
+}
+
+template <int KM>
+__device__ inline void zero_padding(float (&h)[KM / 4], int q, int k) {
+#pragma unroll
+  for (int u = 0; u < KM / 4; ++u) h[u] = q * (KM / 4) + u < k ? h[u] : 0.0f;
+}
+
 template <int KM, bool ROLLED>
 __device__ inline void quad_row_update(float (&h)[KM / 4], const float (&nv)[KM / 4],
                                        const float (&St)[KM][KM], int q, int k, int kind, float eps) {
@@ -53,19 +70,21 @@ __device__ inline void quad_row_update(float (&h)[KM / 4], const float (&nv)[KM 
         for (int u = 0; u < Q; ++u) h[u] = (own && u == xo) ? v : h[u];
       }
     } else {
+      float sx[2][Q];
+      load_st_row<KM>(St, 0, q, sx[0]);
 #pragma unroll
       for (int x = 0; x < KM; ++x) {
-        if (x < k) {
-          const float *sx = &St[x][q * Q];
-          float ps = 0.0f;
+        if ((x & 3) == 0 && x >= k) break;
+        if (x + 1 < KM) load_st_row<KM>(St, x + 1, q, sx[(x + 1) & 1]);  // next column in flight
+        float ps = 0.0f;
 #pragma unroll
-          for (int u = 0; u < Q; ++u) ps = fmaf(h[u], sx[u], ps);
-          ps = quad_sum(ps);
-          float v = h[x % Q] + (nv[x % Q] - ps);
-          v = v < 1e-17f ? 1e-17f : v;
-          if (q == x / Q) h[x % Q] = v;
-        }
+        for (int u = 0; u < Q; ++u) ps = fmaf(h[u], sx[x & 1][u], ps);
+        ps = quad_sum(ps);
+        float v = h[x % Q] + (nv[x % Q] - ps);
+        v = v < 1e-17f ? 1e-17f : v;
+        h[x % Q] = q == x / Q ? v : h[x % Q];
       }
+      zero_padding<KM>(h, q, k);
     }
   } else if (kind == 0) {
     float hn[Q];
@@ -90,18 +109,20 @@ __device__ inline void quad_row_update(float (&h)[KM / 4], const float (&nv)[KM 
         for (int u = 0; u < Q; ++u) hn[u] = (own && u == co) ? v : hn[u];
       }
     } else {
+      float sc[2][Q];
+      load_st_row<KM>(St, 0, q, sc[0]);
 #pragma unroll
       for (int c = 0; c < KM; ++c) {
-        if (c < k) {
-          const float *sc = &St[c][q * Q];
-          float ps = 0.0f;
+        if ((c & 3) == 0 && c >= k) break;
+        if (c + 1 < KM) load_st_row<KM>(St, c + 1, q, sc[(c + 1) & 1]);
+        float ps = 0.0f;
 #pragma unroll
-          for (int u = 0; u < Q; ++u) ps = fmaf(h[u], sc[u], ps);
-          ps = quad_sum(ps);
-          const float v = (h[c % Q] * nv[c % Q]) / (ps + eps);
-          if (q == c / Q) hn[c % Q] = v;
-        }
+        for (int u = 0; u < Q; ++u) ps = fmaf(h[u], sc[c & 1][u], ps);
+        ps = quad_sum(ps);
+        const float v = (h[c % Q] * nv[c % Q]) / (ps + eps);
+        hn[c % Q] = q == c / Q ? v : hn[c % Q];
       }
+      zero_padding<KM>(hn, q, k);
     }
 #pragma unroll
     for (int u = 0; u < Q; ++u) h[u] = hn[u];
