@@ -220,7 +220,7 @@ def main():
     del probe
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "nn_f32_kernel<NT=2> (right partial MTTKRP, X_(65536x256)*A_2; FP32 MFMA, operands streamed to registers)",
+                "kernel": "nn_f32_kernel<NT=2>, or nn_f32_sh_kernel<NT=2> where the per-shape tuner picks the block-shared variant (right partial MTTKRP, X_(65536x256)*A_2; FP32 MFMA, operands streamed to registers)",
                 "algorithmic_bytes": bytes_right, "avg_ms": round(t_right, 5),
                 "achievable_read_GBs": round(read_ceiling, 1),
                 "frac_of_achievable": round(achieved / read_ceiling, 4),
