@@ -158,8 +158,8 @@ def main():
                     help="only launch the dominant kernel (and a calibration copy) for rocprof")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every box here "
-              "has one GPU; the multi-GPU sharding is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this build "
+              "runs on a single GPU; sharding over several GPUs is not implemented)", file=sys.stderr)
         sys.exit(2)
 
     import torch
