@@ -214,50 +214,74 @@ __global__ __launch_bounds__(NTHR) void rows_quad_kernel(float *H, long n, int k
   }
 }
 
-// Grid-wide barrier of a cooperative launch (all blocks co-resident).  Thread
-// 0 of each block, after its earlier stores have completed, adds one arrival
-// to *counter and waits until gridDim.x arrivals of generation gen are
-// visible.  Agent-scope atomics only, no cache writeback: data exchanged
-// across it must be written with agent-scope atomic stores and read with
-// agent-scope atomic loads.  The wait is bounded: on time-out *err is set,
-// later barriers return at once and the caller reports the failure.
-__device__ inline bool grid_barrier(int *counter, int *err, int gen) {
+// Column-sum exchange of a cooperative launch (all blocks co-resident).
+// Thread 0 of each block publishes its partial with an agent-scope atomic
+// store, waits for it to complete, then raises the block's flag for this
+// column (one flag per block and column, zero at launch: plain stores to
+// distinct addresses, no read-modify-write at one shared counter).  Wave 0
+// of every block polls all flags of the column, then reads the partials with
+// agent-scope atomic loads and sums them in a fixed order (identical in
+// every block).  The wait is bounded: on time-out *err is set, later
+// exchanges return at once and the caller reports the failure.
+__device__ inline bool column_exchange(double bs, double *part_col, int *flag_col, int *err, double &total) {
   __shared__ int s_ok;
-  __syncthreads();
+  __shared__ double s_tot;
   if (threadIdx.x == 0) {
+    __hip_atomic_store(&part_col[blockIdx.x], bs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-    __hip_atomic_fetch_add(counter, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const int target = (gen + 1) * (int)gridDim.x;
+    __hip_atomic_store(&flag_col[blockIdx.x], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  if (threadIdx.x < 64) {
+    const int nb = (int)gridDim.x, lane = (int)threadIdx.x;
     int ok = 1;
     for (int spin = 0;; ++spin) {
-      if (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) break;
+      int mine = 1;
+      for (int b = lane; b < nb; b += 64)
+        mine &= __hip_atomic_load(&flag_col[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+      if (__all(mine)) break;
       if (spin >= (1 << 20) || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
-        __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (lane == 0) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         ok = 0;
         break;
       }
       __builtin_amdgcn_s_sleep(1);
     }
-    s_ok = ok;
+    double tot = 0.0;
+    if (ok)
+      for (int b = lane; b < nb; b += 64)
+        tot += __hip_atomic_load(&part_col[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    tot = wave_sum(tot);
+    if (lane == 0) {
+      s_ok = ok;
+      s_tot = tot;
+    }
   }
   __syncthreads();
-  return s_ok != 0;
+  total = s_tot;
+  const bool ok = s_ok != 0;
+  __syncthreads();
+  return ok;
 }
 
 __global__ void zero_int_kernel(int *p) { *p = 0; }
+__global__ void zero_ints_kernel(int *p, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
 
 // HALS over the columns of W with per-column normalisation (nmf/hals.hpp:
 // 89-102).  Persistent cooperative kernel: each thread owns RPT rows of W in
-// registers; one grid barrier per column x publishes the column's sum of
-// squares (block partials summed in block order by every block, so every
-// block derives the identical norm).  sync[0] is the barrier counter (zero
-// at launch), sync[1] the time-out flag.
-// BS: threads per block; 1024-thread blocks make four times fewer barrier
-// participants (the per-column counter atomics serialise at one address).
+// registers; one column_exchange per column x publishes the column's sum of
+// squares (block partials summed in the same order by every block, so every
+// block derives the identical norm).  flags: k x gridDim.x, zero at launch;
+// sync[1] the time-out flag.
+// BS: threads per block; 1024-thread blocks make four times fewer exchange
+// participants.
 template <int KM, int RPT, int BS>
 __global__ __launch_bounds__(BS) void hals_cols_coop_kernel(float *W, long m, int k, long ldw,
                                                             const float *AH, long ldah,
-                                                            const double *G, double *part, int *sync) {
+                                                            const double *G, double *part, int *flags,
+                                                            int *sync) {
   __shared__ __attribute__((aligned(16))) float Gs[KM][KM];
   __shared__ double red[BS / 64];
   load_gram<KM>(Gs, G, k);
@@ -305,20 +329,8 @@ __global__ __launch_bounds__(BS) void hals_cols_coop_kernel(float *W, long m, in
       }
     }
     double bs = block_sum(lsum, red);
-    if (threadIdx.x == 0)
-      __hip_atomic_store(&part[(size_t)x * gridDim.x + blockIdx.x], bs, __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-    if (!grid_barrier(sync, sync + 1, x)) return;
     double tot = 0.0;
-    if (threadIdx.x < 64) {
-      for (long b = threadIdx.x; b < gridDim.x; b += 64)
-        tot += __hip_atomic_load(&part[(size_t)x * gridDim.x + b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      tot = wave_sum(tot);
-      if (threadIdx.x == 0) red[0] = tot;
-    }
-    __syncthreads();
-    tot = red[0];
-    __syncthreads();
+    if (!column_exchange(bs, part + (size_t)x * gridDim.x, flags + (size_t)x * gridDim.x, sync + 1, tot)) return;
     double nrm = sqrt(tot);
     if (nrm != 0.0) {
 #pragma unroll
@@ -646,11 +658,13 @@ int try_coop(planc_ctx *ctx, float *W, long m, int k, long ldw, const float *AH,
   if (need > maxblocks || need <= 0) return PLANC_OK;
   unsigned nb = (unsigned)need;
   double *part = (double *)ws_get(ctx, 2, sizeof(double) * (size_t)KM * nb);
+  int *flags = (int *)ws_get(ctx, 20, sizeof(int) * (size_t)KM * nb);
   int *sync = (int *)ws_get(ctx, 17, 2 * sizeof(int));
-  if (!part || !sync) return set_error(ctx, PLANC_ENOMEM, "hals_cols: workspace");
-  zero_int_kernel<<<1, 1, 0, ctx->stream>>>(sync);
+  if (!part || !flags || !sync) return set_error(ctx, PLANC_ENOMEM, "hals_cols: workspace");
+  const long nflags = (long)k * nb;
+  zero_ints_kernel<<<(unsigned)((nflags + 255) / 256), 256, 0, ctx->stream>>>(flags, nflags);
   PLANC_LAUNCH_CHECK(ctx);
-  void *args[] = {&W, &m, &k, &ldw, &AH, &ldah, &G, &part, &sync};
+  void *args[] = {&W, &m, &k, &ldw, &AH, &ldah, &G, &part, &flags, &sync};
   hipError_t le = hipLaunchCooperativeKernel((const void *)hals_cols_coop_kernel<KM, RPT, BS>, dim3(nb),
                                              dim3(BS), args, 0, ctx->stream);
   if (le != hipSuccess) {
