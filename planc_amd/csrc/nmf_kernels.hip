@@ -215,30 +215,39 @@ __global__ __launch_bounds__(NTHR) void rows_quad_kernel(float *H, long n, int k
 }
 
 // Column-sum exchange of a cooperative launch (all blocks co-resident).
-// Thread 0 of each block publishes its partial with an agent-scope atomic
-// store, waits for it to complete, then raises the block's flag for this
-// column (one flag per block and column, zero at launch: plain stores to
-// distinct addresses, no read-modify-write at one shared counter).  Wave 0
-// of every block polls all flags of the column, then reads the partials with
-// agent-scope atomic loads and sums them in a fixed order (identical in
-// every block).  The wait is bounded: on time-out *err is set, later
-// exchanges return at once and the caller reports the failure.
-__device__ inline bool column_exchange(double bs, double *part_col, int *flag_col, int *err, double &total) {
+// Every slot of a column starts as PART_EMPTY (set before the launch); thread
+// 0 of each block writes its partial into its own slot with an agent-scope
+// atomic store.  Wave 0 of every block polls the column's slots with
+// agent-scope atomic loads until none is empty, summing the values of the
+// same poll in a fixed order (identical in every block): the partials
+// travel with the readiness test, one round trip after the last store.  The
+// wait is bounded: on time-out *err is set, later exchanges return at once
+// and the caller reports the failure.
+constexpr unsigned long long PART_EMPTY = ~0ull;  // a NaN no arithmetic produces (all payload bits set)
+
+__device__ inline bool column_exchange(double bs, double *part_col, int *err, double &total) {
   __shared__ int s_ok;
   __shared__ double s_tot;
-  if (threadIdx.x == 0) {
-    __hip_atomic_store(&part_col[blockIdx.x], bs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-    __hip_atomic_store(&flag_col[blockIdx.x], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
+  unsigned long long *slot = reinterpret_cast<unsigned long long *>(part_col);
+  if (threadIdx.x == 0)
+    __hip_atomic_store(&slot[blockIdx.x], __builtin_bit_cast(unsigned long long, bs), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
   if (threadIdx.x < 64) {
     const int nb = (int)gridDim.x, lane = (int)threadIdx.x;
     int ok = 1;
+    double tot = 0.0;
     for (int spin = 0;; ++spin) {
       int mine = 1;
-      for (int b = lane; b < nb; b += 64)
-        mine &= __hip_atomic_load(&flag_col[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
-      if (__all(mine)) break;
+      double t = 0.0;
+      for (int b = lane; b < nb; b += 64) {
+        const unsigned long long v = __hip_atomic_load(&slot[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        mine &= v != PART_EMPTY;
+        t += __builtin_bit_cast(double, v);
+      }
+      if (__all(mine)) {
+        tot = t;
+        break;
+      }
       if (spin >= (1 << 20) || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
         if (lane == 0) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         ok = 0;
@@ -246,10 +255,6 @@ __device__ inline bool column_exchange(double bs, double *part_col, int *flag_co
       }
       __builtin_amdgcn_s_sleep(1);
     }
-    double tot = 0.0;
-    if (ok)
-      for (int b = lane; b < nb; b += 64)
-        tot += __hip_atomic_load(&part_col[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     tot = wave_sum(tot);
     if (lane == 0) {
       s_ok = ok;
@@ -264,24 +269,23 @@ __device__ inline bool column_exchange(double bs, double *part_col, int *flag_co
 }
 
 __global__ void zero_int_kernel(int *p) { *p = 0; }
-__global__ void zero_ints_kernel(int *p, long n) {
+__global__ void fill_u64_kernel(unsigned long long *p, long n, unsigned long long v) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0;
+  if (i < n) p[i] = v;
 }
 
 // HALS over the columns of W with per-column normalisation (nmf/hals.hpp:
 // 89-102).  Persistent cooperative kernel: each thread owns RPT rows of W in
 // registers; one column_exchange per column x publishes the column's sum of
 // squares (block partials summed in the same order by every block, so every
-// block derives the identical norm).  flags: k x gridDim.x, zero at launch;
-// sync[1] the time-out flag.
+// block derives the identical norm).  part: k x gridDim.x slots, PART_EMPTY
+// at launch; sync[1] the time-out flag.
 // BS: threads per block; 1024-thread blocks make four times fewer exchange
 // participants.
 template <int KM, int RPT, int BS>
 __global__ __launch_bounds__(BS) void hals_cols_coop_kernel(float *W, long m, int k, long ldw,
                                                             const float *AH, long ldah,
-                                                            const double *G, double *part, int *flags,
-                                                            int *sync) {
+                                                            const double *G, double *part, int *sync) {
   __shared__ __attribute__((aligned(16))) float Gs[KM][KM];
   __shared__ double red[BS / 64];
   load_gram<KM>(Gs, G, k);
@@ -330,7 +334,7 @@ __global__ __launch_bounds__(BS) void hals_cols_coop_kernel(float *W, long m, in
     }
     double bs = block_sum(lsum, red);
     double tot = 0.0;
-    if (!column_exchange(bs, part + (size_t)x * gridDim.x, flags + (size_t)x * gridDim.x, sync + 1, tot)) return;
+    if (!column_exchange(bs, part + (size_t)x * gridDim.x, sync + 1, tot)) return;
     double nrm = sqrt(tot);
     if (nrm != 0.0) {
 #pragma unroll
@@ -658,13 +662,13 @@ int try_coop(planc_ctx *ctx, float *W, long m, int k, long ldw, const float *AH,
   if (need > maxblocks || need <= 0) return PLANC_OK;
   unsigned nb = (unsigned)need;
   double *part = (double *)ws_get(ctx, 2, sizeof(double) * (size_t)KM * nb);
-  int *flags = (int *)ws_get(ctx, 20, sizeof(int) * (size_t)KM * nb);
   int *sync = (int *)ws_get(ctx, 17, 2 * sizeof(int));
-  if (!part || !flags || !sync) return set_error(ctx, PLANC_ENOMEM, "hals_cols: workspace");
-  const long nflags = (long)k * nb;
-  zero_ints_kernel<<<(unsigned)((nflags + 255) / 256), 256, 0, ctx->stream>>>(flags, nflags);
+  if (!part || !sync) return set_error(ctx, PLANC_ENOMEM, "hals_cols: workspace");
+  const long nslots = (long)k * nb;
+  fill_u64_kernel<<<(unsigned)((nslots + 255) / 256), 256, 0, ctx->stream>>>((unsigned long long *)part, nslots,
+                                                                             PART_EMPTY);
   PLANC_LAUNCH_CHECK(ctx);
-  void *args[] = {&W, &m, &k, &ldw, &AH, &ldah, &G, &part, &flags, &sync};
+  void *args[] = {&W, &m, &k, &ldw, &AH, &ldah, &G, &part, &sync};
   hipError_t le = hipLaunchCooperativeKernel((const void *)hals_cols_coop_kernel<KM, RPT, BS>, dim3(nb),
                                              dim3(BS), args, 0, ctx->stream);
   if (le != hipSuccess) {
