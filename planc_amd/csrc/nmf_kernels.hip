@@ -833,7 +833,7 @@ int nmf_relerr(planc_ctx *ctx, const double *normA2, const double *dotv, const d
   return PLANC_OK;
 }
 
-// Clears / reads the grid-barrier time-out flag of the HALS column sweep.
+// Clears / reads the column-exchange time-out flag of the HALS column sweep.
 int hals_sync_reset(planc_ctx *ctx) {
   int *sync = (int *)ws_get(ctx, 17, 2 * sizeof(int));
   if (!sync) return set_error(ctx, PLANC_ENOMEM, "hals_cols: workspace");
@@ -848,7 +848,7 @@ int hals_sync_check(planc_ctx *ctx) {
   int flag = 0;
   PLANC_HIP_CHECK(ctx, hipMemcpyAsync(&flag, sync + 1, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   PLANC_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-  if (flag) return set_error(ctx, PLANC_EHIP, "HALS column sweep: grid barrier timed out");
+  if (flag) return set_error(ctx, PLANC_EHIP, "HALS column sweep: column exchange timed out");
   return PLANC_OK;
 }
 
