@@ -235,7 +235,7 @@ bool mode_update_fits(long I, int k);
 // is rel_out[*rel_counter] and the device counter is incremented.
 int mode_update(planc_ctx *ctx, int kind, float eps, float *F, long I, long ldf, int k, const float *M,
                 long mrs, long mcs, const double *const *grams, int ng, int skip, double *gout,
-                double *lambda, const double *normX2, double *rel_out, int *rel_counter);
+                double *lambda, const double *normX2, double *rel_out, int *rel_counter, int *err);
 
 }  // namespace planc
 

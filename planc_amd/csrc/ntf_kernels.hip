@@ -257,6 +257,18 @@ __global__ void transpose_kernel(const float *in, long I, long ldi, int k, float
 //     norms = sqrt(diag(Gram u)), lambda = norms, Gram of the normalised
 //     factor = D^-1 (Gram u) D^-1, the columns of F divided by the norms,
 //     the error.
+//   For a factor of at most XCH_BLOCKS blocks of rows and k <= 48 there is
+//     one launch: every block writes its partials into its own slots of a
+//     device buffer with agent-scope atomic stores (slots start as
+//     PART_EMPTY, an all-ones NaN no arithmetic produces), polls all blocks'
+//     slots until none is empty, sums them in block order (identical in
+//     every block) and divides its own rows, still in registers, by the
+//     norms; block 0 writes lambda, the Gram and the error.  Two slot
+//     buffers alternate by the parity of a device launch counter, and the
+//     blocks empty the whole other buffer for the next launch.  The
+//     blocks of one launch must run concurrently: XCH_BLOCKS is far below
+//     one block per CU, and the wait is bounded (a time-out sets bit 2 of
+//     the run's error flag).
 // kind: 0 MU, 1 HALS, 2 no row update (the factor was solved already, BPP).
 struct ModeParams {
   float *F;
@@ -273,9 +285,22 @@ struct ModeParams {
   double *rel_out;
   double *part;  // per block: k*k Gram of u, then k column dots
   int *rel_counter;  // non-null: the error goes to rel_out[*rel_counter], which is incremented
+  unsigned long long *slots;  // one-launch form: two buffers of XCH_HALF slots
+  int *ctl;                   // one-launch form: ctl[0] launch counter
+  int *err;                   // one-launch form: time-out flag (bit 2)
 };
 
-template <int KM, int KIND, int MRT>
+constexpr int XCH_BLOCKS = 4;                           // blocks of rows in the one-launch form
+constexpr long XCH_HALF = XCH_BLOCKS * (48L * 48 + 48);  // slots per buffer
+constexpr unsigned long long PART_EMPTY = ~0ull;        // a NaN no arithmetic produces
+
+__global__ void xch_init_kernel(unsigned long long *slots, long n, int *ctl) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) slots[i] = PART_EMPTY;
+  if (i == 0) ctl[0] = 0;
+}
+
+template <int KM, int KIND, int MRT, bool XCH>
 __global__ __launch_bounds__(4 * MRT) void mode_rows_kernel(ModeParams p) {
   constexpr int MR = MRT;            // rows per block iteration
   constexpr int NTH = 4 * MR;        // threads: a quad per row
@@ -286,12 +311,18 @@ __global__ __launch_bounds__(4 * MRT) void mode_rows_kernel(ModeParams p) {
   constexpr int TPW = (NUP + NW - 1) / NW;  // tiles per wave
   constexpr int NDP = (4 * KM + NTH - 1) / NTH;  // (column, quarter) dot pairs per thread
   __shared__ __attribute__((aligned(16))) float St[KM][KM];  // St[x][t] = S(t, x)
-  __shared__ float Us[KM][MR + 1];                           // u(row, c) at Us[c][row]
-  __shared__ float Ms[KM][MR + 1];                           // M(c, row)
+  // u(row, c) at Us[c][row], M(c, row) at Ms[c][row]; the one-launch form
+  // reuses the space for the summed partials (MR >= KM: they fit).
+  __shared__ __attribute__((aligned(16))) float UMs[2][KM][MR + 1];
+  float(&Us)[KM][MR + 1] = UMs[0];
+  float(&Ms)[KM][MR + 1] = UMs[1];
+  static_assert(MR >= KM, "summed partials must fit the row tiles");
   const int tid = threadIdx.x, q = tid & 3, rr = tid >> 2;
   const int lane = tid & 63, wave = tid >> 6;
   const int k = p.k;
   const long stride = (long)gridDim.x * MR;
+  int par = 0;  // one-launch form: which slot buffer this launch uses
+  if constexpr (XCH) par = __hip_atomic_load(p.ctl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & 1;
   // Row values and MTTKRP values of rows r0 + rr (clamped to a valid row).
   auto load_rows = [&](long r0, float(&h)[Q], float(&mv)[Q]) {
     const long i = r0 + rr;
@@ -320,8 +351,9 @@ __global__ __launch_bounds__(4 * MRT) void mode_rows_kernel(ModeParams p) {
   // S = Hadamard of the other Grams: every load of the block's entries is
   // issued before the first product (one memory round trip, not one per
   // entry).
+  constexpr int NE = (KM * KM + NTH - 1) / NTH;
+  double dS[NE];  // FP64 Hadamard entries of this thread (the one-launch error)
   {
-    constexpr int NE = (KM * KM + NTH - 1) / NTH;
     double gv[NE][4];
 #pragma unroll
     for (int it = 0; it < NE; ++it) {
@@ -345,6 +377,7 @@ __global__ __launch_bounds__(4 * MRT) void mode_rows_kernel(ModeParams p) {
       for (int gg = p.skip <= 4 ? 5 : 4; gg < p.ng; ++gg)  // more than five modes: the rest
         if (gg != p.skip) d *= (a < k && b < k) ? p.g[gg][a + (long)k * b] : 1.0;
       if (e < KM * KM) St[b][a] = (a < k && b < k) ? (float)d : 0.0f;
+      dS[it] = d;
     }
   }
   doublex4 gacc[TPW];
@@ -363,7 +396,7 @@ __global__ __launch_bounds__(4 * MRT) void mode_rows_kernel(ModeParams p) {
       const int t = q * Q + u;
       Us[t][rr] = valid ? h[u] : 0.0f;
       Ms[t][rr] = valid ? mv[u] : 0.0f;
-      if (valid && t < k) p.F[i + p.ldf * t] = h[u];
+      if (!XCH && valid && t < k) p.F[i + p.ldf * t] = h[u];
     }
     if (r0 + stride < p.I) load_rows(r0 + stride, h, mv);  // next rows in flight
     __syncthreads();
@@ -398,6 +431,17 @@ __global__ __launch_bounds__(4 * MRT) void mode_rows_kernel(ModeParams p) {
     __syncthreads();
   }
   double *dst = p.part + (long)blockIdx.x * (k * k + k);
+  unsigned long long *xdst = nullptr;
+  if constexpr (XCH) xdst = p.slots + par * XCH_HALF + (long)blockIdx.x * (k * k + k);
+  // Partial (a, b) of this block: a plain store, or an atomic store into
+  // its slot in the one-launch form.
+  auto put_part = [&](int idx, double v) {
+    if constexpr (XCH)
+      __hip_atomic_store(&xdst[idx], __builtin_bit_cast(unsigned long long, v), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    else
+      dst[idx] = v;
+  };
   // FP64 D layout: lane holds D(lane / 16 + 4 v, lane % 16), v = 0..3.
 #pragma unroll
   for (int j = 0; j < TPW; ++j) {
@@ -410,8 +454,8 @@ __global__ __launch_bounds__(4 * MRT) void mode_rows_kernel(ModeParams p) {
       for (int v = 0; v < 4; ++v) {
         const int a = 16 * ta + (lane >> 4) + 4 * v;
         if (a < k && b < k && (ta != tb || a <= b)) {
-          dst[a + k * b] = gacc[j][v];
-          dst[b + k * a] = gacc[j][v];
+          put_part(a + k * b, gacc[j][v]);
+          put_part(b + k * a, gacc[j][v]);
         }
       }
     }
@@ -426,7 +470,122 @@ __global__ __launch_bounds__(4 * MRT) void mode_rows_kernel(ModeParams p) {
     double s23 = __shfl_xor(s01, 2, 64);
     double tot = (tid & 2) ? s23 + s01 : s01 + s23;
     const int c = pi >> 2;
-    if (pi < 4 * KM && (pi & 3) == 0 && c < k) dst[k * k + c] = tot;
+    if (pi < 4 * KM && (pi & 3) == 0 && c < k) put_part(k * k + c, tot);
+  }
+  if constexpr (XCH) {
+    const int kk = k * k, nb = (int)gridDim.x;
+    // Empty the whole other buffer for the next launch, whatever its block
+    // count and k (nobody reads that buffer during this launch).
+    unsigned long long *other = p.slots + (1 - par) * XCH_HALF;
+    for (long e = (long)blockIdx.x * NTH + tid; e < XCH_HALF; e += (long)nb * NTH) other[e] = PART_EMPTY;
+    // Poll every block's slots until none is empty; the sums of that poll
+    // are the block-order sums of the partials.
+    const unsigned long long *cur = p.slots + par * XCH_HALF;
+    constexpr int NG = (KM * KM + KM + NTH - 1) / NTH;
+    double gs[NG];
+    bool ok = true;
+    for (int spin = 0;; ++spin) {
+      unsigned long long v[NG][XCH_BLOCKS];
+#pragma unroll
+      for (int j = 0; j < NG; ++j) {
+        const int e = tid + NTH * j;
+#pragma unroll
+        for (int b = 0; b < XCH_BLOCKS; ++b)
+          v[j][b] = (b < nb && e < kk + k)
+                        ? __hip_atomic_load(&cur[(long)b * (kk + k) + e], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                        : 0ull;
+      }
+      int mine = 1;
+#pragma unroll
+      for (int j = 0; j < NG; ++j) {
+        double sj = 0.0;
+#pragma unroll
+        for (int b = 0; b < XCH_BLOCKS; ++b) {
+          if (b < nb) {
+            mine &= v[j][b] != PART_EMPTY;
+            sj += __builtin_bit_cast(double, v[j][b]);
+          }
+        }
+        gs[j] = sj;
+      }
+      if (__syncthreads_and(mine)) break;
+      if (spin >= (1 << 20)) {
+        ok = false;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    if (!ok && tid == 0) __hip_atomic_fetch_or(p.err, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __shared__ double nrm[KM];
+    __shared__ double red[2][NW];
+    double *Gu = reinterpret_cast<double *>(&UMs[0][0][0]);
+#pragma unroll
+    for (int j = 0; j < NG; ++j) {
+      const int e = tid + NTH * j;
+      if (e < kk + k) Gu[e] = gs[j];
+    }
+    __syncthreads();
+    if (tid < k) {
+      nrm[tid] = sqrt(Gu[tid + k * tid]);
+      if (blockIdx.x == 0) p.lambda[tid] = nrm[tid];
+    }
+    __syncthreads();
+    // This block's rows (still in registers) divided by the norms.
+    {
+      const long i = (long)blockIdx.x * MR + rr;
+      if (i < p.I) {
+#pragma unroll
+        for (int u = 0; u < Q; ++u) {
+          const int t = q * Q + u;
+          if (t < k) p.F[i + p.ldf * t] = nrm[t] > 0 ? (float)((double)h[u] / nrm[t]) : h[u];
+        }
+      }
+    }
+    if (blockIdx.x != 0) return;
+    for (int e = tid; e < kk; e += NTH) {
+      const int a = e % k, b = e / k;
+      const double sa = nrm[a] > 0 ? nrm[a] : 1.0, sb = nrm[b] > 0 ? nrm[b] : 1.0;
+      p.gout[e] = Gu[e] / (sa * sb);
+    }
+    if (p.rel_out) {
+      // <X, model> = sum_c (u^T M)(c); ||model||^2 = sum (Gram u) .* S with
+      // S the FP64 Hadamard of the other Grams built above.
+      double t1 = 0.0, xh = 0.0;
+      for (int c = tid; c < k; c += NTH) t1 += Gu[kk + c];
+#pragma unroll
+      for (int it = 0; it < NE; ++it) {
+        const int e = tid + it * NTH;
+        const int a = e % KM, b = e / KM;
+        if (e < KM * KM && a < k && b < k) xh += Gu[a + k * b] * dS[it];
+      }
+      t1 = wave_sum_d(t1);
+      xh = wave_sum_d(xh);
+      if (lane == 0) {
+        red[0][wave] = t1;
+        red[1][wave] = xh;
+      }
+      __syncthreads();
+      if (tid == 0) {
+        t1 = 0.0;
+        xh = 0.0;
+        for (int w = 0; w < NW; ++w) {
+          t1 += red[0][w];
+          xh += red[1][w];
+        }
+        double e2 = *p.normX2 - 2.0 * t1 + xh;
+        if (e2 < 0) e2 = 0;
+        const double rel = sqrt(e2 / *p.normX2);
+        if (p.rel_counter) {
+          const int idx = *p.rel_counter;
+          p.rel_out[idx] = rel;
+          *p.rel_counter = idx + 1;
+        } else {
+          *p.rel_out = rel;
+        }
+      }
+    }
+    // Every block has read the counter (it published after reading it).
+    if (tid == 0) __hip_atomic_fetch_add(p.ctl, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
@@ -601,7 +760,7 @@ bool mode_update_fits(long I, int k) { return k >= 1 && k <= 64 && I >= 1 && I <
 
 int mode_update(planc_ctx *ctx, int kind, float eps, float *F, long I, long ldf, int k, const float *M,
                 long mrs, long mcs, const double *const *grams, int ng, int skip, double *gout,
-                double *lambda, const double *normX2, double *rel_out, int *rel_counter) {
+                double *lambda, const double *normX2, double *rel_out, int *rel_counter, int *err) {
   if (!mode_update_fits(I, k) || ng < 1 || ng > 8 || skip < 0 || skip >= ng)
     return set_error(ctx, PLANC_EINVAL, "mode_update: bad arguments");
   ModeParams p{};
@@ -627,27 +786,52 @@ int mode_update(planc_ctx *ctx, int kind, float eps, float *F, long I, long ldf,
   p.part = (double *)ws_get(ctx, 18, sizeof(double) * (size_t)nb * (k * k + k));
   if (!p.part) return set_error(ctx, PLANC_ENOMEM, "mode_update: workspace");
   p.rel_counter = rel_counter;
-#define ROWS_MR(KMV, MRV)                                                          \
-  do {                                                                             \
-    if (kind == 0)                                                                 \
-      mode_rows_kernel<KMV, 0, MRV><<<(unsigned)nb, 4 * MRV, 0, ctx->stream>>>(p); \
-    else if (kind == 1)                                                            \
-      mode_rows_kernel<KMV, 1, MRV><<<(unsigned)nb, 4 * MRV, 0, ctx->stream>>>(p); \
-    else                                                                           \
-      mode_rows_kernel<KMV, 2, MRV><<<(unsigned)nb, 4 * MRV, 0, ctx->stream>>>(p); \
+  const char *xenv = getenv("PLANC_MODE_EXCHANGE");  // 0: always two launches
+  const bool xch = err != nullptr && nb <= XCH_BLOCKS && k <= 48 && !(xenv && xenv[0] == '0');
+  if (xch) {
+    // Both slot buffers start empty and the launch counter at zero; after
+    // that the kernels keep them so (see mode_rows_kernel).
+    const bool fresh = ctx->ws[19].ptr == nullptr || ctx->ws[21].ptr == nullptr;
+    p.slots = (unsigned long long *)ws_get(ctx, 19, sizeof(unsigned long long) * 2 * XCH_HALF);
+    p.ctl = (int *)ws_get(ctx, 21, 4 * sizeof(int));
+    if (!p.slots || !p.ctl) return set_error(ctx, PLANC_ENOMEM, "mode_update: workspace");
+    if (fresh) {
+      xch_init_kernel<<<(unsigned)((2 * XCH_HALF + 255) / 256), 256, 0, ctx->stream>>>(p.slots, 2 * XCH_HALF,
+                                                                                      p.ctl);
+      PLANC_LAUNCH_CHECK(ctx);
+    }
+    p.err = err;
+  }
+#define ROWS_X(KMV, MRV, X)                                                           \
+  do {                                                                                \
+    if (kind == 0)                                                                    \
+      mode_rows_kernel<KMV, 0, MRV, X><<<(unsigned)nb, 4 * MRV, 0, ctx->stream>>>(p); \
+    else if (kind == 1)                                                               \
+      mode_rows_kernel<KMV, 1, MRV, X><<<(unsigned)nb, 4 * MRV, 0, ctx->stream>>>(p); \
+    else                                                                              \
+      mode_rows_kernel<KMV, 2, MRV, X><<<(unsigned)nb, 4 * MRV, 0, ctx->stream>>>(p); \
   } while (0)
-#define ROWS(KMV) ROWS_MR(KMV, 64)
-  if (k <= 16)
-    ROWS(16);
-  else if (k <= 32)
-    ROWS(32);
-  else if (k <= 48)
-    ROWS(48);
-  else
-    ROWS(64);
-#undef ROWS
-#undef ROWS_MR
+  if (k <= 16) {
+    if (xch)
+      ROWS_X(16, 64, true);
+    else
+      ROWS_X(16, 64, false);
+  } else if (k <= 32) {
+    if (xch)
+      ROWS_X(32, 64, true);
+    else
+      ROWS_X(32, 64, false);
+  } else if (k <= 48) {
+    if (xch)
+      ROWS_X(48, 64, true);
+    else
+      ROWS_X(48, 64, false);
+  } else {
+    ROWS_X(64, 64, false);
+  }
+#undef ROWS_X
   PLANC_LAUNCH_CHECK(ctx);
+  if (xch) return PLANC_OK;
   long fb = (I * k + 255) / 256;
   if (fb > 1024) fb = 1024;
   mode_finalize_kernel<<<(unsigned)fb, 256, 0, ctx->stream>>>(p, (int)nb);

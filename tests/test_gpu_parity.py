@@ -314,6 +314,33 @@ def test_ntf_trajectory(ctx, algo, dimtree, dims, k):
     assert rel_err(lam.cpu().numpy(), lam_r) < 1e-3
 
 
+@pytest.mark.parametrize("algo", [P.MU, P.HALS])
+@pytest.mark.parametrize("dims,k", [((256, 100, 130), 20), ((300, 100, 20), 40), ((128, 128, 64, 40), 48)])
+def test_ntf_mode_exchange_matches_two_launches(ctx, algo, dims, k, monkeypatch):
+    """The one-launch mode update (blocks exchange their partials through
+    device slots) gives the factors, lambda and errors of the two-launch
+    form, over modes of one to four row blocks and beyond (both forms in one
+    run), through the captured-graph iterations and on a second call (slot
+    buffers and launch counter carried over)."""
+    say(f"ntf exchange algo={algo} dims={dims} k={k}")
+    X, _ = D.cp_tensor(dims, k, 0.01, stream=5)
+    X32 = X.astype(np.float32)
+    F0 = [f.astype(np.float32) for f in D.init_factors([(d, k) for d in dims], stream=5)]
+    dX = P.colmajor(X32)
+    runs = []
+    for env in ("0", "1", "1"):
+        monkeypatch.setenv("PLANC_MODE_EXCHANGE", env)
+        dF = [P.colmajor(f) for f in F0]
+        lam, rel = ctx.ntf_run(dX, dF, algo, 11)
+        runs.append(([f.cpu().numpy() for f in dF], lam.cpu().numpy(), rel))
+    (F_two, lam_two, rel_two) = runs[0]
+    for F_one, lam_one, rel_one in runs[1:]:
+        for a, b in zip(F_one, F_two):
+            np.testing.assert_array_equal(a, b)
+        np.testing.assert_array_equal(lam_one, lam_two)
+        assert np.abs(rel_one - rel_two).max() < 1e-12
+
+
 @pytest.mark.parametrize("dt", [P.BF16, P.F16])
 @pytest.mark.parametrize("dimtree", [True, False])
 @pytest.mark.parametrize("dims,k", [((40, 36, 32), 8), ((16, 12, 10, 8), 6)])
