@@ -314,7 +314,7 @@ def test_ntf_trajectory(ctx, algo, dimtree, dims, k):
     assert rel_err(lam.cpu().numpy(), lam_r) < 1e-3
 
 
-@pytest.mark.parametrize("algo", [P.MU, P.HALS])
+@pytest.mark.parametrize("algo", [P.MU, P.HALS, P.ANLSBPP])
 @pytest.mark.parametrize("dims,k", [((256, 100, 130), 20), ((300, 100, 20), 40), ((128, 128, 64, 40), 48)])
 def test_ntf_mode_exchange_matches_two_launches(ctx, algo, dims, k, monkeypatch):
     """The one-launch mode update (blocks exchange their partials through
