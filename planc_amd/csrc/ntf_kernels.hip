@@ -86,36 +86,56 @@ __global__ __launch_bounds__(1024) void contract_outer_kernel(ContractParams p) 
   }
 }
 
-// inner > 1, inner % 4 == 0, 16-byte aligned columns: one wavefront per
-// output (i, c); lanes read 16 consecutive bytes of the contiguous inner
-// index, weights from LDS; FP64 wave reduction in fixed order.
+// inner > 1, inner % 4 == 0, 16-byte aligned columns: a wavefront computes
+// CW_OUT outputs (i, c), i = first + 4 r, with their loads issued together;
+// lanes read 16 consecutive bytes of the contiguous inner index, weights
+// from LDS (computed once per block for its 4 * CW_OUT outputs); per output
+// the same fixed-order sums and FP64 wave reduction.
+constexpr int CW_OUT = 4;
 __global__ __launch_bounds__(256) void contract_wave_kernel(ContractParams p) {
   __shared__ __attribute__((aligned(16))) float wi[WMAX];
   __shared__ float wo[WMAX];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int c = blockIdx.y;
-  const long i = (long)blockIdx.x * 4 + wave;
+  const long i0 = (long)blockIdx.x * (4 * CW_OUT) + wave;
   for (long a = threadIdx.x; a < p.inner; a += blockDim.x) wi[a] = weight(p, (unsigned)a, 0, p.tn, c);
   for (long b = threadIdx.x; b < p.outer; b += blockDim.x) wo[b] = weight(p, (unsigned)b, p.tn + 1, p.nb, c);
   __syncthreads();
-  if (i >= p.In) return;
-  float acc = 0.0f;
+  if (i0 >= p.In) return;
+  float acc[CW_OUT];
+#pragma unroll
+  for (int r = 0; r < CW_OUT; ++r) acc[r] = 0.0f;
   const float *Pc = p.P + p.ldp * c;
   for (long b = 0; b < p.outer; ++b) {
-    const float *src = Pc + p.inner * (i + p.In * b);
-    float sb = 0.0f;
+    float sb[CW_OUT];
+#pragma unroll
+    for (int r = 0; r < CW_OUT; ++r) sb[r] = 0.0f;
     for (long a = 4 * lane; a < p.inner; a += 256) {
-      const floatx4 v = *(const floatx4 *)(src + a);
       const floatx4 w = *(const floatx4 *)&wi[a];
-      sb = fmaf(v[0], w[0], sb);
-      sb = fmaf(v[1], w[1], sb);
-      sb = fmaf(v[2], w[2], sb);
-      sb = fmaf(v[3], w[3], sb);
+      floatx4 v[CW_OUT];
+#pragma unroll
+      for (int r = 0; r < CW_OUT; ++r) {
+        const long i = i0 + 4 * r;
+        const long ic = i < p.In ? i : i0;  // rows past In re-read row i0 (not stored)
+        v[r] = *(const floatx4 *)(Pc + p.inner * (ic + p.In * b) + a);
+      }
+#pragma unroll
+      for (int r = 0; r < CW_OUT; ++r) {
+        sb[r] = fmaf(v[r][0], w[0], sb[r]);
+        sb[r] = fmaf(v[r][1], w[1], sb[r]);
+        sb[r] = fmaf(v[r][2], w[2], sb[r]);
+        sb[r] = fmaf(v[r][3], w[3], sb[r]);
+      }
     }
-    acc = fmaf(sb, wo[b], acc);
+#pragma unroll
+    for (int r = 0; r < CW_OUT; ++r) acc[r] = fmaf(sb[r], wo[b], acc[r]);
   }
-  const double r = wave_sum_d((double)acc);
-  if (lane == 0) p.M[c + p.ldo * i] = (float)r;
+#pragma unroll
+  for (int r = 0; r < CW_OUT; ++r) {
+    const long i = i0 + 4 * r;
+    const double t = wave_sum_d((double)acc[r]);
+    if (lane == 0 && i < p.In) p.M[c + p.ldo * i] = (float)t;
+  }
 }
 
 // inner > 1: one block per (tile of TI output indices, c); threads walk the
@@ -700,7 +720,7 @@ int contract_projection(planc_ctx *ctx, const float *P, long ldp, int nb, const 
     contract_outer_kernel<<<grid, 1024, 0, ctx->stream>>>(p);
   } else if (p.inner % 4 == 0 && p.inner <= WMAX && p.outer <= WMAX && ldp % 4 == 0 &&
              ((uintptr_t)P % 16) == 0) {
-    dim3 grid((unsigned)((p.In + 3) / 4), (unsigned)k);
+    dim3 grid((unsigned)((p.In + 4 * CW_OUT - 1) / (4 * CW_OUT)), (unsigned)k);
     contract_wave_kernel<<<grid, 256, 0, ctx->stream>>>(p);
   } else {
     dim3 grid((unsigned)((p.In + TI - 1) / TI), (unsigned)k);
