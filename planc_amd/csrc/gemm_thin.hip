@@ -785,7 +785,9 @@ struct BufNN {
 
 template <int NT, int TM>
 __global__ __launch_bounds__(NTHR) void nn_f32_kernel(StreamParams p) {
-  constexpr int STEP = 16, PD = 2, NA = 4 * NT;
+  // Three steps in flight: at K = 256 (sixteen steps per wave) a third
+  // step measured 5 % faster than two, a fourth no faster.
+  constexpr int STEP = 16, PD = 3, NA = 4 * NT;
   __shared__ floatx4 red[3][NA][64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int lc = lane & 15, lg = lane >> 4;
