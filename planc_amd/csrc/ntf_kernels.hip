@@ -55,15 +55,29 @@ __global__ __launch_bounds__(1024) void contract_outer_kernel(ContractParams p) 
   const int c = blockIdx.y;
   const long i = (long)blockIdx.x * 64 + lane;
   const bool cached = p.outer <= WMAX;
+  const bool valid = i < p.In;
+  const float *src = p.P + p.ldp * c + (valid ? i : 0);
+  const long st = p.In * 16;
+  // The first group of eight loads is issued before the weights are
+  // computed (independent round trips overlap); same sums in the same order.
+  float v0[8];
+  const bool first = cached && valid && wave + 16 * 7 < p.outer;
+  if (first) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v0[u] = src[p.In * wave + st * u];
+  }
   if (cached)
     for (long b = threadIdx.x; b < p.outer; b += blockDim.x) wo[b] = weight(p, (unsigned)b, p.tn + 1, p.nb, c);
   __syncthreads();
   float acc = 0.0f;
-  if (i < p.In) {
-    const float *src = p.P + p.ldp * c + i;
-    const long st = p.In * 16;
+  if (valid) {
     long b = wave;
     if (cached) {
+      if (first) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc = fmaf(v0[u], wo[b + 16 * u], acc);
+        b += 16 * 8;
+      }
       for (; b + 16 * 7 < p.outer; b += 16 * 8) {
         float v[8];
 #pragma unroll
