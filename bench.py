@@ -192,6 +192,7 @@ def main():
             cal2.copy_(cal)
         torch.cuda.synchronize()
         print(json.dumps({"profile_kernel": "mttkrp_partial right 256^3 k=32 x10; copy 256MiB x10"}))
+        ctx.close()
         return
 
     # ---- the measured workload ------------------------------------------

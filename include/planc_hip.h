@@ -287,8 +287,8 @@ int planc_nmf_run(planc_ctx *ctx, const planc_nmf_desc *desc, const float *d_A, 
 typedef struct planc_ntf_desc {
   int algo;          /* PLANC_MU, PLANC_HALS, PLANC_ANLSBPP */
   int dtype;         /* of the MTTKRP reads of X */
-  int nmodes;        /* 3..8 */
-  long dims[8];
+  int nmodes;        /* 2..7 */
+  long dims[8];      /* dims[0..nmodes-1] used */
   int k;
   int iters;
   int dimtree;       /* 1: dimension-tree MTTKRP (2 full passes over X per sweep) */

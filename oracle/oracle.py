@@ -20,11 +20,22 @@ How each oracle function is pinned (tests/test_oracle.py):
 * nmf MU / HALS / BPP -- NumPy step-by-step restatements on tiny problems
                          (committed trajectory fixtures), monotone objective
                          (testcases.md:3), the exact fixed point A = W H^T for
-                         MU, KKT of each BPP half-step, and the HALS column step
-                         being the exact coordinate minimiser.
-* ntf HALS / MU / BPP -- NumPy restatement with einsum MTTKRP (fixtures),
-                         monotone error, relative error checked against a direct
-                         numpy reconstruction.
+                         MU, KKT of each BPP half-step; every HALS column step
+                         (H sweep and W sweep) is the clamped coordinate
+                         minimiser of ||A - W H^T|| given the other columns,
+                         checked through the explicit residual.
+* ntf_mode_update     -- HALS: every column of the sweep is the clamped
+                         coordinate minimiser of 0.5||X_(n) - H K^T||^2 given the
+                         columns before it (new) and after it (old), checked
+                         through the gradient of the explicit residual; BPP: KKT
+                         of the same objective; MU: at an exact factorisation the
+                         step shrinks H by exactly eps/(HS + eps) with eps = 1e-6
+                         (ntf/ntfmu.hpp:13), and the objective does not increase.
+* ntf HALS / MU / BPP -- one iteration equals mode updates on S and M built from
+                         an explicit (brute-force) Khatri-Rao product, with the
+                         column normalisation and lambda of normalize(mode);
+                         monotone error; relative error checked against a direct
+                         numpy reconstruction; dimtree == direct.
 * cp_loss             -- central differences in FP64 (h = 1e-6) and the einsum
                          reconstruction.
 * r2_update           -- brute force over the four active sets of the
@@ -70,6 +81,9 @@ def lib():
                                  ctypes.POINTER(_dp), ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                  _dp, _dp]
         L.oracle_ntf.restype = ctypes.c_int
+        L.oracle_ntf_mode_update.argtypes = [ctypes.c_int, ctypes.c_long, ctypes.c_int, _dp, _dp,
+                                             _dp]
+        L.oracle_ntf_mode_update.restype = ctypes.c_int
         L.oracle_cp_loss.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_long), _dp,
                                      ctypes.POINTER(_dp), ctypes.c_int, ctypes.POINTER(_dp)]
         L.oracle_cp_loss.restype = ctypes.c_double
@@ -186,6 +200,17 @@ def ntf(algo, X, factors0, iters, dimtree=1):
     rel = np.zeros(iters)
     rc = lib().oracle_ntf(algo, N, dims, _p(Xf), arr, k, iters, dimtree, _p(lam), _p(rel))
     return fs, lam, rel, rc
+
+
+def ntf_mode_update(algo, H, M, S):
+    """One NTF mode update (algo 0 MU, 1 HALS, 2 BPP) of the factor H (I x k)
+    given M (k x I, the MTTKRP) and S (k x k).  Returns (H_new, rc)."""
+    Hn = _f64(H).copy(order="F")
+    Mf = _f64(M)
+    Sf = _f64(S)
+    In, k = Hn.shape
+    rc = lib().oracle_ntf_mode_update(algo, In, k, _p(Hn), _p(Mf), _p(Sf))
+    return Hn, rc
 
 
 def cp_loss(X, factors, grads=False):

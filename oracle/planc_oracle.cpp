@@ -890,6 +890,21 @@ int oracle_nmf(int algo, const double *A, long m, long n, double *W, double *H, 
   return rc;
 }
 
+// One NTF mode update with the Hadamard-of-Grams S (k x k) and the MTTKRP M
+// (k x I_n) given: NTFHALS / NTFMU / NTFANLSBPP::update (ntf/ntfhals.hpp:
+// 13-29, ntf/ntfmu.hpp:11-16, ntf/ntfanlsbpp.hpp:15-57) on the current
+// factor H (I_n x k), overwritten.  Returns 0, -1 (BPP did not converge) or
+// -2 (unknown algo).
+int oracle_ntf_mode_update(int algo, long In, int k, double *H, const double *M, const double *S) {
+  Mat Hm(In, k), Mm(k, In), Sm(k, k);
+  std::memcpy(Hm.d.data(), H, sizeof(double) * In * k);
+  std::memcpy(Mm.d.data(), M, sizeof(double) * In * k);
+  std::memcpy(Sm.d.data(), S, sizeof(double) * k * k);
+  int rc = ntf_update(algo, Hm, Mm, Sm, k);
+  std::memcpy(H, Hm.d.data(), sizeof(double) * In * k);
+  return rc;
+}
+
 // KRP leaving out mode n; out is (prod others) x k column-major.
 void oracle_krp_leave_out_one(int N, const long *dims, const double *const *factors, int k, int n,
                               double *out) {

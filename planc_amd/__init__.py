@@ -8,7 +8,10 @@ Matrices follow Armadillo's column-major layout: a torch tensor of shape
 (strides (1, I0, I0*I1, ...)).  PyTorch only provides device memory and the
 stream; every computation is a HIP kernel of libplanc_hip.so.
 """
+import atexit
 import ctypes
+import sys
+import weakref
 
 import numpy as np
 
@@ -70,11 +73,31 @@ def _is_fortran_tensor(t):
     return True
 
 
+# Contexts still open when the interpreter exits are destroyed by an atexit
+# hook, while the HIP runtime and torch's streams are still alive.  It is
+# registered on the first Context (after torch has initialised HIP, so it
+# runs before torch's own exit handlers).  A context collected during
+# interpreter finalisation is not destroyed: its stream, graphs and scratch
+# go with the process.
+_LIVE = weakref.WeakSet()
+_ATEXIT_REGISTERED = False
+
+
+def close_all():
+    """Destroys every open Context (called at interpreter exit)."""
+    for c in list(_LIVE):
+        try:
+            c.close()
+        except Exception:
+            pass
+
+
 class Context:
     """One planc_ctx (device, stream, scratch).  Calls are ordered on torch's
     current stream of that device."""
 
     def __init__(self, device=0, use_torch_stream=True):
+        global _ATEXIT_REGISTERED
         self.lib = _lib.load()
         self.device = device
         p = ctypes.c_void_p()
@@ -86,13 +109,20 @@ class Context:
             torch = _torch()
             s = torch.cuda.current_stream(device).cuda_stream
             self._check(self.lib.planc_set_stream(self.ctx, ctypes.c_void_p(s)), "planc_set_stream")
+        _LIVE.add(self)
+        if not _ATEXIT_REGISTERED:
+            atexit.register(close_all)
+            _ATEXIT_REGISTERED = True
 
     def close(self):
         if getattr(self, "ctx", None):
             self.lib.planc_destroy(self.ctx)
             self.ctx = None
+        _LIVE.discard(self)
 
     def __del__(self):
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:

@@ -20,6 +20,14 @@ def context(device=0):
     return _CTX[device]
 
 
+def close():
+    """Destroys the contexts this module opened (also done at interpreter
+    exit by planc_amd.close_all)."""
+    while _CTX:
+        _, c = _CTX.popitem()
+        c.close()
+
+
 class CPLoss(torch.autograd.Function):
     @staticmethod
     def forward(fctx, X, *factors):

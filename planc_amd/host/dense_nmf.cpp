@@ -2,6 +2,9 @@
 // on one MI355X.  Same flags; A is generated (rand_uniform, rand_normal,
 // rand_lowrank) from the Philox streams or loaded from a raw-ascii text file;
 // W and H start from U[0,1) drawn with --seed (nmf/nmf.cpp:188-190).
+#include <algorithm>
+#include <cmath>
+
 #include "planc/nmf.hpp"
 #include "planc/parsecommandline.hpp"
 
@@ -40,10 +43,32 @@ class NMFDriver {
             A(i, j) = s;
           }
       }
+      if (pc.adj_rand()) {
+        // A = ceil(kalpha * A + kbeta) with the dense build's kalpha = 1,
+        // kbeta = 0 (nmf/nmf.cpp:39-44,164-167); generated inputs only.
+        for (unsigned long long e = 0; e < m * n; ++e) A.memptr()[e] = std::ceil(A.memptr()[e]);
+      }
     } else {
       if (!A.load_raw_ascii(name)) throw Error("cannot load " + name, PLANC_EINVAL);
       m = A.n_rows;
       n = A.n_cols;
+    }
+    // Input normalisation (nmf/nmf.cpp:174-185): l2 makes every column of
+    // A unit-norm (arma::normalise leaves zero columns zero), max scales A by
+    // 1 / max(A).
+    if (pc.input_normalization() == L2NORM) {
+      for (unsigned long long j = 0; j < n; ++j) {
+        double s = 0;
+        for (unsigned long long i = 0; i < m; ++i) s += A(i, j) * A(i, j);
+        s = std::sqrt(s);
+        if (s > 0)
+          for (unsigned long long i = 0; i < m; ++i) A(i, j) /= s;
+      }
+    } else if (pc.input_normalization() == MAXNORM) {
+      double mx = A.memptr()[0];
+      for (unsigned long long e = 0; e < m * n; ++e) mx = std::max(mx, A.memptr()[e]);
+      const double inv = 1 / mx;
+      for (unsigned long long e = 0; e < m * n; ++e) A.memptr()[e] *= inv;
     }
     MAT W(m, k), H(n, k);
     W.randu((uint64_t)pc.initseed(), 0);

@@ -11,6 +11,9 @@
 
 namespace planc {
 
+// Input normalisation of dense_nmf (normtype, common/utils.h).
+enum normtype { NONE, L2NORM, MAXNORM };
+
 enum { NUMKBLOCKS = 1000, NORMALIZATION, DIMTREE, SYMMETRICREG, ADJRAND, NUMLUCITERS, INITSEED,
        PRECISION, DRYRUN };
 
@@ -38,6 +41,7 @@ class ParseCommandLine {
   int m_num_nodes = 0;
   int m_precision = PLANC_F32;
   bool m_dry_run = false;
+  normtype m_input_normalization = NONE;
 
   void parse_array(char opt, const char *input) {
     std::vector<std::string> toks;
@@ -108,7 +112,14 @@ class ParseCommandLine {
         case 't': m_num_it = atoi(optarg); break;
         case 'n': m_num_nodes = atoi(optarg); break;
         case NUMKBLOCKS: m_num_k_blocks = atoi(optarg); break;
-        case NORMALIZATION: break;
+        case NORMALIZATION: {  // parsecommandline.hpp:210-217
+          std::string t = optarg;
+          if (t == "l2")
+            m_input_normalization = L2NORM;
+          else if (t == "max")
+            m_input_normalization = MAXNORM;
+          break;
+        }
         case DIMTREE: m_dim_tree = atoi(optarg); break;
         case SYMMETRICREG: m_symm_reg = atof(optarg); break;
         case ADJRAND: m_adj_rand = true; break;
@@ -144,7 +155,8 @@ class ParseCommandLine {
   void printConfig() const {
     INFO << "a::" << m_lucalgo << "::i::" << m_Afile_name << "::k::" << m_k << "::m::" << m_globalm
          << "::n::" << m_globaln << "::t::" << m_num_it << "::error::" << m_compute_error
-         << "::dimtree::" << m_dim_tree << "::initseed::" << m_initseed << "::precision::"
+         << "::dimtree::" << m_dim_tree << "::input normalization::" << m_input_normalization
+         << "::adjrand::" << m_adj_rand << "::initseed::" << m_initseed << "::precision::"
          << (m_precision == PLANC_F32 ? "fp32" : m_precision == PLANC_BF16 ? "bf16" : "fp16")
          << "::regW::" << m_regW[0] << "," << m_regW[1] << "::regH::" << m_regH[0] << ","
          << m_regH[1] << std::endl;
@@ -171,6 +183,7 @@ class ParseCommandLine {
   int nodes() const { return m_num_nodes; }
   int max_luciters() const { return m_max_luciters; }
   bool adj_rand() const { return m_adj_rand; }
+  normtype input_normalization() const { return m_input_normalization; }
   int precision() const { return m_precision; }
   bool dry_run() const { return m_dry_run; }
 };

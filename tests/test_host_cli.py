@@ -81,6 +81,47 @@ def test_dense_nmf_matches_oracle(tmp_path, algo):
     assert np.linalg.norm(W - Wr) / np.linalg.norm(Wr) < 1e-3
 
 
+@pytest.mark.skipif(not have_bins(), reason="host drivers not built")
+def test_dense_nmf_normalization_and_adjrand_parsed():
+    p = run([os.path.join(BIN, "dense_nmf"), "-a", "0", "-d", "30 20", "-k", "4", "-i", "rand_lowrank",
+             "--normalization", "l2", "--adjrand", "--dry-run"])
+    assert "input normalization::1" in p.stdout and "adjrand::1" in p.stdout
+    p = run([os.path.join(BIN, "dense_nmf"), "-a", "0", "-d", "30 20", "-k", "4", "-i", "rand_lowrank",
+             "--normalization", "max", "--dry-run"])
+    assert "input normalization::2" in p.stdout and "adjrand::0" in p.stdout
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("norm,adj", [("l2", False), ("max", False), ("max", True)])
+def test_dense_nmf_input_normalization_matches_oracle(tmp_path, norm, adj):
+    """--normalization l2|max and --adjrand change A before the solve exactly as
+    nmf/nmf.cpp:164-185 does (ceil(1*A + 0) for a generated A, then unit
+    columns or A / max(A))."""
+    from oracle import oracle as O
+    from planc_amd import datagen as D
+    m, n, k, t = 120, 90, 6, 6
+    args = [os.path.join(BIN, "dense_nmf"), "-a", "0", "-d", f"{m} {n}", "-k", str(k), "-t", str(t),
+            "-i", "rand_lowrank", "--normalization", norm]
+    if adj:
+        args.append("--adjrand")
+    rel = _relerrs(run(args).stdout)
+    Wt = D.uniform(1210873, 1, (m, k))
+    Ht = D.uniform(1210873, 2, (k, n))
+    A = Wt @ Ht
+    if adj:
+        A = np.ceil(A)
+    if norm == "l2":
+        A = A / np.linalg.norm(A, axis=0, keepdims=True)
+    else:
+        A = A * (1.0 / A.max())
+    A = A.astype(np.float32).astype(np.float64)
+    W0 = D.uniform(193957, 0, (m, k)).astype(np.float32).astype(np.float64)
+    H0 = D.uniform(193957, 1, (n, k)).astype(np.float32).astype(np.float64)
+    _, _, rel_r, _, rc = O.nmf(0, A, W0, H0, t)
+    assert rc == 0
+    assert np.abs(rel - rel_r).max() < 1e-4
+
+
 @pytest.mark.gpu
 def test_dense_ntf_runs_and_error_decreases():
     p = run([os.path.join(BIN, "dense_ntf"), "-a", "1", "-d", "30 31 32", "-k", "5", "-t", "8",

@@ -158,28 +158,201 @@ def test_bpp_nmf_halfsteps_satisfy_kkt():
     assert np.abs((H * Y).sum()) < 1e-8
 
 
-def test_hals_column_step_is_coordinate_minimiser():
-    rng = np.random.default_rng(7)
-    A = _lowrank(rng, 25, 20, 3)
-    W0, H0 = rng.random((25, 3)), rng.random((20, 3))
-    # one iteration; then perturbing any single column of H away from the
-    # iterate cannot lower the objective by more than rounding: the last
-    # W sweep uses H, and H's last column step was exact w.r.t. the W before
-    # that sweep -- check the H half-step directly with W fixed instead.
+def _hals_case(seed):
+    rng = np.random.default_rng(seed)
+    m, n, k = 25, 20, 4
+    A = _lowrank(rng, m, n, 2) + 0.05 * rng.random((m, n))
+    # initial factors far from A's structure, so that some unconstrained
+    # coordinate minimisers are negative and the 1e-17 clamp is exercised
+    W0 = rng.random((m, k)) ** 4
+    H0 = 3.0 * rng.random((n, k))
+    return A, W0, H0
+
+
+@pytest.mark.parametrize("seed", [7, 8, 9])
+def test_hals_h_sweep_columns_are_clamped_coordinate_minimisers(seed):
+    """nmf/hals.hpp:67-75: after the constructor's normalize_by_W, column c of
+    H is the minimiser of ||A - W H^T||^2 over H(:, c) >= 1e-17 given the
+    columns before it (already swept) and after it (not yet).  Checked
+    through the gradient of the explicit residual, not the Gram form."""
+    A, W0, H0 = _hals_case(seed)
+    k = W0.shape[1]
     W, H, rel, obj, rc = O.nmf(1, A, W0, H0, 1)
-    W1 = W / np.linalg.norm(W, axis=0)
-    # rerun one H sweep from (W1, H) in numpy-free form: oracle with W fixed is
-    # not exposed, so check the optimality of the final column of H against W1
-    # scaled back: column k-1 of H minimises ||A - W H^T|| given the others.
-    Hs = H * np.linalg.norm(W, axis=0)
-    x = Hs.shape[1] - 1
-    R = A - W1 @ Hs.T + np.outer(W1[:, x], Hs[:, x])
+    nrm = np.linalg.norm(W0, axis=0)
+    W1, H1 = W0 / nrm, H0 * nrm  # HALSNMF constructor (nmf/hals.hpp:46)
+    # The iteration ends with the W sweep (unit columns) and normalize_by_W,
+    # whose norms are 1 to rounding, so H is the H sweep's result.
+    scale = np.abs(A).sum()
+    clamped = 0
+    for c in range(k):
+        Hc = np.concatenate([H[:, :c + 1], H1[:, c + 1:]], axis=1)
+        g = -(A - W1 @ Hc.T).T @ W1[:, c]
+        at_floor = H[:, c] <= 1e-16
+        clamped += int(at_floor.sum())
+        assert np.all(np.abs(g[~at_floor]) <= 1e-11 * scale)
+        assert np.all(g[at_floor] >= -1e-11 * scale)
+        # the floor is 1e-17, not 0 (common/utils.hpp:59-64 with repl = 1e-17)
+        np.testing.assert_allclose(H[at_floor, c], 1e-17, rtol=1e-12)
+    if seed == 7:
+        assert clamped > 0
 
-    def f(h):
-        return np.linalg.norm(R - np.outer(W1[:, x], h)) ** 2
 
-    best = np.maximum(R.T @ W1[:, x], 0)  # exact minimiser for a unit column
-    assert f(best) <= f(best + 1e-3 * rng.standard_normal(best.shape).clip(0)) + 1e-12
+@pytest.mark.parametrize("seed", [7, 8, 9])
+def test_hals_w_sweep_columns_are_normalised_coordinate_minimisers(seed):
+    """nmf/hals.hpp:89-102: column c of W is the minimiser w* of
+    ||A - W H^T||^2 over W(:, c) (others: swept columns before c, the
+    constructor's W after c), scaled by HtH(c, c), floored at 1e-17 and
+    normalised.  w* comes from the 1-D least squares on the residual."""
+    A, W0, H0 = _hals_case(seed)
+    k = W0.shape[1]
+    W, H, rel, obj, rc = O.nmf(1, A, W0, H0, 1)
+    W1 = W0 / np.linalg.norm(W0, axis=0)
+    for c in range(k):
+        others = [t for t in range(k) if t != c]
+        Wc = np.concatenate([W[:, :c], W1[:, c:]], axis=1)
+        R = A - Wc[:, others] @ H[:, others].T
+        hc = H[:, c]
+        wstar = R @ hc / (hc @ hc)
+        x = np.maximum(wstar * (hc @ hc), 1e-17)
+        np.testing.assert_allclose(W[:, c], x / np.linalg.norm(x), rtol=1e-9, atol=1e-14)
+
+
+def _krp_brute(F, n):
+    """Khatri-Rao product of the factors other than n, rows ordered with the
+    lowest remaining mode fastest, by explicit multi-index loops."""
+    import itertools
+    others = [f for i, f in enumerate(F) if i != n]
+    dims = [f.shape[0] for f in others]
+    k = F[0].shape[1]
+    K = np.zeros((int(np.prod(dims)), k))
+    for idx in itertools.product(*[range(d) for d in reversed(dims)]):
+        idx = idx[::-1]  # idx[0] is the lowest remaining mode
+        row, mul = 0, 1
+        for i, d in zip(idx, dims):
+            row += i * mul
+            mul *= d
+        v = np.ones(k)
+        for i, f in zip(idx, others):
+            v = v * f[i]
+        K[row] = v
+    return K
+
+
+def _unfold(X, n):
+    """Mode-n unfolding whose columns follow _krp_brute's row order."""
+    return np.moveaxis(X, n, 0).reshape(X.shape[n], -1, order="F")
+
+
+def _ntf_mode_case(seed, n=1, k=4, dims=(6, 5, 7)):
+    rng = np.random.default_rng(seed)
+    F = [rng.random((d, k)) for d in dims]
+    F = [f / np.linalg.norm(f, axis=0) for f in F]  # unit columns: S(i, i) = 1
+    X = np.asfortranarray(np.einsum("ir,jr,kr->ijk", *[rng.random((d, k)) for d in dims])
+                          + 0.1 * rng.random(dims))
+    K = _krp_brute(F, n)
+    Xn = _unfold(X, n)
+    return rng, F, X, K, Xn
+
+
+@pytest.mark.parametrize("seed", [11, 12, 13])
+def test_ntf_hals_mode_update_columns_are_clamped_coordinate_minimisers(seed):
+    """ntf/ntfhals.hpp:13-29: every column of the sweep minimises
+    0.5 ||X_(n) - H K^T||^2 over H(:, i) >= 1e-17 with the columns before i
+    already updated and those after i not yet."""
+    rng, F, X, K, Xn = _ntf_mode_case(seed)
+    H0 = 5.0 * rng.random((Xn.shape[0], K.shape[1]))
+    Hn, rc = O.ntf_mode_update(1, H0, K.T @ Xn.T, K.T @ K)
+    assert rc == 0
+    scale = np.abs(Xn).sum() * np.abs(K).max()
+    clamped = 0
+    for c in range(K.shape[1]):
+        Hc = np.concatenate([Hn[:, :c + 1], H0[:, c + 1:]], axis=1)
+        g = -(Xn - Hc @ K.T) @ K[:, c]
+        at_floor = Hn[:, c] <= 1e-16
+        clamped += int(at_floor.sum())
+        assert np.all(np.abs(g[~at_floor]) <= 1e-12 * scale)
+        assert np.all(g[at_floor] >= -1e-12 * scale)
+        assert np.all(Hn[at_floor, c] == 1e-17)
+    assert clamped > 0
+
+
+@pytest.mark.parametrize("seed", [14, 15])
+def test_ntf_bpp_mode_update_satisfies_kkt(seed):
+    """ntf/ntfanlsbpp.hpp:15-57: the factor solves min 0.5||X_(n) - H K^T||^2
+    over H >= 0 exactly: H >= 0, gradient >= 0, complementary slackness."""
+    rng, F, X, K, Xn = _ntf_mode_case(seed, k=5)
+    H0 = rng.random((Xn.shape[0], K.shape[1]))
+    Hn, rc = O.ntf_mode_update(2, H0, K.T @ Xn.T, K.T @ K)
+    assert rc == 0
+    G = -(Xn - Hn @ K.T) @ K
+    scale = np.abs(Xn).sum() * np.abs(K).max()
+    assert Hn.min() >= 0
+    assert G.min() >= -1e-12 * scale
+    assert np.abs(Hn * G).sum() <= 1e-12 * scale * np.abs(Hn).sum()
+    assert (Hn == 0).any() or G.min() > -1e-12 * scale
+
+
+def test_ntf_mu_exact_factorisation_shows_eps_1e6():
+    """ntf/ntfmu.hpp:11-16: H <- H .* M^T ./ (H S + 1e-6).  At an exact
+    factorisation X_(n) = H* K^T the numerator equals H* S, so the step
+    shrinks each entry by exactly 1e-6 / (H*S + 1e-6); with H* of order 1e-4
+    that is a relative change near 1e-3, which the 1e-17 of the NMF rule
+    would not produce."""
+    rng = np.random.default_rng(16)
+    dims, k, n = (6, 5, 7), 3, 1
+    F = [rng.random((d, k)) for d in dims]
+    F = [f / np.linalg.norm(f, axis=0) for f in F]
+    Hstar = 1e-4 * (0.5 + rng.random((dims[n], k)))
+    F[n] = Hstar
+    X = np.asfortranarray(np.einsum("ir,jr,kr->ijk", *F))
+    K = _krp_brute(F, n)
+    Xn = _unfold(X, n)
+    Hn, rc = O.ntf_mode_update(0, Hstar, K.T @ Xn.T, K.T @ K)
+    HS = Hstar @ (K.T @ K)
+    shrink = (Hstar - Hn) / Hstar
+    np.testing.assert_allclose(shrink, 1e-6 / (HS + 1e-6), rtol=1e-6)
+
+
+@pytest.mark.parametrize("seed", [17, 18, 19])
+def test_ntf_mu_mode_update_does_not_increase_objective(seed):
+    rng, F, X, K, Xn = _ntf_mode_case(seed)
+    H0 = rng.random((Xn.shape[0], K.shape[1]))
+    Hn, rc = O.ntf_mode_update(0, H0, K.T @ Xn.T, K.T @ K)
+
+    def f(H):
+        return 0.5 * np.linalg.norm(Xn - H @ K.T) ** 2
+    assert f(Hn) <= f(H0) * (1 + 1e-14)
+    assert f(Hn) < f(H0)
+
+
+@pytest.mark.parametrize("algo", [0, 1, 2])
+def test_ntf_iteration_equals_mode_updates_on_explicit_krp(algo):
+    """ntf/auntf.hpp:128-177 wiring: per mode S = K^T K (the Hadamard of the
+    other Grams), M = K^T X_(n)^T with the brute-force Khatri-Rao product,
+    the (pinned) mode update, then normalize(mode) replacing lambda
+    (common/ncpfactors.hpp:346-351); the constructor's normalize() first."""
+    rng = np.random.default_rng(21 + algo)
+    dims, k = (5, 4, 6), 3
+    X = np.asfortranarray(np.einsum("ir,jr,kr->ijk", *[rng.random((d, k)) for d in dims])
+                          + 0.02 * rng.random(dims))
+    init = [rng.random((d, k)) for d in dims]
+    Fo, lam_o, rel_o, rc = O.ntf(algo, X, init, 2, dimtree=0)
+    assert rc == 0
+    F = [f / np.linalg.norm(f, axis=0) for f in init]
+    lam = np.prod([np.linalg.norm(f, axis=0) for f in init], axis=0)
+    for it in range(2):
+        for n in range(3):
+            K = _krp_brute(F, n)
+            Hn, rc = O.ntf_mode_update(algo, F[n], K.T @ _unfold(X, n).T, K.T @ K)
+            assert rc == 0
+            lam = np.linalg.norm(Hn, axis=0)
+            F[n] = Hn / np.where(lam > 0, lam, 1.0)
+        Xh = np.einsum("r,ir,jr,kr->ijk", lam, *F)
+        np.testing.assert_allclose(rel_o[it], np.linalg.norm(X - Xh) / np.linalg.norm(X),
+                                   rtol=1e-9)
+    for a, b in zip(Fo, F):
+        np.testing.assert_allclose(a, b, rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(lam_o, lam, rtol=1e-9)
 
 
 @pytest.mark.parametrize("algo", [0, 1, 2])
