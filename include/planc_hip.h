@@ -205,6 +205,22 @@ int planc_bpp(planc_ctx *ctx, const double *d_G, int k, const float *d_B, long b
 int planc_cp_loss_fwd(planc_ctx *ctx, const float *d_X, int nmodes, const long *dims,
                       const float *const *d_factors, int k, double *h_loss);
 
+/* The same loss, allocating nothing: no reference counterpart.  For callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving or training loops, harnesses that forbid hipMalloc).  Same
+ * arithmetic, block partition and summation order as planc_cp_loss_fwd, but
+ * the block partials live in a fixed per-GPU device array of 16384 doubles
+ * (128 KiB, held until the process ends) instead of context scratch, and the
+ * loss is written to the caller's device double *d_loss.  Stream-ordered:
+ * returns once enqueued, no synchronisation.  Returns PLANC_EINVAL when the
+ * shape needs more than 16384 partials (the right block of modes -- the
+ * modes after the balanced split -- longer than about 1.8 million rows).
+ * Every call on one GPU shares that array: calls that overlap on one GPU
+ * (two contexts or streams in flight at once) race and must be ordered by
+ * the caller. */
+int planc_cp_loss_fwd_noalloc(planc_ctx *ctx, const float *d_X, int nmodes, const long *dims,
+                              const float *const *d_factors, int k, double *d_loss);
+
 /* Gradient of that loss with respect to every factor:
  * d_grads[n] (I_n x k) = A_n * (Hadamard of the other Grams) - MTTKRP_n^T,
  * the transposed NTFNES gradient S*Y - M (ntf/ntfnes.hpp:138-147) without

@@ -319,6 +319,18 @@ class Context:
                                                ctypes.byref(out)), "planc_cp_loss_fwd")
         return out.value
 
+    def cp_loss_noalloc(self, X, factors, out):
+        """Same loss through planc_cp_loss_fwd_noalloc: written to out (a
+        device float64 tensor with one element), stream-ordered; the library
+        allocates nothing for it."""
+        N = len(factors)
+        cd = (ctypes.c_long * N)(*[f.shape[0] for f in factors])
+        arr = (ctypes.c_void_p * N)(*[f.data_ptr() for f in factors])
+        self._check(self.lib.planc_cp_loss_fwd_noalloc(self.ctx, _ptr(X), N, cd, arr,
+                                                       factors[0].shape[1], _ptr(out)),
+                    "planc_cp_loss_fwd_noalloc")
+        return out
+
     def cp_loss_grad(self, X, factors):
         """[A_n S_n - MTTKRP_n^T] for every mode (I_n x k, column-major)."""
         N = len(factors)

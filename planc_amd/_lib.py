@@ -94,6 +94,8 @@ SIGNATURES = {
                                 ctypes.POINTER(c_void_p), c_int, c_int, c_void_p, c_long]),
     "planc_cp_loss_fwd": (c_int, [c_void_p, c_void_p, c_int, ctypes.POINTER(c_long),
                                   ctypes.POINTER(c_void_p), c_int, ctypes.POINTER(c_double)]),
+    "planc_cp_loss_fwd_noalloc": (c_int, [c_void_p, c_void_p, c_int, ctypes.POINTER(c_long),
+                                          ctypes.POINTER(c_void_p), c_int, c_void_p]),
     "planc_cp_loss_bwd": (c_int, [c_void_p, c_void_p, c_int, ctypes.POINTER(c_long),
                                   ctypes.POINTER(c_void_p), c_int, ctypes.POINTER(c_void_p)]),
     "planc_bpp": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_long, c_long, c_long, c_void_p,

@@ -174,6 +174,9 @@ int planc_create(int device, planc_ctx **out) {
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
     c->num_cu = prop.multiProcessorCount;
+  // Locates this device's copy of the allocation-free CP-loss partials; a
+  // failure is reported by planc_cp_loss_fwd_noalloc, not here.
+  (void)cp_noalloc_setup(device);
   *out = c;
   return PLANC_OK;
 }
@@ -802,6 +805,16 @@ int planc_cp_loss_fwd(planc_ctx *ctx, const float *d_X, int nmodes, const long *
   PLANC_HIP_CHECK(ctx, hipMemcpyAsync(h_loss, dl, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   PLANC_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   return PLANC_OK;
+}
+
+int planc_cp_loss_fwd_noalloc(planc_ctx *ctx, const float *d_X, int nmodes, const long *dims,
+                              const float *const *d_factors, int k, double *d_loss) {
+  if (check_ctx(ctx)) return PLANC_EINVAL;
+  if (!d_X || !dims || !d_factors || !d_loss || nmodes < 2 || nmodes > 7 || k <= 0 || k > 64)
+    return set_error(ctx, PLANC_EINVAL, "planc_cp_loss_fwd_noalloc: bad arguments");
+  for (int i = 0; i < nmodes; ++i)
+    if (dims[i] <= 0) return set_error(ctx, PLANC_EINVAL, "planc_cp_loss_fwd_noalloc: empty mode");
+  return cp_loss_forward_noalloc(ctx, d_X, nmodes, dims, d_factors, k, d_loss);
 }
 
 int planc_cp_loss_bwd(planc_ctx *ctx, const float *d_X, int nmodes, const long *dims,

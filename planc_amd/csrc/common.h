@@ -215,6 +215,11 @@ int bpp_solve(planc_ctx *ctx, const double *G, int k, const float *B, long b_cs,
 // cploss.hip
 int cp_loss_forward(planc_ctx *ctx, const float *X, int N, const long *dims, const float *const *F,
                     int k, double *d_loss);
+// Allocation-free forward: block partials in a per-device static array
+// located once per device by cp_noalloc_setup (called from planc_create).
+int cp_noalloc_setup(int device);
+int cp_loss_forward_noalloc(planc_ctx *ctx, const float *X, int N, const long *dims,
+                            const float *const *F, int k, double *d_loss);
 int cp_grad_from_mttkrp(planc_ctx *ctx, const float *A, long I, int k, const double *S, const float *M,
                         float *grad);
 int cp_split(int N, const long *dims);

@@ -13,6 +13,9 @@
 // transposed NTFNES gradient S Y - M (ntf/ntfnes.hpp:138-147) without the
 // lambda shift.  All N MTTKRPs come from the two partial contractions of
 // the dimension tree, each one pass over X with the KRP fused.
+#include <mutex>
+#include <string>
+
 #include "common.h"
 
 namespace planc {
@@ -154,8 +157,19 @@ int balanced(int N, const long *dims) {
 
 }  // namespace
 
-int cp_loss_forward(planc_ctx *ctx, const float *X, int N, const long *dims, const float *const *F,
-                    int k, double *d_loss) {
+namespace {
+
+// Block partition of one forward evaluation: the grid and how many FP64
+// block partials it writes.  Both entry points use the same partition, so
+// their sums are formed in the same order.
+struct CPLaunch {
+  CPParams p;
+  dim3 grid;
+  long nparts;
+};
+
+int cp_loss_plan(planc_ctx *ctx, const float *X, int N, const long *dims, const float *const *F,
+                 int k, CPLaunch *out) {
   if (N < 2 || N > 7 || k <= 0 || k > 64) return set_error(ctx, PLANC_EINVAL, "cp_loss: 2..7 modes, k 1..64");
   int s = balanced(N, dims);
   CPParams p{};
@@ -183,22 +197,93 @@ int cp_loss_forward(planc_ctx *ctx, const float *X, int N, const long *dims, con
   if (want > ltiles) want = ltiles;
   p.ltiles_per_block = (int)((ltiles + want - 1) / want);
   long lgroups = (ltiles + p.ltiles_per_block - 1) / p.ltiles_per_block;
-  long nparts = rblocks * lgroups;
-  p.part = (double *)ws_get(ctx, 3, sizeof(double) * (size_t)nparts);
-  if (!p.part) return set_error(ctx, PLANC_ENOMEM, "cp_loss: workspace");
-  dim3 grid((unsigned)rblocks, (unsigned)lgroups);
+  out->p = p;
+  out->grid = dim3((unsigned)rblocks, (unsigned)lgroups);
+  out->nparts = rblocks * lgroups;
+  return PLANC_OK;
+}
+
+// Every block writes its own partial (nothing is accumulated into part), and
+// the final kernel reads exactly nparts of them: part needs no clearing.
+int cp_loss_launch(planc_ctx *ctx, CPLaunch &c, double *part, double *d_loss) {
+  c.p.part = part;
+  const int k = c.p.k;
   if (k <= 16)
-    cp_loss_fwd_kernel<16><<<grid, 256, 0, ctx->stream>>>(p);
+    cp_loss_fwd_kernel<16><<<c.grid, 256, 0, ctx->stream>>>(c.p);
   else if (k <= 32)
-    cp_loss_fwd_kernel<32><<<grid, 256, 0, ctx->stream>>>(p);
+    cp_loss_fwd_kernel<32><<<c.grid, 256, 0, ctx->stream>>>(c.p);
   else if (k <= 48)
-    cp_loss_fwd_kernel<48><<<grid, 256, 0, ctx->stream>>>(p);
+    cp_loss_fwd_kernel<48><<<c.grid, 256, 0, ctx->stream>>>(c.p);
   else
-    cp_loss_fwd_kernel<64><<<grid, 256, 0, ctx->stream>>>(p);
+    cp_loss_fwd_kernel<64><<<c.grid, 256, 0, ctx->stream>>>(c.p);
   PLANC_LAUNCH_CHECK(ctx);
-  sum_half_kernel<<<1, 64, 0, ctx->stream>>>(p.part, nparts, d_loss);
+  sum_half_kernel<<<1, 64, 0, ctx->stream>>>(part, c.nparts, d_loss);
   PLANC_LAUNCH_CHECK(ctx);
   return PLANC_OK;
+}
+
+// Partials of the allocation-free forward: a file-scope device array, one
+// copy per GPU, held until the process ends.  Its name is a host-side
+// stand-in; kernels get the per-device address that hipGetSymbolAddress
+// returns at context creation (cp_noalloc_setup), never the name itself.
+constexpr long kNoallocParts = 16384;  // 128 KiB
+static __device__ __align__(256) double scratch[kNoallocParts];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices];
+int g_scratch_state[kMaxDevices];  // 0 not set up, 1 ready, -1 lookup failed
+
+}  // namespace
+
+int cp_loss_forward(planc_ctx *ctx, const float *X, int N, const long *dims, const float *const *F,
+                    int k, double *d_loss) {
+  CPLaunch c;
+  int rc = cp_loss_plan(ctx, X, N, dims, F, k, &c);
+  if (rc) return rc;
+  double *part = (double *)ws_get(ctx, 3, sizeof(double) * (size_t)c.nparts);
+  if (!part) return set_error(ctx, PLANC_ENOMEM, "cp_loss: workspace");
+  return cp_loss_launch(ctx, c, part, d_loss);
+}
+
+// Called by planc_create with `device` current: looks up this device's copy
+// of the partials array once per process.
+int cp_noalloc_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return PLANC_EUNSUPPORTED;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_state[device] != 0) return g_scratch_state[device] > 0 ? PLANC_OK : PLANC_EHIP;
+  void *ptr = nullptr;
+  hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr) {
+    g_scratch_state[device] = -1;
+    return PLANC_EHIP;
+  }
+  g_scratch_ptr[device] = (double *)ptr;
+  g_scratch_state[device] = 1;
+  return PLANC_OK;
+}
+
+int cp_loss_forward_noalloc(planc_ctx *ctx, const float *X, int N, const long *dims,
+                            const float *const *F, int k, double *d_loss) {
+  const int dev = ctx->device;
+  double *part = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (dev >= 0 && dev < kMaxDevices && g_scratch_state[dev] > 0) part = g_scratch_ptr[dev];
+  }
+  if (!part)
+    return set_error(ctx, PLANC_EHIP,
+                     "cp_loss_fwd_noalloc: the device's partials array could not be located "
+                     "(hipGetSymbolAddress failed at context creation)");
+  CPLaunch c;
+  int rc = cp_loss_plan(ctx, X, N, dims, F, k, &c);
+  if (rc) return rc;
+  if (c.nparts > kNoallocParts)
+    return set_error(ctx, PLANC_EINVAL,
+                     "cp_loss_fwd_noalloc: " + std::to_string(c.nparts) +
+                         " block partials exceed the fixed array of " + std::to_string(kNoallocParts) +
+                         " (the right block of modes has too many rows); use planc_cp_loss_fwd");
+  return cp_loss_launch(ctx, c, part, d_loss);
 }
 
 int cp_grad_from_mttkrp(planc_ctx *ctx, const float *A, long I, int k, const double *S, const float *M,
