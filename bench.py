@@ -1,36 +1,42 @@
 #!/usr/bin/env python
 """Benchmark of the MI355X NMF/NTF hot path (one JSON line on stdout).
 
-Workload (the measured `value`): NTF-HALS on a 256 x 256 x 256 FP32 tensor
-built from a CP rank-32 model with U[0,1) factors plus |N(0, 0.01*mean)|
-noise, rank k = 32, dimension-tree MTTKRP, relative error every iteration.
-A "step" is one outer iteration of computeNTF (all three mode updates and
-the error).  W warmup iterations run untimed, then exactly K iterations are
-timed between two device synchronisations; value = K / elapsed.
+Workload of the measured `value`: NMF-MU (MUNMF::computeNMF, nmf/mu.hpp:48-91)
+on A = W0 H0 + 1 % noise, 65536 x 16384 FP32 (rank-64 U[0,1) factors), rank
+k = 64, with the two bulk products W^T A and A H on BF16 MFMA (FP32
+accumulation, per-column power-of-two scales) and the Grams and error in
+FP64.  A "step" is one outer iteration of computeNMF (H update, W update,
+relative error).  W warmup iterations run untimed from the seeded initial
+factors; then, from the same initial factors, exactly K iterations are timed
+between two device synchronisations in one computeNMF call; value = K /
+elapsed.  A and its BF16 copy are resident in HBM before the timed region
+(the one-time conversion is timed separately).
 
-Also reported on the same line:
-  roofline      -- the dominant kernel (the right partial MTTKRP
-                   X_(65536 x 256) * A_2, one launch per outer iteration)
-                   timed with HIP events on the stream it runs on; achieved =
-                   algorithmic bytes (X read once + projection written) / time.
-  cpu_baseline  -- the FP64 oracle restatement of the reference (OpenMP, all
-                   allowed host threads) on a bounded sample of the same
-                   workload from the same initial factors.
-  parity        -- GPU FP32 vs the FP64 oracle relative-error trajectory over
-                   that sample.
-  legs          -- the mixed-precision comparisons of the headline metric:
-                   NTF-HALS 256^3 in BF16/FP16, NMF-MU and NMF-HALS on
-                   65536 x 16384 (k = 64) in FP32 vs BF16, the MTTKRP of a
-                   128^4 tensor (k = 48, 1.07 GB, beyond the 256 MB Infinity
-                   Cache) for the HBM roofline fraction, NTF-HALS on that
-                   tensor in FP32 vs FP16, and the hierarchical NMF.
+Also on the same line:
+  fp32 / ratio  -- the same K iterations with FP32 products (the plain GPU
+                   form) and the BF16/FP32 ratio of outer iterations/s; both
+                   final relative errors next to the FP64 oracle's
+                   (tests/golden/nmf_c3a_fp64.json, same input and init) and
+                   whether both lie within 1e-3 of it.
+  roofline      -- the dominant kernel of the step, the BF16 W^T A product
+                   (k x 16384 = W^T (64 x 65536) A), timed with HIP events on
+                   the stream it runs on; achieved = algorithmic bytes / time.
+  cpu_baseline  -- the FP64 oracle restatement of MUNMF::computeNMF on the
+                   same A and init on the host's cores, its GEMMs through a
+                   host CBLAS when one loads (MKL), for a bounded number of
+                   iterations; plus an FP32 variant.
+  legs          -- NMF-HALS and NMF-ANLS-BPP on the same A in FP32 vs BF16,
+                   NTF-HALS on a 128^4 tensor (k = 48) in FP32 vs FP16 with
+                   its MTTKRP's HBM fraction, NTF-HALS 256^3 (k = 32) against
+                   the FP64 oracle, and the hierarchical rank-2 NMF.
 
-Usage: python bench.py [--gpus 1] [--steps 30] [--warmup 3] [--no-legs]
-       [--no-cpu] [--profile-kernel]
+Usage: python bench.py [--gpus 1] [--steps 50] [--warmup 3] [--no-legs]
+       [--no-cpu] [--cpu-iters 3] [--profile-kernel]
 """
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -41,6 +47,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "outer iters/sec NMF-MU & NTF-HALS: bf16-MFMA vs fp32 at equal rel. error"
 HBM_PEAK_GBS = 8000.0
+M_C3, N_C3, K_C3, R_C3, STREAM_C3 = 65536, 16384, 64, 64, 3
+FP64_FIXTURE = os.path.join(ROOT, "tests", "golden", "nmf_c3a_fp64.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "gemm_tn_bf16_65536x16384_k64_traffic.json")
 
 
 def log(msg):
@@ -68,27 +77,116 @@ def event_time_ms(torch, fn, reps):
     return e0.elapsed_time(e1) / reps
 
 
+def host_info():
+    """lscpu summary and the threads this process may use."""
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            key, _, val = line.partition(":")
+            if key.strip() in ("Model name", "CPU(s)", "Socket(s)", "Core(s) per socket",
+                               "Thread(s) per core", "NUMA node(s)"):
+                info[key.strip()] = val.strip()
+    except Exception as e:  # lscpu missing: record why
+        info["lscpu"] = f"unavailable: {e}"
+    try:
+        info["affinity_cpus"] = len(os.sched_getaffinity(0))
+    except Exception:
+        info["affinity_cpus"] = os.cpu_count()
+    info["OMP_NUM_THREADS"] = os.environ.get("OMP_NUM_THREADS")
+    return info
+
+
+def make_nmf_problem(P, D, ctx, torch, m, n, r, k, stream):
+    """A = W0 H0 + 0.01 mean(W0 H0) |N(0,1)| on the device, and the seeded
+    U[0,1) initial W (m x k) and H (n x k)."""
+    W0 = P.colmajor(D.uniform(D.DATA_SEED, 16 * stream + 0, (m, r)).astype(np.float32))
+    H0 = P.colmajor(D.uniform(D.DATA_SEED, 16 * stream + 1, (r, n)).astype(np.float32))
+    mean = float((W0.double().mean(0) * H0.double().mean(1)).sum().item())
+    A = ctx.gen_lowrank(W0, H0, 0.01 * mean, D.NOISE_SEED, stream)
+    del W0, H0
+    Wi = ctx.gen_uniform((m, k), D.INIT_SEED, 16 * stream + 0)
+    Hi = ctx.gen_uniform((n, k), D.INIT_SEED, 16 * stream + 1)
+    return A, Wi, Hi
+
+
+def norm2_fp64(torch, A, chunk=1024):
+    """||A||_F^2 in FP64 (column blocks, so no full FP64 copy is formed)."""
+    s = 0.0
+    for j in range(0, A.shape[1], chunk):
+        s += float((A[:, j:j + chunk].double() ** 2).sum().item())
+    return s
+
+
+def nmf_timed(P, ctx, torch, A, Wi, Hi, algo, warmup, steps, dtype, A16=None, cs=None,
+              A16T=None, normA2=0.0):
+    """warmup untimed iterations, then `steps` timed iterations from the same
+    initial factors in one computeNMF call; returns (iters/s, seconds, relerr).
+    A16T / normA2 are the constructor-time At and ||A||^2 (made once)."""
+    kw = dict(dtype=dtype, A16=A16, colscale=cs, A16T=A16T, normA2=normA2)
+    if warmup:
+        W, H = Wi.clone(), Hi.clone()
+        ctx.nmf_run(A, W, H, algo, warmup, **kw)
+    W, H = Wi.clone(), Hi.clone()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rel = ctx.nmf_run(A, W, H, algo, steps, **kw)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    return steps / el, el, rel
+
+
+def fp64_reference(name, steps):
+    """FP64 oracle relative error after `steps` iterations (same input and
+    init), from the committed fixture; None when it does not cover them."""
+    if not os.path.exists(FP64_FIXTURE):
+        return None
+    fx = json.load(open(FP64_FIXTURE))
+    traj = fx.get(name)
+    if not traj or steps > len(traj) or steps < 1:
+        return None
+    return float(traj[steps - 1]), [float(x) for x in traj[:steps]]
+
+
+def compare_precisions(P, ctx, torch, A, Wi, Hi, A16, cs, A16T, normA2, algo, name, warmup, steps):
+    out = {}
+    ref = fp64_reference(name, steps)
+    for label, dt in (("fp32", P.F32), ("bf16", P.BF16)):
+        b = dt != P.F32
+        rate, el, rel = nmf_timed(P, ctx, torch, A, Wi, Hi, algo, warmup, steps, dt,
+                                  A16 if b else None, cs if b else None, A16T if b else None,
+                                  normA2)
+        out[label] = {"iters_per_s": round(rate, 3), "ms_per_iter": round(1000 * el / steps, 4),
+                      "final_relerr": float(rel[-1])}
+        if ref is not None:
+            out[label]["final_relerr_minus_fp64"] = float(rel[-1]) - ref[0]
+            out[label]["max_abs_traj_diff_vs_fp64"] = float(np.abs(rel - np.array(ref[1])).max())
+            out[label]["monotone"] = bool(np.all(np.diff(rel) <= 1e-12))
+    out["speedup_bf16_over_fp32"] = round(out["bf16"]["iters_per_s"] / out["fp32"]["iters_per_s"], 4)
+    if ref is not None:
+        out["final_relerr_fp64_oracle"] = ref[0]
+        out["equal_error_within_1e-3"] = bool(
+            abs(out["bf16"]["final_relerr_minus_fp64"]) < 1e-3 and
+            abs(out["fp32"]["final_relerr_minus_fp64"]) < 1e-3)
+    out["iterations"] = steps
+    return out
+
+
 def make_ntf_problem(P, D, ctx, torch, dims, k, rank, stream):
     """CP tensor generated on the device from host-drawn factors."""
     F = [P.colmajor(D.uniform(D.DATA_SEED, 16 * stream + i, (d, rank)).astype(np.float32))
          for i, d in enumerate(dims)]
-    # mean of the noiseless tensor = prod_i colmean -> sum over components
     means = np.prod([f.cpu().numpy().astype(np.float64).mean(0) for f in F], axis=0).sum()
     X = ctx.gen_cp_tensor(F, 0.01 * means, D.NOISE_SEED, stream)
     init = [f.astype(np.float32) for f in D.init_factors([(d, k) for d in dims], stream=stream)]
     return X, init
 
 
-def run_ntf(P, ctx, torch, X, init, iters, dtype, X16=None):
-    F = [P.colmajor(f) for f in init]
-    lam, rel = ctx.ntf_run(X, F, P.HALS, iters, dtype=dtype, X16=X16)
-    return F, rel
-
-
 def ntf_rate(P, ctx, torch, X, init, warmup, steps, dtype, X16=None):
     F = [P.colmajor(f) for f in init]
     if warmup:
         ctx.ntf_run(X, F, P.HALS, warmup, dtype=dtype, X16=X16)
+    F = [P.colmajor(f) for f in init]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     lam, rel = ctx.ntf_run(X, F, P.HALS, steps, dtype=dtype, X16=X16)
@@ -97,42 +195,50 @@ def ntf_rate(P, ctx, torch, X, init, warmup, steps, dtype, X16=None):
     return steps / el, el, rel
 
 
-def nmf_leg(P, D, ctx, torch, algo, m, n, k, iters, stream=3):
-    W0 = P.colmajor(D.uniform(D.DATA_SEED, 16 * stream + 0, (m, k)).astype(np.float32))
-    H0 = P.colmajor(D.uniform(D.DATA_SEED, 16 * stream + 1, (k, n)).astype(np.float32))
-    mean = float((W0.double().mean(0) * H0.double().mean(1)).sum().item())
-    A = ctx.gen_lowrank(W0, H0, 0.01 * mean, D.NOISE_SEED, stream)
-    del W0, H0
-    Wi = ctx.gen_uniform((m, k), D.INIT_SEED, 16 * stream + 0)
-    Hi = ctx.gen_uniform((n, k), D.INIT_SEED, 16 * stream + 1)
-    out = {}
-    for name, dt in (("fp32", P.F32), ("bf16", P.BF16)):
-        A16 = cs = None
-        if dt != P.F32:
-            A16, cs = ctx.convert(A, dt, scaled=True)
-        W, H = Wi.clone(), Hi.clone()
-        ctx.nmf_run(A, W, H, algo, 2, dtype=dt, A16=A16, colscale=cs)  # warmup
-        W, H = Wi.clone(), Hi.clone()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        rel = ctx.nmf_run(A, W, H, algo, iters, dtype=dt, A16=A16, colscale=cs)
-        torch.cuda.synchronize()
-        el = time.perf_counter() - t0
-        out[name] = {"iters_per_s": iters / el, "final_relerr": float(rel[-1])}
-        del A16, cs
-    out["speedup_bf16_over_fp32"] = out["bf16"]["iters_per_s"] / out["fp32"]["iters_per_s"]
-    out["relerr_gap"] = abs(out["bf16"]["final_relerr"] - out["fp32"]["final_relerr"])
-    out["config"] = f"A {m}x{n} = W0 H0 (rank {k}) + 1% noise, k={k}, {iters} iterations"
-    del A
-    torch.cuda.empty_cache()
-    return out
+def cpu_baseline_leg(O, A, Wi, Hi, iters):
+    """FP64 oracle MU (and the FP32 variant) on the host, full shape, `iters`
+    timed iterations after the host BLAS has been warmed up."""
+    info = host_info()
+    nthreads = int(os.environ.get("OMP_NUM_THREADS") or info.get("affinity_cpus") or 1)
+    O.set_threads(nthreads)
+    blas, has_s = O.use_blas(nthreads)
+    t0 = time.perf_counter()
+    A32 = A.cpu().numpy()
+    W0 = Wi.cpu().numpy().astype(np.float64)
+    H0 = Hi.cpu().numpy().astype(np.float64)
+    A64 = np.asfortranarray(A32, dtype=np.float64)
+    copy_s = time.perf_counter() - t0
+    small = np.asfortranarray(np.ones((256, 64)))
+    O.gemm("T", "N", small, small)  # brings up the BLAS thread pool
+    log(f"bench: CPU oracle MU fp64, {iters} iterations on {nthreads} threads ({blas})")
+    _, _, rel64, _, rc = O.nmf(0, A64, W0, H0, iters)
+    s64 = O.last_loop_seconds()
+    del A64
+    f32 = None
+    if has_s:
+        log(f"bench: CPU MU fp32 variant, {iters} iterations")
+        _, _, rel32, rc32 = O.mu_f32(A32, W0.astype(np.float32), H0.astype(np.float32), iters)
+        if rc32 == 0:
+            s32 = O.last_loop_seconds()
+            f32 = {"value": round(iters / s32, 4), "seconds": round(s32, 3),
+                   "final_relerr": float(rel32[-1])}
+    O.no_blas()
+    return {"value": round(iters / s64, 4), "unit": "outer iters/sec", "cores": nthreads,
+            "kind": "port",
+            "sample": f"{iters} outer iterations of NMF-MU on the full {M_C3}x{N_C3} A, k={K_C3}, "
+                      f"same A and init, FP64 oracle restatement of MUNMF::computeNMF "
+                      f"(error by computeObjectiveError every iteration), GEMMs through "
+                      f"{blas or 'the oracle blocked GEMM'}; loop {s64:.2f} s "
+                      f"(steady_clock around the loop; host copy {copy_s:.1f} s outside)",
+            "rc": rc, "final_relerr_fp64": float(rel64[-1]),
+            "fp32_variant": f32, "host": info}
 
 
 def hier_leg(P, D, ctx, m=100000, n=500000, density=1e-4, depth=11, iters=100):
     """Batched hierarchical NMF on a power-law term-document CSR (host CSR in,
     one level-batched launch sequence per tree level)."""
     t0 = time.perf_counter()
-    indptr, indices, data = D.sparse_powerlaw(m, n, density, 1.0, 7)
+    indptr, indices, data = D.sparse_powerlaw(m, n, density, 1.1, 7)
     gen_s = time.perf_counter() - t0
     t0 = time.perf_counter()
     nodes, cols, _, _ = ctx.hiernmf_run(m, n, indptr, indices, data, depth, iters, 193957)
@@ -142,18 +248,19 @@ def hier_leg(P, D, ctx, m=100000, n=500000, density=1e-4, depth=11, iters=100):
             "nnz": int(indptr[-1]), "iters_per_node": iters,
             "node_cols_min": int(min(sizes)), "node_cols_max": int(max(sizes)),
             "generate_s": round(gen_s, 2),
-            "config": f"{m}x{n} CSR, density {density}, log-normal values, power-law rows, "
-                      f"depth {depth}, {iters} iterations per node (host CSR in, tree out)"}
+            "config": f"{m}x{n} CSR, density {density}, log-normal values, power-law rows "
+                      f"(alpha 1.1), depth {depth}, {iters} iterations per node (host CSR in, "
+                      f"tree out)"}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-legs", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-iters", type=int, default=12)
+    ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--profile-kernel", action="store_true",
                     help="only launch the dominant kernel (and a calibration copy) for rocprof")
     args = ap.parse_args()
@@ -169,150 +276,154 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     ctx = P.Context(0)
-    dims, k = (256, 256, 256), 32
-    log("bench: generating 256^3 CP rank-32 tensor on the device")
-    X, init = make_ntf_problem(P, D, ctx, torch, dims, k, 32, stream=2)
-    Fd = [P.colmajor(f) for f in init]
-    L, R = dims[0] * dims[1], dims[2]
-    Pbuf = P.empty_colmajor((L, k))
-    Qbuf = P.empty_colmajor((R, k))
+    m, n, k = M_C3, N_C3, K_C3
+    log(f"bench: generating A {m}x{n} (rank {R_C3} + 1% noise) on the device")
+    A, Wi, Hi = make_nmf_problem(P, D, ctx, torch, m, n, R_C3, k, STREAM_C3)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    A16, cs = ctx.convert(A, P.BF16, scaled=True)
+    A16T = ctx.transpose16(A16)
+    torch.cuda.synchronize()
+    convert_ms = 1000 * (time.perf_counter() - t0)
+    normA2 = norm2_fp64(torch, A)
+    Wt = Wi.clone()
 
-    def right():
-        ctx.mttkrp_partial(X, Fd, 1, 0, out=Pbuf)
-
-    def left():
-        ctx.mttkrp_partial(X, Fd, 1, 1, out=Qbuf)
+    def wta():
+        ctx.gemm_tn(A16, Wt, dtype=P.BF16, colscale=cs)
 
     if args.profile_kernel:
-        cal = torch.empty(64 << 20, dtype=torch.float32, device="cuda")  # 256 MiB
+        cal = torch.empty(256 << 20, dtype=torch.float32, device="cuda")  # 1 GiB
         cal2 = torch.empty_like(cal)
         for _ in range(10):
-            right()
+            wta()
         for _ in range(10):
             cal2.copy_(cal)
         torch.cuda.synchronize()
-        print(json.dumps({"profile_kernel": "mttkrp_partial right 256^3 k=32 x10; copy 256MiB x10"}))
+        print(json.dumps({"profile_kernel": f"BF16 W^T A {m}x{n} k={k} x10; copy 1 GiB x10"}))
+        del cal, cal2
         ctx.close()
         return
 
     # ---- the measured workload ------------------------------------------
-    log(f"bench: NTF-HALS fp32 warmup {args.warmup}, timed {args.steps}")
-    rate, el, rel = ntf_rate(P, ctx, torch, X, init, args.warmup, args.steps, P.F32)
+    log(f"bench: NMF-MU bf16 warmup {args.warmup}, timed {args.steps}")
+    rate, el, rel = nmf_timed(P, ctx, torch, A, Wi, Hi, P.MU, args.warmup, args.steps, P.BF16, A16, cs,
+                              A16T, normA2)
     ms_per_step = 1000.0 * el / args.steps
+    log(f"bench: NMF-MU fp32 warmup {args.warmup}, timed {args.steps}")
+    rate32, el32, rel32 = nmf_timed(P, ctx, torch, A, Wi, Hi, P.MU, args.warmup, args.steps, P.F32,
+                                    normA2=normA2)
+    ref = fp64_reference("mu", args.steps)
+    comparison = {
+        "fp32_iters_per_s": round(rate32, 3), "fp32_ms_per_iter": round(1000 * el32 / args.steps, 4),
+        "speedup_bf16_over_fp32": round(rate / rate32, 4),
+        "final_relerr_bf16": float(rel[-1]), "final_relerr_fp32": float(rel32[-1]),
+        "bf16_conversion_ms_once": round(convert_ms, 3),
+        "note": "A's BF16 copy with column scales, its transpose (the reference's At, formed "
+                "once in the MUNMF constructor) and ||A||^2 (computed once in the NMF "
+                "constructor) are made before the timed region, as in the reference",
+    }
+    if ref is not None:
+        comparison.update({
+            "final_relerr_fp64_oracle": ref[0],
+            "bf16_minus_fp64": float(rel[-1]) - ref[0],
+            "fp32_minus_fp64": float(rel32[-1]) - ref[0],
+            "max_abs_traj_diff_bf16_vs_fp64": float(np.abs(rel - np.array(ref[1])).max()),
+            "equal_error_within_1e-3": bool(abs(rel[-1] - ref[0]) < 1e-3 and abs(rel32[-1] - ref[0]) < 1e-3),
+            "fp64_source": "tests/golden/nmf_c3a_fp64.json (FP64 oracle, same input and init)",
+        })
 
     # ---- dominant kernel roofline ----------------------------------------
-    numel = int(np.prod(dims))
-    t_right = event_time_ms(torch, right, 50)
-    bytes_right = 4 * numel + 4 * L * k + 4 * R * k
-    t_left = event_time_ms(torch, left, 50)
-    bytes_left = 4 * numel + 4 * R * k + 4 * (dims[0] + dims[1]) * k
-    achieved = bytes_right / (t_right * 1e-3) / 1e9
+    t_wta = event_time_ms(torch, wta, 20)
+    bytes_wta = 2 * m * n + 4 * m * k + 4 * n + 4 * k * n
+    achieved = bytes_wta / (t_wta * 1e-3) / 1e9
     traffic = None
-    tf = os.path.join(ROOT, "profiles", "mttkrp_right_256cube_k32_traffic.json")
-    if os.path.exists(tf):
+    if os.path.exists(TRAFFIC_FILE):
         try:
-            traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
+            traffic = json.load(open(TRAFFIC_FILE)).get("hbm_bytes_per_launch")
         except Exception:
             traffic = None
-    # Achievable read bandwidth of this device (16-byte grid-stride reads of
-    # 1 GiB, beyond the Infinity Cache): the ceiling besides the nameplate.
     probe = torch.zeros(1 << 28, dtype=torch.float32, device="cuda")
     read_ceiling = ctx.read_bandwidth(probe, 20)
     del probe
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "nn_f32_kernel<NT=2>, or nn_f32_sh_kernel<NT=2> where the per-shape tuner picks the block-shared variant (right partial MTTKRP, X_(65536x256)*A_2; FP32 MFMA, operands streamed to registers)",
-                "algorithmic_bytes": bytes_right, "avg_ms": round(t_right, 5),
+                "kernel": "BF16 W^T A (thin_gemm TN: tn_h_sh_kernel / tn_h_kernel as tuned; "
+                          "v_mfma_f32_16x16x32_bf16, A streamed from HBM into MFMA operands)",
+                "algorithmic_bytes": bytes_wta,
+                "algorithmic_bytes_formula": "2*m*n (A in BF16) + 4*m*k (W) + 4*n (column "
+                                             "scales) + 4*k*n (output)",
+                "avg_ms": round(t_wta, 5),
                 "achievable_read_GBs": round(read_ceiling, 1),
-                "frac_of_achievable": round(achieved / read_ceiling, 4),
-                "note": "the 67 MB tensor fits the 256 MB Infinity Cache; see legs.mttkrp_128^4 "
-                        "for a tensor that does not; achievable_read_GBs is this device's plain "
-                        "16-byte read rate over 1 GiB (planc_read_bandwidth)"}
-    left_info = {"avg_ms": round(t_left, 5),
-                 "achieved_GBs": round(bytes_left / (t_left * 1e-3) / 1e9, 1),
-                 "kernels": "tn_f32_kernel (KRP of A_0, A_1 fused) split-K + splitk_reduce_kernel"}
+                "frac_of_achievable": round(achieved / read_ceiling, 4)}
 
-    # ---- parity and CPU baseline over a bounded sample -------------------
+    # ---- CPU baseline ------------------------------------------------------
     cpu = None
-    parity = None
     if not args.no_cpu:
         from oracle import oracle as O
-        nthreads = min(16, os.cpu_count() or 1)
-        O.set_threads(nthreads)
-        n = args.cpu_iters
-        Xh = X.cpu().numpy().astype(np.float64)
-        log(f"bench: oracle NTF-HALS fp64, {n} iterations on {nthreads} threads")
-        t0 = time.perf_counter()
-        _, _, rel_cpu, rc = O.ntf(1, Xh, [f.astype(np.float64) for f in init], n, dimtree=1)
-        el_cpu = time.perf_counter() - t0
-        del Xh
-        cpu = {"value": round(n / el_cpu, 4), "unit": "outer iters/sec", "cores": nthreads,
-               "kind": "port",
-               "sample": f"{n} outer iterations of NTF-HALS 256^3 k=32, FP64 oracle restatement "
-                         f"(dimension tree, error every iteration), same input and init; "
-                         f"{el_cpu:.1f} s"}
-        _, rel_gpu = run_ntf(P, ctx, torch, X, init, n, P.F32)
-        parity = {"iters": n, "max_abs_relerr_diff_vs_fp64": float(np.abs(rel_gpu - rel_cpu).max()),
-                  "final_relerr_gpu_fp32": float(rel_gpu[-1]),
-                  "final_relerr_oracle_fp64": float(rel_cpu[-1])}
+        cpu = cpu_baseline_leg(O, A, Wi, Hi, args.cpu_iters)
 
-    legs = {"left_partial_mttkrp": left_info}
+    legs = {}
     if not args.no_legs:
-        total = args.warmup + args.steps
-        log("bench: NTF-HALS bf16/fp16 legs")
-        ntf_leg = {"fp32": {"iters_per_s": rate, "final_relerr": float(rel[-1])}}
-        for name, dt in (("bf16", P.BF16), ("fp16", P.F16)):
-            X16, _ = ctx.convert(flat(X), dt, scaled=False)
-            r16, _, rel16 = ntf_rate(P, ctx, torch, X, init, args.warmup, args.steps, dt, X16=X16)
-            ntf_leg[name] = {"iters_per_s": r16, "final_relerr": float(rel16[-1]),
-                             "relerr_gap_vs_fp32": abs(float(rel16[-1]) - float(rel[-1]))}
-            del X16
-        ntf_leg["iterations"] = total
-        legs["ntf_hals_256^3_k32"] = ntf_leg
+        log("bench: NMF-HALS fp32 vs bf16")
+        legs["nmf_hals_65536x16384_k64"] = compare_precisions(
+            P, ctx, torch, A, Wi, Hi, A16, cs, A16T, normA2, P.HALS, "hals", args.warmup, args.steps)
+        log("bench: NMF-ANLS-BPP fp32 vs bf16")
+        legs["nmf_anlsbpp_65536x16384_k64"] = compare_precisions(
+            P, ctx, torch, A, Wi, Hi, A16, cs, A16T, normA2, P.ANLSBPP, "anlsbpp", 1, args.steps)
+        del A, A16, A16T, cs, Wi, Hi, Wt
+        torch.cuda.empty_cache()
 
-        log("bench: MTTKRP 128^4 k=48 (HBM-resident)")
-        del Pbuf, Qbuf
+        log("bench: NTF-HALS 128^4 k=48 fp32 vs fp16")
         dims4, k4 = (128, 128, 128, 128), 48
         X4, init4 = make_ntf_problem(P, D, ctx, torch, dims4, k4, 48, stream=5)
+        nt = {}
+        r32, el32, rel32n = ntf_rate(P, ctx, torch, X4, init4, 3, 20, P.F32)
+        nt["fp32"] = {"iters_per_s": round(r32, 3), "final_relerr": float(rel32n[-1])}
+        X416, _ = ctx.convert(flat(X4), P.F16, scaled=False)
+        r16, el16, rel16 = ntf_rate(P, ctx, torch, X4, init4, 3, 20, P.F16, X16=X416)
+        nt["fp16"] = {"iters_per_s": round(r16, 3), "final_relerr": float(rel16[-1]),
+                      "relerr_gap_vs_fp32": abs(float(rel16[-1]) - float(rel32n[-1]))}
+        nt["speedup_fp16_over_fp32"] = round(r16 / r32, 4)
         F4 = [P.colmajor(f) for f in init4]
         L4 = dims4[0] * dims4[1]
         P4 = P.empty_colmajor((L4, k4))
         numel4 = int(np.prod(dims4))
-        mt = {}
-        for name, dt, Xop in (("fp32", P.F32, X4), ("fp16", P.F16, None)):
-            if dt != P.F32:
-                Xop, _ = ctx.convert(flat(X4), dt, scaled=False)
+        for name, dt, Xop in (("fp32", P.F32, X4), ("fp16", P.F16, X416)):
             esz = 4 if dt == P.F32 else 2
 
             def r4(Xop=Xop, dt=dt):
                 ctx.mttkrp_partial(Xop, F4, 1, 0, dtype=dt, out=P4)
             t4 = event_time_ms(torch, r4, 10)
             b4 = esz * numel4 + 4 * L4 * k4
-            mt[name] = {"avg_ms": round(t4, 4), "achieved_GBs": round(b4 / (t4 * 1e-3) / 1e9, 1),
-                        "frac_of_8TBs": round(b4 / (t4 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                        "frac_of_achievable_read": round(b4 / (t4 * 1e-3) / 1e9 / read_ceiling, 4)}
-        legs["mttkrp_128^4_k48_right_partial"] = mt
-        del P4, F4
-        # NTF-HALS on the 128^4 tensor (beyond the Infinity Cache), rank 48:
-        # FP32 against FP16 MFMA in the MTTKRP, 10 timed iterations each after
-        # 3 warm-up ones, final relative errors side by side.
-        log("bench: NTF-HALS 128^4 k=48 fp32 vs fp16")
-        nt = {}
-        r32, el32, rel32 = ntf_rate(P, ctx, torch, X4, init4, 3, 10, P.F32)
-        nt["fp32"] = {"iters_per_s": r32, "final_relerr": float(rel32[-1])}
-        X416, _ = ctx.convert(flat(X4), P.F16, scaled=False)
-        r16, el16, rel16 = ntf_rate(P, ctx, torch, X4, init4, 3, 10, P.F16, X16=X416)
-        nt["fp16"] = {"iters_per_s": r16, "final_relerr": float(rel16[-1]),
-                      "relerr_gap_vs_fp32": abs(float(rel16[-1]) - float(rel32[-1]))}
-        nt["speedup_fp16_over_fp32"] = r16 / r32
-        nt["config"] = "128^4 CP rank-48 model + 1% noise, k=48, dimension tree, 10 timed iterations"
+            nt[f"right_partial_mttkrp_{name}"] = {
+                "avg_ms": round(t4, 4), "achieved_GBs": round(b4 / (t4 * 1e-3) / 1e9, 1),
+                "frac_of_8TBs": round(b4 / (t4 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        nt["config"] = "128^4 CP rank-48 model + 1% noise, k=48, dimension tree, 20 timed iterations"
         legs["ntf_hals_128^4_k48"] = nt
-        del X4, X416
+        del X4, X416, F4, P4
         torch.cuda.empty_cache()
 
-        log("bench: NMF 65536x16384 k=64 fp32 vs bf16")
-        legs["nmf_mu_65536x16384_k64"] = nmf_leg(P, D, ctx, torch, P.MU, 65536, 16384, 64, 20)
-        legs["nmf_hals_65536x16384_k64"] = nmf_leg(P, D, ctx, torch, P.HALS, 65536, 16384, 64, 20)
+        log("bench: NTF-HALS 256^3 k=32")
+        dims, k3 = (256, 256, 256), 32
+        X, init = make_ntf_problem(P, D, ctx, torch, dims, k3, 32, stream=2)
+        r3, el3, rel3 = ntf_rate(P, ctx, torch, X, init, 3, 30, P.F32)
+        c2 = {"fp32": {"iters_per_s": round(r3, 3), "final_relerr": float(rel3[-1])}}
+        for name, dt in (("bf16", P.BF16), ("fp16", P.F16)):
+            X16, _ = ctx.convert(flat(X), dt, scaled=False)
+            r, _, rr = ntf_rate(P, ctx, torch, X, init, 3, 30, dt, X16=X16)
+            c2[name] = {"iters_per_s": round(r, 3), "final_relerr": float(rr[-1])}
+            del X16
+        if not args.no_cpu:
+            from oracle import oracle as O
+            O.set_threads(int(os.environ.get("OMP_NUM_THREADS") or 16))
+            Xh = X.cpu().numpy().astype(np.float64)
+            _, _, rel_cpu, _ = O.ntf(1, Xh, [f.astype(np.float64) for f in init], 12, dimtree=1)
+            del Xh
+            c2["fp32_vs_fp64_oracle_12_iters_max_abs_diff"] = float(np.abs(rel3[:12] - rel_cpu).max())
+        c2["config"] = "256^3 CP rank-32 model + 1% noise, k=32, dimension tree, 30 timed iterations"
+        legs["ntf_hals_256^3_k32"] = c2
+        del X
+        torch.cuda.empty_cache()
 
         log("bench: hierarchical rank-2 NMF, 100000x500000 CSR, depth 11")
         legs["hiernmf_100000x500000_depth11"] = hier_leg(P, D, ctx)
@@ -328,14 +439,16 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32",
-        "data": "synthetic (Philox4x64 CP rank-32 model + 1% noise, generated on the device)",
-        "config": {"workload": "NTF-HALS 256x256x256 FP32, k=32, dimension-tree MTTKRP, "
-                               "relative error every iteration",
-                   "dims": list(dims), "k": k, "algorithm": "HALS", "dimtree_split": 1},
+        "dtype": "bf16",
+        "data": "synthetic (Philox4x64: A = W0 H0 with rank-64 U[0,1) factors + 1% |N(0,1)| "
+                "noise, generated on the device; seeded U[0,1) init)",
+        "config": {"workload": f"NMF-MU {m}x{n} FP32 A, k={k}, BF16-MFMA products (FP32 "
+                               f"accumulation, per-column scales), FP64 Grams and relative "
+                               f"error every iteration, {args.steps} iterations",
+                   "m": m, "n": n, "k": k, "algorithm": "MU", "iterations": args.steps},
+        "bf16_vs_fp32": comparison,
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "parity": parity,
         "legs": legs,
     }
     print(json.dumps(line), flush=True)

@@ -51,6 +51,9 @@ extern "C" {
 #define PLANC_MU 0
 #define PLANC_HALS 1
 #define PLANC_ANLSBPP 2
+#define PLANC_AOADMM 4
+#define PLANC_NESTEROV 5
+#define PLANC_GNSYM 7
 #define PLANC_R2 8
 
 typedef struct planc_ctx planc_ctx;
@@ -77,6 +80,12 @@ int planc_memcpy_d2h(planc_ctx *ctx, void *h_dst, const void *d_src, size_t byte
  * counterpart (the reference computes in FP64 only). */
 int planc_convert(planc_ctx *ctx, int dtype, const float *d_src, long rows, long cols, long ld_src,
                   void *d_dst, long ld_dst, float *d_colscale);
+
+/* d_out (cols x rows, ld ldo) = d_in^T for a 16-bit matrix d_in (rows x
+ * cols, ld ldi): the transposed copy A16T of planc_nmf_desc, the analogue of
+ * the reference's At = A.t() (nmf/mu.hpp:42,46). */
+int planc_transpose16(planc_ctx *ctx, const void *d_in, long rows, long cols, long ldi, void *d_out,
+                      long ldo);
 
 /* Synthetic inputs on the device (Philox4x64-10; element e of stream
  * (seed, stream) equals numpy.random.Philox(key=[seed, 0],
@@ -278,7 +287,7 @@ int planc_hiernmf_run(planc_ctx *ctx, const planc_hier_desc *desc, const long *h
 
 /* ---- whole loops (computeNMF / computeNTF on the device) ----------------- */
 typedef struct planc_nmf_desc {
-  int algo;          /* PLANC_MU, PLANC_HALS, PLANC_ANLSBPP */
+  int algo;          /* PLANC_MU, PLANC_HALS, PLANC_ANLSBPP, PLANC_AOADMM, PLANC_GNSYM */
   int dtype;         /* PLANC_F32 / PLANC_BF16 / PLANC_F16 */
   long m, n;         /* A is m x n */
   int k;             /* rank */
@@ -287,12 +296,31 @@ typedef struct planc_nmf_desc {
   double regH[2];
   int compute_error; /* 1: relative error every iteration (always on in the
                         reference's shared-memory driver, nmf/nmf.cpp:205) */
+  const void *A16T;  /* optional, 16-bit forms only: the transposed copy
+                        (n x m, ld n) of the 16-bit A, made once by
+                        planc_transpose16 -- the reference likewise forms
+                        At = A.t() once in the MUNMF constructor
+                        (nmf/mu.hpp:42,46).  Null: made in scratch on every
+                        call. */
+  double normA2;     /* > 0: ||A||_F^2 supplied by the caller, as the reference
+                        computes normA once in the NMF constructor
+                        (common/nmf.hpp:167); <= 0: computed from d_A on
+                        every call. */
+  double symm_reg;   /* > 0: symmetric regularisation of ANLS-BPP
+                        (applySymmetricReg, common/nmf.hpp:123-129, applied in
+                        nmf/bppnmf.hpp:40-44): Gram + symm_reg*I and right-hand
+                        side + symm_reg*given^T; needs m == n.  <= 0: off */
 } planc_nmf_desc;
 
-/* Runs desc->iters iterations of MUNMF / HALSNMF / BPPNMF computeNMF
- * (nmf/mu.hpp:48-91, nmf/hals.hpp:50-116, nmf/bppnmf.hpp:187-241) from the
- * initial W (m x k) and H (n x k), overwriting them, including the
- * constructor/final normalize_by_W of each algorithm.  d_A is FP32 m x n
+/* Runs desc->iters iterations of MUNMF / HALSNMF / BPPNMF / AOADMMNMF /
+ * GNSYMNMF computeNMF (nmf/mu.hpp:48-91, nmf/hals.hpp:50-116,
+ * nmf/bppnmf.hpp:187-241, nmf/aoadmm.hpp:88-192, nmf/gnsym.hpp:178-260) from
+ * the initial W (m x k) and H (n x k), overwriting them, including the
+ * constructor/final normalize_by_W of each algorithm.  AO-ADMM keeps its
+ * duals (zero at the start of each call) in context scratch and returns
+ * PLANC_ENOCONV when a shifted Gram is not positive definite (the
+ * reference's arma::chol fails there).  GNSYM factors a square A as H H^T:
+ * W is not used (the reference clears it) and is left unchanged.  d_A is FP32 m x n
  * (lda); when dtype is 16-bit, d_A16 (optional) is a planc_convert copy with
  * d_colscale, else one is made in scratch.  h_relerr (host, iters, optional)
  * receives the relative error of printObjective (common/nmf.hpp:461-468). */
@@ -300,8 +328,47 @@ int planc_nmf_run(planc_ctx *ctx, const planc_nmf_desc *desc, const float *d_A, 
                   const void *d_A16, const float *d_colscale, float *d_W, float *d_H,
                   double *h_relerr);
 
+/* ---- sparse input (the reference's BUILD_SPARSE build, SP_MAT A) ---------- */
+/* A sparse m x n matrix is passed as CSR: d_indptr (m + 1, int64), d_indices
+ * (int32 column ids, sorted within a row) and d_values (FP32), all device
+ * memory.  At most 2^31 - 1 rows, columns and entries. */
+
+/* CSR of A^T (n x m) from the CSR of A: entries grouped by column of A, rows
+ * increasing within a column, values unchanged (bit-exact; the reference
+ * forms At = A.t() once in the MUNMF / HALSNMF / BPPNMF constructors,
+ * nmf/mu.hpp:42,46).  Outputs hold n + 1 and nnz entries.  Synchronises. */
+int planc_csr_transpose(planc_ctx *ctx, long m, long n, const long *d_indptr, const int *d_indices,
+                        const float *d_values, long *d_out_indptr, int *d_out_indices, float *d_out_values);
+
+/* d_out (rows x k, column-major, ld ldo) = A (rows x cols, CSR) * F (cols x k,
+ * column-major, ld ldf): the sparse A * H of nmf/mu.hpp:71 / nmf/hals.hpp:81
+ * (and, with the CSR of A^T, W^T A transposed).  k <= 64; entries summed in
+ * CSR order in FP32. */
+int planc_spmm(planc_ctx *ctx, long rows, long cols, const long *d_indptr, const int *d_indices,
+               const float *d_values, const float *d_F, long ldf, int k, float *d_out, long ldo);
+
+/* computeObjectiveError(A, W, H) for a sparse A (common/utils.hpp:190-241):
+ * sqrt(sum over non-zeros of (a_ij - w_i.h_j)^2 + ||W H^T||_F^2 - sum over
+ * non-zeros of (w_i.h_j)^2), FP64, into *h_err.  W m x k, H n x k.
+ * Synchronises. */
+int planc_sparse_fit_error(planc_ctx *ctx, long m, long n, const long *d_indptr, const int *d_indices,
+                           const float *d_values, const float *d_W, long ldw, const float *d_H, long ldh,
+                           int k, double *h_err);
+
+/* planc_nmf_run for a sparse A (nmf/nmf.cpp:48-130 with BUILD_SPARSE): the
+ * same MU / HALS / ANLS-BPP loop with the products W^T A and A H taken as
+ * sparse products (FP32 only: desc->dtype must be PLANC_F32) and ||A||^2 over
+ * the stored values.  ANLS-BPP starts, as the sparse build does, by
+ * normalize_by_W: the reference builds a HALSNMF from W and H "to get proper
+ * initializations" and takes its factors without running it
+ * (nmf/bppnmf.hpp:195-201), so only HALSNMF's constructor normalisation
+ * (nmf/hals.hpp:46) takes effect.  desc->A16T is ignored. */
+int planc_nmf_sparse_run(planc_ctx *ctx, const planc_nmf_desc *desc, const long *d_indptr,
+                         const int *d_indices, const float *d_values, float *d_W, float *d_H,
+                         double *h_relerr);
+
 typedef struct planc_ntf_desc {
-  int algo;          /* PLANC_MU, PLANC_HALS, PLANC_ANLSBPP */
+  int algo;          /* PLANC_MU, PLANC_HALS, PLANC_ANLSBPP, PLANC_AOADMM */
   int dtype;         /* of the MTTKRP reads of X */
   int nmodes;        /* 2..7 */
   long dims[8];      /* dims[0..nmodes-1] used */
@@ -312,7 +379,8 @@ typedef struct planc_ntf_desc {
 } planc_ntf_desc;
 
 /* Runs AUNTF::computeNTF (ntf/auntf.hpp:128-177) with NTFHALS / NTFMU /
- * NTFANLSBPP updates.  d_factors (host array of device pointers, I_i x k)
+ * NTFANLSBPP / NTFAOADMM (ntf/ntfaoadmm.hpp:23-77, duals zero at the start
+ * of each call) updates.  d_factors (host array of device pointers, I_i x k)
  * hold the initial factors, which are first normalised as NCPFactors::
  * normalize() does in the AUNTF constructor (ntf/auntf.hpp:88); on return
  * they hold the unit-norm factors and d_lambda (k, FP64) the weights.

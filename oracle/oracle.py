@@ -38,6 +38,21 @@ How each oracle function is pinned (tests/test_oracle.py):
                          numpy reconstruction; dimtree == direct.
 * cp_loss             -- central differences in FP64 (h = 1e-6) and the einsum
                          reconstruction.
+* admm_solve / AO-ADMM NMF and NTF
+                      -- repeated solves (the dual carried over) converge to the
+                         NNLS solution (scipy.optimize.nnls, unique for an SPD
+                         Gram); one AO-ADMM NMF iteration against a NumPy
+                         restatement; NTF AO-ADMM dimtree == direct and its
+                         error against a direct reconstruction.
+* GNSym               -- the exact factorisation A = H H^T is a fixed point
+                         (zero gradient, no step, zero error); the fast error
+                         formula against ||A - H H^T||_F; one iteration against a
+                         NumPy restatement; decreasing error on a well-posed case.
+* nmf_sparse          -- the dense loop on the densified matrix (identical
+                         trajectories for MU / HALS), and for ANLS-BPP the dense
+                         loop started from normalize_by_W'd factors (the sparse
+                         build's start, nmf/bppnmf.hpp:195-201).
+* sparse_fit_error    -- the direct dense ||A - W H^T||_F on small matrices.
 * r2_update           -- brute force over the four active sets of the
                          two-variable NNLS.
 * r2_nmf_csr          -- dense NumPy loop of the same DistAUNMF order on a tiny
@@ -95,6 +110,23 @@ def lib():
         L.oracle_powiter_csr.argtypes = [ctypes.c_long, _lp, _ip, _dp, _lp, ctypes.c_long,
                                          ctypes.c_long, _dp, ctypes.c_int]
         L.oracle_powiter_csr.restype = ctypes.c_double
+        _lp = ctypes.POINTER(ctypes.c_long)
+        _ip = ctypes.POINTER(ctypes.c_int)
+        L.oracle_nmf_sparse.argtypes = [ctypes.c_int, ctypes.c_long, ctypes.c_long, _lp, _ip, _dp, _dp,
+                                        _dp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]
+        L.oracle_nmf_sparse.restype = ctypes.c_int
+        L.oracle_sparse_fit_error.argtypes = [ctypes.c_long, ctypes.c_long, _lp, _ip, _dp, _dp, _dp,
+                                              ctypes.c_int]
+        L.oracle_sparse_fit_error.restype = ctypes.c_double
+        L.oracle_admm_solve.argtypes = [_dp, ctypes.c_int, _dp, ctypes.c_long, _dp, _dp]
+        L.oracle_admm_solve.restype = ctypes.c_int
+        L.oracle_use_blas.argtypes = [ctypes.c_char_p, ctypes.c_int]
+        L.oracle_use_blas.restype = ctypes.c_int
+        L.oracle_last_loop_seconds.restype = ctypes.c_double
+        _fp = ctypes.POINTER(ctypes.c_float)
+        L.oracle_mu_f32.argtypes = [_fp, ctypes.c_long, ctypes.c_long, _fp, _fp, ctypes.c_int,
+                                    ctypes.c_int, _dp]
+        L.oracle_mu_f32.restype = ctypes.c_int
         L.oracle_set_threads.argtypes = [ctypes.c_int]
         L.oracle_get_threads.restype = ctypes.c_int
         _LIB = L
@@ -115,6 +147,49 @@ def set_threads(n):
 
 def get_threads():
     return lib().oracle_get_threads()
+
+
+# Host BLAS libraries tried, in order, for the CPU baseline (bench.py only).
+BLAS_CANDIDATES = ("/opt/conda/lib/libmkl_rt.so", "libmkl_rt.so", "libmkl_rt.so.2",
+                   "libopenblas.so.0", "libopenblas.so")
+
+
+def use_blas(nthreads, candidates=BLAS_CANDIDATES):
+    """Routes the oracle's GEMMs through a host CBLAS (MKL's GNU-OpenMP layer
+    when MKL is found).  Returns (path, has_sgemm) or (None, False) when no
+    library loads, in which case the oracle's blocked GEMM stays in use."""
+    import os
+    os.environ.setdefault("MKL_THREADING_LAYER", "GNU")
+    os.environ.setdefault("MKL_INTERFACE_LAYER", "LP64")
+    for path in candidates:
+        rc = lib().oracle_use_blas(path.encode(), int(nthreads))
+        if rc >= 0:
+            return path, rc == 1
+    return None, False
+
+
+def no_blas():
+    lib().oracle_use_blas(None, 0)
+
+
+def last_loop_seconds():
+    """Seconds of the iteration loop of the last nmf() / mu_f32() call."""
+    return lib().oracle_last_loop_seconds()
+
+
+def mu_f32(A, W0, H0, iters):
+    """FP32 MU loop through the loaded cblas_sgemm (CPU baseline only).
+    Returns (W, H, relerr, rc); rc -3 when no sgemm is loaded."""
+    A = np.require(A, dtype=np.float32, requirements=["F", "A"])
+    W = np.require(W0, dtype=np.float32, requirements=["F", "A", "W", "O"]).copy(order="F")
+    H = np.require(H0, dtype=np.float32, requirements=["F", "A", "W", "O"]).copy(order="F")
+    m, n = A.shape
+    k = W.shape[1]
+    rel = np.zeros(iters)
+    fp = ctypes.POINTER(ctypes.c_float)
+    rc = lib().oracle_mu_f32(A.ctypes.data_as(fp), m, n, W.ctypes.data_as(fp),
+                             H.ctypes.data_as(fp), k, iters, _p(rel))
+    return W, H, rel, rc
 
 
 def gemm(ta, tb, A, B):
@@ -155,6 +230,49 @@ def nmf(algo, A, W0, H0, iters, regW=(0.0, 0.0), regH=(0.0, 0.0), symm_reg=-1.0)
     rc = lib().oracle_nmf(algo, _p(A), m, n, _p(W), _p(H), k, iters, _p(rW), _p(rH),
                           float(symm_reg), _p(rel), _p(obj))
     return W, H, rel, obj, rc
+
+
+def admm_solve(G, R, X, D):
+    """One AO-ADMM solve (at most 5 sweeps): returns the new (X, D, rc)."""
+    G = _f64(G)
+    R = _f64(R)
+    X = _f64(X).copy(order="F")
+    D = _f64(D).copy(order="F")
+    rc = lib().oracle_admm_solve(_p(G), G.shape[0], _p(R), R.shape[1], _p(X), _p(D))
+    return X, D, rc
+
+
+def _csr_arrays(indptr, indices, values):
+    ip = np.ascontiguousarray(indptr, dtype=np.int64)
+    ix = np.ascontiguousarray(indices, dtype=np.int32)
+    vv = np.ascontiguousarray(values, dtype=np.float64)
+    return ip, ix, vv
+
+
+def nmf_sparse(algo, m, n, indptr, indices, values, W0, H0, iters, regW=(0.0, 0.0),
+               regH=(0.0, 0.0)):
+    """Sparse build of the NMF loop, A as CSR.  Returns (W, H, relerr, rc)."""
+    ip, ix, vv = _csr_arrays(indptr, indices, values)
+    W = _f64(W0).copy(order="F")
+    H = _f64(H0).copy(order="F")
+    k = W.shape[1]
+    rel = np.zeros(iters)
+    rW = np.asarray(regW, dtype=np.float64)
+    rH = np.asarray(regH, dtype=np.float64)
+    rc = lib().oracle_nmf_sparse(algo, m, n, ip.ctypes.data_as(ctypes.POINTER(ctypes.c_long)),
+                                 ix.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _p(vv), _p(W), _p(H),
+                                 k, iters, _p(rW), _p(rH), _p(rel))
+    return W, H, rel, rc
+
+
+def sparse_fit_error(m, n, col_ptrs, row_indices, values, W, H):
+    """computeObjectiveError(A, W, H) of common/utils.hpp:190-241, A as CSC."""
+    cp, ri, vv = _csr_arrays(col_ptrs, row_indices, values)
+    W = _f64(W)
+    H = _f64(H)
+    return lib().oracle_sparse_fit_error(m, n, cp.ctypes.data_as(ctypes.POINTER(ctypes.c_long)),
+                                         ri.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _p(vv),
+                                         _p(W), _p(H), W.shape[1])
 
 
 def _factor_ptrs(factors):

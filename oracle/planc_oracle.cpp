@@ -11,9 +11,12 @@
 // Parity status of each function (how it is pinned) is listed in the header
 // comment of oracle/oracle.py and in tests/test_oracle.py.
 
+#include <dlfcn.h>
 #include <omp.h>
 
 #include <algorithm>
+#include <chrono>
+#include <climits>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -102,9 +105,29 @@ static void gemm_block(char ta, char tb, long i0, long i1, long j0, long j1,
   }
 }
 
+// Host BLAS resolved at run time for the CPU baseline only (MKL's or
+// OpenBLAS's cblas_dgemm / cblas_sgemm through dlopen; CBLAS column-major =
+// 102, NoTrans = 111, Trans = 112).  Unset by default and in every test, so
+// the oracle's results come from the blocked GEMM below unless a caller asks.
+typedef void (*cblas_dgemm_fn)(int, int, int, int, int, int, double, const double *, int,
+                               const double *, int, double, double *, int);
+typedef void (*cblas_sgemm_fn)(int, int, int, int, int, int, float, const float *, int,
+                               const float *, int, float, float *, int);
+static cblas_dgemm_fn g_dgemm = nullptr;
+static cblas_sgemm_fn g_sgemm = nullptr;
+
+static bool fits_int(long a, long b, long c, long d, long e, long f) {
+  return a < INT_MAX && b < INT_MAX && c < INT_MAX && d < INT_MAX && e < INT_MAX && f < INT_MAX;
+}
+
 void gemm(char ta, char tb, long M, long N, long K, double alpha, const double *A, long lda,
           const double *B, long ldb, double beta, double *C, long ldc) {
   if (M <= 0 || N <= 0) return;
+  if (g_dgemm && K > 0 && fits_int(M, N, K, lda, ldb, ldc)) {
+    g_dgemm(102, ta == 'N' ? 111 : 112, tb == 'N' ? 111 : 112, (int)M, (int)N, (int)K, alpha, A,
+            (int)lda, B, (int)ldb, beta, C, (int)ldc);
+    return;
+  }
   for (long j = 0; j < N; ++j)
     for (long i = 0; i < M; ++i) C[i + j * ldc] = (beta == 0.0) ? 0.0 : beta * C[i + j * ldc];
   if (K <= 0 || alpha == 0.0) return;
@@ -220,6 +243,61 @@ static void solve_sympd(const std::vector<double> &S, int p, std::vector<double>
       b[i] /= LU[i + p * i];
     }
   }
+}
+
+// One ADMM sweep of AOADMMNMF::computeNMF (nmf/aoadmm.hpp:111-141 for H,
+// :157-185 for W) and NTFAOADMM::update (ntf/ntfaoadmm.hpp:34-72), shared:
+// G (k x k, Gram + regularisation, without the rho shift) is shifted by
+// rho = trace(G)/k (0.01 if not positive) and Cholesky-factored once; then
+// at most 5 sweeps, each
+//   T = (G + rho I)^-1 (R + rho (X + D)^T)    (R is k x n, X and D n x k)
+//   X = max(T^T - D, 0);  D = D + X - T^T
+// stopping when ||X - T^T|| < 0.01 ||X|| and ||X - X_prev|| < 0.01 ||D||.
+// Returns false when G + rho I is not positive definite.
+static bool admm_solve(const Mat &G0, const Mat &R, Mat &X, Mat &D) {
+  const long k = G0.r, n = X.r;
+  double tr = 0;
+  for (long c = 0; c < k; ++c) tr += G0(c, c);
+  double rho = tr / k;
+  rho = rho > 0 ? rho : 0.01;
+  std::vector<double> L(G0.d);
+  for (long c = 0; c < k; ++c) L[c + k * c] += rho;
+  if (!chol_inplace(L, (int)k)) return false;
+  const double tol = 0.01;
+  std::vector<double> T((size_t)k * n);
+  Mat Xprev;
+  for (int it = 0; it < 5; ++it) {
+    Xprev = X;
+    for (long j = 0; j < n; ++j) {
+      double *b = &T[(size_t)k * j];
+      for (long c = 0; c < k; ++c) b[c] = R(c, j) + rho * (X(j, c) + D(j, c));
+      for (long i = 0; i < k; ++i) {  // forward: L y = b
+        double v = b[i];
+        for (long t = 0; t < i; ++t) v -= L[i + k * t] * b[t];
+        b[i] = v / L[i + k * i];
+      }
+      for (long i = k - 1; i >= 0; --i) {  // backward: L^T x = y
+        double v = b[i];
+        for (long t = i + 1; t < k; ++t) v -= L[t + k * i] * b[t];
+        b[i] = v / L[i + k * i];
+      }
+    }
+    double r2 = 0, s2 = 0, x2 = 0, d2 = 0;
+    for (long j = 0; j < n; ++j)
+      for (long c = 0; c < k; ++c) {
+        const double t = T[c + (size_t)k * j];
+        double x = t - D(j, c);
+        x = x > 0.0 ? x : 0.0;
+        X(j, c) = x;
+        D(j, c) = D(j, c) + x - t;
+        r2 += (x - t) * (x - t);
+        s2 += (x - Xprev(j, c)) * (x - Xprev(j, c));
+        x2 += x * x;
+        d2 += D(j, c) * D(j, c);
+      }
+    if (std::sqrt(r2) < tol * std::sqrt(x2) && std::sqrt(s2) < tol * std::sqrt(d2)) break;
+  }
+  return true;
 }
 
 // ----------------------------------------------------------------------------
@@ -730,8 +808,20 @@ static double ntf_rel_error(const NTFState &s) {
 // NTF update rules: NTFHALS::update (ntf/ntfhals.hpp:13-29), NTFMU::update
 // (ntf/ntfmu.hpp:11-16), NTFANLSBPP::update (ntf/ntfanlsbpp.hpp:15-57).
 // M is k x I_n, S k x k; the factor is updated in place (I_n x k).
-static int ntf_update(int algo, Mat &H, const Mat &M, const Mat &S, long k) {
+static int ntf_update(int algo, Mat &H, const Mat &M, const Mat &S, long k, Mat *dual = nullptr) {
   long In = H.r;
+  if (algo == 4) {  // AO-ADMM (ntf/ntfaoadmm.hpp:23-77): dual of this mode in *dual
+    if (!dual || !admm_solve(S, M, H, *dual)) return -1;
+    // m_ncp_aux.normalize(mode): the dual's columns to unit norm (:73).
+    for (long c = 0; c < k; ++c) {
+      double nrm = 0;
+      for (long r = 0; r < In; ++r) nrm += (*dual)(r, c) * (*dual)(r, c);
+      nrm = std::sqrt(nrm);
+      if (nrm > 0)
+        for (long r = 0; r < In; ++r) (*dual)(r, c) /= nrm;
+    }
+    return 0;
+  }
   if (algo == 1) {  // HALS
     std::vector<double> u(In);
     for (long i = 0; i < k; ++i) {
@@ -775,6 +865,8 @@ static int ntf_update(int algo, Mat &H, const Mat &M, const Mat &S, long k) {
 static int ntf_loop(NTFState &s, int algo, int iters, double *relerr) {
   int N = s.X.N;
   Mat krp, M;
+  std::vector<Mat> duals;  // AO-ADMM auxiliaries, zero at construction (:84-86)
+  for (int j = 0; j < N; ++j) duals.emplace_back(s.F[j].r, s.k);
   for (int it = 0; it < iters; ++it) {
     for (int j = 0; j < N; ++j) {
       Mat S = gram_leave_out_one(s, j);
@@ -785,7 +877,7 @@ static int ntf_loop(NTFState &s, int algo, int iters, double *relerr) {
         mttkrp_dimtree(s.X, s.F, j, s.k, s.dt, M);
       else
         mttkrp_direct(s.X, krp, j, s.k, M);
-      int rc = ntf_update(algo, s.F[j], M, S, s.k);
+      int rc = ntf_update(algo, s.F[j], M, S, s.k, &duals[j]);
       if (rc < 0) return rc;
       ncp_normalize_mode(s, j);
     }
@@ -800,6 +892,12 @@ using namespace orc;
 
 // ============================================================================
 // C entry points (loaded with ctypes by the tests and bench.py).
+// The shared loop of oracle_nmf and oracle_nmf_sparse; Am is the (dense)
+// input; sparse_build applies the BUILD_SPARSE start of BPPNMF.
+static int nmf_core(int algo, Mat &Am, long m, long n, double *W, double *H, int k, int iters,
+                    const double *regW, const double *regH, double symm_reg, double *relerr,
+                    double *objective, bool sparse_build);
+
 extern "C" {
 
 void oracle_set_threads(int n) { omp_set_num_threads(n); }
@@ -820,11 +918,90 @@ int oracle_bppnnls(const double *AtA, int n, const double *AtB, long nrhs, doubl
 // NMF(A, W, H) + computeNMF() does (common/nmf.hpp:206-222).  relerr[it] is
 // printObjective's relative error after iteration it; objective[it] the
 // objective.  Returns 0, or -1 when BPP does not converge.
+static double g_last_loop_seconds = 0.0;
+
 int oracle_nmf(int algo, const double *A, long m, long n, double *W, double *H, int k, int iters,
                const double *regW, const double *regH, double symm_reg, double *relerr,
                double *objective) {
   Mat Am(m, n);
   std::memcpy(Am.d.data(), A, sizeof(double) * m * n);
+  return nmf_core(algo, Am, m, n, W, H, k, iters, regW, regH, symm_reg, relerr, objective, false);
+}
+
+// The sparse build of the shared-memory NMF (nmf/nmf.cpp:48-130 with
+// BUILD_SPARSE), A given as CSR (m x n, sorted column ids).  The products
+// with a sparse A sum the same non-zero terms as the dense products of its
+// densified form, so the loop runs on that dense form; what the sparse build
+// changes is BPPNMF's start (nmf/bppnmf.hpp:195-201): it constructs a
+// HALSNMF from W and H and takes its factors without running it, so the
+// factors enter the loop through HALSNMF's constructor normalize_by_W
+// (nmf/hals.hpp:46).
+int oracle_nmf_sparse(int algo, long m, long n, const long *indptr, const int *indices,
+                      const double *values, double *W, double *H, int k, int iters,
+                      const double *regW, const double *regH, double *relerr) {
+  Mat Am(m, n);
+  for (long i = 0; i < m; ++i)
+    for (long e = indptr[i]; e < indptr[i + 1]; ++e) Am(i, indices[e]) += values[e];
+  return nmf_core(algo, Am, m, n, W, H, k, iters, regW, regH, -1.0, relerr, nullptr, true);
+}
+
+// computeObjectiveError(A, W, H) for a sparse A (common/utils.hpp:190-241),
+// A as CSC (col_ptrs n + 1, row_indices, values) as Armadillo's SpMat holds
+// it: sqrt(nnzsse + ||Rw Rh^T||_F^2 - nnzwh) with Rw, Rh the R factors of
+// economy QRs of W (m x k) and H (n x k), here by modified Gram-Schmidt
+// (the R of any QR differs only in the signs of its rows, which the
+// Frobenius norm of Rw Rh^T does not see).
+static void mgs_r(const double *X, long rows, int k, std::vector<double> &R) {
+  std::vector<double> Q(X, X + rows * k);
+  R.assign((size_t)k * k, 0.0);
+  for (int j = 0; j < k; ++j) {
+    double *v = &Q[(size_t)rows * j];
+    for (int i = 0; i < j; ++i) {
+      const double *q = &Q[(size_t)rows * i];
+      double r = 0;
+      for (long t = 0; t < rows; ++t) r += q[t] * v[t];
+      R[i + (size_t)k * j] = r;
+      for (long t = 0; t < rows; ++t) v[t] -= r * q[t];
+    }
+    double nr = 0;
+    for (long t = 0; t < rows; ++t) nr += v[t] * v[t];
+    nr = std::sqrt(nr);
+    R[j + (size_t)k * j] = nr;
+    if (nr > 0)
+      for (long t = 0; t < rows; ++t) v[t] /= nr;
+  }
+}
+
+double oracle_sparse_fit_error(long m, long n, const long *col_ptrs, const int *row_indices,
+                               const double *values, const double *W, const double *H, int k) {
+  double nnzsse = 0, nnzwh = 0;
+  for (long col = 0; col < n; ++col) {
+    for (long ii = col_ptrs[col]; ii < col_ptrs[col + 1]; ++ii) {
+      const long row = row_indices[ii];
+      double t = 0;
+      for (int c = 0; c < k; ++c) t += W[row + m * c] * H[col + n * c];
+      nnzwh += t * t;
+      nnzsse += (values[ii] - t) * (values[ii] - t);
+    }
+  }
+  std::vector<double> Rw, Rh;
+  mgs_r(W, m, k, Rw);
+  mgs_r(H, n, k, Rh);
+  double nrm2 = 0;
+  for (int a = 0; a < k; ++a)
+    for (int b = 0; b < k; ++b) {
+      double s = 0;
+      for (int c = 0; c < k; ++c) s += Rw[a + (size_t)k * c] * Rh[b + (size_t)k * c];
+      nrm2 += s * s;
+    }
+  return std::sqrt(nnzsse + (nrm2 - nnzwh));
+}
+
+}  // extern "C"
+
+static int nmf_core(int algo, Mat &Am, long m, long n, double *W, double *H, int k, int iters,
+                    const double *regW, const double *regH, double symm_reg, double *relerr,
+                    double *objective, bool sparse_build) {
   NMFState s;
   s.A = &Am;
   s.m = m;
@@ -841,6 +1018,9 @@ int oracle_nmf(int algo, const double *A, long m, long n, double *W, double *H, 
   s.symm_reg = symm_reg;
   s.normA = std::sqrt(fro2(Am));
   int rc = 0;
+  // steady_clock around the iteration loop, as the reference's tic/toc
+  // brackets computeNMF (common/utils.hpp:44-57); the input copy is outside.
+  auto t0 = std::chrono::steady_clock::now();
   if (algo == 0) {
     for (int it = 0; it < iters; ++it) {
       // One MU iteration without the trailing normalisation.
@@ -868,6 +1048,7 @@ int oracle_nmf(int algo, const double *A, long m, long n, double *W, double *H, 
       if (objective) objective[it] = s.objective;
     }
   } else if (algo == 2) {
+    if (sparse_build) normalize_by_W(s);  // nmf/bppnmf.hpp:195-201 via nmf/hals.hpp:46
     for (int it = 0; it < iters; ++it) {
       if (bpp_update(s, true, s.H, s.regW, s.W) < 0) {
         rc = -1;
@@ -882,12 +1063,200 @@ int oracle_nmf(int algo, const double *A, long m, long n, double *W, double *H, 
       if (objective) objective[it] = s.objective;
     }
     if (rc == 0) normalize_by_W(s);  // nmf/bppnmf.hpp:236
+  } else if (algo == 4) {
+    // AOADMMNMF (nmf/aoadmm.hpp): constructor normalize_by_W (:78-85), duals
+    // U (m x k) and V (n x k) start at zero and persist; H then W per
+    // iteration; no final normalisation.
+    normalize_by_W(s);
+    Mat U(m, k), V(n, k);
+    for (int it = 0; it < iters && rc == 0; ++it) {
+      Mat WtA = mul(s.W, 'T', *s.A, 'N');  // k x n
+      Mat WtW = mul(s.W, 'T', s.W, 'N');
+      apply_reg(s.regH, WtW);
+      if (!admm_solve(WtW, WtA, s.H, V)) rc = -1;
+      Mat AH = mul(*s.A, 'N', s.H, 'N');  // m x k
+      Mat HtH = mul(s.H, 'T', s.H, 'N');
+      apply_reg(s.regW, HtH);
+      Mat AHt(k, m);
+      for (long i = 0; i < m; ++i)
+        for (long c = 0; c < k; ++c) AHt(c, i) = AH(i, c);
+      if (rc == 0 && !admm_solve(HtH, AHt, s.W, U)) rc = -1;
+      objective_error(s);
+      if (relerr) relerr[it] = rel_error(s);
+      if (objective) objective[it] = s.objective;
+    }
+  } else if (algo == 7) {
+    // GNSYMNMF (nmf/gnsym.hpp:178-260): A ~ H H^T for a square A; W is
+    // cleared and stays empty.  Gauss-Newton step from at most k conjugate-
+    // gradient iterations (tolerance 1e-4 on the squared residual), then the
+    // projection max(H - D^T, 0); error by the fast formula (:145-163).
+    if (m != n) return -3;
+    const double cg_tol = 0.0001;
+    const double sqnormA = s.normA * s.normA;
+    Mat HtH = mul(s.H, 'T', s.H, 'N');
+    Mat AHt = mul(s.H, 'T', *s.A, 'N');  // k x n
+    for (int it = 0; it < iters; ++it) {
+      Mat grad(k, n), Pk, Dt(k, n), Y(k, n);
+      for (long j = 0; j < n; ++j)
+        for (long c = 0; c < k; ++c) {
+          double hh = 0;
+          for (long t = 0; t < k; ++t) hh += HtH(c, t) * s.H(j, t);
+          grad(c, j) = -2 * (AHt(c, j) - hh);
+        }
+      Pk = grad;
+      double rsold = 0;
+      for (double v : grad.d) rsold += v * v;
+      if (rsold > cg_tol) {
+        for (long cg = 0; cg < k; ++cg) {
+          Mat XtX = mul(Pk, 'N', s.H, 'N');  // k x k
+          for (long j = 0; j < n; ++j)
+            for (long c = 0; c < k; ++c) {
+              double a = 0, b = 0;
+              for (long t = 0; t < k; ++t) {
+                a += HtH(c, t) * Pk(t, j);
+                b += XtX(c, t) * s.H(j, t);
+              }
+              Y(c, j) = 2 * (a + b);
+            }
+          double py = 0;
+          for (size_t e = 0; e < Pk.d.size(); ++e) py += Pk.d[e] * Y.d[e];
+          const double alpha = rsold / py;
+          for (size_t e = 0; e < Dt.d.size(); ++e) Dt.d[e] += alpha * Pk.d[e];
+          for (size_t e = 0; e < grad.d.size(); ++e) grad.d[e] -= alpha * Y.d[e];
+          double rsnew = 0;
+          for (double v : grad.d) rsnew += v * v;
+          if (rsnew < cg_tol) break;
+          const double beta = rsnew / rsold;
+          rsold = rsnew;
+          for (size_t e = 0; e < Pk.d.size(); ++e) Pk.d[e] = grad.d[e] + beta * Pk.d[e];
+        }
+      }
+      for (long j = 0; j < n; ++j)
+        for (long c = 0; c < k; ++c) {
+          double v = s.H(j, c) - Dt(c, j);
+          s.H(j, c) = v > 0.0 ? v : 0.0;
+        }
+      HtH = mul(s.H, 'T', s.H, 'N');
+      AHt = mul(s.H, 'T', *s.A, 'N');
+      double tAHHt = 0, tHH = 0;
+      for (long j = 0; j < n; ++j)
+        for (long c = 0; c < k; ++c) tAHHt += AHt(c, j) * s.H(j, c);
+      for (long a = 0; a < k; ++a)
+        for (long b = 0; b < k; ++b) tHH += HtH(a, b) * HtH(b, a);
+      s.fit_err_sq = sqnormA - 2 * tAHHt + tHH;
+      s.objective = s.fit_err_sq;
+      if (relerr) relerr[it] = rel_error(s);
+      if (objective) objective[it] = s.objective;
+    }
   } else {
     return -2;
   }
+  g_last_loop_seconds =
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   std::memcpy(W, s.W.d.data(), sizeof(double) * m * k);
   std::memcpy(H, s.H.d.data(), sizeof(double) * n * k);
   return rc;
+}
+
+extern "C" {
+
+// One call of the ADMM solve shared by AO-ADMM NMF and NTF: G k x k (the
+// Gram with its regularisation, before the rho shift), R k x n, X and D n x k
+// (updated; D persists between calls as the reference's dual does).
+// Returns 0, or -1 when G + rho I is not positive definite.
+int oracle_admm_solve(const double *G, int k, const double *R, long n, double *X, double *D) {
+  Mat Gm(k, k), Rm(k, n), Xm(n, k), Dm(n, k);
+  std::memcpy(Gm.d.data(), G, sizeof(double) * k * k);
+  std::memcpy(Rm.d.data(), R, sizeof(double) * k * n);
+  std::memcpy(Xm.d.data(), X, sizeof(double) * n * k);
+  std::memcpy(Dm.d.data(), D, sizeof(double) * n * k);
+  if (!admm_solve(Gm, Rm, Xm, Dm)) return -1;
+  std::memcpy(X, Xm.d.data(), sizeof(double) * n * k);
+  std::memcpy(D, Dm.d.data(), sizeof(double) * n * k);
+  return 0;
+}
+
+// Wall time of the iteration loop of the last oracle_nmf / oracle_mu_f32 call.
+double oracle_last_loop_seconds() { return g_last_loop_seconds; }
+
+// Loads a host BLAS for the CPU baseline: `path` is a shared library that
+// exports cblas_dgemm (and optionally cblas_sgemm); nthreads > 0 is passed to
+// MKL_Set_Num_Threads / openblas_set_num_threads when the library has one.
+// Returns 1 with cblas_sgemm, 0 with cblas_dgemm only, -1 when the library or
+// cblas_dgemm is missing (the blocked GEMM stays in use).
+int oracle_use_blas(const char *path, int nthreads) {
+  if (!path) {
+    g_dgemm = nullptr;
+    g_sgemm = nullptr;
+    return -1;
+  }
+  void *h = dlopen(path, RTLD_NOW | RTLD_LOCAL);
+  if (!h) return -1;
+  cblas_dgemm_fn d = (cblas_dgemm_fn)dlsym(h, "cblas_dgemm");
+  if (!d) return -1;
+  if (nthreads > 0) {
+    typedef void (*set_fn)(int);
+    set_fn f = (set_fn)dlsym(h, "MKL_Set_Num_Threads");
+    if (!f) f = (set_fn)dlsym(h, "openblas_set_num_threads");
+    if (f) f(nthreads);
+  }
+  g_dgemm = d;
+  g_sgemm = (cblas_sgemm_fn)dlsym(h, "cblas_sgemm");
+  return g_sgemm ? 1 : 0;
+}
+
+// FP32 variant of the MU loop for a dtype-matched CPU baseline (the reference
+// computes in FP64 only): MUNMF::computeNMF (nmf/mu.hpp:48-91) in single
+// precision through the loaded cblas_sgemm, error by computeObjectiveError's
+// trace formula (common/nmf.hpp:390-424, A^T W recomputed) accumulated in
+// FP64.  Returns -3 when no cblas_sgemm was loaded.
+int oracle_mu_f32(const float *A, long m, long n, float *W, float *H, int k, int iters,
+                  double *relerr) {
+  if (!g_sgemm) return -3;
+  if (!fits_int(m, n, k, m, n, m)) return -2;
+  const int M = (int)m, Nn = (int)n, K = (int)k;
+  std::vector<float> AtW((size_t)n * k), WtW((size_t)k * k), HWtW((size_t)n * k);
+  std::vector<float> AH((size_t)m * k), HtH((size_t)k * k), WHtH((size_t)m * k);
+  double normA2 = 0;
+#pragma omp parallel for reduction(+ : normA2)
+  for (long e = 0; e < m * n; ++e) normA2 += (double)A[e] * (double)A[e];
+  const float eps = (float)EPS_1EM16;
+  auto t0 = std::chrono::steady_clock::now();
+  for (int it = 0; it < iters; ++it) {
+    g_sgemm(102, 112, 111, Nn, K, M, 1.0f, A, M, W, M, 0.0f, AtW.data(), Nn);
+    g_sgemm(102, 112, 111, K, K, M, 1.0f, W, M, W, M, 0.0f, WtW.data(), K);
+    g_sgemm(102, 111, 111, Nn, K, K, 1.0f, H, Nn, WtW.data(), K, 0.0f, HWtW.data(), Nn);
+#pragma omp parallel for
+    for (long e = 0; e < n * k; ++e) H[e] = (H[e] * AtW[e]) / (HWtW[e] + eps);
+    g_sgemm(102, 111, 111, M, K, Nn, 1.0f, A, M, H, Nn, 0.0f, AH.data(), M);
+    g_sgemm(102, 112, 111, K, K, Nn, 1.0f, H, Nn, H, Nn, 0.0f, HtH.data(), K);
+    g_sgemm(102, 111, 111, M, K, K, 1.0f, W, M, HtH.data(), K, 0.0f, WHtH.data(), M);
+#pragma omp parallel for
+    for (long e = 0; e < m * k; ++e) W[e] = (W[e] * AH[e]) / (WHtH[e] + eps);
+    // computeObjectiveError: A^T W again, traces in FP64.
+    g_sgemm(102, 112, 111, Nn, K, M, 1.0f, A, M, W, M, 0.0f, AtW.data(), Nn);
+    g_sgemm(102, 112, 111, K, K, M, 1.0f, W, M, W, M, 0.0f, WtW.data(), K);
+    g_sgemm(102, 112, 111, K, K, Nn, 1.0f, H, Nn, H, Nn, 0.0f, HtH.data(), K);
+    double tr1 = 0, tr2 = 0;
+    for (long e = 0; e < n * k; ++e) tr1 += (double)H[e] * (double)AtW[e];
+    for (int a = 0; a < K; ++a)
+      for (int b = 0; b < K; ++b) tr2 += (double)WtW[a + K * b] * (double)HtH[b + K * a];
+    double fit2 = normA2 - 2 * tr1 + tr2;
+    if (relerr) relerr[it] = (fit2 > 0 ? std::sqrt(fit2) : std::sqrt(normA2)) / std::sqrt(normA2);
+  }
+  // normalize_by_W after the loop (nmf/mu.hpp:90, common/nmf.hpp:152-161).
+  for (long c = 0; c < k; ++c) {
+    double nrm = 0;
+    for (long i = 0; i < m; ++i) nrm += (double)W[i + m * c] * (double)W[i + m * c];
+    nrm = std::sqrt(nrm);
+    if (nrm > 0) {
+      for (long i = 0; i < m; ++i) W[i + m * c] = (float)(W[i + m * c] / nrm);
+      for (long j = 0; j < n; ++j) H[j + n * c] = (float)(H[j + n * c] * nrm);
+    }
+  }
+  g_last_loop_seconds =
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return 0;
 }
 
 // One NTF mode update with the Hadamard-of-Grams S (k x k) and the MTTKRP M

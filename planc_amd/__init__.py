@@ -16,7 +16,8 @@ import weakref
 import numpy as np
 
 from . import _lib
-from ._lib import ANLSBPP, BF16, F16, F32, HALS, MU, R2, HierDesc, HierNode, NMFDesc, NTFDesc  # noqa: F401
+from ._lib import (ANLSBPP, AOADMM, BF16, F16, F32, GNSYM, HALS, MU, NESTEROV, R2, HierDesc,  # noqa: F401
+                   HierNode, NMFDesc, NTFDesc)
 
 __all__ = ["Context", "PlancError", "colmajor", "F32", "BF16", "F16", "MU", "HALS", "ANLSBPP"]
 
@@ -394,10 +395,23 @@ class Context:
         return out, cols, H, W
 
     # ---- whole loops -------------------------------------------------------
+    def transpose16(self, A16):
+        """Transposed copy (n x m, column-major) of a 16-bit m x n matrix."""
+        torch = _torch()
+        m, n = A16.shape
+        out = empty_colmajor((n, m), dtype=A16.dtype, device=A16.device)
+        self._check(self.lib.planc_transpose16(self.ctx, _ptr(A16), m, n, _ld(A16), _ptr(out), n),
+                    "planc_transpose16")
+        return out
+
     def nmf_run(self, A, W, H, algo, iters, dtype=F32, regW=(0.0, 0.0), regH=(0.0, 0.0),
-                compute_error=True, A16=None, colscale=None):
+                compute_error=True, A16=None, colscale=None, symm_reg=-1.0, A16T=None,
+                normA2=0.0):
         """computeNMF on the device; W (m x k) and H (n x k) are updated in
-        place.  Returns the per-iteration relative error (numpy)."""
+        place.  Returns the per-iteration relative error (numpy).  symm_reg > 0
+        turns on ANLS-BPP's symmetric regularisation (square A).  A16T (from
+        transpose16(A16)) and normA2 (||A||_F^2) may be passed in once made,
+        as the reference's constructors form At and normA once."""
         m, n = A.shape
         k = W.shape[1]
         d = NMFDesc()
@@ -405,6 +419,9 @@ class Context:
         d.regW[0], d.regW[1] = regW
         d.regH[0], d.regH[1] = regH
         d.compute_error = 1 if compute_error else 0
+        d.symm_reg = float(symm_reg)
+        d.A16T = A16T.data_ptr() if A16T is not None else None
+        d.normA2 = float(normA2)
         if _ld(W) != m or _ld(H) != n:
             raise ValueError("W and H must be tight column-major (ld = m, n)")
         rel = np.zeros(max(iters, 1))
@@ -412,6 +429,55 @@ class Context:
                                            _ptr(colscale), _ptr(W), _ptr(H),
                                            rel.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
                     "planc_nmf_run")
+        return rel[:iters]
+
+    # ---- sparse input (CSR: indptr int64, indices int32, values float32) ----
+    def csr_transpose(self, m, n, indptr, indices, values):
+        """Device CSR of A^T (n x m); returns (indptr, indices, values)."""
+        torch = _torch()
+        nnz = int(indices.numel())
+        tp = torch.empty(n + 1, dtype=torch.int64, device=indptr.device)
+        ti = torch.empty(max(nnz, 1), dtype=torch.int32, device=indptr.device)
+        tv = torch.empty(max(nnz, 1), dtype=torch.float32, device=indptr.device)
+        self._check(self.lib.planc_csr_transpose(self.ctx, m, n, _ptr(indptr), _ptr(indices), _ptr(values),
+                                                 _ptr(tp), _ptr(ti), _ptr(tv)), "planc_csr_transpose")
+        return tp, ti[:nnz], tv[:nnz]
+
+    def spmm(self, rows, cols, indptr, indices, values, F):
+        """A (rows x cols, CSR) * F (cols x k, column-major) -> rows x k."""
+        k = F.shape[1]
+        out = empty_colmajor((rows, k), device=F.device)
+        self._check(self.lib.planc_spmm(self.ctx, rows, cols, _ptr(indptr), _ptr(indices), _ptr(values),
+                                        _ptr(F), _ld(F), k, _ptr(out), rows), "planc_spmm")
+        return out
+
+    def sparse_fit_error(self, m, n, indptr, indices, values, W, H):
+        """computeObjectiveError(A, W, H) of common/utils.hpp:190-241."""
+        err = ctypes.c_double()
+        self._check(self.lib.planc_sparse_fit_error(self.ctx, m, n, _ptr(indptr), _ptr(indices),
+                                                    _ptr(values), _ptr(W), _ld(W), _ptr(H), _ld(H),
+                                                    W.shape[1], ctypes.byref(err)),
+                    "planc_sparse_fit_error")
+        return err.value
+
+    def nmf_sparse_run(self, m, n, indptr, indices, values, W, H, algo, iters, regW=(0.0, 0.0),
+                       regH=(0.0, 0.0), compute_error=True, symm_reg=-1.0):
+        """computeNMF of the sparse build on the device; W (m x k) and H
+        (n x k) updated in place.  Returns the per-iteration relative error."""
+        k = W.shape[1]
+        d = NMFDesc()
+        d.algo, d.dtype, d.m, d.n, d.k, d.iters = algo, F32, m, n, k, iters
+        d.regW[0], d.regW[1] = regW
+        d.regH[0], d.regH[1] = regH
+        d.compute_error = 1 if compute_error else 0
+        d.symm_reg = float(symm_reg)
+        if _ld(W) != m or _ld(H) != n:
+            raise ValueError("W and H must be tight column-major (ld = m, n)")
+        rel = np.zeros(max(iters, 1))
+        self._check(self.lib.planc_nmf_sparse_run(self.ctx, ctypes.byref(d), _ptr(indptr), _ptr(indices),
+                                                  _ptr(values), _ptr(W), _ptr(H),
+                                                  rel.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+                    "planc_nmf_sparse_run")
         return rel[:iters]
 
     def ntf_run(self, X, factors, algo, iters, dtype=F32, dimtree=True, compute_error=True,

@@ -22,13 +22,14 @@ PLANC_ENOMEM = -4
 PLANC_EUNSUPPORTED = -5
 
 F32, BF16, F16 = 0, 1, 2
-MU, HALS, ANLSBPP, R2 = 0, 1, 2, 8
+MU, HALS, ANLSBPP, AOADMM, NESTEROV, GNSYM, R2 = 0, 1, 2, 4, 5, 7, 8
 
 
 class NMFDesc(ctypes.Structure):
     _fields_ = [("algo", c_int), ("dtype", c_int), ("m", c_long), ("n", c_long), ("k", c_int),
                 ("iters", c_int), ("regW", c_double * 2), ("regH", c_double * 2),
-                ("compute_error", c_int)]
+                ("compute_error", c_int), ("A16T", c_void_p), ("normA2", c_double),
+                ("symm_reg", c_double)]
 
 
 class HierDesc(ctypes.Structure):
@@ -62,6 +63,7 @@ SIGNATURES = {
     "planc_memcpy_d2h": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t]),
     "planc_convert": (c_int, [c_void_p, c_int, c_void_p, c_long, c_long, c_long, c_void_p, c_long,
                               c_void_p]),
+    "planc_transpose16": (c_int, [c_void_p, c_void_p, c_long, c_long, c_long, c_void_p, c_long]),
     "planc_gen_uniform": (c_int, [c_void_p, c_void_p, c_long, ctypes.c_ulonglong,
                                   ctypes.c_ulonglong]),
     "planc_gen_lowrank": (c_int, [c_void_p, c_long, c_long, c_int, c_void_p, c_void_p, c_double,
@@ -108,6 +110,14 @@ SIGNATURES = {
                                   ctypes.POINTER(HierNode), ctypes.POINTER(c_int),
                                   ctypes.POINTER(c_long), ctypes.POINTER(c_double),
                                   ctypes.POINTER(c_double)]),
+    "planc_csr_transpose": (c_int, [c_void_p, c_long, c_long, c_void_p, c_void_p, c_void_p, c_void_p,
+                                    c_void_p, c_void_p]),
+    "planc_spmm": (c_int, [c_void_p, c_long, c_long, c_void_p, c_void_p, c_void_p, c_void_p, c_long,
+                           c_int, c_void_p, c_long]),
+    "planc_sparse_fit_error": (c_int, [c_void_p, c_long, c_long, c_void_p, c_void_p, c_void_p, c_void_p,
+                                       c_long, c_void_p, c_long, c_int, ctypes.POINTER(c_double)]),
+    "planc_nmf_sparse_run": (c_int, [c_void_p, ctypes.POINTER(NMFDesc), c_void_p, c_void_p, c_void_p,
+                                     c_void_p, c_void_p, ctypes.POINTER(c_double)]),
     "planc_nmf_run": (c_int, [c_void_p, ctypes.POINTER(NMFDesc), c_void_p, c_long, c_void_p,
                               c_void_p, c_void_p, c_void_p, ctypes.POINTER(c_double)]),
     "planc_ntf_run": (c_int, [c_void_p, ctypes.POINTER(NTFDesc), c_void_p, c_void_p,

@@ -1,6 +1,7 @@
 // C ABI of libplanc_hip.so: contexts, memory helpers, kernel-level entry
 // points and the device-resident computeNMF / computeNTF loops.
 #include <algorithm>
+#include <functional>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -152,6 +153,218 @@ void graph_store(planc_ctx *ctx, const std::vector<uintptr_t> &key, hipGraphExec
   ctx->graphs.push_back({key, exec});
 }
 
+// The computeNMF loop of MUNMF / HALSNMF / BPPNMF (nmf/mu.hpp:48-91,
+// nmf/hals.hpp:50-116, nmf/bppnmf.hpp:187-241) shared by the dense and the
+// sparse entry points; only the two products with A and ||A||^2 differ.
+struct NMFProducts {
+  std::function<int(const float *, float *)> wta;  // F^T A (k x n, ld k), F m x k (ld m)
+  std::function<int(const float *, float *)> ah;   // A F (m x k, ld m), F n x k (ld n)
+  std::function<int(double *)> norma2; // ||A||_F^2 into a device double
+  bool bpp_prenormalize = false;       // sparse builds' BPPNMF start (see below)
+};
+
+int nmf_loop(planc_ctx *ctx, const planc_nmf_desc *desc, NMFProducts &P, float *d_W, float *d_H,
+             double *h_relerr) {
+  const long m = desc->m, n = desc->n;
+  const int k = desc->k, iters = desc->iters, algo = desc->algo;
+  // Loop buffers.
+  float *fbuf = (float *)ws_get(ctx, 4, sizeof(float) * ((size_t)k * n + (size_t)m * k));
+  const size_t kk = (size_t)k * k;
+  double *dbuf = (double *)ws_get(ctx, 5, sizeof(double) * (4 * kk + 8 + (size_t)iters));
+  if (!fbuf || !dbuf) return set_error(ctx, PLANC_ENOMEM, "planc_nmf_run: workspace");
+  float *WtA = fbuf;                // k x n
+  float *AH = fbuf + (size_t)k * n;  // m x k
+  double *WtW = dbuf, *HtH = dbuf + kk, *Greg = dbuf + 2 * kk, *WtW2 = dbuf + 3 * kk;
+  double *normA2 = dbuf + 4 * kk, *dotv = normA2 + 1, *nrm = normA2 + 2;  // nrm unused slot
+  double *rel = normA2 + 8;
+  (void)nrm;
+  int rc;
+#define RC(x)          \
+  do {                 \
+    rc = (x);          \
+    if (rc) return rc; \
+  } while (0)
+  if (desc->compute_error) {
+    if (desc->normA2 > 0)
+      PLANC_HIP_CHECK(ctx, hipMemcpyAsync(normA2, &desc->normA2, sizeof(double), hipMemcpyHostToDevice,
+                                          ctx->stream));
+    else
+      RC(P.norma2(normA2));
+  }
+  const double sr = desc->symm_reg;
+  if (sr > 0 && (algo != PLANC_ANLSBPP || m != n))
+    return set_error(ctx, PLANC_EINVAL,
+                     "planc_nmf_run: symmetric regularisation needs ANLS-BPP and a square A");
+  if (algo == PLANC_AOADMM) {
+    // AOADMMNMF::computeNMF (nmf/aoadmm.hpp:88-192): constructor
+    // normalize_by_W; per iteration an ADMM solve for H (Gram W^T W + regH,
+    // right-hand side W^T A, dual V) then for W (H^T H + regW, (A H)^T, dual
+    // U); duals start at zero; no final normalisation.
+    int *dflag = (int *)ws_get(ctx, 11, sizeof(int));
+    float *duals = (float *)ws_get(ctx, 33, sizeof(float) * (size_t)(m + n) * k);
+    if (!dflag || !duals) return set_error(ctx, PLANC_ENOMEM, "planc_nmf_run: workspace");
+    float *U = duals, *V = duals + (size_t)m * k;
+    PLANC_HIP_CHECK(ctx, hipMemsetAsync(dflag, 0, sizeof(int), ctx->stream));
+    PLANC_HIP_CHECK(ctx, hipMemsetAsync(duals, 0, sizeof(float) * (size_t)(m + n) * k, ctx->stream));
+    RC(planc_normalize_by_w(ctx, d_W, m, m, d_H, n, n, k));
+    for (int it = 0; it < iters; ++it) {
+      RC(P.wta(d_W, WtA));
+      RC(gram(ctx, d_W, m, k, m, WtW));
+      PLANC_HIP_CHECK(ctx, hipMemcpyAsync(Greg, WtW, kk * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+      RC(apply_reg(ctx, Greg, k, desc->regH));
+      RC(admm_solve(ctx, Greg, k, WtA, k, 1, d_H, n, V, n, n, dflag));
+      RC(P.ah(d_H, AH));
+      RC(gram(ctx, d_H, n, k, n, HtH));
+      PLANC_HIP_CHECK(ctx, hipMemcpyAsync(Greg, HtH, kk * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+      RC(apply_reg(ctx, Greg, k, desc->regW));
+      RC(admm_solve(ctx, Greg, k, AH, 1, m, d_W, m, U, m, m, dflag));
+      if (desc->compute_error) {
+        RC(gram(ctx, d_W, m, k, m, WtW2));
+        RC(dot(ctx, d_W, 1, m, AH, 1, m, m, k, dotv, 0));
+        RC(nmf_relerr(ctx, normA2, dotv, WtW2, HtH, k, rel + it, nullptr));
+      }
+    }
+    int hflag = 0;
+    PLANC_HIP_CHECK(ctx, hipMemcpyAsync(&hflag, dflag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    if (h_relerr && desc->compute_error && iters > 0)
+      PLANC_HIP_CHECK(ctx, hipMemcpyAsync(h_relerr, rel, sizeof(double) * iters, hipMemcpyDeviceToHost, ctx->stream));
+    PLANC_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    if (hflag) return set_error(ctx, PLANC_ENOCONV, "AO-ADMM: shifted Gram not positive definite");
+    return PLANC_OK;
+  }
+  if (algo == PLANC_GNSYM) {
+    // GNSYMNMF::computeNMF (nmf/gnsym.hpp:178-260): A ~ H H^T.  H^T A and
+    // H^T H of the current H feed the gradient; after the step they are
+    // recomputed for the error and reused by the next gradient.
+    if (m != n) return set_error(ctx, PLANC_EINVAL, "planc_nmf_run: GNSYM needs a square A");
+    RC(gram(ctx, d_H, n, k, n, HtH));
+    RC(P.wta(d_H, WtA));
+    for (int it = 0; it < iters; ++it) {
+      RC(gnsym_step(ctx, d_H, n, k, WtA, HtH));
+      RC(gram(ctx, d_H, n, k, n, HtH));
+      RC(P.wta(d_H, WtA));
+      if (desc->compute_error) {
+        RC(dot(ctx, d_H, 1, n, WtA, k, 1, n, k, dotv, 0));
+        RC(nmf_relerr(ctx, normA2, dotv, HtH, HtH, k, rel + it, nullptr));
+      }
+    }
+    if (h_relerr && desc->compute_error && iters > 0)
+      PLANC_HIP_CHECK(ctx, hipMemcpyAsync(h_relerr, rel, sizeof(double) * iters, hipMemcpyDeviceToHost, ctx->stream));
+    PLANC_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return PLANC_OK;
+  }
+  if (algo == PLANC_ANLSBPP) {
+    // BPPNMF::computeNMF (nmf/bppnmf.hpp:187-241): W given H first, then H
+    // given W, each a batched NNLS on the FP64 normal equations.
+    int *dflag = (int *)ws_get(ctx, 11, sizeof(int));
+    if (!dflag) return set_error(ctx, PLANC_ENOMEM, "planc_nmf_run: workspace");
+    PLANC_HIP_CHECK(ctx, hipMemsetAsync(dflag, 0, sizeof(int), ctx->stream));
+    // Sparse builds construct a HALSNMF from W, H "to get proper
+    // initializations" and take its factors without running it
+    // (nmf/bppnmf.hpp:195-201): only the HALSNMF constructor's
+    // normalize_by_W (nmf/hals.hpp:46) takes effect.
+    if (P.bpp_prenormalize) RC(planc_normalize_by_w(ctx, d_W, m, m, d_H, n, n, k));
+    for (int it = 0; it < iters; ++it) {
+      RC(gram(ctx, d_H, n, k, n, HtH));
+      PLANC_HIP_CHECK(ctx, hipMemcpyAsync(Greg, HtH, kk * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+      RC(apply_reg(ctx, Greg, k, desc->regW));
+      RC(P.ah(d_H, AH));
+      RC(symm_reg_apply(ctx, Greg, k, AH, 1, m, d_H, n, sr));
+      RC(bpp_solve(ctx, Greg, k, AH, 1, m, m, d_W, 1, m, dflag));
+      RC(gram(ctx, d_W, m, k, m, WtW));
+      PLANC_HIP_CHECK(ctx, hipMemcpyAsync(Greg, WtW, kk * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+      RC(apply_reg(ctx, Greg, k, desc->regH));
+      RC(P.wta(d_W, WtA));
+      if (sr > 0) {
+        // The error below needs the plain W^T A: keep the regularised
+        // right-hand side in the AH buffer (m x k >= k x n when m == n).
+        PLANC_HIP_CHECK(ctx, hipMemcpyAsync(AH, WtA, sizeof(float) * (size_t)k * n,
+                                            hipMemcpyDeviceToDevice, ctx->stream));
+        RC(symm_reg_apply(ctx, Greg, k, AH, k, 1, d_W, m, sr));
+        RC(bpp_solve(ctx, Greg, k, AH, k, 1, n, d_H, 1, n, dflag));
+      } else {
+        RC(bpp_solve(ctx, Greg, k, WtA, k, 1, n, d_H, 1, n, dflag));
+      }
+      if (desc->compute_error) {
+        // tr(H^T A^T W) = <H, (W^T A)^T> with the W^T A of the H half-step.
+        RC(dot(ctx, d_H, 1, n, WtA, k, 1, n, k, dotv, 0));
+        RC(gram(ctx, d_H, n, k, n, HtH));
+        RC(nmf_relerr(ctx, normA2, dotv, WtW, HtH, k, rel + it, nullptr));
+      }
+    }
+    RC(planc_normalize_by_w(ctx, d_W, m, m, d_H, n, n, k));
+    int hflag = 0;
+    PLANC_HIP_CHECK(ctx, hipMemcpyAsync(&hflag, dflag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    if (h_relerr && desc->compute_error && iters > 0)
+      PLANC_HIP_CHECK(ctx, hipMemcpyAsync(h_relerr, rel, sizeof(double) * iters, hipMemcpyDeviceToHost, ctx->stream));
+    PLANC_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    if (hflag) return set_error(ctx, PLANC_ENOCONV, "BPP failed: more than 5k pivot iterations");
+    return PLANC_OK;
+  }
+  double *wnorms = (double *)ws_get(ctx, 9, sizeof(double) * k);
+  if (!wnorms) return set_error(ctx, PLANC_ENOMEM, "planc_nmf_run: workspace");
+  if (algo == PLANC_HALS) {
+    RC(hals_sync_reset(ctx));
+    RC(planc_normalize_by_w(ctx, d_W, m, m, d_H, n, n, k));
+  }
+  const bool regW_on = desc->regW[0] > 0 || desc->regW[1] > 0;
+  const bool regH_on = desc->regH[0] > 0 || desc->regH[1] > 0;
+  // W^T W of the current W: computed once per iteration (after the W
+  // half-step) and carried into the next H half-step -- rescaled by the
+  // normalisation for HALS -- instead of a second pass over W.
+  bool wtw_valid = false;
+  for (int it = 0; it < iters; ++it) {
+    // H half-step: W^T A, W^T W (+ regH).
+    RC(P.wta(d_W, WtA));
+    if (!wtw_valid) RC(gram(ctx, d_W, m, k, m, WtW));
+    const double *Gh = WtW;
+    if (regH_on) {
+      PLANC_HIP_CHECK(ctx, hipMemcpyAsync(Greg, WtW, kk * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+      RC(apply_reg(ctx, Greg, k, desc->regH));
+      Gh = Greg;
+    }
+    RC(row_update(ctx, algo == PLANC_MU ? 0 : 1, d_H, n, k, n, WtA, k, 1, Gh, 1e-17f));
+    // W half-step: A H, H^T H (+ regW).
+    RC(P.ah(d_H, AH));
+    RC(gram(ctx, d_H, n, k, n, HtH));
+    const double *Gw = HtH;
+    if (regW_on) {
+      PLANC_HIP_CHECK(ctx, hipMemcpyAsync(Greg, HtH, kk * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+      RC(apply_reg(ctx, Greg, k, desc->regW));
+      Gw = Greg;
+    }
+    if (algo == PLANC_MU)
+      RC(row_update(ctx, 0, d_W, m, k, m, AH, 1, m, Gw, 1e-17f));
+    else
+      RC(hals_cols(ctx, d_W, m, k, m, AH, m, Gw));
+    RC(gram(ctx, d_W, m, k, m, WtW2));
+    if (desc->compute_error) {
+      // <W, A H> with the H of this iteration equals tr(H^T A^T W) of
+      // computeObjectiveError (common/nmf.hpp:391-396) without the third
+      // product; both traces are invariant under normalize_by_W.
+      RC(dot(ctx, d_W, 1, m, AH, 1, m, m, k, dotv, 0));
+      RC(nmf_relerr(ctx, normA2, dotv, WtW2, HtH, k, rel + it, nullptr));
+    }
+    if (algo == PLANC_HALS) {
+      // normalize_by_W: the column norms are the square roots of diag(W^T W);
+      // the next W^T W is D^-1 (W^T W) D^-1.
+      RC(gram_norms(ctx, WtW2, k, wnorms));
+      RC(scale_cols(ctx, d_W, m, m, d_H, n, n, k, wnorms));
+      RC(gram_rescale(ctx, WtW2, k, wnorms, WtW));
+    } else {
+      PLANC_HIP_CHECK(ctx, hipMemcpyAsync(WtW, WtW2, kk * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    wtw_valid = true;
+  }
+  if (algo == PLANC_MU) RC(planc_normalize_by_w(ctx, d_W, m, m, d_H, n, n, k));
+#undef RC
+  if (h_relerr && desc->compute_error && iters > 0)
+    PLANC_HIP_CHECK(ctx, hipMemcpyAsync(h_relerr, rel, sizeof(double) * iters, hipMemcpyDeviceToHost, ctx->stream));
+  PLANC_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  if (algo == PLANC_HALS) return hals_sync_check(ctx);
+  return PLANC_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -247,6 +460,14 @@ int planc_convert(planc_ctx *ctx, int dtype, const float *d_src, long rows, long
       ld_dst < rows || (rows * cols > 0 && (!d_src || !d_dst)))
     return set_error(ctx, PLANC_EINVAL, "planc_convert: bad arguments");
   return convert16(ctx, dtype, d_src, rows, cols, ld_src, d_dst, ld_dst, d_colscale);
+}
+
+int planc_transpose16(planc_ctx *ctx, const void *d_in, long rows, long cols, long ldi, void *d_out,
+                      long ldo) {
+  if (check_ctx(ctx)) return PLANC_EINVAL;
+  if (rows < 0 || cols < 0 || ldi < rows || ldo < cols || (rows * cols > 0 && (!d_in || !d_out)))
+    return set_error(ctx, PLANC_EINVAL, "planc_transpose16: bad arguments");
+  return transpose16(ctx, d_in, rows, cols, ldi, d_out, ldo);
 }
 
 int planc_gemm_nn(planc_ctx *ctx, int dtype, const void *d_A, long lda, long L, long K,
@@ -411,7 +632,8 @@ int planc_nmf_run(planc_ctx *ctx, const planc_nmf_desc *desc, const float *d_A, 
   if (m <= 0 || n <= 0 || k <= 0 || k > 64 || iters < 0 || lda < m || !d_A || !d_W || !d_H ||
       !valid_dtype(dt))
     return set_error(ctx, PLANC_EINVAL, "planc_nmf_run: bad arguments (k must be 1..64)");
-  if (algo != PLANC_MU && algo != PLANC_HALS && algo != PLANC_ANLSBPP)
+  if (algo != PLANC_MU && algo != PLANC_HALS && algo != PLANC_ANLSBPP && algo != PLANC_AOADMM &&
+      algo != PLANC_GNSYM)
     return set_error(ctx, PLANC_EUNSUPPORTED, "planc_nmf_run: algorithm not supported");
   PLANC_HIP_CHECK(ctx, hipSetDevice(ctx->device));
   const void *Aop = d_A;
@@ -435,131 +657,143 @@ int planc_nmf_run(planc_ctx *ctx, const planc_nmf_desc *desc, const float *d_A, 
   // 16-bit: a transposed copy makes A*H a TN product too (its contraction
   // index n contiguous), so both products stream with 16-byte MFMA operands.
   const void *AopT = nullptr;
-  if (dt != PLANC_F32) {
+  if (dt != PLANC_F32 && desc->A16T) {
+    AopT = desc->A16T;
+  } else if (dt != PLANC_F32) {
     void *at = ws_get(ctx, 16, sizeof(uint16_t) * (size_t)m * n);
     if (!at) return set_error(ctx, PLANC_ENOMEM, "planc_nmf_run: transposed 16-bit copy");
     int rc = transpose16(ctx, Aop, m, n, ldop, at, n);
     if (rc) return rc;
     AopT = at;
   }
-  // A*H (m x k, ld m).
-  auto a_times_h = [&](float *out) -> int {
-    if (AopT) return thin_gemm(ctx, dt, 1, AopT, n, m, n, d_H, n, nullptr, cs, nullptr, k, out, m, 0, 0);
-    return thin_gemm(ctx, dt, 0, Aop, ldop, m, n, d_H, n, nullptr, cs, nullptr, k, out, m, 0, 0);
+  // A*F (m x k, ld m) for F n x k.
+  auto a_times_h = [&](const float *F, float *out) -> int {
+    if (AopT) return thin_gemm(ctx, dt, 1, AopT, n, m, n, F, n, nullptr, cs, nullptr, k, out, m, 0, 0);
+    return thin_gemm(ctx, dt, 0, Aop, ldop, m, n, F, n, nullptr, cs, nullptr, k, out, m, 0, 0);
   };
-  // Loop buffers.
-  float *fbuf = (float *)ws_get(ctx, 4, sizeof(float) * ((size_t)k * n + (size_t)m * k));
-  const size_t kk = (size_t)k * k;
-  double *dbuf = (double *)ws_get(ctx, 5, sizeof(double) * (4 * kk + 8 + (size_t)iters));
-  if (!fbuf || !dbuf) return set_error(ctx, PLANC_ENOMEM, "planc_nmf_run: workspace");
-  float *WtA = fbuf;                // k x n
-  float *AH = fbuf + (size_t)k * n;  // m x k
-  double *WtW = dbuf, *HtH = dbuf + kk, *Greg = dbuf + 2 * kk, *WtW2 = dbuf + 3 * kk;
-  double *normA2 = dbuf + 4 * kk, *dotv = normA2 + 1, *nrm = normA2 + 2;  // nrm unused slot
-  double *rel = normA2 + 8;
-  (void)nrm;
-  int rc;
-#define RC(x)          \
-  do {                 \
-    rc = (x);          \
-    if (rc) return rc; \
-  } while (0)
-  if (desc->compute_error) RC(dot(ctx, d_A, 1, lda, d_A, 1, lda, m, n, normA2, 0));
-  if (algo == PLANC_ANLSBPP) {
-    // BPPNMF::computeNMF (nmf/bppnmf.hpp:187-241): W given H first, then H
-    // given W, each a batched NNLS on the FP64 normal equations.
-    int *dflag = (int *)ws_get(ctx, 11, sizeof(int));
-    if (!dflag) return set_error(ctx, PLANC_ENOMEM, "planc_nmf_run: workspace");
-    PLANC_HIP_CHECK(ctx, hipMemsetAsync(dflag, 0, sizeof(int), ctx->stream));
-    for (int it = 0; it < iters; ++it) {
-      RC(gram(ctx, d_H, n, k, n, HtH));
-      PLANC_HIP_CHECK(ctx, hipMemcpyAsync(Greg, HtH, kk * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
-      RC(apply_reg(ctx, Greg, k, desc->regW));
-      RC(a_times_h(AH));
-      RC(bpp_solve(ctx, Greg, k, AH, 1, m, m, d_W, 1, m, dflag));
-      RC(gram(ctx, d_W, m, k, m, WtW));
-      PLANC_HIP_CHECK(ctx, hipMemcpyAsync(Greg, WtW, kk * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
-      RC(apply_reg(ctx, Greg, k, desc->regH));
-      RC(thin_gemm(ctx, dt, 1, Aop, ldop, n, m, d_W, m, nullptr, nullptr, cs, k, WtA, k, 1, 0));
-      RC(bpp_solve(ctx, Greg, k, WtA, k, 1, n, d_H, 1, n, dflag));
-      if (desc->compute_error) {
-        // tr(H^T A^T W) = <H, (W^T A)^T> with the W^T A of the H half-step.
-        RC(dot(ctx, d_H, 1, n, WtA, k, 1, n, k, dotv, 0));
-        RC(gram(ctx, d_H, n, k, n, HtH));
-        RC(nmf_relerr(ctx, normA2, dotv, WtW, HtH, k, rel + it, nullptr));
-      }
-    }
-    RC(planc_normalize_by_w(ctx, d_W, m, m, d_H, n, n, k));
-    int hflag = 0;
-    PLANC_HIP_CHECK(ctx, hipMemcpyAsync(&hflag, dflag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-    if (h_relerr && desc->compute_error && iters > 0)
-      PLANC_HIP_CHECK(ctx, hipMemcpyAsync(h_relerr, rel, sizeof(double) * iters, hipMemcpyDeviceToHost, ctx->stream));
-    PLANC_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    if (hflag) return set_error(ctx, PLANC_ENOCONV, "BPP failed: more than 5k pivot iterations");
-    return PLANC_OK;
+  NMFProducts prod;
+  prod.wta = [&](const float *F, float *out) -> int {
+    return thin_gemm(ctx, dt, 1, Aop, ldop, n, m, F, m, nullptr, nullptr, cs, k, out, k, 1, 0);
+  };
+  prod.ah = a_times_h;
+  if (dt != PLANC_F32 && algo == PLANC_ANLSBPP) {
+    // ANLS-BPP solves normal equations whose Gram comes from the unrounded
+    // factor; a right-hand side from the 16-bit-rounded factor would make them
+    // inconsistent, an error that the solve amplifies by cond(Gram).  The
+    // factor is therefore split into a 16-bit part and its remainder and both
+    // products are summed: only the rounding of A remains.
+    float *split = (float *)ws_get(ctx, 20, sizeof(float) * (2 * (size_t)std::max(m, n) * k +
+                                                            (size_t)std::max((long)k * n, m * (long)k)));
+    if (!split) return set_error(ctx, PLANC_ENOMEM, "planc_nmf_run: split factor");
+    const size_t fs = (size_t)std::max(m, n) * k;
+    float *hi = split, *lo = split + fs, *tmp = split + 2 * fs;
+    prod.wta = [&, hi, lo, tmp](const float *F, float *out) -> int {
+      int r = split16(ctx, dt, F, m * k, hi, lo);
+      if (!r) r = thin_gemm(ctx, dt, 1, Aop, ldop, n, m, hi, m, nullptr, nullptr, cs, k, out, k, 1, 0);
+      if (!r) r = thin_gemm(ctx, dt, 1, Aop, ldop, n, m, lo, m, nullptr, nullptr, cs, k, tmp, k, 1, 0);
+      if (!r) r = add_inplace(ctx, out, tmp, (long)k * n);
+      return r;
+    };
+    prod.ah = [&, hi, lo, tmp](const float *F, float *out) -> int {
+      int r = split16(ctx, dt, F, n * k, hi, lo);
+      const void *Aa = AopT ? AopT : Aop;
+      const int tr = AopT ? 1 : 0;
+      const long la = AopT ? n : ldop;
+      if (!r) r = thin_gemm(ctx, dt, tr, Aa, la, m, n, hi, n, nullptr, cs, nullptr, k, out, m, 0, 0);
+      if (!r) r = thin_gemm(ctx, dt, tr, Aa, la, m, n, lo, n, nullptr, cs, nullptr, k, tmp, m, 0, 0);
+      if (!r) r = add_inplace(ctx, out, tmp, m * (long)k);
+      return r;
+    };
   }
-  double *wnorms = (double *)ws_get(ctx, 9, sizeof(double) * k);
-  if (!wnorms) return set_error(ctx, PLANC_ENOMEM, "planc_nmf_run: workspace");
-  if (algo == PLANC_HALS) {
-    RC(hals_sync_reset(ctx));
-    RC(planc_normalize_by_w(ctx, d_W, m, m, d_H, n, n, k));
-  }
-  const bool regW_on = desc->regW[0] > 0 || desc->regW[1] > 0;
-  const bool regH_on = desc->regH[0] > 0 || desc->regH[1] > 0;
-  // W^T W of the current W: computed once per iteration (after the W
-  // half-step) and carried into the next H half-step -- rescaled by the
-  // normalisation for HALS -- instead of a second pass over W.
-  bool wtw_valid = false;
-  for (int it = 0; it < iters; ++it) {
-    // H half-step: W^T A, W^T W (+ regH).
-    RC(thin_gemm(ctx, dt, 1, Aop, ldop, n, m, d_W, m, nullptr, nullptr, cs, k, WtA, k, 1, 0));
-    if (!wtw_valid) RC(gram(ctx, d_W, m, k, m, WtW));
-    const double *Gh = WtW;
-    if (regH_on) {
-      PLANC_HIP_CHECK(ctx, hipMemcpyAsync(Greg, WtW, kk * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
-      RC(apply_reg(ctx, Greg, k, desc->regH));
-      Gh = Greg;
-    }
-    RC(row_update(ctx, algo == PLANC_MU ? 0 : 1, d_H, n, k, n, WtA, k, 1, Gh, 1e-17f));
-    // W half-step: A H, H^T H (+ regW).
-    RC(a_times_h(AH));
-    RC(gram(ctx, d_H, n, k, n, HtH));
-    const double *Gw = HtH;
-    if (regW_on) {
-      PLANC_HIP_CHECK(ctx, hipMemcpyAsync(Greg, HtH, kk * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
-      RC(apply_reg(ctx, Greg, k, desc->regW));
-      Gw = Greg;
-    }
-    if (algo == PLANC_MU)
-      RC(row_update(ctx, 0, d_W, m, k, m, AH, 1, m, Gw, 1e-17f));
-    else
-      RC(hals_cols(ctx, d_W, m, k, m, AH, m, Gw));
-    RC(gram(ctx, d_W, m, k, m, WtW2));
-    if (desc->compute_error) {
-      // <W, A H> with the H of this iteration equals tr(H^T A^T W) of
-      // computeObjectiveError (common/nmf.hpp:391-396) without the third
-      // product; both traces are invariant under normalize_by_W.
-      RC(dot(ctx, d_W, 1, m, AH, 1, m, m, k, dotv, 0));
-      RC(nmf_relerr(ctx, normA2, dotv, WtW2, HtH, k, rel + it, nullptr));
-    }
-    if (algo == PLANC_HALS) {
-      // normalize_by_W: the column norms are the square roots of diag(W^T W);
-      // the next W^T W is D^-1 (W^T W) D^-1.
-      RC(gram_norms(ctx, WtW2, k, wnorms));
-      RC(scale_cols(ctx, d_W, m, m, d_H, n, n, k, wnorms));
-      RC(gram_rescale(ctx, WtW2, k, wnorms, WtW));
-    } else {
-      PLANC_HIP_CHECK(ctx, hipMemcpyAsync(WtW, WtW2, kk * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
-    }
-    wtw_valid = true;
-  }
-  if (algo == PLANC_MU) RC(planc_normalize_by_w(ctx, d_W, m, m, d_H, n, n, k));
-#undef RC
-  if (h_relerr && desc->compute_error && iters > 0)
-    PLANC_HIP_CHECK(ctx, hipMemcpyAsync(h_relerr, rel, sizeof(double) * iters, hipMemcpyDeviceToHost, ctx->stream));
+  prod.norma2 = [&](double *out) -> int { return dot(ctx, d_A, 1, lda, d_A, 1, lda, m, n, out, 0); };
+  return nmf_loop(ctx, desc, prod, d_W, d_H, h_relerr);
+}
+
+// ---- sparse input -----------------------------------------------------------
+namespace {
+int read_nnz(planc_ctx *ctx, const long *d_indptr, long m, long *nnz) {
+  PLANC_HIP_CHECK(ctx, hipMemcpyAsync(nnz, d_indptr + m, sizeof(long), hipMemcpyDeviceToHost, ctx->stream));
   PLANC_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-  if (algo == PLANC_HALS) return hals_sync_check(ctx);
+  if (*nnz < 0) return set_error(ctx, PLANC_EINVAL, "sparse: negative entry count");
   return PLANC_OK;
+}
+}  // namespace
+
+int planc_csr_transpose(planc_ctx *ctx, long m, long n, const long *d_indptr, const int *d_indices,
+                        const float *d_values, long *d_out_indptr, int *d_out_indices, float *d_out_values) {
+  if (check_ctx(ctx)) return PLANC_EINVAL;
+  if (m < 0 || n < 0 || !d_indptr || !d_out_indptr)
+    return set_error(ctx, PLANC_EINVAL, "planc_csr_transpose: bad arguments");
+  long nnz = 0;
+  int rc = read_nnz(ctx, d_indptr, m, &nnz);
+  if (rc) return rc;
+  if (nnz > 0 && (!d_indices || !d_values || !d_out_indices || !d_out_values))
+    return set_error(ctx, PLANC_EINVAL, "planc_csr_transpose: null entry arrays");
+  if ((rc = csr_transpose(ctx, m, n, nnz, d_indptr, d_indices, d_values, d_out_indptr, d_out_indices,
+                          d_out_values)))
+    return rc;
+  PLANC_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return PLANC_OK;
+}
+
+int planc_spmm(planc_ctx *ctx, long rows, long cols, const long *d_indptr, const int *d_indices,
+               const float *d_values, const float *d_F, long ldf, int k, float *d_out, long ldo) {
+  if (check_ctx(ctx)) return PLANC_EINVAL;
+  if (rows < 0 || cols < 0 || k <= 0 || k > 64 || ldf < cols || ldo < rows || !d_indptr || !d_F || !d_out)
+    return set_error(ctx, PLANC_EINVAL, "planc_spmm: bad arguments (k must be 1..64)");
+  return spmm_colmajor(ctx, rows, cols, d_indptr, d_indices, d_values, d_F, ldf, k, d_out, ldo);
+}
+
+int planc_sparse_fit_error(planc_ctx *ctx, long m, long n, const long *d_indptr, const int *d_indices,
+                           const float *d_values, const float *d_W, long ldw, const float *d_H, long ldh,
+                           int k, double *h_err) {
+  if (check_ctx(ctx)) return PLANC_EINVAL;
+  if (m <= 0 || n <= 0 || k <= 0 || k > 64 || ldw < m || ldh < n || !d_indptr || !d_W || !d_H || !h_err)
+    return set_error(ctx, PLANC_EINVAL, "planc_sparse_fit_error: bad arguments");
+  double *d = (double *)ws_get(ctx, 12, sizeof(double));
+  if (!d) return set_error(ctx, PLANC_ENOMEM, "planc_sparse_fit_error: workspace");
+  int rc = sparse_fit_error(ctx, m, n, d_indptr, d_indices, d_values, d_W, ldw, d_H, ldh, k, d);
+  if (rc) return rc;
+  PLANC_HIP_CHECK(ctx, hipMemcpyAsync(h_err, d, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  PLANC_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return PLANC_OK;
+}
+
+int planc_nmf_sparse_run(planc_ctx *ctx, const planc_nmf_desc *desc, const long *d_indptr,
+                         const int *d_indices, const float *d_values, float *d_W, float *d_H,
+                         double *h_relerr) {
+  if (check_ctx(ctx) || !desc) return PLANC_EINVAL;
+  const long m = desc->m, n = desc->n;
+  const int k = desc->k, iters = desc->iters, algo = desc->algo;
+  if (m <= 0 || n <= 0 || k <= 0 || k > 64 || iters < 0 || !d_indptr || !d_W || !d_H)
+    return set_error(ctx, PLANC_EINVAL, "planc_nmf_sparse_run: bad arguments (k must be 1..64)");
+  if (algo != PLANC_MU && algo != PLANC_HALS && algo != PLANC_ANLSBPP && algo != PLANC_AOADMM &&
+      algo != PLANC_GNSYM)
+    return set_error(ctx, PLANC_EUNSUPPORTED, "planc_nmf_sparse_run: algorithm not supported");
+  if (desc->dtype != PLANC_F32)
+    return set_error(ctx, PLANC_EUNSUPPORTED, "planc_nmf_sparse_run: sparse products are FP32 only");
+  PLANC_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  long nnz = 0;
+  int rc = read_nnz(ctx, d_indptr, m, &nnz);
+  if (rc) return rc;
+  if (nnz > 0 && (!d_indices || !d_values))
+    return set_error(ctx, PLANC_EINVAL, "planc_nmf_sparse_run: null entry arrays");
+  SparseOps ops;
+  if ((rc = sparse_products(ctx, m, n, nnz, d_indptr, d_indices, d_values, &ops))) return rc;
+  NMFProducts prod;
+  prod.wta = [&](const float *F, float *out) -> int { return sparse_wta(ctx, ops, m, n, F, k, out); };
+  prod.ah = [&](const float *F, float *out) -> int {
+    return sparse_ah(ctx, ops, m, n, d_indptr, d_indices, d_values, F, k, out);
+  };
+  prod.norma2 = [&](double *out) -> int {
+    if (nnz == 0) {
+      PLANC_HIP_CHECK(ctx, hipMemsetAsync(out, 0, sizeof(double), ctx->stream));
+      return PLANC_OK;
+    }
+    return dot(ctx, d_values, 1, nnz, d_values, 1, nnz, nnz, 1, out, 0);
+  };
+  prod.bpp_prenormalize = true;
+  return nmf_loop(ctx, desc, prod, d_W, d_H, h_relerr);
 }
 
 int planc_ntf_run(planc_ctx *ctx, const planc_ntf_desc *desc, const float *d_X, const void *d_X16,
@@ -569,7 +803,7 @@ int planc_ntf_run(planc_ctx *ctx, const planc_ntf_desc *desc, const float *d_X, 
   if (N < 2 || N > 7 || k <= 0 || k > 64 || iters < 0 || !d_X || !d_factors || !d_lambda ||
       !valid_dtype(dt))
     return set_error(ctx, PLANC_EINVAL, "planc_ntf_run: bad arguments (2..7 modes, k 1..64)");
-  if (algo != PLANC_MU && algo != PLANC_HALS && algo != PLANC_ANLSBPP)
+  if (algo != PLANC_MU && algo != PLANC_HALS && algo != PLANC_ANLSBPP && algo != PLANC_AOADMM)
     return set_error(ctx, PLANC_EUNSUPPORTED, "planc_ntf_run: algorithm not supported");
   PLANC_HIP_CHECK(ctx, hipSetDevice(ctx->device));
   const long *dims = desc->dims;
@@ -641,6 +875,22 @@ int planc_ntf_run(planc_ctx *ctx, const planc_ntf_desc *desc, const float *d_X, 
   int *dflag = (int *)ws_get(ctx, 11, sizeof(int));
   if (!dflag) return set_error(ctx, PLANC_ENOMEM, "planc_ntf_run: workspace");
   PLANC_HIP_CHECK(ctx, hipMemsetAsync(dflag, 0, sizeof(int), ctx->stream));
+  // NTFAOADMM's auxiliary (dual) factors, zero at construction
+  // (ntf/ntfaoadmm.hpp:84-86), one I_n x k block per mode, and a norm scratch.
+  std::vector<float *> duals(N, nullptr);
+  double *dnorm = nullptr;
+  if (algo == PLANC_AOADMM) {
+    long tot = 0;
+    for (int i = 0; i < N; ++i) tot += dims[i] * k;
+    float *db = (float *)ws_get(ctx, 32, sizeof(float) * (size_t)tot);
+    dnorm = (double *)ws_get(ctx, 34, sizeof(double) * k);
+    if (!db || !dnorm) return set_error(ctx, PLANC_ENOMEM, "planc_ntf_run: AO-ADMM duals");
+    PLANC_HIP_CHECK(ctx, hipMemsetAsync(db, 0, sizeof(float) * (size_t)tot, ctx->stream));
+    for (int i = 0; i < N; ++i) {
+      duals[i] = db;
+      db += dims[i] * k;
+    }
+  }
   if (desc->compute_error) RC(dot(ctx, d_X, 1, numel, d_X, 1, numel, numel, 1, normX2, 0));
   // NCPFactors::normalize() of the initial factors (ntf/auntf.hpp:88).
   for (int i = 0; i < N; ++i) {
@@ -665,7 +915,7 @@ int planc_ntf_run(planc_ctx *ctx, const planc_ntf_desc *desc, const float *d_X, 
     int rc;
     bool err_done = false;
     for (int n = 0; n < N; ++n) {
-      const bool fused = mode_update_fits(dims[n], k);
+      const bool fused = mode_update_fits(dims[n], k) && algo != PLANC_AOADMM;
       if (!fused || algo == PLANC_ANLSBPP) RC(hadamard_grams(ctx, Gc.data(), N, n, k, S));
       // This mode's MTTKRP: element (x, i) at Mp[i * mrs + x * mcs].
       const float *Mp = M;
@@ -721,12 +971,20 @@ int planc_ntf_run(planc_ctx *ctx, const planc_ntf_desc *desc, const float *d_X, 
         err_done = err_done || last_err;
         continue;
       }
-      if (algo == PLANC_HALS)
+      if (algo == PLANC_HALS) {
         RC(row_update(ctx, 1, Fn, dims[n], k, dims[n], M, k, 1, S, 0.0f));
-      else if (algo == PLANC_MU)
+      } else if (algo == PLANC_MU) {
         RC(row_update(ctx, 0, Fn, dims[n], k, dims[n], M, k, 1, S, 1e-6f));
-      else
+      } else if (algo == PLANC_AOADMM) {
+        // NTFAOADMM::update (ntf/ntfaoadmm.hpp:23-77): ADMM solve on S and
+        // the MTTKRP with this mode's dual, then the dual's columns to unit
+        // norm (m_ncp_aux.normalize(mode)).
+        RC(admm_solve(ctx, S, k, M, k, 1, Fn, dims[n], duals[n], dims[n], dims[n], dflag));
+        RC(col_norms(ctx, duals[n], dims[n], k, dims[n], dnorm));
+        RC(scale_cols(ctx, duals[n], dims[n], dims[n], nullptr, 0, 0, k, dnorm));
+      } else {
         RC(bpp_solve(ctx, S, k, M, k, 1, dims[n], Fn, 1, dims[n], dflag));
+      }
       // update_factor_mode: normalize(mode) replaces lambda (ntf/auntf.hpp:66-79).
       RC(col_norms(ctx, Fn, dims[n], k, dims[n], d_lambda));
       RC(scale_cols(ctx, Fn, dims[n], dims[n], nullptr, 0, 0, k, d_lambda));
@@ -787,6 +1045,7 @@ int planc_ntf_run(planc_ctx *ctx, const planc_ntf_desc *desc, const float *d_X, 
     PLANC_HIP_CHECK(ctx, hipMemcpyAsync(h_relerr, rel, sizeof(double) * iters, hipMemcpyDeviceToHost, ctx->stream));
   PLANC_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   if (hflag & 2) return set_error(ctx, PLANC_EHIP, "NTF mode update: block exchange timed out");
+  if (hflag & 4) return set_error(ctx, PLANC_ENOCONV, "AO-ADMM: shifted Gram not positive definite");
   if (hflag) return set_error(ctx, PLANC_ENOCONV, "BPP failed: more than 5k pivot iterations");
   return PLANC_OK;
 }

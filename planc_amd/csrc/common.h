@@ -96,7 +96,7 @@ struct planc_ctx {
   hipStream_t stream = nullptr;
   bool own_stream = false;
   std::string last_error;
-  planc::Workspace ws[24];
+  planc::Workspace ws[40];
   int num_cu = 256;
   int max_coop_blocks_hals = 0;
   // Tuned streaming-product geometries: shape key -> {cw, wk, nsplit, dual, shared}.
@@ -112,7 +112,11 @@ namespace planc {
 // 10 BPP per-column iteration counts, 11 BPP failure flag, 12 CP-loss scalar,
 // 13 CP-gradient float buffers, 14 CP-gradient Grams, 15 loop-graph counter,
 // 16 transposed 16-bit copy of A / X, 17 HALS grid-barrier counter and flag,
-// 18 fused NTF mode-update partials.
+// 18 fused NTF mode-update partials, 19/21 NTF mode-exchange slots and
+// control, 20 split 16-bit factors of ANLS-BPP, 22 AO-ADMM factor / partials,
+// 23 GNSym CG vectors, 24..31 sparse path (sparse.hip: k-major factor
+// copies, error partials, transposed CSR, sort keys / entries / temporary
+// storage), 32 NTF AO-ADMM duals, 33 NMF AO-ADMM duals, 34 dual norms.
 void *ws_get(planc_ctx *ctx, int slot, size_t bytes);
 
 int set_error(planc_ctx *ctx, int code, const std::string &msg);
@@ -194,6 +198,12 @@ int dot(planc_ctx *ctx, const float *X, long xrs, long xcs, const float *Y, long
 int nmf_relerr(planc_ctx *ctx, const double *normA2, const double *dotv, const double *WtW,
                const double *HtH, int k, double *relerr_out, double *fit_out);
 int apply_reg(planc_ctx *ctx, double *G, int k, const double *reg);
+// hi = X rounded to dtype's 16-bit format (as float), lo = X - hi; Y += X.
+int split16(planc_ctx *ctx, int dtype, const float *X, long n, float *hi, float *lo);
+int add_inplace(planc_ctx *ctx, float *Y, const float *X, long n);
+// G += sr I, R(j, c) += sr F(j, c) (R element (j, c) at R[j*rs + c*cs]); no-op for sr <= 0.
+int symm_reg_apply(planc_ctx *ctx, double *G, int k, float *R, long rs, long cs, const float *F,
+                   long n, double sr);
 // Time-out flag of the HALS column sweep's grid barrier: reset before a
 // run, checked (synchronising) after it.
 int hals_sync_reset(planc_ctx *ctx);
@@ -215,6 +225,37 @@ int bpp_solve(planc_ctx *ctx, const double *G, int k, const float *B, long b_cs,
 // cploss.hip
 int cp_loss_forward(planc_ctx *ctx, const float *X, int N, const long *dims, const float *const *F,
                     int k, double *d_loss);
+// Sparse (CSR) products of the sparse-input NMF (sparse.hip).  SparseOps
+// holds the CSR of A^T built once per run.
+struct SparseOps {
+  const long *t_indptr = nullptr;
+  const int *t_indices = nullptr;
+  const float *t_values = nullptr;
+};
+int csr_transpose(planc_ctx *ctx, long m, long n, long nnz, const long *indptr, const int *indices,
+                  const float *values, long *out_indptr, int *out_indices, float *out_values);
+int spmm_csr(planc_ctx *ctx, long rows, const long *indptr, const int *indices, const float *values,
+             const float *F, long ldf, int k, float *out, long ors, long ocs);
+int spmm_colmajor(planc_ctx *ctx, long rows, long cols, const long *indptr, const int *indices,
+                  const float *values, const float *F, long ldf, int k, float *out, long ldo);
+int sparse_fit_error(planc_ctx *ctx, long m, long n, const long *indptr, const int *indices,
+                     const float *values, const float *W, long ldw, const float *H, long ldh, int k,
+                     double *d_err);
+int sparse_products(planc_ctx *ctx, long m, long n, long nnz, const long *indptr, const int *indices,
+                    const float *values, SparseOps *ops);
+int sparse_wta(planc_ctx *ctx, const SparseOps &o, long m, long n, const float *W, int k, float *out);
+int sparse_ah(planc_ctx *ctx, const SparseOps &o, long m, long n, const long *indptr, const int *indices,
+              const float *values, const float *H, int k, float *out);
+
+// AO-ADMM solve of X (n x k) and its dual D given the regularised Gram G
+// and R(c, j) at R[j*r_js + c*r_cs]; *err |= 4 when G + rho I is not
+// positive definite (admm_gn.hip).
+int admm_solve(planc_ctx *ctx, const double *G, int k, const float *R, long r_js, long r_cs, float *X,
+               long ldx, float *D, long ldd, long n, int *err);
+// One symmetric-NMF Gauss-Newton step of H (n x k) from AHt = H^T A (k x n)
+// and HtH of the current H.
+int gnsym_step(planc_ctx *ctx, float *H, long n, int k, const float *AHt, const double *HtH);
+
 // Allocation-free forward: block partials in a per-device static array
 // located once per device by cp_noalloc_setup (called from planc_create).
 int cp_noalloc_setup(int device);

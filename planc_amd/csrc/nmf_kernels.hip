@@ -527,6 +527,42 @@ __global__ void apply_reg_kernel(double *G, int k, float l2, float l1) {
   G[e] = v;
 }
 
+// hi = X rounded to the 16-bit type (as a float), lo = X - hi (exact in
+// FP32): a thin operand split so that a 16-bit product with hi plus one with
+// lo carries the thin operand to about 2^-17 relative (only A's rounding
+// remains).
+template <int DT>
+__global__ void split16_kernel(const float *X, long n, float *hi, float *lo) {
+  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const float x = X[e];
+  const float h = DT == PLANC_BF16 ? bf16_to_float(float_to_bf16(x)) : f16_to_float(float_to_f16(x));
+  hi[e] = h;
+  lo[e] = x - h;
+}
+
+__global__ void add_inplace_kernel(float *Y, const float *X, long n) {
+  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) Y[e] += X[e];
+}
+
+// Symmetric regularisation (applySymmetricReg, common/nmf.hpp:123-129):
+// G += sr * I and R(j, c) += sr * F(j, c), R's element (j, c) at
+// R[j * rs + c * cs], F n x k column-major (ld n).
+__global__ void add_diag_kernel(double *G, int k, double v) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < k) G[c + (long)k * c] += v;
+}
+
+__global__ void symm_rhs_kernel(float *R, long rs, long cs, const float *F, long n, int k, double sr) {
+  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n * k) return;
+  long j = e % n;
+  long c = e / n;
+  float *r = R + j * rs + c * cs;
+  *r = (float)((double)*r + sr * (double)F[e]);
+}
+
 // ---------------------------------------------------------------------------
 // 16-bit copies: per-column power-of-two scale then round.
 __global__ __launch_bounds__(NTHR) void colmax_scale_kernel(const float *A, long rows, long lda,
@@ -871,6 +907,37 @@ int apply_reg(planc_ctx *ctx, double *G, int k, const double *reg) {
   return PLANC_OK;
 }
 
+int split16(planc_ctx *ctx, int dtype, const float *X, long n, float *hi, float *lo) {
+  if (n <= 0) return PLANC_OK;
+  const unsigned nb = (unsigned)((n + 255) / 256);
+  if (dtype == PLANC_BF16)
+    split16_kernel<PLANC_BF16><<<nb, 256, 0, ctx->stream>>>(X, n, hi, lo);
+  else
+    split16_kernel<PLANC_F16><<<nb, 256, 0, ctx->stream>>>(X, n, hi, lo);
+  PLANC_LAUNCH_CHECK(ctx);
+  return PLANC_OK;
+}
+
+int add_inplace(planc_ctx *ctx, float *Y, const float *X, long n) {
+  if (n <= 0) return PLANC_OK;
+  add_inplace_kernel<<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(Y, X, n);
+  PLANC_LAUNCH_CHECK(ctx);
+  return PLANC_OK;
+}
+
+int symm_reg_apply(planc_ctx *ctx, double *G, int k, float *R, long rs, long cs, const float *F,
+                   long n, double sr) {
+  if (!(sr > 0)) return PLANC_OK;
+  add_diag_kernel<<<(k + 63) / 64, 64, 0, ctx->stream>>>(G, k, sr);
+  PLANC_LAUNCH_CHECK(ctx);
+  long tot = n * k;
+  if (tot > 0) {
+    symm_rhs_kernel<<<(unsigned)((tot + 255) / 256), 256, 0, ctx->stream>>>(R, rs, cs, F, n, k, sr);
+    PLANC_LAUNCH_CHECK(ctx);
+  }
+  return PLANC_OK;
+}
+
 int convert16(planc_ctx *ctx, int dtype, const float *A, long rows, long cols, long lda, void *out,
               long ldo, float *colscale) {
   if (colscale && cols > 0) {
@@ -893,17 +960,27 @@ int convert16(planc_ctx *ctx, int dtype, const float *A, long rows, long cols, l
 
 int transpose16(planc_ctx *ctx, const void *in, long rows, long cols, long ldi, void *out, long ldo) {
   if (rows <= 0 || cols <= 0) return PLANC_OK;
-  const long gx = (rows + 63) / 64, gy = (cols + 63) / 64;
-  if (gy > 65535) return set_error(ctx, PLANC_EINVAL, "transpose16: too many columns");
+  const long gx = (rows + 63) / 64;
+  if (gx > 0x7fffffffL) return set_error(ctx, PLANC_EINVAL, "transpose16: too many rows");
   const bool vec = rows % 8 == 0 && cols % 8 == 0 && ldi % 8 == 0 && ldo % 8 == 0 &&
                    (uintptr_t)in % 16 == 0 && (uintptr_t)out % 16 == 0;
-  if (vec)
-    transpose16_vec_kernel<<<dim3((unsigned)gx, (unsigned)gy), 256, 0, ctx->stream>>>(
-        (const uint16_t *)in, rows, cols, ldi, (uint16_t *)out, ldo);
-  else
-    transpose16_kernel<<<dim3((unsigned)gx, (unsigned)gy), 256, 0, ctx->stream>>>(
-        (const uint16_t *)in, rows, cols, ldi, (uint16_t *)out, ldo);
-  PLANC_LAUNCH_CHECK(ctx);
+  // gridDim.y is at most 65535: wider matrices go in slabs of 65535 tiles
+  // of 64 columns (slab starts are multiples of 64 columns, so the 16-byte
+  // alignment of the vector form holds in every slab).
+  const long slab = 65535L * 64;
+  for (long c0 = 0; c0 < cols; c0 += slab) {
+    const long cc = cols - c0 < slab ? cols - c0 : slab;
+    const long gy = (cc + 63) / 64;
+    const uint16_t *src = (const uint16_t *)in + ldi * c0;
+    uint16_t *dst = (uint16_t *)out + c0;
+    if (vec)
+      transpose16_vec_kernel<<<dim3((unsigned)gx, (unsigned)gy), 256, 0, ctx->stream>>>(src, rows, cc,
+                                                                                       ldi, dst, ldo);
+    else
+      transpose16_kernel<<<dim3((unsigned)gx, (unsigned)gy), 256, 0, ctx->stream>>>(src, rows, cc, ldi,
+                                                                                  dst, ldo);
+    PLANC_LAUNCH_CHECK(ctx);
+  }
   return PLANC_OK;
 }
 

@@ -60,6 +60,7 @@ class NMF {
     d.regH[0] = m_regH[0];
     d.regH[1] = m_regH[1];
     d.compute_error = 1;  // always on in the shared-memory driver (nmf/nmf.cpp:205)
+    d.symm_reg = m_symm_reg;
     m_relerr.assign(m_num_iterations, 0.0);
     dev.check(planc_nmf_run(ctx, &d, dA.get(), m, nullptr, nullptr, dW.get(), dH.get(),
                             m_relerr.data()),

@@ -1,0 +1,115 @@
+"""GPU tests of the further update rules against the FP64 oracle, through the
+C ABI: AO-ADMM NMF (nmf/aoadmm.hpp) and NTF (ntf/ntfaoadmm.hpp), symmetric
+Gauss-Newton NMF (nmf/gnsym.hpp).  Trajectories are held to 1e-4 (the ADMM
+and CG stopping tests compare norms, so an FP32 rounding can end a sweep
+loop one step earlier or later than in FP64; the tests print both)."""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+import planc_amd as P  # noqa: E402
+from oracle import oracle as O  # noqa: E402
+from planc_amd import datagen as D  # noqa: E402
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def rel_err(x, ref):
+    return np.linalg.norm(np.asarray(x, np.float64) - ref) / max(np.linalg.norm(ref), 1e-300)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    c = P.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.mark.parametrize("m,n,k,reg", [(500, 300, 10, False), (2000, 1000, 20, False),
+                                       (400, 350, 16, True), (3000, 64, 64, False)])
+def test_aoadmm_nmf_matches_oracle(ctx, m, n, k, reg):
+    A = D.lowrank_matrix(m, n, k, 0.01, stream=51).astype(np.float32)
+    W0, H0 = D.init_factors([(m, k), (n, k)], stream=51)
+    W0, H0 = W0.astype(np.float32), H0.astype(np.float32)
+    regW, regH = ((0.5, 0.1), (0.25, 0.0)) if reg else ((0.0, 0.0), (0.0, 0.0))
+    Wr, Hr, rel_r, _, rc = O.nmf(4, A.astype(np.float64), W0.astype(np.float64), H0.astype(np.float64), 10,
+                                 regW=regW, regH=regH)
+    assert rc == 0
+    dW, dH = P.colmajor(W0), P.colmajor(H0)
+    rel = ctx.nmf_run(P.colmajor(A), dW, dH, P.AOADMM, 10, regW=regW, regH=regH)
+    say(f"  aoadmm {m}x{n} k={k}: gpu {np.array2string(rel, precision=8)}")
+    say(f"                  oracle {np.array2string(rel_r, precision=8)}")
+    assert np.abs(rel - rel_r).max() < 1e-4
+    assert rel_err(dW.cpu().numpy(), Wr) < 1e-3
+    assert rel_err(dH.cpu().numpy(), Hr) < 1e-3
+
+
+@pytest.mark.parametrize("dims,k", [((20, 18, 16), 5), ((40, 36, 32), 8), ((12, 10, 9, 8), 4)])
+@pytest.mark.parametrize("dimtree", [True, False])
+def test_aoadmm_ntf_matches_oracle(ctx, dims, k, dimtree):
+    X, _ = D.cp_tensor(dims, k, 0.01, stream=52)
+    X32 = X.astype(np.float32)
+    F0 = [f.astype(np.float32) for f in D.init_factors([(d, k) for d in dims], stream=52)]
+    Fr, lam_r, rel_r, rc = O.ntf(4, X32.astype(np.float64), [f.astype(np.float64) for f in F0], 10,
+                                 dimtree=1 if dimtree else 0)
+    assert rc == 0
+    dF = [P.colmajor(f) for f in F0]
+    lam, rel = ctx.ntf_run(P.colmajor(X32), dF, P.AOADMM, 10, dimtree=dimtree)
+    say(f"  ntf aoadmm {dims}: gpu {rel[-1]:.8f} oracle {rel_r[-1]:.8f} max diff {np.abs(rel - rel_r).max():.2e}")
+    assert np.abs(rel - rel_r).max() < 1e-4
+    for f, fr in zip(dF, Fr):
+        assert rel_err(f.cpu().numpy(), fr) < 1e-3
+
+
+@pytest.mark.parametrize("n,k", [(300, 8), (2000, 16), (1500, 64)])
+def test_gnsym_matches_oracle(ctx, n, k):
+    B = D.uniform(D.DATA_SEED, 16 * 53 + k, (n, k))
+    A = B @ B.T
+    A = (A + 0.01 * A.mean() * np.abs(D.normal(D.NOISE_SEED, 53, (n, n)))).astype(np.float32)
+    A = 0.5 * (A + A.T)
+    H0 = D.uniform(D.INIT_SEED, 16 * 53, (n, k))
+    H0 = (2 * np.sqrt(A.astype(np.float64).mean() / k) * H0).astype(np.float32)
+    _, Hr, rel_r, _, rc = O.nmf(7, A.astype(np.float64), np.zeros((n, k)), H0.astype(np.float64), 8)
+    assert rc == 0
+    dW, dH = P.colmajor(np.zeros((n, k), np.float32)), P.colmajor(H0)
+    rel = ctx.nmf_run(P.colmajor(A), dW, dH, P.GNSYM, 8)
+    say(f"  gnsym n={n} k={k}: gpu {np.array2string(rel, precision=8)}")
+    say(f"                 oracle {np.array2string(rel_r, precision=8)}")
+    assert np.abs(rel - rel_r).max() < 1e-4
+    assert rel_err(dH.cpu().numpy(), Hr) < 1e-3
+    assert np.all(dW.cpu().numpy() == 0)  # W is not used
+    with pytest.raises(P.PlancError):
+        ctx.nmf_run(P.colmajor(A[:, :n - 1]), dW, P.colmajor(H0[:n - 1]), P.GNSYM, 1)
+
+
+def test_sparse_aoadmm_and_gnsym(ctx):
+    """The sparse-input loop runs the same rules (products from the CSR)."""
+    sp = pytest.importorskip("scipy.sparse")
+    rng = np.random.default_rng(54)
+    m = n = 600
+    A = sp.random(m, n, density=0.05, format="csr", random_state=rng, data_rvs=lambda s: rng.random(s) + 0.1)
+    A = (0.5 * (A + A.T)).tocsr().astype(np.float32)
+    A.sort_indices()
+    ip = torch.from_numpy(A.indptr.astype(np.int64)).cuda()
+    ix = torch.from_numpy(A.indices.astype(np.int32)).cuda()
+    vv = torch.from_numpy(A.data).cuda()
+    W0, H0 = D.init_factors([(m, 6), (n, 6)], stream=54)
+    W0, H0 = W0.astype(np.float32), H0.astype(np.float32)
+    for algo in (P.AOADMM, P.GNSYM):
+        _, _, rel_r, rc = O.nmf_sparse(algo, m, n, A.indptr, A.indices, A.data.astype(np.float64),
+                                       W0.astype(np.float64), H0.astype(np.float64), 6)
+        assert rc == 0
+        dW, dH = P.colmajor(W0), P.colmajor(H0)
+        rel = ctx.nmf_sparse_run(m, n, ip, ix, vv, dW, dH, algo, 6)
+        say(f"  sparse algo={algo}: gpu {rel[-1]:.8f} oracle {rel_r[-1]:.8f}")
+        assert np.abs(rel - rel_r).max() < 1e-4
