@@ -205,6 +205,19 @@ int planc_multi_ttv(planc_ctx *ctx, const float *d_P, long ldp, int nb, const lo
 int planc_bpp(planc_ctx *ctx, const double *d_G, int k, const float *d_B, long b_cs, long b_rs,
               long nrhs, float *d_X, long x_cs, long x_rs);
 
+/* One AO-ADMM solve (the inner loop of AOADMMNMF, nmf/aoadmm.hpp:105-141 /
+ * :157-185, and of NTFAOADMM::update, ntf/ntfaoadmm.hpp:34-72): with
+ * rho = trace(G)/k (0.01 if not positive) and L = chol(G + rho I), at most 5
+ * sweeps of  T = (G + rho I)^-1 (R + rho (X + D)^T),  X = max(T^T - D, 0),
+ * D += X - T^T,  stopping when ||X - T^T|| < 0.01 ||X|| and ||X - X_prev|| <
+ * 0.01 ||D||.  G k x k FP64 (the Gram with its regularisation), R(c, j) at
+ * d_R[j * r_js + c * r_cs] (k x n), X and its dual D n x k (lds ldx, ldd),
+ * updated in place; the dual carries over between calls as the reference's
+ * does.  Synchronises; PLANC_ENOCONV when G + rho I is not positive definite
+ * (the reference's arma::chol fails).  k <= 64. */
+int planc_admm_update(planc_ctx *ctx, const double *d_G, int k, const float *d_R, long r_js, long r_cs,
+                      float *d_X, long ldx, float *d_D, long ldd, long n);
+
 /* CP reconstruction loss 0.5 ||X - [[A_0..A_{N-1}]]||^2 (no lambda), FP64
  * result in *h_loss: one fused kernel reconstructs tiles of X from
  * Khatri-Rao tiles formed in LDS and reduces the squared residual, reading X
@@ -368,7 +381,7 @@ int planc_nmf_sparse_run(planc_ctx *ctx, const planc_nmf_desc *desc, const long 
                          double *h_relerr);
 
 typedef struct planc_ntf_desc {
-  int algo;          /* PLANC_MU, PLANC_HALS, PLANC_ANLSBPP, PLANC_AOADMM */
+  int algo;          /* PLANC_MU, PLANC_HALS, PLANC_ANLSBPP, PLANC_AOADMM, PLANC_NESTEROV */
   int dtype;         /* of the MTTKRP reads of X */
   int nmodes;        /* 2..7 */
   long dims[8];      /* dims[0..nmodes-1] used */
@@ -380,7 +393,12 @@ typedef struct planc_ntf_desc {
 
 /* Runs AUNTF::computeNTF (ntf/auntf.hpp:128-177) with NTFHALS / NTFMU /
  * NTFANLSBPP / NTFAOADMM (ntf/ntfaoadmm.hpp:23-77, duals zero at the start
- * of each call) updates.  d_factors (host array of device pointers, I_i x k)
+ * of each call) / NTFNES (ntf/ntfnes.hpp:9-188: the proximal Nesterov mode
+ * update and, after iteration 5, accelerate(); the error is computed every
+ * iteration as the reference's accelerated(true) turns it on; iterations run
+ * eagerly with one host decision per acceleration; PLANC_ENOCONV if a mode's
+ * Nesterov loop passes 100000 steps, which the reference does not bound)
+ * updates.  d_factors (host array of device pointers, I_i x k)
  * hold the initial factors, which are first normalised as NCPFactors::
  * normalize() does in the AUNTF constructor (ntf/auntf.hpp:88); on return
  * they hold the unit-norm factors and d_lambda (k, FP64) the weights.

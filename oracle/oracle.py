@@ -44,6 +44,13 @@ How each oracle function is pinned (tests/test_oracle.py):
                          Gram); one AO-ADMM NMF iteration against a NumPy
                          restatement; NTF AO-ADMM dimtree == direct and its
                          error against a direct reconstruction.
+* NTF-NES             -- Jacobi eigenvalues against numpy.linalg.eigvalsh; a mode
+                         update against the BPP (pinned) solution of the same
+                         proximal NNLS (S + lambda I) x = m + lambda h*, to the
+                         reference's absolute stopping tolerance; the error
+                         trajectory against a direct reconstruction; dimtree ==
+                         direct; the accelerated run ends below the plain one's
+                         start.
 * GNSym               -- the exact factorisation A = H H^T is a fixed point
                          (zero gradient, no step, zero error); the fast error
                          formula against ||A - H H^T||_F; one iteration against a
@@ -120,6 +127,9 @@ def lib():
         L.oracle_sparse_fit_error.restype = ctypes.c_double
         L.oracle_admm_solve.argtypes = [_dp, ctypes.c_int, _dp, ctypes.c_long, _dp, _dp]
         L.oracle_admm_solve.restype = ctypes.c_int
+        L.oracle_sym_eig_minmax.argtypes = [_dp, ctypes.c_int, _dp, _dp]
+        L.oracle_nes_mode_update.argtypes = [_dp, ctypes.c_int, _dp, ctypes.c_long, _dp, _dp]
+        L.oracle_nes_mode_update.restype = ctypes.c_int
         L.oracle_use_blas.argtypes = [ctypes.c_char_p, ctypes.c_int]
         L.oracle_use_blas.restype = ctypes.c_int
         L.oracle_last_loop_seconds.restype = ctypes.c_double
@@ -240,6 +250,24 @@ def admm_solve(G, R, X, D):
     D = _f64(D).copy(order="F")
     rc = lib().oracle_admm_solve(_p(G), G.shape[0], _p(R), R.shape[1], _p(X), _p(D))
     return X, D, rc
+
+
+def sym_eig_minmax(S):
+    S = _f64(S)
+    mn, mx = ctypes.c_double(), ctypes.c_double()
+    lib().oracle_sym_eig_minmax(_p(S), S.shape[0], ctypes.byref(mn), ctypes.byref(mx))
+    return mn.value, mx.value
+
+
+def nes_mode_update(S, M, Hprox):
+    """NTFNES update of one mode from the proximal centre Hprox (I x k):
+    returns (H, rc)."""
+    S = _f64(S)
+    M = _f64(M)
+    Hp = _f64(Hprox)
+    H = np.zeros_like(Hp, order="F")
+    rc = lib().oracle_nes_mode_update(_p(S), S.shape[0], _p(M), Hp.shape[0], _p(Hp), _p(H))
+    return H, rc
 
 
 def _csr_arrays(indptr, indices, values):

@@ -861,12 +861,184 @@ static int ntf_update(int algo, Mat &H, const Mat &M, const Mat &S, long k, Mat 
   return 0;
 }
 
+// ---- NTFNES (ntf/ntfnes.hpp) ------------------------------------------------
+// Eigenvalues of a symmetric k x k matrix by cyclic Jacobi rotations
+// (arma::eig_sym in the reference, ntf/ntfnes.hpp:130): the smallest and the
+// largest.
+static void sym_eig_minmax(const Mat &S, double &mn, double &mx) {
+  const long k = S.r;
+  std::vector<double> a(S.d);
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0, diag = 0;
+    for (long p = 0; p < k; ++p)
+      for (long q = 0; q < k; ++q) (p == q ? diag : off) += a[p + k * q] * a[p + k * q];
+    if (off <= 1e-30 * diag || off == 0) break;
+    for (long p = 0; p < k - 1; ++p)
+      for (long q = p + 1; q < k; ++q) {
+        const double apq = a[p + k * q];
+        if (apq == 0) continue;
+        const double app = a[p + k * p], aqq = a[q + k * q];
+        const double theta = (aqq - app) / (2 * apq);
+        const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1));
+        const double c = 1 / std::sqrt(t * t + 1), sn = t * c;
+        for (long r = 0; r < k; ++r) {  // columns p, q
+          const double arp = a[r + k * p], arq = a[r + k * q];
+          a[r + k * p] = c * arp - sn * arq;
+          a[r + k * q] = sn * arp + c * arq;
+        }
+        for (long r = 0; r < k; ++r) {  // rows p, q
+          const double apr = a[p + k * r], aqr = a[q + k * r];
+          a[p + k * r] = c * apr - sn * aqr;
+          a[q + k * r] = sn * apr + c * aqr;
+        }
+      }
+  }
+  mn = mx = a[0];
+  for (long p = 1; p < k; ++p) {
+    mn = std::min(mn, a[p + k * p]);
+    mx = std::max(mx, a[p + k * p]);
+  }
+}
+
+// The factors of NTFNES in its transposed NCPFactors (k x I_n), held here as
+// I_n x k; lambda per NCPFactors.
+struct NESState {
+  std::vector<Mat> prox, prev;        // m_prox_t, m_prev_t (zero at construction, :175-176)
+  std::vector<double> prox_l, prev_l;  // their lambdas (ones at construction)
+  double delta1 = 1e-2, delta2 = 1e-2;
+  int acc_start = 5, acc_exp = 3, acc_fails = 0, fail_limit = 5;  // :180-186
+};
+
+// normalize_rows(mode) of a transposed factor (common/ncpfactors.hpp:358-363).
+static void normalize_rows_t(Mat &F, std::vector<double> &lambda) {
+  for (long c = 0; c < F.c; ++c) {
+    double nrm = 0;
+    for (long r = 0; r < F.r; ++r) nrm += F(r, c) * F(r, c);
+    nrm = std::sqrt(nrm);
+    lambda[c] = nrm;
+    if (nrm > 0)
+      for (long r = 0; r < F.r; ++r) F(r, c) /= nrm;
+  }
+}
+
+// NTFNES::update (ntf/ntfnes.hpp:119-166); the new factor (unnormalised)
+// into H.  Returns -4 when the inner loop runs past 100000 steps (the
+// reference has no bound).
+static int nes_update(NESState &ns, int mode, const Mat &S, const Mat &M, long k, Mat &H) {
+  if (mode == 0) ns.prev_l = ns.prox_l;
+  ns.prev[mode] = ns.prox[mode];
+  Mat Ht = ns.prox[mode], Htprev = Ht, acc = Ht;
+  const long I = Ht.r;
+  double L, mu;
+  sym_eig_minmax(S, mu, L);
+  const double q0 = L / mu;
+  const double lambda = q0 > 1e6 ? 10 * mu : (q0 > 1e3 ? mu : mu / 10);  // get_lambda, :27-38
+  const double q = (mu + lambda) / (L + lambda);
+  Mat G = S;
+  for (long c = 0; c < k; ++c) G(c, c) += lambda;
+  Mat mod(I, k);
+  for (long r = 0; r < I; ++r)
+    for (long c = 0; c < k; ++c) mod(r, c) = -lambda * acc(r, c) - M(c, r);
+  double alpha = 1, alpha_prev = 1, beta = 1;
+  Mat grad(I, k);
+  for (long it = 0;; ++it) {
+    double absmax = 0, gmin = 0;
+    bool first = true;
+    for (long r = 0; r < I; ++r)
+      for (long c = 0; c < k; ++c) {
+        double g = mod(r, c);
+        for (long t = 0; t < k; ++t) g += G(c, t) * acc(r, t);
+        grad(r, c) = g;
+        absmax = std::max(absmax, std::fabs(g * acc(r, c)));
+        gmin = first ? g : std::min(gmin, g);
+        first = false;
+      }
+    if (absmax <= ns.delta1 && gmin >= -ns.delta2) break;  // stop_iter, :56-66
+    if (it >= 100000) return -4;
+    Htprev = Ht;
+    for (long e = 0; e < (long)Ht.d.size(); ++e) {
+      const double v = acc.d[e] - grad.d[e] / (L + lambda);
+      Ht.d[e] = v > 0.0 ? v : 0.0;
+    }
+    alpha_prev = alpha;
+    {  // get_alpha, :40-54: root of x^2 + (alpha^2 - q) x - alpha^2 = 0
+      const double b = alpha * alpha - q, c2 = -alpha * alpha;
+      alpha = (-b + std::sqrt(b * b - 4 * c2)) / 2;
+    }
+    beta = (alpha_prev * (1 - alpha_prev)) / (alpha + alpha_prev * alpha_prev);
+    for (long e = 0; e < (long)Ht.d.size(); ++e) acc.d[e] = Ht.d[e] + beta * (Ht.d[e] - Htprev.d[e]);
+  }
+  ns.prox[mode] = Ht;
+  normalize_rows_t(ns.prox[mode], ns.prox_l);
+  H = Ht;
+  return 0;
+}
+
+// reset(new_factors, trans) (ntf/auntf.hpp:186-198): every mode set and
+// normalised (normalize(mode) replacing lambda), then lambda taken over.
+static void ntf_reset(NTFState &s, const std::vector<Mat> &F, const std::vector<double> &lambda) {
+  for (size_t j = 0; j < F.size(); ++j) {
+    s.F[j] = F[j];
+    ncp_normalize_mode(s, (int)j);
+  }
+  s.lambda = lambda;
+}
+
+// NTFNES::accelerate (ntf/ntfnes.hpp:68-117) after iteration `iter` whose
+// relative error is cur_err.
+static void nes_accelerate(NTFState &s, NESState &ns, int iter, double cur_err) {
+  if (iter <= ns.acc_start) return;
+  const int N = (int)s.F.size();
+  const long k = s.k;
+  const double acc_step = std::pow(iter + 1, 1.0 / ns.acc_exp);
+  std::vector<Mat> acc(N);
+  std::vector<double> acc_l(k, 1.0);
+  for (int mode = 0; mode < N; ++mode) {
+    const Mat &F = s.F[mode];
+    Mat A(F.r, k);
+    for (long c = 0; c < k; ++c) {
+      const double sc = mode == N - 1 ? s.lambda[c] : 1.0;
+      const double sp = mode == N - 1 ? ns.prev_l[c] : 1.0;
+      for (long r = 0; r < F.r; ++r) {
+        const double cur = sc * F(r, c);
+        A(r, c) = (cur - sp * ns.prev[mode](r, c)) * acc_step + cur;
+      }
+    }
+    normalize_rows_t(A, acc_l);
+    acc[mode] = A;
+  }
+  NTFState trial = s;  // computeObjectiveError(m_acc_t) resets the factors to acc
+  ntf_reset(trial, acc, acc_l);
+  const double acc_err = ntf_rel_error(trial);
+  if (acc_err < cur_err) {
+    s.F = trial.F;
+    s.lambda = trial.lambda;
+    for (int mode = 0; mode < N; ++mode) {
+      ns.prox[mode] = acc[mode];
+      normalize_rows_t(ns.prox[mode], ns.prox_l);
+    }
+  } else {
+    ntf_reset(s, ns.prox, ns.prox_l);
+    if (++ns.acc_fails > ns.fail_limit) {
+      ns.acc_fails = 0;
+      ns.acc_exp++;
+    }
+  }
+}
+
 // AUNTF::computeNTF, ntf/auntf.hpp:128-177.
 static int ntf_loop(NTFState &s, int algo, int iters, double *relerr) {
   int N = s.X.N;
   Mat krp, M;
   std::vector<Mat> duals;  // AO-ADMM auxiliaries, zero at construction (:84-86)
-  for (int j = 0; j < N; ++j) duals.emplace_back(s.F[j].r, s.k);
+  NESState ns;
+  for (int j = 0; j < N; ++j) {
+    duals.emplace_back(s.F[j].r, s.k);
+    ns.prox.emplace_back(s.F[j].r, s.k);
+    ns.prev.emplace_back(s.F[j].r, s.k);
+  }
+  ns.prox_l.assign(s.k, 1.0);
+  ns.prev_l.assign(s.k, 1.0);
   for (int it = 0; it < iters; ++it) {
     for (int j = 0; j < N; ++j) {
       Mat S = gram_leave_out_one(s, j);
@@ -877,11 +1049,22 @@ static int ntf_loop(NTFState &s, int algo, int iters, double *relerr) {
         mttkrp_dimtree(s.X, s.F, j, s.k, s.dt, M);
       else
         mttkrp_direct(s.X, krp, j, s.k, M);
-      int rc = ntf_update(algo, s.F[j], M, S, s.k, &duals[j]);
+      int rc;
+      if (algo == 5) {
+        Mat Hn;
+        rc = nes_update(ns, j, S, M, s.k, Hn);
+        if (rc == 0) s.F[j] = Hn;
+      } else {
+        rc = ntf_update(algo, s.F[j], M, S, s.k, &duals[j]);
+      }
       if (rc < 0) return rc;
       ncp_normalize_mode(s, j);
     }
-    if (relerr) relerr[it] = ntf_rel_error(s);
+    // NTFNES turns acceleration on, which turns the error on (:188,
+    // ntf/auntf.hpp:178-181).
+    const double err = (relerr || algo == 5) ? ntf_rel_error(s) : 0.0;
+    if (relerr) relerr[it] = err;
+    if (algo == 5) nes_accelerate(s, ns, it, err);
   }
   return 0;
 }
@@ -1174,6 +1357,31 @@ int oracle_admm_solve(const double *G, int k, const double *R, long n, double *X
   std::memcpy(X, Xm.d.data(), sizeof(double) * n * k);
   std::memcpy(D, Dm.d.data(), sizeof(double) * n * k);
   return 0;
+}
+
+// Smallest and largest eigenvalue of a symmetric k x k matrix (Jacobi).
+void oracle_sym_eig_minmax(const double *S, int k, double *mn, double *mx) {
+  Mat Sm(k, k);
+  std::memcpy(Sm.d.data(), S, sizeof(double) * k * k);
+  sym_eig_minmax(Sm, *mn, *mx);
+}
+
+// One NTFNES mode update with the proximal centre Hprox (I x k): S k x k, M
+// k x I; the new (unnormalised) factor into H (I x k).  Returns 0 or -4.
+int oracle_nes_mode_update(const double *S, int k, const double *M, long I, const double *Hprox,
+                           double *H) {
+  NESState ns;
+  ns.prox.emplace_back(I, k);
+  ns.prev.emplace_back(I, k);
+  ns.prox_l.assign(k, 1.0);
+  ns.prev_l.assign(k, 1.0);
+  std::memcpy(ns.prox[0].d.data(), Hprox, sizeof(double) * I * k);
+  Mat Sm(k, k), Mm(k, I), Hm;
+  std::memcpy(Sm.d.data(), S, sizeof(double) * k * k);
+  std::memcpy(Mm.d.data(), M, sizeof(double) * k * I);
+  int rc = nes_update(ns, 0, Sm, Mm, k, Hm);
+  if (rc == 0) std::memcpy(H, Hm.d.data(), sizeof(double) * I * k);
+  return rc;
 }
 
 // Wall time of the iteration loop of the last oracle_nmf / oracle_mu_f32 call.

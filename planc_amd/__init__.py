@@ -121,10 +121,12 @@ class Context:
             self.ctx = None
         _LIVE.discard(self)
 
-    def __del__(self):
-        if sys.is_finalizing():
-            return
+    def __del__(self, _finalizing=sys.is_finalizing):
+        # (the bound default keeps the check usable after the sys module has
+        # been torn down at interpreter exit)
         try:
+            if _finalizing():
+                return
             self.close()
         except Exception:
             pass

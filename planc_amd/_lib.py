@@ -100,6 +100,8 @@ SIGNATURES = {
                                           ctypes.POINTER(c_void_p), c_int, c_void_p]),
     "planc_cp_loss_bwd": (c_int, [c_void_p, c_void_p, c_int, ctypes.POINTER(c_long),
                                   ctypes.POINTER(c_void_p), c_int, ctypes.POINTER(c_void_p)]),
+    "planc_admm_update": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_long, c_long, c_void_p, c_long,
+                                  c_void_p, c_long, c_long]),
     "planc_bpp": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_long, c_long, c_long, c_void_p,
                           c_long, c_long]),
     "planc_csr_col_subset": (c_int, [c_void_p, c_long, c_long, c_void_p, c_void_p, c_void_p,

@@ -803,7 +803,8 @@ int planc_ntf_run(planc_ctx *ctx, const planc_ntf_desc *desc, const float *d_X, 
   if (N < 2 || N > 7 || k <= 0 || k > 64 || iters < 0 || !d_X || !d_factors || !d_lambda ||
       !valid_dtype(dt))
     return set_error(ctx, PLANC_EINVAL, "planc_ntf_run: bad arguments (2..7 modes, k 1..64)");
-  if (algo != PLANC_MU && algo != PLANC_HALS && algo != PLANC_ANLSBPP && algo != PLANC_AOADMM)
+  if (algo != PLANC_MU && algo != PLANC_HALS && algo != PLANC_ANLSBPP && algo != PLANC_AOADMM &&
+      algo != PLANC_NESTEROV)
     return set_error(ctx, PLANC_EUNSUPPORTED, "planc_ntf_run: algorithm not supported");
   PLANC_HIP_CHECK(ctx, hipSetDevice(ctx->device));
   const long *dims = desc->dims;
@@ -891,7 +892,32 @@ int planc_ntf_run(planc_ctx *ctx, const planc_ntf_desc *desc, const float *d_X, 
       db += dims[i] * k;
     }
   }
-  if (desc->compute_error) RC(dot(ctx, d_X, 1, numel, d_X, 1, numel, numel, 1, normX2, 0));
+  // NTFNES state (ntf/ntfnes.hpp:9-188): the proximal centres and previous
+  // iterates (zero at construction, :175-176, held here as I_n x k), the
+  // accelerated factors, a lambda-scaled copy of mode 0 for the trial error,
+  // and the lambdas (ones at construction).
+  const bool nes = algo == PLANC_NESTEROV;
+  std::vector<float *> nprox(N), nprev(N), nacc(N);
+  float *ntmp0 = nullptr;
+  double *nl = nullptr;  // prox_l, prev_l, acc_l (k each), trial loss
+  if (nes) {
+    long tot = 0;
+    for (int i = 0; i < N; ++i) tot += dims[i] * k;
+    float *b = (float *)ws_get(ctx, 36, sizeof(float) * (size_t)(3 * tot + dims[0] * k));
+    nl = (double *)ws_get(ctx, 37, sizeof(double) * (3 * (size_t)k + 1));
+    if (!b || !nl) return set_error(ctx, PLANC_ENOMEM, "planc_ntf_run: NES state");
+    PLANC_HIP_CHECK(ctx, hipMemsetAsync(b, 0, sizeof(float) * (size_t)(3 * tot), ctx->stream));
+    for (int i = 0; i < N; ++i) {
+      nprox[i] = b;
+      nprev[i] = b + tot;
+      nacc[i] = b + 2 * tot;
+      b += dims[i] * k;
+    }
+    ntmp0 = nacc[N - 1] + dims[N - 1] * k;
+    std::vector<double> ones(2 * (size_t)k, 1.0);
+    PLANC_HIP_CHECK(ctx, hipMemcpyAsync(nl, ones.data(), sizeof(double) * 2 * k, hipMemcpyHostToDevice, ctx->stream));
+  }
+  if (desc->compute_error || nes) RC(dot(ctx, d_X, 1, numel, d_X, 1, numel, numel, 1, normX2, 0));
   // NCPFactors::normalize() of the initial factors (ntf/auntf.hpp:88).
   for (int i = 0; i < N; ++i) {
     RC(col_norms(ctx, d_factors[i], dims[i], k, dims[i], norms));
@@ -915,7 +941,7 @@ int planc_ntf_run(planc_ctx *ctx, const planc_ntf_desc *desc, const float *d_X, 
     int rc;
     bool err_done = false;
     for (int n = 0; n < N; ++n) {
-      const bool fused = mode_update_fits(dims[n], k) && algo != PLANC_AOADMM;
+      const bool fused = mode_update_fits(dims[n], k) && algo != PLANC_AOADMM && !nes;
       if (!fused || algo == PLANC_ANLSBPP) RC(hadamard_grams(ctx, Gc.data(), N, n, k, S));
       // This mode's MTTKRP: element (x, i) at Mp[i * mrs + x * mcs].
       const float *Mp = M;
@@ -975,6 +1001,15 @@ int planc_ntf_run(planc_ctx *ctx, const planc_ntf_desc *desc, const float *d_X, 
         RC(row_update(ctx, 1, Fn, dims[n], k, dims[n], M, k, 1, S, 0.0f));
       } else if (algo == PLANC_MU) {
         RC(row_update(ctx, 0, Fn, dims[n], k, dims[n], M, k, 1, S, 1e-6f));
+      } else if (nes) {
+        // NTFNES::update: the previous iterate (and at mode 0 its lambda)
+        // saved, the proximal Nesterov loop from the centre, the centre
+        // replaced by the result with unit columns.
+        if (n == 0)
+          PLANC_HIP_CHECK(ctx, hipMemcpyAsync(nl + k, nl, sizeof(double) * k, hipMemcpyDeviceToDevice, ctx->stream));
+        PLANC_HIP_CHECK(ctx, hipMemcpyAsync(nprev[n], nprox[n], sizeof(float) * dims[n] * k,
+                                            hipMemcpyDeviceToDevice, ctx->stream));
+        RC(nes_mode_update(ctx, S, k, M, k, 1, nprox[n], dims[n], Fn, dims[n], nl, dflag));
       } else if (algo == PLANC_AOADMM) {
         // NTFAOADMM::update (ntf/ntfaoadmm.hpp:23-77): ADMM solve on S and
         // the MTTKRP with this mode's dual, then the dual's columns to unit
@@ -990,15 +1025,79 @@ int planc_ntf_run(planc_ctx *ctx, const planc_ntf_desc *desc, const float *d_X, 
       RC(scale_cols(ctx, Fn, dims[n], dims[n], nullptr, 0, 0, k, d_lambda));
       RC(gram(ctx, Fn, dims[n], k, dims[n], G[n]));
     }
-    if (desc->compute_error && !err_done) {
+    if ((desc->compute_error || nes) && !err_done) {
       RC(ntf_relerr(ctx, normX2, d_factors[N - 1], dims[N - 1], dims[N - 1], M, k, k, d_lambda,
                     Gc.data(), N, rel_counter ? relslot : rel_out));
       if (rel_counter) RC(counter_append(ctx, relslot, rel_out, rel_counter));
     }
     return PLANC_OK;
   };
-  if (iters > 0) RC(one_iteration(rel, nullptr));
-  if (iters > 1) {
+  if (nes) {
+    // Eager iterations: accelerate() (ntf/ntfnes.hpp:68-117) decides on the
+    // host between the accelerated factors and the proximal ones.
+    double normX2_h = 0;
+    PLANC_HIP_CHECK(ctx, hipMemcpyAsync(&normX2_h, normX2, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    int acc_exp = 3, acc_fails = 0;
+    const int acc_start = 5, fail_limit = 5;
+    double *prox_l = nl, *prev_l = nl + k, *acc_l = nl + 2 * k, *loss = nl + 3 * k;
+    auto reset_to = [&](const std::vector<float *> &src, const double *lam) -> int {
+      // reset(new_factors, true) (ntf/auntf.hpp:186-198)
+      int rc2;
+      for (int i = 0; i < N; ++i) {
+        PLANC_HIP_CHECK(ctx, hipMemcpyAsync(d_factors[i], src[i], sizeof(float) * dims[i] * k,
+                                            hipMemcpyDeviceToDevice, ctx->stream));
+        if ((rc2 = col_norms(ctx, d_factors[i], dims[i], k, dims[i], d_lambda))) return rc2;
+        if ((rc2 = scale_cols(ctx, d_factors[i], dims[i], dims[i], nullptr, 0, 0, k, d_lambda))) return rc2;
+      }
+      PLANC_HIP_CHECK(ctx, hipMemcpyAsync(d_lambda, lam, sizeof(double) * k, hipMemcpyDeviceToDevice, ctx->stream));
+      return PLANC_OK;
+    };
+    for (int it = 0; it < iters; ++it) {
+      RC(one_iteration(rel + it, nullptr));
+      if (it <= acc_start) continue;
+      double cur_err = 0;
+      PLANC_HIP_CHECK(ctx, hipMemcpyAsync(&cur_err, rel + it, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+      PLANC_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+      const double step = std::pow((double)(it + 1), 1.0 / acc_exp);
+      for (int i = 0; i < N; ++i) {
+        const bool last = i == N - 1;  // only the last mode carries lambda
+        RC(nes_acc_build(ctx, d_factors[i], nprev[i], dims[i], k, last ? d_lambda : nullptr,
+                         last ? prev_l : nullptr, step, nacc[i]));
+        RC(col_norms(ctx, nacc[i], dims[i], k, dims[i], acc_l));
+        RC(scale_cols(ctx, nacc[i], dims[i], dims[i], nullptr, 0, 0, k, acc_l));
+      }
+      // computeObjectiveError(m_acc_t): the factors reset to the accelerated
+      // ones, the error from the fused CP loss with lambda folded into mode 0.
+      RC(reset_to(nacc, acc_l));
+      RC(scale_cols_by(ctx, d_factors[0], dims[0], k, d_lambda, ntmp0));
+      std::vector<const float *> fl(N);
+      fl[0] = ntmp0;
+      for (int i = 1; i < N; ++i) fl[i] = d_factors[i];
+      RC(cp_loss_forward(ctx, d_X, N, dims, fl.data(), k, loss));
+      double loss_h = 0;
+      PLANC_HIP_CHECK(ctx, hipMemcpyAsync(&loss_h, loss, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+      PLANC_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+      const double acc_err = std::sqrt(2.0 * loss_h / normX2_h);
+      if (acc_err < cur_err) {
+        for (int i = 0; i < N; ++i) {
+          PLANC_HIP_CHECK(ctx, hipMemcpyAsync(nprox[i], nacc[i], sizeof(float) * dims[i] * k,
+                                              hipMemcpyDeviceToDevice, ctx->stream));
+          RC(col_norms(ctx, nprox[i], dims[i], k, dims[i], prox_l));
+          RC(scale_cols(ctx, nprox[i], dims[i], dims[i], nullptr, 0, 0, k, prox_l));
+        }
+      } else {
+        RC(reset_to(nprox, prox_l));
+        if (++acc_fails > fail_limit) {
+          acc_fails = 0;
+          acc_exp++;
+        }
+      }
+      for (int i = 0; i < N; ++i) RC(gram(ctx, d_factors[i], dims[i], k, dims[i], G[i]));
+    }
+  } else if (iters > 0) {
+    RC(one_iteration(rel, nullptr));
+  }
+  if (!nes && iters > 1) {
     // Iterations 1.. replay a captured graph of GU iterations (at small
     // shapes an iteration is a few tens of microseconds and every graph
     // launch costs several); the remainder runs eagerly.  The graph is
@@ -1046,6 +1145,7 @@ int planc_ntf_run(planc_ctx *ctx, const planc_ntf_desc *desc, const float *d_X, 
   PLANC_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   if (hflag & 2) return set_error(ctx, PLANC_EHIP, "NTF mode update: block exchange timed out");
   if (hflag & 4) return set_error(ctx, PLANC_ENOCONV, "AO-ADMM: shifted Gram not positive definite");
+  if (hflag & 8) return set_error(ctx, PLANC_ENOCONV, "NES: the Nesterov loop ran past 100000 steps");
   if (hflag) return set_error(ctx, PLANC_ENOCONV, "BPP failed: more than 5k pivot iterations");
   return PLANC_OK;
 }
@@ -1136,6 +1236,23 @@ int planc_cp_loss_bwd(planc_ctx *ctx, const float *d_X, int nmodes, const long *
     if (rc) return rc;
   }
   PLANC_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return PLANC_OK;
+}
+
+int planc_admm_update(planc_ctx *ctx, const double *d_G, int k, const float *d_R, long r_js, long r_cs,
+                      float *d_X, long ldx, float *d_D, long ldd, long n) {
+  if (check_ctx(ctx)) return PLANC_EINVAL;
+  if (!d_G || !d_R || !d_X || !d_D || k <= 0 || k > 64 || n < 0 || ldx < n || ldd < n)
+    return set_error(ctx, PLANC_EINVAL, "planc_admm_update: bad arguments (k must be 1..64)");
+  int *dflag = (int *)ws_get(ctx, 11, sizeof(int));
+  if (!dflag) return set_error(ctx, PLANC_ENOMEM, "planc_admm_update: workspace");
+  PLANC_HIP_CHECK(ctx, hipMemsetAsync(dflag, 0, sizeof(int), ctx->stream));
+  int rc = admm_solve(ctx, d_G, k, d_R, r_js, r_cs, d_X, ldx, d_D, ldd, n, dflag);
+  if (rc) return rc;
+  int hflag = 0;
+  PLANC_HIP_CHECK(ctx, hipMemcpyAsync(&hflag, dflag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  PLANC_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  if (hflag) return set_error(ctx, PLANC_ENOCONV, "AO-ADMM: shifted Gram not positive definite");
   return PLANC_OK;
 }
 

@@ -116,7 +116,9 @@ namespace planc {
 // control, 20 split 16-bit factors of ANLS-BPP, 22 AO-ADMM factor / partials,
 // 23 GNSym CG vectors, 24..31 sparse path (sparse.hip: k-major factor
 // copies, error partials, transposed CSR, sort keys / entries / temporary
-// storage), 32 NTF AO-ADMM duals, 33 NMF AO-ADMM duals, 34 dual norms.
+// storage), 32 NTF AO-ADMM duals, 33 NMF AO-ADMM duals, 34 dual norms,
+// 35 NES iterates, 36 NES factors (centres, previous, accelerated), 37 NES
+// lambdas.
 void *ws_get(planc_ctx *ctx, int slot, size_t bytes);
 
 int set_error(planc_ctx *ctx, int code, const std::string &msg);
@@ -255,6 +257,15 @@ int admm_solve(planc_ctx *ctx, const double *G, int k, const float *R, long r_js
 // One symmetric-NMF Gauss-Newton step of H (n x k) from AHt = H^T A (k x n)
 // and HtH of the current H.
 int gnsym_step(planc_ctx *ctx, float *H, long n, int k, const float *AHt, const double *HtH);
+
+// NTFNES pieces (nes.hip): the extreme eigenvalues of a symmetric k x k
+// matrix (host), one mode update, the accelerated factor and a column scaling.
+void sym_eig_extremes(const std::vector<double> &S, int k, double *mn, double *mx);
+int nes_mode_update(planc_ctx *ctx, const double *dS, int k, const float *M, long mrs, long mcs, float *prox,
+                    long I, float *F, long ldf, double *prox_l, int *status);
+int nes_acc_build(planc_ctx *ctx, const float *F, const float *Pv, long I, int k, const double *sc,
+                  const double *sp, double step, float *A);
+int scale_cols_by(planc_ctx *ctx, const float *A, long I, int k, const double *s, float *out);
 
 // Allocation-free forward: block partials in a per-device static array
 // located once per device by cp_noalloc_setup (called from planc_create).

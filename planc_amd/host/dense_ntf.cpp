@@ -13,7 +13,17 @@ class NTFDriver {
     Tensor t(pc.dimensions());
     std::string fn = pc.input_file_name();
     INFO << "Input filename = " << fn << std::endl;
-    if (!fn.empty() && fn.compare(0, 5, "rand_") != 0) t.read(fn);
+    const bool npy = fn.size() > 4 && fn.compare(fn.size() - 4, 4, ".npy") == 0;
+    if (npy) {
+      // A NumPy array (the reader of common/npyio.hpp, which the reference's
+      // distributed driver uses, distntf/distntfio.hpp:141).
+      NumPyArray a;
+      a.load(fn);
+      a.printInfo();
+      t = *a.m_input_tensor;
+    } else if (!fn.empty() && fn.compare(0, 5, "rand_") != 0) {
+      t.read(fn);
+    }
     NTFTYPE solver(t, (int)pc.lowrankk(), pc.lucalgo());
     solver.num_it(pc.iterations());
     solver.compute_error(pc.compute_error());
@@ -38,6 +48,7 @@ int main(int argc, char *argv[]) {
       case MU: d.callNTF<planc::NTFMU>(pc); break;
       case HALS: d.callNTF<planc::NTFHALS>(pc); break;
       case ANLSBPP: d.callNTF<planc::NTFANLSBPP>(pc); break;
+      case AOADMM: d.callNTF<planc::NTFAOADMM>(pc); break;
       default:
         ERR << "Wrong algorithm choice. Quitting.." << pc.lucalgo() << std::endl;
         return 1;

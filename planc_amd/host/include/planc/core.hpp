@@ -6,6 +6,7 @@
 #ifndef PLANC_AMD_HOST_CORE_HPP_
 #define PLANC_AMD_HOST_CORE_HPP_
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdint>
@@ -104,7 +105,75 @@ class Mat {
   std::vector<double> d_;
 };
 
+// ---------------------------------------------------------------------------
+// Sparse matrix (stands in for arma::sp_mat, PLANC's SP_MAT): held as CSR
+// (row pointers, sorted column ids, FP64 values), the layout the device
+// products take.  coord_ascii text is one "row col value" triple per line,
+// 0-based (Armadillo's coord_ascii, read by common/utils.hpp:326-328);
+// entries given twice are summed.
+class SpMat {
+ public:
+  unsigned long long n_rows = 0, n_cols = 0;
+  std::vector<long> row_ptr{0};
+  std::vector<int> col_idx;
+  std::vector<double> values;
+  SpMat() {}
+  SpMat(unsigned long long r, unsigned long long c) : n_rows(r), n_cols(c), row_ptr(r + 1, 0) {}
+  unsigned long long n_nonzero() const { return values.size(); }
+  double norm_fro() const {
+    double s = 0;
+    for (double v : values) s += v * v;
+    return std::sqrt(s);
+  }
+  /// Builds the CSR from triplets (sorted by row, then column; duplicates summed).
+  void from_triplets(unsigned long long r, unsigned long long c,
+                     std::vector<std::pair<std::pair<long, long>, double>> &t) {
+    std::sort(t.begin(), t.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
+    n_rows = r;
+    n_cols = c;
+    row_ptr.assign(r + 1, 0);
+    col_idx.clear();
+    values.clear();
+    for (size_t e = 0; e < t.size(); ++e) {
+      if (!col_idx.empty() && e > 0 && t[e].first == t[e - 1].first) {
+        values.back() += t[e].second;
+        continue;
+      }
+      row_ptr[t[e].first.first + 1]++;
+      col_idx.push_back((int)t[e].first.second);
+      values.push_back(t[e].second);
+    }
+    for (unsigned long long i = 0; i < r; ++i) row_ptr[i + 1] += row_ptr[i];
+  }
+  bool load_coord_ascii(const std::string &fn) {
+    std::ifstream in(fn);
+    if (!in) return false;
+    std::vector<std::pair<std::pair<long, long>, double>> t;
+    long r, c, mr = -1, mc = -1;
+    double v;
+    std::string line;
+    while (std::getline(in, line)) {
+      if (line.find_first_not_of(" \t\r") == std::string::npos) continue;  // blank line
+      std::istringstream ss(line);
+      if (!(ss >> r >> c >> v)) return false;  // malformed triple
+      if (r < 0 || c < 0) return false;
+      t.push_back({{r, c}, v});
+      mr = std::max(mr, r);
+      mc = std::max(mc, c);
+    }
+    from_triplets((unsigned long long)(mr + 1), (unsigned long long)(mc + 1), t);
+    return true;
+  }
+  void save_coord_ascii(const std::string &fn) const {
+    std::ofstream o(fn);
+    o.precision(17);
+    for (unsigned long long i = 0; i < n_rows; ++i)
+      for (long e = row_ptr[i]; e < row_ptr[i + 1]; ++e) o << i << " " << col_idx[e] << " " << values[e] << "\n";
+  }
+};
+
 typedef Mat MAT;
+typedef SpMat SP_MAT;
 typedef std::vector<double> VEC;
 typedef std::vector<float> FVEC;
 typedef std::vector<unsigned long long> UVEC;

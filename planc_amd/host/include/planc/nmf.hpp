@@ -42,12 +42,6 @@ class NMF {
 
   /// One device run of `algo` for num_iterations() iterations.
   void run_on_device(int algo) {
-    Device &dev = Device::get();
-    planc_ctx *ctx = dev.ctx();
-    DeviceBuffer<float> dA((size_t)m * n), dW((size_t)m * k), dH((size_t)n * k);
-    upload_f32(A.memptr(), (size_t)m * n, dA);
-    upload_f32(W.memptr(), (size_t)m * k, dW);
-    upload_f32(H.memptr(), (size_t)n * k, dH);
     planc_nmf_desc d{};
     d.algo = algo;
     d.dtype = m_precision;
@@ -62,13 +56,46 @@ class NMF {
     d.compute_error = 1;  // always on in the shared-memory driver (nmf/nmf.cpp:205)
     d.symm_reg = m_symm_reg;
     m_relerr.assign(m_num_iterations, 0.0);
-    dev.check(planc_nmf_run(ctx, &d, dA.get(), m, nullptr, nullptr, dW.get(), dH.get(),
-                            m_relerr.data()),
-              "planc_nmf_run");
-    download_f32(dW, W.memptr(), (size_t)m * k);
-    download_f32(dH, H.memptr(), (size_t)n * k);
+    device_nmf(A, d, W, H, m_relerr);
     normA = A.norm_fro();
     for (unsigned it = 0; it < m_num_iterations; ++it) printObjective(it);
+  }
+
+  // The device loop for a dense A (planc_nmf_run) ...
+  static void device_nmf(const Mat &A, const planc_nmf_desc &d, MAT &W, MAT &H,
+                         std::vector<double> &relerr) {
+    Device &dev = Device::get();
+    const size_t m = d.m, n = d.n, k = d.k;
+    DeviceBuffer<float> dA(m * n), dW(m * k), dH(n * k);
+    upload_f32(A.memptr(), m * n, dA);
+    upload_f32(W.memptr(), m * k, dW);
+    upload_f32(H.memptr(), n * k, dH);
+    dev.check(planc_nmf_run(dev.ctx(), &d, dA.get(), (long)m, nullptr, nullptr, dW.get(), dH.get(),
+                            relerr.data()),
+              "planc_nmf_run");
+    download_f32(dW, W.memptr(), m * k);
+    download_f32(dH, H.memptr(), n * k);
+  }
+  // ... and for a sparse A (planc_nmf_sparse_run, the reference's BUILD_SPARSE build).
+  static void device_nmf(const SpMat &A, const planc_nmf_desc &d, MAT &W, MAT &H,
+                         std::vector<double> &relerr) {
+    Device &dev = Device::get();
+    const size_t m = d.m, n = d.n, k = d.k, nnz = A.n_nonzero();
+    DeviceBuffer<long> dp(m + 1);
+    DeviceBuffer<int> di(nnz ? nnz : 1);
+    DeviceBuffer<float> dv(nnz ? nnz : 1), dW(m * k), dH(n * k);
+    dp.upload(A.row_ptr.data());
+    if (nnz) {
+      di.upload(A.col_idx.data());
+      upload_f32(A.values.data(), nnz, dv);
+    }
+    upload_f32(W.memptr(), m * k, dW);
+    upload_f32(H.memptr(), n * k, dH);
+    dev.check(planc_nmf_sparse_run(dev.ctx(), &d, dp.get(), di.get(), dv.get(), dW.get(), dH.get(),
+                                   relerr.data()),
+              "planc_nmf_sparse_run");
+    download_f32(dW, W.memptr(), m * k);
+    download_f32(dH, H.memptr(), n * k);
   }
 
  public:
@@ -121,12 +148,7 @@ class NMF {
         WtW[a + (size_t)k * b] = s;
         HtH[a + (size_t)k * b] = t;
       }
-    for (unsigned j = 0; j < n; ++j)
-      for (unsigned i = 0; i < m; ++i) {
-        double wh = 0;
-        for (unsigned c = 0; c < k; ++c) wh += W(i, c) * H(j, c);
-        tr1 += A(i, j) * wh;
-      }
+    tr1 = trace_hatw(A);
     for (unsigned a = 0; a < k; ++a)
       for (unsigned b = 0; b < k; ++b) tr2 += WtW[a + (size_t)k * b] * HtH[b + (size_t)k * a];
     double froW = 0, froH = 0;
@@ -138,6 +160,28 @@ class NMF {
     normH = std::sqrt(froH);
     fit_err_sq = normA * normA - 2 * tr1 + tr2;
     objective_err = fit_err_sq + m_regW[0] * froW + m_regH[0] * froH;
+  }
+
+  /// tr(H^T A^T W) for a dense or a sparse A.
+  double trace_hatw(const Mat &A) const {
+    double t = 0;
+    for (unsigned j = 0; j < n; ++j)
+      for (unsigned i = 0; i < m; ++i) {
+        double wh = 0;
+        for (unsigned c = 0; c < k; ++c) wh += W(i, c) * H(j, c);
+        t += A(i, j) * wh;
+      }
+    return t;
+  }
+  double trace_hatw(const SpMat &A) const {
+    double t = 0;
+    for (unsigned i = 0; i < m; ++i)
+      for (long e = A.row_ptr[i]; e < A.row_ptr[i + 1]; ++e) {
+        double wh = 0;
+        for (unsigned c = 0; c < k; ++c) wh += W(i, c) * H(A.col_idx[e], c);
+        t += A.values[e] * wh;
+      }
+    return t;
   }
 
   /// printObjective (common/nmf.hpp:461-478) for iteration itr of the last
@@ -181,9 +225,13 @@ class MUNMF : public NMF<T> {
 template <class T>
 class HALSNMF : public NMF<T> {
  public:
-  HALSNMF(const T &A, int lowrank) : NMF<T>(A, lowrank) { this->m_updalgo = HALS; }
+  HALSNMF(const T &A, int lowrank) : NMF<T>(A, lowrank) {
+    this->m_updalgo = HALS;
+    this->normalize_by_W();  // nmf/hals.hpp:41
+  }
   HALSNMF(const T &A, const MAT &llf, const MAT &rlf) : NMF<T>(A, llf, rlf) {
     this->m_updalgo = HALS;
+    this->normalize_by_W();  // nmf/hals.hpp:46
   }
   void computeNMF() { this->run_on_device(PLANC_HALS); }
 };
@@ -198,6 +246,39 @@ class BPPNMF : public NMF<T> {
     this->m_updalgo = ANLSBPP;
   }
   void computeNMF() { this->run_on_device(PLANC_ANLSBPP); }
+};
+
+/// AOADMMNMF (nmf/aoadmm.hpp:11-195): normalize_by_W in the constructor,
+/// per iteration an ADMM solve for H then for W (duals carried over, zero at
+/// the start of computeNMF), no final normalisation.
+template <class T>
+class AOADMMNMF : public NMF<T> {
+ public:
+  AOADMMNMF(const T &A, int lowrank) : NMF<T>(A, lowrank) {
+    this->m_updalgo = AOADMM;
+    this->normalize_by_W();  // nmf/aoadmm.hpp:79
+  }
+  AOADMMNMF(const T &A, const MAT &llf, const MAT &rlf) : NMF<T>(A, llf, rlf) {
+    this->m_updalgo = AOADMM;
+    this->normalize_by_W();  // nmf/aoadmm.hpp:83
+  }
+  void computeNMF() { this->run_on_device(PLANC_AOADMM); }
+};
+
+/// GNSYMNMF (nmf/gnsym.hpp:11-262): symmetric NMF A ~ H H^T of a square A by
+/// Gauss-Newton steps with conjugate gradients; W is cleared (empty).
+template <class T>
+class GNSYMNMF : public NMF<T> {
+ public:
+  GNSYMNMF(const T &A, int lowrank) : NMF<T>(A, lowrank) { this->m_updalgo = GNSYM; }
+  GNSYMNMF(const T &A, const MAT &llf, const MAT &rlf) : NMF<T>(A, llf, rlf) {
+    this->m_updalgo = GNSYM;
+  }
+  void computeNMF() {
+    if (this->m != this->n) throw Error("GNSYMNMF: A must be square", PLANC_EINVAL);
+    this->run_on_device(PLANC_GNSYM);
+    this->W.zeros(0, 0);  // this->W.clear(), nmf/gnsym.hpp:73,98
+  }
 };
 
 }  // namespace planc

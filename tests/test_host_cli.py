@@ -130,3 +130,172 @@ def test_dense_ntf_runs_and_error_decreases():
                                                   p.stdout)])
     assert rel.shape == (8,)
     assert np.all(np.diff(rel) <= 1e-6)
+
+
+# ---- on-disk formats (CPU: the format_check tool makes no GPU call) --------------
+def _fc(*args):
+    return run([os.path.join(BIN, "format_check"), *args]).stdout
+
+
+def _vals(out, key):
+    m = re.search(key + r" ([0-9eE.+-]+)", out)
+    return float(m.group(1))
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(BIN, "format_check")), reason="format_check not built")
+def test_tensor_info_raw_round_trip(tmp_path):
+    """Tensor::read / write (common/tensor.hpp:430-503): <name>.info holds the
+    mode count and dimensions, <name> the raw little-endian doubles."""
+    X = np.random.default_rng(3).random((4, 5, 6))
+    fn = tmp_path / "t.bin"
+    X.ravel(order="F").astype("<f8").tofile(fn)
+    (tmp_path / "t.info").write_text("3\n4\n5\n6\n\n")
+    out = _fc("tensor", str(fn))
+    assert "modes 3 numel 120" in out and "roundtrip ok" in out
+    assert abs(_vals(out, "sum") - X.sum()) < 1e-12
+    assert _vals(out, "first") == X[0, 0, 0] and _vals(out, "last") == X[-1, -1, -1]
+    info = (tmp_path / "t.bin.copy.info").read_text().split()
+    assert info[:4] == ["3", "4", "5", "6"]
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(BIN, "format_check")), reason="format_check not built")
+def test_npy_reader(tmp_path):
+    """NumPyArray (common/npyio.hpp): shape and fortran_order from the header,
+    the doubles in file order (mode 0 fastest for a Fortran-order array)."""
+    X = np.asfortranarray(np.random.default_rng(4).random((3, 4, 5, 2)))
+    fn = tmp_path / "x.npy"
+    np.save(fn, X)
+    out = _fc("npy", str(fn))
+    assert "modes 4 fortran 1 word 8 dims 3 4 5 2" in out
+    assert abs(_vals(out, "sum") - X.sum()) < 1e-12
+    assert _vals(out, "first") == X[0, 0, 0, 0] and _vals(out, "last") == X[-1, -1, -1, -1]
+    # C order: the data are taken in file order, as the reference does
+    Y = np.ascontiguousarray(X)
+    np.save(tmp_path / "y.npy", Y)
+    out = _fc("npy", str(tmp_path / "y.npy"))
+    assert "fortran 0" in out and abs(_vals(out, "last") - Y.ravel()[-1]) == 0
+    # other dtypes are refused
+    np.save(tmp_path / "z.npy", X.astype(np.float32))
+    p = run([os.path.join(BIN, "format_check"), "npy", str(tmp_path / "z.npy")], check=False)
+    assert p.returncode != 0 and "8-byte floats" in p.stderr
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(BIN, "format_check")), reason="format_check not built")
+def test_coord_ascii_reader(tmp_path):
+    """coord_ascii (common/utils.hpp:326-328): 0-based "row col value" triples
+    in any order; the size is the largest index + 1; the CSR is sorted."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(5)
+    A = sp.random(40, 30, density=0.1, format="coo", random_state=rng)
+    perm = rng.permutation(A.nnz)
+    r, c, v = A.row[perm], A.col[perm], A.data[perm]
+    r = np.concatenate([r, [39]])
+    c = np.concatenate([c, [29]])
+    v = np.concatenate([v, [0.5]])
+    fn = tmp_path / "a.txt"
+    with open(fn, "w") as f:
+        for a, b, x in zip(r, c, v):
+            f.write(f"{a} {b} {float(x)!r}\n")
+    out = _fc("coord", str(fn))
+    B = sp.coo_matrix((v, (r, c)), shape=(40, 30)).tocsr()
+    assert f"rows 40 cols 30 nnz {B.nnz}" in out and "roundtrip ok" in out
+    assert abs(_vals(out, "sum") - B.sum()) < 1e-12
+
+
+@pytest.mark.skipif(not have_bins(), reason="host drivers not built")
+def test_sparse_nmf_and_symm_flags_dry_run():
+    p = run([os.path.join(BIN, "sparse_nmf"), "-a", "2", "-d", "300 300", "-k", "8", "-s", "0.05",
+             "-i", "rand_uniform", "--symm", "0", "--dry-run"])
+    assert "a::2::" in p.stdout
+    p = run([os.path.join(BIN, "dense_nmf"), "-a", "0", "-d", "300 300", "-k", "8", "-i", "rand_lowrank",
+             "--symm", "0", "--dry-run"], check=False)
+    assert p.returncode != 0 and "only enabled for ANLSBPP and GNSYM" in p.stderr
+    p = run([os.path.join(BIN, "dense_nmf"), "-a", "2", "-d", "300 200", "-k", "8", "-i", "rand_lowrank",
+             "--symm", "0", "--dry-run"], check=False)
+    assert p.returncode != 0 and "not square" in p.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("algo", [0, 1, 2, 4])
+def test_sparse_nmf_cli_matches_oracle(tmp_path, algo):
+    """sparse_nmf on a coord_ascii file: the sparse build's loop against the
+    oracle's sparse restatement from the same init (--seed streams)."""
+    import scipy.sparse as sp
+    from oracle import oracle as O
+    from planc_amd import datagen as D
+    rng = np.random.default_rng(60 + algo)
+    m, n, k, t = 400, 300, 6, 8
+    # HALS on a masked low-rank matrix (on a random sparse one its trajectory
+    # is chaotic: FP64 evaluation orders part by 1e-3 within a few iterations)
+    dens = 0.2 if algo == 1 else 0.05
+    A = sp.random(m, n, density=dens, format="csr", random_state=rng, data_rvs=lambda s: rng.random(s) + 0.1)
+    A.sort_indices()
+    if algo == 1:
+        Wt, Ht = D.uniform(7, 1, (m, k)), D.uniform(7, 2, (k, n))
+        r = np.repeat(np.arange(m), np.diff(A.indptr))
+        A.data = np.einsum("ek,ke->e", Wt[r], Ht[:, A.indices])
+    fn = tmp_path / "a.txt"
+    Acoo = A.tocoo()
+    with open(fn, "w") as f:
+        for a, b, x in zip(Acoo.row, Acoo.col, Acoo.data):
+            f.write(f"{a} {b} {float(x)!r}\n")
+    p = run([os.path.join(BIN, "sparse_nmf"), "-a", str(algo), "-d", f"{m} {n}", "-k", str(k), "-t", str(t),
+             "-i", str(fn)])
+    rel = _relerrs(p.stdout)
+    assert rel.shape == (t,)
+    W0 = D.uniform(193957, 0, (m, k)).astype(np.float32).astype(np.float64)
+    H0 = D.uniform(193957, 1, (n, k)).astype(np.float32).astype(np.float64)
+    A32 = A.astype(np.float32)
+    _, _, rel_r, rc = O.nmf_sparse(algo, m, n, A32.indptr, A32.indices, A32.data.astype(np.float64), W0, H0, t)
+    assert rc == 0
+    assert np.abs(rel - rel_r).max() < 1e-4
+
+
+@pytest.mark.gpu
+def test_dense_ntf_npy_input_matches_oracle(tmp_path):
+    from oracle import oracle as O
+    from planc_amd import datagen as D
+    X, _ = D.cp_tensor((20, 18, 16), 4, 0.01, stream=61)
+    fn = tmp_path / "x.npy"
+    np.save(fn, np.asfortranarray(X))
+    p = run([os.path.join(BIN, "dense_ntf"), "-a", "1", "-d", "20 18 16", "-k", "4", "-t", "6", "-e", "1",
+             "-i", str(fn)])
+    rel = np.array([float(x) for x in re.findall(r"relative_error at it::\d+::([0-9.eE+-]+)", p.stdout)])
+    # the driver's NCPFactors initial factors: Philox seed 103, stream = mode
+    F0 = [D.uniform(103, i, (d, 4)) for i, d in enumerate((20, 18, 16))]
+    _, _, rel_r, rc = O.ntf(1, X.astype(np.float32).astype(np.float64),
+                            [f.astype(np.float32).astype(np.float64) for f in F0], 6, dimtree=1)
+    assert rc == 0
+    assert np.abs(rel - rel_r).max() < 1e-4
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dimtree", [0, 1])
+def test_auntf_host_update_plugin_matches_oracle(tmp_path, dimtree):
+    """The AUNTF plugin contract: a host update(mode) (NTF-MU computed on the
+    host from gram_without_one and ncp_mttkrp_t) with the tensor resident on
+    the device and the MTTKRPs (dimension tree or direct) there, against the
+    oracle's NTF-MU trajectory; the built-in NTFHALS through the host loop
+    equals its device loop."""
+    from oracle import oracle as O
+    from planc_amd import datagen as D
+    dims, k, t = (14, 12, 10), 4, 6
+    X, _ = D.cp_tensor(dims, k, 0.01, stream=62)
+    fn = tmp_path / "x.bin"
+    X.ravel(order="F").astype("<f8").tofile(fn)
+    (tmp_path / "x.info").write_text("3\n14\n12\n10\n\n")
+    p = run([os.path.join(BIN, "ntf_host_rule"), str(fn), str(k), str(t), str(dimtree)])
+    got = {}
+    for line in p.stdout.splitlines():
+        parts = line.split()
+        if parts and parts[0] in ("host_mu", "hals_host_loop", "hals_device_loop"):
+            got[parts[0]] = np.array([float(x) for x in parts[1:]])
+    F0 = [D.uniform(103, i, (d, k)) for i, d in enumerate(dims)]
+    X32 = X.astype(np.float32).astype(np.float64)
+    F32 = [f.astype(np.float32).astype(np.float64) for f in F0]
+    _, _, rel_mu, rc = O.ntf(0, X32, F32, t, dimtree=dimtree)
+    _, _, rel_h, rc2 = O.ntf(1, X32, F32, t, dimtree=dimtree)
+    assert rc == 0 and rc2 == 0
+    assert np.abs(got["host_mu"] - rel_mu).max() < 1e-5
+    assert np.abs(got["hals_host_loop"] - rel_h).max() < 1e-5
+    assert np.abs(got["hals_device_loop"] - rel_h).max() < 1e-5

@@ -612,3 +612,58 @@ def test_gnsym_first_iteration_matches_numpy_and_error_is_direct():
     assert abs(rel[0] - np.linalg.norm(A - H @ H.T) / np.linalg.norm(A)) < 1e-9
     _, _, rel20, _, _ = O.nmf(7, A, np.zeros((40, 4)), H0, 20)
     assert rel20[-1] < rel20[0]
+
+
+# ---- NTF-NES -------------------------------------------------------------------
+def test_sym_eig_minmax_matches_numpy():
+    rng = np.random.default_rng(70)
+    for k in (1, 2, 7, 33, 64):
+        B = rng.random((k + 5, k))
+        S = B.T @ B
+        mn, mx = O.sym_eig_minmax(S)
+        ev = np.linalg.eigvalsh(S)
+        assert abs(mn - ev[0]) <= 1e-12 * ev[-1] and abs(mx - ev[-1]) <= 1e-12 * ev[-1]
+
+
+def test_nes_mode_update_solves_the_proximal_nnls():
+    """The Nesterov inner loop stops once |grad o acc| <= 1e-2 and grad >=
+    -1e-2 (ntf/ntfnes.hpp:56-66): its result is the solution of the proximal
+    NNLS min 0.5 x^T (S + lambda I) x - (m + lambda h*)^T x, x >= 0, to that
+    tolerance; the BPP oracle (pinned against scipy) solves the same problem."""
+    rng = np.random.default_rng(71)
+    k, I = 5, 30
+    F = [rng.random((20, k)), rng.random((25, k))]
+    F = [f / np.linalg.norm(f, axis=0) for f in F]
+    S = (F[0].T @ F[0]) * (F[1].T @ F[1])
+    M = rng.random((k, I)) * 2
+    Hprox = rng.random((I, k))
+    H, rc = O.nes_mode_update(S, M, Hprox)
+    assert rc == 0
+    ev = np.linalg.eigvalsh(S)
+    q = ev[-1] / ev[0]
+    lam = 10 * ev[0] if q > 1e6 else (ev[0] if q > 1e3 else ev[0] / 10)
+    G = S + lam * np.eye(k)
+    R = M + lam * Hprox.T
+    X, it = O.bppnnls(G, R)
+    assert it >= 0
+    # (the stop test is taken at the extrapolated point, one step past the
+    # returned iterate, so the returned factor meets it only approximately)
+    grad = G @ H.T - R
+    assert np.all(H >= 0)
+    assert np.abs(grad * H.T).max() <= 5e-2 and grad.min() >= -5e-2
+    assert np.linalg.norm(H.T - X) / np.linalg.norm(X) < 0.05
+
+
+def test_nes_ntf_error_is_direct_and_dimtree_equals_direct():
+    X, _ = D.cp_tensor((9, 8, 7), 3, 0.01, stream=72)
+    F0 = D.init_factors([(9, 3), (8, 3), (7, 3)], stream=72)
+    Fa, la, ra, rc = O.ntf(5, X, F0, 12, dimtree=1)
+    Fb, lb, rb, rc2 = O.ntf(5, X, F0, 12, dimtree=0)
+    assert rc == 0 and rc2 == 0
+    assert np.allclose(ra, rb, rtol=1e-9)
+    Xh = np.einsum("az,bz,cz->abc", Fa[0] * la, Fa[1], Fa[2])
+    # after iteration 6 the factors may be the accelerated ones (accepted
+    # steps lower the error), so the returned factors reproduce an error no
+    # larger than the last recorded one
+    assert np.linalg.norm(X - Xh) / np.linalg.norm(X) <= ra[-1] + 1e-12
+    assert ra[-1] < ra[0]
