@@ -387,6 +387,11 @@ int planc_create(int device, planc_ctx **out) {
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
     c->num_cu = prop.multiProcessorCount;
+  // PLANC_THIN_PLANES=2: every 16-bit product carries its thin operand as two
+  // 16-bit planes (NTF MTTKRPs and ANLS-BPP always do); unset or 1: one plane
+  // for NMF-MU / HALS and the GEMM entry points.
+  const char *tp = getenv("PLANC_THIN_PLANES");
+  if (tp && tp[0] == '2') c->thin_planes = 2;
   // Locates this device's copy of the allocation-free CP-loss partials; a
   // failure is reported by planc_cp_loss_fwd_noalloc, not here.
   (void)cp_noalloc_setup(device);
@@ -560,6 +565,7 @@ int planc_mttkrp(planc_ctx *ctx, int dtype, const void *d_X, int nmodes, const l
     return set_error(ctx, PLANC_EINVAL, "planc_mttkrp: bad arguments");
   for (int i = 0; i < nmodes; ++i)
     if (dims[i] <= 0) return set_error(ctx, PLANC_EINVAL, "planc_mttkrp: empty mode");
+  ThinPlanes planes(ctx, dtype != PLANC_F32 ? 2 : 1);  // as in planc_ntf_run
   float *P = nullptr;
   if (mode > 0 && mode < nmodes - 1) {
     long Lp = 1;
@@ -577,6 +583,7 @@ int planc_mttkrp_partial(planc_ctx *ctx, int dtype, const void *d_X, int nmodes,
   if (!valid_dtype(dtype) || nmodes < 2 || nmodes > 7 || !dims || !d_factors || split < 0 ||
       split > nmodes - 2 || (side != 0 && side != 1) || k <= 0 || !d_X || !d_out)
     return set_error(ctx, PLANC_EINVAL, "planc_mttkrp_partial: bad arguments");
+  ThinPlanes planes(ctx, dtype != PLANC_F32 ? 2 : 1);  // as in planc_ntf_run
   long L = 1, R = 1;
   for (int i = 0; i < nmodes; ++i) {
     if (dims[i] <= 0) return set_error(ctx, PLANC_EINVAL, "planc_mttkrp_partial: empty mode");
@@ -676,35 +683,12 @@ int planc_nmf_run(planc_ctx *ctx, const planc_nmf_desc *desc, const float *d_A, 
     return thin_gemm(ctx, dt, 1, Aop, ldop, n, m, F, m, nullptr, nullptr, cs, k, out, k, 1, 0);
   };
   prod.ah = a_times_h;
-  if (dt != PLANC_F32 && algo == PLANC_ANLSBPP) {
-    // ANLS-BPP solves normal equations whose Gram comes from the unrounded
-    // factor; a right-hand side from the 16-bit-rounded factor would make them
-    // inconsistent, an error that the solve amplifies by cond(Gram).  The
-    // factor is therefore split into a 16-bit part and its remainder and both
-    // products are summed: only the rounding of A remains.
-    float *split = (float *)ws_get(ctx, 20, sizeof(float) * (2 * (size_t)std::max(m, n) * k +
-                                                            (size_t)std::max((long)k * n, m * (long)k)));
-    if (!split) return set_error(ctx, PLANC_ENOMEM, "planc_nmf_run: split factor");
-    const size_t fs = (size_t)std::max(m, n) * k;
-    float *hi = split, *lo = split + fs, *tmp = split + 2 * fs;
-    prod.wta = [&, hi, lo, tmp](const float *F, float *out) -> int {
-      int r = split16(ctx, dt, F, m * k, hi, lo);
-      if (!r) r = thin_gemm(ctx, dt, 1, Aop, ldop, n, m, hi, m, nullptr, nullptr, cs, k, out, k, 1, 0);
-      if (!r) r = thin_gemm(ctx, dt, 1, Aop, ldop, n, m, lo, m, nullptr, nullptr, cs, k, tmp, k, 1, 0);
-      if (!r) r = add_inplace(ctx, out, tmp, (long)k * n);
-      return r;
-    };
-    prod.ah = [&, hi, lo, tmp](const float *F, float *out) -> int {
-      int r = split16(ctx, dt, F, n * k, hi, lo);
-      const void *Aa = AopT ? AopT : Aop;
-      const int tr = AopT ? 1 : 0;
-      const long la = AopT ? n : ldop;
-      if (!r) r = thin_gemm(ctx, dt, tr, Aa, la, m, n, hi, n, nullptr, cs, nullptr, k, out, m, 0, 0);
-      if (!r) r = thin_gemm(ctx, dt, tr, Aa, la, m, n, lo, n, nullptr, cs, nullptr, k, tmp, m, 0, 0);
-      if (!r) r = add_inplace(ctx, out, tmp, m * (long)k);
-      return r;
-    };
-  }
+  // ANLS-BPP solves normal equations whose Gram comes from the unrounded
+  // factor; a right-hand side from the 16-bit-rounded factor would make them
+  // inconsistent, an error that the solve amplifies by cond(Gram).  The
+  // factor therefore enters the 16-bit products as two 16-bit planes (its
+  // rounding and the rounded remainder): only the rounding of A remains.
+  ThinPlanes planes(ctx, dt != PLANC_F32 && algo == PLANC_ANLSBPP ? 2 : ctx->thin_planes);
   prod.norma2 = [&](double *out) -> int { return dot(ctx, d_A, 1, lda, d_A, 1, lda, m, n, out, 0); };
   return nmf_loop(ctx, desc, prod, d_W, d_H, h_relerr);
 }
@@ -807,6 +791,10 @@ int planc_ntf_run(planc_ctx *ctx, const planc_ntf_desc *desc, const float *d_X, 
       algo != PLANC_NESTEROV)
     return set_error(ctx, PLANC_EUNSUPPORTED, "planc_ntf_run: algorithm not supported");
   PLANC_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  // 16-bit MTTKRPs: the factors / Khatri-Rao rows enter as two 16-bit planes,
+  // so only X's rounding remains (a one-plane KRP error is amplified by the
+  // mode updates several times over).
+  ThinPlanes planes(ctx, dt != PLANC_F32 ? 2 : 1);
   const long *dims = desc->dims;
   long numel = 1, Imax = 0;
   for (int i = 0; i < N; ++i) {

@@ -99,6 +99,10 @@ struct planc_ctx {
   planc::Workspace ws[40];
   int num_cu = 256;
   int max_coop_blocks_hals = 0;
+  // 2: 16-bit products carry the thin operand as two 16-bit planes (hi and
+  // the rounded remainder lo), so only the big operand is rounded; 1: one
+  // plane.  Set per run by ThinPlanes.
+  int thin_planes = 1;
   // Tuned streaming-product geometries: shape key -> {cw, wk, nsplit, dual, shared}.
   std::vector<std::pair<std::vector<long>, std::vector<long>>> gemm_tuned;
 };
@@ -113,13 +117,23 @@ namespace planc {
 // 13 CP-gradient float buffers, 14 CP-gradient Grams, 15 loop-graph counter,
 // 16 transposed 16-bit copy of A / X, 17 HALS grid-barrier counter and flag,
 // 18 fused NTF mode-update partials, 19/21 NTF mode-exchange slots and
-// control, 20 split 16-bit factors of ANLS-BPP, 22 AO-ADMM factor / partials,
+// control, 20 (unused), 22 AO-ADMM factor / partials,
 // 23 GNSym CG vectors, 24..31 sparse path (sparse.hip: k-major factor
 // copies, error partials, transposed CSR, sort keys / entries / temporary
 // storage), 32 NTF AO-ADMM duals, 33 NMF AO-ADMM duals, 34 dual norms,
 // 35 NES iterates, 36 NES factors (centres, previous, accelerated), 37 NES
 // lambdas.
 void *ws_get(planc_ctx *ctx, int slot, size_t bytes);
+
+// Scoped choice of ctx->thin_planes (restored on exit).
+struct ThinPlanes {
+  planc_ctx *ctx;
+  int saved;
+  ThinPlanes(planc_ctx *c, int np) : ctx(c), saved(c->thin_planes) { c->thin_planes = np; }
+  ~ThinPlanes() { ctx->thin_planes = saved; }
+  ThinPlanes(const ThinPlanes &) = delete;
+  ThinPlanes &operator=(const ThinPlanes &) = delete;
+};
 
 int set_error(planc_ctx *ctx, int code, const std::string &msg);
 
@@ -201,8 +215,6 @@ int nmf_relerr(planc_ctx *ctx, const double *normA2, const double *dotv, const d
                const double *HtH, int k, double *relerr_out, double *fit_out);
 int apply_reg(planc_ctx *ctx, double *G, int k, const double *reg);
 // hi = X rounded to dtype's 16-bit format (as float), lo = X - hi; Y += X.
-int split16(planc_ctx *ctx, int dtype, const float *X, long n, float *hi, float *lo);
-int add_inplace(planc_ctx *ctx, float *Y, const float *X, long n);
 // G += sr I, R(j, c) += sr F(j, c) (R element (j, c) at R[j*rs + c*cs]); no-op for sr <= 0.
 int symm_reg_apply(planc_ctx *ctx, double *G, int k, float *R, long rs, long cs, const float *F,
                    long n, double sr);

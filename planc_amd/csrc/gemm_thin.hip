@@ -142,6 +142,17 @@ __device__ inline uint16_t to16<f16_t>(float x) {
 }
 
 template <typename TA>
+__device__ inline float from16(uint16_t h);
+template <>
+__device__ inline float from16<bf16_t>(uint16_t h) {
+  return bf16_to_float(h);
+}
+template <>
+__device__ inline float from16<f16_t>(uint16_t h) {
+  return f16_to_float(h);
+}
+
+template <typename TA>
 __device__ inline floatx4 mfma16(const uint4 &a, const uint4 &b, floatx4 c);
 template <>
 __device__ inline floatx4 mfma16<bf16_t>(const uint4 &a, const uint4 &b, floatx4 c) {
@@ -163,7 +174,9 @@ struct StreamParams {
   const void *A;
   long lda, L, K;
   ThinDev th;
-  const uint4 *thin16;  // packed 16-bit thin operand (16-bit TN)
+  const uint4 *thin16;  // packed 16-bit thin operand (16-bit TN / NN)
+  long plane;           // words of one packed plane; plane 1 (NP == 2) holds the low halves
+  int np;               // packed planes: 1, or 2 for the compensated thin operand
   const float *colscale;
   float *C;
   long ldc;
@@ -663,8 +676,12 @@ __device__ inline long pack_index(long kk, int t, int lc, int NT) {
   return ((2 * S + ((rem >> 3) & 1)) * NT + t) * 64 + lc + 16 * (rem >> 4);
 }
 
-template <typename TA>
-__global__ void pack_thin16_kernel(ThinDev th, long K, int NT, long nsuper, uint4 *out) {
+// NP == 2: compensated operand.  Plane 0 holds hi = rn16(b), plane 1 (at
+// word offset `plane`) lo = rn16(b - hi), so hi + lo carries b to about 16
+// (BF16) or 22 (FP16) significant bits and the products, with both planes
+// contracted against the same big-operand words, round only the big operand.
+template <typename TA, int NP>
+__global__ void pack_thin16_kernel(ThinDev th, long K, int NT, long nsuper, uint4 *out, long plane) {
   const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nsuper * 2 * NT * 64) return;
   const int lane = (int)(e & 63);
@@ -681,19 +698,30 @@ __global__ void pack_thin16_kernel(ThinDev th, long K, int NT, long nsuper, uint
     h[4 + u] = to16<TA>(hi[u]);
   }
   out[e] = __builtin_bit_cast(uint4, h);
+  if constexpr (NP == 2) {
+    uint16_t r[8];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      r[u] = to16<TA>(lo[u] - from16<TA>(h[u]));
+      r[4 + u] = to16<TA>(hi[u] - from16<TA>(h[4 + u]));
+    }
+    out[plane + e] = __builtin_bit_cast(uint4, r);
+  }
 }
 
 // 16-bit TN.  DUAL = 0: step = 32 k, a lane reads 8 consecutive k of a
 // column (16 bytes); DUAL = 1: step = 64 k, a lane reads 16 consecutive k
 // (two 16-byte words, a full 128-byte line per column across the four lane
 // groups) feeding two MFMAs.  SAFE: scalar loads of A (unaligned lda/base).
-template <int NT, int CW, int NS>
+template <int NT, int CW, int NS, int NP>
 struct BufH {
   uint4 x[CW][NS];
-  uint4 a[NT][NS];
+  uint4 a[NP][NT][NS];
 };
 
-template <typename TA, int NT, int CW, int SAFE, int DUAL>
+// NP == 2: the compensated thin operand (two planes, both contracted with
+// the same A words into one accumulator).
+template <typename TA, int NT, int CW, int SAFE, int DUAL, int NP>
 __global__ __launch_bounds__(NTHR) void tn_h_kernel(StreamParams p) {
   constexpr int NS = DUAL ? 2 : 1, STEP = 32 * NS, PD = DUAL ? 2 : 3, NA = CW * NT;
   constexpr int LW = 8 * NS;  // k per lane per step
@@ -715,22 +743,26 @@ __global__ __launch_bounds__(NTHR) void tn_h_kernel(StreamParams p) {
   floatx4 acc[NA];
 #pragma unroll
   for (int a = 0; a < NA; ++a) acc[a] = floatx4{0.f, 0.f, 0.f, 0.f};
-  auto mma = [&](BufH<NT, CW, NS> &b) {
+  auto mma = [&](BufH<NT, CW, NS, NP> &b) {
 #pragma unroll
     for (int h = 0; h < NS; ++h)
 #pragma unroll
-      for (int t = 0; t < NT; ++t)
+      for (int q = 0; q < NP; ++q)
 #pragma unroll
-        for (int cw = 0; cw < CW; ++cw)
-          acc[cw * NT + t] = mfma16<TA>(b.a[t][h], b.x[cw][h], acc[cw * NT + t]);
+        for (int t = 0; t < NT; ++t)
+#pragma unroll
+          for (int cw = 0; cw < CW; ++cw)
+            acc[cw * NT + t] = mfma16<TA>(b.a[q][t][h], b.x[cw][h], acc[cw * NT + t]);
   };
-  auto load_thin = [&](long kb, BufH<NT, CW, NS> &b) {
+  auto load_thin = [&](long kb, BufH<NT, CW, NS, NP> &b) {
 #pragma unroll
-    for (int h = 0; h < NS; ++h)
+    for (int q = 0; q < NP; ++q)
 #pragma unroll
-      for (int t = 0; t < NT; ++t) b.a[t][h] = p.thin16[pack_index(kb + 8 * h, t, lc, NT)];
+      for (int h = 0; h < NS; ++h)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) b.a[q][t][h] = p.thin16[q * p.plane + pack_index(kb + 8 * h, t, lc, NT)];
   };
-  auto load = [&](long i, BufH<NT, CW, NS> &b) {
+  auto load = [&](long i, BufH<NT, CW, NS, NP> &b) {
     const long s = kph + i * p.wk;
     const long kb = kbeg + s * STEP + LW * lg;
 #pragma unroll
@@ -748,11 +780,11 @@ __global__ __launch_bounds__(NTHR) void tn_h_kernel(StreamParams p) {
       }
     load_thin(kb, b);
   };
-  BufH<NT, CW, NS> buf[PD];
+  BufH<NT, CW, NS, NP> buf[PD];
   const long nsw = nfull > kph ? (nfull - kph + p.wk - 1) / p.wk : 0;
   stream_steps<PD>(nsw, buf, load, mma);
   if (kend - kbeg > nfull * STEP && nfull % p.wk == kph) {
-    BufH<NT, CW, NS> b;
+    BufH<NT, CW, NS, NP> b;
     const long kb = kbeg + nfull * STEP + LW * lg;
 #pragma unroll
     for (int cw = 0; cw < CW; ++cw)
@@ -1109,13 +1141,15 @@ __global__ __launch_bounds__(NTHR) void tn_f32_sh_kernel(StreamParams p) {
 // 16-bit TN (64-k steps, two 16-byte words per lane and column as the DUAL
 // tn_h_kernel) with the block-shared thin operand: the packed words of one
 // step (2 halves x NT tiles x 64 lanes, contiguous in the packed buffer) are
-// copied into the LDS stage by the block's threads (two words per thread
-// beyond 256).
-template <typename TA, int NT, int CW>
+// copied into the LDS stage by the block's threads (up to four words per
+// thread: two planes of the compensated operand at NT = 4).
+template <typename TA, int NT, int CW, int NP>
 __global__ __launch_bounds__(NTHR) void tn_h_sh_kernel(StreamParams p) {
   constexpr int NS = 2, STEP = 64, PD = 2, NA = CW * NT;
   constexpr int LW = 8 * NS;
-  constexpr int WORDS = NS * NT * 64;  // <= 2 * NTHR
+  constexpr int WORDS1 = NS * NT * 64;            // words of one plane per step
+  constexpr int WORDS = NP * WORDS1;              // <= 4 * NTHR
+  constexpr int RW = (WORDS + NTHR - 1) / NTHR;  // words per thread
   __shared__ uint4 ts[SH_TS][WORDS];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int lc = lane & 15, lg = lane >> 4;
@@ -1136,10 +1170,13 @@ __global__ __launch_bounds__(NTHR) void tn_h_sh_kernel(StreamParams p) {
   struct BufA {
     uint4 x[CW][NS];
   };
-  // Words threadIdx.x and threadIdx.x + NTHR of a step, as one 32-byte vector.
-  typedef unsigned RawH __attribute__((ext_vector_type(8)));
-  // Step s starts at word (kbeg / STEP + s) * WORDS of the packed buffer.
-  const uint4 *pk = p.thin16 + (kbeg / STEP) * (long)WORDS;
+  // Words threadIdx.x + j NTHR (j < RW) of a step; LDS word q WORDS1 + i
+  // holds word i of plane q.
+  struct RawH {
+    uint4 w[RW];
+  };
+  // Step s starts at word (kbeg / STEP + s) * WORDS1 of each packed plane.
+  const uint4 *pk = p.thin16 + (kbeg / STEP) * (long)WORDS1;
   auto loadA = [&](long s, BufA &b) {
     const long kb = kbeg + s * STEP + LW * lg;
 #pragma unroll
@@ -1147,27 +1184,36 @@ __global__ __launch_bounds__(NTHR) void tn_h_sh_kernel(StreamParams p) {
 #pragma unroll
       for (int h = 0; h < NS; ++h) b.x[cw][h] = *(const uint4 *)(col[cw] + kb + 8 * h);
   };
-  const int w0 = threadIdx.x, w1 = threadIdx.x + NTHR;
   auto rload = [&](long s, RawH &r) {
-    uint4 a = {0u, 0u, 0u, 0u}, b = {0u, 0u, 0u, 0u};
-    if (w0 < WORDS) a = pk[s * WORDS + w0];
-    if (WORDS > NTHR && w1 < WORDS) b = pk[s * WORDS + w1];
-    r = RawH{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+    for (int j = 0; j < RW; ++j) {
+      const int w = threadIdx.x + j * NTHR;
+      r.w[j] = uint4{0u, 0u, 0u, 0u};
+      if (w < WORDS) {
+        const int q = w >= WORDS1 ? 1 : 0;
+        r.w[j] = pk[q * p.plane + s * WORDS1 + (w - q * WORDS1)];
+      }
+    }
   };
   auto rstore = [&](const RawH &r, int stage) {
-    if (w0 < WORDS) ts[stage][w0] = uint4{r[0], r[1], r[2], r[3]};
-    if (WORDS > NTHR && w1 < WORDS) ts[stage][w1] = uint4{r[4], r[5], r[6], r[7]};
+#pragma unroll
+    for (int j = 0; j < RW; ++j) {
+      const int w = threadIdx.x + j * NTHR;
+      if (w < WORDS) ts[stage][w] = r.w[j];
+    }
   };
   auto mma = [&](BufA &b, int stage) {
     const uint4 *st = ts[stage] + lane;
 #pragma unroll
     for (int h = 0; h < NS; ++h)
 #pragma unroll
-      for (int t = 0; t < NT; ++t) {
-        const uint4 a = st[(h * NT + t) * 64];
+      for (int q = 0; q < NP; ++q)
 #pragma unroll
-        for (int cw = 0; cw < CW; ++cw) acc[cw * NT + t] = mfma16<TA>(a, b.x[cw][h], acc[cw * NT + t]);
-      }
+        for (int t = 0; t < NT; ++t) {
+          const uint4 a = st[q * WORDS1 + (h * NT + t) * 64];
+#pragma unroll
+          for (int cw = 0; cw < CW; ++cw) acc[cw * NT + t] = mfma16<TA>(a, b.x[cw][h], acc[cw * NT + t]);
+        }
   };
   BufA buf[PD];
   RawH raw[PD];
@@ -1190,11 +1236,13 @@ __global__ __launch_bounds__(NTHR) void tn_h_sh_kernel(StreamParams p) {
 #pragma unroll
     for (int h = 0; h < NS; ++h)
 #pragma unroll
-      for (int t = 0; t < NT; ++t) {
-        const uint4 a = p.thin16[pack_index(kb + 8 * h, t, lc, NT)];
+      for (int q = 0; q < NP; ++q)
 #pragma unroll
-        for (int cw = 0; cw < CW; ++cw) acc[cw * NT + t] = mfma16<TA>(a, b.x[cw][h], acc[cw * NT + t]);
-      }
+        for (int t = 0; t < NT; ++t) {
+          const uint4 a = p.thin16[q * p.plane + pack_index(kb + 8 * h, t, lc, NT)];
+#pragma unroll
+          for (int cw = 0; cw < CW; ++cw) acc[cw * NT + t] = mfma16<TA>(a, b.x[cw][h], acc[cw * NT + t]);
+        }
   }
   if (p.nsplit > 1) {
     floatx4 *dst = (floatx4 *)p.part + (((long)blockIdx.y * gridDim.x + blockIdx.x) * 4 + wave) * (long)NA * 64;
@@ -1215,10 +1263,10 @@ __global__ __launch_bounds__(NTHR) void tn_h_sh_kernel(StreamParams p) {
 // packed 16-bit B-operand (pack_thin16_kernel, as for 16-bit TN).  acc[8t +
 // j][v] = C(l0 + 32lg + 8v + j, 16t + lc): every lane ends with 32
 // consecutive rows of its column.  One k phase per group (wk == 1).
-template <int NT>
+template <int NT, int NP>
 struct BufNN16 {
   uint4 w[8];
-  uint4 b[NT];
+  uint4 b[NP][NT];
 };
 
 __device__ inline uint4 transpose8_word(const uint4 (&w)[8], int j) {
@@ -1237,9 +1285,10 @@ __device__ inline uint4 transpose8_word(const uint4 (&w)[8], int j) {
 }
 
 // SH: the thin words of each step staged once per block in an LDS ring
-// (stream_steps_shared; the block's four waves share the k range).
+// (stream_steps_shared; the block's four waves share the k range).  NP == 2:
+// the compensated thin operand (both planes against the same A words).
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
-template <typename TA, int NT, int SH>
+template <typename TA, int NT, int SH, int NP>
 __global__ __launch_bounds__(NTHR) void nn16_kernel(StreamParams p) {
   constexpr int STEP = 32, PD = 2, NA = 8 * NT;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1254,63 +1303,87 @@ __global__ __launch_bounds__(NTHR) void nn16_kernel(StreamParams p) {
   floatx4 acc[NA];
 #pragma unroll
   for (int a = 0; a < NA; ++a) acc[a] = floatx4{0.f, 0.f, 0.f, 0.f};
-  auto mma = [&](BufNN16<NT> &b) {
+  auto mma = [&](BufNN16<NT, NP> &b) {
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       const uint4 o = transpose8_word(b.w, j);
 #pragma unroll
-      for (int t = 0; t < NT; ++t) acc[8 * t + j] = mfma16<TA>(o, b.b[t], acc[8 * t + j]);
+      for (int q = 0; q < NP; ++q)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[8 * t + j] = mfma16<TA>(o, b.b[q][t], acc[8 * t + j]);
     }
   };
-  auto load = [&](long i, BufNN16<NT> &b) {
+  auto load = [&](long i, BufNN16<NT, NP> &b) {
     const long kb = kbeg + i * STEP + 8 * lg;
 #pragma unroll
     for (int u = 0; u < 8; ++u) b.w[u] = *(const uint4 *)(base + p.lda * (kb + u));
 #pragma unroll
-    for (int t = 0; t < NT; ++t) b.b[t] = p.thin16[pack_index(kb, t, lc, NT)];
+    for (int q = 0; q < NP; ++q)
+#pragma unroll
+      for (int t = 0; t < NT; ++t) b.b[q][t] = p.thin16[q * p.plane + pack_index(kb, t, lc, NT)];
   };
   if constexpr (SH) {
-    __shared__ u32x4 ts[SH_TS][NT * 64];
+    // LDS word (q NT + t) 64 + lane: plane q, tile t of the step; producer
+    // thread tid + 256 j (j < RW) fills word tid + 256 j.
+    constexpr int WORDS = NP * NT * 64, RW = (WORDS + NTHR - 1) / NTHR;
+    __shared__ u32x4 ts[SH_TS][WORDS];
     struct BufA {
       uint4 w[8];
     };
+    struct RawN {
+      u32x4 w[RW];
+    };
     const int tid = threadIdx.x;
-    const bool prod = tid < NT * 64;
-    const int pt = prod ? tid >> 6 : 0, pl = tid & 63;
     auto loadA = [&](long i, BufA &b) {
       const long kb = kbeg + i * STEP + 8 * lg;
 #pragma unroll
       for (int u = 0; u < 8; ++u) b.w[u] = *(const uint4 *)(base + p.lda * (kb + u));
     };
-    auto rload = [&](long i, u32x4 &r) {
-      if (prod)
-        r = __builtin_bit_cast(u32x4, p.thin16[pack_index(kbeg + i * STEP + 8 * (pl >> 4), pt, pl & 15, NT)]);
+    auto rload = [&](long i, RawN &r) {
+#pragma unroll
+      for (int j = 0; j < RW; ++j) {
+        const int w = tid + j * NTHR;
+        if (w < WORDS) {
+          const int qt = w >> 6, pl = w & 63;
+          const int q = qt / NT, pt = qt - q * NT;
+          r.w[j] = __builtin_bit_cast(
+              u32x4, p.thin16[q * p.plane + pack_index(kbeg + i * STEP + 8 * (pl >> 4), pt, pl & 15, NT)]);
+        }
+      }
     };
-    auto rstore = [&](const u32x4 &r, int stage) {
-      if (prod) ts[stage][tid] = r;
+    auto rstore = [&](const RawN &r, int stage) {
+#pragma unroll
+      for (int j = 0; j < RW; ++j) {
+        const int w = tid + j * NTHR;
+        if (w < WORDS) ts[stage][w] = r.w[j];
+      }
     };
     auto mma_sh = [&](BufA &b, int stage) {
-      uint4 bt[NT];
+      uint4 bt[NP][NT];
 #pragma unroll
-      for (int t = 0; t < NT; ++t) bt[t] = __builtin_bit_cast(uint4, ts[stage][t * 64 + lane]);
+      for (int q = 0; q < NP; ++q)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) bt[q][t] = __builtin_bit_cast(uint4, ts[stage][(q * NT + t) * 64 + lane]);
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
         const uint4 o = transpose8_word(b.w, j);
 #pragma unroll
-        for (int t = 0; t < NT; ++t) acc[8 * t + j] = mfma16<TA>(o, bt[t], acc[8 * t + j]);
+        for (int q = 0; q < NP; ++q)
+#pragma unroll
+          for (int t = 0; t < NT; ++t) acc[8 * t + j] = mfma16<TA>(o, bt[q][t], acc[8 * t + j]);
       }
     };
     BufA buf[PD];
-    u32x4 raw[PD];
+    RawN raw[PD];
     stream_steps_shared<PD, SH_TS>(nfull, buf, raw, loadA, rload, rstore, mma_sh);
   } else {
-    BufNN16<NT> buf[PD];
+    BufNN16<NT, NP> buf[PD];
     stream_steps<PD>(nfull, buf, load, mma);
   }
   if (kend - kbeg > nfull * STEP) {
     // The partial last step, masked (element loads; the packed thin operand
     // is zero past K).
-    BufNN16<NT> b;
+    BufNN16<NT, NP> b;
     const long kb = kbeg + nfull * STEP + 8 * lg;
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
@@ -1320,7 +1393,9 @@ __global__ __launch_bounds__(NTHR) void nn16_kernel(StreamParams p) {
       b.w[u] = __builtin_bit_cast(uint4, v);
     }
 #pragma unroll
-    for (int t = 0; t < NT; ++t) b.b[t] = p.thin16[pack_index(kb, t, lc, NT)];
+    for (int q = 0; q < NP; ++q)
+#pragma unroll
+      for (int t = 0; t < NT; ++t) b.b[q][t] = p.thin16[q * p.plane + pack_index(kb, t, lc, NT)];
     mma(b);
   }
   if (p.nsplit > 1) {
@@ -1376,13 +1451,15 @@ struct GemmParams {
   long kchunk;
 };
 
-template <typename TA, int NT, bool VEC>
+// NP == 2: the compensated thin operand (pack_thin16_kernel), formed here
+// while staging B.
+template <typename TA, int NT, bool VEC, int NP>
 __global__ __launch_bounds__(NTHR) void nn_h_kernel(GemmParams p) {
   constexpr int BK = 64, SA = BK + 8, KP = 16 * NT;
   constexpr int BGROUPS = (BK / 8) * KP;  // groups of 8 consecutive kk of one column
   constexpr int BITER = (BGROUPS + NTHR - 1) / NTHR;
   __shared__ __attribute__((aligned(16))) uint16_t As[2][BL][SA];
-  __shared__ __attribute__((aligned(16))) uint16_t Bs[2][KP][SA];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[NP][2][KP][SA];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const long l0 = (long)blockIdx.x * BL;
   const int split = blockIdx.y;
@@ -1392,7 +1469,7 @@ __global__ __launch_bounds__(NTHR) void nn_h_kernel(GemmParams p) {
   const uint16_t *A = (const uint16_t *)p.A;
 
   uint4 ra[4];
-  uint4 rb[BITER];
+  uint4 rb[NP][BITER];
 
   auto load_A = [&](long k0) {
     // 8 rows x 4 kk per thread: 4 loads of 8 consecutive rows of one column.
@@ -1435,7 +1512,16 @@ __global__ __launch_bounds__(NTHR) void nn_h_kernel(GemmParams p) {
           h[e] = to16<TA>(lo[e]);
           h[4 + e] = to16<TA>(hi[e]);
         }
-        rb[v] = __builtin_bit_cast(uint4, h);
+        rb[0][v] = __builtin_bit_cast(uint4, h);
+        if constexpr (NP == 2) {
+          uint16_t r[8];
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            r[e] = to16<TA>(lo[e] - from16<TA>(h[e]));
+            r[4 + e] = to16<TA>(hi[e] - from16<TA>(h[4 + e]));
+          }
+          rb[NP - 1][v] = __builtin_bit_cast(uint4, r);
+        }
       }
     }
   };
@@ -1445,7 +1531,8 @@ __global__ __launch_bounds__(NTHR) void nn_h_kernel(GemmParams p) {
       int g = tid + NTHR * v;
       if (g < BGROUPS) {
         int c = g / (BK / 8), kk = (g % (BK / 8)) * 8;
-        *(uint4 *)&Bs[buf][c][kk] = rb[v];
+#pragma unroll
+        for (int q = 0; q < NP; ++q) *(uint4 *)&Bs[q][buf][c][kk] = rb[q][v];
       }
     }
   };
@@ -1476,11 +1563,13 @@ __global__ __launch_bounds__(NTHR) void nn_h_kernel(GemmParams p) {
       uint4 a0 = *(const uint4 *)&As[buf][wave * 32 + (lane & 15)][kk];
       uint4 a1 = *(const uint4 *)&As[buf][wave * 32 + 16 + (lane & 15)][kk];
 #pragma unroll
-      for (int t = 0; t < NT; ++t) {
-        uint4 b = *(const uint4 *)&Bs[buf][16 * t + (lane & 15)][kk];
-        acc[0][t] = mfma16<TA>(a0, b, acc[0][t]);
-        acc[1][t] = mfma16<TA>(a1, b, acc[1][t]);
-      }
+      for (int q = 0; q < NP; ++q)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+          uint4 b = *(const uint4 *)&Bs[q][buf][16 * t + (lane & 15)][kk];
+          acc[0][t] = mfma16<TA>(a0, b, acc[0][t]);
+          acc[1][t] = mfma16<TA>(a1, b, acc[1][t]);
+        }
     }
     if (more) {
       store_A(buf ^ 1);
@@ -1678,14 +1767,14 @@ void launch_nn_f32(int tm, dim3 grid, hipStream_t st, const StreamParams &p) {
   else
     nn_f32_kernel<NT, THIN_SAFE><<<grid, NTHR, 0, st>>>(p);
 }
-template <typename TA, int NT>
+template <typename TA, int NT, int NP>
 void launch_tn_h(bool safe, int dual, dim3 grid, hipStream_t st, const StreamParams &p) {
   if (safe)
-    tn_h_kernel<TA, NT, 4, 1, 0><<<grid, NTHR, 0, st>>>(p);
+    tn_h_kernel<TA, NT, 4, 1, 0, NP><<<grid, NTHR, 0, st>>>(p);
   else if (dual)
-    tn_h_kernel<TA, NT, 4, 0, 1><<<grid, NTHR, 0, st>>>(p);
+    tn_h_kernel<TA, NT, 4, 0, 1, NP><<<grid, NTHR, 0, st>>>(p);
   else
-    tn_h_kernel<TA, NT, 4, 0, 0><<<grid, NTHR, 0, st>>>(p);
+    tn_h_kernel<TA, NT, 4, 0, 0, NP><<<grid, NTHR, 0, st>>>(p);
 }
 template <int NT, int CW>
 void launch_tn_f32_sh(int tm, dim3 grid, hipStream_t st, const StreamParams &p) {
@@ -1701,12 +1790,55 @@ void launch_nn_f32_sh(int tm, dim3 grid, hipStream_t st, const StreamParams &p) 
   else
     nn_f32_sh_kernel<NT, THIN_KRP2><<<grid, NTHR, 0, st>>>(p);
 }
-template <typename TA, int CW>
+template <typename TA, int CW, int NP>
 void launch_tn_h_sh(int NT, dim3 grid, hipStream_t st, const StreamParams &p) {
-  if (NT == 1) tn_h_sh_kernel<TA, 1, CW><<<grid, NTHR, 0, st>>>(p);
-  else if (NT == 2) tn_h_sh_kernel<TA, 2, CW><<<grid, NTHR, 0, st>>>(p);
-  else if (NT == 3) tn_h_sh_kernel<TA, 3, CW><<<grid, NTHR, 0, st>>>(p);
-  else tn_h_sh_kernel<TA, 4, CW><<<grid, NTHR, 0, st>>>(p);
+  if (NT == 1) tn_h_sh_kernel<TA, 1, CW, NP><<<grid, NTHR, 0, st>>>(p);
+  else if (NT == 2) tn_h_sh_kernel<TA, 2, CW, NP><<<grid, NTHR, 0, st>>>(p);
+  else if (NT == 3) tn_h_sh_kernel<TA, 3, CW, NP><<<grid, NTHR, 0, st>>>(p);
+  else tn_h_sh_kernel<TA, 4, CW, NP><<<grid, NTHR, 0, st>>>(p);
+}
+template <int NP>
+void launch_nn16(int dtype, int NT, int shared, dim3 grid, hipStream_t st, const StreamParams &p) {
+#define NN16(TA, SHV)                                                        \
+  do {                                                                       \
+    if (NT == 1) nn16_kernel<TA, 1, SHV, NP><<<grid, NTHR, 0, st>>>(p);      \
+    else if (NT == 2) nn16_kernel<TA, 2, SHV, NP><<<grid, NTHR, 0, st>>>(p); \
+    else if (NT == 3) nn16_kernel<TA, 3, SHV, NP><<<grid, NTHR, 0, st>>>(p); \
+    else nn16_kernel<TA, 4, SHV, NP><<<grid, NTHR, 0, st>>>(p);              \
+  } while (0)
+  if (dtype == PLANC_BF16) {
+    if (shared) NN16(bf16_t, 1);
+    else NN16(bf16_t, 0);
+  } else {
+    if (shared) NN16(f16_t, 1);
+    else NN16(f16_t, 0);
+  }
+#undef NN16
+}
+template <int NP>
+void launch_tn16(int dtype, int NT, const Geometry &g, dim3 grid, hipStream_t st, const StreamParams &p) {
+  if (g.shared) {
+    if (dtype == PLANC_BF16) {
+      if (g.cw == 4) launch_tn_h_sh<bf16_t, 4, NP>(NT, grid, st, p);
+      else launch_tn_h_sh<bf16_t, 2, NP>(NT, grid, st, p);
+    } else {
+      if (g.cw == 4) launch_tn_h_sh<f16_t, 4, NP>(NT, grid, st, p);
+      else launch_tn_h_sh<f16_t, 2, NP>(NT, grid, st, p);
+    }
+    return;
+  }
+  const bool safe = !p.avec;
+  if (dtype == PLANC_BF16) {
+    if (NT == 1) launch_tn_h<bf16_t, 1, NP>(safe, g.dual, grid, st, p);
+    else if (NT == 2) launch_tn_h<bf16_t, 2, NP>(safe, g.dual, grid, st, p);
+    else if (NT == 3) launch_tn_h<bf16_t, 3, NP>(safe, g.dual, grid, st, p);
+    else launch_tn_h<bf16_t, 4, NP>(safe, g.dual, grid, st, p);
+  } else {
+    if (NT == 1) launch_tn_h<f16_t, 1, NP>(safe, g.dual, grid, st, p);
+    else if (NT == 2) launch_tn_h<f16_t, 2, NP>(safe, g.dual, grid, st, p);
+    else if (NT == 3) launch_tn_h<f16_t, 3, NP>(safe, g.dual, grid, st, p);
+    else launch_tn_h<f16_t, 4, NP>(safe, g.dual, grid, st, p);
+  }
 }
 template <int NT, int CW>
 void launch_reduce_tn(dim3 g, hipStream_t st, const StreamParams &p, int NA, long ngroups) {
@@ -1728,21 +1860,8 @@ int launch_stream(planc_ctx *ctx, int dtype, int trans, int NT, int tm, StreamPa
   dim3 grid((unsigned)g.blocks_x, (unsigned)g.nsplit);
   hipStream_t st = ctx->stream;
   if (!trans && dtype != PLANC_F32) {
-#define NN16(TA, SHV)                                                    \
-  do {                                                                   \
-    if (NT == 1) nn16_kernel<TA, 1, SHV><<<grid, NTHR, 0, st>>>(p);      \
-    else if (NT == 2) nn16_kernel<TA, 2, SHV><<<grid, NTHR, 0, st>>>(p); \
-    else if (NT == 3) nn16_kernel<TA, 3, SHV><<<grid, NTHR, 0, st>>>(p); \
-    else nn16_kernel<TA, 4, SHV><<<grid, NTHR, 0, st>>>(p);              \
-  } while (0)
-    if (dtype == PLANC_BF16) {
-      if (g.shared) NN16(bf16_t, 1);
-      else NN16(bf16_t, 0);
-    } else {
-      if (g.shared) NN16(f16_t, 1);
-      else NN16(f16_t, 0);
-    }
-#undef NN16
+    if (p.np == 2) launch_nn16<2>(dtype, NT, g.shared, grid, st, p);
+    else launch_nn16<1>(dtype, NT, g.shared, grid, st, p);
     PLANC_LAUNCH_CHECK(ctx);
     if (g.nsplit > 1) {
       const int NA = 8 * NT;
@@ -1764,14 +1883,9 @@ int launch_stream(planc_ctx *ctx, int dtype, int trans, int NT, int tm, StreamPa
     else if (NT == 3) TNS(3);
     else TNS(4);
 #undef TNS
-  } else if (g.shared && trans) {
-    if (dtype == PLANC_BF16) {
-      if (g.cw == 4) launch_tn_h_sh<bf16_t, 4>(NT, grid, st, p);
-      else launch_tn_h_sh<bf16_t, 2>(NT, grid, st, p);
-    } else {
-      if (g.cw == 4) launch_tn_h_sh<f16_t, 4>(NT, grid, st, p);
-      else launch_tn_h_sh<f16_t, 2>(NT, grid, st, p);
-    }
+  } else if (trans && dtype != PLANC_F32) {
+    if (p.np == 2) launch_tn16<2>(dtype, NT, g, grid, st, p);
+    else launch_tn16<1>(dtype, NT, g, grid, st, p);
   } else if (g.shared) {
     if (NT == 1) launch_nn_f32_sh<1>(tm, grid, st, p);
     else if (NT == 2) launch_nn_f32_sh<2>(tm, grid, st, p);
@@ -1790,19 +1904,6 @@ int launch_stream(planc_ctx *ctx, int dtype, int trans, int NT, int tm, StreamPa
     else if (NT == 3) TNF(3);
     else TNF(4);
 #undef TNF
-  } else if (trans) {
-    const bool safe = !p.avec;
-    if (dtype == PLANC_BF16) {
-      if (NT == 1) launch_tn_h<bf16_t, 1>(safe, g.dual, grid, st, p);
-      else if (NT == 2) launch_tn_h<bf16_t, 2>(safe, g.dual, grid, st, p);
-      else if (NT == 3) launch_tn_h<bf16_t, 3>(safe, g.dual, grid, st, p);
-      else launch_tn_h<bf16_t, 4>(safe, g.dual, grid, st, p);
-    } else {
-      if (NT == 1) launch_tn_h<f16_t, 1>(safe, g.dual, grid, st, p);
-      else if (NT == 2) launch_tn_h<f16_t, 2>(safe, g.dual, grid, st, p);
-      else if (NT == 3) launch_tn_h<f16_t, 3>(safe, g.dual, grid, st, p);
-      else launch_tn_h<f16_t, 4>(safe, g.dual, grid, st, p);
-    }
   } else {
     if (NT == 1) launch_nn_f32<1>(tm, grid, st, p);
     else if (NT == 2) launch_nn_f32<2>(tm, grid, st, p);
@@ -1854,7 +1955,7 @@ int pick_geometry(planc_ctx *ctx, int dtype, int trans, int NT, int tm, const St
   const char *venv = getenv("PLANC_STREAM_VARIANT");
   const std::string vs = venv ? venv : "";
   const int smode = vs == "plain" ? 0 : (vs == "shared" ? 1 : (vs == "dual" ? 3 : 2));
-  const std::vector<long> key = {dtype, trans, NT, tm, p.L, p.K, p.lda, p.th.k, p.avec, p.cmajor, smode};
+  const std::vector<long> key = {dtype, trans, NT, tm, p.L, p.K, p.lda, p.th.k, p.avec, p.cmajor, smode, p.np};
   for (const auto &e : ctx->gemm_tuned)
     if (e.first == key) {
       const auto &v = e.second;
@@ -1955,6 +2056,7 @@ int thin_gemm(planc_ctx *ctx, int dtype, int trans, const void *A, long lda, lon
   const int esz = dtype == PLANC_F32 ? 4 : 2;
   const int vecw = 16 / esz;
   const bool avec = (lda % vecw == 0) && (((uintptr_t)A) % 16 == 0);
+  const int np = dtype != PLANC_F32 && ctx->thin_planes == 2 ? 2 : 1;
   for (int c0 = 0; c0 < k; c0 += 64) {
     const int kc = (k - c0 < 64) ? (k - c0) : 64;
     const int NT = (kc + 15) / 16;
@@ -2003,10 +2105,14 @@ int thin_gemm(planc_ctx *ctx, int dtype, int trans, const void *A, long lda, lon
       const bool vec = avec;
 #define NNH(TA, NTV)                                                                   \
   do {                                                                                 \
-    if (vec)                                                                           \
-      nn_h_kernel<TA, NTV, true><<<grid, NTHR, 0, ctx->stream>>>(p);                   \
+    if (np == 2 && vec)                                                                \
+      nn_h_kernel<TA, NTV, true, 2><<<grid, NTHR, 0, ctx->stream>>>(p);                \
+    else if (np == 2)                                                                  \
+      nn_h_kernel<TA, NTV, false, 2><<<grid, NTHR, 0, ctx->stream>>>(p);               \
+    else if (vec)                                                                      \
+      nn_h_kernel<TA, NTV, true, 1><<<grid, NTHR, 0, ctx->stream>>>(p);                \
     else                                                                               \
-      nn_h_kernel<TA, NTV, false><<<grid, NTHR, 0, ctx->stream>>>(p);                  \
+      nn_h_kernel<TA, NTV, false, 1><<<grid, NTHR, 0, ctx->stream>>>(p);               \
   } while (0)
       if (dtype == PLANC_BF16) {
         if (NT == 1) NNH(bf16_t, 1);
@@ -2049,17 +2155,23 @@ int thin_gemm(planc_ctx *ctx, int dtype, int trans, const void *A, long lda, lon
       // 16-bit TN / NN: the thin operand packed once into the 16-bit MFMA
       // operand layout (8 consecutive k of one column per lane).
       const long nsuper = (K + 63) / 64;
-      uint4 *pk = (uint4 *)ws_get(ctx, 6, (size_t)nsuper * 2 * NT * 64 * 16);
+      const long tot = nsuper * 2 * NT * 64;  // words of one plane
+      uint4 *pk = (uint4 *)ws_get(ctx, 6, (size_t)np * tot * 16);
       if (!pk) return set_error(ctx, PLANC_ENOMEM, "thin_gemm: packed operand");
-      const long tot = nsuper * 2 * NT * 64;
       const unsigned nb = (unsigned)((tot + 255) / 256);
-      if (dtype == PLANC_BF16)
-        pack_thin16_kernel<bf16_t><<<nb, 256, 0, ctx->stream>>>(th, K, NT, nsuper, pk);
+      if (dtype == PLANC_BF16 && np == 2)
+        pack_thin16_kernel<bf16_t, 2><<<nb, 256, 0, ctx->stream>>>(th, K, NT, nsuper, pk, tot);
+      else if (dtype == PLANC_BF16)
+        pack_thin16_kernel<bf16_t, 1><<<nb, 256, 0, ctx->stream>>>(th, K, NT, nsuper, pk, tot);
+      else if (np == 2)
+        pack_thin16_kernel<f16_t, 2><<<nb, 256, 0, ctx->stream>>>(th, K, NT, nsuper, pk, tot);
       else
-        pack_thin16_kernel<f16_t><<<nb, 256, 0, ctx->stream>>>(th, K, NT, nsuper, pk);
+        pack_thin16_kernel<f16_t, 1><<<nb, 256, 0, ctx->stream>>>(th, K, NT, nsuper, pk, tot);
       PLANC_LAUNCH_CHECK(ctx);
       p.thin16 = pk;
+      p.plane = tot;
     }
+    p.np = np;
     Geometry g;
     if (splits_hint > 0) {
       g = make_geometry(L, K, trans, dtype, NT, trans && dtype == PLANC_F32 ? 2 : 4, 1, splits_hint);

@@ -527,25 +527,6 @@ __global__ void apply_reg_kernel(double *G, int k, float l2, float l1) {
   G[e] = v;
 }
 
-// hi = X rounded to the 16-bit type (as a float), lo = X - hi (exact in
-// FP32): a thin operand split so that a 16-bit product with hi plus one with
-// lo carries the thin operand to about 2^-17 relative (only A's rounding
-// remains).
-template <int DT>
-__global__ void split16_kernel(const float *X, long n, float *hi, float *lo) {
-  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n) return;
-  const float x = X[e];
-  const float h = DT == PLANC_BF16 ? bf16_to_float(float_to_bf16(x)) : f16_to_float(float_to_f16(x));
-  hi[e] = h;
-  lo[e] = x - h;
-}
-
-__global__ void add_inplace_kernel(float *Y, const float *X, long n) {
-  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e < n) Y[e] += X[e];
-}
-
 // Symmetric regularisation (applySymmetricReg, common/nmf.hpp:123-129):
 // G += sr * I and R(j, c) += sr * F(j, c), R's element (j, c) at
 // R[j * rs + c * cs], F n x k column-major (ld n).
@@ -903,24 +884,6 @@ int gram_norms(planc_ctx *ctx, const double *G, int k, double *norms) {
 int apply_reg(planc_ctx *ctx, double *G, int k, const double *reg) {
   if (!reg || (!(reg[0] > 0) && !(reg[1] > 0))) return PLANC_OK;
   apply_reg_kernel<<<(k * k + 255) / 256, 256, 0, ctx->stream>>>(G, k, (float)reg[0], (float)reg[1]);
-  PLANC_LAUNCH_CHECK(ctx);
-  return PLANC_OK;
-}
-
-int split16(planc_ctx *ctx, int dtype, const float *X, long n, float *hi, float *lo) {
-  if (n <= 0) return PLANC_OK;
-  const unsigned nb = (unsigned)((n + 255) / 256);
-  if (dtype == PLANC_BF16)
-    split16_kernel<PLANC_BF16><<<nb, 256, 0, ctx->stream>>>(X, n, hi, lo);
-  else
-    split16_kernel<PLANC_F16><<<nb, 256, 0, ctx->stream>>>(X, n, hi, lo);
-  PLANC_LAUNCH_CHECK(ctx);
-  return PLANC_OK;
-}
-
-int add_inplace(planc_ctx *ctx, float *Y, const float *X, long n) {
-  if (n <= 0) return PLANC_OK;
-  add_inplace_kernel<<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(Y, X, n);
   PLANC_LAUNCH_CHECK(ctx);
   return PLANC_OK;
 }
