@@ -253,6 +253,29 @@ def hier_leg(P, D, ctx, m=100000, n=500000, density=1e-4, depth=11, iters=100):
                       f"tree out)"}
 
 
+def profile_ntf4(P, D, ctx, torch):
+    """The 128^4 k=48 partial MTTKRPs alone (for counter runs): right and left
+    contractions in FP32 and FP16, ten launches each after tuning, then ten
+    copies of a 1 GiB buffer for the counter calibration."""
+    dims4, k4 = (128, 128, 128, 128), 48
+    X4, init4 = make_ntf_problem(P, D, ctx, torch, dims4, k4, 48, stream=5)
+    X416, _ = ctx.convert(flat(X4), P.F16, scaled=False)
+    F4 = [P.colmajor(f) for f in init4]
+    L4 = dims4[0] * dims4[1]
+    P4 = P.empty_colmajor((L4, k4))
+    Q4 = P.empty_colmajor((L4, k4))
+    for dt, Xop in ((P.F32, X4), (P.F16, X416)):
+        for side, out in ((0, P4), (1, Q4)):
+            for _ in range(13):
+                ctx.mttkrp_partial(Xop, F4, 1, side, dtype=dt, out=out)
+    cal = torch.empty(256 << 20, dtype=torch.float32, device="cuda")  # 1 GiB
+    cal2 = torch.empty_like(cal)
+    for _ in range(10):
+        cal2.copy_(cal)
+    torch.cuda.synchronize()
+    print(json.dumps({"profile_kernel": "128^4 k=48 partial MTTKRP right/left FP32/FP16 x13; copy 1 GiB x10"}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -261,8 +284,13 @@ def main():
     ap.add_argument("--no-legs", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=3)
-    ap.add_argument("--profile-kernel", action="store_true",
-                    help="only launch the dominant kernel (and a calibration copy) for rocprof")
+    ap.add_argument("--legs", default="hals,bpp,ntf4,ntf3,hier",
+                    help="comma-separated legs after the headline: hals, bpp, ntf4 (NTF-HALS "
+                         "128^4), ntf3 (NTF-HALS 256^3), hier (hierarchical NMF)")
+    ap.add_argument("--profile-kernel", nargs="?", const="wta", default=None, choices=["wta", "ntf4"],
+                    help="only launch one kernel family (and a calibration copy) for rocprof: "
+                         "wta = BF16 W^T A of the headline; ntf4 = the 128^4 k=48 partial "
+                         "MTTKRPs (right and left, FP32 and FP16)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this build "
@@ -276,6 +304,11 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     ctx = P.Context(0)
+    if args.profile_kernel == "ntf4":
+        profile_ntf4(P, D, ctx, torch)
+        ctx.close()
+        return
+    legs_on = set(x.strip() for x in args.legs.split(",") if x.strip())
     m, n, k = M_C3, N_C3, K_C3
     log(f"bench: generating A {m}x{n} (rank {R_C3} + 1% noise) on the device")
     A, Wi, Hi = make_nmf_problem(P, D, ctx, torch, m, n, R_C3, k, STREAM_C3)
@@ -363,16 +396,19 @@ def main():
         cpu = cpu_baseline_leg(O, A, Wi, Hi, args.cpu_iters)
 
     legs = {}
-    if not args.no_legs:
+    if args.no_legs:
+        legs_on = set()
+    if "hals" in legs_on:
         log("bench: NMF-HALS fp32 vs bf16")
         legs["nmf_hals_65536x16384_k64"] = compare_precisions(
             P, ctx, torch, A, Wi, Hi, A16, cs, A16T, normA2, P.HALS, "hals", args.warmup, args.steps)
+    if "bpp" in legs_on:
         log("bench: NMF-ANLS-BPP fp32 vs bf16")
         legs["nmf_anlsbpp_65536x16384_k64"] = compare_precisions(
             P, ctx, torch, A, Wi, Hi, A16, cs, A16T, normA2, P.ANLSBPP, "anlsbpp", 1, args.steps)
-        del A, A16, A16T, cs, Wi, Hi, Wt
-        torch.cuda.empty_cache()
-
+    del A, A16, A16T, cs, Wi, Hi, Wt
+    torch.cuda.empty_cache()
+    if "ntf4" in legs_on:
         log("bench: NTF-HALS 128^4 k=48 fp32 vs fp16")
         dims4, k4 = (128, 128, 128, 128), 48
         X4, init4 = make_ntf_problem(P, D, ctx, torch, dims4, k4, 48, stream=5)
@@ -402,7 +438,7 @@ def main():
         legs["ntf_hals_128^4_k48"] = nt
         del X4, X416, F4, P4
         torch.cuda.empty_cache()
-
+    if "ntf3" in legs_on:
         log("bench: NTF-HALS 256^3 k=32")
         dims, k3 = (256, 256, 256), 32
         X, init = make_ntf_problem(P, D, ctx, torch, dims, k3, 32, stream=2)
@@ -424,7 +460,7 @@ def main():
         legs["ntf_hals_256^3_k32"] = c2
         del X
         torch.cuda.empty_cache()
-
+    if "hier" in legs_on:
         log("bench: hierarchical rank-2 NMF, 100000x500000 CSR, depth 11")
         legs["hiernmf_100000x500000_depth11"] = hier_leg(P, D, ctx)
 

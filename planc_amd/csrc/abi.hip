@@ -981,7 +981,7 @@ int planc_ntf_run(planc_ctx *ctx, const planc_ntf_desc *desc, const float *d_X, 
         }
         const bool last_err = desc->compute_error && n == N - 1;
         RC(mode_update(ctx, kind, 1e-6f, Fn, dims[n], dims[n], k, Mp, mrs, mcs, Gc.data(), N, n, G[n],
-                       d_lambda, normX2, last_err ? rel_out : nullptr, rel_counter, dflag));
+                       d_lambda, normX2, last_err ? rel_out : nullptr, rel_counter));
         err_done = err_done || last_err;
         continue;
       }
@@ -1106,8 +1106,8 @@ int planc_ntf_run(planc_ctx *ctx, const planc_ntf_desc *desc, const float *d_X, 
         key.push_back((uintptr_t)dims[i]);
       }
       for (auto &w : ctx->ws) key.push_back((uintptr_t)w.ptr);
-      const char *xe = getenv("PLANC_MODE_EXCHANGE");  // the mode-update form captured
-      key.push_back(xe && xe[0] == '0' ? 0 : 1);
+      const char *oe = getenv("PLANC_MODE_ONE_BLOCK");  // the mode-update form captured
+      key.push_back(oe && oe[0] == '0' ? 0 : 1);
       if (graph_lookup(ctx, key, exec)) return PLANC_OK;
       int rc2 = capture_graph(ctx, [&]() -> int {
         for (int u = 0; u < n_it; ++u) {
@@ -1131,7 +1131,6 @@ int planc_ntf_run(planc_ctx *ctx, const planc_ntf_desc *desc, const float *d_X, 
   if (h_relerr && desc->compute_error && iters > 0)
     PLANC_HIP_CHECK(ctx, hipMemcpyAsync(h_relerr, rel, sizeof(double) * iters, hipMemcpyDeviceToHost, ctx->stream));
   PLANC_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-  if (hflag & 2) return set_error(ctx, PLANC_EHIP, "NTF mode update: block exchange timed out");
   if (hflag & 4) return set_error(ctx, PLANC_ENOCONV, "AO-ADMM: shifted Gram not positive definite");
   if (hflag & 8) return set_error(ctx, PLANC_ENOCONV, "NES: the Nesterov loop ran past 100000 steps");
   if (hflag) return set_error(ctx, PLANC_ENOCONV, "BPP failed: more than 5k pivot iterations");

@@ -116,8 +116,7 @@ namespace planc {
 // 10 BPP per-column iteration counts, 11 BPP failure flag, 12 CP-loss scalar,
 // 13 CP-gradient float buffers, 14 CP-gradient Grams, 15 loop-graph counter,
 // 16 transposed 16-bit copy of A / X, 17 HALS grid-barrier counter and flag,
-// 18 fused NTF mode-update partials, 19/21 NTF mode-exchange slots and
-// control, 20 (unused), 22 AO-ADMM factor / partials,
+// 18 NTF mode-update partials (two-launch form), 19/21 (unused), 20 (unused), 22 AO-ADMM factor / partials,
 // 23 GNSym CG vectors, 24..31 sparse path (sparse.hip: k-major factor
 // copies, error partials, transposed CSR, sort keys / entries / temporary
 // storage), 32 NTF AO-ADMM duals, 33 NMF AO-ADMM duals, 34 dual norms,
@@ -304,7 +303,7 @@ bool mode_update_fits(long I, int k);
 // is rel_out[*rel_counter] and the device counter is incremented.
 int mode_update(planc_ctx *ctx, int kind, float eps, float *F, long I, long ldf, int k, const float *M,
                 long mrs, long mcs, const double *const *grams, int ng, int skip, double *gout,
-                double *lambda, const double *normX2, double *rel_out, int *rel_counter, int *err);
+                double *lambda, const double *normX2, double *rel_out, int *rel_counter);
 
 }  // namespace planc
 

@@ -277,32 +277,28 @@ __global__ void transpose_kernel(const float *in, long I, long ldi, int k, float
 }
 
 // ---------------------------------------------------------------------------
-// One mode update of AUNTF in two launches: S = Hadamard of the other Grams,
-// the HALS (ntf/ntfhals.hpp:13-29) or MU row update from this mode's
-// MTTKRP, normalize(mode) (ntf/auntf.hpp:66-79: lambda = column 2-norms,
-// columns divided by them), the FP64 Gram of the new factor and optionally
-// the CP relative error of ntf/auntf.hpp:224-245 from this mode's MTTKRP.
-//   mode_rows_kernel: MRT = 64 rows per block (16-row blocks over more CUs
-//     measured slower at 256 x 32), a quad of lanes per row (each lane holds
-//     k/4 of the row, the column dot products are quad sums), writes the
-//     unnormalised rows u and accumulates per block the FP64 Gram of u and,
-//     per column, sum_i u(i, c) M(c, i).
-//   mode_finalize_kernel (one block): sums the block partials in block order;
-//     norms = sqrt(diag(Gram u)), lambda = norms, Gram of the normalised
-//     factor = D^-1 (Gram u) D^-1, the columns of F divided by the norms,
-//     the error.
-//   For a factor of at most XCH_BLOCKS blocks of rows and k <= 48 there is
-//     one launch: every block writes its partials into its own slots of a
-//     device buffer with agent-scope atomic stores (slots start as
-//     PART_EMPTY, an all-ones NaN no arithmetic produces), polls all blocks'
-//     slots until none is empty, sums them in block order (identical in
-//     every block) and divides its own rows, still in registers, by the
-//     norms; block 0 writes lambda, the Gram and the error.  Two slot
-//     buffers alternate by the parity of a device launch counter, and the
-//     blocks empty the whole other buffer for the next launch.  The
-//     blocks of one launch must run concurrently: XCH_BLOCKS is far below
-//     one block per CU, and the wait is bounded (a time-out sets bit 2 of
-//     the run's error flag).
+// One mode update of AUNTF: S = Hadamard of the other Grams, the HALS
+// (ntf/ntfhals.hpp:13-29) or MU row update from this mode's MTTKRP,
+// normalize(mode) (ntf/auntf.hpp:66-79: lambda = column 2-norms, columns
+// divided by them), the FP64 Gram of the new factor and optionally the CP
+// relative error of ntf/auntf.hpp:224-245 from this mode's MTTKRP.
+//   mode_rows_kernel: a row group of MRT = 64 rows is handled by 4 waves, a
+//     quad of lanes per row (each lane holds k/4 of the row, the column dot
+//     products are quad sums); the group writes the unnormalised rows u and
+//     accumulates the FP64 Gram of u and, per column, sum_i u(i, c) M(c, i)
+//     (16-row groups over more CUs measured slower at 256 x 32).
+//   NVB == 0 (any I): one group per block, the blocks stride over the rows
+//     and store their partials; mode_finalize_kernel (second launch) sums
+//     them in block order, derives the norms, divides F, writes lambda, the
+//     normalised Gram and the error.
+//   NVB >= 1 (I <= 64 NVB): one launch of a single block of NVB groups
+//     (256 NVB threads) holding every row once; the groups leave their
+//     partials in LDS, every thread sums them in group order (the same
+//     order and grouping as the two-launch form, so the results are
+//     identical), and the rows, still in registers, are divided by the
+//     norms.  No cross-block traffic: at I <= 256 the update is a chain of
+//     dependent round trips, and the second launch or a cross-block
+//     exchange is a large part of it.
 // kind: 0 MU, 1 HALS, 2 no row update (the factor was solved already, BPP).
 struct ModeParams {
   float *F;
@@ -317,46 +313,35 @@ struct ModeParams {
   float eps;
   const double *normX2;
   double *rel_out;
-  double *part;  // per block: k*k Gram of u, then k column dots
+  double *part;  // NVB == 0: per block, the k*k Gram of u, then k column dots
   int *rel_counter;  // non-null: the error goes to rel_out[*rel_counter], which is incremented
-  unsigned long long *slots;  // one-launch form: two buffers of XCH_HALF slots
-  int *ctl;                   // one-launch form: ctl[0] launch counter
-  int *err;                   // one-launch form: time-out flag (bit 2)
 };
 
-constexpr int XCH_BLOCKS = 4;                           // blocks of rows in the one-launch form
-constexpr long XCH_HALF = XCH_BLOCKS * (48L * 48 + 48);  // slots per buffer
-constexpr unsigned long long PART_EMPTY = ~0ull;        // a NaN no arithmetic produces
-
-__global__ void xch_init_kernel(unsigned long long *slots, long n, int *ctl) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) slots[i] = PART_EMPTY;
-  if (i == 0) ctl[0] = 0;
-}
-
-template <int KM, int KIND, int MRT, bool XCH>
-__global__ __launch_bounds__(4 * MRT) void mode_rows_kernel(ModeParams p) {
-  constexpr int MR = MRT;            // rows per block iteration
-  constexpr int NTH = 4 * MR;        // threads: a quad per row
-  constexpr int NW = NTH / 64;       // waves
-  constexpr int Q = KM / 4;          // row values per lane
-  constexpr int NT = KM / 16;        // 16 x 16 tiles per Gram side
-  constexpr int NUP = NT * (NT + 1) / 2;  // upper-triangle tiles
-  constexpr int TPW = (NUP + NW - 1) / NW;  // tiles per wave
+template <int KM, int KIND, int MRT, int NVB>
+__global__ __launch_bounds__(4 * MRT *(NVB > 0 ? NVB : 1)) void mode_rows_kernel(ModeParams p) {
+  constexpr int MR = MRT;                      // rows per group
+  constexpr int NTH = 4 * MR;                  // threads per group: a quad per row
+  constexpr int NG1 = NVB > 0 ? NVB : 1;       // groups per block
+  constexpr int NTT = NTH * NG1;               // threads per block
+  constexpr int NW = NTH / 64;                 // waves per group
+  constexpr int Q = KM / 4;                    // row values per lane
+  constexpr int NT = KM / 16;                  // 16 x 16 tiles per Gram side
+  constexpr int NUP = NT * (NT + 1) / 2;       // upper-triangle tiles
+  constexpr int TPW = (NUP + NW - 1) / NW;     // tiles per wave
   constexpr int NDP = (4 * KM + NTH - 1) / NTH;  // (column, quarter) dot pairs per thread
   __shared__ __attribute__((aligned(16))) float St[KM][KM];  // St[x][t] = S(t, x)
-  // u(row, c) at Us[c][row], M(c, row) at Ms[c][row]; the one-launch form
-  // reuses the space for the summed partials (MR >= KM: they fit).
-  __shared__ __attribute__((aligned(16))) float UMs[2][KM][MR + 1];
-  float(&Us)[KM][MR + 1] = UMs[0];
-  float(&Ms)[KM][MR + 1] = UMs[1];
-  static_assert(MR >= KM, "summed partials must fit the row tiles");
-  const int tid = threadIdx.x, q = tid & 3, rr = tid >> 2;
-  const int lane = tid & 63, wave = tid >> 6;
+  // Per group: u(row, c) at Us[c][row], M(c, row) at Ms[c][row]; the
+  // one-block form then reuses a group's space for its partials (2 KM (MR +
+  // 1) floats >= KM^2 + KM doubles).
+  __shared__ __attribute__((aligned(16))) float UMs[NG1][2][KM][MR + 1];
+  static_assert(MR + 1 >= KM + 1, "a group's partials must fit its row tiles");
+  const int tid = threadIdx.x, vb = tid / NTH, gt = tid - vb * NTH;
+  const int q = gt & 3, rr = gt >> 2;
+  const int lane = tid & 63, wave = gt >> 6;
+  float(&Us)[KM][MR + 1] = UMs[vb][0];
+  float(&Ms)[KM][MR + 1] = UMs[vb][1];
   const int k = p.k;
   const long stride = (long)gridDim.x * MR;
-  int par = 0;  // one-launch form: which slot buffer this launch uses
-  if constexpr (XCH) par = __hip_atomic_load(p.ctl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & 1;
   // Row values and MTTKRP values of rows r0 + rr (clamped to a valid row).
   auto load_rows = [&](long r0, float(&h)[Q], float(&mv)[Q]) {
     const long i = r0 + rr;
@@ -380,18 +365,17 @@ __global__ __launch_bounds__(4 * MRT) void mode_rows_kernel(ModeParams p) {
     tb = ta + rem;
   };
   float h[Q], mv[Q];
-  long r0 = (long)blockIdx.x * MR;
-  load_rows(r0, h, mv);  // in flight while S is built
+  long r0 = NVB > 0 ? (long)vb * MR : (long)blockIdx.x * MR;
   // S = Hadamard of the other Grams: every load of the block's entries is
   // issued before the first product (one memory round trip, not one per
-  // entry).
-  constexpr int NE = (KM * KM + NTH - 1) / NTH;
-  double dS[NE];  // FP64 Hadamard entries of this thread (the one-launch error)
+  // entry), then the rows' loads, which stay in flight while S is built.
+  constexpr int NE = (KM * KM + NTT - 1) / NTT;
+  double dS[NE];  // FP64 Hadamard entries of this thread (the one-block error)
   {
     double gv[NE][4];
 #pragma unroll
     for (int it = 0; it < NE; ++it) {
-      const int e = tid + it * NTH;
+      const int e = tid + it * NTT;
       const int a = e % KM, b = e / KM;
       const bool in = e < KM * KM && a < k && b < k;
 #pragma unroll
@@ -400,9 +384,10 @@ __global__ __launch_bounds__(4 * MRT) void mode_rows_kernel(ModeParams p) {
         gv[it][g] = (in && gg < p.ng) ? p.g[gg][a + (long)k * b] : 1.0;
       }
     }
+    load_rows(r0, h, mv);
 #pragma unroll
     for (int it = 0; it < NE; ++it) {
-      const int e = tid + it * NTH;
+      const int e = tid + it * NTT;
       const int a = e % KM, b = e / KM;
       double d = gv[it][0];  // in Gram order, as the sequential product
       d *= gv[it][1];
@@ -421,7 +406,9 @@ __global__ __launch_bounds__(4 * MRT) void mode_rows_kernel(ModeParams p) {
 #pragma unroll
   for (int j = 0; j < NDP; ++j) dacc[j] = 0.0;
   __syncthreads();
-  for (; r0 < p.I; r0 += stride) {
+  // One pass per group in the one-block form (uniform barriers: every group
+  // runs it); the blocks of the two-launch form stride over the rows.
+  for (;;) {
     const long i = r0 + rr;
     const bool valid = i < p.I;
     quad_row_update<KM, false>(h, mv, St, q, k, KIND, p.eps);
@@ -430,52 +417,70 @@ __global__ __launch_bounds__(4 * MRT) void mode_rows_kernel(ModeParams p) {
       const int t = q * Q + u;
       Us[t][rr] = valid ? h[u] : 0.0f;
       Ms[t][rr] = valid ? mv[u] : 0.0f;
-      if (!XCH && valid && t < k) p.F[i + p.ldf * t] = h[u];
+      if (NVB == 0 && valid && t < k) p.F[i + p.ldf * t] = h[u];
     }
-    if (r0 + stride < p.I) load_rows(r0 + stride, h, mv);  // next rows in flight
+    const bool more = NVB == 0 && r0 + stride < p.I;
+    if (more) load_rows(r0 + stride, h, mv);  // next rows in flight
     __syncthreads();
     // Gram of this chunk's rows on FP64 MFMA: D(a, b) += sum_r U(r, a) U(r, b)
     // per 16 x 16 tile; lane (lane%16, lane/16) feeds A = U(r0 + lane/16, a0 +
-    // lane%16) and B = U(r0 + lane/16, b0 + lane%16).
+    // lane%16) and B = U(r0 + lane/16, b0 + lane%16).  The row steps go
+    // through two accumulators per tile (even and odd steps, summed at the
+    // end) and the wave's tiles are interleaved, so consecutive MFMAs do not
+    // wait on each other.
+    {
+      const float *ua[TPW], *ub[TPW];
+      bool on[TPW];
 #pragma unroll
-    for (int j = 0; j < TPW; ++j) {
-      const int t = wave + NW * j;
-      if (t < NUP) {
-        int ta, tb;
-        tile_of(t, ta, tb);
-        const float *ua = &Us[16 * ta + (lane & 15)][0];
-        const float *ub = &Us[16 * tb + (lane & 15)][0];
-#pragma unroll 4
-        for (int r = lane >> 4; r < MR; r += 4)
-          gacc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)ua[r], (double)ub[r], gacc[j], 0, 0, 0);
+      for (int j = 0; j < TPW; ++j) {
+        const int t = wave + NW * j;
+        on[j] = t < NUP;
+        int ta = 0, tb = 0;
+        if (on[j]) tile_of(t, ta, tb);
+        ua[j] = &Us[16 * ta + (lane & 15)][0];
+        ub[j] = &Us[16 * tb + (lane & 15)][0];
       }
+      doublex4 g2[TPW];
+#pragma unroll
+      for (int j = 0; j < TPW; ++j) g2[j] = doublex4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int r = 0; r < MR / 4; r += 2) {
+        const int ra = (lane >> 4) + 4 * r, rb = ra + 4;
+#pragma unroll
+        for (int j = 0; j < TPW; ++j)
+          if (on[j]) {
+            gacc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)ua[j][ra], (double)ub[j][ra], gacc[j], 0, 0, 0);
+            g2[j] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)ua[j][rb], (double)ub[j][rb], g2[j], 0, 0, 0);
+          }
+      }
+#pragma unroll
+      for (int j = 0; j < TPW; ++j) gacc[j] += g2[j];
     }
-    // Column dots sum_r U(r, c) M(c, r): pair (c, quarter) = tid + NTH j,
+    // Column dots sum_r U(r, c) M(c, r): pair (c, quarter) = gt + NTH j,
     // the four quarters of a column in consecutive lanes (quad sum below).
 #pragma unroll
     for (int j = 0; j < NDP; ++j) {
-      const int pi = tid + NTH * j;
+      const int pi = gt + NTH * j;
       if (pi < 4 * KM) {
         const int c = pi >> 2, part = pi & 3;
-        double sd = 0.0;
-        for (int r = part; r < MR; r += 4) sd += (double)Us[c][r] * (double)Ms[c][r];
-        dacc[j] += sd;
+        double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+        for (int r = part; r < MR; r += 8) {
+          s0 += (double)Us[c][r] * (double)Ms[c][r];
+          s1 += (double)Us[c][r + 4] * (double)Ms[c][r + 4];
+        }
+        dacc[j] += s0 + s1;
       }
     }
     __syncthreads();
+    if (!more) break;
+    r0 += stride;
   }
-  double *dst = p.part + (long)blockIdx.x * (k * k + k);
-  unsigned long long *xdst = nullptr;
-  if constexpr (XCH) xdst = p.slots + par * XCH_HALF + (long)blockIdx.x * (k * k + k);
-  // Partial (a, b) of this block: a plain store, or an atomic store into
-  // its slot in the one-launch form.
-  auto put_part = [&](int idx, double v) {
-    if constexpr (XCH)
-      __hip_atomic_store(&xdst[idx], __builtin_bit_cast(unsigned long long, v), __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-    else
-      dst[idx] = v;
-  };
+  const int kk = k * k;
+  // Partial (a, b) of this group: into global memory (two-launch form) or
+  // into the group's LDS space (one-block form, free after the last barrier).
+  double *dst = NVB > 0 ? reinterpret_cast<double *>(&UMs[vb][0][0][0])
+                        : p.part + (long)blockIdx.x * (kk + k);
   // FP64 D layout: lane holds D(lane / 16 + 4 v, lane % 16), v = 0..3.
 #pragma unroll
   for (int j = 0; j < TPW; ++j) {
@@ -488,8 +493,8 @@ __global__ __launch_bounds__(4 * MRT) void mode_rows_kernel(ModeParams p) {
       for (int v = 0; v < 4; ++v) {
         const int a = 16 * ta + (lane >> 4) + 4 * v;
         if (a < k && b < k && (ta != tb || a <= b)) {
-          put_part(a + k * b, gacc[j][v]);
-          put_part(b + k * a, gacc[j][v]);
+          dst[a + k * b] = gacc[j][v];
+          dst[b + k * a] = gacc[j][v];
         }
       }
     }
@@ -498,75 +503,49 @@ __global__ __launch_bounds__(4 * MRT) void mode_rows_kernel(ModeParams p) {
   for (int j = 0; j < NDP; ++j) {
     // fixed-order quad combine: ((d0 + d1) + (d2 + d3)); every lane of a
     // quad takes part (the shuffles need all of them).
-    const int pi = tid + NTH * j;
+    const int pi = gt + NTH * j;
     double d1 = __shfl_xor(dacc[j], 1, 64);
     double s01 = (tid & 1) ? d1 + dacc[j] : dacc[j] + d1;
     double s23 = __shfl_xor(s01, 2, 64);
     double tot = (tid & 2) ? s23 + s01 : s01 + s23;
     const int c = pi >> 2;
-    if (pi < 4 * KM && (pi & 3) == 0 && c < k) put_part(k * k + c, tot);
+    if (pi < 4 * KM && (pi & 3) == 0 && c < k) dst[kk + c] = tot;
   }
-  if constexpr (XCH) {
-    const int kk = k * k, nb = (int)gridDim.x;
-    // Empty the whole other buffer for the next launch, whatever its block
-    // count and k (nobody reads that buffer during this launch).
-    unsigned long long *other = p.slots + (1 - par) * XCH_HALF;
-    for (long e = (long)blockIdx.x * NTH + tid; e < XCH_HALF; e += (long)nb * NTH) other[e] = PART_EMPTY;
-    // Poll every block's slots until none is empty; the sums of that poll
-    // are the block-order sums of the partials.
-    const unsigned long long *cur = p.slots + par * XCH_HALF;
-    constexpr int NG = (KM * KM + KM + NTH - 1) / NTH;
-    double gs[NG];
-    bool ok = true;
-    for (int spin = 0;; ++spin) {
-      unsigned long long v[NG][XCH_BLOCKS];
+  if constexpr (NVB > 0) {
+    __syncthreads();
+    // Group-order sums of the partials (0 + P0 + P1 + ..., the order of
+    // mode_finalize_kernel), kept in registers until every group's space
+    // has been read, then stored over group 0's.
+    constexpr int NGS = (KM * KM + KM + NTT - 1) / NTT;
+    double gs[NGS];
 #pragma unroll
-      for (int j = 0; j < NG; ++j) {
-        const int e = tid + NTH * j;
+    for (int j = 0; j < NGS; ++j) {
+      const int e = tid + NTT * j;
+      double sj = 0.0;
+      if (e < kk + k) {
 #pragma unroll
-        for (int b = 0; b < XCH_BLOCKS; ++b)
-          v[j][b] = (b < nb && e < kk + k)
-                        ? __hip_atomic_load(&cur[(long)b * (kk + k) + e], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                        : 0ull;
+        for (int g = 0; g < NVB; ++g) sj += reinterpret_cast<const double *>(&UMs[g][0][0][0])[e];
       }
-      int mine = 1;
-#pragma unroll
-      for (int j = 0; j < NG; ++j) {
-        double sj = 0.0;
-#pragma unroll
-        for (int b = 0; b < XCH_BLOCKS; ++b) {
-          if (b < nb) {
-            mine &= v[j][b] != PART_EMPTY;
-            sj += __builtin_bit_cast(double, v[j][b]);
-          }
-        }
-        gs[j] = sj;
-      }
-      if (__syncthreads_and(mine)) break;
-      if (spin >= (1 << 20)) {
-        ok = false;
-        break;
-      }
-      __builtin_amdgcn_s_sleep(1);
+      gs[j] = sj;
     }
-    if (!ok && tid == 0) __hip_atomic_fetch_or(p.err, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
     __shared__ double nrm[KM];
-    __shared__ double red[2][NW];
-    double *Gu = reinterpret_cast<double *>(&UMs[0][0][0]);
+    __shared__ double red[2][NTT / 64];
+    double *Gu = reinterpret_cast<double *>(&UMs[0][0][0][0]);
 #pragma unroll
-    for (int j = 0; j < NG; ++j) {
-      const int e = tid + NTH * j;
+    for (int j = 0; j < NGS; ++j) {
+      const int e = tid + NTT * j;
       if (e < kk + k) Gu[e] = gs[j];
     }
     __syncthreads();
     if (tid < k) {
       nrm[tid] = sqrt(Gu[tid + k * tid]);
-      if (blockIdx.x == 0) p.lambda[tid] = nrm[tid];
+      p.lambda[tid] = nrm[tid];
     }
     __syncthreads();
-    // This block's rows (still in registers) divided by the norms.
+    // This group's rows (still in registers) divided by the norms.
     {
-      const long i = (long)blockIdx.x * MR + rr;
+      const long i = r0 + rr;
       if (i < p.I) {
 #pragma unroll
         for (int u = 0; u < Q; ++u) {
@@ -575,8 +554,7 @@ __global__ __launch_bounds__(4 * MRT) void mode_rows_kernel(ModeParams p) {
         }
       }
     }
-    if (blockIdx.x != 0) return;
-    for (int e = tid; e < kk; e += NTH) {
+    for (int e = tid; e < kk; e += NTT) {
       const int a = e % k, b = e / k;
       const double sa = nrm[a] > 0 ? nrm[a] : 1.0, sb = nrm[b] > 0 ? nrm[b] : 1.0;
       p.gout[e] = Gu[e] / (sa * sb);
@@ -585,24 +563,24 @@ __global__ __launch_bounds__(4 * MRT) void mode_rows_kernel(ModeParams p) {
       // <X, model> = sum_c (u^T M)(c); ||model||^2 = sum (Gram u) .* S with
       // S the FP64 Hadamard of the other Grams built above.
       double t1 = 0.0, xh = 0.0;
-      for (int c = tid; c < k; c += NTH) t1 += Gu[kk + c];
+      for (int c = tid; c < k; c += NTT) t1 += Gu[kk + c];
 #pragma unroll
       for (int it = 0; it < NE; ++it) {
-        const int e = tid + it * NTH;
+        const int e = tid + it * NTT;
         const int a = e % KM, b = e / KM;
         if (e < KM * KM && a < k && b < k) xh += Gu[a + k * b] * dS[it];
       }
       t1 = wave_sum_d(t1);
       xh = wave_sum_d(xh);
       if (lane == 0) {
-        red[0][wave] = t1;
-        red[1][wave] = xh;
+        red[0][tid >> 6] = t1;
+        red[1][tid >> 6] = xh;
       }
       __syncthreads();
       if (tid == 0) {
         t1 = 0.0;
         xh = 0.0;
-        for (int w = 0; w < NW; ++w) {
+        for (int w = 0; w < NTT / 64; ++w) {
           t1 += red[0][w];
           xh += red[1][w];
         }
@@ -618,8 +596,6 @@ __global__ __launch_bounds__(4 * MRT) void mode_rows_kernel(ModeParams p) {
         }
       }
     }
-    // Every block has read the counter (it published after reading it).
-    if (tid == 0) __hip_atomic_fetch_add(p.ctl, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
@@ -794,7 +770,7 @@ bool mode_update_fits(long I, int k) { return k >= 1 && k <= 64 && I >= 1 && I <
 
 int mode_update(planc_ctx *ctx, int kind, float eps, float *F, long I, long ldf, int k, const float *M,
                 long mrs, long mcs, const double *const *grams, int ng, int skip, double *gout,
-                double *lambda, const double *normX2, double *rel_out, int *rel_counter, int *err) {
+                double *lambda, const double *normX2, double *rel_out, int *rel_counter) {
   if (!mode_update_fits(I, k) || ng < 1 || ng > 8 || skip < 0 || skip >= ng)
     return set_error(ctx, PLANC_EINVAL, "mode_update: bad arguments");
   ModeParams p{};
@@ -814,58 +790,53 @@ int mode_update(planc_ctx *ctx, int kind, float eps, float *F, long I, long ldf,
   p.eps = eps;
   p.normX2 = normX2;
   p.rel_out = rel_out;
+  p.rel_counter = rel_counter;
   const int mr = 64;
-  long nb = (I + mr - 1) / mr;
+  const long groups = (I + mr - 1) / mr;
+  // One block of `groups` row groups: up to 4 (1024 threads; LDS 2 k x 65
+  // floats per group) for k <= 48, up to 2 for k <= 64.
+  const char *oenv = getenv("PLANC_MODE_ONE_BLOCK");  // 0: always the two-launch form
+  const bool one = !(oenv && oenv[0] == '0') && groups <= (k <= 48 ? 4 : 2);
+  if (one) {
+#define ONE(KMV, G)                                                                     \
+  do {                                                                                  \
+    if (kind == 0) mode_rows_kernel<KMV, 0, 64, G><<<1, 256 * G, 0, ctx->stream>>>(p);      \
+    else if (kind == 1) mode_rows_kernel<KMV, 1, 64, G><<<1, 256 * G, 0, ctx->stream>>>(p); \
+    else mode_rows_kernel<KMV, 2, 64, G><<<1, 256 * G, 0, ctx->stream>>>(p);                \
+  } while (0)
+#define ONE_K(KMV)                    \
+  do {                                \
+    if (groups == 1) ONE(KMV, 1);     \
+    else if (groups == 2) ONE(KMV, 2); \
+    else if (groups == 3) ONE(KMV, 3); \
+    else ONE(KMV, 4);                 \
+  } while (0)
+    if (k <= 16) ONE_K(16);
+    else if (k <= 32) ONE_K(32);
+    else if (k <= 48) ONE_K(48);
+    else if (groups == 1) ONE(64, 1);
+    else ONE(64, 2);
+#undef ONE_K
+#undef ONE
+    PLANC_LAUNCH_CHECK(ctx);
+    return PLANC_OK;
+  }
+  long nb = groups;
   if (nb > ctx->num_cu) nb = ctx->num_cu;
   p.part = (double *)ws_get(ctx, 18, sizeof(double) * (size_t)nb * (k * k + k));
   if (!p.part) return set_error(ctx, PLANC_ENOMEM, "mode_update: workspace");
-  p.rel_counter = rel_counter;
-  const char *xenv = getenv("PLANC_MODE_EXCHANGE");  // 0: always two launches
-  const bool xch = err != nullptr && nb <= XCH_BLOCKS && k <= 48 && !(xenv && xenv[0] == '0');
-  if (xch) {
-    // Both slot buffers start empty and the launch counter at zero; after
-    // that the kernels keep them so (see mode_rows_kernel).
-    const bool fresh = ctx->ws[19].ptr == nullptr || ctx->ws[21].ptr == nullptr;
-    p.slots = (unsigned long long *)ws_get(ctx, 19, sizeof(unsigned long long) * 2 * XCH_HALF);
-    p.ctl = (int *)ws_get(ctx, 21, 4 * sizeof(int));
-    if (!p.slots || !p.ctl) return set_error(ctx, PLANC_ENOMEM, "mode_update: workspace");
-    if (fresh) {
-      xch_init_kernel<<<(unsigned)((2 * XCH_HALF + 255) / 256), 256, 0, ctx->stream>>>(p.slots, 2 * XCH_HALF,
-                                                                                      p.ctl);
-      PLANC_LAUNCH_CHECK(ctx);
-    }
-    p.err = err;
-  }
-#define ROWS_X(KMV, MRV, X)                                                           \
-  do {                                                                                \
-    if (kind == 0)                                                                    \
-      mode_rows_kernel<KMV, 0, MRV, X><<<(unsigned)nb, 4 * MRV, 0, ctx->stream>>>(p); \
-    else if (kind == 1)                                                               \
-      mode_rows_kernel<KMV, 1, MRV, X><<<(unsigned)nb, 4 * MRV, 0, ctx->stream>>>(p); \
-    else                                                                              \
-      mode_rows_kernel<KMV, 2, MRV, X><<<(unsigned)nb, 4 * MRV, 0, ctx->stream>>>(p); \
+#define ROWS(KMV)                                                                    \
+  do {                                                                               \
+    if (kind == 0) mode_rows_kernel<KMV, 0, 64, 0><<<(unsigned)nb, 256, 0, ctx->stream>>>(p);      \
+    else if (kind == 1) mode_rows_kernel<KMV, 1, 64, 0><<<(unsigned)nb, 256, 0, ctx->stream>>>(p); \
+    else mode_rows_kernel<KMV, 2, 64, 0><<<(unsigned)nb, 256, 0, ctx->stream>>>(p);                \
   } while (0)
-  if (k <= 16) {
-    if (xch)
-      ROWS_X(16, 64, true);
-    else
-      ROWS_X(16, 64, false);
-  } else if (k <= 32) {
-    if (xch)
-      ROWS_X(32, 64, true);
-    else
-      ROWS_X(32, 64, false);
-  } else if (k <= 48) {
-    if (xch)
-      ROWS_X(48, 64, true);
-    else
-      ROWS_X(48, 64, false);
-  } else {
-    ROWS_X(64, 64, false);
-  }
-#undef ROWS_X
+  if (k <= 16) ROWS(16);
+  else if (k <= 32) ROWS(32);
+  else if (k <= 48) ROWS(48);
+  else ROWS(64);
+#undef ROWS
   PLANC_LAUNCH_CHECK(ctx);
-  if (xch) return PLANC_OK;
   long fb = (I * k + 255) / 256;
   if (fb > 1024) fb = 1024;
   mode_finalize_kernel<<<(unsigned)fb, 256, 0, ctx->stream>>>(p, (int)nb);

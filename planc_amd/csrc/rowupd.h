@@ -2,7 +2,7 @@
 // and NTF kernels).  Lane q of the quad holds the Q = KM/4 consecutive row
 // values t = q*Q .. q*Q+Q-1 of h and of its numerator row; St is the k x k
 // FP32 Gram in LDS, transposed (St[x][t] = G(t, x)).  A column dot product
-// h . G(:, x) is the quad sum of the four per-lane partial chains.
+// h . G(:, x) is the quad sum of the four lanes' partial dots (lane_dot).
 #ifndef PLANC_AMD_ROWUPD_H_
 #define PLANC_AMD_ROWUPD_H_
 
@@ -27,7 +27,7 @@ __device__ inline float quad_sum(float v) {
 // read and written through selects (small code, for kernels that run once
 // per call and are bound by instruction fetch); otherwise the loop is fully
 // unrolled with static register indexes and no branch per column: the Gram
-// row of the next column is read from LDS while this one is computed, the
+// rows of the next columns are read from LDS while this one is computed, the
 // columns go in groups of four, and a group's columns at or past k (zero
 // Gram rows and numerators, as every caller pads them) are computed and then
 // zeroed instead of branched around (fewer instructions per column, for
@@ -37,6 +37,18 @@ __device__ inline void load_st_row(const float (&St)[KM][KM], int x, int q, floa
   const float *src = &St[x][q * (KM / 4)];
 #pragma unroll
   for (int u = 0; u < KM / 4; ++u) dst[u] = src[u];
+}
+
+// The lane's part of h . s: four interleaved chains (u mod 4) summed as
+// ((c0 + c1) + (c2 + c3)), a dependent chain of Q/4 + 2 operations instead
+// of Q (the HALS sweep waits on one such dot per column).
+template <int Q>
+__device__ inline float lane_dot(const float (&h)[Q], const float (&s)[Q]) {
+  static_assert(Q % 4 == 0, "Q is KM / 4 with KM a multiple of 16");
+  float c[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int u = 0; u < Q; ++u) c[u & 3] = fmaf(h[u], s[u], c[u & 3]);
+  return (c[0] + c[1]) + (c[2] + c[3]);
 }
 
 template <int KM>
@@ -70,16 +82,20 @@ __device__ inline void quad_row_update(float (&h)[KM / 4], const float (&nv)[KM 
         for (int u = 0; u < Q; ++u) h[u] = (own && u == xo) ? v : h[u];
       }
     } else {
-      float sx[2][Q];
-      load_st_row<KM>(St, 0, q, sx[0]);
+      // Gram rows of the next PF columns in flight while column x is
+      // updated; the scheduling barrier keeps their loads ahead of the
+      // column's arithmetic (left alone, the compiler sinks each load to
+      // just before its use and every column waits out the LDS latency).
+      constexpr int PF = 2;
+      float sx[PF + 1][Q];
+#pragma unroll
+      for (int j = 0; j < PF; ++j) load_st_row<KM>(St, j, q, sx[j]);
 #pragma unroll
       for (int x = 0; x < KM; ++x) {
         if ((x & 3) == 0 && x >= k) break;
-        if (x + 1 < KM) load_st_row<KM>(St, x + 1, q, sx[(x + 1) & 1]);  // next column in flight
-        float ps = 0.0f;
-#pragma unroll
-        for (int u = 0; u < Q; ++u) ps = fmaf(h[u], sx[x & 1][u], ps);
-        ps = quad_sum(ps);
+        if (x + PF < KM) load_st_row<KM>(St, x + PF, q, sx[(x + PF) % (PF + 1)]);
+        __builtin_amdgcn_sched_barrier(0);
+        const float ps = quad_sum(lane_dot<Q>(h, sx[x % (PF + 1)]));
         float v = h[x % Q] + (nv[x % Q] - ps);
         v = v < 1e-17f ? 1e-17f : v;
         h[x % Q] = q == x / Q ? v : h[x % Q];
@@ -115,10 +131,7 @@ __device__ inline void quad_row_update(float (&h)[KM / 4], const float (&nv)[KM 
       for (int c = 0; c < KM; ++c) {
         if ((c & 3) == 0 && c >= k) break;
         if (c + 1 < KM) load_st_row<KM>(St, c + 1, q, sc[(c + 1) & 1]);
-        float ps = 0.0f;
-#pragma unroll
-        for (int u = 0; u < Q; ++u) ps = fmaf(h[u], sc[c & 1][u], ps);
-        ps = quad_sum(ps);
+        const float ps = quad_sum(lane_dot<Q>(h, sc[c & 1]));
         const float v = (h[c % Q] * nv[c % Q]) / (ps + eps);
         hn[c % Q] = q == c / Q ? v : hn[c % Q];
       }

@@ -315,21 +315,22 @@ def test_ntf_trajectory(ctx, algo, dimtree, dims, k):
 
 
 @pytest.mark.parametrize("algo", [P.MU, P.HALS, P.ANLSBPP])
-@pytest.mark.parametrize("dims,k", [((256, 100, 130), 20), ((300, 100, 20), 40), ((128, 128, 64, 40), 48)])
-def test_ntf_mode_exchange_matches_two_launches(ctx, algo, dims, k, monkeypatch):
-    """The one-launch mode update (blocks exchange their partials through
-    device slots) gives the factors, lambda and errors of the two-launch
-    form, over modes of one to four row blocks and beyond (both forms in one
-    run), through the captured-graph iterations and on a second call (slot
-    buffers and launch counter carried over)."""
-    say(f"ntf exchange algo={algo} dims={dims} k={k}")
+@pytest.mark.parametrize("dims,k", [((256, 100, 130), 20), ((300, 100, 20), 40), ((128, 128, 64, 40), 48),
+                                    ((128, 100, 64), 64)])
+def test_ntf_one_block_mode_update_matches_two_launches(ctx, algo, dims, k, monkeypatch):
+    """The one-block mode update (up to four 64-row groups that combine their
+    partials in LDS) gives the factors and lambda of the two-launch form bit
+    for bit and its errors to 1e-12, over modes of one to four row groups and
+    beyond (both forms in one run), through the captured-graph iterations
+    and on a second call."""
+    say(f"ntf one-block mode update algo={algo} dims={dims} k={k}")
     X, _ = D.cp_tensor(dims, k, 0.01, stream=5)
     X32 = X.astype(np.float32)
     F0 = [f.astype(np.float32) for f in D.init_factors([(d, k) for d in dims], stream=5)]
     dX = P.colmajor(X32)
     runs = []
     for env in ("0", "1", "1"):
-        monkeypatch.setenv("PLANC_MODE_EXCHANGE", env)
+        monkeypatch.setenv("PLANC_MODE_ONE_BLOCK", env)
         dF = [P.colmajor(f) for f in F0]
         lam, rel = ctx.ntf_run(dX, dF, algo, 11)
         runs.append(([f.cpu().numpy() for f in dF], lam.cpu().numpy(), rel))
