@@ -342,8 +342,10 @@ int nmf_loop(planc_ctx *ctx, const planc_nmf_desc *desc, NMFProducts &P, float *
       // <W, A H> with the H of this iteration equals tr(H^T A^T W) of
       // computeObjectiveError (common/nmf.hpp:391-396) without the third
       // product; both traces are invariant under normalize_by_W.
-      RC(dot(ctx, d_W, 1, m, AH, 1, m, m, k, dotv, 0));
-      RC(nmf_relerr(ctx, normA2, dotv, WtW2, HtH, k, rel + it, nullptr));
+      double *dparts = nullptr;
+      long ndp = 0;
+      RC(dot_parts(ctx, d_W, 1, m, AH, 1, m, m, k, &dparts, &ndp));
+      RC(nmf_relerr_parts(ctx, normA2, dparts, ndp, WtW2, HtH, k, rel + it));
     }
     if (algo == PLANC_HALS) {
       // normalize_by_W: the column norms are the square roots of diag(W^T W);
@@ -352,7 +354,7 @@ int nmf_loop(planc_ctx *ctx, const planc_nmf_desc *desc, NMFProducts &P, float *
       RC(scale_cols(ctx, d_W, m, m, d_H, n, n, k, wnorms));
       RC(gram_rescale(ctx, WtW2, k, wnorms, WtW));
     } else {
-      PLANC_HIP_CHECK(ctx, hipMemcpyAsync(WtW, WtW2, kk * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+      std::swap(WtW, WtW2);  // this W^T W serves the next H half-step
     }
     wtw_valid = true;
   }

@@ -212,6 +212,12 @@ int dot(planc_ctx *ctx, const float *X, long xrs, long xcs, const float *Y, long
         long rows, long cols, double *out, int accumulate);
 int nmf_relerr(planc_ctx *ctx, const double *normA2, const double *dotv, const double *WtW,
                const double *HtH, int k, double *relerr_out, double *fit_out);
+// dot() without its final sum: the block partials (workspace slot 3) and
+// their count; nmf_relerr_parts sums them inside the error kernel.
+int dot_parts(planc_ctx *ctx, const float *X, long xrs, long xcs, const float *Y, long yrs, long ycs,
+              long rows, long cols, double **part, long *nparts);
+int nmf_relerr_parts(planc_ctx *ctx, const double *normA2, const double *part, long nparts, const double *WtW,
+                     const double *HtH, int k, double *relerr_out);
 int apply_reg(planc_ctx *ctx, double *G, int k, const double *reg);
 // hi = X rounded to dtype's 16-bit format (as float), lo = X - hi; Y += X.
 // G += sr I, R(j, c) += sr F(j, c) (R element (j, c) at R[j*rs + c*cs]); no-op for sr <= 0.

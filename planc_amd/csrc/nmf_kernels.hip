@@ -45,20 +45,32 @@ __global__ __launch_bounds__(NTHR) void read_probe_kernel(const floatx4 *p, long
 }
 
 // ---------------------------------------------------------------------------
-// G = X^T X on FP64 MFMA (v_mfma_f64_16x16x4f64): 64-row chunks of X go
-// through LDS (the next chunk's loads are in flight during the current
-// chunk's MFMAs); wave w owns the upper-triangle 16 x 16 tiles w, w+4, ...
-// of the KM x KM Gram and accumulates them over its block's chunks.  One
-// full KM x KM partial per block (mirrored), summed by gram_reduce_kernel.
+// G = X^T X on FP64 MFMA (v_mfma_f64_16x16x4f64).  A block is four groups
+// of four waves (1024 threads: four waves per SIMD, so one wave's dependent
+// MFMAs and LDS reads overlap another's); at step s group g takes the
+// 64-row chunk (4 s + g) gridDim.x + blockIdx.x through its own LDS tile
+// (the next chunk's loads in flight during the current chunk's MFMAs), so a
+// small X still spreads over all blocks.  Wave w of a
+// group owns the upper-triangle 16 x 16 tiles w, w+4, ... of the KM x KM
+// Gram; its tiles are interleaved and each goes through two accumulators
+// (even and odd row steps), so consecutive MFMAs are independent.  The four
+// groups' accumulators are combined in LDS as (g0 + g2) + (g1 + g3) and the
+// block writes one full KM x KM partial (mirrored), summed by
+// gram_reduce_kernel.
 constexpr int GCR = 64;
+constexpr int GGROUPS = 4;
 
 template <int KM>
-__global__ __launch_bounds__(NTHR) void gram_mfma_kernel(const float *X, long rows, int k, long ldx,
-                                                         double *part) {
+__global__ __launch_bounds__(NTHR * GGROUPS) void gram_mfma_kernel(const float *X, long rows, int k, long ldx,
+                                                                   double *part) {
   constexpr int NT = KM / 16, NUP = NT * (NT + 1) / 2, TPW = (NUP + 3) / 4;
   constexpr int PER = GCR * KM / NTHR;  // chunk values per thread
-  __shared__ float Xs[KM][GCR + 1];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  constexpr int XS = KM * (GCR + 1);    // floats of one group's tile
+  static_assert(GGROUPS * XS * 4 >= 2 * NTHR * TPW * 32, "the combine buffer must fit the tiles");
+  __shared__ __attribute__((aligned(32))) float Xall[GGROUPS * XS];
+  const int tid = threadIdx.x, grp = tid / NTHR, gt = tid - grp * NTHR;
+  const int lane = tid & 63, wave = gt >> 6;
+  float(*Xs)[GCR + 1] = reinterpret_cast<float(*)[GCR + 1]>(Xall + grp * XS);
   auto tile_of = [&](int t, int &ta, int &tb) {
     ta = 0;
     int rem = t;
@@ -71,41 +83,76 @@ __global__ __launch_bounds__(NTHR) void gram_mfma_kernel(const float *X, long ro
   auto load = [&](long r0, float(&v)[PER]) {
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
-      const int e = tid + NTHR * j, r = e % GCR, c = e / GCR;
+      const int e = gt + NTHR * j, r = e % GCR, c = e / GCR;
       const long gr = r0 + r;
       v[j] = (gr < rows && c < k) ? X[gr + ldx * c] : 0.0f;
     }
   };
-  doublex4 acc[TPW];
+  const float *ua[TPW], *ub[TPW];
+  bool on[TPW];
 #pragma unroll
-  for (int j = 0; j < TPW; ++j) acc[j] = doublex4{0.0, 0.0, 0.0, 0.0};
-  const long stride = (long)gridDim.x * GCR;
+  for (int j = 0; j < TPW; ++j) {
+    const int t = wave + 4 * j;
+    on[j] = t < NUP;
+    int ta = 0, tb = 0;
+    if (on[j]) tile_of(t, ta, tb);
+    ua[j] = &Xs[16 * ta + (lane & 15)][0];
+    ub[j] = &Xs[16 * tb + (lane & 15)][0];
+  }
+  doublex4 acc[TPW], acc2[TPW];
+#pragma unroll
+  for (int j = 0; j < TPW; ++j) {
+    acc[j] = doublex4{0.0, 0.0, 0.0, 0.0};
+    acc2[j] = doublex4{0.0, 0.0, 0.0, 0.0};
+  }
+  // Every group runs the same number of chunk steps (the barriers are
+  // block-wide); a group past the last chunk loads zeros.
+  const long cstride = (long)gridDim.x * GGROUPS * GCR;
+  const long nsteps = (rows + cstride - 1) / cstride;
   float v[PER];
-  long r0 = (long)blockIdx.x * GCR;
-  if (r0 < rows) load(r0, v);
-  for (; r0 < rows; r0 += stride) {
+  long r0 = ((long)grp * gridDim.x + blockIdx.x) * GCR;
+  load(r0, v);
+  for (long st = 0; st < nsteps; ++st, r0 += cstride) {
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
-      const int e = tid + NTHR * j;
+      const int e = gt + NTHR * j;
       Xs[e / GCR][e % GCR] = v[j];
     }
     __syncthreads();
-    if (r0 + stride < rows) load(r0 + stride, v);
+    if (st + 1 < nsteps) load(r0 + cstride, v);
 #pragma unroll
-    for (int j = 0; j < TPW; ++j) {
-      const int t = wave + 4 * j;
-      if (t < NUP) {
-        int ta, tb;
-        tile_of(t, ta, tb);
-        const float *ua = &Xs[16 * ta + (lane & 15)][0];
-        const float *ub = &Xs[16 * tb + (lane & 15)][0];
-#pragma unroll 4
-        for (int r = lane >> 4; r < GCR; r += 4)
-          acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)ua[r], (double)ub[r], acc[j], 0, 0, 0);
-      }
+    for (int r = 0; r < GCR / 4; r += 2) {
+      const int ra = (lane >> 4) + 4 * r, rb = ra + 4;
+#pragma unroll
+      for (int j = 0; j < TPW; ++j)
+        if (on[j]) {
+          acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)ua[j][ra], (double)ub[j][ra], acc[j], 0, 0, 0);
+          acc2[j] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)ua[j][rb], (double)ub[j][rb], acc2[j], 0, 0, 0);
+        }
     }
   }
+#pragma unroll
+  for (int j = 0; j < TPW; ++j) acc[j] += acc2[j];
+  // Combine the groups: (g0 + g2) + (g1 + g3), through the tile space.
+  doublex4 *cb = reinterpret_cast<doublex4 *>(Xall);  // [2][NTHR][TPW]
+  auto slot = [&](int half, int j) -> doublex4 & { return cb[(half * NTHR + gt) * TPW + j]; };
+  __syncthreads();
+  if (grp >= 2)
+#pragma unroll
+    for (int j = 0; j < TPW; ++j) slot(grp - 2, j) = acc[j];
+  __syncthreads();
+  if (grp < 2)
+#pragma unroll
+    for (int j = 0; j < TPW; ++j) acc[j] += slot(grp, j);
+  __syncthreads();
+  if (grp == 1)
+#pragma unroll
+    for (int j = 0; j < TPW; ++j) slot(0, j) = acc[j];
+  __syncthreads();
+  if (grp != 0) return;
+#pragma unroll
+  for (int j = 0; j < TPW; ++j) acc[j] += slot(0, j);
   // FP64 D layout: lane holds D(lane / 16 + 4 v, lane % 16).
   double *dst = part + (long)blockIdx.x * KM * KM;
 #pragma unroll
@@ -188,10 +235,13 @@ __device__ inline float dot_gcol(const float (&h)[KM], const float (&Gt)[KM][KM]
 // nmf/hals.hpp:67-75, ntf/ntfhals.hpp:13-29: columns in order, each using
 // the already-updated columns before it), N(j, c) at N[j*nrs + c*ncs].  A
 // quad of lanes per row (quad_row_update), 64 rows per block.
-template <int KM>
+// KIND is a template argument so that each kernel carries only its own
+// rule's registers (the HALS sweep keeps more Gram rows in flight).
+template <int KM, int KIND>
 __global__ __launch_bounds__(NTHR) void rows_quad_kernel(float *H, long n, int k, long ldh,
                                                          const float *N, long nrs, long ncs,
-                                                         const double *G, int kind, float eps) {
+                                                         const double *G, float eps) {
+  constexpr int kind = KIND;
   constexpr int Q = KM / 4;
   __shared__ __attribute__((aligned(16))) float Gs[KM][KM];
   load_gram<KM>(Gs, G, k);
@@ -470,21 +520,43 @@ __global__ __launch_bounds__(NTHR) void dot_partial_kernel(const float *X, long 
   if (threadIdx.x == 0) part[blockIdx.x] = bs;
 }
 
-__global__ void sum_partials_kernel(const double *part, long n, double *out, int accumulate) {
-  if (threadIdx.x >= 64) return;
+// Sum of n block partials by one block of NTHR threads: thread j adds
+// partials j, j + NTHR, ... with all its loads in flight at once (PSUM_MAX
+// per thread), then the fixed-order block sum.
+constexpr int PSUM_MAX = 8;  // partials per thread: n <= NTHR * PSUM_MAX
+__device__ inline double sum_parts(const double *part, long n, double *red) {
+  double v[PSUM_MAX];
+#pragma unroll
+  for (int u = 0; u < PSUM_MAX; ++u) {
+    const long b = threadIdx.x + (long)NTHR * u;
+    v[u] = b < n ? part[b] : 0.0;
+  }
   double t = 0.0;
-  for (long b = threadIdx.x; b < n; b += 64) t += part[b];
-  t = wave_sum(t);
+#pragma unroll
+  for (int u = 0; u < PSUM_MAX; ++u) t += v[u];
+  for (long b = threadIdx.x + (long)NTHR * PSUM_MAX; b < n; b += NTHR) t += part[b];  // longer lists
+  return block_sum(t, red);
+}
+
+__global__ __launch_bounds__(NTHR) void sum_partials_kernel(const double *part, long n, double *out,
+                                                           int accumulate) {
+  __shared__ double red[NTHR / 64];
+  const double t = sum_parts(part, n, red);
   if (threadIdx.x == 0) *out = accumulate ? (*out + t) : t;
 }
 
 // NMF relative error from the trace identity (common/nmf.hpp:390-424):
 // fit^2 = ||A||^2 - 2 <W, A H> + sum(WtW .* HtH); printObjective prints
-// sqrt(fit^2)/||A||, or 1 when fit^2 <= 0 (common/nmf.hpp:462).
+// sqrt(fit^2)/||A||, or 1 when fit^2 <= 0 (common/nmf.hpp:462).  <W, A H>
+// is *dotv, or (dotv null) the sum of the nparts block partials of a dot
+// kernel, taken here as sum_partials_kernel takes it.
 __global__ __launch_bounds__(NTHR) void nmf_relerr_kernel(const double *normA2, const double *dotv,
                                                           const double *WtW, const double *HtH, int k,
-                                                          double *relerr_out, double *fit_out) {
+                                                          double *relerr_out, double *fit_out,
+                                                          const double *part, long nparts) {
   __shared__ double red[NTHR / 64];
+  const double dsum = dotv ? 0.0 : sum_parts(part, nparts, red);
+  __syncthreads();  // red is reused below
   double tr = 0.0;
   for (int e = threadIdx.x; e < k * k; e += NTHR) {
     const int a = e % k, b = e / k;
@@ -492,7 +564,7 @@ __global__ __launch_bounds__(NTHR) void nmf_relerr_kernel(const double *normA2, 
   }
   tr = block_sum(tr, red);
   if (threadIdx.x != 0) return;
-  double fit = *normA2 - 2.0 * (*dotv) + tr;
+  double fit = *normA2 - 2.0 * (dotv ? *dotv : dsum) + tr;
   double nA = sqrt(*normA2);
   double err = (fit > 0) ? sqrt(fit) : nA;
   *relerr_out = err / nA;
@@ -661,7 +733,10 @@ int launch_rows(planc_ctx *ctx, int kind, float *H, long n, int k, long ldh, con
   const long rows_per_block = NTHR / 4;
   unsigned nb = (unsigned)((n + rows_per_block - 1) / rows_per_block);
   if (nb == 0) return PLANC_OK;
-  rows_quad_kernel<KM><<<nb, NTHR, 0, ctx->stream>>>(H, n, k, ldh, N, nrs, ncs, G, kind == 0 ? 0 : 1, eps);
+  if (kind == 0)
+    rows_quad_kernel<KM, 0><<<nb, NTHR, 0, ctx->stream>>>(H, n, k, ldh, N, nrs, ncs, G, eps);
+  else
+    rows_quad_kernel<KM, 1><<<nb, NTHR, 0, ctx->stream>>>(H, n, k, ldh, N, nrs, ncs, G, eps);
   PLANC_LAUNCH_CHECK(ctx);
   return PLANC_OK;
 }
@@ -764,14 +839,15 @@ int gram(planc_ctx *ctx, const float *X, long rows, int k, long ldx, double *G) 
   // still writes zero partials).
   double *part = (double *)ws_get(ctx, 1, sizeof(double) * (size_t)nb * kp * kp);
   if (!part) return set_error(ctx, PLANC_ENOMEM, "gram: workspace");
+  const unsigned nt = NTHR * GGROUPS;
   if (kp == 16)
-    gram_mfma_kernel<16><<<(unsigned)nb, NTHR, 0, ctx->stream>>>(X, rows, k, ldx, part);
+    gram_mfma_kernel<16><<<(unsigned)nb, nt, 0, ctx->stream>>>(X, rows, k, ldx, part);
   else if (kp == 32)
-    gram_mfma_kernel<32><<<(unsigned)nb, NTHR, 0, ctx->stream>>>(X, rows, k, ldx, part);
+    gram_mfma_kernel<32><<<(unsigned)nb, nt, 0, ctx->stream>>>(X, rows, k, ldx, part);
   else if (kp == 48)
-    gram_mfma_kernel<48><<<(unsigned)nb, NTHR, 0, ctx->stream>>>(X, rows, k, ldx, part);
+    gram_mfma_kernel<48><<<(unsigned)nb, nt, 0, ctx->stream>>>(X, rows, k, ldx, part);
   else
-    gram_mfma_kernel<64><<<(unsigned)nb, NTHR, 0, ctx->stream>>>(X, rows, k, ldx, part);
+    gram_mfma_kernel<64><<<(unsigned)nb, nt, 0, ctx->stream>>>(X, rows, k, ldx, part);
   PLANC_LAUNCH_CHECK(ctx);
   gram_reduce_kernel<<<(unsigned)((kp * kp + 63) / 64), NTHR, 0, ctx->stream>>>(part, nb, k, kp, G);
   PLANC_LAUNCH_CHECK(ctx);
@@ -822,6 +898,29 @@ int scale_cols(planc_ctx *ctx, float *X, long rows, long ldx, float *Y, long yro
 
 // *out = sum_{r,c} X(r,c) Y(r,c) in FP64 (accumulate: *out += ...), with
 // X(r,c) = X[r*xrs + c*xcs] and likewise Y.
+// The partial kernel of dot(): one FP64 partial per block into workspace
+// slot 3 (*part, *nparts).
+int dot_parts(planc_ctx *ctx, const float *X, long xrs, long xcs, const float *Y, long yrs, long ycs,
+              long rows, long cols, double **part_out, long *nparts) {
+  long tot = rows * cols;
+  long nb = (tot + NTHR * 8 - 1) / (NTHR * 8);
+  if (nb > 2048) nb = 2048;
+  if (nb < 1) nb = 1;
+  double *part = (double *)ws_get(ctx, 3, sizeof(double) * (size_t)nb);
+  if (!part) return set_error(ctx, PLANC_ENOMEM, "dot: workspace");
+  const bool flat = xrs == 1 && yrs == 1 && (cols == 1 || (xcs == rows && ycs == rows)) &&
+                    (uintptr_t)X % 16 == 0 && (uintptr_t)Y % 16 == 0;
+  if (flat)
+    dot_flat_kernel<<<(unsigned)nb, NTHR, 0, ctx->stream>>>(X, Y, tot, part);
+  else
+    dot_partial_kernel<<<(unsigned)nb, NTHR, 0, ctx->stream>>>(X, xrs, xcs, Y, yrs, ycs, rows, cols,
+                                                               part);
+  PLANC_LAUNCH_CHECK(ctx);
+  *part_out = part;
+  *nparts = nb;
+  return PLANC_OK;
+}
+
 int dot(planc_ctx *ctx, const float *X, long xrs, long xcs, const float *Y, long yrs, long ycs,
         long rows, long cols, double *out, int accumulate) {
   long tot = rows * cols;
@@ -838,14 +937,23 @@ int dot(planc_ctx *ctx, const float *X, long xrs, long xcs, const float *Y, long
     dot_partial_kernel<<<(unsigned)nb, NTHR, 0, ctx->stream>>>(X, xrs, xcs, Y, yrs, ycs, rows, cols,
                                                                part);
   PLANC_LAUNCH_CHECK(ctx);
-  sum_partials_kernel<<<1, 64, 0, ctx->stream>>>(part, nb, out, accumulate);
+  sum_partials_kernel<<<1, NTHR, 0, ctx->stream>>>(part, nb, out, accumulate);
   PLANC_LAUNCH_CHECK(ctx);
   return PLANC_OK;
 }
 
 int nmf_relerr(planc_ctx *ctx, const double *normA2, const double *dotv, const double *WtW,
                const double *HtH, int k, double *relerr_out, double *fit_out) {
-  nmf_relerr_kernel<<<1, NTHR, 0, ctx->stream>>>(normA2, dotv, WtW, HtH, k, relerr_out, fit_out);
+  nmf_relerr_kernel<<<1, NTHR, 0, ctx->stream>>>(normA2, dotv, WtW, HtH, k, relerr_out, fit_out, nullptr, 0);
+  PLANC_LAUNCH_CHECK(ctx);
+  return PLANC_OK;
+}
+
+// The NMF error of <W, A H> = sum of the dot partials of dot_parts(): the
+// partial sum and the error in one launch.
+int nmf_relerr_parts(planc_ctx *ctx, const double *normA2, const double *part, long nparts, const double *WtW,
+                     const double *HtH, int k, double *relerr_out) {
+  nmf_relerr_kernel<<<1, NTHR, 0, ctx->stream>>>(normA2, nullptr, WtW, HtH, k, relerr_out, nullptr, part, nparts);
   PLANC_LAUNCH_CHECK(ctx);
   return PLANC_OK;
 }

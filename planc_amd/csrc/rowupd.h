@@ -41,7 +41,8 @@ __device__ inline void load_st_row(const float (&St)[KM][KM], int x, int q, floa
 
 // The lane's part of h . s: four interleaved chains (u mod 4) summed as
 // ((c0 + c1) + (c2 + c3)), a dependent chain of Q/4 + 2 operations instead
-// of Q (the HALS sweep waits on one such dot per column).
+// of Q (the HALS sweep waits on one such dot per column; MU's columns are
+// independent and keep one chain each, fewer registers).
 template <int Q>
 __device__ inline float lane_dot(const float (&h)[Q], const float (&s)[Q]) {
   static_assert(Q % 4 == 0, "Q is KM / 4 with KM a multiple of 16");
@@ -131,7 +132,10 @@ __device__ inline void quad_row_update(float (&h)[KM / 4], const float (&nv)[KM 
       for (int c = 0; c < KM; ++c) {
         if ((c & 3) == 0 && c >= k) break;
         if (c + 1 < KM) load_st_row<KM>(St, c + 1, q, sc[(c + 1) & 1]);
-        const float ps = quad_sum(lane_dot<Q>(h, sc[c & 1]));
+        float ps = 0.0f;  // columns are independent in MU: one chain per column
+#pragma unroll
+        for (int u = 0; u < Q; ++u) ps = fmaf(h[u], sc[c & 1][u], ps);
+        ps = quad_sum(ps);
         const float v = (h[c % Q] * nv[c % Q]) / (ps + eps);
         hn[c % Q] = q == c / Q ? v : hn[c % Q];
       }
