@@ -171,7 +171,7 @@ int planc_hadamard_grams(planc_ctx *ctx, const double *const *d_grams, int nmode
  * with the Khatri-Rao product formed on the fly, never stored.  Output is
  * k x I_mode (ld ldo).  Replaces Tensor::mttkrp (common/tensor.hpp:246-327)
  * plus krp_leave_out_one (common/ncpfactors.hpp:171-233).  d_factors is a
- * host array of device pointers (I_i x k, column-major).  nmodes <= 8. */
+ * host array of device pointers (I_i x k, column-major).  nmodes 2..7. */
 int planc_mttkrp(planc_ctx *ctx, int dtype, const void *d_X, int nmodes, const long *dims,
                  const float *const *d_factors, int k, int mode, float *d_out, long ldo);
 
@@ -251,6 +251,25 @@ int planc_cp_loss_fwd_noalloc(planc_ctx *ctx, const float *d_X, int nmodes, cons
  * Synchronises. */
 int planc_cp_loss_bwd(planc_ctx *ctx, const float *d_X, int nmodes, const long *dims,
                       const float *const *d_factors, int k, float *const *d_grads);
+
+/* One rank-2 NNLS update (DistR2::update, distnmf/distr2.hpp:35-61): for
+ * each of n columns, X(:, j) solves the 2 x 2 normal equations G x = R(:, j)
+ * (LU with partial pivoting, as arma::solve's fast path); a column with a
+ * negative entry is replaced by the better one-variable solution
+ * (r_c / G_cc, compared as sqrt(G_cc) r_c / G_cc).  d_G 2 x 2 column-major
+ * FP64; R(c, j) at d_R[j * r_js + c * r_cs], X(c, j) at d_X[j * x_js + c *
+ * x_cs] (c = 0, 1).  Stream-ordered. */
+int planc_r2_update(planc_ctx *ctx, const double *d_G, const float *d_R, long r_js, long r_cs, long n,
+                    float *d_X, long x_js, long x_cs);
+
+/* Relative error of NMF from the Gram trace identity (computeObjectiveError,
+ * common/nmf.hpp:390-424, with printObjective's relative form, :462):
+ * fit^2 = ||A||^2 - 2 tr(H^T A^T W) + sum(W^T W .* H^T H), *d_rel =
+ * sqrt(fit^2) / ||A|| (1 when fit^2 <= 0).  All inputs are device FP64:
+ * *d_normA2 = ||A||_F^2, *d_trace = tr(H^T A^T W) = <W, A H>, the k x k Grams.
+ * Stream-ordered. */
+int planc_rel_error_gram(planc_ctx *ctx, const double *d_normA2, const double *d_trace, const double *d_WtW,
+                         const double *d_HtH, int k, double *d_rel);
 
 /* Column subset of a CSR matrix (m x n): the node matrix A0(:, cols) of
  * hiernmf (Node::allocate, the sparse selection A0 * S, hiernmf/node.hpp:
@@ -402,7 +421,11 @@ typedef struct planc_ntf_desc {
  * hold the initial factors, which are first normalised as NCPFactors::
  * normalize() does in the AUNTF constructor (ntf/auntf.hpp:88); on return
  * they hold the unit-norm factors and d_lambda (k, FP64) the weights.
- * d_X16 optional 16-bit copy of X for dtype != F32. */
+ * d_X16: optional unscaled 16-bit copy of X for dtype != F32; without it the
+ * run makes its own copy, which with the dimension tree carries per-column
+ * power-of-two scales over the tree's L x R unfolding (so FP16 holds data
+ * beyond its 65504 range; the scales are folded into the Khatri-Rao rows of
+ * the right contraction and the output columns of the left one). */
 int planc_ntf_run(planc_ctx *ctx, const planc_ntf_desc *desc, const float *d_X, const void *d_X16,
                   float *const *d_factors, double *d_lambda, double *h_relerr);
 

@@ -345,6 +345,28 @@ class Context:
                                                garr), "planc_cp_loss_bwd")
         return grads
 
+    def r2_update(self, G, R):
+        """Rank-2 NNLS update of DistR2 (distnmf/distr2.hpp:35-61): G a 2 x 2
+        FP64 device Gram, R a 2 x n FP32 device right-hand side (column-major);
+        returns X (2 x n FP32)."""
+        torch = _torch()
+        assert G.dtype == torch.float64 and G.shape == (2, 2) and R.shape[0] == 2
+        n = R.shape[1]
+        X = empty_colmajor((2, n), device=R.device)
+        self._check(self.lib.planc_r2_update(self.ctx, _ptr(G), _ptr(R), _ld(R), 1, n, _ptr(X), _ld(X), 1),
+                    "planc_r2_update")
+        return X
+
+    def rel_error_gram(self, normA2, trace, WtW, HtH):
+        """Relative NMF error from the trace identity (device FP64 scalars and
+        k x k Grams); returns it as a host float."""
+        torch = _torch()
+        k = WtW.shape[0]
+        out = torch.empty(1, dtype=torch.float64, device=WtW.device)
+        self._check(self.lib.planc_rel_error_gram(self.ctx, _ptr(normA2), _ptr(trace), _ptr(WtW), _ptr(HtH), k,
+                                                  _ptr(out)), "planc_rel_error_gram")
+        return float(out.cpu()[0])
+
     def csr_col_subset(self, indptr, indices, data, n, cols):
         """Device CSR column subset A(:, cols) (cols sorted, unique).  indptr,
         indices, data are device tensors (int64, int32, float32)."""

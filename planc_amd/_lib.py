@@ -104,6 +104,9 @@ SIGNATURES = {
                                   c_void_p, c_long, c_long]),
     "planc_bpp": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_long, c_long, c_long, c_void_p,
                           c_long, c_long]),
+    "planc_r2_update": (c_int, [c_void_p, c_void_p, c_void_p, c_long, c_long, c_long, c_void_p, c_long,
+                                c_long]),
+    "planc_rel_error_gram": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "planc_csr_col_subset": (c_int, [c_void_p, c_long, c_long, c_void_p, c_void_p, c_void_p,
                                      ctypes.POINTER(c_long), c_long, c_void_p, c_void_p, c_void_p,
                                      ctypes.POINTER(c_long)]),

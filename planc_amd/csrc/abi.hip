@@ -559,6 +559,15 @@ int planc_hadamard_grams(planc_ctx *ctx, const double *const *d_grams, int nmode
   return hadamard_grams(ctx, d_grams, nmodes, skip, k, d_out);
 }
 
+int planc_rel_error_gram(planc_ctx *ctx, const double *d_normA2, const double *d_trace, const double *d_WtW,
+                         const double *d_HtH, int k, double *d_rel) {
+  if (check_ctx(ctx)) return PLANC_EINVAL;
+  if (!d_normA2 || !d_trace || !d_WtW || !d_HtH || !d_rel || k <= 0)
+    return set_error(ctx, PLANC_EINVAL, "planc_rel_error_gram: bad arguments");
+  PLANC_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  return nmf_relerr(ctx, d_normA2, d_trace, d_WtW, d_HtH, k, d_rel, nullptr);
+}
+
 int planc_mttkrp(planc_ctx *ctx, int dtype, const void *d_X, int nmodes, const long *dims,
                  const float *const *d_factors, int k, int mode, float *d_out, long ldo) {
   if (check_ctx(ctx)) return PLANC_EINVAL;
@@ -810,13 +819,40 @@ int planc_ntf_run(planc_ctx *ctx, const planc_ntf_desc *desc, const float *d_X, 
   for (int i = 0; i <= s; ++i) L *= dims[i];
   for (int i = s + 1; i < N; ++i) R *= dims[i];
   const void *Xop = d_X;
+  // Per-column power-of-two scales s_j of the 16-bit copy, columns of the
+  // dimension tree's L x R unfolding (the copy made here; a caller's copy is
+  // unscaled).  The left contraction takes s_j as scales of its output
+  // columns (FP32).  The right one contracts over j: its Khatri-Rao rows take
+  // r_j = s_j / S (S = max_j s_j, so the 16-bit operand stays at most 1 and
+  // within FP16's range) and its FP32 output is multiplied by S.
+  const float *xscale = nullptr, *xrscale = nullptr;
+  float xS = 1.0f;
   if (dt != PLANC_F32) {
     if (d_X16) {
       Xop = d_X16;
     } else {
       void *x16 = ws_get(ctx, 7, sizeof(uint16_t) * (size_t)numel);
       if (!x16) return set_error(ctx, PLANC_ENOMEM, "planc_ntf_run: 16-bit copy");
-      int rc = convert16(ctx, dt, d_X, numel, 1, numel, x16, numel, nullptr);
+      int rc;
+      if (dimtree) {
+        float *sc = (float *)ws_get(ctx, 8, sizeof(float) * 2 * (size_t)R);
+        if (!sc) return set_error(ctx, PLANC_ENOMEM, "planc_ntf_run: 16-bit scales");
+        rc = convert16(ctx, dt, d_X, L, R, L, x16, L, sc);
+        if (rc) return rc;
+        std::vector<float> hs((size_t)R);
+        PLANC_HIP_CHECK(ctx, hipMemcpyAsync(hs.data(), sc, sizeof(float) * (size_t)R, hipMemcpyDeviceToHost,
+                                            ctx->stream));
+        PLANC_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+        for (float v : hs) xS = std::max(xS, v);
+        for (float &v : hs) v /= xS;  // powers of two: exact
+        PLANC_HIP_CHECK(ctx, hipMemcpyAsync(sc + R, hs.data(), sizeof(float) * (size_t)R, hipMemcpyHostToDevice,
+                                            ctx->stream));
+        PLANC_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+        xscale = sc;
+        xrscale = sc + R;
+      } else {
+        rc = convert16(ctx, dt, d_X, numel, 1, numel, x16, numel, nullptr);
+      }
       if (rc) return rc;
       Xop = x16;
     }
@@ -946,9 +982,9 @@ int planc_ntf_run(planc_ctx *ctx, const planc_ntf_desc *desc, const float *d_X, 
             kd.ld[q - s - 1] = dims[q];
           }
           if (XopT)
-            RC(thin_gemm(ctx, dt, 1, XopT, R, L, R, nullptr, 0, &kd, nullptr, nullptr, k, P, L, 0, 0));
+            RC(thin_gemm(ctx, dt, 1, XopT, R, L, R, nullptr, 0, &kd, xrscale, nullptr, k, P, L, 0, 0, xS));
           else
-            RC(thin_gemm(ctx, dt, 0, Xop, L, L, R, nullptr, 0, &kd, nullptr, nullptr, k, P, L, 0, 0));
+            RC(thin_gemm(ctx, dt, 0, Xop, L, L, R, nullptr, 0, &kd, xrscale, nullptr, k, P, L, 0, 0, xS));
         }
         if (n == s + 1) {
           KrpDesc kd{};
@@ -958,7 +994,7 @@ int planc_ntf_run(planc_ctx *ctx, const planc_ntf_desc *desc, const float *d_X, 
             kd.dim[q] = dims[q];
             kd.ld[q] = dims[q];
           }
-          RC(thin_gemm(ctx, dt, 1, Xop, L, R, L, nullptr, 0, &kd, nullptr, nullptr, k, Q, R, 0, 0));
+          RC(thin_gemm(ctx, dt, 1, Xop, L, R, L, nullptr, 0, &kd, nullptr, xscale, k, Q, R, 0, 0));
         }
         if (n <= s) {
           RC(contract_projection(ctx, P, L, s + 1, dims, d_factors, ldfL, n, k, M, k));
@@ -1110,6 +1146,9 @@ int planc_ntf_run(planc_ctx *ctx, const planc_ntf_desc *desc, const float *d_X, 
       for (auto &w : ctx->ws) key.push_back((uintptr_t)w.ptr);
       const char *oe = getenv("PLANC_MODE_ONE_BLOCK");  // the mode-update form captured
       key.push_back(oe && oe[0] == '0' ? 0 : 1);
+      uint32_t xs_bits;  // the 16-bit copy's output scale is a captured kernel argument
+      std::memcpy(&xs_bits, &xS, sizeof xs_bits);
+      key.push_back(xs_bits);
       if (graph_lookup(ctx, key, exec)) return PLANC_OK;
       int rc2 = capture_graph(ctx, [&]() -> int {
         for (int u = 0; u < n_it; ++u) {

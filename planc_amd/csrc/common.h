@@ -192,10 +192,12 @@ struct KrpDesc {
 // product of krp.f (lowest factor fastest).  rowscale (K, optional) scales
 // the rows of B; colscale (L, optional, trans == 1) scales output columns.
 // cmajor selects the output layout: 1 -> C[c + ldc*l] (k x L), 0 -> C[l + ldc*c].
-// splits_hint > 0 forces the split-K factor (0: automatic).
+// splits_hint > 0 forces the split-K factor (0: automatic).  oscale (16-bit
+// A only) multiplies every output element after the FP32 accumulation.
 int thin_gemm(planc_ctx *ctx, int dtype, int trans, const void *A, long lda, long L, long K,
               const float *B, long ldb, const KrpDesc *krp, const float *rowscale,
-              const float *colscale, int k, float *C, long ldc, int cmajor, int splits_hint);
+              const float *colscale, int k, float *C, long ldc, int cmajor, int splits_hint,
+              float oscale = 1.0f);
 
 // nmf_kernels.hip
 int gram(planc_ctx *ctx, const float *X, long rows, int k, long ldx, double *G);

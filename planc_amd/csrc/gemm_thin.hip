@@ -188,6 +188,7 @@ struct StreamParams {
   float *part;
   int avec;   // 16-byte loads of A valid (lda and base aligned)
   int cvec;   // 16-byte stores of C valid
+  float oscale;  // every output element is multiplied by it (1: none; 16-bit products only)
 };
 
 // Sums the k-phase accumulators of a block in LDS (phase 0 .. wk-1 order).
@@ -240,7 +241,7 @@ __device__ inline void store_acc_tn(const StreamParams &p, long l0, int a, float
   const int cw = a / NT, t = a % NT;
   const long l = l0 + 16 * cw + lc;
   if (l >= p.L) return;
-  const float sc = p.colscale ? p.colscale[l] : 1.0f;
+  const float sc = (p.colscale ? p.colscale[l] : 1.0f) * p.oscale;
   v[0] *= sc;
   v[1] *= sc;
   v[2] *= sc;
@@ -280,6 +281,10 @@ __device__ inline void store_acc_nn16(const StreamParams &p, long l0, int a, flo
   const int t = a / 8, j = a % 8;
   const int c = 16 * t + lc;
   if (c >= p.th.k) return;
+  v[0] *= p.oscale;
+  v[1] *= p.oscale;
+  v[2] *= p.oscale;
+  v[3] *= p.oscale;
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
     const long l = l0 + 32 * lg + 8 * u + j;
@@ -1413,8 +1418,9 @@ __global__ __launch_bounds__(NTHR) void nn16_kernel(StreamParams p) {
 #pragma unroll
       for (int hf = 0; hf < 2; ++hf) {
         const long l = l0 + 32 * lg + 8 * v + 4 * hf;
-        const floatx4 o = {acc[8 * t + 4 * hf + 0][v], acc[8 * t + 4 * hf + 1][v], acc[8 * t + 4 * hf + 2][v],
-                           acc[8 * t + 4 * hf + 3][v]};
+        const floatx4 o = floatx4{acc[8 * t + 4 * hf + 0][v], acc[8 * t + 4 * hf + 1][v],
+                                  acc[8 * t + 4 * hf + 2][v], acc[8 * t + 4 * hf + 3][v]} *
+                          p.oscale;
         if (!p.cmajor && p.cvec && l + 3 < p.L) {
           *(floatx4 *)(p.C + l + p.ldc * c) = o;
         } else {
@@ -1449,6 +1455,7 @@ struct GemmParams {
   float *partial;
   int splits;
   long kchunk;
+  float oscale;  // output multiplier (1: none)
 };
 
 // NP == 2: the compensated thin operand (pack_thin16_kernel), formed here
@@ -1586,7 +1593,7 @@ __global__ __launch_bounds__(NTHR) void nn_h_kernel(GemmParams p) {
         long gl = l0 + wave * 32 + 16 * r + 4 * (lane >> 4) + i;
         int c = 16 * t + (lane & 15);
         if (gl >= p.L) continue;
-        float v = acc[r][t][i];
+        float v = acc[r][t][i] * (p.splits > 1 ? 1.0f : p.oscale);
         if (p.splits > 1) {
           p.partial[((long)split * p.L + gl) * KP + c] = v;
         } else if (c < p.th.k) {
@@ -1600,13 +1607,14 @@ __global__ __launch_bounds__(NTHR) void nn_h_kernel(GemmParams p) {
 
 // Sum of the 16-bit NN split-K partials (fixed order) into the final layout.
 __global__ void reduce_splits_kernel(const float *partial, int splits, long L, int KP, int k,
-                                     float *C, long ldc, int cmajor) {
+                                     float *C, long ldc, int cmajor, float oscale) {
   long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= L * k) return;
   long gl = idx / k;
   int c = (int)(idx % k);
   float s = 0.0f;
   for (int sp = 0; sp < splits; ++sp) s += partial[((long)sp * L + gl) * KP + c];
+  s *= oscale;
   if (cmajor)
     C[c + ldc * gl] = s;
   else
@@ -2048,8 +2056,9 @@ int fill_thin(planc_ctx *ctx, ThinDev &th, const float *B, long ldb, const KrpDe
 
 int thin_gemm(planc_ctx *ctx, int dtype, int trans, const void *A, long lda, long L, long K,
               const float *B, long ldb, const KrpDesc *krp, const float *rowscale,
-              const float *colscale, int k, float *C, long ldc, int cmajor, int splits_hint) {
+              const float *colscale, int k, float *C, long ldc, int cmajor, int splits_hint, float oscale) {
   if (L < 0 || K < 0 || k <= 0 || !A || !C) return set_error(ctx, PLANC_EINVAL, "thin_gemm: bad shape/pointer");
+  if (dtype == PLANC_F32 && oscale != 1.0f) return set_error(ctx, PLANC_EINVAL, "thin_gemm: output scale with FP32");
   if (dtype != PLANC_F32 && dtype != PLANC_BF16 && dtype != PLANC_F16)
     return set_error(ctx, PLANC_EINVAL, "thin_gemm: bad dtype");
   if (L == 0) return PLANC_OK;
@@ -2083,6 +2092,7 @@ int thin_gemm(planc_ctx *ctx, int dtype, int trans, const void *A, long lda, lon
       p.C = Cs;
       p.ldc = ldc;
       p.cmajor = cmajor;
+      p.oscale = oscale;
       const int BK = 64;
       long lblocks = (L + BL - 1) / BL;
       long want = (4L * ctx->num_cu + lblocks - 1) / lblocks;
@@ -2130,7 +2140,8 @@ int thin_gemm(planc_ctx *ctx, int dtype, int trans, const void *A, long lda, lon
       if (splits > 1) {
         long tot = L * kc;
         unsigned nb = (unsigned)((tot + 255) / 256);
-        reduce_splits_kernel<<<nb, 256, 0, ctx->stream>>>(p.partial, splits, L, KP, kc, Cs, ldc, cmajor);
+        reduce_splits_kernel<<<nb, 256, 0, ctx->stream>>>(p.partial, splits, L, KP, kc, Cs, ldc, cmajor,
+                                                          oscale);
         PLANC_LAUNCH_CHECK(ctx);
       }
       continue;
@@ -2149,6 +2160,7 @@ int thin_gemm(planc_ctx *ctx, int dtype, int trans, const void *A, long lda, lon
     p.cmajor = cmajor;
     p.avec = avec ? 1 : 0;
     p.cvec = (ldc % 4 == 0 && ((uintptr_t)Cs % 16 == 0)) ? 1 : 0;
+    p.oscale = oscale;
     int tm = thin_mode(th, avec, rowscale);
     if (!trans && L % 4 != 0) tm = THIN_SAFE;
     if (dtype != PLANC_F32) {

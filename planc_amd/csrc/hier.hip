@@ -152,6 +152,17 @@ __device__ inline void r2_solve(const double *G, double r0, double r1, double &x
   x1 = g1;
 }
 
+// One rank-2 update of n columns: X(:, j) = r2_solve(G, R(:, j)).
+__global__ void r2_update_kernel(const double *G, const float *R, long r_js, long r_cs, long n, float *X,
+                                 long x_js, long x_cs) {
+  const long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  double x0, x1;
+  r2_solve(G, (double)R[j * r_js], (double)R[j * r_js + r_cs], x0, x1);
+  X[j * x_js] = (float)x0;
+  X[j * x_js + x_cs] = (float)x1;
+}
+
 struct Chunk {
   int node;
   long b, e;
@@ -1120,6 +1131,18 @@ int planc_hiernmf_run(planc_ctx *ctx, const planc_hier_desc *desc, const long *h
     o.rvalid = nd.rvalid;
   }
   *h_num_nodes = (int)cnt;
+  return PLANC_OK;
+}
+
+int planc_r2_update(planc_ctx *ctx, const double *d_G, const float *d_R, long r_js, long r_cs, long n,
+                    float *d_X, long x_js, long x_cs) {
+  if (!ctx) return PLANC_EINVAL;
+  if (n < 0 || !d_G || (n > 0 && (!d_R || !d_X)))
+    return set_error(ctx, PLANC_EINVAL, "planc_r2_update: bad arguments");
+  if (n == 0) return PLANC_OK;
+  PLANC_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  r2_update_kernel<<<nb(n), 256, 0, ctx->stream>>>(d_G, d_R, r_js, r_cs, n, d_X, x_js, x_cs);
+  PLANC_LAUNCH_CHECK(ctx);
   return PLANC_OK;
 }
 

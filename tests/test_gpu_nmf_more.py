@@ -205,6 +205,30 @@ def test_ntf_one_iteration_factors(ctx, algo, dt):
     assert rel_err(lam.cpu().numpy(), lam_r) < tol
 
 
+@pytest.mark.parametrize("dims", [(40, 36, 32), (36, 40, 30)])  # NN right contraction / transposed copy (L % 8 != 0)
+@pytest.mark.parametrize("dt", [P.BF16, P.F16])
+def test_ntf_16bit_copy_with_column_scales(ctx, dt, dims):
+    """The 16-bit copy planc_ntf_run makes itself carries per-column
+    power-of-two scales: a CP tensor times 2^20 (far beyond FP16's 65504) with
+    its mode-2 slices further scaled by 2^0 .. 2^6 (columns of the tree's
+    unfolding six binades apart; still a rank-k CP tensor) runs in FP16 and
+    BF16 with finite errors whose final value is within 1e-3 of the FP64
+    oracle's on the same tensor."""
+    k = 8
+    X, _ = D.cp_tensor(dims, k, 0.01, stream=41)
+    slab = 2.0 ** (20 + np.arange(dims[2]) % 7)
+    Xs = (X * slab[None, None, :]).astype(np.float32)
+    assert np.abs(Xs).max() > 65504 * 64
+    F0 = [f.astype(np.float32) for f in D.init_factors([(d, k) for d in dims], stream=41)]
+    _, _, rel_r, _ = O.ntf(P.HALS, Xs.astype(np.float64), [f.astype(np.float64) for f in F0], 10, dimtree=1)
+    dF = [P.colmajor(f) for f in F0]
+    lam, rel = ctx.ntf_run(P.colmajor(Xs), dF, P.HALS, 10, dtype=dt)
+    say(f"  dt={dt} gpu {rel} oracle {rel_r}")
+    assert np.all(np.isfinite(rel)) and np.all(np.isfinite(lam.cpu().numpy()))
+    assert abs(rel[-1] - rel_r[-1]) < 1e-3
+    assert np.abs(rel - rel_r).max() < 1e-2
+
+
 # ---- the mixed-precision configuration at full size -------------------------
 M3, N3, K3 = 65536, 16384, 64
 
@@ -295,6 +319,47 @@ def test_normalize_cols_entry(ctx):
     ref = F.astype(np.float64) / np.where(nr > 0, nr, 1.0)
     assert rel_err(out, ref) < 1e-6
     assert np.all(out[:, 3] == 0)
+
+
+def test_r2_update_entry(ctx):
+    """DistR2::update (distnmf/distr2.hpp:35-61) against the oracle's r2_update
+    (itself pinned by brute force over the four active sets): columns with a
+    non-negative solution, with either variable negative, and all-zero
+    columns."""
+    rng = np.random.default_rng(77)
+    F = rng.random((50, 2))
+    G = F.T @ F
+    n = 4000
+    R = np.empty((2, n))
+    R[:, : n // 2] = rng.random((2, n // 2))                  # mostly feasible
+    R[0, n // 2:] = rng.random(n // 2)                         # second variable pushed negative
+    R[1, n // 2:] = -rng.random(n // 2) * 0.5
+    R[:, -7:] = 0.0
+    R[:, 100:200] = R[::-1, 100:200] * np.array([[1.0], [-1.0]])  # first variable negative
+    R32 = R.astype(np.float32)
+    out = ctx.r2_update(P.colmajor(G, dtype=np.float64), P.colmajor(R32)).cpu().numpy().astype(np.float64)
+    ref = O.r2_update(G, R32.astype(np.float64))
+    assert np.all(out >= 0)
+    assert np.allclose(out, ref, rtol=1e-6, atol=1e-30)
+    assert np.all(out[:, -7:] == 0)
+    assert np.all((out[0] == 0) | (out[1] == 0) | (ref.min(axis=0) > 0))
+
+
+def test_rel_error_gram_entry(ctx):
+    """computeObjectiveError from the trace identity equals the direct
+    ||A - W H^T|| / ||A||; a non-positive fit prints 1 (common/nmf.hpp:462)."""
+    torch = pytest.importorskip("torch")
+    m, n, k = 700, 500, 12
+    A = D.lowrank_matrix(m, n, k, 0.01, stream=31).astype(np.float64)
+    W = D.uniform(D.INIT_SEED, 611, (m, k))
+    H = D.uniform(D.INIT_SEED, 612, (n, k))
+    direct = np.linalg.norm(A - W @ H.T) / np.linalg.norm(A)
+    dev = lambda v: torch.tensor(np.atleast_1d(v), dtype=torch.float64, device="cuda")
+    WtW, HtH = P.colmajor(W.T @ W, dtype=np.float64), P.colmajor(H.T @ H, dtype=np.float64)
+    rel = ctx.rel_error_gram(dev(np.sum(A * A)), dev(np.sum(W * (A @ H))), WtW, HtH)
+    assert abs(rel - direct) <= 1e-10 * direct
+    tr = float(np.sum((W.T @ W) * (H.T @ H)))
+    assert ctx.rel_error_gram(dev(1.0), dev(0.5 * (1.0 + tr) + 1.0), WtW, HtH) == 1.0  # fit^2 = -2
 
 
 def test_hadamard_grams_entry(ctx):
