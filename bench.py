@@ -234,7 +234,7 @@ def cpu_baseline_leg(O, A, Wi, Hi, iters):
             "fp32_variant": f32, "host": info}
 
 
-def hier_leg(P, D, ctx, m=100000, n=500000, density=1e-4, depth=11, iters=100):
+def hier_leg(P, D, ctx, m=100000, n=500000, density=1e-4, depth=11, iters=100, frontier=1024):
     """Batched hierarchical NMF on a power-law term-document CSR (host CSR in,
     one level-batched launch sequence per tree level)."""
     t0 = time.perf_counter()
@@ -244,13 +244,21 @@ def hier_leg(P, D, ctx, m=100000, n=500000, density=1e-4, depth=11, iters=100):
     nodes, cols, _, _ = ctx.hiernmf_run(m, n, indptr, indices, data, depth, iters, 193957)
     el = time.perf_counter() - t0
     sizes = [x["ncols"] for x in nodes]
+    # The reference's best-first frontier (--nodes budget) on the same matrix:
+    # 1024 accepted nodes report 2047, as many as the complete tree of depth 11.
+    t0 = time.perf_counter()
+    fnodes, _, _, _ = ctx.hiernmf_run(m, n, indptr, indices, data, 62, iters, 193957, nodes=frontier)
+    fel = time.perf_counter() - t0
     return {"seconds": round(el, 4), "nodes": len(nodes), "nodes_per_s": round(len(nodes) / el, 1),
             "nnz": int(indptr[-1]), "iters_per_node": iters,
             "node_cols_min": int(min(sizes)), "node_cols_max": int(max(sizes)),
             "generate_s": round(gen_s, 2),
+            "frontier": {"budget": frontier, "seconds": round(fel, 4), "nodes": len(fnodes),
+                         "max_level": int(max(x["level"] for x in fnodes)),
+                         "nodes_per_s": round(len(fnodes) / fel, 1)},
             "config": f"{m}x{n} CSR, density {density}, log-normal values, power-law rows "
                       f"(alpha 1.1), depth {depth}, {iters} iterations per node (host CSR in, "
-                      f"tree out)"}
+                      f"tree out); frontier: budget {frontier}, level cap 62 (a max_level below 61 means the cap never bound)"}
 
 
 def profile_ntf4(P, D, ctx, torch):
@@ -287,10 +295,10 @@ def main():
     ap.add_argument("--legs", default="hals,bpp,ntf4,ntf3,hier",
                     help="comma-separated legs after the headline: hals, bpp, ntf4 (NTF-HALS "
                          "128^4), ntf3 (NTF-HALS 256^3), hier (hierarchical NMF)")
-    ap.add_argument("--profile-kernel", nargs="?", const="wta", default=None, choices=["wta", "ntf4"],
+    ap.add_argument("--profile-kernel", nargs="?", const="wta", default=None, choices=["wta", "ntf4", "hier"],
                     help="only launch one kernel family (and a calibration copy) for rocprof: "
                          "wta = BF16 W^T A of the headline; ntf4 = the 128^4 k=48 partial "
-                         "MTTKRPs (right and left, FP32 and FP16)")
+                         "MTTKRPs (right and left, FP32 and FP16); hier = the hierarchical NMF leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this build "
@@ -306,6 +314,10 @@ def main():
     ctx = P.Context(0)
     if args.profile_kernel == "ntf4":
         profile_ntf4(P, D, ctx, torch)
+        ctx.close()
+        return
+    if args.profile_kernel == "hier":  # the hierarchical NMF leg alone (kernel traces)
+        print(json.dumps({"profile_kernel": "hiernmf 100000x500000", "leg": hier_leg(P, D, ctx)}))
         ctx.close()
         return
     legs_on = set(x.strip() for x in args.legs.split(",") if x.strip())

@@ -281,26 +281,40 @@ int planc_csr_col_subset(planc_ctx *ctx, long m, long n, const long *d_indptr, c
                          const float *d_data, const long *h_cols, long ncols, long *d_out_indptr,
                          int *d_out_indices, float *d_out_data, long *h_out_nnz);
 
-/* Hierarchical rank-2 NMF over a complete tree of `depth` levels
- * (HierNMFDriver::buildTree, hiernmf/hiernmf.cpp:65-184, with a level-by-level
- * schedule instead of the score-ordered frontier): every node of a level runs
- * `iters` iterations of the rank-2 NMF of Node::split (hiernmf/node.hpp:
- * 104-188; DistAUNMF order H then W, DistR2 update) in one batched, hipGraph-
- * captured launch sequence; its columns split by H(:,0) >= H(:,1), the larger
- * side becoming child 2i+1; every child gets sigma by an `iters`-step power
- * iteration (Node::compute_sigma).  Node seeds: initseed * index (rank 0);
- * W from stream 0, H from stream 1, the power-iteration start from normals
- * of stream 2 (Philox).  A is a host CSR (m x n, FP32 values, sorted column
- * indices).  Outputs (host): h_nodes (capacity 2^depth - 1) in level order,
- * h_cols (capacity n * depth) each node's sorted columns at its col_offset,
- * optional h_H (2 doubles per entry of h_cols: the node's final H rows) and
- * h_W (2*m doubles per node, indexed by node index; zero rows where the
- * node's matrix is empty). */
+/* Hierarchical rank-2 NMF (HierNMFDriver::buildTree, hiernmf/hiernmf.cpp:
+ * 65-184).  Splitting a node runs `iters` iterations of the rank-2 NMF of
+ * Node::split (hiernmf/node.hpp:104-188; DistAUNMF order H then W, DistR2
+ * update); its columns split by H(:,0) >= H(:,1), the larger side becoming
+ * child 2i+1; every child gets sigma by an `iters`-step power iteration
+ * (Node::compute_sigma) and its parent the score of Node::compute_score.
+ * Node seeds: initseed * index (rank 0); W from stream 0, H from stream 1,
+ * the power-iteration start from normals of stream 2 (Philox).  Nodes are
+ * processed in batches (all nodes of a batch as one block-diagonal matrix,
+ * one hipGraph-captured iteration loop).
+ *   desc->nodes <= 0: the complete tree of `depth` levels, level by level;
+ *     h_nodes in level order (capacity 2^depth - 1).
+ *   desc->nodes > 0: the reference's best-first frontier with that node
+ *     budget (hiernmf/hiernmf.cpp:138-147): the root is split and accepted,
+ *     then nodes - 2 times the frontier node of highest score (ties: smaller
+ *     index) is accepted; h_nodes holds the reference's output queue (the
+ *     root, its children, then the children of each accepted node in order;
+ *     capacity max(3, 2 nodes - 1)); `depth` (2 .. 62) caps the levels (a
+ *     node on level depth - 1 is never split); the complete tree takes
+ *     depth 1 .. 24.
+ * A is a host CSR (m x n, FP32 values, sorted column indices).  Outputs
+ * (host): h_cols (capacity n * depth) each node's sorted columns at its
+ * col_offset, optional h_H (2 doubles per entry of h_cols: the node's own
+ * final H rows when it was split) and h_W (2*m doubles per reported node, in
+ * h_nodes order: the node's own W; zero rows where the node's matrix is
+ * empty).  The reference writes for every node but the root its parent's
+ * factor column on its side (Node's W and H): column parent_side of the
+ * parent's W and H. */
 typedef struct planc_hier_desc {
   long m, n;
-  int depth;     /* levels, root = level 0 */
+  int depth;     /* levels, root = level 0 (the cap in frontier mode) */
   int iters;     /* NMF and power iterations per node (-t) */
   int initseed;  /* --seed */
+  int nodes;     /* --nodes: > 0 best-first frontier with this budget; <= 0 complete tree */
 } planc_hier_desc;
 
 typedef struct planc_hier_node {
@@ -311,6 +325,8 @@ typedef struct planc_hier_node {
   double sigma;    /* 0 for the root (RootNode) */
   double score;    /* Node::compute_score, hiernmf/node.hpp:71-81 */
   int lvalid, rvalid;
+  int split;       /* 1: the node's rank-2 NMF ran (its W / H are in h_W / h_H) */
+  int parent_side; /* column of the parent's W / H this node carries (-1: root) */
 } planc_hier_node;
 
 int planc_hiernmf_run(planc_ctx *ctx, const planc_hier_desc *desc, const long *h_indptr,

@@ -389,12 +389,15 @@ class Context:
         return out_ptr, oi[:cnt.value], ov[:cnt.value]
 
     def hiernmf_run(self, m, n, indptr, indices, data, depth, iters, initseed, want_H=False,
-                    want_W=False):
-        """Hierarchical rank-2 NMF over a complete tree (host CSR input).
-        Returns (nodes: list of dicts, cols (int64 array), H or None, W or None)."""
-        d = HierDesc(m, n, depth, iters, initseed)
-        cap = (1 << depth) - 1
-        nodes = (HierNode * cap)()
+                    want_W=False, nodes=0):
+        """Hierarchical rank-2 NMF (host CSR input): the complete tree of
+        `depth` levels (nodes <= 0) or the best-first frontier with a budget
+        of `nodes` (depth capping the levels).  Returns (nodes: list of
+        dicts in the reported order, cols (int64 array), H or None, W or
+        None; W holds 2*m doubles per reported node, in that order)."""
+        d = HierDesc(m, n, depth, iters, initseed, nodes)
+        cap = max(3, 2 * nodes - 1) if nodes > 0 else (1 << depth) - 1
+        table = (HierNode * cap)()
         nn = ctypes.c_int()
         ip = np.ascontiguousarray(indptr, dtype=np.int64)
         ix = np.ascontiguousarray(indices, dtype=np.int32)
@@ -406,16 +409,17 @@ class Context:
         self._check(self.lib.planc_hiernmf_run(
             self.ctx, ctypes.byref(d), ip.ctypes.data_as(ctypes.POINTER(ctypes.c_long)),
             ix.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
-            dv.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), nodes, ctypes.byref(nn),
+            dv.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), table, ctypes.byref(nn),
             cols.ctypes.data_as(ctypes.POINTER(ctypes.c_long)),
             H.ctypes.data_as(dp) if H is not None else None,
             W.ctypes.data_as(dp) if W is not None else None), "planc_hiernmf_run")
         out = []
         for i in range(nn.value):
-            x = nodes[i]
+            x = table[i]
             out.append({"index": x.index, "level": x.level, "ncols": x.ncols,
                         "col_offset": x.col_offset, "sigma": x.sigma, "score": x.score,
-                        "lvalid": x.lvalid, "rvalid": x.rvalid})
+                        "lvalid": x.lvalid, "rvalid": x.rvalid, "split": x.split,
+                        "parent_side": x.parent_side, "slot": i})
         return out, cols, H, W
 
     # ---- whole loops -------------------------------------------------------

@@ -34,12 +34,13 @@ class NMFDesc(ctypes.Structure):
 
 class HierDesc(ctypes.Structure):
     _fields_ = [("m", c_long), ("n", c_long), ("depth", c_int), ("iters", c_int),
-                ("initseed", c_int)]
+                ("initseed", c_int), ("nodes", c_int)]
 
 
 class HierNode(ctypes.Structure):
     _fields_ = [("index", c_long), ("level", c_int), ("ncols", c_long), ("col_offset", c_long),
-                ("sigma", c_double), ("score", c_double), ("lvalid", c_int), ("rvalid", c_int)]
+                ("sigma", c_double), ("score", c_double), ("lvalid", c_int), ("rvalid", c_int),
+                ("split", c_int), ("parent_side", c_int)]
 
 
 class NTFDesc(ctypes.Structure):

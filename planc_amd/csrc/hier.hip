@@ -21,6 +21,8 @@
 // launched as one graph.
 #include <algorithm>
 #include <cstring>
+#include <map>
+#include <queue>
 #include <vector>
 
 #include "common.h"
@@ -743,7 +745,10 @@ int planc_hiernmf_run(planc_ctx *ctx, const planc_hier_desc *desc, const long *h
     return PLANC_EINVAL;
   const long m = desc->m, n = desc->n;
   const int depth = desc->depth, iters = desc->iters;
-  if (m <= 0 || n <= 0 || depth < 1 || depth > 24 || iters < 0)
+  // Node indices 2i+1 / 2i+2 stay below 2^63 up to level 61; the complete
+  // tree's table has 2^depth - 1 entries.
+  if (m <= 0 || n <= 0 || depth < 1 || depth > (desc->nodes > 0 ? 62 : 24) || iters < 0 ||
+      (desc->nodes > 0 && depth < 2))
     return set_error(ctx, PLANC_EINVAL, "planc_hiernmf_run: bad arguments");
   const long nnz = h_indptr[m];
   if (nnz < 0 || nnz >= (1L << 31)) return set_error(ctx, PLANC_EINVAL, "planc_hiernmf_run: nnz");
@@ -817,18 +822,23 @@ int planc_hiernmf_run(planc_ctx *ctx, const planc_hier_desc *desc, const long *h
     double sigma = 0, score = 0;
     int lvalid = 0, rvalid = 0;
     long lchild = -1, rchild = -1;
-    long out_slot = -1;
+    long out_slot = -1;  // column offset in h_cols / h_H (-1: not recorded yet)
+    bool split = false;  // rank-2 NMF done, children created
+    int lidx = 0;        // factor column of the left child (1 when the sides were swapped)
+    long parent = -1;
   };
   std::vector<HNode> nodes;
-  nodes.push_back({0, 0, {}, 0, 0, 0, 0, -1, -1, -1});
+  nodes.push_back(HNode{0, 0, {}});
   nodes[0].cols.resize(n);
   for (long j = 0; j < n; ++j) nodes[0].cols[j] = j;
-  std::vector<long> level_nodes = {0};
   long cols_used = 0;
-  const long cap_nodes = (1L << depth) - 1;
+  const long cols_cap = n * (long)depth;
   const unsigned long long seed0 = (unsigned long long)(long long)desc->initseed;
+  std::map<long, std::vector<double>> wstore;  // node id -> its rank-2 W (m x 2), when h_W is asked for
 
-  for (int lev = 0; lev < depth && !level_nodes.empty(); ++lev) {
+  auto run_batch = [&](const std::vector<long> &level_nodes, bool sigma_on, bool split_on,
+                       std::vector<long> *children) -> int {
+    if (level_nodes.empty()) return PLANC_OK;
     const int nn = (int)level_nodes.size();
     // Level columns: concatenation of the nodes' (sorted) column lists.
     std::vector<long> perm, noff(nn + 1, 0);
@@ -973,7 +983,7 @@ int planc_hiernmf_run(planc_ctx *ctx, const planc_hier_desc *desc, const long *h
     // Sigma of this level's nodes (children created by the previous level):
     // 100-step power iteration of Node::compute_sigma (hiernmf/node.hpp:66-69).
     std::vector<double> sig(nn, 0.0);
-    if (lev > 0) {
+    if (sigma_on) {
       pow_init<<<nb(nlev), 256, 0, ctx->stream>>>(nlev, E.node_of_col.p, E.node_col0.p, E.seeds.p, E.q.p);
       sumsq_chunks<<<(unsigned)chc.size(), 256, 0, ctx->stream>>>(E.ch_cols.p, E.q.p, E.part.p);
       norm_finalize<<<nb(nn), 256, 0, ctx->stream>>>(E.part.p, E.cb_cols.p, nn, E.sig.p);
@@ -997,14 +1007,17 @@ int planc_hiernmf_run(planc_ctx *ctx, const planc_hier_desc *desc, const long *h
       PLANC_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
       for (int t = 0; t < nn; ++t) nodes[level_nodes[t]].sigma = iters > 0 ? sig[t] : 0.0;
     }
-    // Record the level's nodes.
+    // Record the batch's nodes (columns once per node).
     for (int t = 0; t < nn; ++t) {
       HNode &nd = nodes[level_nodes[t]];
+      if (nd.out_slot >= 0) continue;
+      if (cols_used + (long)nd.cols.size() > cols_cap)
+        return set_error(ctx, PLANC_EINVAL, "planc_hiernmf_run: column output capacity (n * depth) exceeded");
       nd.out_slot = cols_used;
       std::memcpy(h_cols + cols_used, nd.cols.data(), sizeof(long) * nd.cols.size());
       cols_used += (long)nd.cols.size();
     }
-    if (lev == depth - 1) break;  // leaves are not split
+    if (!split_on) return PLANC_OK;  // sigma only (leaves of this pass)
 
     // Rank-2 NMF of every node of the level: the iteration loop is captured
     // into one hipGraph and launched once.
@@ -1065,7 +1078,6 @@ int planc_hiernmf_run(planc_ctx *ctx, const planc_hier_desc *desc, const long *h
       }
     }
     PLANC_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    std::vector<long> next;
     for (int t = 0; t < nn; ++t) {
       long id = level_nodes[t];
       long slot = nodes[id].out_slot;  // H / W rows of this node
@@ -1075,9 +1087,8 @@ int planc_hiernmf_run(planc_ctx *ctx, const planc_hier_desc *desc, const long *h
           h_H[2 * (slot + (lc - noff[t])) + 1] = Hh[2 * lc + 1];
         }
       if (h_W) {
-        long widx = nodes[id].index;
-        double *Wf = h_W + (size_t)widx * 2 * m;
-        std::fill(Wf, Wf + 2 * m, 0.0);
+        std::vector<double> &Wf = wstore[id];
+        Wf.assign(2 * (size_t)m, 0.0);
         for (long r = roff[t]; r < roff[t + 1]; ++r) {
           Wf[rorig[r]] = Wc[2 * r];
           Wf[rorig[r] + m] = Wc[2 * r + 1];
@@ -1085,42 +1096,128 @@ int planc_hiernmf_run(planc_ctx *ctx, const planc_hier_desc *desc, const long *h
       }
       std::vector<long> L, R;
       for (long lc = noff[t]; lc < noff[t + 1]; ++lc) (mk[lc] ? L : R).push_back(perm[lc]);
-      if (R.size() > L.size()) std::swap(L, R);
+      const bool swapped = R.size() > L.size();
+      if (swapped) std::swap(L, R);
       HNode &nd = nodes[id];
+      nd.split = true;
+      nd.lidx = swapped ? 1 : 0;
       nd.lvalid = !L.empty();
       nd.rvalid = !R.empty();
       long idx = nd.index;
       int lv = nd.level;
       if (nd.lvalid) {
-        nodes.push_back({2 * idx + 1, lv + 1, L, 0, 0, 0, 0, -1, -1, -1});
+        nodes.push_back(HNode{2 * idx + 1, lv + 1, L});
+        nodes.back().parent = id;
         nodes[id].lchild = (long)nodes.size() - 1;
-        next.push_back((long)nodes.size() - 1);
+        if (children) children->push_back((long)nodes.size() - 1);
       }
       if (nodes[id].rvalid) {
-        nodes.push_back({2 * idx + 2, lv + 1, R, 0, 0, 0, 0, -1, -1, -1});
+        nodes.push_back(HNode{2 * idx + 2, lv + 1, R});
+        nodes.back().parent = id;
         nodes[id].rchild = (long)nodes.size() - 1;
-        next.push_back((long)nodes.size() - 1);
+        if (children) children->push_back((long)nodes.size() - 1);
       }
     }
-    level_nodes.swap(next);
+    return PLANC_OK;
+  };
+
+  // Node::compute_score (hiernmf/node.hpp:71-81) from the children's sigmas.
+  auto score_of = [&](const HNode &nd) {
+    const double ls = nd.lchild >= 0 ? nodes[nd.lchild].sigma : 0.0;
+    const double rs = nd.rchild >= 0 ? nodes[nd.rchild].sigma : 0.0;
+    if (nd.lvalid && nd.rvalid) return (ls + rs) - nd.sigma;
+    if (nd.lvalid) return ls - nd.sigma;
+    if (nd.rvalid) return rs - nd.sigma;
+    return 0.0;
+  };
+  std::vector<long> out_order;  // the reported nodes, in the reference's queue order
+  if (desc->nodes <= 0) {
+    // Complete tree: every node of levels 0 .. depth-2 is split, level by
+    // level; the last level only gets its sigma.
+    std::vector<long> level_nodes = {0};
+    for (int lev = 0; lev < depth && !level_nodes.empty(); ++lev) {
+      std::vector<long> next;
+      RC(run_batch(level_nodes, lev > 0, lev < depth - 1, &next));
+      out_order.insert(out_order.end(), level_nodes.begin(), level_nodes.end());
+      level_nodes.swap(next);
+    }
+  } else {
+    // Best-first frontier with a node budget (hiernmf/hiernmf.cpp:138-147):
+    // the root is split and accepted (its children split), then nodes - 2
+    // times the frontier node of highest score is accepted (its children
+    // split).  Accepting needs the children's rank-2 NMFs and the sigmas of
+    // their children (for the children's scores), so the device works in
+    // rounds: the host replays the reference's queue with what is known,
+    // collects every split the replay is missing, and computes them (and
+    // the sigmas of the children they create) as one batch each; the replay
+    // that misses nothing is the result.  Ties in score go to the smaller
+    // node index.  A node at level depth - 1 is not split (treated as having
+    // no children).
+    std::vector<long> kids;
+    RC(run_batch({0}, false, true, &kids));
+    std::vector<long> grand;
+    RC(run_batch(kids, true, depth > 2, &grand));
+    RC(run_batch(grand, true, false, nullptr));
+    for (;;) {
+      std::vector<long> need;
+      auto want_split = [&](long id) {
+        HNode &nd = nodes[id];
+        if (nd.split || nd.level >= depth - 1) return true;
+        if (std::find(need.begin(), need.end(), id) == need.end()) need.push_back(id);
+        return false;
+      };
+      auto cmp = [&](long a, long b) {
+        const double sa = nodes[a].score, sb = nodes[b].score;
+        if (sa != sb) return sa < sb;
+        return nodes[a].index > nodes[b].index;
+      };
+      std::priority_queue<long, std::vector<long>, decltype(cmp)> frontier(cmp);
+      std::vector<long> order = {0};
+      auto accept = [&](long id) {  // enqueue_children after accept(); false: a split is missing
+        const HNode &nd = nodes[id];
+        bool ok = true;
+        for (long c : {nd.lchild, nd.rchild})
+          if (c >= 0) ok = want_split(c) && ok;
+        if (!ok) return false;
+        for (long c : {nd.lchild, nd.rchild})
+          if (c >= 0) {
+            nodes[c].score = score_of(nodes[c]);
+            frontier.push(c);
+            order.push_back(c);
+          }
+        return true;
+      };
+      accept(0);
+      for (long i = 2; i < desc->nodes && !frontier.empty(); ++i) {
+        const long x = frontier.top();
+        frontier.pop();
+        accept(x);
+      }
+      if (need.empty()) {
+        out_order = order;
+        break;
+      }
+      std::vector<long> created;
+      RC(run_batch(need, false, true, &created));
+      RC(run_batch(created, true, false, nullptr));
+    }
   }
 #undef RC
-  // Scores (Node::compute_score, hiernmf/node.hpp:71-81) and the node table.
+  // The node table, in out_order.
+  if (h_W)
+    for (size_t t = 0; t < out_order.size(); ++t) {
+      const auto it = wstore.find(out_order[t]);
+      double *dst = h_W + t * 2 * (size_t)m;
+      if (it != wstore.end())
+        std::memcpy(dst, it->second.data(), sizeof(double) * 2 * (size_t)m);
+      else
+        std::fill(dst, dst + 2 * m, 0.0);
+    }
   long cnt = 0;
-  for (auto &nd : nodes) {
-    if (nd.out_slot < 0) continue;
-    if (cnt >= cap_nodes) break;
-    double ls = nd.lchild >= 0 ? nodes[nd.lchild].sigma : 0.0;
-    double rs = nd.rchild >= 0 ? nodes[nd.rchild].sigma : 0.0;
-    if (nd.lvalid && nd.rvalid)
-      nd.score = (ls + rs) - nd.sigma;
-    else if (nd.lvalid)
-      nd.score = ls - nd.sigma;
-    else if (nd.rvalid)
-      nd.score = rs - nd.sigma;
-    else
-      nd.score = 0;
-    planc_hier_node &o = h_nodes[cnt++];
+  for (long id : out_order) {
+    HNode &nd = nodes[id];
+    nd.score = score_of(nd);
+    planc_hier_node &o = h_nodes[cnt];
     o.index = nd.index;
     o.level = nd.level;
     o.ncols = (long)nd.cols.size();
@@ -1129,6 +1226,13 @@ int planc_hiernmf_run(planc_ctx *ctx, const planc_hier_desc *desc, const long *h
     o.score = nd.score;
     o.lvalid = nd.lvalid;
     o.rvalid = nd.rvalid;
+    o.split = nd.split ? 1 : 0;
+    o.parent_side = -1;
+    if (nd.parent >= 0) {
+      const HNode &pa = nodes[nd.parent];
+      o.parent_side = nd.index == 2 * pa.index + 1 ? pa.lidx : 1 - pa.lidx;
+    }
+    ++cnt;
   }
   *h_num_nodes = (int)cnt;
   return PLANC_OK;

@@ -14,12 +14,17 @@
 
 namespace planc {
 
+// The input stage shared by the drivers: the options, and A generated or
+// loaded as nmf/nmf.cpp:73-168 does.
 template <class T>
-class NMFDriver {
+class DriverInput {
+ protected:
   ParseCommandLine pc;
   static constexpr uint64_t kSeed = 1210873;  // kW_seed_idx of nmf/nmf.cpp:70
   bool m_symm_flag = false;
   double m_symm_reg = -1;
+
+  DriverInput(int argc, char *argv[]) : pc(argc, argv) {}
 
   // ---- dense inputs (nmf/nmf.cpp:131-168) --------------------------------
   void generate(MAT &A, const std::string &type, unsigned long long m, unsigned long long n,
@@ -153,9 +158,9 @@ class NMFDriver {
     return mx;
   }
 
-  template <class NMFTYPE>
-  void callNMF() {
-    unsigned long long m = pc.globalm(), n = pc.globaln(), k = pc.lowrankk();
+  // A from -i: "rand_uniform" / "rand_normal" / "rand_lowrank" generated at
+  // -d m n with rank k, otherwise the file loaded (m, n set from it).
+  T read_input(unsigned long long &m, unsigned long long &n, unsigned long long k) {
     T A;
     std::string name = pc.input_file_name();
     const std::string pre = "rand_";
@@ -169,18 +174,32 @@ class NMFDriver {
       m = A.n_rows;
       n = A.n_cols;
     }
-    normalise(A);
+    return A;
+  }
+};
+
+template <class T>
+class NMFDriver : DriverInput<T> {
+  using DriverInput<T>::pc;
+  using DriverInput<T>::m_symm_flag;
+  using DriverInput<T>::m_symm_reg;
+
+  template <class NMFTYPE>
+  void callNMF() {
+    unsigned long long m = pc.globalm(), n = pc.globaln(), k = pc.lowrankk();
+    T A = this->read_input(m, n, k);
+    this->normalise(A);
     MAT W(m, k), H(n, k);
     W.randu((uint64_t)pc.initseed(), 0);
     H.randu((uint64_t)pc.initseed(), 1);
     if (m_symm_flag) {
       // nmf/nmf.cpp:191-199: H scaled by 2 sqrt(mean(A)/k), W = H; a zero
       // --symm value becomes max(A)^2.
-      const double s = 2 * std::sqrt(mean(A) / (double)k);
+      const double s = 2 * std::sqrt(this->mean(A) / (double)k);
       for (unsigned long long e = 0; e < H.n_elem(); ++e) H.memptr()[e] *= s;
       W = H;
       if (m_symm_reg == 0.0) {
-        const double mx = maxval(A);
+        const double mx = this->maxval(A);
         m_symm_reg = mx * mx;
       }
     }
@@ -202,7 +221,7 @@ class NMFDriver {
   }
 
  public:
-  NMFDriver(int argc, char *argv[]) : pc(argc, argv) {
+  NMFDriver(int argc, char *argv[]) : DriverInput<T>(argc, argv) {
     pc.parseplancopts();
     m_symm_reg = pc.symm_reg();
     // Symmetric NMF (nmf/nmf.cpp:256-268): a square A and ANLS-BPP or GNSYM.

@@ -651,8 +651,96 @@ def test_hiernmf_tree_matches_oracle(ctx):
         if "H" in r:
             h = H[2 * o["col_offset"]:2 * (o["col_offset"] + o["ncols"])].reshape(-1, 2)
             assert rel_err(h, r["H"]) < 1e-9
-            w = W[2 * m * o["index"]:2 * m * (o["index"] + 1)].reshape(2, m).T
+            w = W[2 * m * o["slot"]:2 * m * (o["slot"] + 1)].reshape(2, m).T
             assert rel_err(w, r["W"]) < 1e-9
+
+
+def _hier_frontier_reference(m, n, ip, ix, dv, budget, iters, seed):
+    """HierNMFDriver::buildTree's best-first loop (hiernmf/hiernmf.cpp:
+    138-147) node by node with the oracle's rank-2 NMF and power iteration:
+    returns the output queue (root, its children, then the children of each
+    accepted node), ties in score broken by the smaller index."""
+    import heapq
+    info = {}
+
+    def make(idx, cols, level):
+        rec = {"index": idx, "level": level, "cols": cols, "sigma": 0.0, "kids": None}
+        if idx > 0:
+            sd = (seed * idx) % (1 << 64)
+            rec["sigma"] = O.powiter_csr(m, n, ip, ix, dv, cols, D.normal(sd, 2, (len(cols),)), iters)
+        info[idx] = rec
+        return rec
+
+    def split(rec):
+        if rec["kids"] is not None:
+            return
+        sd = (seed * rec["index"]) % (1 << 64)
+        W0 = D.uniform(sd, 0, (m, 2))
+        H0 = D.uniform(sd, 1, (len(rec["cols"]), 2))
+        W, H = O.r2_nmf_csr(m, n, ip, ix, dv, rec["cols"], W0, H0, iters)
+        rec["W"], rec["H"] = W, H
+        left = H[:, 0] >= H[:, 1]
+        L, R = rec["cols"][left], rec["cols"][~left]
+        if len(R) > len(L):
+            L, R = R, L
+        rec["kids"] = [make(2 * rec["index"] + 1 + j, c, rec["level"] + 1) for j, c in enumerate((L, R)) if len(c)]
+
+    def score(rec):
+        s = [k["sigma"] for k in rec["kids"] or []]
+        return sum(s) - rec["sigma"] if s else 0.0
+
+    root = make(0, np.arange(n), 0)
+    split(root)
+    order, heap = [root], []
+
+    def accept(rec):
+        for k in rec["kids"]:
+            split(k)
+        for k in rec["kids"]:
+            k["score"] = score(k)
+            heapq.heappush(heap, (-k["score"], k["index"]))
+            order.append(k)
+
+    accept(root)
+    for _ in range(2, budget):
+        if not heap:
+            break
+        accept(info[heapq.heappop(heap)[1]])
+    root["score"] = score(root)
+    return order
+
+
+@pytest.mark.parametrize("budget", [2, 5, 9])
+def test_hiernmf_frontier_matches_oracle(ctx, budget):
+    """The reference's best-first frontier with a --nodes budget: the device's
+    rounds (replay, compute the missing splits as a batch, replay again) give
+    the reference's output queue -- node order, columns, sigma, score and each
+    node's own rank-2 W and H -- against the oracle's node-by-node loop."""
+    say(f"hiernmf frontier budget={budget}")
+    m, n, iters, seed = 300, 400, 15, 193957
+    ip, ix, dv = D.sparse_powerlaw(m, n, 0.03, 1.1, stream=9)
+    ref = _hier_frontier_reference(m, n, ip, ix, dv.astype(np.float64), budget, iters, seed)
+    out, cols, H, W = ctx.hiernmf_run(m, n, ip, ix, dv, 12, iters, seed, want_H=True, want_W=True,
+                                      nodes=budget)
+    assert [o["index"] for o in out] == [r["index"] for r in ref]
+    for o, r in zip(out, ref):
+        assert o["level"] == r["level"]
+        c = cols[o["col_offset"]:o["col_offset"] + o["ncols"]]
+        assert np.array_equal(c, r["cols"])
+        assert abs(o["sigma"] - r["sigma"]) <= 1e-9 * max(1.0, r["sigma"])
+        assert abs(o["score"] - r["score"]) <= 1e-8 * max(1.0, abs(r["score"]))
+        assert o["split"] == 1
+        h = H[2 * o["col_offset"]:2 * (o["col_offset"] + o["ncols"])].reshape(-1, 2)
+        assert rel_err(h, r["H"]) < 1e-9
+        w = W[2 * m * o["slot"]:2 * m * (o["slot"] + 1)].reshape(2, m).T
+        assert rel_err(w, r["W"]) < 1e-9
+        if o["index"] > 0:  # the parent's factor column this node carries
+            side = o["parent_side"]
+            parent = next(p for p in ref if p["index"] == (o["index"] - 1) // 2)
+            left = o["index"] % 2 == 1
+            sizes = (parent["H"][:, 0] >= parent["H"][:, 1]).sum(), (parent["H"][:, 0] < parent["H"][:, 1]).sum()
+            swapped = sizes[1] > sizes[0]
+            assert side == ((1 if swapped else 0) if left else (0 if swapped else 1))
 
 
 def test_hiernmf_long_rows_match_oracle(ctx):
@@ -681,17 +769,19 @@ def test_hiernmf_long_rows_match_oracle(ctx):
         if "H" in r:
             h = H[2 * o["col_offset"]:2 * (o["col_offset"] + o["ncols"])].reshape(-1, 2)
             assert rel_err(h, r["H"]) < 1e-9
-            w = W[2 * m * o["index"]:2 * m * (o["index"] + 1)].reshape(2, m).T
+            w = W[2 * m * o["slot"]:2 * m * (o["slot"] + 1)].reshape(2, m).T
             assert rel_err(w, r["W"]) < 1e-9
 
 
-def test_hiernmf_level_invariants_medium(ctx):
-    """20000 x 60000 power-law CSR, depth 7: every level partitions the
-    columns of its parents, children are indexed 2i+1 / 2i+2 with the larger
-    side left, and scores follow Node::compute_score."""
-    say("hiernmf medium tree")
-    m, n, depth = 20000, 60000, 7
-    ip, ix, dv = D.sparse_powerlaw(m, n, 2e-4, 1.1, stream=10)
+@pytest.mark.parametrize("m,n,dens,depth,stream", [(20000, 60000, 2e-4, 7, 10),
+                                                    (100000, 500000, 1e-4, 11, 7)])
+def test_hiernmf_level_invariants(ctx, m, n, dens, depth, stream):
+    """Power-law CSR trees (the second is the 100000 x 500000 term-document
+    matrix at density 1e-4, depth 11): every level partitions the columns of
+    its parents, children are indexed 2i+1 / 2i+2 with the larger side left,
+    and scores follow Node::compute_score."""
+    say(f"hiernmf tree {m}x{n} depth {depth}")
+    ip, ix, dv = D.sparse_powerlaw(m, n, dens, 1.1, stream=stream)
     out, cols, _, _ = ctx.hiernmf_run(m, n, ip, ix, dv, depth, 30, 7)
     by_index = {o["index"]: o for o in out}
     for o in out:
@@ -706,6 +796,51 @@ def test_hiernmf_level_invariants_medium(ctx):
                 assert kids[0]["ncols"] >= kids[1]["ncols"]
                 assert abs(o["score"] - (kids[0]["sigma"] + kids[1]["sigma"] - o["sigma"])) < 1e-6 * (1 + abs(o["score"]))
     assert max(o["level"] for o in out) == depth - 1
+
+
+def test_hiernmf_frontier_invariants_full_size(ctx):
+    """The best-first frontier on the 100000 x 500000 power-law term-document
+    CSR (density 1e-4) with a budget of 1024 nodes and the level cap out of
+    reach: replaying the reference's priority queue (hiernmf/hiernmf.cpp:
+    138-147) on the reported scores gives the reported order; children
+    partition their parent's columns, larger side left; scores follow
+    Node::compute_score."""
+    say("hiernmf frontier 100000x500000 budget 1024")
+    import heapq
+    m, n, budget = 100000, 500000, 1024
+    ip, ix, dv = D.sparse_powerlaw(m, n, 1e-4, 1.1, 7)
+    out, cols, _, _ = ctx.hiernmf_run(m, n, ip, ix, dv, 62, 30, 193957, nodes=budget)
+    assert len(out) == 2 * budget - 1
+    assert max(o["level"] for o in out) < 61
+    by_index = {o["index"]: o for o in out}
+    assert len(by_index) == len(out) and out[0]["index"] == 0
+
+    def kids_of(o):
+        return [by_index[c] for c in (2 * o["index"] + 1, 2 * o["index"] + 2) if c in by_index]
+
+    heap, order = [], [0]
+
+    def accept(o):
+        for k in kids_of(o):
+            heapq.heappush(heap, (-k["score"], k["index"]))
+            order.append(k["index"])
+
+    accept(out[0])
+    for _ in range(2, budget):
+        accept(by_index[heapq.heappop(heap)[1]])
+    assert order == [o["index"] for o in out]
+    for o in out:
+        assert o["split"] == 1
+        c = cols[o["col_offset"]:o["col_offset"] + o["ncols"]]
+        assert np.all(np.diff(c) > 0)
+        kids = kids_of(o)
+        if kids:
+            u = np.concatenate([cols[k["col_offset"]:k["col_offset"] + k["ncols"]] for k in kids])
+            assert np.array_equal(np.sort(u), c)
+            if len(kids) == 2:
+                assert kids[0]["ncols"] >= kids[1]["ncols"]
+        if o["lvalid"] and o["rvalid"] and len(kids) == 2:
+            assert abs(o["score"] - (kids[0]["sigma"] + kids[1]["sigma"] - o["sigma"])) < 1e-6 * (1 + abs(o["score"]))
 
 
 def test_read_bandwidth_probe(ctx):

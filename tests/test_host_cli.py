@@ -299,3 +299,48 @@ def test_auntf_host_update_plugin_matches_oracle(tmp_path, dimtree):
     assert np.abs(got["host_mu"] - rel_mu).max() < 1e-5
     assert np.abs(got["hals_host_loop"] - rel_h).max() < 1e-5
     assert np.abs(got["hals_device_loop"] - rel_h).max() < 1e-5
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(BIN, "hiernmf")), reason="host drivers not built")
+def test_hiernmf_flags_dry_run():
+    p = run([os.path.join(BIN, "hiernmf"), "-i", "rand_uniform", "-d", "500 400", "-s", "0.01",
+             "-t", "30", "--nodes", "64", "--seed", "11", "--dry-run"])
+    assert "::m::500::n::400::t::30::" in p.stdout and "initseed::11" in p.stdout
+
+
+@pytest.mark.gpu
+def test_hiernmf_cli_writes_reference_outputs(tmp_path):
+    """hiernmf on a coord_ascii power-law matrix with a budget of 6 nodes:
+    the printed queue, and each non-root node's _W / _H (the parent's factor
+    column on the node's side, raw FP64) and _cols files, against the
+    oracle's node-by-node best-first loop."""
+    from planc_amd import datagen as D
+    from test_gpu_parity import _hier_frontier_reference
+    m, n, iters, budget, seed = 300, 400, 15, 6, 193957
+    ip, ix, dv = D.sparse_powerlaw(m, n, 0.03, 1.1, stream=9)
+    m = int(np.nonzero(np.diff(ip))[0].max()) + 1  # the file's row count: last non-empty row
+    ip = ip[:m + 1]
+    fn = tmp_path / "a.txt"
+    rows = np.repeat(np.arange(m), np.diff(ip))
+    with open(fn, "w") as f:
+        for a, b, x in zip(rows, ix, dv):
+            f.write(f"{a} {b} {float(x)!r}\n")
+    pre = str(tmp_path / "tree")
+    p = run([os.path.join(BIN, "hiernmf"), "-i", str(fn), "-t", str(iters), "--nodes", str(budget),
+             "--seed", str(seed), "-o", pre])
+    ref = _hier_frontier_reference(m, n, ip, ix, dv.astype(np.float64), budget, iters, seed)
+    table = p.stdout.split("idx level m n sigma score\n")[1].strip().splitlines()
+    assert [int(x.split()[0]) for x in table] == [r["index"] for r in ref]
+    by = {r["index"]: r for r in ref}
+    for r in ref[1:]:
+        par = by[(r["index"] - 1) // 2]
+        L = par["H"][:, 0] >= par["H"][:, 1]
+        swapped = (~L).sum() > L.sum()
+        side = (1 if swapped else 0) if r["index"] % 2 == 1 else (0 if swapped else 1)
+        w = np.fromfile(f"{pre}_{r['index']}_W", dtype=np.float64)
+        h = np.fromfile(f"{pre}_{r['index']}_H", dtype=np.float64)
+        c = np.loadtxt(f"{pre}_{r['index']}_cols", dtype=np.int64, ndmin=1)
+        assert np.abs(w - par["W"][:, side]).max() <= 1e-9 * np.abs(par["W"]).max()
+        assert np.abs(h - par["H"][:, side]).max() <= 1e-9 * np.abs(par["H"]).max()
+        assert np.array_equal(c, r["cols"])
+    assert not os.path.exists(f"{pre}_0_W")
