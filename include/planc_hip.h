@@ -424,7 +424,20 @@ typedef struct planc_ntf_desc {
   int iters;
   int dimtree;       /* 1: dimension-tree MTTKRP (2 full passes over X per sweep) */
   int compute_error; /* relative error after each iteration (-e 1) */
+  const void *X16T;  /* optional, with d_X16 and the dimension tree: the R x L
+                        transpose (ld R) of d_X16's L x R unfolding, L = dims[0]
+                        * .. * dims[s] with s = planc_ntf_tree_split(); made once
+                        by planc_transpose16.  The left contraction then runs as
+                        an NN product on it.  Null: the run transposes when it
+                        needs to. */
 } planc_ntf_desc;
+
+/* The dimension tree's split (AUNTF's dimension tree, ntf/auntf.hpp:123,
+ * dimtree/dimtrees.hpp): the left block is modes 0..s, the right block modes
+ * s+1..nmodes-1; s in [0, nmodes-2] minimises max(L, R) (L, R the products of
+ * the two blocks' dimensions), the reference's nmodes/2 on ties.  Returns -1
+ * for bad arguments. */
+int planc_ntf_tree_split(int nmodes, const long *dims);
 
 /* Runs AUNTF::computeNTF (ntf/auntf.hpp:128-177) with NTFHALS / NTFMU /
  * NTFANLSBPP / NTFAOADMM (ntf/ntfaoadmm.hpp:23-77, duals zero at the start
@@ -437,11 +450,12 @@ typedef struct planc_ntf_desc {
  * hold the initial factors, which are first normalised as NCPFactors::
  * normalize() does in the AUNTF constructor (ntf/auntf.hpp:88); on return
  * they hold the unit-norm factors and d_lambda (k, FP64) the weights.
- * d_X16: optional unscaled 16-bit copy of X for dtype != F32; without it the
- * run makes its own copy, which with the dimension tree carries per-column
- * power-of-two scales over the tree's L x R unfolding (so FP16 holds data
- * beyond its 65504 range; the scales are folded into the Khatri-Rao rows of
- * the right contraction and the output columns of the left one). */
+ * d_X16: optional unscaled 16-bit copy of X for dtype != F32 (and desc->X16T
+ * its transpose); without it the run makes its own copy, which with the
+ * dimension tree carries per-column power-of-two scales over the tree's L x R
+ * unfolding (so FP16 holds data beyond its 65504 range; the scales are folded
+ * into the Khatri-Rao rows of the right contraction and the output rows of the
+ * left one), and the copy's transpose. */
 int planc_ntf_run(planc_ctx *ctx, const planc_ntf_desc *desc, const float *d_X, const void *d_X16,
                   float *const *d_factors, double *d_lambda, double *h_relerr);
 

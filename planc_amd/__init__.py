@@ -508,10 +508,36 @@ class Context:
                     "planc_nmf_sparse_run")
         return rel[:iters]
 
+    def ntf_tree_split(self, dims):
+        """The dimension tree's split s (left block modes 0..s)."""
+        d = (ctypes.c_long * len(dims))(*dims)
+        s = self.lib.planc_ntf_tree_split(len(dims), d)
+        if s < 0:
+            raise ValueError(f"bad tensor dimensions {dims}")
+        return s
+
+    def ntf_copies16(self, X, dtype):
+        """Unscaled 16-bit copy of the tensor X (modes 0-fastest in memory,
+        any torch shape) and its dimension-tree transpose, the pair
+        ntf_run(X16=, X16T=) takes: made once, as the reference forms At once
+        for NMF."""
+        torch = _torch()
+        dims = [int(x) for x in X.shape]
+        s = self.ntf_tree_split(dims)
+        L = int(np.prod(dims[:s + 1]))
+        R = int(np.prod(dims[s + 1:]))
+        flatX = X.permute(*reversed(range(X.dim()))).reshape(-1)[:, None]
+        X16, _ = self.convert(flatX, dtype, scaled=False)
+        X16T = torch.empty(L * R, dtype=torch.int16, device=X.device)
+        self._check(self.lib.planc_transpose16(self.ctx, _ptr(X16), L, R, L, _ptr(X16T), R),
+                    "planc_transpose16")
+        return X16, X16T
+
     def ntf_run(self, X, factors, algo, iters, dtype=F32, dimtree=True, compute_error=True,
-                X16=None):
+                X16=None, X16T=None):
         """computeNTF on the device; factors (I_i x k, tight column-major) are
-        updated in place.  Returns (lambda, relerr)."""
+        updated in place.  Returns (lambda, relerr).  X16 / X16T: a caller's
+        unscaled 16-bit copy and its dimension-tree transpose (ntf_copies16)."""
         torch = _torch()
         N = len(factors)
         d = NTFDesc()
@@ -522,6 +548,7 @@ class Context:
                 raise ValueError("factors must be tight column-major")
         d.k = factors[0].shape[1]
         d.iters, d.dimtree, d.compute_error = iters, 1 if dimtree else 0, 1 if compute_error else 0
+        d.X16T = X16T.data_ptr() if X16T is not None else None
         lam = torch.empty(d.k, dtype=torch.float64, device=factors[0].device)
         arr = (ctypes.c_void_p * N)(*[f.data_ptr() for f in factors])
         rel = np.zeros(max(iters, 1))

@@ -45,7 +45,8 @@ class HierNode(ctypes.Structure):
 
 class NTFDesc(ctypes.Structure):
     _fields_ = [("algo", c_int), ("dtype", c_int), ("nmodes", c_int), ("dims", c_long * 8),
-                ("k", c_int), ("iters", c_int), ("dimtree", c_int), ("compute_error", c_int)]
+                ("k", c_int), ("iters", c_int), ("dimtree", c_int), ("compute_error", c_int),
+                ("X16T", c_void_p)]
 
 
 # name -> (restype, argtypes); the exported C ABI.
@@ -126,6 +127,7 @@ SIGNATURES = {
                                      c_void_p, c_void_p, ctypes.POINTER(c_double)]),
     "planc_nmf_run": (c_int, [c_void_p, ctypes.POINTER(NMFDesc), c_void_p, c_long, c_void_p,
                               c_void_p, c_void_p, c_void_p, ctypes.POINTER(c_double)]),
+    "planc_ntf_tree_split": (c_int, [c_int, ctypes.POINTER(c_long)]),
     "planc_ntf_run": (c_int, [c_void_p, ctypes.POINTER(NTFDesc), c_void_p, c_void_p,
                               ctypes.POINTER(c_void_p), c_void_p, ctypes.POINTER(c_double)]),
 }

@@ -791,6 +791,13 @@ int planc_nmf_sparse_run(planc_ctx *ctx, const planc_nmf_desc *desc, const long 
   return nmf_loop(ctx, desc, prod, d_W, d_H, h_relerr);
 }
 
+int planc_ntf_tree_split(int nmodes, const long *dims) {
+  if (nmodes < 2 || nmodes > 7 || !dims) return -1;
+  for (int i = 0; i < nmodes; ++i)
+    if (dims[i] <= 0) return -1;
+  return balanced_split(nmodes, dims);
+}
+
 int planc_ntf_run(planc_ctx *ctx, const planc_ntf_desc *desc, const float *d_X, const void *d_X16,
                   float *const *d_factors, double *d_lambda, double *h_relerr) {
   if (check_ctx(ctx) || !desc) return PLANC_EINVAL;
@@ -857,18 +864,28 @@ int planc_ntf_run(planc_ctx *ctx, const planc_ntf_desc *desc, const float *d_X, 
       Xop = x16;
     }
   }
-  // 16-bit dimension tree: the right contraction X_(LxR) * KRP_R runs on the
-  // streaming 16-bit NN kernel when X's rows come in 16-byte groups (L % 8
-  // == 0, aligned copy); otherwise a transposed copy (R x L) turns it into a
-  // TN product with contiguous k.
+  // 16-bit dimension tree: both unfoldings, X_(LxR) and its transpose
+  // (R x L, the same values and scales), so that each contraction runs on the
+  // streaming 16-bit NN kernel where its rows come in 16-byte groups: the
+  // right one X_(LxR) KRP_R on X when L % 8 == 0, else as a TN product on the
+  // transpose; the left one X_(LxR)^T KRP_L on the transpose when R % 8 == 0,
+  // else as a TN product on X.  The run makes the transpose with its own
+  // copy; with a caller's copy it is desc->X16T, or made here when the right
+  // contraction needs it.
+  const bool nnR = L % 8 == 0 && (uintptr_t)Xop % 16 == 0;
   const void *XopT = nullptr;
-  if (dt != PLANC_F32 && dimtree && !(L % 8 == 0 && (uintptr_t)Xop % 16 == 0)) {
-    void *xt = ws_get(ctx, 16, sizeof(uint16_t) * (size_t)numel);
-    if (!xt) return set_error(ctx, PLANC_ENOMEM, "planc_ntf_run: transposed 16-bit copy");
-    int rc = transpose16(ctx, Xop, L, R, L, xt, R);
-    if (rc) return rc;
-    XopT = xt;
+  if (dt != PLANC_F32 && dimtree) {
+    if (d_X16 && desc->X16T) {
+      XopT = desc->X16T;
+    } else if (!d_X16 || !nnR) {
+      void *xt = ws_get(ctx, 16, sizeof(uint16_t) * (size_t)numel);
+      if (!xt) return set_error(ctx, PLANC_ENOMEM, "planc_ntf_run: transposed 16-bit copy");
+      int rc = transpose16(ctx, Xop, L, R, L, xt, R);
+      if (rc) return rc;
+      XopT = xt;
+    }
   }
+  const bool nnQ = XopT && R % 8 == 0 && (uintptr_t)XopT % 16 == 0;
   // Float scratch: P (L x k) and Q (R x k) for the tree (or the direct
   // middle-mode projection), M (k x Imax).
   long Pmax = 0;
@@ -981,7 +998,7 @@ int planc_ntf_run(planc_ctx *ctx, const planc_ntf_desc *desc, const float *d_X, 
             kd.dim[q - s - 1] = dims[q];
             kd.ld[q - s - 1] = dims[q];
           }
-          if (XopT)
+          if (XopT && !nnR)
             RC(thin_gemm(ctx, dt, 1, XopT, R, L, R, nullptr, 0, &kd, xrscale, nullptr, k, P, L, 0, 0, xS));
           else
             RC(thin_gemm(ctx, dt, 0, Xop, L, L, R, nullptr, 0, &kd, xrscale, nullptr, k, P, L, 0, 0, xS));
@@ -994,7 +1011,10 @@ int planc_ntf_run(planc_ctx *ctx, const planc_ntf_desc *desc, const float *d_X, 
             kd.dim[q] = dims[q];
             kd.ld[q] = dims[q];
           }
-          RC(thin_gemm(ctx, dt, 1, Xop, L, R, L, nullptr, 0, &kd, nullptr, xscale, k, Q, R, 0, 0));
+          if (nnQ)
+            RC(thin_gemm(ctx, dt, 0, XopT, R, R, L, nullptr, 0, &kd, nullptr, xscale, k, Q, R, 0, 0));
+          else
+            RC(thin_gemm(ctx, dt, 1, Xop, L, R, L, nullptr, 0, &kd, nullptr, xscale, k, Q, R, 0, 0));
         }
         if (n <= s) {
           RC(contract_projection(ctx, P, L, s + 1, dims, d_factors, ldfL, n, k, M, k));
@@ -1138,7 +1158,7 @@ int planc_ntf_run(planc_ctx *ctx, const planc_ntf_desc *desc, const float *d_X, 
     auto graph_of = [&](int n_it, hipGraphExec_t *exec, hipGraphExec_t keep) -> int {
       std::vector<uintptr_t> key = {1, (uintptr_t)N, (uintptr_t)k, (uintptr_t)algo, (uintptr_t)dt,
                                     (uintptr_t)n_it, (uintptr_t)dimtree, (uintptr_t)desc->compute_error,
-                                    (uintptr_t)Xop, (uintptr_t)d_lambda, (uintptr_t)rel};
+                                    (uintptr_t)Xop, (uintptr_t)XopT, (uintptr_t)d_lambda, (uintptr_t)rel};
       for (int i = 0; i < N; ++i) {
         key.push_back((uintptr_t)d_factors[i]);
         key.push_back((uintptr_t)dims[i]);

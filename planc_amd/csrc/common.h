@@ -190,7 +190,8 @@ struct KrpDesc {
 // C = B^T * A (trans == 1; A is K x L col-major, C is k x L col-major), with
 // B the K x k operand: a column-major matrix (krp.nf == 0) or the Khatri-Rao
 // product of krp.f (lowest factor fastest).  rowscale (K, optional) scales
-// the rows of B; colscale (L, optional, trans == 1) scales output columns.
+// the rows of B; colscale (L, optional; trans == 1 or a 16-bit A) scales
+// output element l (TN: C's column l; NN: C's row l).
 // cmajor selects the output layout: 1 -> C[c + ldc*l] (k x L), 0 -> C[l + ldc*c].
 // splits_hint > 0 forces the split-K factor (0: automatic).  oscale (16-bit
 // A only) multiplies every output element after the FP32 accumulation.

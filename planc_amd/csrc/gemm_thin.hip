@@ -177,7 +177,7 @@ struct StreamParams {
   const uint4 *thin16;  // packed 16-bit thin operand (16-bit TN / NN)
   long plane;           // words of one packed plane; plane 1 (NP == 2) holds the low halves
   int np;               // packed planes: 1, or 2 for the compensated thin operand
-  const float *colscale;
+  const float *colscale;  // scale of output row l (TN: A's column l; 16-bit NN: A's row l), or null
   float *C;
   long ldc;
   int cmajor;
@@ -281,18 +281,15 @@ __device__ inline void store_acc_nn16(const StreamParams &p, long l0, int a, flo
   const int t = a / 8, j = a % 8;
   const int c = 16 * t + lc;
   if (c >= p.th.k) return;
-  v[0] *= p.oscale;
-  v[1] *= p.oscale;
-  v[2] *= p.oscale;
-  v[3] *= p.oscale;
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
     const long l = l0 + 32 * lg + 8 * u + j;
     if (l >= p.L) continue;
+    const float o = v[u] * ((p.colscale ? p.colscale[l] : 1.0f) * p.oscale);
     if (p.cmajor)
-      p.C[c + p.ldc * l] = v[u];
+      p.C[c + p.ldc * l] = o;
     else
-      p.C[l + p.ldc * c] = v[u];
+      p.C[l + p.ldc * c] = o;
   }
 }
 
@@ -1418,9 +1415,13 @@ __global__ __launch_bounds__(NTHR) void nn16_kernel(StreamParams p) {
 #pragma unroll
       for (int hf = 0; hf < 2; ++hf) {
         const long l = l0 + 32 * lg + 8 * v + 4 * hf;
-        const floatx4 o = floatx4{acc[8 * t + 4 * hf + 0][v], acc[8 * t + 4 * hf + 1][v],
-                                  acc[8 * t + 4 * hf + 2][v], acc[8 * t + 4 * hf + 3][v]} *
-                          p.oscale;
+        floatx4 o = floatx4{acc[8 * t + 4 * hf + 0][v], acc[8 * t + 4 * hf + 1][v],
+                            acc[8 * t + 4 * hf + 2][v], acc[8 * t + 4 * hf + 3][v]} *
+                    p.oscale;
+        if (p.colscale) {
+#pragma unroll
+          for (int e = 0; e < 4; ++e) o[e] *= l + e < p.L ? p.colscale[l + e] : 1.0f;
+        }
         if (!p.cmajor && p.cvec && l + 3 < p.L) {
           *(floatx4 *)(p.C + l + p.ldc * c) = o;
         } else {
@@ -1456,6 +1457,7 @@ struct GemmParams {
   int splits;
   long kchunk;
   float oscale;  // output multiplier (1: none)
+  const float *colscale;  // scale of output row l (A's row l), or null
 };
 
 // NP == 2: the compensated thin operand (pack_thin16_kernel), formed here
@@ -1593,7 +1595,7 @@ __global__ __launch_bounds__(NTHR) void nn_h_kernel(GemmParams p) {
         long gl = l0 + wave * 32 + 16 * r + 4 * (lane >> 4) + i;
         int c = 16 * t + (lane & 15);
         if (gl >= p.L) continue;
-        float v = acc[r][t][i] * (p.splits > 1 ? 1.0f : p.oscale);
+        float v = acc[r][t][i] * (p.splits > 1 ? 1.0f : p.oscale * (p.colscale ? p.colscale[gl] : 1.0f));
         if (p.splits > 1) {
           p.partial[((long)split * p.L + gl) * KP + c] = v;
         } else if (c < p.th.k) {
@@ -1607,14 +1609,14 @@ __global__ __launch_bounds__(NTHR) void nn_h_kernel(GemmParams p) {
 
 // Sum of the 16-bit NN split-K partials (fixed order) into the final layout.
 __global__ void reduce_splits_kernel(const float *partial, int splits, long L, int KP, int k,
-                                     float *C, long ldc, int cmajor, float oscale) {
+                                     float *C, long ldc, int cmajor, float oscale, const float *rowsc) {
   long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= L * k) return;
   long gl = idx / k;
   int c = (int)(idx % k);
   float s = 0.0f;
   for (int sp = 0; sp < splits; ++sp) s += partial[((long)sp * L + gl) * KP + c];
-  s *= oscale;
+  s *= oscale * (rowsc ? rowsc[gl] : 1.0f);
   if (cmajor)
     C[c + ldc * gl] = s;
   else
@@ -2059,6 +2061,8 @@ int thin_gemm(planc_ctx *ctx, int dtype, int trans, const void *A, long lda, lon
               const float *colscale, int k, float *C, long ldc, int cmajor, int splits_hint, float oscale) {
   if (L < 0 || K < 0 || k <= 0 || !A || !C) return set_error(ctx, PLANC_EINVAL, "thin_gemm: bad shape/pointer");
   if (dtype == PLANC_F32 && oscale != 1.0f) return set_error(ctx, PLANC_EINVAL, "thin_gemm: output scale with FP32");
+  if (dtype == PLANC_F32 && !trans && colscale)
+    return set_error(ctx, PLANC_EINVAL, "thin_gemm: output row scales with an FP32 NN product");
   if (dtype != PLANC_F32 && dtype != PLANC_BF16 && dtype != PLANC_F16)
     return set_error(ctx, PLANC_EINVAL, "thin_gemm: bad dtype");
   if (L == 0) return PLANC_OK;
@@ -2093,6 +2097,7 @@ int thin_gemm(planc_ctx *ctx, int dtype, int trans, const void *A, long lda, lon
       p.ldc = ldc;
       p.cmajor = cmajor;
       p.oscale = oscale;
+      p.colscale = colscale;
       const int BK = 64;
       long lblocks = (L + BL - 1) / BL;
       long want = (4L * ctx->num_cu + lblocks - 1) / lblocks;
@@ -2141,7 +2146,7 @@ int thin_gemm(planc_ctx *ctx, int dtype, int trans, const void *A, long lda, lon
         long tot = L * kc;
         unsigned nb = (unsigned)((tot + 255) / 256);
         reduce_splits_kernel<<<nb, 256, 0, ctx->stream>>>(p.partial, splits, L, KP, kc, Cs, ldc, cmajor,
-                                                          oscale);
+                                                          oscale, colscale);
         PLANC_LAUNCH_CHECK(ctx);
       }
       continue;
@@ -2154,7 +2159,7 @@ int thin_gemm(planc_ctx *ctx, int dtype, int trans, const void *A, long lda, lon
     p.L = L;
     p.K = K;
     p.th = th;
-    p.colscale = trans ? colscale : nullptr;
+    p.colscale = colscale;
     p.C = Cs;
     p.ldc = ldc;
     p.cmajor = cmajor;

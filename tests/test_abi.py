@@ -37,6 +37,32 @@ def test_library_exports_every_declared_symbol():
     assert sorted(_lib.SIGNATURES) == declared_functions()
 
 
+def test_ntf_tree_split_rule():
+    """planc_ntf_tree_split (host arithmetic, no GPU): the s minimising the
+    larger block product, nmodes/2 on ties, against a brute force."""
+    from planc_amd import _lib
+    lib = _lib.load()
+
+    def brute(dims):
+        N = len(dims)
+        best = None
+        for s in range(N - 1):
+            v = max(int(np.prod(dims[:s + 1])), int(np.prod(dims[s + 1:])))
+            if best is None or v < best[0] or (v == best[0] and s == N // 2):
+                best = (v, s)
+        return best[1]
+
+    rng = np.random.default_rng(3)
+    cases = [(128, 128, 128, 128), (256, 256, 256), (10, 1000, 10, 10), (5, 7), (2, 3, 5, 7, 11, 13, 17)]
+    cases += [tuple(int(x) for x in rng.integers(1, 60, size=rng.integers(2, 8))) for _ in range(200)]
+    for dims in cases:
+        arr = (ctypes.c_long * len(dims))(*dims)
+        assert lib.planc_ntf_tree_split(len(dims), arr) == brute(dims), dims
+    assert lib.planc_ntf_tree_split(1, (ctypes.c_long * 1)(5)) == -1
+    assert lib.planc_ntf_tree_split(3, (ctypes.c_long * 3)(4, 0, 2)) == -1
+    assert _lib.SIGNATURES["planc_ntf_tree_split"][0] is ctypes.c_int
+
+
 def test_version_string_without_gpu():
     from planc_amd import _lib
     lib = _lib.load()

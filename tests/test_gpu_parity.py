@@ -344,13 +344,18 @@ def test_ntf_one_block_mode_update_matches_two_launches(ctx, algo, dims, k, monk
 
 @pytest.mark.parametrize("dt", [P.BF16, P.F16])
 @pytest.mark.parametrize("dimtree", [True, False])
-@pytest.mark.parametrize("dims,k", [((40, 36, 32), 8), ((16, 12, 10, 8), 6)])
-def test_ntf_mixed_precision_final_error(ctx, dt, dimtree, dims, k):
+@pytest.mark.parametrize("dims,k", [((40, 36, 32), 8), ((16, 12, 10, 8), 6), ((24, 20, 14), 5)])
+@pytest.mark.parametrize("copies", ["x16", "pair", "own"])
+def test_ntf_mixed_precision_final_error(ctx, dt, dimtree, dims, k, copies):
     """NTF-HALS with BF16/FP16 MFMA in the MTTKRP (16-bit copy of X and its
     transpose, FP32 factors, FP64 Grams): the final relative error after 15
     iterations is within 1e-3 of the FP64 oracle's, the precision criterion
-    the mixed-precision path is held to."""
-    say(f"ntf mixed dt={dt} dimtree={dimtree} dims={dims}")
+    the mixed-precision path is held to.  copies: the caller's 16-bit copy
+    alone (left contraction as a TN product), the caller's copy and its
+    dimension-tree transpose (left contraction as an NN product on it), or
+    the run's own scaled pair; (24, 20, 14) has an unfolding whose rows do
+    not come in groups of eight (the other product form)."""
+    say(f"ntf mixed dt={dt} dimtree={dimtree} dims={dims} copies={copies}")
     X, _ = D.cp_tensor(dims, k, 0.01, stream=4)
     X32 = X.astype(np.float32)
     F0 = [f.astype(np.float32) for f in D.init_factors([(d, k) for d in dims], stream=4)]
@@ -358,9 +363,17 @@ def test_ntf_mixed_precision_final_error(ctx, dt, dimtree, dims, k):
                            dimtree=1 if dimtree else 0)
     dX = P.colmajor(X32)
     flat = dX.permute(*reversed(range(dX.dim()))).reshape(-1)[:, None]  # memory order
-    X16, _ = ctx.convert(flat, dt, scaled=False)
+    X16, X16T = None, None
+    if copies == "x16":
+        X16, _ = ctx.convert(flat, dt, scaled=False)
+    elif copies == "pair":
+        X16, X16T = ctx.ntf_copies16(dX, dt)
+        s = ctx.ntf_tree_split(list(dims))
+        L = int(np.prod(dims[:s + 1]))
+        x16 = X16.view(-1).cpu().numpy().reshape(-1, L)  # R rows of the L x R unfolding's columns
+        assert np.array_equal(X16T.cpu().numpy().reshape(L, -1), x16.T)
     dF = [P.colmajor(f) for f in F0]
-    lam, rel = ctx.ntf_run(dX, dF, P.HALS, 15, dtype=dt, dimtree=dimtree, X16=X16)
+    lam, rel = ctx.ntf_run(dX, dF, P.HALS, 15, dtype=dt, dimtree=dimtree, X16=X16, X16T=X16T)
     say(f"  gpu {rel[-1]:.6f} oracle {rel_r[-1]:.6f}")
     assert np.all(np.isfinite(rel))
     assert abs(rel[-1] - rel_r[-1]) < 1e-3
