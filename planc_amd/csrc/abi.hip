@@ -974,6 +974,80 @@ int planc_ntf_run(planc_ctx *ctx, const planc_ntf_desc *desc, const float *d_X, 
     ldfL[i] = dims[i];
     ldfR[i] = dims[i];
   }
+  // The dimension tree's two contractions: P = X_(LxR) KRP_R (right) and
+  // Q = X_(LxR)^T KRP_L (left), each in one of its two forms.
+  auto right_contraction = [&](bool tn_on_t) -> int {
+    KrpDesc kd{};
+    kd.nf = N - 1 - s;
+    for (int q = s + 1; q < N; ++q) {
+      kd.f[q - s - 1] = d_factors[q];
+      kd.dim[q - s - 1] = dims[q];
+      kd.ld[q - s - 1] = dims[q];
+    }
+    if (tn_on_t) return thin_gemm(ctx, dt, 1, XopT, R, L, R, nullptr, 0, &kd, xrscale, nullptr, k, P, L, 0, 0, xS);
+    return thin_gemm(ctx, dt, 0, Xop, L, L, R, nullptr, 0, &kd, xrscale, nullptr, k, P, L, 0, 0, xS);
+  };
+  auto left_contraction = [&](bool nn_on_t) -> int {
+    KrpDesc kd{};
+    kd.nf = s + 1;
+    for (int q = 0; q <= s; ++q) {
+      kd.f[q] = d_factors[q];
+      kd.dim[q] = dims[q];
+      kd.ld[q] = dims[q];
+    }
+    if (nn_on_t) return thin_gemm(ctx, dt, 0, XopT, R, R, L, nullptr, 0, &kd, nullptr, xscale, k, Q, R, 0, 0);
+    return thin_gemm(ctx, dt, 1, Xop, L, R, L, nullptr, 0, &kd, nullptr, xscale, k, Q, R, 0, 0);
+  };
+  // With both unfoldings each contraction takes the faster of its feasible
+  // forms on this shape (NN on X or TN on the transpose; TN on X or NN on
+  // the transpose), timed once per context and shape.
+  bool tnR = XopT && !nnR, nnL = nnQ;
+  if (dimtree && dt != PLANC_F32 && XopT && (nnR || nnQ)) {
+    std::vector<long> key = {dt, N, k, s, L, R, nnR ? 1 : 0, nnQ ? 1 : 0};
+    for (int i = 0; i < N; ++i) key.push_back(dims[i]);
+    int form = -1;
+    for (const auto &e : ctx->tree_forms)
+      if (e.first == key) form = e.second;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    PLANC_HIP_CHECK(ctx, hipStreamIsCapturing(ctx->stream, &cs));
+    if (form < 0 && cs == hipStreamCaptureStatusNone) {
+      hipEvent_t e0, e1;
+      PLANC_HIP_CHECK(ctx, hipEventCreate(&e0));
+      PLANC_HIP_CHECK(ctx, hipEventCreate(&e1));
+      auto timed = [&](const std::function<int()> &f, float *ms) -> int {
+        for (int r = 0; r < 2; ++r)  // geometry tuning and warm-up
+          if (int e = f()) return e;
+        PLANC_HIP_CHECK(ctx, hipEventRecord(e0, ctx->stream));
+        for (int r = 0; r < 3; ++r)
+          if (int e = f()) return e;
+        PLANC_HIP_CHECK(ctx, hipEventRecord(e1, ctx->stream));
+        PLANC_HIP_CHECK(ctx, hipEventSynchronize(e1));
+        PLANC_HIP_CHECK(ctx, hipEventElapsedTime(ms, e0, e1));
+        return PLANC_OK;
+      };
+      form = 0;
+      float a = 0, b = 0;
+      if (nnR) {
+        RC(timed([&] { return right_contraction(false); }, &a));
+        RC(timed([&] { return right_contraction(true); }, &b));
+        if (b < a) form |= 1;
+      } else {
+        form |= 1;
+      }
+      if (nnQ) {
+        RC(timed([&] { return left_contraction(true); }, &a));
+        RC(timed([&] { return left_contraction(false); }, &b));
+        if (a <= b) form |= 2;
+      }
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      ctx->tree_forms.push_back({key, form});
+    }
+    if (form >= 0) {
+      tnR = (form & 1) != 0;
+      nnL = (form & 2) != 0;
+    }
+  }
   // One outer iteration (all mode updates + the error), written so the same
   // launch sequence can be captured into a hipGraph: no allocation, no host
   // synchronisation, the error written to *rel_out.
@@ -990,32 +1064,8 @@ int planc_ntf_run(planc_ctx *ctx, const planc_ntf_desc *desc, const float *d_X, 
       const float *Mp = M;
       long mrs = k, mcs = 1;
       if (dimtree) {
-        if (n == 0) {
-          KrpDesc kd{};
-          kd.nf = N - 1 - s;
-          for (int q = s + 1; q < N; ++q) {
-            kd.f[q - s - 1] = d_factors[q];
-            kd.dim[q - s - 1] = dims[q];
-            kd.ld[q - s - 1] = dims[q];
-          }
-          if (XopT && !nnR)
-            RC(thin_gemm(ctx, dt, 1, XopT, R, L, R, nullptr, 0, &kd, xrscale, nullptr, k, P, L, 0, 0, xS));
-          else
-            RC(thin_gemm(ctx, dt, 0, Xop, L, L, R, nullptr, 0, &kd, xrscale, nullptr, k, P, L, 0, 0, xS));
-        }
-        if (n == s + 1) {
-          KrpDesc kd{};
-          kd.nf = s + 1;
-          for (int q = 0; q <= s; ++q) {
-            kd.f[q] = d_factors[q];
-            kd.dim[q] = dims[q];
-            kd.ld[q] = dims[q];
-          }
-          if (nnQ)
-            RC(thin_gemm(ctx, dt, 0, XopT, R, R, L, nullptr, 0, &kd, nullptr, xscale, k, Q, R, 0, 0));
-          else
-            RC(thin_gemm(ctx, dt, 1, Xop, L, R, L, nullptr, 0, &kd, nullptr, xscale, k, Q, R, 0, 0));
-        }
+        if (n == 0) RC(right_contraction(tnR));
+        if (n == s + 1) RC(left_contraction(nnL));
         if (n <= s) {
           RC(contract_projection(ctx, P, L, s + 1, dims, d_factors, ldfL, n, k, M, k));
         } else if (fused && N - 1 - s == 1) {

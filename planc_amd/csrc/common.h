@@ -105,6 +105,10 @@ struct planc_ctx {
   int thin_planes = 1;
   // Tuned streaming-product geometries: shape key -> {cw, wk, nsplit, dual, shared}.
   std::vector<std::pair<std::vector<long>, std::vector<long>>> gemm_tuned;
+  // 16-bit dimension-tree contraction forms chosen per shape: key -> bit 0
+  // (right contraction as TN on the transposed copy), bit 1 (left
+  // contraction as NN on the transposed copy).
+  std::vector<std::pair<std::vector<long>, int>> tree_forms;
 };
 
 namespace planc {

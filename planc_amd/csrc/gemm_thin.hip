@@ -43,6 +43,7 @@
 // LDS-transposing kernel with a separate split-K reduction.
 #include <cstdio>
 #include <cstdlib>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -682,6 +683,27 @@ __device__ inline long pack_index(long kk, int t, int lc, int NT) {
 // word offset `plane`) lo = rn16(b - hi), so hi + lo carries b to about 16
 // (BF16) or 22 (FP16) significant bits and the products, with both planes
 // contracted against the same big-operand words, round only the big operand.
+// The NP words of 8 consecutive values (lo: the first four, hi: the next).
+template <typename TA, int NP>
+__device__ inline void pack8(const floatx4 &lo, const floatx4 &hi, uint4 (&w)[NP]) {
+  uint16_t h[8];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    h[u] = to16<TA>(lo[u]);
+    h[4 + u] = to16<TA>(hi[u]);
+  }
+  w[0] = __builtin_bit_cast(uint4, h);
+  if constexpr (NP == 2) {
+    uint16_t r[8];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      r[u] = to16<TA>(lo[u] - from16<TA>(h[u]));
+      r[4 + u] = to16<TA>(hi[u] - from16<TA>(h[4 + u]));
+    }
+    w[1] = __builtin_bit_cast(uint4, r);
+  }
+}
+
 template <typename TA, int NP>
 __global__ void pack_thin16_kernel(ThinDev th, long K, int NT, long nsuper, uint4 *out, long plane) {
   const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -692,24 +714,76 @@ __global__ void pack_thin16_kernel(ThinDev th, long K, int NT, long nsuper, uint
   const long sh = st / NT;  // 2 S + h
   const int c = 16 * t + (lane & 15);
   const long kb = 64 * (sh >> 1) + 16 * (lane >> 4) + 8 * (sh & 1);
-  floatx4 lo = thin4(th, kb, K, c), hi = thin4(th, kb + 4, K, c);
-  uint16_t h[8];
-#pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    h[u] = to16<TA>(lo[u]);
-    h[4 + u] = to16<TA>(hi[u]);
-  }
-  out[e] = __builtin_bit_cast(uint4, h);
-  if constexpr (NP == 2) {
-    uint16_t r[8];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      r[u] = to16<TA>(lo[u] - from16<TA>(h[u]));
-      r[4 + u] = to16<TA>(hi[u] - from16<TA>(h[4 + u]));
-    }
-    out[plane + e] = __builtin_bit_cast(uint4, r);
-  }
+  uint4 w[NP];
+  pack8<TA, NP>(thin4(th, kb, K, c), thin4(th, kb + 4, K, c), w);
+  out[e] = w[0];
+  if constexpr (NP == 2) out[plane + e] = w[1];
 }
+
+// On-the-fly 16-bit thin operand of the block-shared NN kernel (instead of
+// the packed buffer): producer thread tid < NT * 64 owns position (t, ln) of
+// a 32-k step, column c = 16 t + (ln & 15) (clamped to k - 1: columns past k
+// only feed output columns that are never stored) and the 8 rows kb + 8 (ln
+// >> 4) + 0..7.  load() issues the loads: B's rows, or the first KRP factor's
+// rows and the second factor's entry (the 8 rows share it: dim0 % 8 == 0),
+// and the row scales; store() multiplies, rounds to the NP planes and writes
+// the LDS words (q NT + t) 64 + ln.
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+struct RawFly {
+  floatx4 a0, a1, r0, r1;
+  float sc;
+};
+struct ThinProducer16 {
+  const float *tb, *tf1, *rs;
+  unsigned dim0;
+  int t, ln;
+  bool on, krp2;
+  __device__ ThinProducer16(const ThinDev &th, int NT) {
+    const int tid = threadIdx.x;
+    on = tid < NT * 64;
+    t = on ? tid >> 6 : 0;
+    ln = tid & 63;
+    int c = 16 * t + (ln & 15);
+    c = c < th.k ? c : th.k - 1;
+    tb = th.nf == 0 ? th.B + th.ldb * c : th.f[0] + th.ld[0] * c;
+    krp2 = th.nf == 2;
+    tf1 = krp2 ? th.f[1] + th.ld[1] * c : nullptr;
+    dim0 = krp2 ? th.dim[0] : 1u;
+    rs = th.rowscale;
+  }
+  __device__ void load(long kb, RawFly &r) const {
+    if (!on) return;
+    kb += 8 * (ln >> 4);
+    if (krp2) {
+      const unsigned q = (unsigned)kb / dim0;
+      const unsigned d0 = (unsigned)kb - q * dim0;
+      r.a0 = *(const floatx4 *)(tb + d0);
+      r.a1 = *(const floatx4 *)(tb + d0 + 4);
+      r.sc = tf1[q];
+    } else {
+      r.a0 = *(const floatx4 *)(tb + kb);
+      r.a1 = *(const floatx4 *)(tb + kb + 4);
+      r.sc = 1.0f;
+    }
+    if (rs) {
+      r.r0 = *(const floatx4 *)(rs + kb);
+      r.r1 = *(const floatx4 *)(rs + kb + 4);
+    }
+  }
+  template <typename TA, int NT, int NP>
+  __device__ void store(const RawFly &r, u32x4 (*ts)[NP * NT * 64], int stage) const {
+    if (!on) return;
+    floatx4 lo = r.a0 * r.sc, hi = r.a1 * r.sc;
+    if (rs) {
+      lo *= r.r0;
+      hi *= r.r1;
+    }
+    uint4 w[NP];
+    pack8<TA, NP>(lo, hi, w);
+#pragma unroll
+    for (int q = 0; q < NP; ++q) ts[stage][(q * NT + t) * 64 + ln] = __builtin_bit_cast(u32x4, w[q]);
+  }
+};
 
 // 16-bit TN.  DUAL = 0: step = 32 k, a lane reads 8 consecutive k of a
 // column (16 bytes); DUAL = 1: step = 64 k, a lane reads 16 consecutive k
@@ -1287,9 +1361,10 @@ __device__ inline uint4 transpose8_word(const uint4 (&w)[8], int j) {
 }
 
 // SH: the thin words of each step staged once per block in an LDS ring
-// (stream_steps_shared; the block's four waves share the k range).  NP == 2:
-// the compensated thin operand (both planes against the same A words).
-typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+// (stream_steps_shared; the block's four waves share the k range); SH == 2:
+// the staged words are formed from the FP32 thin operand on the fly
+// (ThinProducer16) rather than read from the packed buffer.  NP == 2: the
+// compensated thin operand (both planes against the same A words).
 template <typename TA, int NT, int SH, int NP>
 __global__ __launch_bounds__(NTHR) void nn16_kernel(StreamParams p) {
   constexpr int STEP = 32, PD = 2, NA = 8 * NT;
@@ -1324,7 +1399,7 @@ __global__ __launch_bounds__(NTHR) void nn16_kernel(StreamParams p) {
 #pragma unroll
       for (int t = 0; t < NT; ++t) b.b[q][t] = p.thin16[q * p.plane + pack_index(kb, t, lc, NT)];
   };
-  if constexpr (SH) {
+  if constexpr (SH != 0) {
     // LDS word (q NT + t) 64 + lane: plane q, tile t of the step; producer
     // thread tid + 256 j (j < RW) fills word tid + 256 j.
     constexpr int WORDS = NP * NT * 64, RW = (WORDS + NTHR - 1) / NTHR;
@@ -1341,6 +1416,29 @@ __global__ __launch_bounds__(NTHR) void nn16_kernel(StreamParams p) {
 #pragma unroll
       for (int u = 0; u < 8; ++u) b.w[u] = *(const uint4 *)(base + p.lda * (kb + u));
     };
+    auto mma_st = [&](BufA &b, int stage) {
+      uint4 bt[NP][NT];
+#pragma unroll
+      for (int q = 0; q < NP; ++q)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) bt[q][t] = __builtin_bit_cast(uint4, ts[stage][(q * NT + t) * 64 + lane]);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const uint4 o = transpose8_word(b.w, j);
+#pragma unroll
+        for (int q = 0; q < NP; ++q)
+#pragma unroll
+          for (int t = 0; t < NT; ++t) acc[8 * t + j] = mfma16<TA>(o, bt[q][t], acc[8 * t + j]);
+      }
+    };
+    BufA buf[PD];
+    if constexpr (SH == 2) {
+      const ThinProducer16 prod(p.th, NT);
+      RawFly raw[PD];
+      auto rload = [&](long i, RawFly &r) { prod.load(kbeg + i * STEP, r); };
+      auto rstore = [&](const RawFly &r, int stage) { prod.template store<TA, NT, NP>(r, ts, stage); };
+      stream_steps_shared<PD, SH_TS>(nfull, buf, raw, loadA, rload, rstore, mma_st);
+    } else {
     auto rload = [&](long i, RawN &r) {
 #pragma unroll
       for (int j = 0; j < RW; ++j) {
@@ -1360,24 +1458,9 @@ __global__ __launch_bounds__(NTHR) void nn16_kernel(StreamParams p) {
         if (w < WORDS) ts[stage][w] = r.w[j];
       }
     };
-    auto mma_sh = [&](BufA &b, int stage) {
-      uint4 bt[NP][NT];
-#pragma unroll
-      for (int q = 0; q < NP; ++q)
-#pragma unroll
-        for (int t = 0; t < NT; ++t) bt[q][t] = __builtin_bit_cast(uint4, ts[stage][(q * NT + t) * 64 + lane]);
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const uint4 o = transpose8_word(b.w, j);
-#pragma unroll
-        for (int q = 0; q < NP; ++q)
-#pragma unroll
-          for (int t = 0; t < NT; ++t) acc[8 * t + j] = mfma16<TA>(o, bt[q][t], acc[8 * t + j]);
-      }
-    };
-    BufA buf[PD];
     RawN raw[PD];
-    stream_steps_shared<PD, SH_TS>(nfull, buf, raw, loadA, rload, rstore, mma_sh);
+    stream_steps_shared<PD, SH_TS>(nfull, buf, raw, loadA, rload, rstore, mma_st);
+    }
   } else {
     BufNN16<NT, NP> buf[PD];
     stream_steps<PD>(nfull, buf, load, mma);
@@ -1394,10 +1477,20 @@ __global__ __launch_bounds__(NTHR) void nn16_kernel(StreamParams p) {
       for (int e = 0; e < 8; ++e) v[e] = kb + u < kend ? base[e + p.lda * (kb + u)] : (uint16_t)0;
       b.w[u] = __builtin_bit_cast(uint4, v);
     }
+    if constexpr (SH == 2) {
 #pragma unroll
-    for (int q = 0; q < NP; ++q)
+      for (int t = 0; t < NT; ++t) {
+        uint4 w[NP];
+        pack8<TA, NP>(thin4(p.th, kb, kend, 16 * t + lc), thin4(p.th, kb + 4, kend, 16 * t + lc), w);
 #pragma unroll
-      for (int t = 0; t < NT; ++t) b.b[q][t] = p.thin16[q * p.plane + pack_index(kb, t, lc, NT)];
+        for (int q = 0; q < NP; ++q) b.b[q][t] = w[q];
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < NP; ++q)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) b.b[q][t] = p.thin16[q * p.plane + pack_index(kb, t, lc, NT)];
+    }
     mma(b);
   }
   if (p.nsplit > 1) {
@@ -1667,7 +1760,7 @@ Geometry make_geometry(long L, long K, int trans, int dtype, int NT, int cw, int
 // and the grid stays under 16k waves.  The first entry is the static choice
 // used when tuning is off or impossible (graph capture).
 std::vector<Geometry> candidate_geometries(const planc_ctx *ctx, long L, long K, int trans, int dtype,
-                                           int NT, bool shared_ok, bool f32dual_ok) {
+                                           int NT, bool shared_ok, bool f32dual_ok, bool fly_ok = false) {
   std::vector<Geometry> out;
   const int step = dtype == PLANC_F32 ? 16 : 32;
   const long ksteps = (K + step - 1) / step;
@@ -1676,16 +1769,28 @@ std::vector<Geometry> candidate_geometries(const planc_ctx *ctx, long L, long K,
   if (!trans && dtype != PLANC_F32) {
     // Streaming 16-bit NN: one k phase, K splits only; per-wave or
     // block-shared thin operand.
-    for (int sh = 0; sh < (shared_ok ? 2 : 1); ++sh)
+    // With few row groups the partial-traffic bound can stop the splits
+    // before the grid fills the device; the largest split count reached is
+    // then a candidate too.
+    for (int sh = 0; sh < (shared_ok ? (fly_ok ? 3 : 2) : 1); ++sh) {
+      Geometry last{};
+      bool skipped = false, split_added = false;
       for (long ns = 1; ns <= 4096; ns *= 2) {
         Geometry g = make_geometry(L, K, trans, dtype, NT, 1, 1, ns, 0, sh);
         if (ns > 1 && (g.nsplit < ns || ksteps / g.nsplit < 8)) break;
         if (ns > 1 && 2.0 * g.nsplit * g.groups * g.tile_bytes > 0.10 * abytes) break;
         const long waves = g.blocks_x * 4 * g.nsplit;
         if (ns > 1 && waves > 16L * 1024) break;
-        if (!out.empty() && waves < 2L * ctx->num_cu && ns < 4096) continue;
+        if (!out.empty() && waves < 2L * ctx->num_cu && ns < 4096) {
+          last = g;
+          skipped = true;
+          continue;
+        }
         out.push_back(g);
+        split_added = split_added || ns > 1;
       }
+      if (skipped && !split_added) out.push_back(last);
+    }
     return out;
   }
   std::vector<int> cws;
@@ -1817,10 +1922,12 @@ void launch_nn16(int dtype, int NT, int shared, dim3 grid, hipStream_t st, const
     else nn16_kernel<TA, 4, SHV, NP><<<grid, NTHR, 0, st>>>(p);              \
   } while (0)
   if (dtype == PLANC_BF16) {
-    if (shared) NN16(bf16_t, 1);
+    if (shared == 2) NN16(bf16_t, 2);
+    else if (shared) NN16(bf16_t, 1);
     else NN16(bf16_t, 0);
   } else {
-    if (shared) NN16(f16_t, 1);
+    if (shared == 2) NN16(f16_t, 2);
+    else if (shared) NN16(f16_t, 1);
     else NN16(f16_t, 0);
   }
 #undef NN16
@@ -1956,29 +2063,38 @@ bool stream_capturing(hipStream_t st) {
 // timed launches each; the products are idempotent) unless tuning is off
 // (PLANC_AUTOTUNE=0) or the stream is being captured into a graph, where the
 // static choice is used.  PLANC_TUNE_LOG=1 prints each candidate's time.
-int pick_geometry(planc_ctx *ctx, int dtype, int trans, int NT, int tm, const StreamParams &p,
-                  Geometry *out) {
+// fly_ok: the shared 16-bit NN kernel may form its thin operand on the fly
+// (no packed buffer).  pack() launches the packing of the thin operand that
+// the other 16-bit candidates read: it runs before the first candidate is
+// timed, and with fly_ok it is timed too and its time charged to every
+// candidate that needs it.
+int pick_geometry(planc_ctx *ctx, int dtype, int trans, int NT, int tm, const StreamParams &p, bool fly_ok,
+                  const std::function<int()> &pack, Geometry *out) {
   // PLANC_STREAM_VARIANT restricts the tuner to one kernel family (used to
   // test each path): "plain" (per-wave thin operand, 16/32-k steps), "dual"
-  // (per-wave, 32-k FP32 / 64-k 16-bit TN steps) or "shared" (block-shared
-  // LDS thin operand), each where it applies; unset: all.
+  // (per-wave, 32-k FP32 / 64-k 16-bit TN steps), "shared" (block-shared
+  // LDS thin operand) or "fly" (block-shared, formed on the fly: 16-bit NN),
+  // each where it applies; unset: all.
   const char *venv = getenv("PLANC_STREAM_VARIANT");
   const std::string vs = venv ? venv : "";
-  const int smode = vs == "plain" ? 0 : (vs == "shared" ? 1 : (vs == "dual" ? 3 : 2));
-  const std::vector<long> key = {dtype, trans, NT, tm, p.L, p.K, p.lda, p.th.k, p.avec, p.cmajor, smode, p.np};
+  const int smode = vs == "plain" ? 0 : (vs == "shared" ? 1 : (vs == "dual" ? 3 : (vs == "fly" ? 4 : 2)));
+  const std::vector<long> key = {dtype, trans, NT, tm, p.L, p.K, p.lda, p.th.k, p.avec, p.cmajor, smode, p.np,
+                                 fly_ok ? 1 : 0};
   for (const auto &e : ctx->gemm_tuned)
     if (e.first == key) {
       const auto &v = e.second;
       *out = make_geometry(p.L, p.K, trans, dtype, NT, (int)v[0], (int)v[1], v[2], (int)v[3], (int)v[4]);
       return PLANC_OK;
     }
-  const bool shared_ok = (dtype == PLANC_F32 ? tm != THIN_SAFE : p.avec != 0) && (smode == 1 || smode == 2);
-  std::vector<Geometry> cands =
-      candidate_geometries(ctx, p.L, p.K, trans, dtype, NT, shared_ok, dtype == PLANC_F32 && tm != THIN_SAFE);
+  const bool shared_ok = (dtype == PLANC_F32 ? tm != THIN_SAFE : p.avec != 0) && smode != 0 && smode != 3;
+  std::vector<Geometry> cands = candidate_geometries(ctx, p.L, p.K, trans, dtype, NT, shared_ok,
+                                                     dtype == PLANC_F32 && tm != THIN_SAFE, fly_ok && shared_ok);
   if (smode != 2) {
     std::vector<Geometry> sel;
     for (const Geometry &g : cands)
-      if (smode == 1 ? g.shared != 0 : (g.shared == 0 && (smode == 3) == (g.dual != 0))) sel.push_back(g);
+      if (smode == 4 ? g.shared == 2
+                     : smode == 1 ? g.shared != 0 : (g.shared == 0 && (smode == 3) == (g.dual != 0)))
+        sel.push_back(g);
     if (!sel.empty()) cands.swap(sel);
   }
   const char *env = getenv("PLANC_AUTOTUNE");
@@ -1990,6 +2106,21 @@ int pick_geometry(planc_ctx *ctx, int dtype, int trans, int NT, int tm, const St
     hipEvent_t e0, e1;
     PLANC_HIP_CHECK(ctx, hipEventCreate(&e0));
     PLANC_HIP_CHECK(ctx, hipEventCreate(&e1));
+    float pack_ms = 0.0f;  // two pack launches, as each candidate is timed over two launches
+    if (dtype != PLANC_F32) {
+      int prc = pack();
+      if (prc) return prc;
+      bool any_fly = false;
+      for (const Geometry &g : cands) any_fly = any_fly || g.shared == 2;
+      if (any_fly) {
+        PLANC_HIP_CHECK(ctx, hipEventRecord(e0, ctx->stream));
+        for (int r = 0; r < 2 && !prc; ++r) prc = pack();
+        if (prc) return prc;
+        PLANC_HIP_CHECK(ctx, hipEventRecord(e1, ctx->stream));
+        PLANC_HIP_CHECK(ctx, hipEventSynchronize(e1));
+        PLANC_HIP_CHECK(ctx, hipEventElapsedTime(&pack_ms, e0, e1));
+      }
+    }
     float best_ms = 0.0f;
     bool have = false;
     for (const Geometry &g : cands) {
@@ -2004,6 +2135,7 @@ int pick_geometry(planc_ctx *ctx, int dtype, int trans, int NT, int tm, const St
       PLANC_HIP_CHECK(ctx, hipEventSynchronize(e1));
       float ms = 0.0f;
       PLANC_HIP_CHECK(ctx, hipEventElapsedTime(&ms, e0, e1));
+      if (g.shared != 2) ms += pack_ms;
       if (log)
         fprintf(stderr,
                 "planc tune dtype %d trans %d L %ld K %ld k %d: cw %d wk %d nsplit %ld dual %d shared %d -> %.2f us\n",
@@ -2168,9 +2300,12 @@ int thin_gemm(planc_ctx *ctx, int dtype, int trans, const void *A, long lda, lon
     p.oscale = oscale;
     int tm = thin_mode(th, avec, rowscale);
     if (!trans && L % 4 != 0) tm = THIN_SAFE;
-    if (dtype != PLANC_F32) {
-      // 16-bit TN / NN: the thin operand packed once into the 16-bit MFMA
-      // operand layout (8 consecutive k of one column per lane).
+    // 16-bit TN / NN: the thin operand packed into the 16-bit MFMA operand
+    // layout (8 consecutive k of one column per lane) once per product,
+    // unless the chosen kernel forms it on the fly.
+    bool packed = false;
+    auto pack = [&]() -> int {
+      if (dtype == PLANC_F32) return PLANC_OK;
       const long nsuper = (K + 63) / 64;
       const long tot = nsuper * 2 * NT * 64;  // words of one plane
       uint4 *pk = (uint4 *)ws_get(ctx, 6, (size_t)np * tot * 16);
@@ -2187,13 +2322,23 @@ int thin_gemm(planc_ctx *ctx, int dtype, int trans, const void *A, long lda, lon
       PLANC_LAUNCH_CHECK(ctx);
       p.thin16 = pk;
       p.plane = tot;
-    }
+      packed = true;
+      return PLANC_OK;
+    };
     p.np = np;
+    // The on-the-fly operand reads B (or the first KRP factor, with the
+    // second's entry shared by 8 consecutive rows) with 16-byte loads.
+    const bool fly_ok = dtype != PLANC_F32 && !trans && th.vec && th.nf <= 2 &&
+                        (th.nf < 2 || th.dim[0] % 8 == 0);
     Geometry g;
     if (splits_hint > 0) {
       g = make_geometry(L, K, trans, dtype, NT, trans && dtype == PLANC_F32 ? 2 : 4, 1, splits_hint);
     } else {
-      rc = pick_geometry(ctx, dtype, trans, NT, tm, p, &g);
+      rc = pick_geometry(ctx, dtype, trans, NT, tm, p, fly_ok, pack, &g);
+      if (rc) return rc;
+    }
+    if (!packed && !(g.shared == 2 && !trans && dtype != PLANC_F32)) {
+      rc = pack();
       if (rc) return rc;
     }
     rc = launch_stream(ctx, dtype, trans, NT, tm, p, g);

@@ -73,12 +73,13 @@ GEMM_CASES = [
 ]
 
 
-@pytest.mark.parametrize("variant", ["plain", "dual", "shared"])
+@pytest.mark.parametrize("variant", ["plain", "dual", "shared", "fly"])
 @pytest.mark.parametrize("dt", [P.F32, P.BF16, P.F16])
 @pytest.mark.parametrize("L,K,k", GEMM_CASES)
 def test_gemm_nn_tn(ctx, monkeypatch, dt, L, K, k, variant):
     """Every streaming family: per-wave thin operand with 16/32-k steps, with
-    32-k (FP32) / 64-k (16-bit) TN steps, and the block-shared LDS ring
+    32-k (FP32) / 64-k (16-bit) TN steps, the block-shared LDS ring, and
+    (16-bit NN) the block-shared ring formed on the fly from the FP32 operand
     (each where it applies; otherwise the tuner's other candidates run)."""
     monkeypatch.setenv("PLANC_STREAM_VARIANT", variant)
     say(f"gemm dt={dt} L={L} K={K} k={k} variant={variant}")
@@ -475,7 +476,7 @@ def test_partial_mttkrp_and_multi_ttv(ctx):
             assert rel_err(M.cpu().numpy(), ref) < 1e-5, (s, n)
 
 
-@pytest.mark.parametrize("variant", ["plain", "dual", "shared"])
+@pytest.mark.parametrize("variant", ["plain", "dual", "shared", "fly"])
 @pytest.mark.parametrize("dt", [P.F32, P.BF16, P.F16])
 def test_partial_mttkrp_streaming_paths(ctx, monkeypatch, dt, variant):
     """Right (X_(L x R) * KRP_R) and left (KRP_L^T * X) partial contractions
