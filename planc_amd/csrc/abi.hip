@@ -313,6 +313,11 @@ int nmf_loop(planc_ctx *ctx, const planc_nmf_desc *desc, NMFProducts &P, float *
   // half-step) and carried into the next H half-step -- rescaled by the
   // normalisation for HALS -- instead of a second pass over W.
   bool wtw_valid = false;
+  // The H rows' update and H^T H in one pass over H (and, for MU, the W
+  // rows' update, W^T W and <W, A H> in one pass over W), where the layouts
+  // allow; the Grams are the same sums, accumulated in another order.
+  const bool fuseH = rows_gram_ok(d_H, n, WtA, k, 1, k);
+  const bool fuseW = algo == PLANC_MU && rows_gram_ok(d_W, m, AH, 1, m, k);
   for (int it = 0; it < iters; ++it) {
     // H half-step: W^T A, W^T W (+ regH).
     RC(P.wta(d_W, WtA));
@@ -323,30 +328,37 @@ int nmf_loop(planc_ctx *ctx, const planc_nmf_desc *desc, NMFProducts &P, float *
       RC(apply_reg(ctx, Greg, k, desc->regH));
       Gh = Greg;
     }
-    RC(row_update(ctx, algo == PLANC_MU ? 0 : 1, d_H, n, k, n, WtA, k, 1, Gh, 1e-17f));
+    if (fuseH)
+      RC(row_update_gram(ctx, algo == PLANC_MU ? 0 : 1, d_H, n, k, n, WtA, k, 1, Gh, 1e-17f, HtH, nullptr,
+                         nullptr));
+    else
+      RC(row_update(ctx, algo == PLANC_MU ? 0 : 1, d_H, n, k, n, WtA, k, 1, Gh, 1e-17f));
     // W half-step: A H, H^T H (+ regW).
     RC(P.ah(d_H, AH));
-    RC(gram(ctx, d_H, n, k, n, HtH));
+    if (!fuseH) RC(gram(ctx, d_H, n, k, n, HtH));
     const double *Gw = HtH;
     if (regW_on) {
       PLANC_HIP_CHECK(ctx, hipMemcpyAsync(Greg, HtH, kk * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
       RC(apply_reg(ctx, Greg, k, desc->regW));
       Gw = Greg;
     }
-    if (algo == PLANC_MU)
-      RC(row_update(ctx, 0, d_W, m, k, m, AH, 1, m, Gw, 1e-17f));
-    else
-      RC(hals_cols(ctx, d_W, m, k, m, AH, m, Gw));
-    RC(gram(ctx, d_W, m, k, m, WtW2));
-    if (desc->compute_error) {
-      // <W, A H> with the H of this iteration equals tr(H^T A^T W) of
-      // computeObjectiveError (common/nmf.hpp:391-396) without the third
-      // product; both traces are invariant under normalize_by_W.
-      double *dparts = nullptr;
-      long ndp = 0;
-      RC(dot_parts(ctx, d_W, 1, m, AH, 1, m, m, k, &dparts, &ndp));
-      RC(nmf_relerr_parts(ctx, normA2, dparts, ndp, WtW2, HtH, k, rel + it));
+    // <W, A H> with the H of this iteration equals tr(H^T A^T W) of
+    // computeObjectiveError (common/nmf.hpp:391-396) without the third
+    // product; both traces are invariant under normalize_by_W.
+    double *dparts = nullptr;
+    long ndp = 0;
+    if (fuseW) {
+      RC(row_update_gram(ctx, 0, d_W, m, k, m, AH, 1, m, Gw, 1e-17f, WtW2,
+                         desc->compute_error ? &dparts : nullptr, &ndp));
+    } else {
+      if (algo == PLANC_MU)
+        RC(row_update(ctx, 0, d_W, m, k, m, AH, 1, m, Gw, 1e-17f));
+      else
+        RC(hals_cols(ctx, d_W, m, k, m, AH, m, Gw));
+      RC(gram(ctx, d_W, m, k, m, WtW2));
+      if (desc->compute_error) RC(dot_parts(ctx, d_W, 1, m, AH, 1, m, m, k, &dparts, &ndp));
     }
+    if (desc->compute_error) RC(nmf_relerr_parts(ctx, normA2, dparts, ndp, WtW2, HtH, k, rel + it));
     if (algo == PLANC_HALS) {
       // normalize_by_W: the column norms are the square roots of diag(W^T W);
       // the next W^T W is D^-1 (W^T W) D^-1.

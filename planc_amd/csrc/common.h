@@ -212,6 +212,14 @@ int row_update(planc_ctx *ctx, int kind, float *H, long n, int k, long ldh, cons
                long ncs, const double *G, float eps);
 int hals_cols(planc_ctx *ctx, float *W, long m, int k, long ldw, const float *AH, long ldah,
               const double *G);
+// row_update fused with the FP64 Gram of the updated rows (into Gout, k x k)
+// and, with dparts, FP64 block partials of <updated rows, N> (*ndp of them,
+// workspace slot 3, for nmf_relerr_parts).  rows_gram_ok: the layouts it
+// takes (N column-major, nrs == 1, or row-major, ncs == 1; leading
+// dimensions multiples of 4; 16-byte aligned H and N; k <= 64).
+bool rows_gram_ok(const float *H, long ldh, const float *N, long nrs, long ncs, int k);
+int row_update_gram(planc_ctx *ctx, int kind, float *H, long n, int k, long ldh, const float *N, long nrs,
+                    long ncs, const double *G, float eps, double *Gout, double **dparts, long *ndp);
 int col_norms(planc_ctx *ctx, const float *X, long rows, int k, long ldx, double *norms);
 int scale_cols(planc_ctx *ctx, float *X, long rows, long ldx, float *Y, long yrows, long ldy, int k,
                const double *norms);

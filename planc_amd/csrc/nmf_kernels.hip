@@ -206,12 +206,21 @@ __global__ __launch_bounds__(NTHR) void gram_reduce_kernel(const double *part, l
 // Loads the k x k FP64 matrix G into LDS as floats, transposed and zero
 // padded: Gt[x][t] = G(t, x), so the column G(:, x) that a row update
 // contracts with is one contiguous row of 16-byte words (uniform address:
-// one LDS broadcast per word for the whole wave).
-template <int KM>
+// one LDS broadcast per word for the whole wave).  NTH = blockDim.x: every
+// thread's loads are issued before the first store (one memory round trip).
+template <int KM, int NTH>
 __device__ inline void load_gram(float (&Gt)[KM][KM], const double *G, int k) {
-  for (int e = threadIdx.x; e < KM * KM; e += blockDim.x) {
-    int t = e % KM, x = e / KM;
-    Gt[x][t] = (t < k && x < k) ? (float)G[t + (size_t)k * x] : 0.0f;
+  constexpr int PER = (KM * KM + NTH - 1) / NTH;
+  double v[PER];
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int e = threadIdx.x + NTH * j, t = e % KM, x = e / KM;
+    v[j] = (e < KM * KM && t < k && x < k) ? G[t + (size_t)k * x] : 0.0;
+  }
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int e = threadIdx.x + NTH * j;
+    if (e < KM * KM) Gt[e / KM][e % KM] = (float)v[j];
   }
   __syncthreads();
 }
@@ -244,7 +253,7 @@ __global__ __launch_bounds__(NTHR) void rows_quad_kernel(float *H, long n, int k
   constexpr int kind = KIND;
   constexpr int Q = KM / 4;
   __shared__ __attribute__((aligned(16))) float Gs[KM][KM];
-  load_gram<KM>(Gs, G, k);
+  load_gram<KM, NTHR>(Gs, G, k);
   const int q = threadIdx.x & 3;
   const long j = (long)blockIdx.x * (NTHR / 4) + (threadIdx.x >> 2);
   if (j >= n) return;
@@ -261,6 +270,278 @@ __global__ __launch_bounds__(NTHR) void rows_quad_kernel(float *H, long n, int k
   for (int u = 0; u < Q; ++u) {
     const int t = q * Q + u;
     if (t < k) H[j + ldh * t] = h[u];
+  }
+}
+
+// Row update fused with the FP64 Gram of the updated rows and, optionally,
+// the FP64 dot <updated rows, N> (the error's cross term): blocks of RG
+// groups of 256 threads; a group takes a 64-row chunk per step (chunk (grp
+// gridDim.x + blockIdx.x) + step RG gridDim.x), loads the chunk of H and of
+// N with 16-byte loads into LDS tiles (column c of the chunk contiguous:
+// coalesced for a column-major H / N, and for a row-major N, NROW), updates
+// the rows in the quad layout (quad_row_update, the rule of
+// rows_quad_kernel), stores them back through the tile with 16-byte stores
+// and accumulates the tile's Gram on FP64 MFMA as gram_mfma_kernel does.
+// The groups' accumulators are combined in LDS; each block writes one
+// KM x KM partial (mirrored, summed by gram_reduce_kernel) and one dot
+// partial.  The factor makes one round trip through HBM instead of three
+// (update, Gram, dot).  Requires ldh % 4 == 0, ldn % 4 == 0 and 16-byte
+// aligned H and N.
+constexpr int RG = 2;  // groups per block (1 block of 512 threads per CU: the LDS tiles)
+
+template <int KM, int KIND, int NROW>
+__global__ __launch_bounds__(NTHR * RG) void rows_gram_kernel(float *H, long n, int k, long ldh,
+                                                              const float *N, long ldn, const double *G,
+                                                              float eps, double *gpart, double *dpart) {
+  constexpr int Q = KM / 4;
+  constexpr int NT = KM / 16, NUP = NT * (NT + 1) / 2, TPW = (NUP + 3) / 4;
+  constexpr int PER = GCR * KM / 4 / NTHR;  // 16-byte pieces of a tile per thread
+  constexpr int XS = KM * (GCR + 1);        // floats of one tile
+  static_assert(2 * RG * XS * 4 >= NTHR * TPW * 32, "the combine buffer must fit the tiles");
+  // KIND 1: the Gram transposed for quad_row_update (Gs[x][t] = G(t, x)).
+  // KIND 0: G (symmetric) with rows padded to KM + 16 floats, read as the
+  // MFMA B operand G(4 s + l / 16, 16 tj + l % 16): the four k rows of a
+  // read land 16 banks apart (two-way at most, no wider conflict).
+  constexpr int GP = KIND == 0 ? KM + 16 : KM;
+  __shared__ __attribute__((aligned(16))) float Gs[KM][GP];
+  __shared__ __attribute__((aligned(32))) float tiles[RG][2][XS];
+  __shared__ double red[NTHR * RG / 64];
+  const int tid = threadIdx.x, grp = tid / NTHR, gt = tid - grp * NTHR;
+  const int lane = tid & 63, wave = gt >> 6;
+  const int q = gt & 3, rr = gt >> 2;
+  float(*Hs)[GCR + 1] = reinterpret_cast<float(*)[GCR + 1]>(&tiles[grp][0][0]);
+  float(*Ns)[GCR + 1] = reinterpret_cast<float(*)[GCR + 1]>(&tiles[grp][1][0]);
+  auto tile_of = [&](int t, int &ta, int &tb) {
+    ta = 0;
+    int rem = t;
+    while (rem >= NT - ta) {
+      rem -= NT - ta;
+      ++ta;
+    }
+    tb = ta + rem;
+  };
+  // Piece p = gt + NTHR j of a column-major tile: column p / 16, rows 4 (p %
+  // 16) .. + 3; of a row-major one: row p / (KM / 4), columns 4 (p % (KM /
+  // 4)) .. + 3.
+  auto load_cm = [&](const float *X, long ldx, long r0, floatx4 (&v)[PER]) {
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      const int p = gt + NTHR * j, c = p >> 4, r4 = 4 * (p & 15);
+      const long gr = r0 + r4;
+      floatx4 x = {0.f, 0.f, 0.f, 0.f};
+      if (c < k) {
+        if (gr + 3 < n) {
+          x = *(const floatx4 *)(X + gr + ldx * c);
+        } else {
+#pragma unroll
+          for (int u = 0; u < 4; ++u) x[u] = gr + u < n ? X[gr + u + ldx * c] : 0.0f;
+        }
+      }
+      v[j] = x;
+    }
+  };
+  auto load_rm = [&](const float *X, long ldx, long r0, floatx4 (&v)[PER]) {
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      const int p = gt + NTHR * j, r = p / (KM / 4), c4 = 4 * (p % (KM / 4));
+      const long gr = r0 + r;
+      floatx4 x = {0.f, 0.f, 0.f, 0.f};
+      if (gr < n) {
+        if (c4 + 3 < k) {
+          x = *(const floatx4 *)(X + gr * ldx + c4);
+        } else {
+#pragma unroll
+          for (int u = 0; u < 4; ++u) x[u] = c4 + u < k ? X[gr * ldx + c4 + u] : 0.0f;
+        }
+      }
+      v[j] = x;
+    }
+  };
+  auto put_cm = [&](float(*T)[GCR + 1], const floatx4 (&v)[PER]) {
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      const int p = gt + NTHR * j, c = p >> 4, r4 = 4 * (p & 15);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) T[c][r4 + u] = v[j][u];
+    }
+  };
+  auto put_rm = [&](float(*T)[GCR + 1], const floatx4 (&v)[PER]) {
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      const int p = gt + NTHR * j, r = p / (KM / 4), c4 = 4 * (p % (KM / 4));
+#pragma unroll
+      for (int u = 0; u < 4; ++u) T[c4 + u][r] = v[j][u];
+    }
+  };
+  const float *ua[TPW], *ub[TPW];
+  bool on[TPW];
+#pragma unroll
+  for (int j = 0; j < TPW; ++j) {
+    const int t = wave + 4 * j;
+    on[j] = t < NUP;
+    int ta = 0, tb = 0;
+    if (on[j]) tile_of(t, ta, tb);
+    ua[j] = &Hs[16 * ta + (lane & 15)][0];
+    ub[j] = &Hs[16 * tb + (lane & 15)][0];
+  }
+  doublex4 acc[TPW];
+#pragma unroll
+  for (int j = 0; j < TPW; ++j) acc[j] = doublex4{0.0, 0.0, 0.0, 0.0};
+  double dsum = 0.0;
+  // Every group runs the same number of steps (block-wide barriers); a group
+  // past the last chunk works on zero rows (no stores, zero contributions).
+  // The next chunk's loads are issued after the row update (fewer registers
+  // live through it) and stay in flight through the stores and the Gram.
+  const long cstride = (long)gridDim.x * RG * GCR;
+  const long nsteps = (n + cstride - 1) / cstride;
+  long r0 = ((long)grp * gridDim.x + blockIdx.x) * GCR;
+  floatx4 vh[PER], vn[PER];
+  load_cm(H, ldh, r0, vh);  // in flight with the Gram's loads
+  if (NROW) load_rm(N, ldn, r0, vn);
+  else load_cm(N, ldn, r0, vn);
+  if constexpr (KIND == 0) {
+    constexpr int GPER = (KM * KM + NTHR * RG - 1) / (NTHR * RG);
+    double gv[GPER];
+#pragma unroll
+    for (int j = 0; j < GPER; ++j) {
+      const int e = tid + NTHR * RG * j, t = e % KM, x = e / KM;
+      gv[j] = (e < KM * KM && t < k && x < k) ? G[t + (size_t)k * x] : 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < GPER; ++j) {
+      const int e = tid + NTHR * RG * j;
+      if (e < KM * KM) Gs[e / KM][e % KM] = (float)gv[j];
+    }
+    __syncthreads();
+  } else {
+    load_gram<KM, NTHR * RG>(reinterpret_cast<float(&)[KM][KM]>(Gs), G, k);
+  }
+  for (long st = 0; st < nsteps; ++st, r0 += cstride) {
+    __syncthreads();
+    put_cm(Hs, vh);
+    if (NROW) put_rm(Ns, vn);
+    else put_cm(Ns, vn);
+    __syncthreads();
+    if constexpr (KIND == 0) {
+      // MU: the denominators H G of the chunk on FP32 MFMA.  Wave w owns
+      // rows 16 w .. 16 w + 15 of the tile (it reads and rewrites only
+      // those): D(i, j) = sum_t H(16 w + i, t) G(t, 16 tj + j), lane l
+      // supplying H(16 w + l % 16, 4 s + l / 16) and G(4 s + l / 16, 16 tj +
+      // l % 16) per 4-k step s, and holding D(4 (l / 16) + v, l % 16).
+      floatx4 den[NT];
+#pragma unroll
+      for (int tj = 0; tj < NT; ++tj) den[tj] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int st4 = 0; st4 < KM / 4; ++st4) {
+        const int kq = 4 * st4 + (lane >> 4);
+        const float a = Hs[kq][16 * wave + (lane & 15)];
+#pragma unroll
+        for (int tj = 0; tj < NT; ++tj)
+          den[tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Gs[kq][16 * tj + (lane & 15)], den[tj], 0, 0, 0);
+      }
+#pragma unroll
+      for (int tj = 0; tj < NT; ++tj) {
+        const int c = 16 * tj + (lane & 15);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int r = 16 * wave + 4 * (lane >> 4) + v;
+          const float hv0 = Hs[c][r], nv0 = Ns[c][r];
+          const float hv = (r0 + r < n && c < k) ? (hv0 * nv0) / (den[tj][v] + eps) : 0.0f;
+          Hs[c][r] = hv;
+          dsum += (double)hv * (double)nv0;
+        }
+      }
+    } else {
+      float h[Q], nv[Q];
+#pragma unroll
+      for (int u = 0; u < Q; ++u) {
+        h[u] = Hs[q * Q + u][rr];
+        nv[u] = Ns[q * Q + u][rr];
+      }
+      quad_row_update<KM, false>(h, nv, reinterpret_cast<const float(&)[KM][KM]>(Gs), q, k, KIND, eps);
+      const bool valid = r0 + rr < n;
+#pragma unroll
+      for (int u = 0; u < Q; ++u) {
+        const float hv = valid ? h[u] : 0.0f;  // rows past n stay zero (HALS floors them)
+        Hs[q * Q + u][rr] = hv;
+        dsum += (double)hv * (double)nv[u];
+      }
+    }
+    __syncthreads();
+    if (st + 1 < nsteps) {
+      load_cm(H, ldh, r0 + cstride, vh);
+      if (NROW) load_rm(N, ldn, r0 + cstride, vn);
+      else load_cm(N, ldn, r0 + cstride, vn);
+    }
+    // Store the updated chunk (column pieces of 16 bytes).
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      const int p = gt + NTHR * j, c = p >> 4, r4 = 4 * (p & 15);
+      const long gr = r0 + r4;
+      if (c < k) {
+        const floatx4 x = {Hs[c][r4], Hs[c][r4 + 1], Hs[c][r4 + 2], Hs[c][r4 + 3]};
+        if (gr + 3 < n) {
+          *(floatx4 *)(H + gr + ldh * c) = x;
+        } else {
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            if (gr + u < n) H[gr + u + ldh * c] = x[u];
+        }
+      }
+    }
+    {
+      doublex4 acc2[TPW];  // odd row steps (independent consecutive MFMAs)
+#pragma unroll
+      for (int j = 0; j < TPW; ++j) acc2[j] = doublex4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int r = 0; r < GCR / 4; r += 2) {
+        const int ra = (lane >> 4) + 4 * r, rb = ra + 4;
+#pragma unroll
+        for (int j = 0; j < TPW; ++j)
+          if (on[j]) {
+            acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)ua[j][ra], (double)ub[j][ra], acc[j], 0, 0, 0);
+            acc2[j] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)ua[j][rb], (double)ub[j][rb], acc2[j], 0, 0, 0);
+          }
+      }
+#pragma unroll
+      for (int j = 0; j < TPW; ++j) acc[j] += acc2[j];
+    }
+  }
+  // Dot partial of the block (fixed order: waves in order).
+  if (dpart) {
+    const double bs = block_sum(dsum, red);
+    if (tid == 0) dpart[blockIdx.x] = bs;
+  }
+  // Combine the groups in order (g0 + g1 (+ ...)) through the tile space.
+  doublex4 *cb = reinterpret_cast<doublex4 *>(&tiles[0][0][0]);  // [NTHR][TPW]
+  for (int g = 1; g < RG; ++g) {
+    __syncthreads();
+    if (grp == g)
+#pragma unroll
+      for (int j = 0; j < TPW; ++j) cb[gt * TPW + j] = acc[j];
+    __syncthreads();
+    if (grp == 0)
+#pragma unroll
+      for (int j = 0; j < TPW; ++j) acc[j] += cb[gt * TPW + j];
+  }
+  if (grp != 0) return;
+  double *dst = gpart + (long)blockIdx.x * KM * KM;
+#pragma unroll
+  for (int j = 0; j < TPW; ++j) {
+    const int t = wave + 4 * j;
+    if (t < NUP) {
+      int ta, tb;
+      tile_of(t, ta, tb);
+      const int b = 16 * tb + (lane & 15);
+#pragma unroll
+      for (int w = 0; w < 4; ++w) {
+        const int a = 16 * ta + (lane >> 4) + 4 * w;
+        if (ta != tb || a <= b) {
+          dst[a + KM * b] = acc[j][w];
+          dst[b + KM * a] = acc[j][w];
+        }
+      }
+    }
   }
 }
 
@@ -338,7 +619,7 @@ __global__ __launch_bounds__(BS) void hals_cols_coop_kernel(float *W, long m, in
                                                             const double *G, double *part, int *sync) {
   __shared__ __attribute__((aligned(16))) float Gs[KM][KM];
   __shared__ double red[BS / 64];
-  load_gram<KM>(Gs, G, k);
+  load_gram<KM, BS>(Gs, G, k);
   const long total = (long)gridDim.x * blockDim.x;
   const long gtid = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const float floor_v = 1e-17f;
@@ -415,7 +696,7 @@ __global__ __launch_bounds__(NTHR) void hals_col_step_kernel(float *W, long m, i
                                                              double *part) {
   __shared__ __attribute__((aligned(16))) float Gs[KM][KM];
   __shared__ double red[NTHR / 64];
-  load_gram<KM>(Gs, G, k);
+  load_gram<KM, NTHR>(Gs, G, k);
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   double v2 = 0.0;
   if (i < m) {
@@ -863,6 +1144,21 @@ int row_update(planc_ctx *ctx, int kind, float *H, long n, int k, long ldh, cons
   return set_error(ctx, PLANC_EUNSUPPORTED, "row update: k > 64");
 }
 
+template <int KM>
+int launch_rows_gram(planc_ctx *ctx, int kind, float *H, long n, int k, long ldh, const float *N, long ldn,
+                     int nrow, const double *G, float eps, double *gpart, long nb, double *dpart) {
+#define RGK(KINDV, NROWV)                                                                                   \
+  rows_gram_kernel<KM, KINDV, NROWV><<<(unsigned)nb, NTHR * RG, 0, ctx->stream>>>(H, n, k, ldh, N, ldn, G, \
+                                                                                 eps, gpart, dpart)
+  if (kind == 0 && nrow) RGK(0, 1);
+  else if (kind == 0) RGK(0, 0);
+  else if (nrow) RGK(1, 1);
+  else RGK(1, 0);
+#undef RGK
+  PLANC_LAUNCH_CHECK(ctx);
+  return PLANC_OK;
+}
+
 int hals_cols(planc_ctx *ctx, float *W, long m, int k, long ldw, const float *AH, long ldah,
               const double *G) {
   if (m <= 0) return PLANC_OK;
@@ -871,6 +1167,42 @@ int hals_cols(planc_ctx *ctx, float *W, long m, int k, long ldw, const float *AH
   if (k <= 48) return hals_cols_impl<48>(ctx, W, m, k, ldw, AH, ldah, G);
   if (k <= 64) return hals_cols_impl<64>(ctx, W, m, k, ldw, AH, ldah, G);
   return set_error(ctx, PLANC_EUNSUPPORTED, "hals_cols: k > 64");
+}
+
+bool rows_gram_ok(const float *H, long ldh, const float *N, long nrs, long ncs, int k) {
+  const bool nrow = ncs == 1;
+  const long ldn = nrow ? nrs : ncs;
+  return k >= 1 && k <= 64 && (nrow || nrs == 1) && ldh % 4 == 0 && ldn % 4 == 0 &&
+         (uintptr_t)H % 16 == 0 && (uintptr_t)N % 16 == 0;
+}
+
+int row_update_gram(planc_ctx *ctx, int kind, float *H, long n, int k, long ldh, const float *N, long nrs,
+                    long ncs, const double *G, float eps, double *Gout, double **dparts, long *ndp) {
+  if (!rows_gram_ok(H, ldh, N, nrs, ncs, k))
+    return set_error(ctx, PLANC_EINVAL, "row_update_gram: layout not supported");
+  const bool nrow = ncs == 1;
+  const long ldn = nrow ? nrs : ncs;
+  const int kp = ((k + 15) / 16) * 16;
+  long nchunks = (n + GCR - 1) / GCR;
+  long nb = (nchunks + RG - 1) / RG;
+  if (nb > ctx->num_cu) nb = ctx->num_cu;
+  if (nb < 1) nb = 1;
+  double *gpart = (double *)ws_get(ctx, 1, sizeof(double) * (size_t)nb * kp * kp);
+  double *dpart = dparts ? (double *)ws_get(ctx, 3, sizeof(double) * (size_t)nb) : nullptr;
+  if (!gpart || (dparts && !dpart)) return set_error(ctx, PLANC_ENOMEM, "row_update_gram: workspace");
+  int rc;
+  if (kp == 16) rc = launch_rows_gram<16>(ctx, kind, H, n, k, ldh, N, ldn, nrow, G, eps, gpart, nb, dpart);
+  else if (kp == 32) rc = launch_rows_gram<32>(ctx, kind, H, n, k, ldh, N, ldn, nrow, G, eps, gpart, nb, dpart);
+  else if (kp == 48) rc = launch_rows_gram<48>(ctx, kind, H, n, k, ldh, N, ldn, nrow, G, eps, gpart, nb, dpart);
+  else rc = launch_rows_gram<64>(ctx, kind, H, n, k, ldh, N, ldn, nrow, G, eps, gpart, nb, dpart);
+  if (rc) return rc;
+  gram_reduce_kernel<<<(unsigned)((kp * kp + 63) / 64), NTHR, 0, ctx->stream>>>(gpart, nb, k, kp, Gout);
+  PLANC_LAUNCH_CHECK(ctx);
+  if (dparts) {
+    *dparts = dpart;
+    *ndp = nb;
+  }
+  return PLANC_OK;
 }
 
 // norms (device, k doubles) of the columns of X.

@@ -126,7 +126,11 @@ __device__ inline void quad_row_update(float (&h)[KM / 4], const float (&nv)[KM 
         for (int u = 0; u < Q; ++u) hn[u] = (own && u == co) ? v : hn[u];
       }
     } else {
-      float sc[2][Q];
+      // Every lane of the quad gets each column's sum; a lane keeps the sums
+      // of its own columns and divides only those (Q divisions, not KM).
+      float sc[2][Q], den[Q];
+#pragma unroll
+      for (int u = 0; u < Q; ++u) den[u] = 1.0f;
       load_st_row<KM>(St, 0, q, sc[0]);
 #pragma unroll
       for (int c = 0; c < KM; ++c) {
@@ -136,9 +140,10 @@ __device__ inline void quad_row_update(float (&h)[KM / 4], const float (&nv)[KM 
 #pragma unroll
         for (int u = 0; u < Q; ++u) ps = fmaf(h[u], sc[c & 1][u], ps);
         ps = quad_sum(ps);
-        const float v = (h[c % Q] * nv[c % Q]) / (ps + eps);
-        hn[c % Q] = q == c / Q ? v : hn[c % Q];
+        den[c % Q] = q == c / Q ? ps : den[c % Q];
       }
+#pragma unroll
+      for (int u = 0; u < Q; ++u) hn[u] = (h[u] * nv[u]) / (den[u] + eps);
       zero_padding<KM>(hn, q, k);
     }
 #pragma unroll
