@@ -174,31 +174,38 @@ __global__ __launch_bounds__(NTHR * GGROUPS) void gram_mfma_kernel(const float *
   }
 }
 
-// Block of 256 threads per 64 consecutive entries e of the kp x kp partial.
+// Block of 256 threads per GR_E = 16 consecutive entries e of the kp x kp
+// partial: thread (e, j) sums partials j, j + 16, ... (up to GR_U of them
+// with every load in flight at once), and the 16 sums of an entry are added
+// in j order.
+constexpr int GR_E = 16, GR_J = NTHR / GR_E, GR_U = 16;
+constexpr int gram_reduce_blocks(int kp) { return (kp * kp + GR_E - 1) / GR_E; }
 __global__ __launch_bounds__(NTHR) void gram_reduce_kernel(const double *part, long nparts, int k, int kp,
                                                            double *G) {
-  __shared__ double red[4][64];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const long e = (long)blockIdx.x * 64 + lane;
+  __shared__ double red[GR_J][GR_E];
+  const int el = threadIdx.x % GR_E, j = threadIdx.x / GR_E;
+  const long e = (long)blockIdx.x * GR_E + el;
   const long kk = (long)kp * kp;
   double s = 0.0;
   if (e < kk) {
-    long b = wave;
-    for (; b + 12 < nparts; b += 16) {
-      double v0 = part[b * kk + e], v1 = part[(b + 4) * kk + e];
-      double v2 = part[(b + 8) * kk + e], v3 = part[(b + 12) * kk + e];
-      s += v0;
-      s += v1;
-      s += v2;
-      s += v3;
+    double v[GR_U];
+#pragma unroll
+    for (int u = 0; u < GR_U; ++u) {
+      const long b = j + (long)GR_J * u;
+      v[u] = b < nparts ? part[b * kk + e] : 0.0;
     }
-    for (; b < nparts; b += 4) s += part[b * kk + e];
+#pragma unroll
+    for (int u = 0; u < GR_U; ++u) s += v[u];
+    for (long b = j + (long)GR_J * GR_U; b < nparts; b += GR_J) s += part[b * kk + e];  // longer lists
   }
-  red[wave][lane] = s;
+  red[j][el] = s;
   __syncthreads();
-  if (wave == 0 && e < kk) {
+  if (j == 0 && e < kk) {
+    double t = 0.0;
+#pragma unroll
+    for (int q = 0; q < GR_J; ++q) t += red[q][el];
     const int a = (int)(e % kp), b = (int)(e / kp);
-    if (a < k && b < k) G[a + (long)k * b] = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
+    if (a < k && b < k) G[a + (long)k * b] = t;
   }
 }
 
@@ -836,10 +843,24 @@ __global__ __launch_bounds__(NTHR) void nmf_relerr_kernel(const double *normA2, 
                                                           double *relerr_out, double *fit_out,
                                                           const double *part, long nparts) {
   __shared__ double red[NTHR / 64];
+  // tr(WtW HtH): the loads of the first 16 NTHR products are issued before
+  // the partial sum (one round trip for both at k <= 64).
+  constexpr int TU = 16;
+  double wv[TU], hv[TU];
+#pragma unroll
+  for (int u = 0; u < TU; ++u) {
+    const int e = threadIdx.x + NTHR * u;
+    const bool in = e < k * k;
+    const int a = in ? e % k : 0, b = in ? e / k : 0;
+    wv[u] = in ? WtW[a + (size_t)k * b] : 0.0;
+    hv[u] = in ? HtH[b + (size_t)k * a] : 0.0;
+  }
   const double dsum = dotv ? 0.0 : sum_parts(part, nparts, red);
   __syncthreads();  // red is reused below
   double tr = 0.0;
-  for (int e = threadIdx.x; e < k * k; e += NTHR) {
+#pragma unroll
+  for (int u = 0; u < TU; ++u) tr += wv[u] * hv[u];
+  for (int e = threadIdx.x + NTHR * TU; e < k * k; e += NTHR) {
     const int a = e % k, b = e / k;
     tr += WtW[a + (size_t)k * b] * HtH[b + (size_t)k * a];
   }
@@ -1130,7 +1151,7 @@ int gram(planc_ctx *ctx, const float *X, long rows, int k, long ldx, double *G) 
   else
     gram_mfma_kernel<64><<<(unsigned)nb, nt, 0, ctx->stream>>>(X, rows, k, ldx, part);
   PLANC_LAUNCH_CHECK(ctx);
-  gram_reduce_kernel<<<(unsigned)((kp * kp + 63) / 64), NTHR, 0, ctx->stream>>>(part, nb, k, kp, G);
+  gram_reduce_kernel<<<(unsigned)gram_reduce_blocks(kp), NTHR, 0, ctx->stream>>>(part, nb, k, kp, G);
   PLANC_LAUNCH_CHECK(ctx);
   return PLANC_OK;
 }
@@ -1196,7 +1217,7 @@ int row_update_gram(planc_ctx *ctx, int kind, float *H, long n, int k, long ldh,
   else if (kp == 48) rc = launch_rows_gram<48>(ctx, kind, H, n, k, ldh, N, ldn, nrow, G, eps, gpart, nb, dpart);
   else rc = launch_rows_gram<64>(ctx, kind, H, n, k, ldh, N, ldn, nrow, G, eps, gpart, nb, dpart);
   if (rc) return rc;
-  gram_reduce_kernel<<<(unsigned)((kp * kp + 63) / 64), NTHR, 0, ctx->stream>>>(gpart, nb, k, kp, Gout);
+  gram_reduce_kernel<<<(unsigned)gram_reduce_blocks(kp), NTHR, 0, ctx->stream>>>(gpart, nb, k, kp, Gout);
   PLANC_LAUNCH_CHECK(ctx);
   if (dparts) {
     *dparts = dpart;
