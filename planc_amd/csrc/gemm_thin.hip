@@ -704,20 +704,29 @@ __device__ inline void pack8(const floatx4 &lo, const floatx4 &hi, uint4 (&w)[NP
   }
 }
 
+// One thread per (S, t, lane): both halves h of its column's 16 rows 64 S +
+// 16 (lane >> 4) .. + 15 (64 contiguous bytes: a wave reads whole 256-byte
+// column segments).
 template <typename TA, int NP>
 __global__ void pack_thin16_kernel(ThinDev th, long K, int NT, long nsuper, uint4 *out, long plane) {
   const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= nsuper * 2 * NT * 64) return;
+  if (e >= nsuper * NT * 64) return;
   const int lane = (int)(e & 63);
   const long st = e >> 6;
   const int t = (int)(st % NT);
-  const long sh = st / NT;  // 2 S + h
+  const long S = st / NT;
   const int c = 16 * t + (lane & 15);
-  const long kb = 64 * (sh >> 1) + 16 * (lane >> 4) + 8 * (sh & 1);
-  uint4 w[NP];
-  pack8<TA, NP>(thin4(th, kb, K, c), thin4(th, kb + 4, K, c), w);
-  out[e] = w[0];
-  if constexpr (NP == 2) out[plane + e] = w[1];
+  const long kb = 64 * S + 16 * (lane >> 4);
+  const floatx4 v0 = thin4(th, kb, K, c), v1 = thin4(th, kb + 4, K, c);
+  const floatx4 v2 = thin4(th, kb + 8, K, c), v3 = thin4(th, kb + 12, K, c);
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    uint4 w[NP];
+    pack8<TA, NP>(h ? v2 : v0, h ? v3 : v1, w);
+    const long o = ((2 * S + h) * NT + t) * 64 + lane;
+    out[o] = w[0];
+    if constexpr (NP == 2) out[plane + o] = w[1];
+  }
 }
 
 // On-the-fly 16-bit thin operand of the block-shared NN kernel (instead of
@@ -2310,7 +2319,7 @@ int thin_gemm(planc_ctx *ctx, int dtype, int trans, const void *A, long lda, lon
       const long tot = nsuper * 2 * NT * 64;  // words of one plane
       uint4 *pk = (uint4 *)ws_get(ctx, 6, (size_t)np * tot * 16);
       if (!pk) return set_error(ctx, PLANC_ENOMEM, "thin_gemm: packed operand");
-      const unsigned nb = (unsigned)((tot + 255) / 256);
+      const unsigned nb = (unsigned)((tot / 2 + 255) / 256);
       if (dtype == PLANC_BF16 && np == 2)
         pack_thin16_kernel<bf16_t, 2><<<nb, 256, 0, ctx->stream>>>(th, K, NT, nsuper, pk, tot);
       else if (dtype == PLANC_BF16)
