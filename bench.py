@@ -263,19 +263,22 @@ def hier_leg(P, D, ctx, m=100000, n=500000, density=1e-4, depth=11, iters=100, f
 
 def profile_ntf4(P, D, ctx, torch):
     """The 128^4 k=48 partial MTTKRPs alone (for counter runs): right and left
-    contractions in FP32 and FP16, ten launches each after tuning, then ten
-    copies of a 1 GiB buffer for the counter calibration."""
+    contractions in FP32 (left: TN on X) and FP16 (left: NN on the transposed
+    copy, as planc_ntf_run runs it), thirteen launches each (the first tunes),
+    then ten copies of a 1 GiB buffer for the counter calibration."""
     dims4, k4 = (128, 128, 128, 128), 48
     X4, init4 = make_ntf_problem(P, D, ctx, torch, dims4, k4, 48, stream=5)
-    X416, _ = ctx.convert(flat(X4), P.F16, scaled=False)
+    X416, X416T = ctx.ntf_copies16(X4, P.F16)
     F4 = [P.colmajor(f) for f in init4]
+    F4T = [F4[2], F4[3], F4[0], F4[1]]
     L4 = dims4[0] * dims4[1]
     P4 = P.empty_colmajor((L4, k4))
     Q4 = P.empty_colmajor((L4, k4))
-    for dt, Xop in ((P.F32, X4), (P.F16, X416)):
-        for side, out in ((0, P4), (1, Q4)):
+    for dt, Xop, Xl, Fl, sl in ((P.F32, X4, X4, F4, 1), (P.F16, X416, X416T, F4T, 0)):
+        for Xs, Fs, side, out in ((Xop, F4, 0, P4), (Xl, Fl, sl, Q4)):
+            torch.zeros(7, device="cuda")  # a fill kernel: marks the start of a phase in the counter lists
             for _ in range(13):
-                ctx.mttkrp_partial(Xop, F4, 1, side, dtype=dt, out=out)
+                ctx.mttkrp_partial(Xs, Fs, 1, side, dtype=dt, out=out)
     cal = torch.empty(256 << 20, dtype=torch.float32, device="cuda")  # 1 GiB
     cal2 = torch.empty_like(cal)
     for _ in range(10):
