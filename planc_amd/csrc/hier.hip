@@ -980,6 +980,14 @@ int planc_hiernmf_run(planc_ctx *ctx, const planc_hier_desc *desc, const long *h
     RC(E.z.get(ctx, nrows ? nrows : 1));
     RC(E.mask.get(ctx, nlev));
 
+    // Lanes per CSR row of the row kernels (W update and the power
+    // iteration's row pass), timed once per batch (a choice from the mean
+    // row length was measured slower on the frontier's small batches).
+    int Gb = -1;
+    auto lanes = [&]() {
+      if (Gb < 0) Gb = E.tune_group(nrows, LONGLEN);
+      return Gb;
+    };
     // Sigma of this level's nodes (children created by the previous level):
     // 100-step power iteration of Node::compute_sigma (hiernmf/node.hpp:66-69).
     std::vector<double> sig(nn, 0.0);
@@ -987,7 +995,7 @@ int planc_hiernmf_run(planc_ctx *ctx, const planc_hier_desc *desc, const long *h
       pow_init<<<nb(nlev), 256, 0, ctx->stream>>>(nlev, E.node_of_col.p, E.node_col0.p, E.seeds.p, E.q.p);
       sumsq_chunks<<<(unsigned)chc.size(), 256, 0, ctx->stream>>>(E.ch_cols.p, E.q.p, E.part.p);
       norm_finalize<<<nb(nn), 256, 0, ctx->stream>>>(E.part.p, E.cb_cols.p, nn, E.sig.p);
-      const int Gp = E.tune_group(nrows, LONGLEN);
+      const int Gp = lanes();
       for (int it = 0; it < iters; ++it) {
         E.launch_pow_short(Gp, nrows, LONGLEN);
         if (nlong) {
@@ -1027,7 +1035,7 @@ int planc_hiernmf_run(planc_ctx *ctx, const planc_hier_desc *desc, const long *h
     init_gram_chunks<<<(unsigned)chi.size(), 256, 0, ctx->stream>>>(E.ch_init.p, E.seeds.p, m, E.part.p);
     gram2_finalize<<<nb(nn), 256, 0, ctx->stream>>>(E.part.p, E.cb_init.p, nn, E.Gw.p);
     PLANC_LAUNCH_CHECK(ctx);
-    const int Gw = E.tune_group(nrows, LONGLEN);
+    const int Gw = lanes();
     if (iters > 0) {
       hipGraphExec_t exec = nullptr;
       RC(capture_graph(ctx, [&]() -> int {
