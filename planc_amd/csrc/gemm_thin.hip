@@ -358,35 +358,27 @@ __device__ inline void epilogue_nn(const StreamParams &p, floatx4 (&acc)[4 * NT]
   }
 }
 
-// Sum over the K splits (split order) of one accumulator slab per block:
-// block (group g, accumulator a); wave w adds splits w, w+4, ...; the four
-// wave sums are combined in wave order.  kind 0: TN with NT, CW; 1: NN.
+// Sum over the K splits of accumulator slabs: wave w of block (g, y) takes
+// slab a = 4 y + w of group g and adds its splits in split order, eight
+// loads in flight at a time.  kind 0: TN with NT, CW; 1: NN; 2: 16-bit NN.
 template <int KIND, int NT, int CW>
 __global__ __launch_bounds__(256) void splitk_reduce_kernel(StreamParams p, int NA, long ngroups) {
-  __shared__ floatx4 red[4][64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const long g = blockIdx.x;
-  const int a = blockIdx.y;
+  const int a = 4 * blockIdx.y + wave;
+  if (a >= NA) return;
   const long split_stride = ngroups * (long)NA * 64;
   const floatx4 *src = (const floatx4 *)p.part + g * (long)NA * 64 + (long)a * 64 + lane;
-  floatx4 s = {0.f, 0.f, 0.f, 0.f};
-  int sp = wave;
-  for (; sp + 12 < p.nsplit; sp += 16) {
-    floatx4 v0 = src[(long)sp * split_stride], v1 = src[(long)(sp + 4) * split_stride];
-    floatx4 v2 = src[(long)(sp + 8) * split_stride], v3 = src[(long)(sp + 12) * split_stride];
-    s += v0;
-    s += v1;
-    s += v2;
-    s += v3;
+  floatx4 t = {0.f, 0.f, 0.f, 0.f};
+  int sp = 0;
+  for (; sp + 8 <= p.nsplit; sp += 8) {
+    floatx4 v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = src[(long)(sp + u) * split_stride];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) t += v[u];
   }
-  for (; sp < p.nsplit; sp += 4) s += src[(long)sp * split_stride];
-  red[wave][lane] = s;
-  __syncthreads();
-  if (wave != 0) return;
-  floatx4 t = red[0][lane];
-  t += red[1][lane];
-  t += red[2][lane];
-  t += red[3][lane];
+  for (; sp < p.nsplit; ++sp) t += src[(long)sp * split_stride];
   if (KIND == 0)
     store_acc_tn<NT>(p, g * (16 * CW), a, t);
   else if (KIND == 1)
@@ -1968,7 +1960,7 @@ void launch_tn16(int dtype, int NT, const Geometry &g, dim3 grid, hipStream_t st
 }
 template <int NT, int CW>
 void launch_reduce_tn(dim3 g, hipStream_t st, const StreamParams &p, int NA, long ngroups) {
-  splitk_reduce_kernel<0, NT, CW><<<g, 256, 0, st>>>(p, NA, ngroups);
+  splitk_reduce_kernel<0, NT, CW><<<dim3(g.x, (unsigned)((NA + 3) / 4)), 256, 0, st>>>(p, NA, ngroups);
 }
 
 // Launches one streaming product (and its split-K reduction) with geometry g.
@@ -1991,7 +1983,7 @@ int launch_stream(planc_ctx *ctx, int dtype, int trans, int NT, int tm, StreamPa
     PLANC_LAUNCH_CHECK(ctx);
     if (g.nsplit > 1) {
       const int NA = 8 * NT;
-      splitk_reduce_kernel<2, 1, 1><<<dim3((unsigned)gpad, (unsigned)NA), 256, 0, st>>>(p, NA, gpad);
+      splitk_reduce_kernel<2, 1, 1><<<dim3((unsigned)gpad, (unsigned)((NA + 3) / 4)), 256, 0, st>>>(p, NA, gpad);
       PLANC_LAUNCH_CHECK(ctx);
     }
     return PLANC_OK;
@@ -2041,7 +2033,7 @@ int launch_stream(planc_ctx *ctx, int dtype, int trans, int NT, int tm, StreamPa
     const int NA = trans ? g.cw * NT : 4 * NT;
     dim3 rgrid((unsigned)gpad, (unsigned)NA);
     if (!trans) {
-      splitk_reduce_kernel<1, 1, 1><<<rgrid, 256, 0, st>>>(p, NA, gpad);
+      splitk_reduce_kernel<1, 1, 1><<<dim3(rgrid.x, (unsigned)((NA + 3) / 4)), 256, 0, st>>>(p, NA, gpad);
     } else {
 #define RED(NTV)                                                 \
   do {                                                           \
