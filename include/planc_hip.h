@@ -430,6 +430,10 @@ typedef struct planc_ntf_desc {
                         by planc_transpose16.  The left contraction then runs as
                         an NN product on it.  Null: the run transposes when it
                         needs to. */
+  double normX2;     /* > 0: ||X||_F^2 supplied by the caller, as AUNTF computes
+                        the tensor's norm once in its constructor
+                        (ntf/auntf.hpp:101); <= 0: computed from d_X on every
+                        call that reports the error */
 } planc_ntf_desc;
 
 /* The dimension tree's split (AUNTF's dimension tree, ntf/auntf.hpp:123,

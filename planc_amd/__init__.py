@@ -534,10 +534,11 @@ class Context:
         return X16, X16T
 
     def ntf_run(self, X, factors, algo, iters, dtype=F32, dimtree=True, compute_error=True,
-                X16=None, X16T=None):
+                X16=None, X16T=None, normX2=0.0):
         """computeNTF on the device; factors (I_i x k, tight column-major) are
         updated in place.  Returns (lambda, relerr).  X16 / X16T: a caller's
-        unscaled 16-bit copy and its dimension-tree transpose (ntf_copies16)."""
+        unscaled 16-bit copy and its dimension-tree transpose (ntf_copies16);
+        normX2 (> 0): ||X||_F^2 computed once by the caller."""
         torch = _torch()
         N = len(factors)
         d = NTFDesc()
@@ -549,6 +550,7 @@ class Context:
         d.k = factors[0].shape[1]
         d.iters, d.dimtree, d.compute_error = iters, 1 if dimtree else 0, 1 if compute_error else 0
         d.X16T = X16T.data_ptr() if X16T is not None else None
+        d.normX2 = float(normX2)
         lam = torch.empty(d.k, dtype=torch.float64, device=factors[0].device)
         arr = (ctypes.c_void_p * N)(*[f.data_ptr() for f in factors])
         rel = np.zeros(max(iters, 1))

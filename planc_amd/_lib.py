@@ -46,7 +46,7 @@ class HierNode(ctypes.Structure):
 class NTFDesc(ctypes.Structure):
     _fields_ = [("algo", c_int), ("dtype", c_int), ("nmodes", c_int), ("dims", c_long * 8),
                 ("k", c_int), ("iters", c_int), ("dimtree", c_int), ("compute_error", c_int),
-                ("X16T", c_void_p)]
+                ("X16T", c_void_p), ("normX2", c_double)]
 
 
 # name -> (restype, argtypes); the exported C ABI.

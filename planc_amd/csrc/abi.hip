@@ -972,7 +972,13 @@ int planc_ntf_run(planc_ctx *ctx, const planc_ntf_desc *desc, const float *d_X, 
     std::vector<double> ones(2 * (size_t)k, 1.0);
     PLANC_HIP_CHECK(ctx, hipMemcpyAsync(nl, ones.data(), sizeof(double) * 2 * k, hipMemcpyHostToDevice, ctx->stream));
   }
-  if (desc->compute_error || nes) RC(dot(ctx, d_X, 1, numel, d_X, 1, numel, numel, 1, normX2, 0));
+  if (desc->compute_error || nes) {
+    if (desc->normX2 > 0)
+      PLANC_HIP_CHECK(ctx, hipMemcpyAsync(normX2, &desc->normX2, sizeof(double), hipMemcpyHostToDevice,
+                                          ctx->stream));
+    else
+      RC(dot(ctx, d_X, 1, numel, d_X, 1, numel, numel, 1, normX2, 0));
+  }
   // NCPFactors::normalize() of the initial factors (ntf/auntf.hpp:88).
   for (int i = 0; i < N; ++i) {
     RC(col_norms(ctx, d_factors[i], dims[i], k, dims[i], norms));

@@ -338,6 +338,7 @@ class AUNTF {
   const int m_low_rank_k;
   const algotype m_updalgo;
   double m_rel_error = 0;
+  double m_normX2 = 0;
   std::vector<double> m_relerr;
   int m_precision = default_precision();
   // Device copy of the tensor for the host-update loop (uploaded once per
@@ -375,6 +376,7 @@ class AUNTF {
     d.iters = m_num_it;
     d.dimtree = m_enable_dim_tree ? 1 : 0;
     d.compute_error = m_compute_error ? 1 : 0;
+    d.normX2 = m_normX2;
     DeviceBuffer<double> dl(k);
     m_relerr.assign(m_num_it, 0.0);
     dev.check(planc_ntf_run(dev.ctx(), &d, dX.get(), nullptr, ptrs.data(), dl.get(), m_relerr.data()),
@@ -498,6 +500,8 @@ class AUNTF {
     m_ncp_factors.normalize();
     gram_without_one.zeros(i_k, i_k);
     ncp_mttkrp_t.resize(i_tensor.modes());
+    // ||X||^2 once, as the reference's constructor takes m_normA (ntf/auntf.hpp:101).
+    for (unsigned long long e = 0; e < i_tensor.numel(); ++e) m_normX2 += i_tensor.m_data[e] * i_tensor.m_data[e];
   }
   virtual ~AUNTF() {}
   NCPFactors &ncp_factors() { return m_ncp_factors; }

@@ -380,6 +380,21 @@ def test_ntf_mixed_precision_final_error(ctx, dt, dimtree, dims, k, copies):
     assert abs(rel[-1] - rel_r[-1]) < 1e-3
 
 
+def test_ntf_normx2_supplied_once(ctx):
+    """planc_ntf_desc.normX2: a caller's ||X||^2 (AUNTF's constructor norm)
+    gives the error trajectory of the run that computes it itself."""
+    say("ntf normX2 supplied")
+    dims, k = (24, 20, 16), 6
+    X, _ = D.cp_tensor(dims, k, 0.01, stream=12)
+    X32 = X.astype(np.float32)
+    F0 = [f.astype(np.float32) for f in D.init_factors([(d, k) for d in dims], stream=12)]
+    dX = P.colmajor(X32)
+    _, rel_a = ctx.ntf_run(dX, [P.colmajor(f) for f in F0], P.HALS, 6)
+    nx2 = float((X32.astype(np.float64) ** 2).sum())
+    _, rel_b = ctx.ntf_run(dX, [P.colmajor(f) for f in F0], P.HALS, 6, normX2=nx2)
+    assert np.abs(rel_a - rel_b).max() < 1e-12
+
+
 def test_ntf_config2_shape(ctx):
     """Config 2 shape: 256^3 CP rank-32 + 1% noise, k = 32, HALS, dimension
     tree; 5 iterations against the FP64 oracle (relative error to 1e-5)."""
