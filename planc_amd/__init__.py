@@ -312,6 +312,18 @@ class Context:
                                        1), "planc_bpp")
         return X
 
+    def admm_update(self, G, R, X, D):
+        """One AO-ADMM solve in place: X (n x k FP32) and its dual D (n x k)
+        given the regularised Gram G (k x k FP64) and the right-hand side R
+        (k x n FP32); raises PlancError (rc PLANC_ENOCONV) when G + rho I is
+        not positive definite, leaving X and D as they were."""
+        k, n = R.shape
+        if G.shape != (k, k) or X.shape != (n, k) or D.shape != (n, k):
+            raise ValueError("admm_update: G must be k x k, R k x n, X and D n x k")
+        self._check(self.lib.planc_admm_update(self.ctx, _ptr(G), k, _ptr(R), _ld(R), 1, _ptr(X),
+                                               _ld(X), _ptr(D), _ld(D), n), "planc_admm_update")
+        return X, D
+
     def cp_loss(self, X, factors):
         """0.5 ||X - [[A_0..A_{N-1}]]||^2 (FP64 float), fused on the device."""
         N = len(factors)

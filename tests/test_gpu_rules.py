@@ -137,3 +137,50 @@ def test_nes_ntf_matches_oracle(ctx, dims, k, dimtree):
     say(f"       oracle {np.array2string(rel_r, precision=7)}")
     assert np.abs(rel - rel_r).max() < 1e-3
     assert abs(rel[-1] - rel_r[-1]) < 2e-4
+
+
+def test_admm_update_entry(ctx):
+    """planc_admm_update, the solve that AO-ADMM NMF and NTF call for every
+    factor (nmf/aoadmm.hpp, ntf/ntfaoadmm.hpp:23-77): two calls in a row with
+    the dual carried against the oracle's solve (1e-4); repeated calls
+    converge to the NNLS solution of each column (the oracle's BPP, pinned to
+    scipy's NNLS); a Gram whose shift is not positive definite, set from the
+    host, returns PLANC_ENOCONV as the oracle's solve fails, with X and D
+    untouched (the sweeps are skipped on the device)."""
+    say("admm_update entry")
+    rng = np.random.default_rng(77)
+    k, n = 16, 3000
+    B = rng.random((60, k))
+    G = B.T @ B
+    R = (B.T @ (rng.random((60, n)) - 0.2)).astype(np.float32)
+    X0 = rng.random((n, k)).astype(np.float32)
+    Xr, Dr = X0.astype(np.float64), np.zeros((n, k))
+    dG, dR = P.colmajor(G), P.colmajor(R)
+    dX, dD = P.colmajor(X0), P.colmajor(np.zeros((n, k), np.float32))
+    for call in range(2):
+        Xr, Dr, rc = O.admm_solve(G, R.astype(np.float64), Xr, Dr)
+        assert rc == 0
+        ctx.admm_update(dG, dR, dX, dD)
+        ex, ed = rel_err(dX.cpu().numpy(), Xr), rel_err(dD.cpu().numpy(), Dr)
+        say(f"  call {call}: ||dX||/||X|| {ex:.2e}, ||dD||/||D|| {ed:.2e}")
+        assert ex < 1e-4 and ed < 1e-4
+    # Fixed point: the NNLS solutions of G x = r_j.
+    Xs, _ = O.bppnnls(G, R[:, :200].astype(np.float64))
+    dR2 = P.colmajor(R[:, :200])
+    dX2, dD2 = P.colmajor(np.zeros((200, k), np.float32)), P.colmajor(np.zeros((200, k), np.float32))
+    for _ in range(300):
+        ctx.admm_update(dG, dR2, dX2, dD2)
+    es = rel_err(dX2.cpu().numpy(), Xs.T)
+    say(f"  300 calls vs NNLS: {es:.2e}")
+    assert es < 1e-4
+    # Not positive definite: G = -I (rho falls back to 0.01, G + rho I = -0.99 I).
+    Gbad = -np.eye(k)
+    _, _, rc = O.admm_solve(Gbad, R[:, :10].astype(np.float64), np.zeros((10, k)), np.zeros((10, k)))
+    assert rc == -1
+    before = dX.cpu().numpy().copy(), dD.cpu().numpy().copy()
+    with pytest.raises(P.PlancError) as e:
+        ctx.admm_update(P.colmajor(Gbad), dR, dX, dD)
+    assert e.value.rc == P._lib.PLANC_ENOCONV
+    assert np.array_equal(dX.cpu().numpy(), before[0]) and np.array_equal(dD.cpu().numpy(), before[1])
+    # The context stays usable after the error.
+    ctx.admm_update(dG, dR, dX, dD)
