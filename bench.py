@@ -27,8 +27,10 @@ Also on the same line:
                    iterations; plus an FP32 variant.
   legs          -- NMF-HALS and NMF-ANLS-BPP on the same A in FP32 vs BF16,
                    NTF-HALS on a 128^4 tensor (k = 48) in FP32 vs FP16 with
-                   its MTTKRP's HBM fraction, NTF-HALS 256^3 (k = 32) against
-                   the FP64 oracle, and the hierarchical rank-2 NMF.
+                   its MTTKRP's HBM fraction, NTF-HALS 256^3 (k = 32) in FP32,
+                   BF16 and FP16 (both NTF legs next to the FP64 oracle's
+                   trajectory, tests/golden/ntf_fp64.json, same input and
+                   init), and the hierarchical rank-2 NMF.
 
 Usage: python bench.py [--gpus 1] [--steps 50] [--warmup 3] [--no-legs]
        [--no-cpu] [--cpu-iters 3] [--profile-kernel]
@@ -49,6 +51,7 @@ METRIC = "outer iters/sec NMF-MU & NTF-HALS: bf16-MFMA vs fp32 at equal rel. err
 HBM_PEAK_GBS = 8000.0
 M_C3, N_C3, K_C3, R_C3, STREAM_C3 = 65536, 16384, 64, 64, 3
 FP64_FIXTURE = os.path.join(ROOT, "tests", "golden", "nmf_c3a_fp64.json")
+NTF_FP64_FIXTURE = os.path.join(ROOT, "tests", "golden", "ntf_fp64.json")
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "gemm_tn_bf16_65536x16384_k64_traffic.json")
 
 
@@ -148,6 +151,29 @@ def fp64_reference(name, steps):
     return float(traj[steps - 1]), [float(x) for x in traj[:steps]]
 
 
+def ntf_fp64_compare(case, steps, runs):
+    """Each run's final error and trajectory against the FP64 oracle's
+    NTF-HALS trajectory on the same input and init (committed fixture);
+    `runs` maps a label to its relative-error trajectory."""
+    if not os.path.exists(NTF_FP64_FIXTURE):
+        return {}
+    fx = json.load(open(NTF_FP64_FIXTURE)).get(case)
+    if not fx or len(fx["relerr"]) != steps:
+        return {}
+    ref = np.array(fx["relerr"])
+    out = {"final_relerr_fp64_oracle": float(ref[-1]),
+           "fp64_source": "tests/golden/ntf_fp64.json (FP64 oracle, same input and init)"}
+    ok = True
+    for label, rel in runs.items():
+        rel = np.asarray(rel, dtype=np.float64)
+        d = float(rel[-1] - ref[-1])
+        out[f"{label}_minus_fp64"] = d
+        out[f"{label}_max_abs_traj_diff_vs_fp64"] = float(np.abs(rel - ref).max())
+        ok = ok and abs(d) < 1e-3
+    out["equal_error_within_1e-3"] = bool(ok)
+    return out
+
+
 def compare_precisions(P, ctx, torch, A, Wi, Hi, A16, cs, A16T, normA2, algo, name, warmup, steps):
     out = {}
     ref = fp64_reference(name, steps)
@@ -182,19 +208,25 @@ def make_ntf_problem(P, D, ctx, torch, dims, k, rank, stream):
     return X, init
 
 
-def ntf_rate(P, ctx, torch, X, init, warmup, steps, dtype, X16=None, X16T=None):
-    """computeNTF (HALS) timed over `steps` iterations after `warmup`; ||X||^2
-    is computed once before, as AUNTF's constructor computes the norm."""
+def ntf_rate(P, ctx, torch, X, init, warmup, steps, dtype, X16=None, X16T=None, reps=5):
+    """computeNTF (HALS) timed over `steps` iterations after `warmup`, `reps`
+    times from the same initial factors (a run of 20 iterations at 128^4 is
+    about 10 ms); returns the median rate, its elapsed time and the
+    trajectory (the runs are deterministic).  ||X||^2 is computed once
+    before, as AUNTF's constructor computes the norm."""
     normX2 = float((flat(X).double() ** 2).sum().item())
     F = [P.colmajor(f) for f in init]
     if warmup:
         ctx.ntf_run(X, F, P.HALS, warmup, dtype=dtype, X16=X16, X16T=X16T, normX2=normX2)
-    F = [P.colmajor(f) for f in init]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    lam, rel = ctx.ntf_run(X, F, P.HALS, steps, dtype=dtype, X16=X16, X16T=X16T, normX2=normX2)
-    torch.cuda.synchronize()
-    el = time.perf_counter() - t0
+    els = []
+    for _ in range(reps):
+        F = [P.colmajor(f) for f in init]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        lam, rel = ctx.ntf_run(X, F, P.HALS, steps, dtype=dtype, X16=X16, X16T=X16T, normX2=normX2)
+        torch.cuda.synchronize()
+        els.append(time.perf_counter() - t0)
+    el = float(np.median(els))
     return steps / el, el, rel
 
 
@@ -440,6 +472,7 @@ def main():
         nt["fp16"] = {"iters_per_s": round(r16, 3), "final_relerr": float(rel16[-1]),
                       "relerr_gap_vs_fp32": abs(float(rel16[-1]) - float(rel32n[-1]))}
         nt["speedup_fp16_over_fp32"] = round(r16 / r32, 4)
+        nt.update(ntf_fp64_compare("128^4", 20, {"fp32": rel32n, "fp16": rel16}))
         F4 = [P.colmajor(f) for f in init4]
         L4 = dims4[0] * dims4[1]
         P4 = P.empty_colmajor((L4, k4))
@@ -459,7 +492,8 @@ def main():
                 nt[f"{part}_partial_mttkrp_{name}"] = {
                     "avg_ms": round(t4, 4), "achieved_GBs": round(b4 / (t4 * 1e-3) / 1e9, 1),
                     "frac_of_8TBs": round(b4 / (t4 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
-        nt["config"] = "128^4 CP rank-48 model + 1% noise, k=48, dimension tree, 20 timed iterations"
+        nt["config"] = ("128^4 CP rank-48 model + 1% noise, k=48, dimension tree, 20 timed iterations "
+                        "(median of 5 timed runs)")
         legs["ntf_hals_128^4_k48"] = nt
         del X4, X416, X416T, F4, P4
         torch.cuda.empty_cache()
@@ -469,19 +503,16 @@ def main():
         X, init = make_ntf_problem(P, D, ctx, torch, dims, k3, 32, stream=2)
         r3, el3, rel3 = ntf_rate(P, ctx, torch, X, init, 3, 30, P.F32)
         c2 = {"fp32": {"iters_per_s": round(r3, 3), "final_relerr": float(rel3[-1])}}
+        trajs = {"fp32": rel3}
         for name, dt in (("bf16", P.BF16), ("fp16", P.F16)):
             X16, X16T = ctx.ntf_copies16(X, dt)
             r, _, rr = ntf_rate(P, ctx, torch, X, init, 3, 30, dt, X16=X16, X16T=X16T)
             c2[name] = {"iters_per_s": round(r, 3), "final_relerr": float(rr[-1])}
+            trajs[name] = rr
             del X16, X16T
-        if not args.no_cpu:
-            from oracle import oracle as O
-            O.set_threads(int(os.environ.get("OMP_NUM_THREADS") or 16))
-            Xh = X.cpu().numpy().astype(np.float64)
-            _, _, rel_cpu, _ = O.ntf(1, Xh, [f.astype(np.float64) for f in init], 12, dimtree=1)
-            del Xh
-            c2["fp32_vs_fp64_oracle_12_iters_max_abs_diff"] = float(np.abs(rel3[:12] - rel_cpu).max())
-        c2["config"] = "256^3 CP rank-32 model + 1% noise, k=32, dimension tree, 30 timed iterations"
+        c2.update(ntf_fp64_compare("256^3", 30, trajs))
+        c2["config"] = ("256^3 CP rank-32 model + 1% noise, k=32, dimension tree, 30 timed iterations "
+                        "(median of 5 timed runs)")
         legs["ntf_hals_256^3_k32"] = c2
         del X
         torch.cuda.empty_cache()

@@ -411,6 +411,50 @@ def test_ntf_config2_shape(ctx):
     assert np.abs(rel - rel_r).max() < 1e-5
 
 
+@pytest.mark.parametrize("case", ["256^3", "128^4"])
+def test_ntf_full_size_vs_fp64_fixture(ctx, case):
+    """NTF-HALS (dimension tree) at the full sizes of the two NTF
+    configurations (256^3, CP rank 32, k = 32, 30 iterations; 128^4, CP rank
+    48, k = 48, 20 iterations), on the device-generated input the FP64
+    fixture was computed on (tests/golden/ntf_fp64.json, written by
+    tests/golden/make_ntf_fp64.py from a host restatement of
+    planc_gen_cp_tensor): the input's FP64 sum matches to 1e-9; the FP32
+    trajectory stays within 2e-5 of FP64 at every iteration; with FP16 and
+    BF16 products (the run's own scaled copy, and the FP16 caller pair that
+    bench.py passes) the final error is within 1e-3 of FP64."""
+    import json
+    import os
+    fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "ntf_fp64.json")))[case]
+    dims, rank, k, stream, iters = tuple(fx["dims"]), fx["rank"], fx["k"], fx["stream"], fx["iters"]
+    say(f"ntf full size {case}: k={k}, {iters} iterations")
+    F = [P.colmajor(D.uniform(D.DATA_SEED, 16 * stream + i, (d, rank)).astype(np.float32))
+         for i, d in enumerate(dims)]
+    sigma = 0.01 * float(np.prod([f.cpu().numpy().astype(np.float64).mean(0) for f in F], axis=0).sum())
+    assert abs(sigma - fx["sigma"]) <= 1e-12 * fx["sigma"]
+    X = ctx.gen_cp_tensor(F, sigma, D.NOISE_SEED, stream)
+    del F
+    s = float(X.sum(dtype=torch.float64).item())
+    say(f"  sum(X) device {s:.6f} fixture {fx['sum_X_fp64']:.6f}")
+    assert abs(s - fx["sum_X_fp64"]) < 1e-9 * fx["sum_X_fp64"]
+    ref = np.array(fx["relerr"])
+    init = [D.uniform(D.INIT_SEED, 16 * stream + i, (d, k)).astype(np.float32) for i, d in enumerate(dims)]
+    _, rel32 = ctx.ntf_run(X, [P.colmajor(f) for f in init], P.HALS, iters)
+    d32 = np.abs(rel32 - ref).max()
+    say(f"  fp32 final {rel32[-1]:.9f} fp64 {ref[-1]:.9f}; max |fp32 - fp64| {d32:.2e}")
+    assert d32 < 2e-5
+    runs = [("fp16", P.F16, None), ("bf16", P.BF16, None), ("fp16 pair", P.F16, ctx.ntf_copies16(X, P.F16))]
+    for name, dt, pair in runs:
+        X16, X16T = pair if pair else (None, None)
+        _, rel = ctx.ntf_run(X, [P.colmajor(f) for f in init], P.HALS, iters, dtype=dt, X16=X16, X16T=X16T)
+        say(f"  {name} final {rel[-1]:.9f}: - fp64 {rel[-1] - ref[-1]:.2e}, "
+            f"max |traj - fp64| {np.abs(rel - ref).max():.2e}")
+        assert np.all(np.isfinite(rel))
+        assert abs(rel[-1] - ref[-1]) < 1e-3
+        del X16, X16T, pair
+    del X
+    torch.cuda.empty_cache()
+
+
 @pytest.mark.parametrize("k,nrhs", [(1, 7), (8, 12), (20, 3000), (33, 500), (64, 700)])
 def test_bpp_batched_matches_oracle(ctx, k, nrhs):
     """Batched FP64 BPP vs the oracle (itself pinned to scipy's NNLS): the
