@@ -360,7 +360,8 @@ __device__ inline void epilogue_nn(const StreamParams &p, floatx4 (&acc)[4 * NT]
 
 // Sum over the K splits of accumulator slabs: wave w of block (g, y) takes
 // slab a = 4 y + w of group g and adds its splits in split order, eight
-// loads in flight at a time.  kind 0: TN with NT, CW; 1: NN; 2: 16-bit NN.
+// loads in flight at a time.  KIND 0 (the only one): the TN forms with NT,
+// CW; the NN forms use splitk_reduce_nn_kernel below.
 template <int KIND, int NT, int CW>
 __global__ __launch_bounds__(256) void splitk_reduce_kernel(StreamParams p, int NA, long ngroups) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -379,12 +380,87 @@ __global__ __launch_bounds__(256) void splitk_reduce_kernel(StreamParams p, int 
     for (int u = 0; u < 8; ++u) t += v[u];
   }
   for (; sp < p.nsplit; ++sp) t += src[(long)sp * split_stride];
-  if (KIND == 0)
-    store_acc_tn<NT>(p, g * (16 * CW), a, t);
-  else if (KIND == 1)
-    store_acc_nn(p, g * 64, a, t);
-  else
-    store_acc_nn16(p, g * 128, a, t);
+  static_assert(KIND == 0, "TN forms only");
+  store_acc_tn<NT>(p, g * (16 * CW), a, t);
+}
+
+// The same sum for the NN forms, written through LDS.  Block (g, t) owns the
+// G x 16 output tile of rows l0 = G g .. l0 + G - 1 and columns 16t .. 16t +
+// 15, held in J accumulator slabs a = J t + j (FP32 NN: G = 64, J = 4, slab
+// j holds rows 16lg + 4u + j; 16-bit NN: G = 128, J = 8, rows 32lg + 8u + j);
+// wave w adds the splits of slabs j = w and w + 4 in split order (the sums of
+// splitk_reduce_kernel, bit for bit), the sums go to LDS, and each thread
+// then stores four consecutive rows of one column with one 16-byte store, so
+// every column segment of the tile is written whole by one wave instead of
+// element by element from slabs rows apart.  Tiles that end past L, a
+// row-major C or unaligned stores take the per-slab stores.
+template <int KIND>
+__global__ __launch_bounds__(256) void splitk_reduce_nn_kernel(StreamParams p, int NA, long ngroups) {
+  constexpr int G = KIND == 1 ? 64 : 128;  // rows of a group
+  constexpr int J = KIND == 1 ? 4 : 8;     // slabs of a 16-column tile
+  constexpr int JW = J / 4;                // slabs per wave
+  constexpr int LD = G + 4;                // LDS row pitch (16-byte aligned)
+  __shared__ __attribute__((aligned(16))) float tile[16 * LD];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, lc = lane & 15, lg = lane >> 4;
+  const long g = blockIdx.x;
+  const int t = blockIdx.y;
+  const long l0 = g * G;
+  const long split_stride = ngroups * (long)NA * 64;
+  const floatx4 *src = (const floatx4 *)p.part + g * (long)NA * 64 + (long)(J * t + wave) * 64 + lane;
+  floatx4 s[JW];
+#pragma unroll
+  for (int h = 0; h < JW; ++h) s[h] = floatx4{0.f, 0.f, 0.f, 0.f};
+  int sp = 0;
+  for (; sp + 8 <= p.nsplit; sp += 8) {
+    floatx4 v[JW][8];
+#pragma unroll
+    for (int h = 0; h < JW; ++h)
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[h][u] = src[(long)(sp + u) * split_stride + 4 * 64 * h];
+#pragma unroll
+    for (int h = 0; h < JW; ++h)
+#pragma unroll
+      for (int u = 0; u < 8; ++u) s[h] += v[h][u];
+  }
+  for (; sp < p.nsplit; ++sp)
+#pragma unroll
+    for (int h = 0; h < JW; ++h) s[h] += src[(long)sp * split_stride + 4 * 64 * h];
+  const bool fast = !p.cmajor && p.cvec && l0 + G <= p.L;  // uniform over the block
+  if (!fast) {
+#pragma unroll
+    for (int h = 0; h < JW; ++h) {
+      if (KIND == 1)
+        store_acc_nn(p, l0, J * t + wave + 4 * h, s[h]);
+      else
+        store_acc_nn16(p, l0, J * t + wave + 4 * h, s[h]);
+    }
+    return;
+  }
+#pragma unroll
+  for (int h = 0; h < JW; ++h) {
+    const int j = wave + 4 * h;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int r = KIND == 1 ? 16 * lg + 4 * u + j : 32 * lg + 8 * u + j;
+      tile[lc * LD + r] = s[h][u];
+    }
+  }
+  __syncthreads();
+  constexpr int RC = G / 4;  // 16-byte row chunks of a column
+#pragma unroll
+  for (int i = 0; i < JW; ++i) {
+    const int q = threadIdx.x + 256 * i;
+    const int cc = q / RC, rc = q % RC;
+    const int c = 16 * t + cc;
+    if (c >= p.th.k) continue;
+    const long l = l0 + 4 * rc;
+    floatx4 o = *(const floatx4 *)(tile + cc * LD + 4 * rc);
+    if (KIND == 2) {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) o[e] = o[e] * ((p.colscale ? p.colscale[l + e] : 1.0f) * p.oscale);
+    }
+    *(floatx4 *)(p.C + l + p.ldc * c) = o;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1983,7 +2059,7 @@ int launch_stream(planc_ctx *ctx, int dtype, int trans, int NT, int tm, StreamPa
     PLANC_LAUNCH_CHECK(ctx);
     if (g.nsplit > 1) {
       const int NA = 8 * NT;
-      splitk_reduce_kernel<2, 1, 1><<<dim3((unsigned)gpad, (unsigned)((NA + 3) / 4)), 256, 0, st>>>(p, NA, gpad);
+      splitk_reduce_nn_kernel<2><<<dim3((unsigned)gpad, (unsigned)NT), 256, 0, st>>>(p, NA, gpad);
       PLANC_LAUNCH_CHECK(ctx);
     }
     return PLANC_OK;
@@ -2033,7 +2109,7 @@ int launch_stream(planc_ctx *ctx, int dtype, int trans, int NT, int tm, StreamPa
     const int NA = trans ? g.cw * NT : 4 * NT;
     dim3 rgrid((unsigned)gpad, (unsigned)NA);
     if (!trans) {
-      splitk_reduce_kernel<1, 1, 1><<<dim3(rgrid.x, (unsigned)((NA + 3) / 4)), 256, 0, st>>>(p, NA, gpad);
+      splitk_reduce_nn_kernel<1><<<dim3(rgrid.x, (unsigned)NT), 256, 0, st>>>(p, NA, gpad);
     } else {
 #define RED(NTV)                                                 \
   do {                                                           \
