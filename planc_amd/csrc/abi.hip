@@ -979,12 +979,17 @@ int planc_ntf_run(planc_ctx *ctx, const planc_ntf_desc *desc, const float *d_X, 
     else
       RC(dot(ctx, d_X, 1, numel, d_X, 1, numel, numel, 1, normX2, 0));
   }
-  // NCPFactors::normalize() of the initial factors (ntf/auntf.hpp:88).
-  for (int i = 0; i < N; ++i) {
-    RC(col_norms(ctx, d_factors[i], dims[i], k, dims[i], norms));
-    RC(scale_cols(ctx, d_factors[i], dims[i], dims[i], nullptr, 0, 0, k, norms));
-    RC(lambda_mul(ctx, d_lambda, norms, k, i == 0));
-    RC(gram(ctx, d_factors[i], dims[i], k, dims[i], G[i]));
+  // NCPFactors::normalize() of the initial factors (ntf/auntf.hpp:88) and
+  // their Grams: small factors in one launch for all modes.
+  if (init_normalize_fits(dims, N, k)) {
+    RC(init_normalize(ctx, d_factors, dims, N, k, G.data(), d_lambda));
+  } else {
+    for (int i = 0; i < N; ++i) {
+      RC(col_norms(ctx, d_factors[i], dims[i], k, dims[i], norms));
+      RC(scale_cols(ctx, d_factors[i], dims[i], dims[i], nullptr, 0, 0, k, norms));
+      RC(lambda_mul(ctx, d_lambda, norms, k, i == 0));
+      RC(gram(ctx, d_factors[i], dims[i], k, dims[i], G[i]));
+    }
   }
   std::vector<const double *> Gc(G.begin(), G.end());
   long ldfL[8], ldfR[8];

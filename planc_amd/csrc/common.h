@@ -221,6 +221,13 @@ bool rows_gram_ok(const float *H, long ldh, const float *N, long nrs, long ncs, 
 int row_update_gram(planc_ctx *ctx, int kind, float *H, long n, int k, long ldh, const float *N, long nrs,
                     long ncs, const double *G, float eps, double *Gout, double **dparts, long *ndp);
 int col_norms(planc_ctx *ctx, const float *X, long rows, int k, long ldx, double *norms);
+// NCPFactors::normalize() of n tight factors (dims[i] x k, ld dims[i]) with
+// lambda = the product of their column norms and G[i] = the FP64 Gram of
+// normalised factor i, in two launches; only for factors of at most 1024
+// rows and 16384 values (init_normalize_fits), k <= 64.
+bool init_normalize_fits(const long *dims, int n, int k);
+int init_normalize(planc_ctx *ctx, float *const *F, const long *dims, int n, int k, double *const *G,
+                   double *lambda);
 int scale_cols(planc_ctx *ctx, float *X, long rows, long ldx, float *Y, long yrows, long ldy, int k,
                const double *norms);
 int dot(planc_ctx *ctx, const float *X, long xrs, long xcs, const float *Y, long yrs, long ycs,

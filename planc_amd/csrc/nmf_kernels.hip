@@ -763,6 +763,134 @@ __global__ __launch_bounds__(NTHR) void colsumsq_partial_kernel(const float *X, 
   }
 }
 
+// NCPFactors::normalize() of the initial factors (ntf/auntf.hpp:88) and
+// their Grams for factors of at most INIT_ROWS rows and INIT_VALS values, in
+// two launches for all modes.  init_gram_kernel, INIT_GS blocks per mode:
+// every block reads the factor into LDS (column pitch I + 1, which spreads
+// the columns of one row over the banks for even I), computes the column
+// norms there exactly as col_norms computes them for one chunk (a wave per
+// column, the same lane order and wave sum, so the same norms), divides its
+// LDS copy by them as scale_cols does, and forms its share of the FP64 Gram
+// of the normalised factor; block 0 of a mode stores the norms.
+// init_scale_kernel then divides the factors in HBM by the norms (after
+// every block has read them) and forms lambda.  This replaces a run-once
+// prologue of 6 launches per mode, each a few microseconds of latency.
+constexpr int INIT_ROWS = (int)COL_CHUNK;
+constexpr int INIT_VALS = 16384;
+constexpr int INIT_GS = 8;
+struct InitFacs {
+  float *f[8];
+  double *g[8];
+  int dim[8];
+};
+
+__global__ __launch_bounds__(NTHR) void init_gram_kernel(InitFacs fs, int k, double *norms) {
+  __shared__ float Fs[INIT_VALS + 64];
+  __shared__ double nrm[64];
+  const int m = blockIdx.x, part = blockIdx.y, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const float *F = fs.f[m];
+  const int I = fs.dim[m], P = I + 1, tot = I * k;
+  // Sixteen loads in flight per thread (a loop of single loads waits a
+  // memory round trip per element).
+  for (int e0 = threadIdx.x; e0 < tot; e0 += 16 * NTHR) {
+    float v[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int e = e0 + u * NTHR;
+      v[u] = e < tot ? F[e] : 0.0f;
+    }
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int e = e0 + u * NTHR;
+      if (e < tot) Fs[e + e / I] = v[u];  // (e - I c) + P c
+    }
+  }
+  __syncthreads();
+  for (int c = wave; c < k; c += NTHR / 64) {
+    const float *col = Fs + P * c;
+    double s = 0.0;
+    int r = lane;
+    for (; r + 192 < I; r += 256) {
+      const double v0 = col[r], v1 = col[r + 64];
+      const double v2 = col[r + 128], v3 = col[r + 192];
+      s += v0 * v0;
+      s += v1 * v1;
+      s += v2 * v2;
+      s += v3 * v3;
+    }
+    for (; r < I; r += 64) {
+      double v = col[r];
+      s += v * v;
+    }
+    s = wave_sum(s);
+    if (lane == 0) {
+      const double nv = sqrt(0.0 + s);
+      nrm[c] = nv;
+      if (part == 0) norms[m * k + c] = nv;
+    }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < tot; e += NTHR) {
+    const int c = e / I;
+    const double nv = nrm[c];
+    if (nv > 0) Fs[e + c] = (float)((double)Fs[e + c] / nv);
+  }
+  __syncthreads();
+  double *G = fs.g[m];
+  for (int e = part * NTHR + threadIdx.x; e < k * k; e += INIT_GS * NTHR) {
+    const int a = e % k, b = e / k;
+    if (a > b) continue;
+    const float *fa = Fs + P * a, *fb = Fs + P * b;
+    // Four interleaved partial sums: the LDS loads of several rows stay in
+    // flight instead of one dependent FP64 chain per row.
+    double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
+    int r = 0;
+#pragma unroll 4
+    for (; r + 3 < I; r += 4) {
+      t0 += (double)fa[r] * (double)fb[r];
+      t1 += (double)fa[r + 1] * (double)fb[r + 1];
+      t2 += (double)fa[r + 2] * (double)fb[r + 2];
+      t3 += (double)fa[r + 3] * (double)fb[r + 3];
+    }
+    for (; r < I; ++r) t0 += (double)fa[r] * (double)fb[r];
+    const double t = (t0 + t1) + (t2 + t3);
+    G[a + k * b] = t;
+    G[b + k * a] = t;
+  }
+}
+
+// The factors divided by their norms (scale_cols' arithmetic), INIT_GS
+// blocks per mode, each thread's loads issued together; block (0, 0) also
+// forms lambda = the product of the modes' norms in mode order (lambda_mul's
+// sequence: the first mode's norms, then multiplied by each further mode's).
+__global__ __launch_bounds__(NTHR) void init_scale_kernel(InitFacs fs, int n, int k, const double *norms,
+                                                         double *lambda) {
+  const int m = blockIdx.x;
+  float *F = fs.f[m];
+  const int I = fs.dim[m], tot = I * k;
+  const double *nm = norms + m * k;
+  constexpr int U = INIT_VALS / (INIT_GS * NTHR);  // elements per thread at most
+  float v[U];
+  double nv[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int e = (blockIdx.y * U + u) * NTHR + threadIdx.x;
+    v[u] = e < tot ? F[e] : 0.0f;
+    nv[u] = e < tot ? nm[e / I] : 0.0;
+  }
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int e = (blockIdx.y * U + u) * NTHR + threadIdx.x;
+    if (e < tot && nv[u] > 0) F[e] = (float)((double)v[u] / nv[u]);
+  }
+  if (m == 0 && blockIdx.y == 0 && threadIdx.x < k) {
+    const int c = threadIdx.x;
+    double l = 1.0 * norms[c];
+    for (int q = 1; q < n; ++q) l = l * norms[q * k + c];
+    lambda[c] = l;
+  }
+}
+
 // norms[c] = sqrt(sum of partials) in chunk order.
 __global__ void colnorm_finalize_kernel(const double *part, long nchunks, int k, double *norms) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1235,6 +1363,31 @@ int col_norms(planc_ctx *ctx, const float *X, long rows, int k, long ldx, double
   colsumsq_partial_kernel<<<(unsigned)nchunks, NTHR, 0, ctx->stream>>>(X, rows, k, ldx, part);
   PLANC_LAUNCH_CHECK(ctx);
   colnorm_finalize_kernel<<<(k + 63) / 64, 64, 0, ctx->stream>>>(part, nchunks, k, norms);
+  PLANC_LAUNCH_CHECK(ctx);
+  return PLANC_OK;
+}
+
+bool init_normalize_fits(const long *dims, int n, int k) {
+  if (n < 1 || n > 8 || k < 1 || k > 64) return false;
+  for (int i = 0; i < n; ++i)
+    if (dims[i] < 1 || dims[i] > INIT_ROWS || dims[i] * k > INIT_VALS) return false;
+  return true;
+}
+
+int init_normalize(planc_ctx *ctx, float *const *F, const long *dims, int n, int k, double *const *G,
+                   double *lambda) {
+  if (!init_normalize_fits(dims, n, k)) return set_error(ctx, PLANC_EINVAL, "init_normalize: shape");
+  double *norms = (double *)ws_get(ctx, 38, sizeof(double) * (size_t)n * k);
+  if (!norms) return set_error(ctx, PLANC_ENOMEM, "init_normalize: workspace");
+  InitFacs fs{};
+  for (int i = 0; i < n; ++i) {
+    fs.f[i] = F[i];
+    fs.g[i] = G[i];
+    fs.dim[i] = (int)dims[i];
+  }
+  init_gram_kernel<<<dim3(n, INIT_GS), NTHR, 0, ctx->stream>>>(fs, k, norms);
+  PLANC_LAUNCH_CHECK(ctx);
+  init_scale_kernel<<<dim3(n, INIT_GS), NTHR, 0, ctx->stream>>>(fs, n, k, norms, lambda);
   PLANC_LAUNCH_CHECK(ctx);
   return PLANC_OK;
 }
